@@ -1,0 +1,91 @@
+"""ctypes binding of libheddle.so (the C ABI declared in include/heddle.h).
+
+The library is built in-tree by ``heddlemd_amd/csrc/Makefile`` (hipcc,
+gfx950).  There is no fallback: if the shared library is missing or fails to
+load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libheddle.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "heddle.h")
+
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_ctx = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hd_version": (C.c_char_p, []),
+    "hd_ctx_create": (C.c_int, [C.c_int, C.POINTER(_ctx)]),
+    "hd_ctx_destroy": (None, [_ctx]),
+    "hd_last_error": (C.c_char_p, [_ctx]),
+    "hd_set_system": (C.c_int, [_ctx, C.c_uint64, _f32p, _f32p, _u32p, _vp]),
+    "hd_set_lattice": (C.c_int, [_ctx, _f64p]),
+    "hd_set_velocities": (C.c_int, [_ctx, _f32p, _f32p, _f32p]),
+    "hd_set_images": (C.c_int, [_ctx, _vp, _vp, _vp]),
+    "hd_set_lj": (C.c_int, [_ctx, C.c_int, _f64p, _f64p, _f64p, _f64p]),
+    "hd_set_spme": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_double]),
+    "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
+    "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
+                                C.c_double]),
+    "hd_set_integrator": (C.c_int, [_ctx, C.c_double, C.c_int]),
+    "hd_set_nhc": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                             C.c_uint64]),
+    "hd_setup": (C.c_int, [_ctx]),
+    "hd_force_eval": (C.c_int, [_ctx, C.c_int]),
+    "hd_constrain_velocities": (C.c_int, [_ctx]),
+    "hd_step": (C.c_int, [_ctx, C.c_int, C.c_int]),
+    "hd_use_graph": (C.c_int, [_ctx, C.c_int]),
+    "hd_sync": (C.c_int, [_ctx]),
+    "hd_get_positions": (C.c_int, [_ctx, _f32p, _vp, _vp, _vp]),
+    "hd_get_velocities": (C.c_int, [_ctx, _f32p, _f32p, _f32p]),
+    "hd_get_forces": (C.c_int, [_ctx, _f32p, _f32p, _f32p]),
+    "hd_get_per_atom": (C.c_int, [_ctx, _f32p, _f32p]),
+    "hd_get_lattice": (C.c_int, [_ctx, _f64p]),
+    "hd_get_scalars": (C.c_int, [_ctx, _f64p]),
+    "hd_get_pairs": (C.c_int, [_ctx, _vp, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "hd_get_neighbor_stats": (C.c_int, [_ctx, _u64p]),
+    "hd_get_spme_grids": (C.c_int, [_ctx, _vp, _vp]),
+    "hd_selftest_dpp": (C.c_int, [_ctx, _i32p]),
+    "hd_stage_times": (C.c_int, [_ctx, C.c_char_p, C.c_int, _f64p, C.c_int]),
+    "hd_set_timing": (C.c_int, [_ctx, C.c_int]),
+}
+
+
+class HeddleError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libheddle.so (raises if it is missing: no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HeddleError(f"{LIB_PATH} not built: run __graft_entry__.build() "
+                          "(make -C heddlemd_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    lib = load()
+    return [n for n in SIGNATURES if hasattr(lib, n)]

@@ -1,0 +1,105 @@
+// Device-side helpers shared by every HeddleMD HIP kernel (gfx950, wave64).
+//
+// Conventions (all kernels):
+//   * Hartree atomic units; k_B = 1, k_C = 1.
+//   * Lattice [lx, ly, lz, xy, xz, yz] (lower-triangular box), kept on the
+//     device as float[6] so that a barostat can rescale it without a host
+//     round trip.
+//   * posq float4 (x, y, z, q) per atom; velocities/forces are SoA float.
+//   * "slot" = position of an atom in the spatially sorted order built by
+//     the neighbour list (sorted_id[slot] = atom id).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HD_WAVE 64
+#define HD_SENTINEL 0xFFFFFFFFu
+
+struct HdLattice {
+  float lx, ly, lz, xy, xz, yz;
+};
+
+__device__ __forceinline__ HdLattice hd_load_lattice(const float* L) {
+  HdLattice l;
+  l.lx = L[0]; l.ly = L[1]; l.lz = L[2];
+  l.xy = L[3]; l.xz = L[4]; l.yz = L[5];
+  return l;
+}
+
+// Cartesian -> fractional by back substitution (same operation order as
+// the reference, kernels/pbc.cuh:18-27).
+__device__ __forceinline__ void hd_cart_to_frac(float x, float y, float z, const HdLattice& L,
+                                                float& sa, float& sb, float& sc) {
+  sc = z / L.lz;
+  sb = (y - sc * L.yz) / L.ly;
+  sa = (x - sb * L.xy - sc * L.xz) / L.lx;
+}
+
+// In-place minimum image (kernels/pbc.cuh:30-43).
+__device__ __forceinline__ void hd_min_image(float& dx, float& dy, float& dz, const HdLattice& L) {
+  float sa, sb, sc;
+  hd_cart_to_frac(dx, dy, dz, L, sa, sb, sc);
+  float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+  dx -= ka * L.lx + kb * L.xy + kc * L.xz;
+  dy -= kb * L.ly + kc * L.yz;
+  dz -= kc * L.lz;
+}
+
+// In-place wrap into the primary cell returning the image triple
+// (kernels/pbc.cuh:74-92).
+__device__ __forceinline__ void hd_wrap_with_image(float& x, float& y, float& z, int& ka_o,
+                                                   int& kb_o, int& kc_o, const HdLattice& L) {
+  float sa, sb, sc;
+  hd_cart_to_frac(x, y, z, L, sa, sb, sc);
+  float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+  x -= ka * L.lx + kb * L.xy + kc * L.xz;
+  y -= kb * L.ly + kc * L.yz;
+  z -= kc * L.lz;
+  ka_o = (int)ka;
+  kb_o = (int)kb;
+  kc_o = (int)kc;
+}
+
+// Shift (x,y,z) to the periodic image nearest the centre (cx,cy,cz)
+// (kernels/pbc.cuh:54-72).
+__device__ __forceinline__ void hd_wrap_against_center(float& x, float& y, float& z, float cx,
+                                                       float cy, float cz, const HdLattice& L) {
+  float dx = x - cx, dy = y - cy, dz = z - cz;
+  float sa, sb, sc;
+  hd_cart_to_frac(dx, dy, dz, L, sa, sb, sc);
+  float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+  x -= ka * L.lx + kb * L.xy + kc * L.xz;
+  y -= kb * L.ly + kc * L.yz;
+  z -= kc * L.lz;
+}
+
+// Wave64 rotation through DPP: every lane receives the value held by lane
+// (lane + 1) mod 64 ("wave_rol:1"; a full-rate VALU move, no LDS traffic).
+__device__ __forceinline__ int hd_rot1_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float hd_rot1(float v) {
+  return __int_as_float(hd_rot1_i(__float_as_int(v)));
+}
+__device__ __forceinline__ unsigned hd_rot1_u(unsigned v) {
+  return (unsigned)hd_rot1_i((int)v);
+}
+
+// f32 -> i64 fixed point at scale 2^48, truncating toward zero; the scale
+// is applied as two exact power-of-two products so no intermediate
+// rounding occurs (src/forces/jit_composed.rs:1462-1475).
+__device__ __forceinline__ long long hd_to_fixed48(float f) {
+  float s = f * 16777216.0f;
+  s *= 16777216.0f;
+  return (long long)s;
+}
+// i64 -> f32 (round to nearest) then * 2^-48 (kernels/neighbor.cu:744-758).
+__device__ __forceinline__ float hd_from_fixed48(long long v) {
+  const float inv = (1.0f / 16777216.0f) / 16777216.0f;
+  return (float)v * inv;
+}
+
+__device__ __forceinline__ unsigned long long hd_lane_mask_lt() {
+  unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}

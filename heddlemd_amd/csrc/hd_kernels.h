@@ -1,0 +1,132 @@
+// Host-side launch wrappers for the HeddleMD HIP kernels.  Every wrapper
+// enqueues on the caller's stream and never synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+struct HdNeighborArgs {
+  unsigned n, npad, nvox, cap;
+  unsigned nv[3];
+  float rs2, disp_thr2;
+  const float4* posq;
+  const float* lattice;
+  float4* ref_pos;
+  const unsigned* voxel_rank;
+  unsigned *atom_key, *voxel_count, *voxel_off, *voxel_cursor;
+  unsigned *sorted_id, *slot_of;
+  const unsigned *type, *key;
+  unsigned *tile_type, *tile_key;
+  float4* tile_posq;
+  float4 *center, *half;
+  unsigned *entry_atoms, *entry_count, *max_count;
+  unsigned *status, *rebuild_count;
+};
+
+void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a);
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
+void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
+                           const float* lattice, float4* tile_posq, long long* fpj,
+                           long long* fpj_ew, unsigned n, unsigned npad);
+
+struct HdPairArgs {
+  unsigned n, npad, cap;
+  int ntypes;
+  const double *sigma, *eps, *cut, *rsw;  // host arrays, ntypes^2
+  double kc, alpha, coul_rc, rmax, r_skin;
+  const float* lattice;
+  const float4* tile_posq;
+  const unsigned *tile_type, *tile_key;
+  const float4 *center, *half;
+  const unsigned *entry_atoms, *entry_count;
+  long long *fpi, *fpi_ew, *fpj, *fpj_ew;
+};
+void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev);
+
+struct HdSpmeArgs {
+  unsigned n, na, nb, nc;
+  double kc, alpha;
+  const float* lattice;
+  const float4* tile_posq;
+  long long* rho_fixed;
+  float2* cgrid;
+  float* vgrid;
+  const float2 *tw_a, *tw_b, *tw_c;
+  const float *bf_a, *bf_b, *bf_c;
+  double2* partials;
+  double* recip;   // [E_recip, W_recip]
+  float* w_share;  // W_recip / N
+};
+void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
+unsigned hd_spme_partials_count(unsigned nb, unsigned nc);
+
+struct HdFinalizeArgs {
+  unsigned n, npad;
+  const unsigned* sorted_id;
+  const float4* tile_posq;
+  const long long *fpi, *fpi_ew, *fpj, *fpj_ew;
+  // SPME gather (vgrid == nullptr disables)
+  const float* vgrid;
+  unsigned na, nb, nc;
+  double kc, alpha;
+  const float* w_share;
+  const float* lattice;
+  const float* w_cons;  // constraint virial per atom (nullptr = none)
+  float *fx, *fy, *fz, *pe, *vir;
+  // fused post-force kick (kick == false disables)
+  bool kick;
+  float dt;
+  float *vx, *vy, *vz;
+  double *vxlo, *vylo, *vzlo;  // lossless residuals (nullptr = f32 mode)
+  const float* mass;
+  const float* factor;  // thermostat factor applied after the kick
+};
+void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev);
+
+struct HdIntegrateArgs {
+  unsigned n;
+  float dt;
+  float4* posq;
+  int *ix, *iy, *iz;
+  float *vx, *vy, *vz;
+  double *xlo, *ylo, *zlo, *vxlo, *vylo, *vzlo;
+  const float *fx, *fy, *fz, *mass;
+  const float* lattice;
+  const float* factor;  // thermostat factor applied before the kick
+  float4* snap;         // pre-drift positions (nullptr = no snapshot)
+};
+void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a);
+
+struct HdSettleArgs {
+  unsigned n_groups;
+  const unsigned* groups;  // 3 per group: O, H1, H2
+  float ra, rb, rc, m_o, m_h, dt;
+  float4* posq;
+  const float4* snap;
+  float *vx, *vy, *vz;
+  float* w_cons;  // per-atom constraint virial (written)
+  const float* lattice;
+};
+void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a);
+void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool accumulate_virial);
+
+struct HdNhcArgs {
+  int chain, n_yoshida, n_resp;
+  double w[7];
+  double kt, g_dof;
+  double* state;  // xi[chain], pxi[chain], q[chain]
+};
+// KE reduction in FP64 (fixed chunking and tree) into ke[0]; when nhc is
+// non-null also advances the chain by one half step and writes the
+// cumulative velocity factor (f32) to factor[0].
+void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
+                  const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
+                  float dt, float* factor);
+// Fixed-order FP64 sums of per-atom arrays a and b into out[0], out[1].
+void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
+                    double* out);
+void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count);
+void hd_launch_selftest_dpp(hipStream_t st, int* out);
+
+#define HD_REDUCE_BLOCKS 256

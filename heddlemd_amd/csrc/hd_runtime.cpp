@@ -1,0 +1,1205 @@
+// Host runtime of libheddle: context, device buffers, step orchestration
+// and hipGraph capture behind the C ABI in include/heddle.h.
+//
+// Step sequence (one MD step, velocity Verlet + SETTLE + Nose-Hoover chain;
+// the reference runner's order, src/runner.rs:2585-2687 and
+// src/integrator/mod.rs:344-429):
+//   1. thermostat pre half step: KE -> chain -> factor      (apply_pre)
+//   2. rescale + kick + drift + wrap (+ SETTLE snapshot)    (vv_kick_drift)
+//   3. SETTLE positions                                     (apply_after_drift)
+//   4. SETTLE velocities + constraint virial                (apply_after_kick)
+//   5. thermostat post half step: KE -> chain -> factor     (apply_post)
+//   6. force evaluation: displacement check, gated rebuild, tile gather,
+//      pair kernel, SPME, finalize                          (ForceField::step)
+//   7. post-force kick * thermostat factor (fused into the finalize)
+// Steps 4 and 5 read only velocities and constrained positions, which the
+// force evaluation does not change, so running them before it (the
+// reference runs them after) gives the same numbers and lets the post-force
+// kick fuse into the force finalize.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/heddle.h"
+#include "hd_kernels.h"
+
+#define HD_VERSION "heddlemd-amd 0.1.0 (gfx950)"
+#define HD_SENTINEL_KEY_LIMIT 0xFFFFFFFEu
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+struct Stage {
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+struct hd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // configuration
+  uint64_t n = 0;
+  std::vector<float> h_posq, h_mass, h_vx, h_vy, h_vz;
+  std::vector<uint32_t> h_type, h_key;
+  std::vector<int32_t> h_ix, h_iy, h_iz;
+  double lattice[6] = {0, 0, 0, 0, 0, 0};
+  int ntypes = 0;
+  std::vector<double> lj_sigma, lj_eps, lj_cut, lj_rsw;
+  bool have_spme = false;
+  double alpha = 0, coul_rc = 0, kc = 1.0;
+  int grid[3] = {0, 0, 0};
+  double r_skin = -1.0;
+  std::vector<uint32_t> h_groups;
+  double d_oh = 0, d_hh = 0, m_o = 0, m_h = 0;
+  double dt = 0;
+  bool lossless = false;
+  bool have_nhc = false;
+  double nhc_kt = 0, nhc_tau = 0;
+  int nhc_chain = 3, nhc_yoshida = 3, nhc_resp = 1;
+  uint64_t nhc_nc = 0;
+  bool use_graph = true;
+  bool timing = false;
+  bool ready = false;
+  bool forces_valid = false;
+
+  // derived
+  unsigned npad = 0, nblocks = 0, cap = 0, nvox = 0;
+  unsigned nv[3] = {0, 0, 0};
+  double rmax = 0;
+
+  // device: atom order
+  float4* posq = nullptr;
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  float *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  float *mass = nullptr, *pe = nullptr, *vir = nullptr, *w_cons = nullptr;
+  int *ix = nullptr, *iy = nullptr, *iz = nullptr;
+  unsigned *type = nullptr, *key = nullptr;
+  double *xlo = nullptr, *ylo = nullptr, *zlo = nullptr, *vxlo = nullptr, *vylo = nullptr,
+         *vzlo = nullptr;
+  float4 *snap = nullptr, *ref_pos = nullptr;
+  float* d_lattice = nullptr;
+  // device: neighbour list
+  unsigned *voxel_rank = nullptr, *atom_key = nullptr, *voxel_count = nullptr,
+           *voxel_off = nullptr, *voxel_cursor = nullptr;
+  unsigned *sorted_id = nullptr, *slot_of = nullptr, *tile_type = nullptr, *tile_key = nullptr;
+  float4 *tile_posq = nullptr, *center = nullptr, *half = nullptr;
+  unsigned *entry_atoms = nullptr, *entry_count = nullptr;
+  unsigned* ctrl = nullptr;  // [0] status, [1] rebuild count, [2] max entries
+  long long *fpi = nullptr, *fpi_ew = nullptr, *fpj = nullptr, *fpj_ew = nullptr;
+  // device: SPME
+  long long* rho_fixed = nullptr;
+  float2* cgrid = nullptr;
+  float* vgrid = nullptr;
+  float2 *tw_a = nullptr, *tw_b = nullptr, *tw_c = nullptr;
+  float *bf_a = nullptr, *bf_b = nullptr, *bf_c = nullptr;
+  double2* recip_partials = nullptr;
+  double* recip = nullptr;
+  float* w_share = nullptr;
+  // device: scalars / thermostat
+  double* red_partials = nullptr;
+  double* scal = nullptr;  // [0] ke, [1] pe, [2] vir, [3] ke scratch
+  float* factors = nullptr;  // [0] pre, [1] post
+  double* nhc_state = nullptr;
+  unsigned* d_groups = nullptr;
+
+  // graphs
+  hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
+
+  std::vector<Stage> stages;
+  std::vector<double> stage_ms;
+
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(hd_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                   \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return fail((c), HD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                    \
+  } while (0)
+
+template <typename T>
+int dalloc(hd_ctx* c, T** p, size_t count) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess)
+    return fail(c, HD_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+  c->allocs.push_back((void*)*p);
+  return HD_OK;
+}
+
+#define DALLOC(c, p, count)                  \
+  do {                                       \
+    int _r = dalloc((c), &(p), (count));     \
+    if (_r != HD_OK) return _r;              \
+  } while (0)
+
+void dfree(hd_ctx* c, void* p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+bool is_pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+
+uint64_t morton3(uint32_t a, uint32_t b, uint32_t c) {
+  uint64_t code = 0;
+  for (int bit = 0; bit < 21; ++bit) {
+    code |= (uint64_t)((a >> bit) & 1u) << (3 * bit + 2);
+    code |= (uint64_t)((b >> bit) & 1u) << (3 * bit + 1);
+    code |= (uint64_t)((c >> bit) & 1u) << (3 * bit);
+  }
+  return code;
+}
+
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(std::max(n / 2, 1));
+  for (int k = 0; k < n / 2; ++k) {
+    double ang = -2.0 * M_PI * k / n;
+    t[k] = make_float2((float)cos(ang), (float)sin(ang));
+  }
+  return t;
+}
+
+double bspline_host(int p, double x) {
+  double v[16];
+  for (int i = 0; i < p; ++i) {
+    double xi = x - i;
+    v[i] = (xi >= 0.0 && xi < 1.0) ? 1.0 : 0.0;
+  }
+  for (int order = 2; order <= p; ++order) {
+    double inv = 1.0 / (order - 1.0);
+    for (int i = 0; i < p - order + 1; ++i) {
+      double xi = x - i;
+      v[i] = xi * inv * v[i] + (order - xi) * inv * v[i + 1];
+    }
+  }
+  return v[0];
+}
+
+// |b(k)|^2 corrections, src/forces/spme.rs:524-585.
+std::vector<float> b_factors(int n, int p) {
+  std::vector<double> ms(p - 1), mod(n);
+  for (int j = 1; j < p; ++j) ms[j - 1] = bspline_host(p, (double)j);
+  for (int k = 0; k < n; ++k) {
+    double th = 2.0 * M_PI * k / n, re = 0, im = 0;
+    for (int j = 0; j < p - 1; ++j) {
+      re += ms[j] * cos(th * j);
+      im += ms[j] * sin(th * j);
+    }
+    mod[k] = re * re + im * im;
+  }
+  for (int k = 0; k < n; ++k)
+    if (mod[k] < 1e-7) mod[k] = 0.5 * (mod[(k + n - 1) % n] + mod[(k + 1) % n]);
+  std::vector<float> out(n);
+  for (int k = 0; k < n; ++k) out[k] = mod[k] > 0 ? (float)(1.0 / mod[k]) : 0.f;
+  return out;
+}
+
+template <typename T>
+int upload(hd_ctx* c, T* dst, const T* src, size_t count) {
+  if (!count) return HD_OK;
+  HIPCHK(c, hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return HD_OK;
+}
+
+HdNeighborArgs nb_args(hd_ctx* c) {
+  HdNeighborArgs a;
+  memset(&a, 0, sizeof(a));
+  a.n = (unsigned)c->n;
+  a.npad = c->npad;
+  a.nvox = c->nvox;
+  a.cap = c->cap;
+  for (int d = 0; d < 3; ++d) a.nv[d] = c->nv[d];
+  double rs = c->rmax + c->r_skin;
+  a.rs2 = (float)(rs * rs);
+  a.disp_thr2 = (float)(0.25 * c->r_skin * c->r_skin);
+  a.posq = c->posq;
+  a.lattice = c->d_lattice;
+  a.ref_pos = c->ref_pos;
+  a.voxel_rank = c->voxel_rank;
+  a.atom_key = c->atom_key;
+  a.voxel_count = c->voxel_count;
+  a.voxel_off = c->voxel_off;
+  a.voxel_cursor = c->voxel_cursor;
+  a.sorted_id = c->sorted_id;
+  a.slot_of = c->slot_of;
+  a.type = c->type;
+  a.key = c->key;
+  a.tile_type = c->tile_type;
+  a.tile_key = c->tile_key;
+  a.tile_posq = c->tile_posq;
+  a.center = c->center;
+  a.half = c->half;
+  a.entry_atoms = c->entry_atoms;
+  a.entry_count = c->entry_count;
+  a.max_count = c->ctrl + 2;
+  a.status = c->ctrl;
+  a.rebuild_count = c->ctrl + 1;
+  return a;
+}
+
+HdNhcArgs nhc_args(hd_ctx* c) {
+  HdNhcArgs a;
+  memset(&a, 0, sizeof(a));
+  static const double y1[1] = {1.0};
+  static const double y3[3] = {1.3512071919596577, -1.7024143839193155, 1.3512071919596577};
+  static const double y5[5] = {0.41449077179437574, 0.41449077179437574, -0.6579630871775030,
+                               0.41449077179437574, 0.41449077179437574};
+  static const double y7[7] = {0.7845136104775573, 0.2355732133593582, -1.1776799841788710,
+                               1.3151863206839023, -1.1776799841788710, 0.2355732133593582,
+                               0.7845136104775573};
+  const double* w = c->nhc_yoshida == 1 ? y1 : c->nhc_yoshida == 3 ? y3 : c->nhc_yoshida == 5 ? y5 : y7;
+  a.chain = c->nhc_chain;
+  a.n_yoshida = c->nhc_yoshida;
+  a.n_resp = c->nhc_resp;
+  for (int i = 0; i < c->nhc_yoshida; ++i) a.w[i] = w[i];
+  a.kt = c->nhc_kt;
+  long long dof = 3LL * (long long)c->n - (long long)c->nhc_nc - 3LL;
+  a.g_dof = (double)std::max(dof, 0LL);
+  a.state = c->nhc_state;
+  return a;
+}
+
+void stage_begin(hd_ctx* c, const char* name, bool on) {
+  if (!on) return;
+  Stage s;
+  s.name = name;
+  (void)hipEventCreate(&s.a);
+  (void)hipEventCreate(&s.b);
+  (void)hipEventRecord(s.a, c->stream);
+  c->stages.push_back(s);
+}
+void stage_end(hd_ctx* c, bool on) {
+  if (!on) return;
+  (void)hipEventRecord(c->stages.back().b, c->stream);
+}
+
+// Force evaluation; kick = fuse the post-force half kick with factor
+// factors[1].  Enqueue only.
+void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
+  hipStream_t st = c->stream;
+  HdNeighborArgs na = nb_args(c);
+  stage_begin(c, "neighbor_update", timed);
+  hd_launch_neighbor_update(st, na);
+  stage_end(c, timed);
+  stage_begin(c, "tile_gather", timed);
+  hd_launch_tile_gather(st, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
+                        ev ? c->fpj_ew : nullptr, (unsigned)c->n, c->npad);
+  stage_end(c, timed);
+  stage_begin(c, "neighbor_build", timed);
+  hd_launch_neighbor_build(st, na);
+  stage_end(c, timed);
+  HdPairArgs pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.n = (unsigned)c->n;
+  pa.npad = c->npad;
+  pa.cap = c->cap;
+  pa.ntypes = c->ntypes;
+  pa.sigma = c->lj_sigma.data();
+  pa.eps = c->lj_eps.data();
+  pa.cut = c->lj_cut.data();
+  pa.rsw = c->lj_rsw.data();
+  pa.kc = c->kc;
+  pa.alpha = c->have_spme ? c->alpha : 0.0;
+  pa.coul_rc = c->have_spme ? c->coul_rc : 0.0;
+  pa.rmax = c->rmax;
+  pa.r_skin = c->r_skin;
+  pa.lattice = c->d_lattice;
+  pa.tile_posq = c->tile_posq;
+  pa.tile_type = c->tile_type;
+  pa.tile_key = c->tile_key;
+  pa.center = c->center;
+  pa.half = c->half;
+  pa.entry_atoms = c->entry_atoms;
+  pa.entry_count = c->entry_count;
+  pa.fpi = c->fpi;
+  pa.fpi_ew = c->fpi_ew;
+  pa.fpj = c->fpj;
+  pa.fpj_ew = c->fpj_ew;
+  stage_begin(c, "pair_force", timed);
+  hd_launch_pair(st, pa, ev);
+  stage_end(c, timed);
+  if (c->have_spme) {
+    HdSpmeArgs sa;
+    memset(&sa, 0, sizeof(sa));
+    sa.n = (unsigned)c->n;
+    sa.na = c->grid[0];
+    sa.nb = c->grid[1];
+    sa.nc = c->grid[2];
+    sa.kc = c->kc;
+    sa.alpha = c->alpha;
+    sa.lattice = c->d_lattice;
+    sa.tile_posq = c->tile_posq;
+    sa.rho_fixed = c->rho_fixed;
+    sa.cgrid = c->cgrid;
+    sa.vgrid = c->vgrid;
+    sa.tw_a = c->tw_a;
+    sa.tw_b = c->tw_b;
+    sa.tw_c = c->tw_c;
+    sa.bf_a = c->bf_a;
+    sa.bf_b = c->bf_b;
+    sa.bf_c = c->bf_c;
+    sa.partials = c->recip_partials;
+    sa.recip = c->recip;
+    sa.w_share = c->w_share;
+    stage_begin(c, "spme_recip", timed);
+    hd_launch_spme(st, sa);
+    stage_end(c, timed);
+  }
+  HdFinalizeArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.n = (unsigned)c->n;
+  fa.npad = c->npad;
+  fa.sorted_id = c->sorted_id;
+  fa.tile_posq = c->tile_posq;
+  fa.fpi = c->fpi;
+  fa.fpi_ew = c->fpi_ew;
+  fa.fpj = c->fpj;
+  fa.fpj_ew = c->fpj_ew;
+  fa.vgrid = c->have_spme ? c->vgrid : nullptr;
+  fa.na = c->grid[0];
+  fa.nb = c->grid[1];
+  fa.nc = c->grid[2];
+  fa.kc = c->kc;
+  fa.alpha = c->alpha;
+  fa.w_share = c->w_share;
+  fa.lattice = c->d_lattice;
+  fa.w_cons = c->h_groups.empty() ? nullptr : c->w_cons;
+  fa.fx = c->fx;
+  fa.fy = c->fy;
+  fa.fz = c->fz;
+  fa.pe = c->pe;
+  fa.vir = c->vir;
+  fa.kick = kick;
+  fa.dt = (float)c->dt;
+  fa.vx = c->vx;
+  fa.vy = c->vy;
+  fa.vz = c->vz;
+  fa.vxlo = c->lossless ? c->vxlo : nullptr;
+  fa.vylo = c->lossless ? c->vylo : nullptr;
+  fa.vzlo = c->lossless ? c->vzlo : nullptr;
+  fa.mass = c->mass;
+  fa.factor = c->have_nhc ? c->factors + 1 : nullptr;
+  stage_begin(c, kick ? "finalize_kick" : "finalize", timed);
+  hd_launch_finalize(st, fa, ev);
+  stage_end(c, timed);
+  if (ev) {
+    stage_begin(c, "reduce_scalars", timed);
+    hd_launch_sum2(st, (unsigned)c->n, c->pe, c->vir, c->red_partials, c->scal + 1);
+    stage_end(c, timed);
+  }
+}
+
+void enqueue_step(hd_ctx* c, bool ev, bool timed) {
+  hipStream_t st = c->stream;
+  unsigned n = (unsigned)c->n;
+  HdNhcArgs nhc = nhc_args(c);
+  if (c->have_nhc) {
+    stage_begin(c, "nhc_pre", timed);
+    hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
+                 (float)c->dt, c->factors + 0);
+    stage_end(c, timed);
+  }
+  bool settle = !c->h_groups.empty();
+  HdIntegrateArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = n;
+  ia.dt = (float)c->dt;
+  ia.posq = c->posq;
+  ia.ix = c->ix;
+  ia.iy = c->iy;
+  ia.iz = c->iz;
+  ia.vx = c->vx;
+  ia.vy = c->vy;
+  ia.vz = c->vz;
+  if (c->lossless) {
+    ia.xlo = c->xlo; ia.ylo = c->ylo; ia.zlo = c->zlo;
+    ia.vxlo = c->vxlo; ia.vylo = c->vylo; ia.vzlo = c->vzlo;
+  }
+  ia.fx = c->fx;
+  ia.fy = c->fy;
+  ia.fz = c->fz;
+  ia.mass = c->mass;
+  ia.lattice = c->d_lattice;
+  ia.factor = c->have_nhc ? c->factors + 0 : nullptr;
+  ia.snap = settle ? c->snap : nullptr;
+  stage_begin(c, "vv_kick_drift", timed);
+  hd_launch_kick_drift(st, ia);
+  stage_end(c, timed);
+  if (settle) {
+    HdSettleArgs sa;
+    memset(&sa, 0, sizeof(sa));
+    sa.n_groups = (unsigned)(c->h_groups.size() / 3);
+    sa.groups = c->d_groups;
+    double total = c->m_o + 2.0 * c->m_h;
+    double h = sqrt(c->d_oh * c->d_oh - 0.25 * c->d_hh * c->d_hh);
+    sa.ra = (float)((2.0 * c->m_h / total) * h);
+    sa.rb = (float)((c->m_o / total) * h);
+    sa.rc = (float)(c->d_hh * 0.5);
+    sa.m_o = (float)c->m_o;
+    sa.m_h = (float)c->m_h;
+    sa.dt = (float)c->dt;
+    sa.posq = c->posq;
+    sa.snap = c->snap;
+    sa.vx = c->vx;
+    sa.vy = c->vy;
+    sa.vz = c->vz;
+    sa.w_cons = c->w_cons;
+    sa.lattice = c->d_lattice;
+    stage_begin(c, "settle_positions", timed);
+    hd_launch_settle_positions(st, sa);
+    stage_end(c, timed);
+    stage_begin(c, "settle_velocities", timed);
+    hd_launch_settle_velocities(st, sa, true);
+    stage_end(c, timed);
+  }
+  if (c->have_nhc) {
+    stage_begin(c, "nhc_post", timed);
+    hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
+                 (float)c->dt, c->factors + 1);
+    stage_end(c, timed);
+  }
+  enqueue_forces(c, ev, true, timed);
+  if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
+                       nullptr, 0.f, nullptr);
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+static int download(hd_ctx* c, T* dst, const T* src, size_t count) {
+  if (!dst) return HD_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return HD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hd_version(void) { return HD_VERSION; }
+
+int hd_ctx_create(int device, hd_ctx** out) {
+  if (!out) return HD_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return HD_ERR_HIP;
+  if (device < 0 || device >= count) return HD_ERR_ARG;
+  hd_ctx* c = new hd_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return HD_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HD_ERR_HIP;
+  }
+  *out = c;
+  return HD_OK;
+}
+
+void hd_ctx_destroy(hd_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
+  if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
+  for (auto& s : c->stages) {
+    (void)hipEventDestroy(s.a);
+    (void)hipEventDestroy(s.b);
+  }
+  for (void* p : c->allocs) (void)hipFree(p);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* hd_last_error(const hd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hd_set_system(hd_ctx* c, uint64_t n, const float* posq, const float* mass,
+                  const uint32_t* type, const uint32_t* key) {
+  if (!c || !posq || !mass || !type) return fail(c, HD_ERR_ARG, "hd_set_system: null argument");
+  if (n == 0 || n >= (1ull << 31)) return fail(c, HD_ERR_ARG, "hd_set_system: bad n %llu",
+                                               (unsigned long long)n);
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_system after hd_setup");
+  c->n = n;
+  c->h_posq.assign(posq, posq + 4 * n);
+  c->h_mass.assign(mass, mass + n);
+  c->h_type.assign(type, type + n);
+  c->h_key.resize(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    c->h_key[i] = key ? key[i] : (uint32_t)i;
+    if (c->h_key[i] >= HD_SENTINEL_KEY_LIMIT)
+      return fail(c, HD_ERR_ARG, "exclusion key %u too large", c->h_key[i]);
+    if (!(mass[i] > 0.f)) return fail(c, HD_ERR_ARG, "mass[%llu] must be positive",
+                                      (unsigned long long)i);
+  }
+  c->h_vx.assign(n, 0.f);
+  c->h_vy.assign(n, 0.f);
+  c->h_vz.assign(n, 0.f);
+  c->h_ix.assign(n, 0);
+  c->h_iy.assign(n, 0);
+  c->h_iz.assign(n, 0);
+  return HD_OK;
+}
+
+int hd_set_lattice(hd_ctx* c, const double* L) {
+  if (!c || !L) return HD_ERR_ARG;
+  if (!(L[0] > 0 && L[1] > 0 && L[2] > 0)) return fail(c, HD_ERR_ARG, "lattice lengths must be > 0");
+  memcpy(c->lattice, L, sizeof(double) * 6);
+  if (c->ready) {
+    float lf[6];
+    for (int i = 0; i < 6; ++i) lf[i] = (float)L[i];
+    HIPCHK(c, hipMemcpy(c->d_lattice, lf, sizeof(lf), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->ctrl, 0, sizeof(unsigned)));
+    unsigned force = 0x100u;
+    HIPCHK(c, hipMemcpy(c->ctrl, &force, sizeof(unsigned), hipMemcpyHostToDevice));
+  }
+  return HD_OK;
+}
+
+int hd_set_velocities(hd_ctx* c, const float* vx, const float* vy, const float* vz) {
+  if (!c || !vx || !vy || !vz) return HD_ERR_ARG;
+  if (!c->n) return fail(c, HD_ERR_STATE, "hd_set_velocities before hd_set_system");
+  if (c->ready) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = upload(c, c->vx, vx, c->n))) return r;
+    if ((r = upload(c, c->vy, vy, c->n))) return r;
+    if ((r = upload(c, c->vz, vz, c->n))) return r;
+    if (c->lossless) {
+      HIPCHK(c, hipMemset(c->vxlo, 0, sizeof(double) * c->n));
+      HIPCHK(c, hipMemset(c->vylo, 0, sizeof(double) * c->n));
+      HIPCHK(c, hipMemset(c->vzlo, 0, sizeof(double) * c->n));
+    }
+    return HD_OK;
+  }
+  c->h_vx.assign(vx, vx + c->n);
+  c->h_vy.assign(vy, vy + c->n);
+  c->h_vz.assign(vz, vz + c->n);
+  return HD_OK;
+}
+
+int hd_set_images(hd_ctx* c, const int32_t* ix, const int32_t* iy, const int32_t* iz) {
+  if (!c || !c->n) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_images after hd_setup");
+  for (uint64_t i = 0; i < c->n; ++i) {
+    c->h_ix[i] = ix ? ix[i] : 0;
+    c->h_iy[i] = iy ? iy[i] : 0;
+    c->h_iz[i] = iz ? iz[i] : 0;
+  }
+  return HD_OK;
+}
+
+int hd_set_lj(hd_ctx* c, int ntypes, const double* sigma, const double* eps, const double* cut,
+              const double* rsw) {
+  if (!c || !sigma || !eps || !cut || !rsw) return HD_ERR_ARG;
+  if (ntypes < 1 || ntypes > 4)
+    return fail(c, HD_ERR_UNSUPPORTED, "ntypes=%d: the pair kernel holds at most 4 types", ntypes);
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_lj after hd_setup");
+  int m = ntypes * ntypes;
+  for (int k = 0; k < m; ++k) {
+    if (!(cut[k] > 0) || rsw[k] > cut[k] || rsw[k] < 0 || sigma[k] < 0 || eps[k] < 0)
+      return fail(c, HD_ERR_ARG, "invalid LJ entry %d", k);
+    int i = k / ntypes, j = k % ntypes;
+    int t = j * ntypes + i;
+    if (sigma[k] != sigma[t] || eps[k] != eps[t] || cut[k] != cut[t] || rsw[k] != rsw[t])
+      return fail(c, HD_ERR_ARG, "LJ tables must be symmetric");
+  }
+  c->ntypes = ntypes;
+  c->lj_sigma.assign(sigma, sigma + m);
+  c->lj_eps.assign(eps, eps + m);
+  c->lj_cut.assign(cut, cut + m);
+  c->lj_rsw.assign(rsw, rsw + m);
+  return HD_OK;
+}
+
+int hd_set_spme(hd_ctx* c, double alpha, double r_cut, int na, int nb, int nc, int order,
+                double k_c) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_spme after hd_setup");
+  if (order != 4) return fail(c, HD_ERR_UNSUPPORTED, "spline order %d: only 4 is built", order);
+  if (!is_pow2(na) || !is_pow2(nb) || !is_pow2(nc) || na < 8 || nb < 8 || nc < 8 || na > 256 ||
+      nb > 256 || nc > 512)
+    return fail(c, HD_ERR_UNSUPPORTED, "SPME grid %dx%dx%d: powers of two in [8,256] required",
+                na, nb, nc);
+  if (!(alpha > 0) || !(r_cut > 0)) return fail(c, HD_ERR_ARG, "alpha and r_cut must be > 0");
+  c->have_spme = true;
+  c->alpha = alpha;
+  c->coul_rc = r_cut;
+  c->grid[0] = na;
+  c->grid[1] = nb;
+  c->grid[2] = nc;
+  c->kc = k_c;
+  return HD_OK;
+}
+
+int hd_set_neighbor(hd_ctx* c, double r_skin) {
+  if (!c || !(r_skin >= 0)) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_neighbor after hd_setup");
+  c->r_skin = r_skin;
+  return HD_OK;
+}
+
+int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, double d_hh,
+                  double m_o, double m_h) {
+  if (!c || (ng && !groups)) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_settle after hd_setup");
+  if (!(d_oh > 0 && d_hh > 0 && d_hh < 2 * d_oh && m_o > 0 && m_h > 0))
+    return fail(c, HD_ERR_ARG, "invalid SETTLE geometry");
+  std::vector<char> seen(c->n, 0);
+  for (uint64_t k = 0; k < 3 * ng; ++k) {
+    if (groups[k] >= c->n) return fail(c, HD_ERR_ARG, "SETTLE atom %u out of range", groups[k]);
+    if (seen[groups[k]]) return fail(c, HD_ERR_ARG, "atom %u in two SETTLE groups", groups[k]);
+    seen[groups[k]] = 1;
+  }
+  c->h_groups.assign(groups, groups + 3 * ng);
+  c->d_oh = d_oh;
+  c->d_hh = d_hh;
+  c->m_o = m_o;
+  c->m_h = m_h;
+  return HD_OK;
+}
+
+int hd_set_integrator(hd_ctx* c, double dt, int lossless) {
+  if (!c || !(dt > 0)) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_integrator after hd_setup");
+  c->dt = dt;
+  c->lossless = lossless != 0;
+  return HD_OK;
+}
+
+int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_resp,
+               uint64_t n_constraints) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_nhc after hd_setup");
+  if (kt <= 0) {
+    c->have_nhc = false;
+    return HD_OK;
+  }
+  if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
+      !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
+    return fail(c, HD_ERR_ARG, "invalid Nose-Hoover chain parameters");
+  c->have_nhc = true;
+  c->nhc_kt = kt;
+  c->nhc_tau = tau;
+  c->nhc_chain = chain;
+  c->nhc_yoshida = yoshida;
+  c->nhc_resp = n_resp;
+  c->nhc_nc = n_constraints;
+  return HD_OK;
+}
+
+static int build_list(hd_ctx* c) {
+  // force a rebuild and run the neighbour pipeline once
+  unsigned force = 0x100u;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl, &force, sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+  unsigned zero = 0;
+  HIPCHK(c, hipMemcpyAsync(c->ctrl + 2, &zero, sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+  HdNeighborArgs na = nb_args(c);
+  hd_launch_neighbor_update(c->stream, na);
+  hd_launch_tile_gather(c->stream, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
+                        c->fpj_ew, (unsigned)c->n, c->npad);
+  hd_launch_neighbor_build(c->stream, na);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+static int alloc_entries(hd_ctx* c, unsigned cap) {
+  if (c->entry_atoms) dfree(c, c->entry_atoms);
+  c->entry_atoms = nullptr;
+  c->cap = cap;
+  DALLOC(c, c->entry_atoms, (size_t)c->nblocks * cap * 64);
+  return HD_OK;
+}
+
+static void drop_graphs(hd_ctx* c) {
+  if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
+  if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
+  c->graph_f = c->graph_fev = nullptr;
+}
+
+int hd_setup(hd_ctx* c) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_setup called twice");
+  if (!c->n) return fail(c, HD_ERR_STATE, "hd_setup before hd_set_system");
+  if (!c->ntypes) return fail(c, HD_ERR_STATE, "hd_setup before hd_set_lj");
+  if (!(c->lattice[0] > 0)) return fail(c, HD_ERR_STATE, "hd_setup before hd_set_lattice");
+  for (uint64_t i = 0; i < c->n; ++i)
+    if (c->h_type[i] >= (uint32_t)c->ntypes) return fail(c, HD_ERR_ARG, "type[%llu] out of range",
+                                                         (unsigned long long)i);
+  if (c->lattice[3] != 0 || c->lattice[4] != 0 || c->lattice[5] != 0)
+    return fail(c, HD_ERR_UNSUPPORTED, "triclinic boxes are not built (orthorhombic only)");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->rmax = c->have_spme ? c->coul_rc : 0.0;
+  for (double v : c->lj_cut) c->rmax = std::max(c->rmax, v);
+  if (c->r_skin < 0) c->r_skin = 0.3 * c->rmax;
+  double rs = c->rmax + c->r_skin;
+  for (int d = 0; d < 3; ++d)
+    if (rs > 0.5 * c->lattice[d])
+      return fail(c, HD_ERR_UNSUPPORTED,
+                  "r_cut + r_skin = %g exceeds half the box (%g): minimum image breaks", rs,
+                  0.5 * c->lattice[d]);
+  unsigned n = (unsigned)c->n;
+  c->nblocks = (n + 63) / 64;
+  c->npad = c->nblocks * 64;
+  // voxel grid: about two atoms per voxel, proportional to the box edges
+  double vol = c->lattice[0] * c->lattice[1] * c->lattice[2];
+  double edge = cbrt(2.0 * vol / (double)n);
+  for (int d = 0; d < 3; ++d) c->nv[d] = (unsigned)std::max(1.0, std::min(1024.0, floor(c->lattice[d] / edge)));
+  c->nvox = c->nv[0] * c->nv[1] * c->nv[2];
+  std::vector<std::pair<uint64_t, uint32_t>> codes(c->nvox);
+  for (unsigned a = 0; a < c->nv[0]; ++a)
+    for (unsigned b = 0; b < c->nv[1]; ++b)
+      for (unsigned cc = 0; cc < c->nv[2]; ++cc) {
+        unsigned lin = (a * c->nv[1] + b) * c->nv[2] + cc;
+        codes[lin] = {morton3(a, b, cc), lin};
+      }
+  std::sort(codes.begin(), codes.end());
+  std::vector<unsigned> rank(c->nvox);
+  for (unsigned r = 0; r < c->nvox; ++r) rank[codes[r].second] = r;
+
+  DALLOC(c, c->posq, n);
+  DALLOC(c, c->vx, n); DALLOC(c, c->vy, n); DALLOC(c, c->vz, n);
+  DALLOC(c, c->fx, n); DALLOC(c, c->fy, n); DALLOC(c, c->fz, n);
+  DALLOC(c, c->mass, n); DALLOC(c, c->pe, n); DALLOC(c, c->vir, n); DALLOC(c, c->w_cons, n);
+  DALLOC(c, c->ix, n); DALLOC(c, c->iy, n); DALLOC(c, c->iz, n);
+  DALLOC(c, c->type, n); DALLOC(c, c->key, n);
+  DALLOC(c, c->snap, n); DALLOC(c, c->ref_pos, n);
+  if (c->lossless) {
+    DALLOC(c, c->xlo, n); DALLOC(c, c->ylo, n); DALLOC(c, c->zlo, n);
+    DALLOC(c, c->vxlo, n); DALLOC(c, c->vylo, n); DALLOC(c, c->vzlo, n);
+  }
+  DALLOC(c, c->d_lattice, 6);
+  DALLOC(c, c->voxel_rank, c->nvox); DALLOC(c, c->atom_key, n);
+  DALLOC(c, c->voxel_count, c->nvox); DALLOC(c, c->voxel_off, c->nvox + 1);
+  DALLOC(c, c->voxel_cursor, c->nvox);
+  DALLOC(c, c->sorted_id, c->npad); DALLOC(c, c->slot_of, n);
+  DALLOC(c, c->tile_type, c->npad); DALLOC(c, c->tile_key, c->npad);
+  DALLOC(c, c->tile_posq, c->npad);
+  DALLOC(c, c->center, c->nblocks); DALLOC(c, c->half, c->nblocks);
+  DALLOC(c, c->entry_count, c->nblocks);
+  DALLOC(c, c->ctrl, 4);
+  DALLOC(c, c->fpi, (size_t)c->npad * 4); DALLOC(c, c->fpj, (size_t)c->npad * 4);
+  DALLOC(c, c->fpi_ew, (size_t)c->npad * 2); DALLOC(c, c->fpj_ew, (size_t)c->npad * 2);
+  DALLOC(c, c->red_partials, 2 * HD_REDUCE_BLOCKS);
+  DALLOC(c, c->scal, 8);
+  DALLOC(c, c->factors, 2);
+  DALLOC(c, c->recip, 2);
+  DALLOC(c, c->w_share, 1);
+  int r;
+  if ((r = upload(c, c->posq, (const float4*)c->h_posq.data(), n))) return r;
+  if ((r = upload(c, c->vx, c->h_vx.data(), n))) return r;
+  if ((r = upload(c, c->vy, c->h_vy.data(), n))) return r;
+  if ((r = upload(c, c->vz, c->h_vz.data(), n))) return r;
+  if ((r = upload(c, c->mass, c->h_mass.data(), n))) return r;
+  if ((r = upload(c, c->ix, c->h_ix.data(), n))) return r;
+  if ((r = upload(c, c->iy, c->h_iy.data(), n))) return r;
+  if ((r = upload(c, c->iz, c->h_iz.data(), n))) return r;
+  if ((r = upload(c, c->type, c->h_type.data(), n))) return r;
+  if ((r = upload(c, c->key, c->h_key.data(), n))) return r;
+  if ((r = upload(c, c->voxel_rank, rank.data(), c->nvox))) return r;
+  {
+    float lf[6];
+    for (int i = 0; i < 6; ++i) lf[i] = (float)c->lattice[i];
+    if ((r = upload(c, c->d_lattice, lf, 6))) return r;
+    float ones[2] = {1.f, 1.f};
+    if ((r = upload(c, c->factors, ones, 2))) return r;
+  }
+  if (!c->h_groups.empty()) {
+    DALLOC(c, c->d_groups, c->h_groups.size());
+    if ((r = upload(c, c->d_groups, c->h_groups.data(), c->h_groups.size()))) return r;
+  }
+  if (c->have_spme) {
+    size_t M = (size_t)c->grid[0] * c->grid[1] * c->grid[2];
+    size_t Mc = (size_t)c->grid[0] * c->grid[1] * (c->grid[2] / 2 + 1);
+    DALLOC(c, c->rho_fixed, M);
+    DALLOC(c, c->cgrid, Mc);
+    DALLOC(c, c->vgrid, M);
+    auto ta = twiddles(c->grid[0]), tb = twiddles(c->grid[1]), tc = twiddles(c->grid[2]);
+    DALLOC(c, c->tw_a, ta.size()); DALLOC(c, c->tw_b, tb.size()); DALLOC(c, c->tw_c, tc.size());
+    if ((r = upload(c, c->tw_a, ta.data(), ta.size()))) return r;
+    if ((r = upload(c, c->tw_b, tb.data(), tb.size()))) return r;
+    if ((r = upload(c, c->tw_c, tc.data(), tc.size()))) return r;
+    auto ba = b_factors(c->grid[0], 4), bb = b_factors(c->grid[1], 4), bc = b_factors(c->grid[2], 4);
+    DALLOC(c, c->bf_a, ba.size()); DALLOC(c, c->bf_b, bb.size()); DALLOC(c, c->bf_c, bc.size());
+    if ((r = upload(c, c->bf_a, ba.data(), ba.size()))) return r;
+    if ((r = upload(c, c->bf_b, bb.data(), bb.size()))) return r;
+    if ((r = upload(c, c->bf_c, bc.data(), bc.size()))) return r;
+    DALLOC(c, c->recip_partials, hd_spme_partials_count(c->grid[1], c->grid[2]));
+  }
+  if (c->have_nhc) {
+    int m = c->nhc_chain;
+    std::vector<double> st(3 * m, 0.0);
+    long long dof = std::max(3LL * (long long)c->n - (long long)c->nhc_nc - 3LL, 0LL);
+    double t2 = c->nhc_tau * c->nhc_tau;
+    st[2 * m] = (double)dof * c->nhc_kt * t2;
+    for (int j = 1; j < m; ++j) st[2 * m + j] = c->nhc_kt * t2;
+    DALLOC(c, c->nhc_state, 3 * m);
+    if ((r = upload(c, c->nhc_state, st.data(), st.size()))) return r;
+  }
+  // probe the list size, then size the entry regions with headroom
+  if ((r = alloc_entries(c, 64))) return r;
+  if ((r = build_list(c))) return r;
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  unsigned need = ctrl[2];
+  unsigned cap = std::max(4u, (unsigned)ceil(need * 1.5) + 2);
+  if (cap != c->cap) {
+    if ((r = alloc_entries(c, cap))) return r;
+    if ((r = build_list(c))) return r;
+  }
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  if (ctrl[0] & 8u) return fail(c, HD_ERR_OVERFLOW, "neighbour list overflow at setup");
+  unsigned zero = 0;
+  HIPCHK(c, hipMemcpy(c->ctrl, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  c->ready = true;
+  return HD_OK;
+}
+
+static int check_status(hd_ctx* c) {
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpyAsync(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ctrl[0] & 8u) {
+    return fail(c, HD_ERR_OVERFLOW,
+                "neighbour list overflow (a block needed %u entries, capacity %u)", ctrl[2],
+                c->cap);
+  }
+  if (ctrl[0] & 2u) {
+    // high water: grow the entry regions and rebuild before the next step
+    unsigned cap = (unsigned)ceil(ctrl[2] * 1.5) + 2;
+    drop_graphs(c);
+    int r = alloc_entries(c, std::max(cap, c->cap));
+    if (r) return r;
+    unsigned zero = 0;
+    HIPCHK(c, hipMemcpy(c->ctrl, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+    if ((r = build_list(c))) return r;
+    HIPCHK(c, hipMemcpy(c->ctrl, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  }
+  return HD_OK;
+}
+
+int hd_force_eval(hd_ctx* c, int want_scalars) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_force_eval before hd_setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  enqueue_forces(c, want_scalars != 0, false, false);
+  c->forces_valid = true;
+  if (want_scalars)
+    hd_launch_ke(c->stream, (unsigned)c->n, c->vx, c->vy, c->vz, c->mass, c->red_partials,
+                 c->scal + 0, nullptr, 0.f, nullptr);
+  HIPCHK(c, hipGetLastError());
+  return check_status(c);
+}
+
+int hd_constrain_velocities(hd_ctx* c) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_constrain_velocities before hd_setup");
+  if (c->h_groups.empty()) return HD_OK;
+  HdSettleArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.n_groups = (unsigned)(c->h_groups.size() / 3);
+  sa.groups = c->d_groups;
+  sa.m_o = (float)c->m_o;
+  sa.m_h = (float)c->m_h;
+  sa.dt = 0.f;
+  sa.posq = c->posq;
+  sa.vx = c->vx;
+  sa.vy = c->vy;
+  sa.vz = c->vz;
+  sa.w_cons = c->w_cons;
+  sa.lattice = c->d_lattice;
+  hd_launch_settle_velocities(c->stream, sa, false);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+static int capture(hd_ctx* c, bool ev, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(c, ev, false);
+  HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+  HIPCHK(c, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  return HD_OK;
+}
+
+int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_step before hd_setup");
+  if (n_steps < 0) return fail(c, HD_ERR_ARG, "n_steps < 0");
+  if (!(c->dt > 0)) return fail(c, HD_ERR_STATE, "hd_step before hd_set_integrator");
+  HIPCHK(c, hipSetDevice(c->device));
+  bool timed = c->timing && !c->use_graph;
+  if (!c->forces_valid) {
+    enqueue_forces(c, false, false, false);
+    c->forces_valid = true;
+  }
+  for (int s = 0; s < n_steps; ++s) {
+    bool ev = scalars_last && s == n_steps - 1;
+    if (c->use_graph) {
+      hipGraphExec_t* gx = ev ? &c->graph_fev : &c->graph_f;
+      if (!*gx) {
+        int r = capture(c, ev, gx);
+        if (r) return r;
+      }
+      HIPCHK(c, hipGraphLaunch(*gx, c->stream));
+    } else {
+      if (timed) {
+        for (auto& st : c->stages) {
+          (void)hipEventDestroy(st.a);
+          (void)hipEventDestroy(st.b);
+        }
+        c->stages.clear();
+      }
+      enqueue_step(c, ev, timed);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  int r = check_status(c);
+  if (r) return r;
+  if (timed) {
+    c->stage_ms.clear();
+    for (auto& st : c->stages) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, st.a, st.b);
+      c->stage_ms.push_back(ms);
+    }
+  }
+  return HD_OK;
+}
+
+int hd_use_graph(hd_ctx* c, int enable) {
+  if (!c) return HD_ERR_ARG;
+  c->use_graph = enable != 0;
+  return HD_OK;
+}
+
+int hd_set_timing(hd_ctx* c, int enable) {
+  if (!c) return HD_ERR_ARG;
+  c->timing = enable != 0;
+  return HD_OK;
+}
+
+int hd_stage_times(hd_ctx* c, char* names, int name_len, double* ms, int cap) {
+  if (!c || !names || !ms || name_len <= 0) return HD_ERR_ARG;
+  int k = 0;
+  for (size_t i = 0; i < c->stage_ms.size() && k < cap; ++i, ++k) {
+    snprintf(names + (size_t)k * name_len, name_len, "%s", c->stages[i].name.c_str());
+    ms[k] = c->stage_ms[i];
+  }
+  return k;
+}
+
+int hd_sync(hd_ctx* c) {
+  if (!c) return HD_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_get_positions(hd_ctx* c, float* posq, int32_t* ix, int32_t* iy, int32_t* iz) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "not set up");
+  int r;
+  if ((r = download(c, (float4*)posq, c->posq, c->n))) return r;
+  if ((r = download(c, ix, c->ix, c->n))) return r;
+  if ((r = download(c, iy, c->iy, c->n))) return r;
+  if ((r = download(c, iz, c->iz, c->n))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_get_velocities(hd_ctx* c, float* vx, float* vy, float* vz) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "not set up");
+  int r;
+  if ((r = download(c, vx, c->vx, c->n))) return r;
+  if ((r = download(c, vy, c->vy, c->n))) return r;
+  if ((r = download(c, vz, c->vz, c->n))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_get_forces(hd_ctx* c, float* fx, float* fy, float* fz) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "not set up");
+  int r;
+  if ((r = download(c, fx, c->fx, c->n))) return r;
+  if ((r = download(c, fy, c->fy, c->n))) return r;
+  if ((r = download(c, fz, c->fz, c->n))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_get_per_atom(hd_ctx* c, float* pe, float* vir) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "not set up");
+  int r;
+  if ((r = download(c, pe, c->pe, c->n))) return r;
+  if ((r = download(c, vir, c->vir, c->n))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_get_lattice(hd_ctx* c, double* L) {
+  if (!c || !L) return HD_ERR_ARG;
+  if (!c->ready) {
+    memcpy(L, c->lattice, sizeof(double) * 6);
+    return HD_OK;
+  }
+  float lf[6];
+  HIPCHK(c, hipMemcpyAsync(lf, c->d_lattice, sizeof(lf), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 6; ++i) L[i] = lf[i];
+  return HD_OK;
+}
+
+int hd_get_scalars(hd_ctx* c, double* out) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  double s[8];
+  HIPCHK(c, hipMemcpyAsync(s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  double rc[2] = {0, 0};
+  if (c->have_spme) HIPCHK(c, hipMemcpyAsync(rc, c->recip, sizeof(rc), hipMemcpyDeviceToHost, c->stream));
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpyAsync(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> st;
+  if (c->have_nhc) {
+    st.resize(3 * c->nhc_chain);
+    HIPCHK(c, hipMemcpyAsync(st.data(), c->nhc_state, st.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = s[0];
+  out[1] = s[1];
+  out[2] = s[2];
+  out[3] = rc[0];
+  out[4] = rc[1];
+  double chain = 0.0;
+  if (c->have_nhc) {
+    // conserved-quantity chain term (nose_hoover_chain.rs:365-381)
+    int m = c->nhc_chain;
+    long long dof = std::max(3LL * (long long)c->n - (long long)c->nhc_nc - 3LL, 0LL);
+    for (int j = 0; j < m; ++j) chain += st[m + j] * st[m + j] / (2.0 * st[2 * m + j]);
+    chain += (double)dof * c->nhc_kt * st[0];
+    for (int j = 1; j < m; ++j) chain += c->nhc_kt * st[j];
+  }
+  out[5] = chain;
+  out[6] = ctrl[1];
+  out[7] = ctrl[0];
+  return HD_OK;
+}
+
+int hd_get_pairs(hd_ctx* c, uint32_t* out, uint64_t cap, uint64_t* n_pairs) {
+  if (!c || !c->ready || !n_pairs) return fail(c, HD_ERR_STATE, "not set up");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<unsigned> ents((size_t)c->nblocks * c->cap * 64), cnt(c->nblocks), sid(c->npad);
+  HIPCHK(c, hipMemcpy(ents.data(), c->entry_atoms, ents.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cnt.data(), c->entry_count, cnt.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(sid.data(), c->sorted_id, sid.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> pairs;
+  unsigned n = (unsigned)c->n;
+  for (unsigned b = 0; b < c->nblocks; ++b) {
+    for (unsigned e = 0; e < cnt[b]; ++e)
+      for (unsigned l = 0; l < 64; ++l) {
+        unsigned js = ents[((size_t)b * c->cap + e) * 64 + l];
+        if (js >= n) continue;
+        unsigned jid = sid[js];
+        for (unsigned m = 0; m < 64; ++m) {
+          unsigned is = b * 64 + m;
+          if (is >= n || is == js) continue;
+          unsigned iid = sid[is];
+          uint32_t lo = std::min(iid, jid), hi = std::max(iid, jid);
+          pairs.push_back(((uint64_t)lo << 32) | hi);
+        }
+      }
+  }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  *n_pairs = pairs.size();
+  if (out)
+    for (uint64_t k = 0; k < std::min<uint64_t>(cap, pairs.size()); ++k) {
+      out[2 * k] = (uint32_t)(pairs[k] >> 32);
+      out[2 * k + 1] = (uint32_t)(pairs[k] & 0xFFFFFFFFu);
+    }
+  return HD_OK;
+}
+
+int hd_get_neighbor_stats(hd_ctx* c, uint64_t* out) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<unsigned> cnt(c->nblocks);
+  HIPCHK(c, hipMemcpy(cnt.data(), c->entry_count, cnt.size() * 4, hipMemcpyDeviceToHost));
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  uint64_t tot = 0, mx = 0;
+  for (unsigned v : cnt) {
+    tot += v;
+    mx = std::max<uint64_t>(mx, v);
+  }
+  out[0] = c->nblocks;
+  out[1] = c->cap;
+  out[2] = mx;
+  out[3] = tot;
+  out[4] = ctrl[1];
+  out[5] = ctrl[0];
+  return HD_OK;
+}
+
+int hd_get_spme_grids(hd_ctx* c, float* rho, float* v) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "not set up");
+  if (!c->have_spme) return fail(c, HD_ERR_STATE, "no SPME configured");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t M = (size_t)c->grid[0] * c->grid[1] * c->grid[2];
+  if (rho) {
+    std::vector<long long> fx(M);
+    HIPCHK(c, hipMemcpy(fx.data(), c->rho_fixed, M * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < M; ++i) rho[i] = (float)fx[i] * (1.0f / 4294967296.0f);
+  }
+  if (v) HIPCHK(c, hipMemcpy(v, c->vgrid, M * 4, hipMemcpyDeviceToHost));
+  return HD_OK;
+}
+
+int hd_selftest_dpp(hd_ctx* c, int32_t* out) {
+  if (!c || !out) return HD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, 64 * sizeof(int)));
+  hd_launch_selftest_dpp(c->stream, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, 64 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "selftest: %s", hipGetErrorString(e));
+  return HD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,577 @@
+// Integration, constraints, reductions and the force finalize on gfx950.
+//
+// Arithmetic in this file is kept operation-for-operation identical to the
+// reference's single-precision kernels (no FMA contraction), so the f32
+// velocity-Verlet update is bit-exact against a scalar restatement:
+//   * velocity Verlet kick/drift, f32 and compensated (f32, f64)
+//     (kernels/integrate.cu:24-263);
+//   * SETTLE positions / velocities (kernels/settle.cu:145-497);
+//   * Nose-Hoover chain (src/integrator/nose_hoover_chain.rs:109-279), here
+//     advanced on the device in FP64 so the step needs no host round trip;
+//   * the post-force kick fragment and thermostat factor
+//     (src/integrator/velocity_verlet.rs:75-124,
+//      src/integrator/nose_hoover_chain.rs:282-300).
+// Reductions (kinetic energy, potential energy, virial) are FP64 with a
+// fixed chunking and a fixed pairwise tree, so they are bit-identical run
+// to run (the reference's f32 trees: kernels/nose_hoover.cu:22-105,
+// kernels/barostat.cu:11-78).
+#include "hd_device.h"
+#include "hd_kernels.h"
+
+#pragma clang fp contract(off)
+
+template <bool LOSSLESS, bool SNAP>
+__global__ void k_kick_drift(HdIntegrateArgs a) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  float f = a.factor ? a.factor[0] : 1.0f;
+  float m = a.mass[i];
+  float ax = a.fx[i] / m, ay = a.fy[i] / m, az = a.fz[i] / m;
+  float half_dt = a.dt * 0.5f;
+  float4 pq = a.posq[i];
+  if (SNAP) a.snap[i] = pq;
+  float vx = a.vx[i] * f, vy = a.vy[i] * f, vz = a.vz[i] * f;
+  int nx = a.ix[i], ny = a.iy[i], nz = a.iz[i];
+  float px, py, pz;
+  if (LOSSLESS) {
+    double ex = (double)vx + a.vxlo[i] + (double)(ax * half_dt);
+    double ey = (double)vy + a.vylo[i] + (double)(ay * half_dt);
+    double ez = (double)vz + a.vzlo[i] + (double)(az * half_dt);
+    vx = (float)ex; vy = (float)ey; vz = (float)ez;
+    double lx = ex - (double)vx, ly = ey - (double)vy, lz = ez - (double)vz;
+    a.vxlo[i] = lx; a.vylo[i] = ly; a.vzlo[i] = lz;
+    double dx = ((double)vx + lx) * (double)a.dt;
+    double dy = ((double)vy + ly) * (double)a.dt;
+    double dz = ((double)vz + lz) * (double)a.dt;
+    double qx = (double)pq.x + a.xlo[i] + dx;
+    double qy = (double)pq.y + a.ylo[i] + dy;
+    double qz = (double)pq.z + a.zlo[i] + dz;
+    px = (float)qx; py = (float)qy; pz = (float)qz;
+    a.xlo[i] = qx - (double)px; a.ylo[i] = qy - (double)py; a.zlo[i] = qz - (double)pz;
+  } else {
+    vx = vx + ax * half_dt;
+    vy = vy + ay * half_dt;
+    vz = vz + az * half_dt;
+    px = pq.x + vx * a.dt;
+    py = pq.y + vy * a.dt;
+    pz = pq.z + vz * a.dt;
+  }
+  int ka, kb, kc;
+  hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+  a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+  a.posq[i] = make_float4(px, py, pz, pq.w);
+  a.ix[i] = nx + ka; a.iy[i] = ny + kb; a.iz[i] = nz + kc;
+}
+
+void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a) {
+  if (a.n == 0) return;
+  dim3 g((a.n + 255) / 256), b(256);
+  bool ll = a.xlo != nullptr, sn = a.snap != nullptr;
+  if (ll && sn) hipLaunchKernelGGL((k_kick_drift<true, true>), g, b, 0, st, a);
+  else if (ll) hipLaunchKernelGGL((k_kick_drift<true, false>), g, b, 0, st, a);
+  else if (sn) hipLaunchKernelGGL((k_kick_drift<false, true>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_kick_drift<false, false>), g, b, 0, st, a);
+}
+
+// ---------------------------------------------------------------------
+// SETTLE
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float settle_csqrt(float x) { return sqrtf(x > 0.f ? x : 0.f); }
+
+__global__ void k_settle_positions(HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
+  float4 p[3] = {a.posq[idx[0]], a.posq[idx[1]], a.posq[idx[2]]};
+  float4 q[3] = {a.snap[idx[0]], a.snap[idx[1]], a.snap[idx[2]]};
+  float cur[3][3] = {{p[0].x, p[0].y, p[0].z}, {p[1].x, p[1].y, p[1].z}, {p[2].x, p[2].y, p[2].z}};
+  float snp[3][3] = {{q[0].x, q[0].y, q[0].z}, {q[1].x, q[1].y, q[1].z}, {q[2].x, q[2].y, q[2].z}};
+  float xb0 = snp[1][0] - snp[0][0], yb0 = snp[1][1] - snp[0][1], zb0 = snp[1][2] - snp[0][2];
+  float xc0 = snp[2][0] - snp[0][0], yc0 = snp[2][1] - snp[0][1], zc0 = snp[2][2] - snp[0][2];
+  hd_min_image(xb0, yb0, zb0, L);
+  hd_min_image(xc0, yc0, zc0, L);
+  float xp[3][3];
+  for (int k = 0; k < 3; ++k) {
+    float dx = cur[k][0] - snp[k][0], dy = cur[k][1] - snp[k][1], dz = cur[k][2] - snp[k][2];
+    hd_min_image(dx, dy, dz, L);
+    xp[k][0] = dx; xp[k][1] = dy; xp[k][2] = dz;
+  }
+  float m0 = a.m_o, m1 = a.m_h, m2 = a.m_h;
+  float itm = 1.0f / (m0 + m1 + m2);
+  float ra = a.ra, rb = a.rb, rc = a.rc;
+  float d2 = 2.0f * rc, d2sq = d2 * d2;
+  float xcom = (xp[0][0] * m0 + (xb0 + xp[1][0]) * m1 + (xc0 + xp[2][0]) * m2) * itm;
+  float ycom = (xp[0][1] * m0 + (yb0 + xp[1][1]) * m1 + (yc0 + xp[2][1]) * m2) * itm;
+  float zcom = (xp[0][2] * m0 + (zb0 + xp[1][2]) * m1 + (zc0 + xp[2][2]) * m2) * itm;
+  float xa1 = xp[0][0] - xcom, ya1 = xp[0][1] - ycom, za1 = xp[0][2] - zcom;
+  float xb1 = xb0 + xp[1][0] - xcom, yb1 = yb0 + xp[1][1] - ycom, zb1 = zb0 + xp[1][2] - zcom;
+  float xc1 = xc0 + xp[2][0] - xcom, yc1 = yc0 + xp[2][1] - ycom, zc1 = zc0 + xp[2][2] - zcom;
+  float ez_x = yb0 * zc0 - zb0 * yc0, ez_y = zb0 * xc0 - xb0 * zc0, ez_z = xb0 * yc0 - yb0 * xc0;
+  float ex_x = ya1 * ez_z - za1 * ez_y, ex_y = za1 * ez_x - xa1 * ez_z, ex_z = xa1 * ez_y - ya1 * ez_x;
+  float ey_x = ez_y * ex_z - ez_z * ex_y, ey_y = ez_z * ex_x - ez_x * ex_z, ey_z = ez_x * ex_y - ez_y * ex_x;
+  float exl = settle_csqrt(ex_x * ex_x + ex_y * ex_y + ex_z * ex_z);
+  float eyl = settle_csqrt(ey_x * ey_x + ey_y * ey_y + ey_z * ey_z);
+  float ezl = settle_csqrt(ez_x * ez_x + ez_y * ez_y + ez_z * ez_z);
+  float iex = exl > 0.f ? 1.0f / exl : 0.f, iey = eyl > 0.f ? 1.0f / eyl : 0.f;
+  float iez = ezl > 0.f ? 1.0f / ezl : 0.f;
+  float r11 = ex_x * iex, r21 = ex_y * iex, r31 = ex_z * iex;
+  float r12 = ey_x * iey, r22 = ey_y * iey, r32 = ey_z * iey;
+  float r13 = ez_x * iez, r23 = ez_y * iez, r33 = ez_z * iez;
+  float b0x = r11 * xb0 + r21 * yb0 + r31 * zb0, b0y = r12 * xb0 + r22 * yb0 + r32 * zb0;
+  float c0x = r11 * xc0 + r21 * yc0 + r31 * zc0, c0y = r12 * xc0 + r22 * yc0 + r32 * zc0;
+  float a1z = r13 * xa1 + r23 * ya1 + r33 * za1;
+  float b1x = r11 * xb1 + r21 * yb1 + r31 * zb1, b1y = r12 * xb1 + r22 * yb1 + r32 * zb1;
+  float b1z = r13 * xb1 + r23 * yb1 + r33 * zb1;
+  float c1x = r11 * xc1 + r21 * yc1 + r31 * zc1, c1y = r12 * xc1 + r22 * yc1 + r32 * zc1;
+  float c1z = r13 * xc1 + r23 * yc1 + r33 * zc1;
+  float sinphi = a1z / ra;
+  float cosphi = settle_csqrt(1.0f - sinphi * sinphi);
+  float sinpsi = (b1z - c1z) / (2.0f * rc * cosphi);
+  float cospsi = settle_csqrt(1.0f - sinpsi * sinpsi);
+  float a2y = ra * cosphi;
+  float b2x = -rc * cospsi;
+  float b2y = -rb * cosphi - rc * sinpsi * sinphi;
+  float c2y = -rb * cosphi + rc * sinpsi * sinphi;
+  float b2xsq = b2x * b2x;
+  float hhsq = 4.0f * b2xsq + (b2y - c2y) * (b2y - c2y) + (b1z - c1z) * (b1z - c1z);
+  float delta_x = 2.0f * b2x + settle_csqrt(4.0f * b2xsq - hhsq + d2sq);
+  b2x -= delta_x * 0.5f;
+  float al = b2x * (b0x - c0x) + b0y * b2y + c0y * c2y;
+  float be = b2x * (c0y - b0y) + b0x * b2y + c0x * c2y;
+  float ga = b0x * b1y - b1x * b0y + c0x * c1y - c1x * c0y;
+  float ab2 = al * al + be * be;
+  float iab2 = ab2 > 0.f ? 1.0f / ab2 : 0.f;
+  float sinth = (al * ga - be * settle_csqrt(ab2 - ga * ga)) * iab2;
+  float costh = settle_csqrt(1.0f - sinth * sinth);
+  float a3x = -a2y * sinth, a3y = a2y * costh, a3z = a1z;
+  float b3x = b2x * costh - b2y * sinth, b3y = b2x * sinth + b2y * costh, b3z = b1z;
+  float c3x = -b2x * costh - c2y * sinth, c3y = -b2x * sinth + c2y * costh, c3z = c1z;
+  float rcom[3][3] = {
+      {r11 * a3x + r12 * a3y + r13 * a3z, r21 * a3x + r22 * a3y + r23 * a3z, r31 * a3x + r32 * a3y + r33 * a3z},
+      {r11 * b3x + r12 * b3y + r13 * b3z, r21 * b3x + r22 * b3y + r23 * b3z, r31 * b3x + r32 * b3y + r33 * b3z},
+      {r11 * c3x + r12 * c3y + r13 * c3z, r21 * c3x + r22 * c3y + r23 * c3z, r31 * c3x + r32 * c3y + r33 * c3z}};
+  float com[3] = {xcom, ycom, zcom};
+  float unc[3][3] = {{xp[0][0], xp[0][1], xp[0][2]},
+                     {xb0 + xp[1][0], yb0 + xp[1][1], zb0 + xp[1][2]},
+                     {xc0 + xp[2][0], yc0 + xp[2][1], zc0 + xp[2][2]}};
+  float mat[3] = {m0, m1, m2};
+  float idt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
+  float idt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+  for (int k = 0; k < 3; ++k) {
+    float c0 = (com[0] + rcom[k][0]) - unc[k][0];
+    float c1 = (com[1] + rcom[k][1]) - unc[k][1];
+    float c2 = (com[2] + rcom[k][2]) - unc[k][2];
+    unsigned id = idx[k];
+    a.vx[id] += c0 * idt;
+    a.vy[id] += c1 * idt;
+    a.vz[id] += c2 * idt;
+    a.posq[id] = make_float4(cur[k][0] + c0, cur[k][1] + c1, cur[k][2] + c2, p[k].w);
+    a.w_cons[id] = mat[k] * idt2 * (c0 * rcom[k][0] + c1 * rcom[k][1] + c2 * rcom[k][2]);
+  }
+}
+
+template <bool ACC>
+__global__ void k_settle_velocities(HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned i0 = a.groups[3 * g], i1 = a.groups[3 * g + 1], i2 = a.groups[3 * g + 2];
+  float4 p0 = a.posq[i0], p1 = a.posq[i1], p2 = a.posq[i2];
+  float px[3] = {p0.x, p1.x, p2.x}, py[3] = {p0.y, p1.y, p2.y}, pz[3] = {p0.z, p1.z, p2.z};
+  for (int k = 1; k < 3; ++k) {
+    float dx = px[k] - px[0], dy = py[k] - py[0], dz = pz[k] - pz[0];
+    hd_min_image(dx, dy, dz, L);
+    px[k] = px[0] + dx; py[k] = py[0] + dy; pz[k] = pz[0] + dz;
+  }
+  float m_o = a.m_o, m_h = a.m_h;
+  float io = m_o > 0.f ? 1.0f / m_o : 0.f, ih = m_h > 0.f ? 1.0f / m_h : 0.f;
+  float vx[3] = {a.vx[i0], a.vx[i1], a.vx[i2]};
+  float vy[3] = {a.vy[i0], a.vy[i1], a.vy[i2]};
+  float vz[3] = {a.vz[i0], a.vz[i1], a.vz[i2]};
+  float r1x = px[0] - px[1], r1y = py[0] - py[1], r1z = pz[0] - pz[1];
+  float r2x = px[0] - px[2], r2y = py[0] - py[2], r2z = pz[0] - pz[2];
+  float r3x = px[1] - px[2], r3y = py[1] - py[2], r3z = pz[1] - pz[2];
+  float d11 = r1x * r1x + r1y * r1y + r1z * r1z;
+  float d22 = r2x * r2x + r2y * r2y + r2z * r2z;
+  float d33 = r3x * r3x + r3y * r3y + r3z * r3z;
+  float d12 = r1x * r2x + r1y * r2y + r1z * r2z;
+  float d13 = r1x * r3x + r1y * r3y + r1z * r3z;
+  float d23 = r2x * r3x + r2y * r3y + r2z * r3z;
+  float b1 = (vx[0] - vx[1]) * r1x + (vy[0] - vy[1]) * r1y + (vz[0] - vz[1]) * r1z;
+  float b2 = (vx[0] - vx[2]) * r2x + (vy[0] - vy[2]) * r2y + (vz[0] - vz[2]) * r2z;
+  float b3 = (vx[1] - vx[2]) * r3x + (vy[1] - vy[2]) * r3y + (vz[1] - vz[2]) * r3z;
+  float ioh = io + ih;
+  float m[3][3] = {{d11 * ioh, d12 * io, -d13 * ih},
+                   {d12 * io, d22 * ioh, d23 * ih},
+                   {-d13 * ih, d23 * ih, 2.0f * d33 * ih}};
+  float rhs[3] = {-b1, -b2, -b3};
+  float det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+              m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+              m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  float gg[3] = {0.f, 0.f, 0.f};
+  if (det != 0.f) {
+    float idet = 1.0f / det;
+    for (int c = 0; c < 3; ++c) {
+      float t[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) t[r][q] = (q == c) ? rhs[r] : m[r][q];
+      float dc = t[0][0] * (t[1][1] * t[2][2] - t[1][2] * t[2][1]) -
+                 t[0][1] * (t[1][0] * t[2][2] - t[1][2] * t[2][0]) +
+                 t[0][2] * (t[1][0] * t[2][1] - t[1][1] * t[2][0]);
+      gg[c] = dc * idet;
+    }
+  }
+  float g1 = gg[0], g2 = gg[1], g3 = gg[2];
+  float dv[3][3] = {
+      {(g1 * r1x + g2 * r2x) * io, (g1 * r1y + g2 * r2y) * io, (g1 * r1z + g2 * r2z) * io},
+      {(-g1 * r1x + g3 * r3x) * ih, (-g1 * r1y + g3 * r3y) * ih, (-g1 * r1z + g3 * r3z) * ih},
+      {(-g2 * r2x - g3 * r3x) * ih, (-g2 * r2y - g3 * r3y) * ih, (-g2 * r2z - g3 * r3z) * ih}};
+  unsigned ids[3] = {i0, i1, i2};
+  for (int k = 0; k < 3; ++k) {
+    a.vx[ids[k]] = vx[k] + dv[k][0];
+    a.vy[ids[k]] = vy[k] + dv[k][1];
+    a.vz[ids[k]] = vz[k] + dv[k][2];
+  }
+  if (ACC && a.dt > 0.f) {
+    float idt = 1.0f / a.dt;
+    float tot = m_o + 2.0f * m_h;
+    float iM = tot > 0.f ? 1.0f / tot : 0.f;
+    float cx = (m_o * px[0] + m_h * px[1] + m_h * px[2]) * iM;
+    float cy = (m_o * py[0] + m_h * py[1] + m_h * py[2]) * iM;
+    float cz = (m_o * pz[0] + m_h * pz[1] + m_h * pz[2]) * iM;
+    float ma[3] = {m_o, m_h, m_h};
+    for (int k = 0; k < 3; ++k) {
+      float sc = ma[k] * idt;
+      a.w_cons[ids[k]] += sc * (dv[k][0] * (px[k] - cx) + dv[k][1] * (py[k] - cy) +
+                               dv[k][2] * (pz[k] - cz));
+    }
+  }
+}
+
+void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a) {
+  if (!a.n_groups) return;
+  hipLaunchKernelGGL(k_settle_positions, dim3((a.n_groups + 127) / 128), dim3(128), 0, st, a);
+}
+
+void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool acc) {
+  if (!a.n_groups) return;
+  if (acc)
+    hipLaunchKernelGGL(k_settle_velocities<true>, dim3((a.n_groups + 127) / 128), dim3(128), 0,
+                       st, a);
+  else
+    hipLaunchKernelGGL(k_settle_velocities<false>, dim3((a.n_groups + 127) / 128), dim3(128), 0,
+                       st, a);
+}
+
+// ---------------------------------------------------------------------
+// Deterministic FP64 reductions
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double block_tree_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (unsigned s = 1; s < blockDim.x; s <<= 1) {
+    if ((threadIdx.x % (2 * s)) == 0) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Block b owns the contiguous chunk [b*chunk, (b+1)*chunk).
+__global__ void __launch_bounds__(256) k_ke_partials(unsigned n, const float* __restrict__ vx,
+                                                     const float* __restrict__ vy,
+                                                     const float* __restrict__ vz,
+                                                     const float* __restrict__ mass,
+                                                     double* __restrict__ partials) {
+  __shared__ double sh[256];
+  unsigned chunk = (n + gridDim.x - 1) / gridDim.x;
+  unsigned b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, n);
+  double s = 0.0;
+  for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    double x = vx[i], y = vy[i], z = vz[i];
+    s += 0.5 * (double)mass[i] * (x * x + y * y + z * z);
+  }
+  double t = block_tree_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// One Nose-Hoover chain sub-step (nose_hoover_chain.rs:109-180), FP64.
+__device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, double dt,
+                               double k, double g_dof, double kt) {
+  for (int j = m - 1; j >= 0; --j) {
+    double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
+    pxi[j] *= s;
+    double gj = (j == 0) ? k - g_dof * kt : pxi[j - 1] * pxi[j - 1] / q[j - 1] - kt;
+    pxi[j] += dt / 4.0 * gj;
+    pxi[j] *= s;
+  }
+  double factor = exp(-dt / 2.0 * pxi[0] / q[0]);
+  k *= factor * factor;
+  for (int j = 0; j < m; ++j) xi[j] += dt / 2.0 * pxi[j] / q[j];
+  for (int j = 0; j < m; ++j) {
+    double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
+    pxi[j] *= s;
+    double gj = (j == 0) ? k - g_dof * kt : pxi[j - 1] * pxi[j - 1] / q[j - 1] - kt;
+    pxi[j] += dt / 4.0 * gj;
+    pxi[j] *= s;
+  }
+  return factor;
+}
+
+__global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ partials,
+                                                  unsigned np, double* __restrict__ ke,
+                                                  HdNhcArgs nhc, int use_nhc, float dt,
+                                                  float* __restrict__ factor) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+  double k = block_tree_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  ke[0] = k;
+  if (!use_nhc) return;
+  int m = nhc.chain;
+  double* xi = nhc.state;
+  double* pxi = nhc.state + m;
+  const double* q = nhc.state + 2 * m;
+  double cum = 1.0;
+  for (int y = 0; y < nhc.n_yoshida; ++y)
+    for (int r = 0; r < nhc.n_resp; ++r) {
+      double delta = nhc.w[y] * (double)dt / (2.0 * nhc.n_resp);
+      double f = nhc_sub_step(m, xi, pxi, q, delta, 2.0 * k, nhc.g_dof, nhc.kt);
+      cum *= f;
+      k *= f * f;
+    }
+  factor[0] = (float)cum;
+}
+
+void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
+                  const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
+                  float dt, float* factor) {
+  hipLaunchKernelGGL(k_ke_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, vx, vy, vz,
+                     mass, partials);
+  HdNhcArgs na;
+  memset(&na, 0, sizeof(na));
+  if (nhc) na = *nhc;
+  hipLaunchKernelGGL(k_ke_final, dim3(1), dim3(256), 0, st, partials, (unsigned)HD_REDUCE_BLOCKS,
+                     ke, na, nhc ? 1 : 0, dt, factor);
+}
+
+__global__ void __launch_bounds__(256) k_sum2_partials(unsigned n, const float* __restrict__ a,
+                                                       const float* __restrict__ b,
+                                                       double* __restrict__ partials) {
+  __shared__ double sh[256];
+  unsigned chunk = (n + gridDim.x - 1) / gridDim.x;
+  unsigned b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, n);
+  double sa = 0.0, sb = 0.0;
+  for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    sa += (double)a[i];
+    sb += (double)b[i];
+  }
+  double ta = block_tree_sum(sa, sh);
+  __syncthreads();
+  double tb = block_tree_sum(sb, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = ta;
+    partials[2 * blockIdx.x + 1] = tb;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sum2_final(const double* __restrict__ partials,
+                                                    unsigned np, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double sa = 0.0, sb = 0.0;
+  for (unsigned i = threadIdx.x; i < np; i += blockDim.x) {
+    sa += partials[2 * i];
+    sb += partials[2 * i + 1];
+  }
+  double ta = block_tree_sum(sa, sh);
+  __syncthreads();
+  double tb = block_tree_sum(sb, sh);
+  if (threadIdx.x == 0) {
+    out[0] = ta;
+    out[1] = tb;
+  }
+}
+
+void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
+                    double* out) {
+  hipLaunchKernelGGL(k_sum2_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, a, b,
+                     partials);
+  hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(256), 0, st, partials,
+                     (unsigned)HD_REDUCE_BLOCKS, out);
+}
+
+__global__ void k_fill_f32(float* p, float v, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, v,
+                     count);
+}
+
+// ---------------------------------------------------------------------
+// Force finalize: pair fixed point + SPME gather (+ constraint virial),
+// scattered back to atom order, optionally fused with the post-force kick.
+// ---------------------------------------------------------------------
+#define GATHER_P 4
+__device__ __forceinline__ void bspline_wd(float x, float& w, float& dw) {
+  // M_{p-1}(x) and M_{p-1}(x-1) by Cox-de Boor, then M_p (spme_spread_gather.cuh:56-86)
+  float v[GATHER_P + 1];
+#pragma unroll
+  for (int i = 0; i < GATHER_P + 1; ++i) {
+    float xi = x - (float)i;
+    v[i] = (xi >= 0.f && xi < 1.f) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int order = 2; order < GATHER_P; ++order) {
+    float inv = 1.0f / (float)(order - 1);
+#pragma unroll
+    for (int i = 0; i < GATHER_P - order + 1; ++i) {
+      float xi = x - (float)i;
+      v[i] = xi * inv * v[i] + ((float)order - xi) * inv * v[i + 1];
+    }
+  }
+  dw = v[0] - v[1];
+  float inv = 1.0f / (float)(GATHER_P - 1);
+  w = x * inv * v[0] + ((float)GATHER_P - x) * inv * v[1];
+}
+
+template <bool EV, bool GATHER, bool KICK, bool LOSSLESS>
+__global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  unsigned id = a.sorted_id[s];
+  const longlong2* fi = reinterpret_cast<const longlong2*>(a.fpi) + 2 * s;
+  const longlong2* fj = reinterpret_cast<const longlong2*>(a.fpj) + 2 * s;
+  longlong2 i01 = fi[0], i2x = fi[1], j01 = fj[0], j2x = fj[1];
+  float fx = hd_from_fixed48(i01.x + j01.x);
+  float fy = hd_from_fixed48(i01.y + j01.y);
+  float fz = hd_from_fixed48(i2x.x + j2x.x);
+  float pe = 0.f, vir = 0.f;
+  if (EV) {
+    longlong2 ie = reinterpret_cast<const longlong2*>(a.fpi_ew)[s];
+    longlong2 je = reinterpret_cast<const longlong2*>(a.fpj_ew)[s];
+    pe = hd_from_fixed48(ie.x + je.x);
+    vir = hd_from_fixed48(ie.y + je.y);
+  }
+  if (GATHER) {
+    HdLattice L = hd_load_lattice(a.lattice);
+    float4 pq = a.tile_posq[s];
+    float qi = pq.w;
+    float px = pq.x, py = pq.y, pz = pq.z;
+    int wa_, wb_, wc_;
+    hd_wrap_with_image(px, py, pz, wa_, wb_, wc_, L);
+    float sa, sb, sc;
+    hd_cart_to_frac(px, py, pz, L, sa, sb, sc);
+    float ua = (sa + 0.5f) * (float)a.na, ub = (sb + 0.5f) * (float)a.nb,
+          uc = (sc + 0.5f) * (float)a.nc;
+    int ga0 = (int)floorf(ua), gb0 = (int)floorf(ub), gc0 = (int)floorf(uc);
+    float ta = ua - (float)ga0, tb = ub - (float)gb0, tc = uc - (float)gc0;
+    float wa[GATHER_P], dwa[GATHER_P], wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
+    int gia[GATHER_P], gib[GATHER_P], gic[GATHER_P];
+#pragma unroll
+    for (int d = 0; d < GATHER_P; ++d) {
+      bspline_wd((float)d + ta, wa[d], dwa[d]);
+      bspline_wd((float)d + tb, wb[d], dwb[d]);
+      bspline_wd((float)d + tc, wc[d], dwc[d]);
+      int g;
+      g = ga0 - d; gia[d] = ((g % (int)a.na) + (int)a.na) % (int)a.na;
+      g = gb0 - d; gib[d] = ((g % (int)a.nb) + (int)a.nb) % (int)a.nb;
+      g = gc0 - d; gic[d] = ((g % (int)a.nc) + (int)a.nc) % (int)a.nc;
+    }
+    float phi = 0.f, gra = 0.f, grb = 0.f, grc = 0.f;
+#pragma unroll
+    for (int da = 0; da < GATHER_P; ++da) {
+#pragma unroll
+      for (int db = 0; db < GATHER_P; ++db) {
+        const float* row = a.vgrid + ((size_t)gia[da] * a.nb + gib[db]) * a.nc;
+#pragma unroll
+        for (int dc = 0; dc < GATHER_P; ++dc) {
+          float v = row[gic[dc]];
+          phi += v * wa[da] * wb[db] * wc[dc];
+          gra += v * dwa[da] * wb[db] * wc[dc];
+          grb += v * wa[da] * dwb[db] * wc[dc];
+          grc += v * wa[da] * wb[db] * dwc[dc];
+        }
+      }
+    }
+    float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
+    float bax = ilx, bay = -L.xy * ilx * ily, baz = (L.xy * L.yz - L.xz * L.ly) * ilx * ily * ilz;
+    float bby = ily, bbz = -L.yz * ily * ilz, bcz = ilz;
+    float g_a = (float)a.na * gra, g_b = (float)a.nb * grb, g_c = (float)a.nc * grc;
+    float rfx = -qi * (g_a * bax);
+    float rfy = -qi * (g_a * bay + g_b * bby);
+    float rfz = -qi * (g_a * baz + g_b * bbz + g_c * bcz);
+    fx = fx + rfx;
+    fy = fy + rfy;
+    fz = fz + rfz;
+    if (EV) {
+      float self = (float)(a.kc * a.alpha * 0.5641895835477563) * qi * qi;
+      pe = pe + (0.5f * qi * phi - self);
+      vir = vir + a.w_share[0];
+    }
+  }
+  a.fx[id] = fx;
+  a.fy[id] = fy;
+  a.fz[id] = fz;
+  if (EV) {
+    if (a.w_cons) vir = vir + a.w_cons[id];
+    a.pe[id] = pe;
+    a.vir[id] = vir;
+  }
+  if (KICK) {
+    float m = a.mass[id];
+    float ax = fx / m, ay = fy / m, az = fz / m;
+    float half_dt = a.dt * 0.5f;
+    float f = a.factor ? a.factor[0] : 1.0f;
+    if (LOSSLESS) {
+      double ex = (double)a.vx[id] + a.vxlo[id] + (double)(ax * half_dt);
+      double ey = (double)a.vy[id] + a.vylo[id] + (double)(ay * half_dt);
+      double ez = (double)a.vz[id] + a.vzlo[id] + (double)(az * half_dt);
+      float nx = (float)ex, ny = (float)ey, nz = (float)ez;
+      a.vxlo[id] = ex - (double)nx;
+      a.vylo[id] = ey - (double)ny;
+      a.vzlo[id] = ez - (double)nz;
+      a.vx[id] = nx * f;
+      a.vy[id] = ny * f;
+      a.vz[id] = nz * f;
+    } else {
+      a.vx[id] = (a.vx[id] + ax * half_dt) * f;
+      a.vy[id] = (a.vy[id] + ay * half_dt) * f;
+      a.vz[id] = (a.vz[id] + az * half_dt) * f;
+    }
+  }
+}
+
+void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev) {
+  if (!a.n) return;
+  dim3 g((a.n + 255) / 256), b(256);
+  bool ga = a.vgrid != nullptr, k = a.kick, ll = a.vxlo != nullptr;
+#define HD_FIN(E, G, K, LL) \
+  if (ev == E && ga == G && k == K && ll == LL) { \
+    hipLaunchKernelGGL((k_finalize<E, G, K, LL>), g, b, 0, st, a); \
+    return; \
+  }
+  HD_FIN(false, false, false, false) HD_FIN(false, false, true, false)
+  HD_FIN(false, true, false, false) HD_FIN(false, true, true, false)
+  HD_FIN(true, false, false, false) HD_FIN(true, false, true, false)
+  HD_FIN(true, true, false, false) HD_FIN(true, true, true, false)
+  HD_FIN(false, false, true, true) HD_FIN(false, true, true, true)
+  HD_FIN(true, false, true, true) HD_FIN(true, true, true, true)
+  HD_FIN(false, false, false, true) HD_FIN(false, true, false, true)
+  HD_FIN(true, false, false, true) HD_FIN(true, true, false, true)
+#undef HD_FIN
+}
+
+// In-bounds check of the DPP wave_rol:1 lane layout: out[l] = value of
+// lane (l+1) mod 64 after one rotation.
+__global__ void k_selftest_dpp(int* out) {
+  int v = (int)threadIdx.x * 7 + 3;
+  out[threadIdx.x] = hd_rot1_i(v);
+}
+void hd_launch_selftest_dpp(hipStream_t st, int* out) {
+  hipLaunchKernelGGL(k_selftest_dpp, dim3(1), dim3(64), 0, st, out);
+}

@@ -1,0 +1,376 @@
+// Neighbour maintenance for the packed 64-atom cluster list (gfx950).
+//
+// Reference behaviour (what is kept):
+//   * displacement check |x - x_ref|^2 > (r_skin/2)^2 sets bit 0 of a status
+//     word (kernels/neighbor.cu:50-75);
+//   * the list holds, per i-block of consecutive sorted atoms, "entries" of
+//     individual j-atoms, each j-atom within r_search of at least one i-atom,
+//     j-blocks >= i-block only (kernels/neighbor.cu:416-631);
+//   * capacity overflow / high-water bits in the same status word
+//     (kernels/neighbor.cu:643-671).
+// MI355X design (what changes):
+//   * blocks are 64 atoms = one wave64; entries hold 64 j-atoms;
+//   * atoms are ordered along a Morton curve over a fixed voxel grid
+//     (voxel = fractional-coordinate cell, so the order survives box
+//     rescaling) instead of by particle id inside r_search cells, which
+//     makes a block spatially compact;
+//   * each i-block owns a fixed-capacity entry region, written in a fixed
+//     order (no atomics, no scatter pass), so the list is deterministic;
+//   * every kernel here is gated on the status word on the device, so the
+//     whole rebuild lives inside a captured step and needs no host sync.
+#include "hd_device.h"
+#include "hd_kernels.h"
+
+#define NB_STATUS_REBUILD 1u
+#define NB_STATUS_HIGHWATER 2u
+#define NB_STATUS_OVERFLOW 8u
+#define NB_STATUS_FORCE 0x100u
+
+__device__ __forceinline__ bool nb_rebuild_due(const unsigned* status) {
+  return (*(volatile const unsigned*)status & (NB_STATUS_REBUILD | NB_STATUS_FORCE)) != 0u;
+}
+
+// Displacement check against the positions of the last rebuild.
+__global__ void k_disp_check(const float4* __restrict__ posq, const float4* __restrict__ ref,
+                             const float* __restrict__ lattice, float thr2, unsigned* status,
+                             unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool over = false;
+  if (i < n) {
+    HdLattice L = hd_load_lattice(lattice);
+    float4 p = posq[i], r = ref[i];
+    float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+    hd_min_image(dx, dy, dz, L);
+    over = dx * dx + dy * dy + dz * dz > thr2;
+  }
+  if (__any(over) && (threadIdx.x & 63) == 0) atomicOr(status, NB_STATUS_REBUILD);
+}
+
+// Voxel key = Morton rank of the atom's fractional voxel; histogram.
+__global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __restrict__ lattice,
+                             const unsigned* __restrict__ voxel_rank, unsigned nva, unsigned nvb,
+                             unsigned nvc, unsigned* __restrict__ atom_key,
+                             unsigned* __restrict__ voxel_count, const unsigned* status,
+                             unsigned n) {
+  if (!nb_rebuild_due(status)) return;
+  HdLattice L = hd_load_lattice(lattice);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 p = posq[i];
+    float x = p.x, y = p.y, z = p.z;
+    int ia, ib, ic;
+    hd_wrap_with_image(x, y, z, ia, ib, ic, L);
+    float sa, sb, sc;
+    hd_cart_to_frac(x, y, z, L, sa, sb, sc);
+    int va = (int)floorf((sa + 0.5f) * (float)nva);
+    int vb = (int)floorf((sb + 0.5f) * (float)nvb);
+    int vc = (int)floorf((sc + 0.5f) * (float)nvc);
+    va = min(max(va, 0), (int)nva - 1);
+    vb = min(max(vb, 0), (int)nvb - 1);
+    vc = min(max(vc, 0), (int)nvc - 1);
+    unsigned key = voxel_rank[((unsigned)va * nvb + (unsigned)vb) * nvc + (unsigned)vc];
+    atom_key[i] = key;
+    atomicAdd(&voxel_count[key], 1u);
+  }
+}
+
+// Single-workgroup exclusive scan of voxel_count -> voxel_off, and
+// clears the scatter cursors.  1024 threads, each owns a contiguous run.
+__global__ void __launch_bounds__(1024) k_voxel_scan(const unsigned* __restrict__ voxel_count,
+                                                     unsigned* __restrict__ voxel_off,
+                                                     unsigned* __restrict__ voxel_cursor,
+                                                     const unsigned* status, unsigned nvox) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ unsigned part[1024];
+  unsigned t = threadIdx.x;
+  unsigned per = (nvox + 1023u) / 1024u;
+  unsigned b = t * per, e = min(b + per, nvox);
+  unsigned s = 0;
+  for (unsigned k = b; k < e; ++k) s += voxel_count[k];
+  part[t] = s;
+  __syncthreads();
+  for (unsigned off = 1; off < 1024; off <<= 1) {
+    unsigned v = (t >= off) ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = part[t] - s;
+  for (unsigned k = b; k < e; ++k) {
+    voxel_off[k] = run;
+    run += voxel_count[k];
+    voxel_cursor[k] = 0u;
+  }
+  if (t == 1023) voxel_off[nvox] = part[1023];
+}
+
+__global__ void k_voxel_scatter(const unsigned* __restrict__ atom_key,
+                                const unsigned* __restrict__ voxel_off,
+                                unsigned* __restrict__ voxel_cursor,
+                                unsigned* __restrict__ sorted_id, const unsigned* status,
+                                unsigned n) {
+  if (!nb_rebuild_due(status)) return;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    unsigned k = atom_key[i];
+    unsigned slot = voxel_off[k] + atomicAdd(&voxel_cursor[k], 1u);
+    sorted_id[slot] = i;
+  }
+}
+
+// Per voxel: sort its atom ids ascending (a voxel holds a handful of
+// atoms) so the slot order is independent of atomic arrival order; then
+// clear the voxel histogram for the next rebuild and record slot_of[id].
+__global__ void k_voxel_sort(const unsigned* __restrict__ voxel_off,
+                             unsigned* __restrict__ voxel_count, unsigned* __restrict__ sorted_id,
+                             unsigned* __restrict__ slot_of, const unsigned* status,
+                             unsigned nvox) {
+  if (!nb_rebuild_due(status)) return;
+  for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += gridDim.x * blockDim.x) {
+    unsigned b = voxel_off[v], e = voxel_off[v + 1];
+    for (unsigned k = b + 1; k < e; ++k) {
+      unsigned key = sorted_id[k];
+      int pos = (int)k - 1;
+      while (pos >= (int)b && sorted_id[pos] > key) {
+        sorted_id[pos + 1] = sorted_id[pos];
+        --pos;
+      }
+      sorted_id[pos + 1] = key;
+    }
+    for (unsigned k = b; k < e; ++k) slot_of[sorted_id[k]] = k;
+    voxel_count[v] = 0u;
+  }
+}
+
+// Per-slot static data refreshed at a rebuild: type and exclusion key in
+// slot order, padding lanes marked with sentinels.
+__global__ void k_slot_static(const unsigned* __restrict__ sorted_id,
+                              const unsigned* __restrict__ type, const unsigned* __restrict__ key,
+                              unsigned* __restrict__ tile_type, unsigned* __restrict__ tile_key,
+                              const unsigned* status, unsigned n, unsigned npad) {
+  if (!nb_rebuild_due(status)) return;
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < npad;
+       s += gridDim.x * blockDim.x) {
+    if (s < n) {
+      unsigned id = sorted_id[s];
+      tile_type[s] = type[id];
+      tile_key[s] = key[id];
+    } else {
+      tile_type[s] = 0u;
+      tile_key[s] = HD_SENTINEL - 1u;
+    }
+  }
+}
+
+// Every step: gather positions into slot order (wrapped into the primary
+// cell) and clear the j-side fixed-point accumulators of the pair kernel.
+__global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* __restrict__ sorted_id,
+                              const float* __restrict__ lattice, float4* __restrict__ tile_posq,
+                              long long* __restrict__ fpj, long long* __restrict__ fpj_ew,
+                              unsigned n, unsigned npad) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  float4 out;
+  if (s < n) {
+    HdLattice L = hd_load_lattice(lattice);
+    float4 p = posq[sorted_id[s]];
+    int a, b, c;
+    hd_wrap_with_image(p.x, p.y, p.z, a, b, c, L);
+    out = p;
+  } else {
+    out = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  tile_posq[s] = out;
+  longlong2 z = make_longlong2(0, 0);
+  reinterpret_cast<longlong2*>(fpj)[2 * s] = z;
+  reinterpret_cast<longlong2*>(fpj)[2 * s + 1] = z;
+  if (fpj_ew) reinterpret_cast<longlong2*>(fpj_ew)[s] = z;
+}
+
+// Per-block axis-aligned bounding box: centre and half extents (one wave
+// per block).  Padding lanes are ignored.
+__global__ void k_block_bbox(const float4* __restrict__ tile_posq, float4* __restrict__ center,
+                             float4* __restrict__ half, const unsigned* status, unsigned n,
+                             unsigned nblocks) {
+  if (!nb_rebuild_due(status)) return;
+  unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  unsigned lane = threadIdx.x & 63;
+  if (wave >= nblocks) return;
+  unsigned s = wave * 64 + lane;
+  bool v = s < n;
+  float4 p = tile_posq[s];
+  float lox = v ? p.x : 3.0e38f, loy = v ? p.y : 3.0e38f, loz = v ? p.z : 3.0e38f;
+  float hix = v ? p.x : -3.0e38f, hiy = v ? p.y : -3.0e38f, hiz = v ? p.z : -3.0e38f;
+  for (int off = 32; off > 0; off >>= 1) {
+    lox = fminf(lox, __shfl_xor(lox, off));
+    loy = fminf(loy, __shfl_xor(loy, off));
+    loz = fminf(loz, __shfl_xor(loz, off));
+    hix = fmaxf(hix, __shfl_xor(hix, off));
+    hiy = fmaxf(hiy, __shfl_xor(hiy, off));
+    hiz = fmaxf(hiz, __shfl_xor(hiz, off));
+  }
+  if (lane == 0) {
+    center[wave] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
+    half[wave] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
+  }
+}
+
+// Closest approach of two boxes under the minimum image (per axis; exact
+// for orthorhombic cells), as kernels/neighbor.cu:364-388.
+__device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, float4 hb,
+                                              const HdLattice& L) {
+  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
+  hd_min_image(dx, dy, dz, L);
+  float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.f);
+  float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.f);
+  float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.f);
+  return ex * ex + ey * ey + ez * ez;
+}
+
+// Build the packed list: one wave per i-block.  For every j-block >= b
+// (in increasing order) whose box lies within r_search, each lane tests
+// its j-atom against the 64 i-atoms; j-atoms with at least one hit are
+// appended, in lane order, to a staging buffer flushed 64 at a time.
+// The entry layout is therefore a pure function of the positions.
+#define NB_WAVES 4
+__global__ void __launch_bounds__(256) k_build_entries(
+    const float4* __restrict__ tile_posq, const float4* __restrict__ center,
+    const float4* __restrict__ half, const float* __restrict__ lattice, float rs2,
+    unsigned* __restrict__ entry_atoms, unsigned* __restrict__ entry_count, unsigned cap,
+    unsigned* __restrict__ max_count, unsigned* status, unsigned n, unsigned nblocks) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ float4 ipos[NB_WAVES][64];
+  __shared__ unsigned buf[NB_WAVES][128];
+  unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned b = blockIdx.x * NB_WAVES + w;
+  if (b >= nblocks) return;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned islot = b * 64 + lane;
+  float4 pi = tile_posq[islot];
+  if (islot >= n) pi = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.f);
+  ipos[w][lane] = pi;
+  __builtin_amdgcn_wave_barrier();
+  float4 cb = center[b], hb = half[b];
+  unsigned ni = min(64u, n - b * 64);
+  unsigned nbuf = 0, nent = 0;
+  unsigned* out = entry_atoms + (size_t)b * cap * 64;
+  for (unsigned jb0 = b; jb0 < nblocks; jb0 += 64) {
+    unsigned jbl = jb0 + lane;
+    bool cand = false;
+    if (jbl < nblocks) cand = nb_box_dist2(cb, hb, center[jbl], half[jbl], L) <= rs2;
+    unsigned long long cmask = __ballot(cand);
+    while (cmask) {
+      unsigned bit = __builtin_ctzll(cmask);
+      cmask &= cmask - 1;
+      unsigned jb = jb0 + bit;
+      unsigned jslot = jb * 64 + lane;
+      bool jvalid = jslot < n;
+      float4 pj = tile_posq[jslot];
+      bool hit = false;
+      if (jvalid) {
+        for (unsigned m = 0; m < ni; ++m) {
+          float4 a = ipos[w][m];
+          float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
+          hd_min_image(dx, dy, dz, L);
+          float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 <= rs2 && (b * 64 + m) != jslot) {
+            hit = true;
+            break;
+          }
+        }
+      }
+      unsigned long long hmask = __ballot(hit);
+      if (hit) {
+        unsigned pos = nbuf + __popcll(hmask & hd_lane_mask_lt());
+        buf[w][pos] = jslot;
+      }
+      nbuf += __popcll(hmask);
+      __builtin_amdgcn_wave_barrier();
+      if (nbuf >= 64) {
+        if (nent < cap) out[(size_t)nent * 64 + lane] = buf[w][lane];
+        ++nent;
+        unsigned rem = nbuf - 64;
+        unsigned v = (lane < rem) ? buf[w][64 + lane] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rem) buf[w][lane] = v;
+        nbuf = rem;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  if (nbuf > 0) {
+    if (nent < cap) out[(size_t)nent * 64 + lane] = (lane < nbuf) ? buf[w][lane] : HD_SENTINEL;
+    ++nent;
+  }
+  if (lane == 0) {
+    entry_count[b] = min(nent, cap);
+    atomicMax(max_count, nent);
+    if (nent > cap) atomicOr(status, NB_STATUS_OVERFLOW);
+    else if (4 * nent > 3 * cap) atomicOr(status, NB_STATUS_HIGHWATER);
+  }
+}
+
+// End of a rebuild: remember the reference positions and clear the
+// rebuild request bits (sticky overflow/high-water bits stay for the host).
+__global__ void k_rebuild_finish(const float4* __restrict__ posq, float4* __restrict__ ref,
+                                 unsigned* status, unsigned* rebuild_count, unsigned n) {
+  if (!nb_rebuild_due(status)) return;
+  // every thread observed the request before the first write below: the
+  // clearing is done by the last workgroup after a grid-wide fence is not
+  // possible, so it is done in a separate one-thread kernel.
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    ref[i] = posq[i];
+}
+
+__global__ void k_rebuild_clear(unsigned* status, unsigned* rebuild_count) {
+  unsigned s = *status;
+  if (s & (NB_STATUS_REBUILD | NB_STATUS_FORCE)) {
+    *status = s & ~(NB_STATUS_REBUILD | NB_STATUS_FORCE);
+    *rebuild_count += 1u;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Host launch wrappers
+// ---------------------------------------------------------------------
+static inline unsigned cdiv(unsigned a, unsigned b) { return (a + b - 1) / b; }
+
+void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
+  const unsigned T = 256;
+  unsigned gridN = min(cdiv(a.n, T), 2048u);
+  if (gridN == 0) gridN = 1;
+  hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st, a.posq,
+                     a.ref_pos, a.lattice, a.disp_thr2, a.status, a.n);
+  hipLaunchKernelGGL(k_voxel_keys, dim3(gridN), dim3(T), 0, st, a.posq, a.lattice, a.voxel_rank,
+                     a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.status, a.n);
+  hipLaunchKernelGGL(k_voxel_scan, dim3(1), dim3(1024), 0, st, a.voxel_count, a.voxel_off,
+                     a.voxel_cursor, a.status, a.nvox);
+  hipLaunchKernelGGL(k_voxel_scatter, dim3(gridN), dim3(T), 0, st, a.atom_key, a.voxel_off,
+                     a.voxel_cursor, a.sorted_id, a.status, a.n);
+  unsigned gridV = min(cdiv(a.nvox, T), 2048u);
+  hipLaunchKernelGGL(k_voxel_sort, dim3(gridV), dim3(T), 0, st, a.voxel_off, a.voxel_count,
+                     a.sorted_id, a.slot_of, a.status, a.nvox);
+  unsigned gridP = min(cdiv(a.npad, T), 2048u);
+  hipLaunchKernelGGL(k_slot_static, dim3(gridP), dim3(T), 0, st, a.sorted_id, a.type, a.key,
+                     a.tile_type, a.tile_key, a.status, a.n, a.npad);
+}
+
+void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
+                           const float* lattice, float4* tile_posq, long long* fpj,
+                           long long* fpj_ew, unsigned n, unsigned npad) {
+  hipLaunchKernelGGL(k_tile_gather, dim3(cdiv(npad, 256)), dim3(256), 0, st, posq, sorted_id,
+                     lattice, tile_posq, fpj, fpj_ew, n, npad);
+}
+
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
+  unsigned nblocks = a.npad / 64;
+  hipLaunchKernelGGL(k_block_bbox, dim3(cdiv(nblocks * 64, 256)), dim3(256), 0, st, a.tile_posq,
+                     a.center, a.half, a.status, a.n, nblocks);
+  hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0, st,
+                     a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.entry_atoms,
+                     a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
+  unsigned gridN = min(cdiv(a.n, 256), 2048u);
+  if (gridN == 0) gridN = 1;
+  hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,
+                     a.status, a.rebuild_count, a.n);
+  hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(1), 0, st, a.status, a.rebuild_count);
+}

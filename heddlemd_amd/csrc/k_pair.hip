@@ -1,0 +1,263 @@
+// Fused Lennard-Jones + real-space Ewald pair forces over the packed
+// 64-atom cluster list (gfx950, wave64).
+//
+// Reference: the JIT-composed packed pair kernel
+// (src/forces/jit_composed.rs:1490-1838) with the LJ functor
+// (src/forces/lj.rs:225-335) and the SPME real-space functor
+// (src/forces/spme.rs:708-783).  Kept: per-pair arithmetic (f32, Hastings
+// erfc A&S 7.1.26, optional cubic LJ switch), per-entry f32 sums folded
+// into i64 fixed point at 2^48 with truncation, Newton's third law with the
+// j-side suppressed for j-atoms that belong to the i-block, per-atom energy
+// and virial split half/half.
+//
+// MI355X design:
+//   * one workgroup of 4 waves per 64-atom i-block; lane = i-atom; the 64
+//     j-atoms of an entry rotate through the wave with DPP wave_rol:1
+//     (a VALU modifier, no LDS round trip) in 64 lock-step steps;
+//   * i-side sums stay in registers across entries, are combined across the
+//     4 waves in LDS and stored once with a plain store (no atomics, no
+//     clearing pass); only the j-side uses i64 atomics;
+//   * the LJ type-pair row of the i-atom is held in registers (<= 4 types)
+//     and selected per pair, so the inner loop touches no memory;
+//   * exclusions by molecule key: a pair is excluded when both atoms carry
+//     the same key (every intra-molecule pair of a rigid group, as the
+//     reference derives them, src/forces/topology.rs:1225-1245); the atom's
+//     self pair is excluded the same way;
+//   * when the box is orthorhombic and wide enough, every j-atom is shifted
+//     once per entry to the image nearest the i-block centre, so the inner
+//     loop needs no minimum-image arithmetic
+//     (the single-periodic-copy path, jit_composed.rs:1565-1600).
+#include "hd_device.h"
+#include "hd_kernels.h"
+
+#define PAIR_WAVES 4
+#define PAIR_MAXT 4
+
+struct PairParams {
+  float c6[PAIR_MAXT * PAIR_MAXT];   // 4 eps sigma^6
+  float c12[PAIR_MAXT * PAIR_MAXT];  // 4 eps sigma^12
+  float ljcut2[PAIR_MAXT * PAIR_MAXT];
+  float rsw2[PAIR_MAXT * PAIR_MAXT];
+  float inv_delta[PAIR_MAXT * PAIR_MAXT];  // 1/(rc^2 - rs^2), 0 when no switch
+  int ntypes;
+  int any_switch;
+  float kc, alpha, coul_rc2;  // coul_rc2 = 0 disables the Coulomb term
+  float rmax2;
+  float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
+};
+
+__device__ __forceinline__ float sel4(const float* row, unsigned t) {
+  float v = row[0];
+  v = (t == 1u) ? row[1] : v;
+  v = (t == 2u) ? row[2] : v;
+  v = (t == 3u) ? row[3] : v;
+  return v;
+}
+
+template <bool EV, bool SPC>
+__device__ __forceinline__ void pair_entries(
+    const PairParams& P, const HdLattice& L, unsigned b, unsigned w, unsigned lane, float pix,
+    float piy, float piz, float qi, unsigned ikey, const float* c6row, const float* c12row,
+    const float* cutrow, const float* rswrow, const float* idrow, float4 cb,
+    const float4* __restrict__ tile_posq, const unsigned* __restrict__ tile_type,
+    const unsigned* __restrict__ tile_key, const unsigned* __restrict__ ents, unsigned nent,
+    unsigned n, unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
+    long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w) {
+  const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
+  for (unsigned e = w; e < nent; e += PAIR_WAVES) {
+    unsigned jslot = ents[(size_t)e * 64 + lane];
+    bool jvalid = jslot < n;
+    float4 pj = jvalid ? tile_posq[jslot] : cb;
+    unsigned jtype = jvalid ? tile_type[jslot] : 0u;
+    unsigned jkey = jvalid ? tile_key[jslot] : HD_SENTINEL;
+    unsigned jin = (jvalid && (jslot >> 6) == b) ? 1u : 0u;
+    float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = pj.w;
+    if (SPC) hd_wrap_against_center(pjx, pjy, pjz, cb.x, cb.y, cb.z, L);
+    float jfx = 0.f, jfy = 0.f, jfz = 0.f, je = 0.f, jw = 0.f;
+    float efx = 0.f, efy = 0.f, efz = 0.f, ee = 0.f, ew = 0.f;
+#pragma unroll 4
+    for (int t = 0; t < 64; ++t) {
+      float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
+      if (!SPC) hd_min_image(dx, dy, dz, L);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      float inv_r = __builtin_amdgcn_rsqf(r2);
+      float inv_r2 = inv_r * inv_r;
+      // LJ (lj.rs:237-247, 250-287), row of the i-type selected by j-type
+      float c6 = sel4(c6row, jtype), c12 = sel4(c12row, jtype);
+      float ir6 = inv_r2 * inv_r2 * inv_r2;
+      float elj = ir6 * (c12 * ir6 - c6);
+      float flj = inv_r2 * ir6 * (12.0f * c12 * ir6 - 6.0f * c6);
+      if (P.any_switch) {
+        float rs2 = sel4(rswrow, jtype), idl = sel4(idrow, jtype);
+        if (idl > 0.f && r2 > rs2) {
+          float tau = (r2 - rs2) * idl;
+          float omt = 1.0f - tau;
+          float s = omt * omt * (1.0f + 2.0f * tau);
+          float chain = 12.0f * tau * omt * idl;
+          flj = s * flj + chain * elj;
+          elj = s * elj;
+        }
+      }
+      bool lj_on = r2 <= sel4(cutrow, jtype);
+      // real-space Coulomb, f32 Hastings erfc (spme.rs:744-770)
+      float r = r2 * inv_r;
+      float qq = P.kc * qi * qj;
+      float ar = P.alpha * r;
+      float gauss = __expf(-(ar * ar));
+      float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+      float erfc_ar =
+          (0.254829592f +
+           (-0.284496736f + (1.421413741f + (-1.453152027f + 1.061405429f * tt) * tt) * tt) * tt) *
+          tt * gauss;
+      float ec = qq * erfc_ar * inv_r;
+      float fc = qq * inv_r2 * (erfc_ar * inv_r + two_a_sqrtpi * gauss);
+      bool c_on = r2 <= P.coul_rc2;
+      bool ok = (jkey != ikey) & (max(jkey, ikey) < HD_SENTINEL - 1u);
+      float factor = (ok & lj_on) ? flj : 0.f;
+      factor += (ok & c_on) ? fc : 0.f;
+      float fx = factor * dx, fy = factor * dy, fz = factor * dz;
+      efx += fx; efy += fy; efz += fz;
+      float fj = jin ? 0.f : 1.f;
+      jfx -= fj * fx; jfy -= fj * fy; jfz -= fj * fz;
+      if (EV) {
+        float energy = (ok & lj_on) ? elj : 0.f;
+        energy += (ok & c_on) ? ec : 0.f;
+        float he = 0.5f * energy, hw = 0.5f * factor * r2;
+        ee += he; ew += hw;
+        je += fj * he; jw += fj * hw;
+      }
+      pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
+      jtype = hd_rot1_u(jtype); jkey = hd_rot1_u(jkey); jin = hd_rot1_u(jin);
+      jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
+      if (EV) { je = hd_rot1(je); jw = hd_rot1(jw); }
+    }
+    ai_fx += hd_to_fixed48(efx);
+    ai_fy += hd_to_fixed48(efy);
+    ai_fz += hd_to_fixed48(efz);
+    if (EV) {
+      ai_e += hd_to_fixed48(ee);
+      ai_w += hd_to_fixed48(ew);
+    }
+    if (jvalid && !jin) {
+      unsigned long long* d = fpj + (size_t)jslot * 4;
+      long long vx = hd_to_fixed48(jfx), vy = hd_to_fixed48(jfy), vz = hd_to_fixed48(jfz);
+      if (vx) atomicAdd(d + 0, (unsigned long long)vx);
+      if (vy) atomicAdd(d + 1, (unsigned long long)vy);
+      if (vz) atomicAdd(d + 2, (unsigned long long)vz);
+      if (EV) {
+        long long ve = hd_to_fixed48(je), vw = hd_to_fixed48(jw);
+        if (ve) atomicAdd(fpj_ew + (size_t)jslot * 2, (unsigned long long)ve);
+        if (vw) atomicAdd(fpj_ew + (size_t)jslot * 2 + 1, (unsigned long long)vw);
+      }
+    }
+  }
+}
+
+template <bool EV>
+__global__ void __launch_bounds__(256) k_pair(
+    PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
+    const unsigned* __restrict__ tile_type, const unsigned* __restrict__ tile_key,
+    const float4* __restrict__ center, const float4* __restrict__ half,
+    const unsigned* __restrict__ entry_atoms, const unsigned* __restrict__ entry_count,
+    unsigned cap, unsigned n, long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
+    unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
+  __shared__ long long red[PAIR_WAVES][5][64];
+  unsigned b = blockIdx.x;
+  unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned islot = b * 64 + lane;
+  bool ivalid = islot < n;
+  float4 pi = tile_posq[islot];
+  unsigned itype = ivalid ? tile_type[islot] : 0u;
+  unsigned ikey = ivalid ? tile_key[islot] : (HD_SENTINEL - 1u);
+  float4 cb = center[b], hb = half[b];
+  bool spc = (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) &&
+             (0.5f * L.lx - hb.x >= P.spc_margin) && (0.5f * L.ly - hb.y >= P.spc_margin) &&
+             (0.5f * L.lz - hb.z >= P.spc_margin);
+  float pix = pi.x, piy = pi.y, piz = pi.z;
+  if (!ivalid) { pix = cb.x; piy = cb.y; piz = cb.z; }
+  if (spc) hd_wrap_against_center(pix, piy, piz, cb.x, cb.y, cb.z, L);
+  float c6row[PAIR_MAXT], c12row[PAIR_MAXT], cutrow[PAIR_MAXT], rswrow[PAIR_MAXT],
+      idrow[PAIR_MAXT];
+#pragma unroll
+  for (int t = 0; t < PAIR_MAXT; ++t) {
+    int k = (t < P.ntypes) ? (int)itype * P.ntypes + t : 0;
+    c6row[t] = P.c6[k];
+    c12row[t] = P.c12[k];
+    cutrow[t] = ivalid ? P.ljcut2[k] : -1.0f;
+    rswrow[t] = P.rsw2[k];
+    idrow[t] = P.inv_delta[k];
+  }
+  unsigned nent = entry_count[b];
+  const unsigned* ents = entry_atoms + (size_t)b * cap * 64;
+  long long ax = 0, ay = 0, az = 0, ae = 0, aw = 0;
+  if (spc)
+    pair_entries<EV, true>(P, L, b, w, lane, pix, piy, piz, pi.w, ikey, c6row, c12row, cutrow,
+                           rswrow, idrow, cb, tile_posq, tile_type, tile_key, ents, nent, n, fpj,
+                           fpj_ew, ax, ay, az, ae, aw);
+  else
+    pair_entries<EV, false>(P, L, b, w, lane, pix, piy, piz, pi.w, ikey, c6row, c12row, cutrow,
+                            rswrow, idrow, cb, tile_posq, tile_type, tile_key, ents, nent, n, fpj,
+                            fpj_ew, ax, ay, az, ae, aw);
+  red[w][0][lane] = ax;
+  red[w][1][lane] = ay;
+  red[w][2][lane] = az;
+  if (EV) {
+    red[w][3][lane] = ae;
+    red[w][4][lane] = aw;
+  }
+  __syncthreads();
+  if (w == 0) {
+    long long s[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < PAIR_WAVES; ++k) {
+      s[0] += red[k][0][lane];
+      s[1] += red[k][1][lane];
+      s[2] += red[k][2][lane];
+      if (EV) {
+        s[3] += red[k][3][lane];
+        s[4] += red[k][4][lane];
+      }
+    }
+    reinterpret_cast<longlong2*>(fpi)[2 * islot] = make_longlong2(s[0], s[1]);
+    reinterpret_cast<longlong2*>(fpi)[2 * islot + 1] = make_longlong2(s[2], 0);
+    if (EV) reinterpret_cast<longlong2*>(fpi_ew)[islot] = make_longlong2(s[3], s[4]);
+  }
+}
+
+void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
+  PairParams P;
+  memset(&P, 0, sizeof(P));
+  P.ntypes = a.ntypes;
+  P.any_switch = 0;
+  for (int i = 0; i < a.ntypes; ++i)
+    for (int j = 0; j < a.ntypes; ++j) {
+      int k = i * a.ntypes + j;
+      double s = a.sigma[k], e = a.eps[k];
+      double s6 = s * s * s * s * s * s;
+      P.c6[k] = (float)(4.0 * e * s6);
+      P.c12[k] = (float)(4.0 * e * s6 * s6);
+      double rc = a.cut[k], rs = a.rsw[k];
+      P.ljcut2[k] = (e != 0.0) ? (float)(rc * rc) : -1.0f;
+      P.rsw2[k] = (float)(rs * rs);
+      double d = rc * rc - rs * rs;
+      P.inv_delta[k] = d > 0.0 ? (float)(1.0 / d) : 0.0f;
+      if (d > 0.0 && e != 0.0) P.any_switch = 1;
+    }
+  P.kc = (float)a.kc;
+  P.alpha = (float)a.alpha;
+  P.coul_rc2 = a.coul_rc > 0.0 ? (float)(a.coul_rc * a.coul_rc) : -1.0f;
+  P.rmax2 = (float)(a.rmax * a.rmax);
+  P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
+  unsigned nblocks = a.npad / 64;
+  if (ev)
+    hipLaunchKernelGGL(k_pair<true>, dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P, a.lattice,
+                       a.tile_posq, a.tile_type, a.tile_key, a.center, a.half, a.entry_atoms,
+                       a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
+                       (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
+  else
+    hipLaunchKernelGGL(k_pair<false>, dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P, a.lattice,
+                       a.tile_posq, a.tile_type, a.tile_key, a.center, a.half, a.entry_atoms,
+                       a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
+                       (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
+}

@@ -1,0 +1,222 @@
+"""Python face of the MD step: a thin owner of an ``hd_ctx``.
+
+This mirrors the reference's set-up path (src/runner.rs SimulationSetup,
+ForceField::new, run_md_phase): upload the particle state, configure the
+force slots (Lennard-Jones, SPME), the integrator (velocity Verlet, f32 or
+lossless), the constraint (SETTLE) and the thermostat (Nose-Hoover chain),
+then advance steps.  Every numeric operation happens inside libheddle.so on
+the GPU; this module only moves host arrays across the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import HeddleError
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        self._ctx = C.c_void_p()
+        rc = self._lib.hd_ctx_create(device, C.byref(self._ctx))
+        if rc != 0:
+            raise HeddleError(f"hd_ctx_create({device}) failed with {rc}")
+        self.n = 0
+        self._keep = []
+
+    def close(self):
+        if self._ctx:
+            self._lib.hd_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._lib.hd_last_error(self._ctx)
+            raise HeddleError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---- configuration -------------------------------------------------
+    def set_system(self, posq, mass, types, excl_key=None):
+        posq = _c(posq, np.float32).reshape(-1, 4)
+        self.n = posq.shape[0]
+        key = None
+        if excl_key is not None:
+            key = _c(excl_key, np.uint32)
+            self._keep.append(key)
+        self._chk(self._lib.hd_set_system(self._ctx, self.n, posq.reshape(-1),
+                                          _c(mass, np.float32), _c(types, np.uint32),
+                                          key.ctypes.data if key is not None else None),
+                  "hd_set_system")
+
+    def set_lattice(self, lattice):
+        self._chk(self._lib.hd_set_lattice(self._ctx, _c(lattice, np.float64)), "hd_set_lattice")
+
+    def set_velocities(self, v):
+        v = _c(v, np.float32).reshape(3, -1)
+        self._chk(self._lib.hd_set_velocities(self._ctx, _c(v[0], np.float32),
+                                              _c(v[1], np.float32), _c(v[2], np.float32)),
+                  "hd_set_velocities")
+
+    def set_lj(self, lj):
+        nt = int(lj.ntypes)
+        f = lambda a: _c(np.asarray(a, np.float64).reshape(-1), np.float64)
+        self._chk(self._lib.hd_set_lj(self._ctx, nt, f(lj.sigma), f(lj.epsilon), f(lj.cutoff),
+                                      f(lj.r_switch)), "hd_set_lj")
+
+    def set_spme(self, spme):
+        na, nb, nc = spme.grid
+        self._chk(self._lib.hd_set_spme(self._ctx, spme.alpha, spme.r_cut, na, nb, nc,
+                                        spme.order, spme.k_c), "hd_set_spme")
+
+    def set_neighbor(self, r_skin):
+        self._chk(self._lib.hd_set_neighbor(self._ctx, float(r_skin)), "hd_set_neighbor")
+
+    def set_settle(self, groups, d_oh, d_hh, m_o, m_h):
+        g = _c(groups, np.uint32).reshape(-1)
+        self._chk(self._lib.hd_set_settle(self._ctx, g.shape[0] // 3, g, d_oh, d_hh, m_o, m_h),
+                  "hd_set_settle")
+
+    def set_integrator(self, dt, lossless=False):
+        self._chk(self._lib.hd_set_integrator(self._ctx, float(dt), int(lossless)),
+                  "hd_set_integrator")
+
+    def set_nhc(self, kt, tau, chain=3, yoshida=3, n_resp=1, n_constraints=0):
+        self._chk(self._lib.hd_set_nhc(self._ctx, float(kt), float(tau), chain, yoshida, n_resp,
+                                       int(n_constraints)), "hd_set_nhc")
+
+    def setup(self):
+        self._chk(self._lib.hd_setup(self._ctx), "hd_setup")
+
+    def use_graph(self, on):
+        self._chk(self._lib.hd_use_graph(self._ctx, int(bool(on))), "hd_use_graph")
+
+    def set_timing(self, on):
+        self._chk(self._lib.hd_set_timing(self._ctx, int(bool(on))), "hd_set_timing")
+
+    # ---- work ------------------------------------------------------------
+    def force_eval(self, scalars=True):
+        self._chk(self._lib.hd_force_eval(self._ctx, int(scalars)), "hd_force_eval")
+
+    def constrain_velocities(self):
+        self._chk(self._lib.hd_constrain_velocities(self._ctx), "hd_constrain_velocities")
+
+    def step(self, n_steps, scalars_last=False):
+        self._chk(self._lib.hd_step(self._ctx, int(n_steps), int(bool(scalars_last))), "hd_step")
+
+    def sync(self):
+        self._chk(self._lib.hd_sync(self._ctx), "hd_sync")
+
+    # ---- downloads ---------------------------------------------------------
+    def positions(self, images=False):
+        posq = np.zeros((self.n, 4), np.float32)
+        if images:
+            im = np.zeros((3, self.n), np.int32)
+            self._chk(self._lib.hd_get_positions(self._ctx, posq.reshape(-1), im[0].ctypes.data,
+                                                 im[1].ctypes.data, im[2].ctypes.data),
+                      "hd_get_positions")
+            return posq, im
+        self._chk(self._lib.hd_get_positions(self._ctx, posq.reshape(-1), None, None, None),
+                  "hd_get_positions")
+        return posq
+
+    def velocities(self):
+        v = np.zeros((3, self.n), np.float32)
+        self._chk(self._lib.hd_get_velocities(self._ctx, v[0], v[1], v[2]), "hd_get_velocities")
+        return v
+
+    def forces(self):
+        f = np.zeros((3, self.n), np.float32)
+        self._chk(self._lib.hd_get_forces(self._ctx, f[0], f[1], f[2]), "hd_get_forces")
+        return f.T.copy()
+
+    def per_atom(self):
+        pe = np.zeros(self.n, np.float32)
+        w = np.zeros(self.n, np.float32)
+        self._chk(self._lib.hd_get_per_atom(self._ctx, pe, w), "hd_get_per_atom")
+        return pe, w
+
+    def lattice(self):
+        L = np.zeros(6)
+        self._chk(self._lib.hd_get_lattice(self._ctx, L), "hd_get_lattice")
+        return L
+
+    def scalars(self):
+        out = np.zeros(8)
+        self._chk(self._lib.hd_get_scalars(self._ctx, out), "hd_get_scalars")
+        keys = ["ke", "pe", "virial", "e_recip", "w_recip", "nhc_chain", "rebuilds", "status"]
+        return dict(zip(keys, out.tolist()))
+
+    def pairs(self):
+        n = C.c_uint64(0)
+        self._chk(self._lib.hd_get_pairs(self._ctx, None, 0, C.byref(n)), "hd_get_pairs")
+        out = np.zeros(2 * max(n.value, 1), np.uint32)
+        self._chk(self._lib.hd_get_pairs(self._ctx, out.ctypes.data, n.value, C.byref(n)),
+                  "hd_get_pairs")
+        return out[:2 * n.value].reshape(-1, 2)
+
+    def neighbor_stats(self):
+        out = np.zeros(6, np.uint64)
+        self._chk(self._lib.hd_get_neighbor_stats(self._ctx, out), "hd_get_neighbor_stats")
+        keys = ["blocks", "capacity", "max_entries", "entries", "rebuilds", "status"]
+        return dict(zip(keys, [int(x) for x in out]))
+
+    def spme_grids(self, grid):
+        na, nb, nc = grid
+        rho = np.zeros(na * nb * nc, np.float32)
+        v = np.zeros(na * nb * nc, np.float32)
+        self._chk(self._lib.hd_get_spme_grids(self._ctx, rho.ctypes.data, v.ctypes.data),
+                  "hd_get_spme_grids")
+        return rho.reshape(grid), v.reshape(grid)
+
+    def selftest_dpp(self):
+        out = np.zeros(64, np.int32)
+        self._chk(self._lib.hd_selftest_dpp(self._ctx, out), "hd_selftest_dpp")
+        return out
+
+    def stage_times(self):
+        names = C.create_string_buffer(64 * 64)
+        ms = np.zeros(64)
+        k = self._lib.hd_stage_times(self._ctx, names, 64, ms, 64)
+        if k < 0:
+            self._chk(k, "hd_stage_times")
+        raw = names.raw
+        return [(raw[i * 64:(i + 1) * 64].split(b"\0")[0].decode(), float(ms[i]))
+                for i in range(k)]
+
+
+def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
+               velocities=None, device=0):
+    """Configure an Engine for a ``systems.System``."""
+    e = Engine(device)
+    e.set_system(system.posq, system.mass, system.types, system.excl_key)
+    e.set_lattice(system.lattice)
+    e.set_lj(system.lj)
+    if system.spme is not None:
+        e.set_spme(system.spme)
+    if r_skin is not None:
+        e.set_neighbor(r_skin)
+    if system.water_groups is not None:
+        d_oh, d_hh = system.water_geom
+        m_o = float(system.mass[system.water_groups[0, 0]])
+        m_h = float(system.mass[system.water_groups[0, 1]])
+        e.set_settle(system.water_groups, d_oh, d_hh, m_o, m_h)
+    if dt is not None:
+        e.set_integrator(dt, lossless)
+    if nhc_kt is not None:
+        e.set_nhc(nhc_kt, nhc_tau, n_constraints=system.n_constraints)
+    if velocities is not None:
+        e.set_velocities(velocities)
+    e.setup()
+    return e

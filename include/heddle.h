@@ -1,0 +1,135 @@
+/*
+ * heddle.h -- C ABI of the MI355X-native HeddleMD MD step (libheddle.so).
+ *
+ * The reference binds its GPU work from Rust through cudarc on one CUDA
+ * stream (src/gpu/device.rs:34-73, src/gpu/kernels.rs) behind the
+ * Potential / Integrator / Thermostat / Constraint slot traits
+ * (src/forces/mod.rs:79-292, src/integrator/mod.rs:233-967,
+ * src/integrator/constraint.rs:43).  This header is the boundary that
+ * replaces that binding: plain pointers and sizes, no C++ or torch types,
+ * host arrays owned by the caller, device memory owned by the context, all
+ * work ordered on one HIP stream per context.  A context is not thread
+ * safe; separate contexts are independent.
+ *
+ * Units: Hartree atomic units (Bohr, electron mass, e, Hartree, atomic
+ * time; k_B = k_C = 1) as the reference uses internally
+ * (src/units/atomic_constants.rs).  Lattice = [lx, ly, lz, xy, xz, yz].
+ *
+ * Every int-returning call returns HD_OK (0) or a negative HD_ERR_* code;
+ * hd_last_error() then holds a message.
+ */
+#ifndef HEDDLE_H
+#define HEDDLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HD_OK 0
+#define HD_ERR_ARG -1
+#define HD_ERR_HIP -2
+#define HD_ERR_STATE -3
+#define HD_ERR_UNSUPPORTED -4
+#define HD_ERR_OVERFLOW -5
+
+typedef struct hd_ctx hd_ctx;
+
+/* Library version string. */
+const char *hd_version(void);
+
+/* Create a context on HIP device `device` (the reference hard-wires device 0
+ * and one stream: src/gpu/device.rs:66-73). */
+int hd_ctx_create(int device, hd_ctx **out);
+void hd_ctx_destroy(hd_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *hd_last_error(const hd_ctx *ctx);
+
+/* Particle state upload (src/gpu/buffers.rs:11-44 ParticleBuffers::new).
+ * posq: 4n floats (x, y, z, q); mass, type: n each; excl_key: n molecule
+ * keys (pairs with equal keys are excluded from the pair terms, the
+ * constraint-group exclusions of src/forces/topology.rs:1225-1245), or
+ * NULL for "no exclusions". */
+int hd_set_system(hd_ctx *ctx, uint64_t n, const float *posq, const float *mass,
+                  const uint32_t *type, const uint32_t *excl_key);
+/* Box (src/pbc.rs SimulationBox lattice). */
+int hd_set_lattice(hd_ctx *ctx, const double lattice[6]);
+int hd_set_velocities(hd_ctx *ctx, const float *vx, const float *vy, const float *vz);
+/* Image counters (io/init_state.rs InitImages); NULL resets to zero. */
+int hd_set_images(hd_ctx *ctx, const int32_t *ix, const int32_t *iy, const int32_t *iz);
+
+/* Lennard-Jones type-pair tables, ntypes <= 4 (src/forces/lj.rs:22-335). */
+int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilon,
+              const double *cutoff, const double *r_switch);
+/* SPME: real-space erfc pair term + reciprocal grid
+ * (src/forces/spme.rs:36-52 SpmeParameters).  Grid dims must be powers of
+ * two >= 8; order must be 4. */
+int hd_set_spme(hd_ctx *ctx, double alpha, double r_cut, int na, int nb, int nc, int order,
+                double k_c);
+/* Cell-list neighbour skin (src/io/config.rs:1232, default 0.3 * r_cut). */
+int hd_set_neighbor(hd_ctx *ctx, double r_skin);
+/* Rigid three-site water via SETTLE (src/integrator/settle.rs):
+ * groups = 3 * n_groups atom ids (O, H1, H2). */
+int hd_set_settle(hd_ctx *ctx, uint64_t n_groups, const uint32_t *groups, double d_oh,
+                  double d_hh, double m_o, double m_h);
+/* Velocity Verlet (src/integrator/velocity_verlet.rs); lossless = the
+ * compensated (f32, f64) mode. */
+int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);
+/* Nose-Hoover chain thermostat (src/integrator/nose_hoover_chain.rs:17-31);
+ * kt <= 0 removes it.  n_constraints feeds the thermal DOF 3N - n_c - 3. */
+int hd_set_nhc(hd_ctx *ctx, double kt, double tau, int chain, int yoshida, int n_resp,
+               uint64_t n_constraints);
+
+/* Allocate device state and build the first neighbour list; must follow
+ * the hd_set_* calls and precede any evaluation. */
+int hd_setup(hd_ctx *ctx);
+/* One force evaluation at the current positions (ForceField::step,
+ * src/forces/mod.rs:662-757); want_scalars also fills per-atom energy and
+ * virial and the totals. */
+int hd_force_eval(hd_ctx *ctx, int want_scalars);
+/* Project velocities onto the constraint manifold
+ * (Constraint::apply_initial_velocity_projection, settle.rs:406-430). */
+int hd_constrain_velocities(hd_ctx *ctx);
+/* Advance n_steps MD steps (the reference runner's per-step sequence,
+ * src/runner.rs:2585-2687); the last step evaluates scalars when
+ * scalars_last != 0.  Uses a captured hipGraph per step when enabled. */
+int hd_step(hd_ctx *ctx, int n_steps, int scalars_last);
+int hd_use_graph(hd_ctx *ctx, int enable);
+int hd_sync(hd_ctx *ctx);
+
+/* Downloads (caller-owned host arrays of n entries; posq has 4n). */
+int hd_get_positions(hd_ctx *ctx, float *posq, int32_t *ix, int32_t *iy, int32_t *iz);
+int hd_get_velocities(hd_ctx *ctx, float *vx, float *vy, float *vz);
+int hd_get_forces(hd_ctx *ctx, float *fx, float *fy, float *fz);
+int hd_get_per_atom(hd_ctx *ctx, float *pe, float *virial);
+int hd_get_lattice(hd_ctx *ctx, double lattice[6]);
+/* Scalars of the last scalar evaluation, FP64:
+ * out[0] kinetic energy, out[1] potential energy, out[2] virial,
+ * out[3] E_recip (without self term), out[4] W_recip,
+ * out[5] Nose-Hoover chain term of the conserved quantity,
+ * out[6] neighbour rebuild count, out[7] neighbour status word. */
+int hd_get_scalars(hd_ctx *ctx, double out[8]);
+/* Canonical pair set of the current packed list: unique (min id, max id)
+ * pairs with min < max, excluding sentinels (neighbor_list.rs:1366-1506).
+ * Returns the count in *n_pairs; writes at most cap pairs (2*cap u32). */
+int hd_get_pairs(hd_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *n_pairs);
+/* out[0] blocks, out[1] entry capacity per block, out[2] max entries in a
+ * block, out[3] total entries, out[4] rebuilds, out[5] status. */
+int hd_get_neighbor_stats(hd_ctx *ctx, uint64_t out[6]);
+/* SPME grids of the last evaluation: rho (charge grid, na*nb*nc, from the
+ * fixed-point spread) and V (potential grid after the convolution). */
+int hd_get_spme_grids(hd_ctx *ctx, float *rho, float *v);
+/* Device self test of the wave64 rotation: out[l] = 7*((l+1)%64)+3. */
+int hd_selftest_dpp(hd_ctx *ctx, int32_t out[64]);
+/* Per-stage device time (ms) of the last timed step, names as in the
+ * reference .timings table (src/timings.rs); returns the number of
+ * stages written (<= cap). */
+int hd_stage_times(hd_ctx *ctx, char *names, int name_len, double *ms, int cap);
+/* Enable per-stage event timing for non-graph steps. */
+int hd_set_timing(hd_ctx *ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

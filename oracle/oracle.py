@@ -1,0 +1,290 @@
+"""Python face of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the ``cpu_baseline`` leg of ``bench.py``.  The product path in
+``heddlemd_amd`` never imports this module.
+
+Every function mirrors a reference routine; the C file cites file:line for
+each.  ``md_step`` at the bottom composes them in the order the reference
+runner executes one velocity-Verlet + SETTLE + Nose-Hoover-chain step
+(src/runner.rs:2585-2687, src/integrator/mod.rs:344-429).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i64 = C.c_int64
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_pair_set.restype = _i64
+        L.or_pair_set.argtypes = [C.c_int, _f32p, _f64p, C.c_double, C.c_double, _i64,
+                                  _u32p, _i64, _u32p, C.POINTER(_i64)]
+        L.or_pair_forces.restype = None
+        L.or_pair_forces.argtypes = [C.c_int, _f32p, _u32p, C.c_int, _f64p, _f64p, _f64p,
+                                     _f64p, C.c_double, C.c_double, C.c_double,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, _f64p, C.c_double,
+                                     _f64p, _f64p, _f64p, _u8p]
+        L.or_bspline.restype = C.c_double
+        L.or_bspline.argtypes = [C.c_int, C.c_double]
+        L.or_b_factors.argtypes = [C.c_int, C.c_int, _f64p]
+        L.or_fft3d.argtypes = [_f64p, _f64p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_spme_spread.argtypes = [C.c_int, _f32p, _f64p, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _f64p]
+        L.or_spme_recip.argtypes = [C.c_int, _f32p, _f64p, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]
+        L.or_vv_kick_drift_f32.argtypes = [C.c_int, _f32p, _i32p, _f32p, _f32p, _f32p,
+                                           _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.or_vv_kick_f32.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                     _f32p, C.c_float]
+        L.or_vv_kick_drift_lossless.argtypes = [C.c_int, _f32p, _i32p, _f32p, _f32p, _f32p,
+                                                _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                                _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.or_vv_kick_lossless.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f64p, _f64p, _f64p,
+                                          _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.or_settle_positions.argtypes = [C.c_int, _u32p, _f64p, _f64p, _f64p, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          _f64p, C.c_double, _f64p]
+        L.or_settle_velocities.argtypes = [C.c_int, _u32p, _f64p, _f64p, C.c_double,
+                                           C.c_double, _f64p, C.c_double, _f64p]
+        L.or_nhc_half_step.restype = C.c_double
+        L.or_nhc_half_step.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, C.c_int,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double)]
+        L.or_kinetic_energy.restype = C.c_double
+        L.or_kinetic_energy.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
+        L.or_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _lat(lattice):
+    return np.ascontiguousarray(lattice, dtype=np.float64)
+
+
+def pair_set(posq, lattice, r_search, band=4.0 * 2.0 ** -23):
+    """Canonical set of pairs (i<j) with r^2 <= r_search^2, plus the pairs in
+    the rounding band around the threshold (neighbor_list.rs:1366-1506)."""
+    posq = np.ascontiguousarray(posq, np.float32)
+    n = posq.shape[0]
+    cap = 1 << 20
+    bcap = 1 << 16
+    while True:
+        out = np.zeros(2 * cap, np.uint32)
+        bout = np.zeros(2 * bcap, np.uint32)
+        nb = _i64(0)
+        cnt = lib().or_pair_set(n, posq.reshape(-1), _lat(lattice), float(r_search) ** 2,
+                                band, cap, out, bcap, bout, C.byref(nb))
+        if cnt <= cap and nb.value <= bcap:
+            return out[:2 * cnt].reshape(-1, 2), bout[:2 * nb.value].reshape(-1, 2)
+        cap = max(cap, int(cnt) + 1)
+        bcap = max(bcap, int(nb.value) + 1)
+
+
+def pair_forces(posq, types, lj, lattice, coul=None, excl=None, band=0.0):
+    """LJ + real-space Ewald pair forces in double.
+
+    lj: dict with ntypes and (ntypes x ntypes) arrays sigma, epsilon, cutoff,
+        r_switch (atomic units).
+    coul: None or dict(alpha=, r_cut=, k_c=1.0).
+    excl: None or (offsets u32[n+1], partners u32[], scales f64[]).
+    Returns F (n,3), E (n,), W (n,), band_flag (n,) bool.
+    """
+    posq = np.ascontiguousarray(posq, np.float32)
+    n = posq.shape[0]
+    nt = int(lj["ntypes"])
+    arr = lambda k: np.ascontiguousarray(np.asarray(lj[k], np.float64).reshape(-1))
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(n)
+    bf = np.zeros(max(n, 1), np.uint8)
+    alpha = rc = 0.0
+    kc = 1.0
+    if coul is not None:
+        alpha, rc, kc = float(coul["alpha"]), float(coul["r_cut"]), float(coul.get("k_c", 1.0))
+    keep = []
+    if excl is not None:
+        off = np.ascontiguousarray(excl[0], np.uint32)
+        par = np.ascontiguousarray(excl[1], np.uint32)
+        sca = np.ascontiguousarray(excl[2], np.float64)
+        keep = [off, par, sca]
+        ptrs = [a.ctypes.data for a in keep]
+    else:
+        ptrs = [None, None, None]
+    lib().or_pair_forces(n, posq.reshape(-1), np.ascontiguousarray(types, np.uint32), nt,
+                         arr("sigma"), arr("epsilon"), arr("cutoff"), arr("r_switch"),
+                         kc, alpha, rc, ptrs[0], ptrs[1], ptrs[2], _lat(lattice), band,
+                         F, E, W, bf)
+    del keep
+    return F.reshape(n, 3), E, W, bf[:n].astype(bool)
+
+
+def bspline(p, x):
+    return lib().or_bspline(int(p), float(x))
+
+
+def b_factors(n, p):
+    out = np.zeros(n)
+    lib().or_b_factors(n, p, out)
+    return out
+
+
+def fft3d(re, im, sign):
+    re = np.ascontiguousarray(re, np.float64).copy()
+    im = np.ascontiguousarray(im, np.float64).copy()
+    na, nb, nc = re.shape
+    lib().or_fft3d(re.reshape(-1), im.reshape(-1), na, nb, nc, sign)
+    return re, im
+
+
+def spme_spread(posq, lattice, grid, order):
+    posq = np.ascontiguousarray(posq, np.float32)
+    na, nb, nc = grid
+    rho = np.zeros(na * nb * nc)
+    lib().or_spme_spread(posq.shape[0], posq.reshape(-1), _lat(lattice), na, nb, nc, order, rho)
+    return rho.reshape(na, nb, nc)
+
+
+def spme_recip(posq, lattice, grid, order, alpha, k_c=1.0, want_v=False):
+    """Reciprocal SPME in double.  Returns F (n,3), E (n,), W (n,), e_recip,
+    w_recip and (optionally) the potential grid V."""
+    posq = np.ascontiguousarray(posq, np.float32)
+    n = posq.shape[0]
+    na, nb, nc = grid
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(n)
+    er = C.c_double(0)
+    wr = C.c_double(0)
+    V = np.zeros(na * nb * nc) if want_v else None
+    lib().or_spme_recip(n, posq.reshape(-1), _lat(lattice), na, nb, nc, order, float(alpha),
+                        float(k_c), F, E, W, C.byref(er), C.byref(wr),
+                        V.ctypes.data if want_v else None)
+    out = (F.reshape(n, 3), E, W, er.value, wr.value)
+    if want_v:
+        out = out + (V.reshape(na, nb, nc),)
+    return out
+
+
+def vv_kick_drift_f32(posq, img, v, f, mass, lattice, dt):
+    """In-place f32 kick+drift (integrate.cu:24-140 with LOSSLESS=false)."""
+    n = posq.shape[0]
+    lib().or_vv_kick_drift_f32(n, posq.reshape(-1), img.reshape(-1), v[0], v[1], v[2],
+                               f[0], f[1], f[2], mass,
+                               np.ascontiguousarray(lattice, np.float32), np.float32(dt))
+
+
+def vv_kick_f32(v, f, mass, dt):
+    n = v.shape[1]
+    lib().or_vv_kick_f32(n, v[0], v[1], v[2], f[0], f[1], f[2], mass, np.float32(dt))
+
+
+def vv_kick_drift_lossless(posq, img, v, xlo, vlo, f, mass, lattice, dt):
+    n = posq.shape[0]
+    lib().or_vv_kick_drift_lossless(n, posq.reshape(-1), img.reshape(-1), v[0], v[1], v[2],
+                                    xlo[0], xlo[1], xlo[2], vlo[0], vlo[1], vlo[2],
+                                    f[0], f[1], f[2], mass,
+                                    np.ascontiguousarray(lattice, np.float32), np.float32(dt))
+
+
+def vv_kick_lossless(v, vlo, f, mass, dt):
+    n = v.shape[1]
+    lib().or_vv_kick_lossless(n, v[0], v[1], v[2], vlo[0], vlo[1], vlo[2], f[0], f[1], f[2],
+                              mass, np.float32(dt))
+
+
+def settle_geometry(d_oh, d_hh, m_o, m_h):
+    """Canonical (ra, rb, rc) (src/integrator/settle.rs:83-91)."""
+    total = m_o + 2.0 * m_h
+    h = np.sqrt(d_oh * d_oh - (d_hh * 0.5) ** 2)
+    return (2.0 * m_h / total) * h, (m_o / total) * h, d_hh * 0.5
+
+
+def settle_positions(groups, pos, vel, snap, geom, m_o, m_h, lattice, dt):
+    """In-place double SETTLE position reset; returns the per-slot virial."""
+    g = np.ascontiguousarray(groups, np.uint32)
+    vir = np.zeros(3 * g.shape[0])
+    ra, rb, rc = geom
+    lib().or_settle_positions(g.shape[0], g.reshape(-1), pos.reshape(-1), vel.reshape(-1),
+                              np.ascontiguousarray(snap, np.float64).reshape(-1),
+                              ra, rb, rc, m_o, m_h, _lat(lattice), dt, vir)
+    return vir
+
+
+def settle_velocities(groups, pos, vel, m_o, m_h, lattice, dt, virial=None):
+    g = np.ascontiguousarray(groups, np.uint32)
+    vir = np.zeros(3 * g.shape[0]) if virial is None else virial
+    lib().or_settle_velocities(g.shape[0], g.reshape(-1),
+                               np.ascontiguousarray(pos, np.float64).reshape(-1),
+                               vel.reshape(-1), m_o, m_h, _lat(lattice), dt, vir)
+    return vir
+
+
+YOSHIDA = {
+    1: [1.0],
+    3: [1.3512071919596577, -1.7024143839193155, 1.3512071919596577],
+    5: [0.41449077179437574, 0.41449077179437574, -0.6579630871775030,
+        0.41449077179437574, 0.41449077179437574],
+    7: [0.7845136104775573, 0.2355732133593582, -1.1776799841788710, 1.3151863206839023,
+        -1.1776799841788710, 0.2355732133593582, 0.7845136104775573],
+}
+
+
+class NHCState:
+    """Host Nose-Hoover chain (src/integrator/nose_hoover_chain.rs:182-279)."""
+
+    def __init__(self, n_atoms, n_constraints, kt, tau, chain=3, yoshida=3, n_resp=1):
+        self.g_dof = float(max(3 * n_atoms - n_constraints - 3, 0))
+        self.kt = float(kt)
+        t2 = tau * tau
+        self.q = np.array([self.g_dof * kt * t2] + [kt * t2] * (chain - 1), np.float64)
+        self.xi = np.zeros(chain)
+        self.pxi = np.zeros(chain)
+        self.w = np.array(YOSHIDA[yoshida], np.float64)
+        self.n_resp = n_resp
+
+    def half_step(self, dt, k):
+        kout = C.c_double(0)
+        f = lib().or_nhc_half_step(len(self.q), self.xi, self.pxi, self.q, len(self.w), self.w,
+                                   self.n_resp, float(dt), float(k), self.g_dof, self.kt,
+                                   C.byref(kout))
+        return f, kout.value
+
+    def conserved_extra(self):
+        """Chain term of the nhc_conserved log column
+        (nose_hoover_chain.rs:365-381)."""
+        c = float(np.sum(self.pxi ** 2 / (2.0 * self.q)))
+        if len(self.xi):
+            c += self.g_dof * self.kt * self.xi[0] + self.kt * float(np.sum(self.xi[1:]))
+        return c
+
+
+def kinetic_energy(v, mass):
+    return lib().or_kinetic_energy(v.shape[1], v[0], v[1], v[2], mass)
+
+
+def num_threads():
+    return lib().or_num_threads()
