@@ -1,0 +1,235 @@
+"""Headline benchmark: one MI355X running the 196,608-atom SPC/SPME MD step.
+
+Workload (one "step" = one full velocity-Verlet MD step on the GPU):
+  SPC water, 65,536 molecules = 196,608 atoms, cubic box at 1.0 g/cm^3
+  (L = 12.516 nm), O on the first sites of a 42^3 grid with Shoemake-uniform
+  orientations (seed 7); LJ O-O sigma 0.3166 nm / eps 0.650 kJ/mol and
+  real-space Ewald with r_c = 0.9 nm (erfc(alpha r_c) = 1e-5); SPME 128^3,
+  order 4; SETTLE; Nose-Hoover chain at 300 K; C-rescale at 1 bar; dt 2 fs.
+Untimed set-up: Maxwell-Boltzmann velocities (ChaCha8 seed 7), an
+equilibration phase with a tight thermostat (the lattice start is far
+from equilibrium), then `--warmup` steps at the production settings.
+Timed: exactly `--steps` steps replayed as captured hipGraphs, bracketed by
+device synchronisation.  Afterwards a second pass of the same number of
+steps runs without graphs with hipEvents around every stage (same stream),
+which gives the per-kernel durations used for the roofline entry.
+
+Prints one JSON line (the driver's contract).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ns/day (196k-atom SPC SPME) & batched atom-steps/s vs plain, same error"
+# MI355X published peaks used for the roofline fraction
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Flops per useful pair (LJ + erfc-Coulomb, transcendental = 1 flop)
+FLOPS_PER_PAIR = 55.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_engine(system, U, args):
+    """NVT engine at dt = 1 fs with a tight thermostat (the equilibration
+    phase); the production settings are switched on after equilibration."""
+    from heddlemd_amd.engine import engine_for
+    from heddlemd_amd.rng import maxwell_boltzmann
+
+    kt = U.temperature(args.temperature)
+    v = maxwell_boltzmann(system.mass, kt, seed=7, n_constraints=system.n_constraints)
+    e = engine_for(system, dt=U.time(1e-15), nhc_kt=kt,
+                   nhc_tau=U.time(args.equil_tau_fs * 1e-15), velocities=v)
+    e.constrain_velocities()
+    e.force_eval(scalars=True)
+    return e, kt
+
+
+def equilibrate(e, system, U, kt, args):
+    """The lattice start releases a large amount of potential energy; drain
+    it with Maxwell-Boltzmann redraws at 1 fs, then settle at the production
+    time step with a tight thermostat."""
+    from heddlemd_amd.rng import maxwell_boltzmann
+
+    ndof = 3 * system.n - system.n_constraints - 3
+
+    def temp():
+        sc = e.scalars()
+        return 2 * sc["ke"] / ndof * U.TEMPERATURE_AU_TO_K, sc
+
+    tau = U.time(args.tau_fs * 1e-15)
+    for k in range(args.equil_redraws):
+        e.step(50, scalars_last=True)
+        T, sc = temp()
+        log(f"equil redraw {k}: T = {T:.1f} K before redraw, PE = {sc['pe']:.2f} Eh")
+        e.set_velocities(maxwell_boltzmann(system.mass, kt, seed=100 + k,
+                                           n_constraints=system.n_constraints))
+        e.constrain_velocities()
+        e.set_nhc(kt, tau, n_constraints=system.n_constraints)  # fresh chain
+    e.set_integrator(U.time(args.dt_fs * 1e-15))
+    done = 0
+    while done < args.equil:
+        k = min(250, args.equil - done)
+        e.step(k, scalars_last=True)
+        done += k
+        T, sc = temp()
+        log(f"equil {done}/{args.equil}: T = {T:.1f} K, PE = {sc['pe']:.2f} Eh, "
+            f"rebuilds {int(sc['rebuilds'])}")
+    e.set_nhc(kt, U.time(args.tau_fs * 1e-15), n_constraints=system.n_constraints)
+    if not args.no_barostat:
+        e.set_crescale(pressure=U.pressure(1.0e5), tau=U.time(1.0e-12),
+                       compressibility=U.inv_pressure(4.5e-10), kt=kt, seed=13)
+
+
+def cpu_baseline(system, args):
+    """Oracle restatement timed on the host: one full force evaluation
+    (pair + reciprocal) plus the velocity-Verlet/SETTLE update of the same
+    196,608-atom state, single thread, scaled to ns/day."""
+    import numpy as np
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    F, E, W, _ = O.pair_forces(system.posq, system.types, system.lj.as_dict(), system.lattice,
+                               coul=dict(alpha=system.spme.alpha, r_cut=system.spme.r_cut),
+                               excl=system.exclusion_csr())
+    Fr, Er, Wr, er, wr = O.spme_recip(system.posq, system.lattice, system.spme.grid,
+                                      system.spme.order, system.spme.alpha)
+    f = (F + Fr).T.astype(np.float32).copy()
+    posq = system.posq.copy()
+    img = np.zeros((system.n, 3), np.int32)
+    v = np.zeros((3, system.n), np.float32)
+    dt = np.float32(args.dt_fs * 1e-15 / 2.4188843265857195e-17)
+    O.vv_kick_drift_f32(posq, img, v, f, system.mass, system.lattice.astype(np.float32), dt)
+    O.vv_kick_f32(v, f, system.mass, dt)
+    t = time.perf_counter() - t0
+    ns_day = args.dt_fs * 1e-6 * 86400.0 / t
+    return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
+            "sample": f"1 MD step (pair forces + SPME 128^3 + VV update) of the same "
+                      f"{system.n}-atom state on 1 host thread of {os.cpu_count()} "
+                      f"({platform.processor() or platform.machine()}): {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--equil", type=int, default=1000,
+                    help="untimed equilibration steps at the production dt, tight thermostat")
+    ap.add_argument("--equil-redraws", type=int, default=10,
+                    help="50-step blocks at 1 fs, each ended by a velocity redraw")
+    ap.add_argument("--equil-tau-fs", type=float, default=100.0)
+    ap.add_argument("--tau-fs", type=float, default=100.0)
+    ap.add_argument("--dt-fs", type=float, default=2.0)
+    ap.add_argument("--temperature", type=float, default=300.0)
+    ap.add_argument("--molecules", type=int, default=65536)
+    ap.add_argument("--no-barostat", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline-pass", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU domain decomposition is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from heddlemd_amd import systems, units as U
+
+    if args.molecules == 65536:
+        system = systems.config_water_65536()
+    else:
+        side = int(math.ceil(args.molecules ** (1.0 / 3.0)))
+        system = systems.spc_water(args.molecules, side,
+                                   grid=(64, 64, 64) if args.molecules <= 16384 else (128,) * 3)
+    log(f"system {system.name}: {system.n} atoms, L = {system.lattice[0] * 0.0529177:.4f} nm, "
+        f"grid {system.spme.grid}")
+    t0 = time.time()
+    e, kt = build_engine(system, U, args)
+    log(f"setup {time.time() - t0:.1f} s; neighbour list {e.neighbor_stats()}")
+
+    t0 = time.time()
+    equilibrate(e, system, U, kt, args)
+    log(f"equilibration {time.time() - t0:.1f} s")
+    e.step(args.warmup, scalars_last=True)
+    e.sync()
+    reb0 = e.scalars()["rebuilds"]
+
+    # ---- timed region ------------------------------------------------------
+    e.use_graph(True)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(args.steps, scalars_last=True)
+    e.sync()
+    elapsed = time.perf_counter() - t0
+    sc = e.scalars()
+    ms_per_step = elapsed / args.steps * 1e3
+    ns_day = args.dt_fs * 1e-6 * (args.steps / elapsed) * 86400.0
+    atom_steps = system.n * args.steps / elapsed
+    rebuilds = int(sc["rebuilds"] - reb0)
+    T = 2 * sc["ke"] / (3 * system.n - system.n_constraints - 3) * U.TEMPERATURE_AU_TO_K
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, {ms_per_step:.3f} ms/step, {ns_day:.2f} ns/day,"
+        f" T = {T:.1f} K, rebuilds {rebuilds}")
+
+    result = {
+        "metric": METRIC, "value": round(ns_day, 3), "unit": "ns/day", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"SPC water {system.n // 3} molecules ({system.n} atoms), SPME "
+                         f"{'x'.join(map(str, system.spme.grid))} order 4, r_c 0.9 nm, SETTLE, "
+                         f"NHC 300 K" + ("" if args.no_barostat else " + C-rescale 1 bar")
+                         + f", dt {args.dt_fs} fs"),
+            "atoms": system.n, "parallelism": "single GPU",
+        },
+        "atom_steps_per_s": round(atom_steps, 1),
+        "neighbour_rebuilds_in_timed_region": rebuilds,
+        "temperature_K": round(T, 2),
+    }
+
+    # ---- roofline pass: same steps, no graph, hipEvents per stage ------------
+    if not args.no_roofline_pass:
+        e.use_graph(False)
+        e.set_timing(True)
+        t0 = time.perf_counter()
+        e.step(args.steps, scalars_last=True)
+        e.sync()
+        el2 = time.perf_counter() - t0
+        e.set_timing(False)
+        stages = e.stage_times()
+        pairs = e.count_pairs()
+        pair_ms = dict(stages).get("pair_force")
+        flops = pairs * FLOPS_PER_PAIR
+        achieved = flops / (pair_ms * 1e-3) / 1e12
+        result["roofline"] = {
+            "kernel": "k_pair (fused LJ + real-space Ewald, packed wave64 list)",
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "useful_pairs": pairs, "flops_per_pair": FLOPS_PER_PAIR,
+            "mean_launch_ms": round(pair_ms, 4),
+            "timing": f"hipEvents on the context stream over {args.steps} non-graph steps "
+                      f"({el2 / args.steps * 1e3:.3f} ms/step in that pass)",
+        }
+        result["stages_ms"] = {k: round(v, 4) for k, v in stages}
+        log(f"stages: {result['stages_ms']}")
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle restatement) ...")
+        result["cpu_baseline"] = cpu_baseline(system, args)
+    e.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

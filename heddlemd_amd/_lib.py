@@ -44,10 +44,13 @@ SIGNATURES = {
     "hd_set_integrator": (C.c_int, [_ctx, C.c_double, C.c_int]),
     "hd_set_nhc": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                              C.c_uint64]),
+    "hd_set_crescale": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_uint64]),
     "hd_setup": (C.c_int, [_ctx]),
     "hd_force_eval": (C.c_int, [_ctx, C.c_int]),
     "hd_constrain_velocities": (C.c_int, [_ctx]),
     "hd_step": (C.c_int, [_ctx, C.c_int, C.c_int]),
+    "hd_negate_velocities": (C.c_int, [_ctx]),
     "hd_use_graph": (C.c_int, [_ctx, C.c_int]),
     "hd_sync": (C.c_int, [_ctx]),
     "hd_get_positions": (C.c_int, [_ctx, _f32p, _vp, _vp, _vp]),
@@ -57,6 +60,8 @@ SIGNATURES = {
     "hd_get_lattice": (C.c_int, [_ctx, _f64p]),
     "hd_get_scalars": (C.c_int, [_ctx, _f64p]),
     "hd_get_pairs": (C.c_int, [_ctx, _vp, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "hd_count_pairs": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
+    "hd_get_barostat": (C.c_int, [_ctx, _f64p]),
     "hd_get_neighbor_stats": (C.c_int, [_ctx, _u64p]),
     "hd_get_spme_grids": (C.c_int, [_ctx, _vp, _vp]),
     "hd_selftest_dpp": (C.c_int, [_ctx, _i32p]),

@@ -73,6 +73,50 @@ __device__ __forceinline__ void hd_wrap_against_center(float& x, float& y, float
   z -= kc * L.lz;
 }
 
+// Position relative to a centre, shifted to the nearest periodic image,
+// evaluated in FP64 and rounded once: the result is small (|d| <~ r_cut +
+// block extent), so it keeps ~1e-7 relative precision where the f32
+// difference of two wrapped absolute coordinates would carry the rounding
+// error of the large absolute value.  Orthorhombic cells only.
+__device__ __forceinline__ void hd_rel_to_center(float x, float y, float z, float cx, float cy,
+                                                 float cz, const HdLattice& L, float& rx,
+                                                 float& ry, float& rz) {
+  double dx = (double)x - (double)cx, dy = (double)y - (double)cy, dz = (double)z - (double)cz;
+  double lx = L.lx, ly = L.ly, lz = L.lz;
+  dx -= lx * floor(dx / lx + 0.5);
+  dy -= ly * floor(dy / ly + 0.5);
+  dz -= lz * floor(dz / lz + 0.5);
+  rx = (float)dx;
+  ry = (float)dy;
+  rz = (float)dz;
+}
+
+// SPME spline origin of one atom: wrap into the primary cell, fractional
+// coordinate + 1/2, u = s * n, primary bin floor(u) (mod n) and offset
+// t = u - floor(u) (the geometry of kernels/spme_spread_gather.cuh:90-126).
+// Evaluated in FP64 and rounded once, so the offset t keeps ~1e-7
+// precision on 128^3 grids (f32 would lose log2(n) bits of it).
+__device__ __forceinline__ void hd_spline_origin(float x, float y, float z, const HdLattice& L,
+                                                 unsigned na, unsigned nb, unsigned nc,
+                                                 float& ta, float& tb, float& tc, int& ga,
+                                                 int& gb, int& gc) {
+  double sc = (double)z / (double)L.lz;
+  double sb = ((double)y - sc * (double)L.yz) / (double)L.ly;
+  double sa = ((double)x - sb * (double)L.xy - sc * (double)L.xz) / (double)L.lx;
+  sa -= floor(sa + 0.5);
+  sb -= floor(sb + 0.5);
+  sc -= floor(sc + 0.5);
+  double ua = (sa + 0.5) * (double)na, ub = (sb + 0.5) * (double)nb, uc = (sc + 0.5) * (double)nc;
+  double fa = floor(ua), fb = floor(ub), fc = floor(uc);
+  ta = (float)(ua - fa);
+  tb = (float)(ub - fb);
+  tc = (float)(uc - fc);
+  int la = (int)fa, lb = (int)fb, lc = (int)fc;
+  ga = ((la % (int)na) + (int)na) % (int)na;
+  gb = ((lb % (int)nb) + (int)nb) % (int)nb;
+  gc = ((lc % (int)nc) + (int)nc) % (int)nc;
+}
+
 // Wave64 rotation through DPP: every lane receives the value held by lane
 // (lane + 1) mod 64 ("wave_rol:1"; a full-rate VALU move, no LDS traffic).
 __device__ __forceinline__ int hd_rot1_i(int v) {

@@ -9,9 +9,10 @@
 struct HdNeighborArgs {
   unsigned n, npad, nvox, cap;
   unsigned nv[3];
-  float rs2, disp_thr2;
+  float rs2, half_skin;
   const float4* posq;
   const float* lattice;
+  float* ref_lattice;
   float4* ref_pos;
   const unsigned* voxel_rank;
   unsigned *atom_key, *voxel_count, *voxel_off, *voxel_cursor;
@@ -126,7 +127,20 @@ void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, 
 // Fixed-order FP64 sums of per-atom arrays a and b into out[0], out[1].
 void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
                     double* out);
+struct HdBarostatArgs {
+  unsigned seed_lo, seed_hi;
+  double pressure, tau, compressibility, kt, dt, mu3_min;
+};
+// C-rescale: mu from K, W (FP64 scalars) and a Philox normal; scales the
+// device lattice and then every position by mu.
+void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float* lattice,
+                        float* mu, double* diag, unsigned long long* counter,
+                        const HdBarostatArgs& b, float4* posq, unsigned n);
+// Pairs within r_c (both atoms valid, not excluded) in the current list.
+void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out);
 void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count);
+void hd_launch_negate_velocities(hipStream_t st, float* vx, float* vy, float* vz, double* vxlo,
+                                 double* vylo, double* vzlo, unsigned n);
 void hd_launch_selftest_dpp(hipStream_t st, int* out);
 
 #define HD_REDUCE_BLOCKS 256

@@ -72,6 +72,9 @@ struct hd_ctx {
   double nhc_kt = 0, nhc_tau = 0;
   int nhc_chain = 3, nhc_yoshida = 3, nhc_resp = 1;
   uint64_t nhc_nc = 0;
+  bool have_baro = false;
+  double baro_p = 0, baro_tau = 0, baro_kappa = 0, baro_kt = 0;
+  uint64_t baro_seed = 0;
   bool use_graph = true;
   bool timing = false;
   bool ready = false;
@@ -93,6 +96,10 @@ struct hd_ctx {
          *vzlo = nullptr;
   float4 *snap = nullptr, *ref_pos = nullptr;
   float* d_lattice = nullptr;
+  float* ref_lattice = nullptr;
+  float* baro_mu = nullptr;
+  double* baro_diag = nullptr;
+  unsigned long long* baro_counter = nullptr;
   // device: neighbour list
   unsigned *voxel_rank = nullptr, *atom_key = nullptr, *voxel_count = nullptr,
            *voxel_off = nullptr, *voxel_cursor = nullptr;
@@ -120,8 +127,11 @@ struct hd_ctx {
   // graphs
   hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
 
-  std::vector<Stage> stages;
-  std::vector<double> stage_ms;
+  std::vector<Stage> stages;  // event pool; the first n_stage_used are live
+  size_t n_stage_used = 0;
+  std::vector<std::string> stage_names;  // aggregated by name
+  std::vector<double> stage_ms;          // mean ms per occurrence
+  std::vector<long> stage_count;
 
   std::vector<void*> allocs;
 };
@@ -244,7 +254,8 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   for (int d = 0; d < 3; ++d) a.nv[d] = c->nv[d];
   double rs = c->rmax + c->r_skin;
   a.rs2 = (float)(rs * rs);
-  a.disp_thr2 = (float)(0.25 * c->r_skin * c->r_skin);
+  a.half_skin = (float)(0.5 * c->r_skin);
+  a.ref_lattice = c->ref_lattice;
   a.posq = c->posq;
   a.lattice = c->d_lattice;
   a.ref_pos = c->ref_pos;
@@ -294,16 +305,19 @@ HdNhcArgs nhc_args(hd_ctx* c) {
 
 void stage_begin(hd_ctx* c, const char* name, bool on) {
   if (!on) return;
-  Stage s;
+  if (c->n_stage_used == c->stages.size()) {
+    Stage s;
+    (void)hipEventCreate(&s.a);
+    (void)hipEventCreate(&s.b);
+    c->stages.push_back(s);
+  }
+  Stage& s = c->stages[c->n_stage_used++];
   s.name = name;
-  (void)hipEventCreate(&s.a);
-  (void)hipEventCreate(&s.b);
   (void)hipEventRecord(s.a, c->stream);
-  c->stages.push_back(s);
 }
 void stage_end(hd_ctx* c, bool on) {
   if (!on) return;
-  (void)hipEventRecord(c->stages.back().b, c->stream);
+  (void)hipEventRecord(c->stages[c->n_stage_used - 1].b, c->stream);
 }
 
 // Force evaluation; kick = fuse the post-force half kick with factor
@@ -423,6 +437,7 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
 }
 
 void enqueue_step(hd_ctx* c, bool ev, bool timed) {
+  // ev may be forced on below (a barostat consumes the virial every step)
   hipStream_t st = c->stream;
   unsigned n = (unsigned)c->n;
   HdNhcArgs nhc = nhc_args(c);
@@ -490,8 +505,31 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
                  (float)c->dt, c->factors + 1);
     stage_end(c, timed);
+  } else if (c->have_baro) {
+    hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
+                 0.f, nullptr);
   }
+  if (c->have_baro) ev = true;
   enqueue_forces(c, ev, true, timed);
+  if (c->have_baro) {
+    // C-rescale after the thermostat (src/runner.rs:2664-2668): K of the
+    // constrained, not yet kicked velocities and W of this evaluation;
+    // positions are scaled after the kick (post-force fragment order).
+    HdBarostatArgs b;
+    memset(&b, 0, sizeof(b));
+    b.seed_lo = (unsigned)(c->baro_seed & 0xFFFFFFFFull);
+    b.seed_hi = (unsigned)(c->baro_seed >> 32);
+    b.pressure = c->baro_p;
+    b.tau = c->baro_tau;
+    b.compressibility = c->baro_kappa;
+    b.kt = c->baro_kt;
+    b.dt = c->dt;
+    b.mu3_min = 1e-18;
+    stage_begin(c, "c_rescale", timed);
+    hd_launch_crescale(st, c->scal + 3, c->scal + 2, c->d_lattice, c->baro_mu, c->baro_diag,
+                       c->baro_counter, b, c->posq, n);
+    stage_end(c, timed);
+  }
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
@@ -694,25 +732,60 @@ int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, d
   return HD_OK;
 }
 
+static void drop_graphs(hd_ctx* c);
+
 int hd_set_integrator(hd_ctx* c, double dt, int lossless) {
   if (!c || !(dt > 0)) return HD_ERR_ARG;
-  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_integrator after hd_setup");
+  if (c->ready) {
+    // a phase change of the time step keeps the state; the mode is fixed
+    if ((lossless != 0) != c->lossless)
+      return fail(c, HD_ERR_STATE, "lossless mode is fixed at hd_setup");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dt = dt;
+    drop_graphs(c);
+    return HD_OK;
+  }
   c->dt = dt;
   c->lossless = lossless != 0;
+  return HD_OK;
+}
+
+static int upload_nhc_state(hd_ctx* c) {
+  int m = c->nhc_chain;
+  std::vector<double> st(3 * m, 0.0);
+  long long dof = std::max(3LL * (long long)c->n - (long long)c->nhc_nc - 3LL, 0LL);
+  double t2 = c->nhc_tau * c->nhc_tau;
+  st[2 * m] = (double)dof * c->nhc_kt * t2;
+  for (int j = 1; j < m; ++j) st[2 * m + j] = c->nhc_kt * t2;
+  HIPCHK(c, hipMemcpy(c->nhc_state, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
   return HD_OK;
 }
 
 int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_resp,
                uint64_t n_constraints) {
   if (!c) return HD_ERR_ARG;
-  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_nhc after hd_setup");
+  if (c->ready && !c->have_nhc)
+    return fail(c, HD_ERR_STATE, "a thermostat can only be re-parameterised after hd_setup");
   if (kt <= 0) {
+    if (c->ready) return fail(c, HD_ERR_STATE, "cannot remove the thermostat after hd_setup");
     c->have_nhc = false;
     return HD_OK;
   }
   if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
       !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
     return fail(c, HD_ERR_ARG, "invalid Nose-Hoover chain parameters");
+  if (c->ready) {
+    // new phase: fresh chain (xi = p_xi = 0) with the new masses
+    if (chain != c->nhc_chain) return fail(c, HD_ERR_STATE, "chain length is fixed at hd_setup");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nhc_kt = kt;
+    c->nhc_tau = tau;
+    c->nhc_yoshida = yoshida;
+    c->nhc_resp = n_resp;
+    c->nhc_nc = n_constraints;
+    drop_graphs(c);
+    return upload_nhc_state(c);
+  }
   c->have_nhc = true;
   c->nhc_kt = kt;
   c->nhc_tau = tau;
@@ -720,6 +793,28 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
   c->nhc_yoshida = yoshida;
   c->nhc_resp = n_resp;
   c->nhc_nc = n_constraints;
+  return HD_OK;
+}
+
+int hd_set_crescale(hd_ctx* c, double pressure, double tau, double compressibility, double kt,
+                    uint64_t seed) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) {
+    // a new phase may switch the barostat on or off or change it
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  if (tau <= 0) {
+    c->have_baro = false;
+    return HD_OK;
+  }
+  if (!(compressibility > 0) || !(kt >= 0)) return fail(c, HD_ERR_ARG, "invalid C-rescale parameters");
+  c->have_baro = true;
+  c->baro_p = pressure;
+  c->baro_tau = tau;
+  c->baro_kappa = compressibility;
+  c->baro_kt = kt;
+  c->baro_seed = seed;
   return HD_OK;
 }
 
@@ -805,6 +900,10 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->vxlo, n); DALLOC(c, c->vylo, n); DALLOC(c, c->vzlo, n);
   }
   DALLOC(c, c->d_lattice, 6);
+  DALLOC(c, c->ref_lattice, 6);
+  DALLOC(c, c->baro_mu, 1);
+  DALLOC(c, c->baro_diag, 4);
+  DALLOC(c, c->baro_counter, 1);
   DALLOC(c, c->voxel_rank, c->nvox); DALLOC(c, c->atom_key, n);
   DALLOC(c, c->voxel_count, c->nvox); DALLOC(c, c->voxel_off, c->nvox + 1);
   DALLOC(c, c->voxel_cursor, c->nvox);
@@ -978,13 +1077,7 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
       }
       HIPCHK(c, hipGraphLaunch(*gx, c->stream));
     } else {
-      if (timed) {
-        for (auto& st : c->stages) {
-          (void)hipEventDestroy(st.a);
-          (void)hipEventDestroy(st.b);
-        }
-        c->stages.clear();
-      }
+      if (timed && s == 0) c->n_stage_used = 0;
       enqueue_step(c, ev, timed);
       HIPCHK(c, hipGetLastError());
     }
@@ -992,13 +1085,36 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
   int r = check_status(c);
   if (r) return r;
   if (timed) {
+    // mean device time per stage occurrence over this call's steps
+    c->stage_names.clear();
     c->stage_ms.clear();
-    for (auto& st : c->stages) {
+    c->stage_count.clear();
+    for (size_t k = 0; k < c->n_stage_used; ++k) {
+      const Stage& st = c->stages[k];
       float ms = 0;
       (void)hipEventElapsedTime(&ms, st.a, st.b);
-      c->stage_ms.push_back(ms);
+      size_t j = 0;
+      while (j < c->stage_names.size() && c->stage_names[j] != st.name) ++j;
+      if (j == c->stage_names.size()) {
+        c->stage_names.push_back(st.name);
+        c->stage_ms.push_back(0.0);
+        c->stage_count.push_back(0);
+      }
+      c->stage_ms[j] += ms;
+      c->stage_count[j] += 1;
     }
+    for (size_t j = 0; j < c->stage_ms.size(); ++j) c->stage_ms[j] /= (double)c->stage_count[j];
   }
+  return HD_OK;
+}
+
+int hd_negate_velocities(hd_ctx* c) {
+  if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_negate_velocities before hd_setup");
+  hd_launch_negate_velocities(c->stream, c->vx, c->vy, c->vz, c->lossless ? c->vxlo : nullptr,
+                              c->lossless ? c->vylo : nullptr, c->lossless ? c->vzlo : nullptr,
+                              (unsigned)c->n);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return HD_OK;
 }
 
@@ -1018,7 +1134,7 @@ int hd_stage_times(hd_ctx* c, char* names, int name_len, double* ms, int cap) {
   if (!c || !names || !ms || name_len <= 0) return HD_ERR_ARG;
   int k = 0;
   for (size_t i = 0; i < c->stage_ms.size() && k < cap; ++i, ++k) {
-    snprintf(names + (size_t)k * name_len, name_len, "%s", c->stages[i].name.c_str());
+    snprintf(names + (size_t)k * name_len, name_len, "%s", c->stage_names[i].c_str());
     ms[k] = c->stage_ms[i];
   }
   return k;
@@ -1150,6 +1266,54 @@ int hd_get_pairs(hd_ctx* c, uint32_t* out, uint64_t cap, uint64_t* n_pairs) {
       out[2 * k] = (uint32_t)(pairs[k] >> 32);
       out[2 * k + 1] = (uint32_t)(pairs[k] & 0xFFFFFFFFu);
     }
+  return HD_OK;
+}
+
+int hd_count_pairs(hd_ctx* c, uint64_t* out) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  unsigned long long* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, sizeof(unsigned long long)));
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), c->stream);
+  HdPairArgs pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.n = (unsigned)c->n;
+  pa.npad = c->npad;
+  pa.cap = c->cap;
+  pa.rmax = c->rmax;
+  pa.lattice = c->d_lattice;
+  pa.tile_posq = c->tile_posq;
+  pa.tile_key = c->tile_key;
+  pa.entry_atoms = c->entry_atoms;
+  pa.entry_count = c->entry_count;
+  if (e == hipSuccess) {
+    hd_launch_tile_gather(c->stream, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
+                          c->fpj_ew, (unsigned)c->n, c->npad);
+    hd_launch_count_pairs(c->stream, pa, d);
+    e = hipGetLastError();
+  }
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "count pairs: %s", hipGetErrorString(e));
+  *out = h;
+  return HD_OK;
+}
+
+int hd_get_barostat(hd_ctx* c, double* out) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  float mu = 1.f;
+  double d[4] = {0, 0, 0, 0};
+  unsigned long long ctr = 0;
+  HIPCHK(c, hipMemcpyAsync(&mu, c->baro_mu, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d, c->baro_diag, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&ctr, c->baro_counter, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = mu;
+  out[1] = d[0];
+  out[2] = d[1];
+  out[3] = d[2];
+  out[4] = (double)ctr;
   return HD_OK;
 }
 

@@ -15,10 +15,12 @@
 // fixed chunking and a fixed pairwise tree, so they are bit-identical run
 // to run (the reference's f32 trees: kernels/nose_hoover.cu:22-105,
 // kernels/barostat.cu:11-78).
+// (before the includes, so the shared helpers compile without contraction
+// in this translation unit too)
+#pragma clang fp contract(off)
+
 #include "hd_device.h"
 #include "hd_kernels.h"
-
-#pragma clang fp contract(off)
 
 template <bool LOSSLESS, bool SNAP>
 __global__ void k_kick_drift(HdIntegrateArgs a) {
@@ -402,6 +404,29 @@ void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, 
                      (unsigned)HD_REDUCE_BLOCKS, out);
 }
 
+// Time reversal: v -> -v including the compensated residuals (lossless
+// mode), so a following run retraces the trajectory
+// (rqm/pipeline-reversibility.md, tests/integration_lossless.rs:302-395).
+__global__ void k_negate_velocities(float* vx, float* vy, float* vz, double* vxlo, double* vylo,
+                                    double* vzlo, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = -vx[i];
+  vy[i] = -vy[i];
+  vz[i] = -vz[i];
+  if (vxlo) {
+    vxlo[i] = -vxlo[i];
+    vylo[i] = -vylo[i];
+    vzlo[i] = -vzlo[i];
+  }
+}
+void hd_launch_negate_velocities(hipStream_t st, float* vx, float* vy, float* vz, double* vxlo,
+                                 double* vylo, double* vzlo, unsigned n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_negate_velocities, dim3((n + 255) / 256), dim3(256), 0, st, vx, vy, vz,
+                     vxlo, vylo, vzlo, n);
+}
+
 __global__ void k_fill_f32(float* p, float v, size_t count) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
@@ -461,15 +486,9 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     HdLattice L = hd_load_lattice(a.lattice);
     float4 pq = a.tile_posq[s];
     float qi = pq.w;
-    float px = pq.x, py = pq.y, pz = pq.z;
-    int wa_, wb_, wc_;
-    hd_wrap_with_image(px, py, pz, wa_, wb_, wc_, L);
-    float sa, sb, sc;
-    hd_cart_to_frac(px, py, pz, L, sa, sb, sc);
-    float ua = (sa + 0.5f) * (float)a.na, ub = (sb + 0.5f) * (float)a.nb,
-          uc = (sc + 0.5f) * (float)a.nc;
-    int ga0 = (int)floorf(ua), gb0 = (int)floorf(ub), gc0 = (int)floorf(uc);
-    float ta = ua - (float)ga0, tb = ub - (float)gb0, tc = uc - (float)gc0;
+    float ta, tb, tc;
+    int ga0, gb0, gc0;
+    hd_spline_origin(pq.x, pq.y, pq.z, L, a.na, a.nb, a.nc, ta, tb, tc, ga0, gb0, gc0);
     float wa[GATHER_P], dwa[GATHER_P], wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
     int gia[GATHER_P], gib[GATHER_P], gic[GATHER_P];
 #pragma unroll
@@ -564,6 +583,80 @@ void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev) {
   HD_FIN(false, false, false, true) HD_FIN(false, true, false, true)
   HD_FIN(true, false, false, true) HD_FIN(true, true, false, true)
 #undef HD_FIN
+}
+
+// ---------------------------------------------------------------------
+// Philox-4x32-10 (Salmon et al., SC11; kernels/philox.cuh:20-70) and the
+// C-rescale barostat (kernels/barostat.cu:144-232).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(unsigned k0, unsigned k1, unsigned c0, unsigned c1,
+                                              unsigned c2, unsigned c3, unsigned out[4]) {
+  for (int r = 0; r < 10; ++r) {
+    unsigned hi0 = __umulhi(c0, 0xD2511F53u), lo0 = c0 * 0xD2511F53u;
+    unsigned hi2 = __umulhi(c2, 0xCD9E8D57u), lo2 = c2 * 0xCD9E8D57u;
+    unsigned n0 = hi2 ^ c1 ^ k0, n1 = lo2, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, unsigned c0,
+                                                      unsigned c1, unsigned c2, unsigned c3) {
+  unsigned o[4];
+  philox4x32_10(k0, k1, c0, c1, c2, c3, o);
+  const double sc = 1.0 / 4294967296.0;
+  double u1 = ((double)o[0] + 0.5) * sc, u2 = ((double)o[1] + 0.5) * sc;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// One thread: P = (2K + W)/(3V), mu^3 = 1 - kappa (dt/tau)(P0 - P)
+// + sqrt(2 kappa kT dt/(tau V)) r, clamped, lattice *= (f32) mu; the
+// draw counter advances by one.  K and W are read from FP64 scalars.
+__global__ void k_crescale_mu(const double* __restrict__ k, const double* __restrict__ w,
+                              float* lattice, float* mu_out, double* diag,
+                              unsigned long long* counter, HdBarostatArgs b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long ctr = *counter;
+  double lx = lattice[0], ly = lattice[1], lz = lattice[2];
+  double v_pre = lx * ly * lz;
+  double p = (2.0 * k[0] + w[0]) / (3.0 * v_pre);
+  double r = philox_gaussian_f64(b.seed_lo, b.seed_hi, (unsigned)(ctr & 0xFFFFFFFFull),
+                                 (unsigned)(ctr >> 32), 0u, 0u);
+  double det = -b.compressibility * (b.dt / b.tau) * (b.pressure - p);
+  double amp = sqrt(2.0 * b.compressibility * b.kt * b.dt / (b.tau * v_pre));
+  double mu3 = 1.0 + det + amp * r;
+  double mu3c = mu3 > b.mu3_min ? mu3 : b.mu3_min;
+  double mu = cbrt(mu3c);
+  double v_post = v_pre * mu3c;
+  float muf = (float)mu;
+  mu_out[0] = muf;
+  for (int i = 0; i < 6; ++i) lattice[i] *= muf;
+  diag[0] = p;
+  diag[1] = v_post;
+  diag[2] += b.pressure * (v_post - v_pre);
+  *counter = ctr + 1ull;
+}
+
+__global__ void k_scale_positions(float4* __restrict__ posq, const float* __restrict__ mu,
+                                  unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f = mu[0];
+  float4 p = posq[i];
+  p.x *= f;
+  p.y *= f;
+  p.z *= f;
+  posq[i] = p;
+}
+
+void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float* lattice,
+                        float* mu, double* diag, unsigned long long* counter,
+                        const HdBarostatArgs& b, float4* posq, unsigned n) {
+  hipLaunchKernelGGL(k_crescale_mu, dim3(1), dim3(64), 0, st, k, w, lattice, mu, diag, counter, b);
+  if (n)
+    hipLaunchKernelGGL(k_scale_positions, dim3((n + 255) / 256), dim3(256), 0, st, posq, mu, n);
 }
 
 // In-bounds check of the DPP wave_rol:1 lane layout: out[l] = value of

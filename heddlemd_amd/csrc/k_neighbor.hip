@@ -30,18 +30,26 @@ __device__ __forceinline__ bool nb_rebuild_due(const unsigned* status) {
   return (*(volatile const unsigned*)status & (NB_STATUS_REBUILD | NB_STATUS_FORCE)) != 0u;
 }
 
-// Displacement check against the positions of the last rebuild.
+// Displacement check against the positions of the last rebuild.  A box
+// change since the rebuild (barostat) moves every minimum-image pair
+// vector by at most sqrt(3) * max|dL|, which is added to each atom's
+// displacement so that |x - x_ref| + sqrt(3) max|dL| <= r_skin/2 still
+// bounds every pair-distance change by r_skin.
 __global__ void k_disp_check(const float4* __restrict__ posq, const float4* __restrict__ ref,
-                             const float* __restrict__ lattice, float thr2, unsigned* status,
-                             unsigned n) {
+                             const float* __restrict__ lattice,
+                             const float* __restrict__ ref_lattice, float half_skin,
+                             unsigned* status, unsigned n) {
   unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   bool over = false;
   if (i < n) {
     HdLattice L = hd_load_lattice(lattice);
+    float dl = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dl = fmaxf(dl, fabsf(lattice[k] - ref_lattice[k]));
     float4 p = posq[i], r = ref[i];
     float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
     hd_min_image(dx, dy, dz, L);
-    over = dx * dx + dy * dy + dz * dz > thr2;
+    over = sqrtf(dx * dx + dy * dy + dz * dz) + 1.7320508f * dl > half_skin;
   }
   if (__any(over) && (threadIdx.x & 63) == 0) atomicOr(status, NB_STATUS_REBUILD);
 }
@@ -251,6 +259,20 @@ __global__ void __launch_bounds__(256) k_build_entries(
   __builtin_amdgcn_wave_barrier();
   float4 cb = center[b], hb = half[b];
   unsigned ni = min(64u, n - b * 64);
+  // When the block is far from every periodic boundary (orthorhombic box,
+  // L/2 - half extent >= r_search), hold i and j relative to the block
+  // centre with the periodic shift applied once per atom, so each of the
+  // 64 x 64 distance tests is plain arithmetic (no min-image, no division).
+  float rsearch = sqrtf(rs2);
+  bool rel = (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) && (0.5f * L.lx - hb.x >= rsearch) &&
+             (0.5f * L.ly - hb.y >= rsearch) && (0.5f * L.lz - hb.z >= rsearch);
+  if (rel) {
+    float4 r;
+    hd_rel_to_center(pi.x, pi.y, pi.z, cb.x, cb.y, cb.z, L, r.x, r.y, r.z);
+    r.w = 0.f;
+    ipos[w][lane] = r;
+  }
+  __builtin_amdgcn_wave_barrier();
   unsigned nbuf = 0, nent = 0;
   unsigned* out = entry_atoms + (size_t)b * cap * 64;
   for (unsigned jb0 = b; jb0 < nblocks; jb0 += 64) {
@@ -267,14 +289,22 @@ __global__ void __launch_bounds__(256) k_build_entries(
       float4 pj = tile_posq[jslot];
       bool hit = false;
       if (jvalid) {
-        for (unsigned m = 0; m < ni; ++m) {
-          float4 a = ipos[w][m];
-          float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
-          hd_min_image(dx, dy, dz, L);
-          float r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 <= rs2 && (b * 64 + m) != jslot) {
-            hit = true;
-            break;
+        if (rel) {
+          float jx, jy, jz;
+          hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, jx, jy, jz);
+          for (unsigned m = 0; m < ni; ++m) {
+            float4 a = ipos[w][m];
+            float dx = a.x - jx, dy = a.y - jy, dz = a.z - jz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            hit |= (r2 <= rs2) & ((b * 64 + m) != jslot);
+          }
+        } else {
+          for (unsigned m = 0; m < ni; ++m) {
+            float4 a = ipos[w][m];
+            float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
+            hd_min_image(dx, dy, dz, L);
+            float r2 = dx * dx + dy * dy + dz * dz;
+            hit |= (r2 <= rs2) & ((b * 64 + m) != jslot);
           }
         }
       }
@@ -312,8 +342,10 @@ __global__ void __launch_bounds__(256) k_build_entries(
 // End of a rebuild: remember the reference positions and clear the
 // rebuild request bits (sticky overflow/high-water bits stay for the host).
 __global__ void k_rebuild_finish(const float4* __restrict__ posq, float4* __restrict__ ref,
-                                 unsigned* status, unsigned* rebuild_count, unsigned n) {
+                                 const float* __restrict__ lattice, float* __restrict__ ref_lattice,
+                                 unsigned* status, unsigned n) {
   if (!nb_rebuild_due(status)) return;
+  if (blockIdx.x == 0 && threadIdx.x < 6) ref_lattice[threadIdx.x] = lattice[threadIdx.x];
   // every thread observed the request before the first write below: the
   // clearing is done by the last workgroup after a grid-wide fence is not
   // possible, so it is done in a separate one-thread kernel.
@@ -339,7 +371,7 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
   unsigned gridN = min(cdiv(a.n, T), 2048u);
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st, a.posq,
-                     a.ref_pos, a.lattice, a.disp_thr2, a.status, a.n);
+                     a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
   hipLaunchKernelGGL(k_voxel_keys, dim3(gridN), dim3(T), 0, st, a.posq, a.lattice, a.voxel_rank,
                      a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.status, a.n);
   hipLaunchKernelGGL(k_voxel_scan, dim3(1), dim3(1024), 0, st, a.voxel_count, a.voxel_off,
@@ -371,6 +403,6 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
   unsigned gridN = min(cdiv(a.n, 256), 2048u);
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,
-                     a.status, a.rebuild_count, a.n);
+                     a.lattice, a.ref_lattice, a.status, a.n);
   hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(1), 0, st, a.status, a.rebuild_count);
 }

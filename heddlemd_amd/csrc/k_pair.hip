@@ -25,8 +25,10 @@
 //     self pair is excluded the same way;
 //   * when the box is orthorhombic and wide enough, every j-atom is shifted
 //     once per entry to the image nearest the i-block centre, so the inner
-//     loop needs no minimum-image arithmetic
-//     (the single-periodic-copy path, jit_composed.rs:1565-1600).
+//     loop needs no minimum-image arithmetic (the single-periodic-copy path,
+//     jit_composed.rs:1565-1600); both i and j are then held relative to
+//     the block centre (shift computed in FP64, rounded once), which keeps
+//     the f32 pair displacement accurate in boxes of 100+ Bohr.
 #include "hd_device.h"
 #include "hd_kernels.h"
 
@@ -72,7 +74,7 @@ __device__ __forceinline__ void pair_entries(
     unsigned jkey = jvalid ? tile_key[jslot] : HD_SENTINEL;
     unsigned jin = (jvalid && (jslot >> 6) == b) ? 1u : 0u;
     float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = pj.w;
-    if (SPC) hd_wrap_against_center(pjx, pjy, pjz, cb.x, cb.y, cb.z, L);
+    if (SPC) hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, pjx, pjy, pjz);
     float jfx = 0.f, jfy = 0.f, jfz = 0.f, je = 0.f, jw = 0.f;
     float efx = 0.f, efy = 0.f, efz = 0.f, ee = 0.f, ew = 0.f;
 #pragma unroll 4
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(256) k_pair(
              (0.5f * L.lz - hb.z >= P.spc_margin);
   float pix = pi.x, piy = pi.y, piz = pi.z;
   if (!ivalid) { pix = cb.x; piy = cb.y; piz = cb.z; }
-  if (spc) hd_wrap_against_center(pix, piy, piz, cb.x, cb.y, cb.z, L);
+  if (spc) hd_rel_to_center(pix, piy, piz, cb.x, cb.y, cb.z, L, pix, piy, piz);
   float c6row[PAIR_MAXT], c12row[PAIR_MAXT], cutrow[PAIR_MAXT], rswrow[PAIR_MAXT],
       idrow[PAIR_MAXT];
 #pragma unroll
@@ -223,6 +225,44 @@ __global__ void __launch_bounds__(256) k_pair(
     reinterpret_cast<longlong2*>(fpi)[2 * islot + 1] = make_longlong2(s[2], 0);
     if (EV) reinterpret_cast<longlong2*>(fpi_ew)[islot] = make_longlong2(s[3], s[4]);
   }
+}
+
+// Diagnostic: number of distinct pairs of the current list with
+// r^2 <= r_max^2 that are neither excluded nor padding (the "useful" pairs
+// the pair kernel evaluates).  One wave per i-block, plain loops.
+__global__ void __launch_bounds__(64) k_count_pairs(
+    float rmax2, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
+    const unsigned* __restrict__ tile_key, const unsigned* __restrict__ entry_atoms,
+    const unsigned* __restrict__ entry_count, unsigned cap, unsigned n,
+    unsigned long long* __restrict__ out) {
+  unsigned b = blockIdx.x, lane = threadIdx.x;
+  unsigned islot = b * 64 + lane;
+  if (islot >= n) return;
+  HdLattice L = hd_load_lattice(lattice);
+  float4 pi = tile_posq[islot];
+  unsigned ikey = tile_key[islot];
+  unsigned nent = entry_count[b];
+  const unsigned* ents = entry_atoms + (size_t)b * cap * 64;
+  unsigned long long cnt = 0;
+  for (unsigned e = 0; e < nent; ++e)
+    for (unsigned t = 0; t < 64; ++t) {
+      unsigned js = ents[(size_t)e * 64 + t];
+      if (js >= n) continue;
+      if ((js >> 6) == b && js <= islot) continue;
+      if (tile_key[js] == ikey) continue;
+      float4 pj = tile_posq[js];
+      float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      hd_min_image(dx, dy, dz, L);
+      if (dx * dx + dy * dy + dz * dz <= rmax2) ++cnt;
+    }
+  atomicAdd(out, cnt);
+}
+
+void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out) {
+  unsigned nblocks = a.npad / 64;
+  hipLaunchKernelGGL(k_count_pairs, dim3(nblocks), dim3(64), 0, st, (float)(a.rmax * a.rmax),
+                     a.lattice, a.tile_posq, a.tile_key, a.entry_atoms, a.entry_count, a.cap,
+                     a.n, out);
 }
 
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {

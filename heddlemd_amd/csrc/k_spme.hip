@@ -82,7 +82,7 @@ __global__ void k_spread(const float4* __restrict__ tile_posq, const float* __re
   HdLattice L = hd_load_lattice(lattice);
   float ta, tb, tc;
   int ga, gb, gc;
-  spline_setup(pq.x, pq.y, pq.z, L, na, nb, nc, ta, tb, tc, ga, gb, gc);
+  hd_spline_origin(pq.x, pq.y, pq.z, L, na, nb, nc, ta, tb, tc, ga, gb, gc);
   float wa[SPME_P], wb[SPME_P];
 #pragma unroll
   for (int d = 0; d < SPME_P; ++d) {

@@ -96,6 +96,11 @@ class Engine:
         self._chk(self._lib.hd_set_nhc(self._ctx, float(kt), float(tau), chain, yoshida, n_resp,
                                        int(n_constraints)), "hd_set_nhc")
 
+    def set_crescale(self, pressure, tau, compressibility, kt, seed):
+        self._chk(self._lib.hd_set_crescale(self._ctx, float(pressure), float(tau),
+                                            float(compressibility), float(kt), int(seed)),
+                  "hd_set_crescale")
+
     def setup(self):
         self._chk(self._lib.hd_setup(self._ctx), "hd_setup")
 
@@ -114,6 +119,9 @@ class Engine:
 
     def step(self, n_steps, scalars_last=False):
         self._chk(self._lib.hd_step(self._ctx, int(n_steps), int(bool(scalars_last))), "hd_step")
+
+    def negate_velocities(self):
+        self._chk(self._lib.hd_negate_velocities(self._ctx), "hd_negate_velocities")
 
     def sync(self):
         self._chk(self._lib.hd_sync(self._ctx), "hd_sync")
@@ -166,6 +174,16 @@ class Engine:
                   "hd_get_pairs")
         return out[:2 * n.value].reshape(-1, 2)
 
+    def count_pairs(self):
+        n = C.c_uint64(0)
+        self._chk(self._lib.hd_count_pairs(self._ctx, C.byref(n)), "hd_count_pairs")
+        return int(n.value)
+
+    def barostat(self):
+        out = np.zeros(5)
+        self._chk(self._lib.hd_get_barostat(self._ctx, out), "hd_get_barostat")
+        return dict(zip(["mu", "pressure", "volume", "injection", "draws"], out.tolist()))
+
     def neighbor_stats(self):
         out = np.zeros(6, np.uint64)
         self._chk(self._lib.hd_get_neighbor_stats(self._ctx, out), "hd_get_neighbor_stats")
@@ -197,7 +215,7 @@ class Engine:
 
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
-               velocities=None, device=0):
+               velocities=None, crescale=None, device=0):
     """Configure an Engine for a ``systems.System``."""
     e = Engine(device)
     e.set_system(system.posq, system.mass, system.types, system.excl_key)
@@ -216,6 +234,8 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_integrator(dt, lossless)
     if nhc_kt is not None:
         e.set_nhc(nhc_kt, nhc_tau, n_constraints=system.n_constraints)
+    if crescale is not None:
+        e.set_crescale(**crescale)
     if velocities is not None:
         e.set_velocities(velocities)
     e.setup()

@@ -81,6 +81,13 @@ int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);
 int hd_set_nhc(hd_ctx *ctx, double kt, double tau, int chain, int yoshida, int n_resp,
                uint64_t n_constraints);
 
+/* C-rescale barostat (src/integrator/c_rescale_barostat.rs,
+ * kernels/barostat.cu:144-232): target pressure, tau, isothermal
+ * compressibility (atomic units), k_B*T of the noise term and the Philox
+ * seed.  tau <= 0 removes it.  Forces scalars (virial) on every step. */
+int hd_set_crescale(hd_ctx *ctx, double pressure, double tau, double compressibility, double kt,
+                    uint64_t seed);
+
 /* Allocate device state and build the first neighbour list; must follow
  * the hd_set_* calls and precede any evaluation. */
 int hd_setup(hd_ctx *ctx);
@@ -95,6 +102,9 @@ int hd_constrain_velocities(hd_ctx *ctx);
  * src/runner.rs:2585-2687); the last step evaluates scalars when
  * scalars_last != 0.  Uses a captured hipGraph per step when enabled. */
 int hd_step(hd_ctx *ctx, int n_steps, int scalars_last);
+/* Time reversal v -> -v, including the FP64 residuals of the lossless mode
+ * (the velocity flip of tests/integration_lossless.rs:302-395). */
+int hd_negate_velocities(hd_ctx *ctx);
 int hd_use_graph(hd_ctx *ctx, int enable);
 int hd_sync(hd_ctx *ctx);
 
@@ -114,6 +124,12 @@ int hd_get_scalars(hd_ctx *ctx, double out[8]);
  * pairs with min < max, excluding sentinels (neighbor_list.rs:1366-1506).
  * Returns the count in *n_pairs; writes at most cap pairs (2*cap u32). */
 int hd_get_pairs(hd_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *n_pairs);
+/* Number of pairs with r <= r_cut (max cutoff) in the current list that
+ * are neither excluded nor padding: the pair kernel's useful work. */
+int hd_count_pairs(hd_ctx *ctx, uint64_t *count);
+/* Barostat state: out[0] last mu, out[1] pressure, out[2] volume after the
+ * move, out[3] cumulative P0*dV, out[4] Philox draw counter. */
+int hd_get_barostat(hd_ctx *ctx, double out[5]);
 /* out[0] blocks, out[1] entry capacity per block, out[2] max entries in a
  * block, out[3] total entries, out[4] rebuilds, out[5] status. */
 int hd_get_neighbor_stats(hd_ctx *ctx, uint64_t out[6]);
@@ -122,9 +138,10 @@ int hd_get_neighbor_stats(hd_ctx *ctx, uint64_t out[6]);
 int hd_get_spme_grids(hd_ctx *ctx, float *rho, float *v);
 /* Device self test of the wave64 rotation: out[l] = 7*((l+1)%64)+3. */
 int hd_selftest_dpp(hd_ctx *ctx, int32_t out[64]);
-/* Per-stage device time (ms) of the last timed step, names as in the
- * reference .timings table (src/timings.rs); returns the number of
- * stages written (<= cap). */
+/* Mean device time (ms) per occurrence of each stage over the steps of the
+ * last timed hd_step call (hipEvents on the context stream; stage names
+ * follow the reference's .timings table, src/timings.rs); returns the
+ * number of stages written (<= cap). */
 int hd_stage_times(hd_ctx *ctx, char *names, int name_len, double *ms, int cap);
 /* Enable per-stage event timing for non-graph steps. */
 int hd_set_timing(hd_ctx *ctx, int enable);

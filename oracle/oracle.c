@@ -226,6 +226,10 @@ int64_t or_pair_set(int n, const float *posq, const double *L, double rs2,
 /* (src/forces/jit_composed.rs:1445-1458, 1130-1206).                  */
 /* ------------------------------------------------------------------ */
 
+/* 0: exact erfc (the reference's f64 path); 1: Hastings polynomial. */
+static int g_erfc_hastings = 0;
+void or_set_erfc_mode(int hastings) { g_erfc_hastings = hastings; }
+
 typedef struct {
     const uint32_t *type;
     int ntypes;
@@ -297,7 +301,15 @@ static void pairforce_visit(int i, int j, double dx, double dy, double dz,
             double qq = c->q[i] * c->q[j];
             double ar = c->alpha * r;
             double gauss = exp(-ar * ar);
-            double ec = erfc(ar);
+            double ec;
+            if (g_erfc_hastings) {
+                /* A&S 7.1.26 (the reference's f32 path, src/forces/spme.rs:744-770) */
+                double t = 1.0 / (1.0 + 0.3275911 * ar);
+                ec = (0.254829592 + (-0.284496736 + (1.421413741 + (-1.453152027 +
+                      1.061405429 * t) * t) * t) * t) * t * gauss;
+            } else {
+                ec = erfc(ar);
+            }
             double e = c->kc * qq * ec * inv_r;
             double f = c->kc * qq * inv_r * inv_r *
                        (ec * inv_r + 2.0 * c->alpha * 0.5641895835477563 * gauss);

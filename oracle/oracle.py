@@ -77,6 +77,7 @@ def lib():
         L.or_kinetic_energy.restype = C.c_double
         L.or_kinetic_energy.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.or_num_threads.restype = C.c_int
+        L.or_set_erfc_mode.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -140,6 +141,12 @@ def pair_forces(posq, types, lj, lattice, coul=None, excl=None, band=0.0):
                          F, E, W, bf)
     del keep
     return F.reshape(n, 3), E, W, bf[:n].astype(bool)
+
+
+def set_erfc_mode(hastings: bool):
+    """Select the real-space erfc: exact (double erfc, the reference's f64
+    build) or the Hastings polynomial of its f32 build."""
+    lib().or_set_erfc_mode(1 if hastings else 0)
 
 
 def bspline(p, x):

@@ -1,0 +1,148 @@
+"""Pins the oracle's integrator, SETTLE and Nose-Hoover chain
+(kernels/integrate.cu, kernels/settle.cu, src/integrator/settle.rs,
+src/integrator/nose_hoover_chain.rs) against kinematics, rigid geometry,
+conservation laws and the published Suzuki-Yoshida weights."""
+import math
+
+import numpy as np
+import pytest
+
+from heddlemd_amd import systems
+
+
+def _state(n=8, seed=0):
+    rng = np.random.default_rng(seed)
+    posq = np.zeros((n, 4), np.float32)
+    posq[:, :3] = rng.uniform(-5, 5, size=(n, 3))
+    v = rng.normal(0, 1e-3, size=(3, n)).astype(np.float32)
+    f = rng.normal(0, 1e-2, size=(3, n)).astype(np.float32)
+    m = rng.uniform(1e3, 4e4, size=n).astype(np.float32)
+    return posq, v, f, m
+
+
+def test_vv_constant_force_kinematics(oracle):
+    # one kick-drift + kick with a constant force: x1 = x0 + v0 dt + a dt^2/2, v1 = v0 + a dt
+    posq, v, f, m = _state()
+    L = np.array([100, 100, 100, 0, 0, 0], np.float32)
+    dt = 40.0
+    x0 = posq[:, :3].astype(np.float64).copy()
+    v0 = v.astype(np.float64).copy()
+    img = np.zeros((8, 3), np.int32)
+    oracle.vv_kick_drift_f32(posq, img, v, f, m, L, dt)
+    oracle.vv_kick_f32(v, f, m, dt)
+    a = f.astype(np.float64) / m
+    assert np.allclose(posq[:, :3], x0 + (v0 * dt + 0.5 * a * dt * dt).T, rtol=0, atol=2e-5)
+    assert np.allclose(v, v0 + a * dt, rtol=1e-6, atol=1e-10)
+
+
+def test_vv_wrap_counts_images(oracle):
+    posq = np.array([[9.9, -9.9, 0.0, 0.0]], np.float32)
+    v = np.array([[0.01], [-0.01], [0.0]], np.float32)
+    f = np.zeros((3, 1), np.float32)
+    m = np.ones(1, np.float32)
+    L = np.array([20, 20, 20, 0, 0, 0], np.float32)
+    img = np.zeros((1, 3), np.int32)
+    oracle.vv_kick_drift_f32(posq, img, v, f, m, L, 20.0)
+    assert img.tolist() == [[1, -1, 0]]
+    assert -10 <= posq[0, 0] < 10 and -10 <= posq[0, 1] < 10
+    assert posq[0, 0] == pytest.approx(9.9 + 0.2 - 20, abs=1e-5)
+
+
+def test_lossless_time_reversal_is_bit_exact(oracle):
+    # compensated (f32, f64) VV retraces its f32 state exactly after v -> -v
+    # (tests/integration_lossless.rs:302-395), with forces from an analytic
+    # harmonic well evaluated in f32 at the current positions
+    posq, v, f, m = _state(16, 3)
+    L = np.array([100, 100, 100, 0, 0, 0], np.float32)
+    k = np.float32(1e-4)
+
+    def force(p):
+        return (-k * p[:, :3]).T.astype(np.float32).copy()
+
+    n = 16
+    xlo = np.zeros((3, n))
+    vlo = np.zeros((3, n))
+    img = np.zeros((n, 3), np.int32)
+    x_start, v_start = posq.copy(), v.copy()
+    f = force(posq)
+    for _ in range(50):
+        oracle.vv_kick_drift_lossless(posq, img, v, xlo, vlo, f, m, L, 40.0)
+        f = force(posq)
+        oracle.vv_kick_lossless(v, vlo, f, m, 40.0)
+    v *= -1
+    vlo *= -1
+    for _ in range(50):
+        oracle.vv_kick_drift_lossless(posq, img, v, xlo, vlo, f, m, L, 40.0)
+        f = force(posq)
+        oracle.vv_kick_lossless(v, vlo, f, m, 40.0)
+    assert np.array_equal(posq, x_start)
+    assert np.array_equal(-v, v_start)
+
+
+def _water(seed=1, n_mol=4):
+    s = systems.spc_water(n_mol=n_mol, n_side=2, grid=(16, 16, 16))
+    return s
+
+
+def test_settle_restores_rigid_geometry_and_com(oracle):
+    s = _water()
+    d_oh, d_hh = s.water_geom
+    m_o, m_h = float(s.mass[0]), float(s.mass[1])
+    geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
+    snap = s.posq[:, :3].astype(np.float64).copy()
+    rng = np.random.default_rng(5)
+    vel = rng.normal(0, 2e-3, size=(s.n, 3))
+    dt = 82.7
+    pos = snap + vel * dt  # unconstrained drift
+    masses = s.mass.astype(np.float64)
+    com_before = [(masses[g, None] * pos[g]).sum(0) / masses[g].sum() for g in s.water_groups]
+    vir = oracle.settle_positions(s.water_groups, pos, vel.copy(), snap, geom, m_o, m_h,
+                                  s.lattice, dt)
+    for gi, g in enumerate(s.water_groups):
+        o, h1, h2 = pos[g]
+        assert np.linalg.norm(o - h1) == pytest.approx(d_oh, rel=1e-9)
+        assert np.linalg.norm(o - h2) == pytest.approx(d_oh, rel=1e-9)
+        assert np.linalg.norm(h1 - h2) == pytest.approx(d_hh, rel=1e-9)
+        com = (masses[g, None] * pos[g]).sum(0) / masses[g].sum()
+        assert np.allclose(com, com_before[gi], atol=1e-10)
+    assert np.isfinite(vir).all()
+
+
+def test_settle_velocities_zero_bond_rates(oracle):
+    s = _water()
+    m_o, m_h = float(s.mass[0]), float(s.mass[1])
+    pos = s.posq[:, :3].astype(np.float64)
+    rng = np.random.default_rng(8)
+    vel = rng.normal(0, 1e-3, size=(s.n, 3))
+    p_before = (s.mass[:, None] * vel).sum(0)
+    oracle.settle_velocities(s.water_groups, pos, vel, m_o, m_h, s.lattice, 0.0)
+    for g in s.water_groups:
+        for a, b in ((0, 1), (0, 2), (1, 2)):
+            r = pos[g[a]] - pos[g[b]]
+            dv = vel[g[a]] - vel[g[b]]
+            assert abs(np.dot(r, dv)) < 1e-12
+    assert np.allclose((s.mass[:, None] * vel).sum(0), p_before, atol=1e-9)
+
+
+def test_yoshida_weights_are_consistent(oracle):
+    for order, w in oracle.YOSHIDA.items():
+        assert sum(w) == pytest.approx(1.0, abs=1e-14)
+    # Suzuki-Yoshida 3: w1 = 1/(2 - 2^(1/3)), w2 = 1 - 2 w1
+    w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    assert oracle.YOSHIDA[3][0] == pytest.approx(w1, rel=1e-15)
+    assert oracle.YOSHIDA[3][1] == pytest.approx(1 - 2 * w1, rel=1e-15)
+
+
+def test_nhc_chain_conserves_extended_energy(oracle):
+    # a free ideal gas of N_f DOF coupled to the chain: K * f^2 flows into
+    # the thermostat; K + chain term is conserved by the chain propagator
+    n, kt, tau = 100, 1e-3, 400.0
+    st = oracle.NHCState(n, 0, kt, tau)
+    k = 0.5 * st.g_dof * kt * 1.7
+    h0 = k + st.conserved_extra()
+    for _ in range(200):
+        f, k = st.half_step(40.0, k)
+    h1 = k + st.conserved_extra()
+    assert h1 == pytest.approx(h0, rel=1e-5)  # Trotter splitting error O(dt^3)
+    # and it relaxes toward the target temperature
+    assert abs(k / (0.5 * st.g_dof * kt) - 1.0) < 0.7
