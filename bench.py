@@ -137,7 +137,24 @@ def main():
     ap.add_argument("--no-barostat", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (the reference kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counter readings to bytes) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ok = bool(torch.equal(dst, src))
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "dispatches": 3, "ok": ok}),
+              flush=True)
+        sys.exit(0 if ok else 1)
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -159,16 +176,18 @@ def main():
     e, kt = build_engine(system, U, args)
     log(f"setup {time.time() - t0:.1f} s; neighbour list {e.neighbor_stats()}")
 
+    # the equilibration phases re-parameterise the thermostat often; they run
+    # as plain stream launches and only production captures step graphs
     t0 = time.time()
+    e.use_graph(False)
     equilibrate(e, system, U, kt, args)
     log(f"equilibration {time.time() - t0:.1f} s")
+    e.use_graph(True)
     e.step(args.warmup, scalars_last=True)
     e.sync()
     reb0 = e.scalars()["rebuilds"]
 
     # ---- timed region ------------------------------------------------------
-    e.use_graph(True)
-    e.sync()
     t0 = time.perf_counter()
     e.step(args.steps, scalars_last=True)
     e.sync()

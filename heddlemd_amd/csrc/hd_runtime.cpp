@@ -183,13 +183,34 @@ void dfree(hd_ctx* c, void* p) {
 
 bool is_pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
 
-uint64_t morton3(uint32_t a, uint32_t b, uint32_t c) {
-  uint64_t code = 0;
-  for (int bit = 0; bit < 21; ++bit) {
-    code |= (uint64_t)((a >> bit) & 1u) << (3 * bit + 2);
-    code |= (uint64_t)((b >> bit) & 1u) << (3 * bit + 1);
-    code |= (uint64_t)((c >> bit) & 1u) << (3 * bit);
+// 3D Hilbert index of cell (a, b, c) on a 2^bits cube, by Skilling's
+// transpose algorithm ("Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 2004).  Consecutive indices are face-adjacent cells, so a run of atoms
+// in this order is spatially compact (a Morton order jumps across the box
+// at octant boundaries, which gives some blocks box-sized extents).
+uint64_t hilbert3(uint32_t a, uint32_t b, uint32_t c, int bits) {
+  uint32_t X[3] = {a, b, c};
+  uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
   }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t code = 0;
+  for (int bit = bits - 1; bit >= 0; --bit)
+    for (int i = 0; i < 3; ++i) code = (code << 1) | ((X[i] >> bit) & 1u);
   return code;
 }
 
@@ -872,17 +893,26 @@ int hd_setup(hd_ctx* c) {
   unsigned n = (unsigned)c->n;
   c->nblocks = (n + 63) / 64;
   c->npad = c->nblocks * 64;
-  // voxel grid: about two atoms per voxel, proportional to the box edges
+  // voxel grid in fractional coordinates (so the order survives box
+  // rescaling): a power of two per axis, about two atoms per voxel, ranked
+  // along a Hilbert curve over the enclosing 2^bits cube
   double vol = c->lattice[0] * c->lattice[1] * c->lattice[2];
   double edge = cbrt(2.0 * vol / (double)n);
-  for (int d = 0; d < 3; ++d) c->nv[d] = (unsigned)std::max(1.0, std::min(1024.0, floor(c->lattice[d] / edge)));
+  int bits = 1;
+  for (int d = 0; d < 3; ++d) {
+    double want = std::max(1.0, c->lattice[d] / edge);
+    int k = (int)std::lround(std::log2(want));
+    k = std::max(0, std::min(k, 10));
+    c->nv[d] = 1u << k;
+    bits = std::max(bits, std::max(k, 1));
+  }
   c->nvox = c->nv[0] * c->nv[1] * c->nv[2];
   std::vector<std::pair<uint64_t, uint32_t>> codes(c->nvox);
   for (unsigned a = 0; a < c->nv[0]; ++a)
     for (unsigned b = 0; b < c->nv[1]; ++b)
       for (unsigned cc = 0; cc < c->nv[2]; ++cc) {
         unsigned lin = (a * c->nv[1] + b) * c->nv[2] + cc;
-        codes[lin] = {morton3(a, b, cc), lin};
+        codes[lin] = {hilbert3(a, b, cc, bits), lin};
       }
   std::sort(codes.begin(), codes.end());
   std::vector<unsigned> rank(c->nvox);
