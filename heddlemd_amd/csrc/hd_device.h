@@ -45,6 +45,23 @@ __device__ __forceinline__ void hd_min_image(float& dx, float& dy, float& dz, co
   dz -= kc * L.lz;
 }
 
+// Minimum image with the reciprocal box lengths precomputed (inv = 1/lx,
+// 1/ly, 1/lz).  It differs from hd_min_image only by the rounding of the
+// fractional coordinate, i.e. only for separations within an ulp of half a
+// box length, which lie far beyond any cutoff (setup requires r_search <
+// L/2).  Used by the pair kernel's general-cell path.
+__device__ __forceinline__ void hd_min_image_rcp(float& dx, float& dy, float& dz,
+                                                 const HdLattice& L, float ilx, float ily,
+                                                 float ilz) {
+  float sc = dz * ilz;
+  float sb = (dy - sc * L.yz) * ily;
+  float sa = (dx - sb * L.xy - sc * L.xz) * ilx;
+  float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+  dx -= ka * L.lx + kb * L.xy + kc * L.xz;
+  dy -= kb * L.ly + kc * L.yz;
+  dz -= kc * L.lz;
+}
+
 // In-place wrap into the primary cell returning the image triple
 // (kernels/pbc.cuh:74-92).
 __device__ __forceinline__ void hd_wrap_with_image(float& x, float& y, float& z, int& ka_o,
@@ -120,7 +137,9 @@ __device__ __forceinline__ void hd_spline_origin(float x, float y, float z, cons
 // Wave64 rotation through DPP: every lane receives the value held by lane
 // (lane + 1) mod 64 ("wave_rol:1"; a full-rate VALU move, no LDS traffic).
 __device__ __forceinline__ int hd_rot1_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, false);
+  // every lane is written (no row or bank masked, no lane out of range), so
+  // the "old" operand is never observed and bound_ctrl is irrelevant
+  return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float hd_rot1(float v) {
   return __int_as_float(hd_rot1_i(__float_as_int(v)));

@@ -16,6 +16,7 @@ struct HdNeighborArgs {
   float4* ref_pos;
   const unsigned* voxel_rank;
   unsigned *atom_key, *voxel_count, *voxel_off, *voxel_cursor;
+  unsigned* voxel_part;  // hd_voxel_scan_parts(nvox) + 1 chunk sums
   unsigned *sorted_id, *slot_of;
   const unsigned *type, *key;
   unsigned *tile_type, *tile_key;
@@ -26,6 +27,8 @@ struct HdNeighborArgs {
 };
 
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a);
+// number of chunk sums of the voxel-count scan (the buffer needs one more)
+unsigned hd_voxel_scan_parts(unsigned nvox);
 void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
 void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
                            const float* lattice, float4* tile_posq, long long* fpj,
@@ -58,17 +61,27 @@ struct HdSpmeArgs {
   double2* partials;
   double* recip;   // [E_recip, W_recip]
   float* w_share;  // W_recip / N
+  float4* spl;   // per slot (t_a, t_b, t_c, q) of the spline origin
+  unsigned* sg;  // per slot packed origin cell, 0xFFFFFFFF when q == 0
+  // voxel binning of the last neighbour rebuild (slot ranges per voxel)
+  const unsigned *voxel_rank, *voxel_off;
+  unsigned nv[3];
+  int mv[3];  // voxel margin per axis: >= half the skin in voxel units, plus one
 };
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
 unsigned hd_spme_partials_count(unsigned nb, unsigned nc);
+// brick shape of the owner-computes spread; returns the number of bricks
+unsigned hd_spread_bricks(unsigned na, unsigned nb, unsigned nc, unsigned brick[3]);
 
 struct HdFinalizeArgs {
   unsigned n, npad;
   const unsigned* sorted_id;
   const float4* tile_posq;
   const long long *fpi, *fpi_ew, *fpj, *fpj_ew;
-  // SPME gather (vgrid == nullptr disables)
+  // SPME gather (vgrid == nullptr disables), from the spline records
   const float* vgrid;
+  const float4* spl;
+  const unsigned* sg;
   unsigned na, nb, nc;
   double kc, alpha;
   const float* w_share;

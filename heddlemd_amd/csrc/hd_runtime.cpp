@@ -102,7 +102,7 @@ struct hd_ctx {
   unsigned long long* baro_counter = nullptr;
   // device: neighbour list
   unsigned *voxel_rank = nullptr, *atom_key = nullptr, *voxel_count = nullptr,
-           *voxel_off = nullptr, *voxel_cursor = nullptr;
+           *voxel_off = nullptr, *voxel_cursor = nullptr, *voxel_part = nullptr;
   unsigned *sorted_id = nullptr, *slot_of = nullptr, *tile_type = nullptr, *tile_key = nullptr;
   float4 *tile_posq = nullptr, *center = nullptr, *half = nullptr;
   unsigned *entry_atoms = nullptr, *entry_count = nullptr;
@@ -112,6 +112,8 @@ struct hd_ctx {
   long long* rho_fixed = nullptr;
   float2* cgrid = nullptr;
   float* vgrid = nullptr;
+  float4* spl = nullptr;  // SPME spline records per slot
+  unsigned* sg = nullptr;
   float2 *tw_a = nullptr, *tw_b = nullptr, *tw_c = nullptr;
   float *bf_a = nullptr, *bf_b = nullptr, *bf_c = nullptr;
   double2* recip_partials = nullptr;
@@ -285,6 +287,7 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   a.voxel_count = c->voxel_count;
   a.voxel_off = c->voxel_off;
   a.voxel_cursor = c->voxel_cursor;
+  a.voxel_part = c->voxel_part;
   a.sorted_id = c->sorted_id;
   a.slot_of = c->slot_of;
   a.type = c->type;
@@ -400,6 +403,8 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
     sa.rho_fixed = c->rho_fixed;
     sa.cgrid = c->cgrid;
     sa.vgrid = c->vgrid;
+    sa.spl = c->spl;
+    sa.sg = c->sg;
     sa.tw_a = c->tw_a;
     sa.tw_b = c->tw_b;
     sa.tw_c = c->tw_c;
@@ -409,6 +414,14 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
     sa.partials = c->recip_partials;
     sa.recip = c->recip;
     sa.w_share = c->w_share;
+    sa.voxel_rank = c->voxel_rank;
+    sa.voxel_off = c->voxel_off;
+    for (int d = 0; d < 3; ++d) {
+      sa.nv[d] = c->nv[d];
+      // atoms move at most r_skin/2 between rebuilds; 20 % slack for box
+      // changes since set-up, plus one voxel for the binning's rounding
+      sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] / c->lattice[d]) + 1;
+    }
     stage_begin(c, "spme_recip", timed);
     hd_launch_spme(st, sa);
     stage_end(c, timed);
@@ -424,6 +437,8 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   fa.fpj = c->fpj;
   fa.fpj_ew = c->fpj_ew;
   fa.vgrid = c->have_spme ? c->vgrid : nullptr;
+  fa.spl = c->spl;
+  fa.sg = c->sg;
   fa.na = c->grid[0];
   fa.nb = c->grid[1];
   fa.nc = c->grid[2];
@@ -937,6 +952,7 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->voxel_rank, c->nvox); DALLOC(c, c->atom_key, n);
   DALLOC(c, c->voxel_count, c->nvox); DALLOC(c, c->voxel_off, c->nvox + 1);
   DALLOC(c, c->voxel_cursor, c->nvox);
+  DALLOC(c, c->voxel_part, hd_voxel_scan_parts(c->nvox) + 1);
   DALLOC(c, c->sorted_id, c->npad); DALLOC(c, c->slot_of, n);
   DALLOC(c, c->tile_type, c->npad); DALLOC(c, c->tile_key, c->npad);
   DALLOC(c, c->tile_posq, c->npad);
@@ -979,6 +995,8 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->rho_fixed, M);
     DALLOC(c, c->cgrid, Mc);
     DALLOC(c, c->vgrid, M);
+    DALLOC(c, c->spl, c->npad);
+    DALLOC(c, c->sg, c->npad);
     auto ta = twiddles(c->grid[0]), tb = twiddles(c->grid[1]), tc = twiddles(c->grid[2]);
     DALLOC(c, c->tw_a, ta.size()); DALLOC(c, c->tw_b, tb.size()); DALLOC(c, c->tw_c, tc.size());
     if ((r = upload(c, c->tw_a, ta.data(), ta.size()))) return r;

@@ -482,13 +482,15 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     pe = hd_from_fixed48(ie.x + je.x);
     vir = hd_from_fixed48(ie.y + je.y);
   }
-  if (GATHER) {
+  unsigned gcell = GATHER ? a.sg[s] : 0xFFFFFFFFu;
+  if (GATHER && gcell != 0xFFFFFFFFu) {
+    // spline record of this slot (k_spline_setup): (t_a, t_b, t_c, q) and
+    // the packed origin cell; uncharged atoms have none and no SPME terms
     HdLattice L = hd_load_lattice(a.lattice);
-    float4 pq = a.tile_posq[s];
-    float qi = pq.w;
-    float ta, tb, tc;
-    int ga0, gb0, gc0;
-    hd_spline_origin(pq.x, pq.y, pq.z, L, a.na, a.nb, a.nc, ta, tb, tc, ga0, gb0, gc0);
+    float4 rec = a.spl[s];
+    float qi = rec.w;
+    float ta = rec.x, tb = rec.y, tc = rec.z;
+    int ga0 = (int)(gcell & 1023u), gb0 = (int)((gcell >> 10) & 1023u), gc0 = (int)(gcell >> 20);
     float wa[GATHER_P], dwa[GATHER_P], wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
     int gia[GATHER_P], gib[GATHER_P], gic[GATHER_P];
 #pragma unroll
@@ -530,9 +532,10 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     if (EV) {
       float self = (float)(a.kc * a.alpha * 0.5641895835477563) * qi * qi;
       pe = pe + (0.5f * qi * phi - self);
-      vir = vir + a.w_share[0];
     }
   }
+  // every atom, charged or not, carries W_recip / N (spme.rs:815-823)
+  if (GATHER && EV) vir = vir + a.w_share[0];
   a.fx[id] = fx;
   a.fy[id] = fy;
   a.fz[id] = fz;

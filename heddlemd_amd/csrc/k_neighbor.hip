@@ -10,7 +10,7 @@
 //     (kernels/neighbor.cu:643-671).
 // MI355X design (what changes):
 //   * blocks are 64 atoms = one wave64; entries hold 64 j-atoms;
-//   * atoms are ordered along a Morton curve over a fixed voxel grid
+//   * atoms are ordered along a Hilbert curve over a fixed voxel grid
 //     (voxel = fractional-coordinate cell, so the order survives box
 //     rescaling) instead of by particle id inside r_search cells, which
 //     makes a block spatially compact;
@@ -54,7 +54,7 @@ __global__ void k_disp_check(const float4* __restrict__ posq, const float4* __re
   if (__any(over) && (threadIdx.x & 63) == 0) atomicOr(status, NB_STATUS_REBUILD);
 }
 
-// Voxel key = Morton rank of the atom's fractional voxel; histogram.
+// Voxel key = curve rank of the atom's fractional voxel; histogram.
 __global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __restrict__ lattice,
                              const unsigned* __restrict__ voxel_rank, unsigned nva, unsigned nvb,
                              unsigned nvc, unsigned* __restrict__ atom_key,
@@ -81,34 +81,97 @@ __global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __res
   }
 }
 
-// Single-workgroup exclusive scan of voxel_count -> voxel_off, and
-// clears the scatter cursors.  1024 threads, each owns a contiguous run.
-__global__ void __launch_bounds__(1024) k_voxel_scan(const unsigned* __restrict__ voxel_count,
-                                                     unsigned* __restrict__ voxel_off,
-                                                     unsigned* __restrict__ voxel_cursor,
-                                                     const unsigned* status, unsigned nvox) {
-  if (!nb_rebuild_due(status)) return;
-  __shared__ unsigned part[1024];
+// Exclusive scan of voxel_count -> voxel_off (and cleared scatter
+// cursors) in three passes over chunks of NB_SCAN_CHUNK voxels: chunk sums,
+// a one-workgroup scan of the chunk sums, then a workgroup scan per chunk.
+#define NB_SCAN_CHUNK 2048u  // 256 threads x 8 voxels
+
+__device__ __forceinline__ unsigned nb_block_excl_scan256(unsigned v, unsigned* tmp,
+                                                          unsigned& total) {
   unsigned t = threadIdx.x;
-  unsigned per = (nvox + 1023u) / 1024u;
-  unsigned b = t * per, e = min(b + per, nvox);
-  unsigned s = 0;
-  for (unsigned k = b; k < e; ++k) s += voxel_count[k];
-  part[t] = s;
+  tmp[t] = v;
   __syncthreads();
-  for (unsigned off = 1; off < 1024; off <<= 1) {
-    unsigned v = (t >= off) ? part[t - off] : 0u;
+  for (unsigned off = 1; off < 256u; off <<= 1) {
+    unsigned u = (t >= off) ? tmp[t - off] : 0u;
     __syncthreads();
-    part[t] += v;
+    tmp[t] += u;
     __syncthreads();
   }
-  unsigned run = part[t] - s;
-  for (unsigned k = b; k < e; ++k) {
-    voxel_off[k] = run;
-    run += voxel_count[k];
-    voxel_cursor[k] = 0u;
+  total = tmp[255];
+  unsigned r = tmp[t] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_voxel_scan_reduce(const unsigned* __restrict__ count,
+                                                           unsigned* __restrict__ part,
+                                                           const unsigned* status, unsigned nvox) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ unsigned tmp[256];
+  unsigned base = blockIdx.x * NB_SCAN_CHUNK;
+  unsigned s = 0;
+#pragma unroll
+  for (unsigned k = 0; k < NB_SCAN_CHUNK / 256u; ++k) {
+    unsigned i = base + k * 256u + threadIdx.x;
+    if (i < nvox) s += count[i];
   }
-  if (t == 1023) voxel_off[nvox] = part[1023];
+  unsigned total;
+  (void)nb_block_excl_scan256(s, tmp, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_voxel_scan_parts(unsigned* __restrict__ part,
+                                                           const unsigned* status,
+                                                           unsigned nparts) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ unsigned tmp[1024];
+  unsigned t = threadIdx.x, carry = 0;
+  for (unsigned base = 0; base < nparts; base += 1024u) {
+    unsigned i = base + t;
+    unsigned v = (i < nparts) ? part[i] : 0u;
+    tmp[t] = v;
+    __syncthreads();
+    for (unsigned off = 1; off < 1024u; off <<= 1) {
+      unsigned u = (t >= off) ? tmp[t - off] : 0u;
+      __syncthreads();
+      tmp[t] += u;
+      __syncthreads();
+    }
+    if (i < nparts) part[i] = carry + tmp[t] - v;
+    carry += tmp[1023];
+    __syncthreads();
+  }
+  if (t == 0) part[nparts] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_voxel_scan_apply(const unsigned* __restrict__ count,
+                                                          const unsigned* __restrict__ part,
+                                                          unsigned* __restrict__ voxel_off,
+                                                          unsigned* __restrict__ voxel_cursor,
+                                                          const unsigned* status, unsigned nvox,
+                                                          unsigned nparts) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ unsigned tmp[256];
+  const unsigned per = NB_SCAN_CHUNK / 256u;
+  unsigned b0 = blockIdx.x * NB_SCAN_CHUNK + threadIdx.x * per;
+  unsigned c[NB_SCAN_CHUNK / 256u];
+  unsigned s = 0;
+#pragma unroll
+  for (unsigned k = 0; k < per; ++k) {
+    c[k] = (b0 + k < nvox) ? count[b0 + k] : 0u;
+    s += c[k];
+  }
+  unsigned total;
+  unsigned run = part[blockIdx.x] + nb_block_excl_scan256(s, tmp, total);
+#pragma unroll
+  for (unsigned k = 0; k < per; ++k) {
+    if (b0 + k < nvox) {
+      voxel_off[b0 + k] = run;
+      voxel_cursor[b0 + k] = 0u;
+    }
+    run += c[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) voxel_off[nvox] = part[nparts];
 }
 
 __global__ void k_voxel_scatter(const unsigned* __restrict__ atom_key,
@@ -366,6 +429,8 @@ __global__ void k_rebuild_clear(unsigned* status, unsigned* rebuild_count) {
 // ---------------------------------------------------------------------
 static inline unsigned cdiv(unsigned a, unsigned b) { return (a + b - 1) / b; }
 
+unsigned hd_voxel_scan_parts(unsigned nvox) { return cdiv(nvox, NB_SCAN_CHUNK); }
+
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
   const unsigned T = 256;
   unsigned gridN = min(cdiv(a.n, T), 2048u);
@@ -374,8 +439,13 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
                      a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
   hipLaunchKernelGGL(k_voxel_keys, dim3(gridN), dim3(T), 0, st, a.posq, a.lattice, a.voxel_rank,
                      a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.status, a.n);
-  hipLaunchKernelGGL(k_voxel_scan, dim3(1), dim3(1024), 0, st, a.voxel_count, a.voxel_off,
-                     a.voxel_cursor, a.status, a.nvox);
+  unsigned nparts = hd_voxel_scan_parts(a.nvox);
+  hipLaunchKernelGGL(k_voxel_scan_reduce, dim3(nparts), dim3(256), 0, st, a.voxel_count,
+                     a.voxel_part, a.status, a.nvox);
+  hipLaunchKernelGGL(k_voxel_scan_parts, dim3(1), dim3(1024), 0, st, a.voxel_part, a.status,
+                     nparts);
+  hipLaunchKernelGGL(k_voxel_scan_apply, dim3(nparts), dim3(256), 0, st, a.voxel_count,
+                     a.voxel_part, a.voxel_off, a.voxel_cursor, a.status, a.nvox, nparts);
   hipLaunchKernelGGL(k_voxel_scatter, dim3(gridN), dim3(T), 0, st, a.atom_key, a.voxel_off,
                      a.voxel_cursor, a.sorted_id, a.status, a.n);
   unsigned gridV = min(cdiv(a.nvox, T), 2048u);

@@ -42,69 +42,120 @@ struct PairParams {
   float rsw2[PAIR_MAXT * PAIR_MAXT];
   float inv_delta[PAIR_MAXT * PAIR_MAXT];  // 1/(rc^2 - rs^2), 0 when no switch
   int ntypes;
-  int any_switch;
-  float kc, alpha, coul_rc2;  // coul_rc2 = 0 disables the Coulomb term
+  float kc, alpha, coul_rc2;  // coul_rc2 < 0 disables the Coulomb term
   float rmax2;
   float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
 };
 
-__device__ __forceinline__ float sel4(const float* row, unsigned t) {
-  float v = row[0];
-  v = (t == 1u) ? row[1] : v;
-  v = (t == 2u) ? row[2] : v;
-  v = (t == 3u) ? row[3] : v;
-  return v;
+// The i-atom's row of the LJ type-pair table, held in VGPRs for the whole
+// kernel.  NT = table width (1, 2 or 4).  Each value passes through an
+// empty asm that pins it to a register: otherwise the compiler folds the
+// per-pair select chain back into an indexed load from a private array,
+// which lands in scratch memory inside the inner loop.
+template <int NT>
+struct LjRow {
+  float c6[NT], c12[NT], f6[NT], f12[NT], cut2[NT], rsw2[NT], idl[NT];
+};
+
+template <int NT>
+__device__ __forceinline__ float pick(const float (&v)[NT], unsigned t) {
+  if (NT == 1) return v[0];
+  if (NT == 2) return t ? v[1] : v[0];
+  float r = v[0];
+  r = (t == 1u) ? v[1] : r;
+  r = (t == 2u) ? v[2] : r;
+  r = (t == 3u) ? v[3] : r;
+  return r;
 }
 
-template <bool EV, bool SPC>
+__device__ __forceinline__ void pin(float& x) { __asm__("" : "+v"(x)); }
+
+template <int NT>
+__device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned itype, bool ivalid) {
+  LjRow<NT> r;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    int k = (t < P.ntypes) ? (int)itype * P.ntypes + t : 0;
+    r.c6[t] = P.c6[k];
+    r.c12[t] = P.c12[k];
+    r.f6[t] = 6.0f * P.c6[k];
+    r.f12[t] = 12.0f * P.c12[k];
+    r.cut2[t] = ivalid ? P.ljcut2[k] : -1.0f;
+    r.rsw2[t] = P.rsw2[k];
+    r.idl[t] = P.inv_delta[k];
+    pin(r.c6[t]); pin(r.c12[t]); pin(r.f6[t]); pin(r.f12[t]);
+    pin(r.cut2[t]); pin(r.rsw2[t]); pin(r.idl[t]);
+  }
+  return r;
+}
+
+// Far-away coordinates for padding lanes on the shifted path: any pair
+// with one of them fails both cutoffs, and |d|^2 stays finite (< 1e37).
+#define PAIR_FAR_I 1.0e18f
+#define PAIR_FAR_J -1.0e18f
+
+// One wave's share of the entries of block b.  Lane = i-atom; the 64
+// j-atoms of an entry rotate through the wave with DPP wave_rol:1 and the
+// j-side partial sums travel with them, arriving back in the loading lane
+// after 64 steps.  SPC: coordinates are relative to the block centre (no
+// per-pair minimum image); otherwise general-cell minimum image per pair.
+template <bool EV, int NT, bool SW, bool SPC>
 __device__ __forceinline__ void pair_entries(
     const PairParams& P, const HdLattice& L, unsigned b, unsigned w, unsigned lane, float pix,
-    float piy, float piz, float qi, unsigned ikey, const float* c6row, const float* c12row,
-    const float* cutrow, const float* rswrow, const float* idrow, float4 cb,
+    float piy, float piz, float qi, unsigned ikey, const LjRow<NT>& R, float4 cb,
     const float4* __restrict__ tile_posq, const unsigned* __restrict__ tile_type,
     const unsigned* __restrict__ tile_key, const unsigned* __restrict__ ents, unsigned nent,
     unsigned n, unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
     long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w) {
   const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
+  const float alpha = P.alpha, coul_rc2 = P.coul_rc2;
+  const float kqi = P.kc * qi;
+  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
   for (unsigned e = w; e < nent; e += PAIR_WAVES) {
     unsigned jslot = ents[(size_t)e * 64 + lane];
     bool jvalid = jslot < n;
-    float4 pj = jvalid ? tile_posq[jslot] : cb;
-    unsigned jtype = jvalid ? tile_type[jslot] : 0u;
-    unsigned jkey = jvalid ? tile_key[jslot] : HD_SENTINEL;
-    unsigned jin = (jvalid && (jslot >> 6) == b) ? 1u : 0u;
-    float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = pj.w;
-    if (SPC) hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, pjx, pjy, pjz);
+    unsigned jsafe = jvalid ? jslot : 0u;
+    float4 pj = tile_posq[jsafe];
+    unsigned jtype = (NT > 1) ? tile_type[jsafe] : 0u;
+    unsigned jkey = tile_key[jsafe];
+    bool jin = jvalid && (jslot >> 6) == b;
+    float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = jvalid ? pj.w : 0.f;
+    if (SPC) {
+      hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, pjx, pjy, pjz);
+      if (!jvalid) pjx = pjy = pjz = PAIR_FAR_J;
+    } else {
+      // validity travels in the key: padding lanes can never pass the
+      // key test (i padding carries HD_SENTINEL - 1, j padding HD_SENTINEL)
+      jkey = jvalid ? jkey : HD_SENTINEL;
+    }
     float jfx = 0.f, jfy = 0.f, jfz = 0.f, je = 0.f, jw = 0.f;
     float efx = 0.f, efy = 0.f, efz = 0.f, ee = 0.f, ew = 0.f;
 #pragma unroll 4
     for (int t = 0; t < 64; ++t) {
       float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
-      if (!SPC) hd_min_image(dx, dy, dz, L);
+      if (!SPC) hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
       float r2 = dx * dx + dy * dy + dz * dz;
       float inv_r = __builtin_amdgcn_rsqf(r2);
       float inv_r2 = inv_r * inv_r;
-      // LJ (lj.rs:237-247, 250-287), row of the i-type selected by j-type
-      float c6 = sel4(c6row, jtype), c12 = sel4(c12row, jtype);
+      // LJ (lj.rs:237-247, 250-287) from the i-row, column = j type
       float ir6 = inv_r2 * inv_r2 * inv_r2;
-      float elj = ir6 * (c12 * ir6 - c6);
-      float flj = inv_r2 * ir6 * (12.0f * c12 * ir6 - 6.0f * c6);
-      if (P.any_switch) {
-        float rs2 = sel4(rswrow, jtype), idl = sel4(idrow, jtype);
-        if (idl > 0.f && r2 > rs2) {
-          float tau = (r2 - rs2) * idl;
-          float omt = 1.0f - tau;
-          float s = omt * omt * (1.0f + 2.0f * tau);
-          float chain = 12.0f * tau * omt * idl;
-          flj = s * flj + chain * elj;
-          elj = s * elj;
-        }
+      float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+      float elj = 0.f;
+      if (EV || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+      if (SW) {
+        float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
+        float tau = fmaxf((r2 - rs2) * idl, 0.0f);
+        float omt = 1.0f - tau;
+        float s = omt * omt * (1.0f + 2.0f * tau);
+        float chain = 12.0f * tau * omt * idl;
+        flj = s * flj + chain * elj;
+        elj = s * elj;
       }
-      bool lj_on = r2 <= sel4(cutrow, jtype);
+      bool lj_on = r2 <= pick<NT>(R.cut2, jtype);
       // real-space Coulomb, f32 Hastings erfc (spme.rs:744-770)
       float r = r2 * inv_r;
-      float qq = P.kc * qi * qj;
-      float ar = P.alpha * r;
+      float qq = kqi * qj;
+      float ar = alpha * r;
       float gauss = __expf(-(ar * ar));
       float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
       float erfc_ar =
@@ -113,23 +164,24 @@ __device__ __forceinline__ void pair_entries(
           tt * gauss;
       float ec = qq * erfc_ar * inv_r;
       float fc = qq * inv_r2 * (erfc_ar * inv_r + two_a_sqrtpi * gauss);
-      bool c_on = r2 <= P.coul_rc2;
-      bool ok = (jkey != ikey) & (max(jkey, ikey) < HD_SENTINEL - 1u);
+      bool c_on = r2 <= coul_rc2;
+      bool ok = (jkey != ikey);
+      if (!SPC) ok = ok & (max(jkey, ikey) < HD_SENTINEL - 1u);
       float factor = (ok & lj_on) ? flj : 0.f;
       factor += (ok & c_on) ? fc : 0.f;
       float fx = factor * dx, fy = factor * dy, fz = factor * dz;
       efx += fx; efy += fy; efz += fz;
-      float fj = jin ? 0.f : 1.f;
-      jfx -= fj * fx; jfy -= fj * fy; jfz -= fj * fz;
+      jfx -= fx; jfy -= fy; jfz -= fz;
       if (EV) {
         float energy = (ok & lj_on) ? elj : 0.f;
         energy += (ok & c_on) ? ec : 0.f;
         float he = 0.5f * energy, hw = 0.5f * factor * r2;
         ee += he; ew += hw;
-        je += fj * he; jw += fj * hw;
+        je += he; jw += hw;
       }
       pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
-      jtype = hd_rot1_u(jtype); jkey = hd_rot1_u(jkey); jin = hd_rot1_u(jin);
+      jkey = hd_rot1_u(jkey);
+      if (NT > 1) jtype = hd_rot1_u(jtype);
       jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
       if (EV) { je = hd_rot1(je); jw = hd_rot1(jw); }
     }
@@ -140,6 +192,8 @@ __device__ __forceinline__ void pair_entries(
       ai_e += hd_to_fixed48(ee);
       ai_w += hd_to_fixed48(ew);
     }
+    // Newton's third law: the j-side sums of atoms that belong to block b
+    // are dropped (their pairs were evaluated from both sides as i-pairs)
     if (jvalid && !jin) {
       unsigned long long* d = fpj + (size_t)jslot * 4;
       long long vx = hd_to_fixed48(jfx), vy = hd_to_fixed48(jfy), vz = hd_to_fixed48(jfz);
@@ -155,7 +209,7 @@ __device__ __forceinline__ void pair_entries(
   }
 }
 
-template <bool EV>
+template <bool EV, int NT, bool SW>
 __global__ void __launch_bounds__(256) k_pair(
     PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
     const unsigned* __restrict__ tile_type, const unsigned* __restrict__ tile_key,
@@ -169,38 +223,31 @@ __global__ void __launch_bounds__(256) k_pair(
   HdLattice L = hd_load_lattice(lattice);
   unsigned islot = b * 64 + lane;
   bool ivalid = islot < n;
-  float4 pi = tile_posq[islot];
+  float4 pi = tile_posq[ivalid ? islot : 0u];
   unsigned itype = ivalid ? tile_type[islot] : 0u;
   unsigned ikey = ivalid ? tile_key[islot] : (HD_SENTINEL - 1u);
   float4 cb = center[b], hb = half[b];
   bool spc = (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) &&
              (0.5f * L.lx - hb.x >= P.spc_margin) && (0.5f * L.ly - hb.y >= P.spc_margin) &&
              (0.5f * L.lz - hb.z >= P.spc_margin);
-  float pix = pi.x, piy = pi.y, piz = pi.z;
+  float pix = pi.x, piy = pi.y, piz = pi.z, qi = ivalid ? pi.w : 0.f;
   if (!ivalid) { pix = cb.x; piy = cb.y; piz = cb.z; }
-  if (spc) hd_rel_to_center(pix, piy, piz, cb.x, cb.y, cb.z, L, pix, piy, piz);
-  float c6row[PAIR_MAXT], c12row[PAIR_MAXT], cutrow[PAIR_MAXT], rswrow[PAIR_MAXT],
-      idrow[PAIR_MAXT];
-#pragma unroll
-  for (int t = 0; t < PAIR_MAXT; ++t) {
-    int k = (t < P.ntypes) ? (int)itype * P.ntypes + t : 0;
-    c6row[t] = P.c6[k];
-    c12row[t] = P.c12[k];
-    cutrow[t] = ivalid ? P.ljcut2[k] : -1.0f;
-    rswrow[t] = P.rsw2[k];
-    idrow[t] = P.inv_delta[k];
+  if (spc) {
+    hd_rel_to_center(pix, piy, piz, cb.x, cb.y, cb.z, L, pix, piy, piz);
+    if (!ivalid) pix = piy = piz = PAIR_FAR_I;
   }
+  LjRow<NT> R = load_row<NT>(P, itype, ivalid);
   unsigned nent = entry_count[b];
   const unsigned* ents = entry_atoms + (size_t)b * cap * 64;
   long long ax = 0, ay = 0, az = 0, ae = 0, aw = 0;
   if (spc)
-    pair_entries<EV, true>(P, L, b, w, lane, pix, piy, piz, pi.w, ikey, c6row, c12row, cutrow,
-                           rswrow, idrow, cb, tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                           fpj_ew, ax, ay, az, ae, aw);
+    pair_entries<EV, NT, SW, true>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb, tile_posq,
+                                   tile_type, tile_key, ents, nent, n, fpj, fpj_ew, ax, ay, az,
+                                   ae, aw);
   else
-    pair_entries<EV, false>(P, L, b, w, lane, pix, piy, piz, pi.w, ikey, c6row, c12row, cutrow,
-                            rswrow, idrow, cb, tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                            fpj_ew, ax, ay, az, ae, aw);
+    pair_entries<EV, NT, SW, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb, tile_posq,
+                                    tile_type, tile_key, ents, nent, n, fpj, fpj_ew, ax, ay, az,
+                                    ae, aw);
   red[w][0][lane] = ax;
   red[w][1][lane] = ay;
   red[w][2][lane] = az;
@@ -265,11 +312,33 @@ void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long lo
                      a.n, out);
 }
 
+template <bool EV, int NT, bool SW>
+static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams& P) {
+  unsigned nblocks = a.npad / 64;
+  hipLaunchKernelGGL((k_pair<EV, NT, SW>), dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P,
+                     a.lattice, a.tile_posq, a.tile_type, a.tile_key, a.center, a.half,
+                     a.entry_atoms, a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
+                     (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
+}
+
+template <bool EV, int NT>
+static void launch_pair_nt(hipStream_t st, const HdPairArgs& a, const PairParams& P, bool sw) {
+  if (sw) launch_pair_t<EV, NT, true>(st, a, P);
+  else launch_pair_t<EV, NT, false>(st, a, P);
+}
+
+template <bool EV>
+static void launch_pair_ev(hipStream_t st, const HdPairArgs& a, const PairParams& P, bool sw) {
+  if (P.ntypes <= 1) launch_pair_nt<EV, 1>(st, a, P, sw);
+  else if (P.ntypes == 2) launch_pair_nt<EV, 2>(st, a, P, sw);
+  else launch_pair_nt<EV, 4>(st, a, P, sw);
+}
+
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
   PairParams P;
   memset(&P, 0, sizeof(P));
   P.ntypes = a.ntypes;
-  P.any_switch = 0;
+  bool any_switch = false;
   for (int i = 0; i < a.ntypes; ++i)
     for (int j = 0; j < a.ntypes; ++j) {
       int k = i * a.ntypes + j;
@@ -282,22 +351,13 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
       P.rsw2[k] = (float)(rs * rs);
       double d = rc * rc - rs * rs;
       P.inv_delta[k] = d > 0.0 ? (float)(1.0 / d) : 0.0f;
-      if (d > 0.0 && e != 0.0) P.any_switch = 1;
+      if (d > 0.0 && e != 0.0) any_switch = true;
     }
   P.kc = (float)a.kc;
   P.alpha = (float)a.alpha;
   P.coul_rc2 = a.coul_rc > 0.0 ? (float)(a.coul_rc * a.coul_rc) : -1.0f;
   P.rmax2 = (float)(a.rmax * a.rmax);
   P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
-  unsigned nblocks = a.npad / 64;
-  if (ev)
-    hipLaunchKernelGGL(k_pair<true>, dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P, a.lattice,
-                       a.tile_posq, a.tile_type, a.tile_key, a.center, a.half, a.entry_atoms,
-                       a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
-                       (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
-  else
-    hipLaunchKernelGGL(k_pair<false>, dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P, a.lattice,
-                       a.tile_posq, a.tile_type, a.tile_key, a.center, a.half, a.entry_atoms,
-                       a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
-                       (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
+  if (ev) launch_pair_ev<true>(st, a, P, any_switch);
+  else launch_pair_ev<false>(st, a, P, any_switch);
 }

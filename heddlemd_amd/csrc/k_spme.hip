@@ -48,64 +48,199 @@ __device__ __forceinline__ float bspline_w(float x) {
   return v[0];
 }
 
-// Spline setup (spme_spread_gather.cuh:90-126): wrap, fractional + 1/2,
-// u = s*n, primary bin floor(u), fractional offset t.
-__device__ __forceinline__ void spline_setup(float px, float py, float pz, const HdLattice& L,
-                                             unsigned na, unsigned nb, unsigned nc, float& ta,
-                                             float& tb, float& tc, int& ga, int& gb, int& gc) {
-  int ia, ib, ic;
-  hd_wrap_with_image(px, py, pz, ia, ib, ic, L);
-  float sa, sb, sc;
-  hd_cart_to_frac(px, py, pz, L, sa, sb, sc);
-  float ua = (sa + 0.5f) * (float)na, ub = (sb + 0.5f) * (float)nb, uc = (sc + 0.5f) * (float)nc;
-  float fa = floorf(ua), fb = floorf(ub), fc = floorf(uc);
-  ta = ua - fa; tb = ub - fb; tc = uc - fc;
-  int la = (int)fa, lb = (int)fb, lc = (int)fc;
-  ga = ((la % (int)na) + (int)na) % (int)na;
-  gb = ((lb % (int)nb) + (int)nb) % (int)nb;
-  gc = ((lc % (int)nc) + (int)nc) % (int)nc;
-}
+// Spline records, once per step in slot order: the FP64 spline origin
+// (hd_spline_origin) of every charged atom, as (t_a, t_b, t_c, q) plus
+// the packed origin cell g_a | g_b << 10 | g_c << 20 (SPME_NO_CELL for
+// uncharged atoms).  Read by the spread and by the force gather.
+#define SPME_NO_CELL 0xFFFFFFFFu
 
-// Spread: SPME_P threads per atom (slot order, so neighbouring threads
-// touch neighbouring cells); thread iz owns one c-slice and adds its
-// P*P cells with i64 atomics.  Integer addition is associative, so the
-// grid is bit-identical run to run.
-__global__ void k_spread(const float4* __restrict__ tile_posq, const float* __restrict__ lattice,
-                         unsigned na, unsigned nb, unsigned nc,
-                         unsigned long long* __restrict__ rho, unsigned n) {
-  unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= n * SPME_P) return;
-  unsigned s = gid / SPME_P, iz = gid - s * SPME_P;
+__global__ void k_spline_setup(const float4* __restrict__ tile_posq,
+                               const float* __restrict__ lattice, unsigned na, unsigned nb,
+                               unsigned nc, float4* __restrict__ spl, unsigned* __restrict__ sg,
+                               unsigned n) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
   float4 pq = tile_posq[s];
-  float qi = pq.w;
-  if (qi == 0.f) return;
+  if (pq.w == 0.f) {
+    sg[s] = SPME_NO_CELL;
+    return;
+  }
   HdLattice L = hd_load_lattice(lattice);
   float ta, tb, tc;
   int ga, gb, gc;
   hd_spline_origin(pq.x, pq.y, pq.z, L, na, nb, nc, ta, tb, tc, ga, gb, gc);
-  float wa[SPME_P], wb[SPME_P];
+  spl[s] = make_float4(ta, tb, tc, pq.w);
+  sg[s] = (unsigned)ga | ((unsigned)gb << 10) | ((unsigned)gc << 20);
+}
+
+// Spread: the reference accumulates each atom's p^3 contributions with
+// i64 atomics at 2^32, rounded to nearest (spme_spread_gather.cuh:151-221).
+// Here every workgroup owns one brick of grid cells and computes it
+// completely: it gathers the atoms of the neighbour list's voxels that can
+// reach the brick, adds their in-brick contributions into LDS with 64-bit
+// LDS atomics, and stores the brick with plain coalesced stores.  No global
+// atomics (device-scope atomics on MI355X are resolved past the per-XCD L2)
+// and no clearing pass.  The per-cell values are the same integers in any
+// order, so the grid equals per-atom global accumulation bit for bit.
+//
+// Which atoms can reach a brick: cells [c0, c1) receive contributions from
+// atoms whose origin cell lies in [c0, c1 + p - 1), i.e. whose fractional
+// coordinate (+1/2) lies in [c0/n, (c1 + p - 1)/n).  Atoms sit in the voxel
+// they were binned into at the last rebuild, and have moved less than half
+// the skin since (the rebuild criterion), so the window is widened by mv
+// voxels on each side (mv >= skin/2 in voxel units, plus one).  The
+// window's atoms are flattened (per-voxel slot ranges + an LDS scan) so
+// that the threads share them evenly.
+#define SPREAD_THREADS 256
+#define SPREAD_BRICK_MAX 2048  // cells per brick (16 KiB of i64)
+#define SPREAD_WIN_MAX 3072    // voxels per window held in LDS
+
+__device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
+                                            const unsigned* __restrict__ sg, unsigned s,
+                                            unsigned na, unsigned nb, unsigned nc, int c0a,
+                                            int c0b, int c0c, unsigned ba, unsigned bb,
+                                            unsigned bc, unsigned long long* acc) {
+  unsigned g = sg[s];
+  if (g == SPME_NO_CELL) return;
+  int ga = (int)(g & 1023u), gb = (int)((g >> 10) & 1023u), gc = (int)(g >> 20);
+  // brick-relative index of the stencil's first cell per axis; cell
+  // (g - d) mod n lies in the brick when 0 <= rel - d < extent
+  int rla = ((ga - c0a) % (int)na + (int)na) % (int)na;
+  int rlb = ((gb - c0b) % (int)nb + (int)nb) % (int)nb;
+  int rlc = ((gc - c0c) % (int)nc + (int)nc) % (int)nc;
+  if (rla >= (int)ba + SPME_P - 1 || rlb >= (int)bb + SPME_P - 1 ||
+      rlc >= (int)bc + SPME_P - 1)
+    return;
+  float4 r = spl[s];
+  float wa[SPME_P], wb[SPME_P], wc[SPME_P];
 #pragma unroll
   for (int d = 0; d < SPME_P; ++d) {
-    wa[d] = bspline_w((float)d + ta);
-    wb[d] = bspline_w((float)d + tb);
+    wa[d] = bspline_w((float)d + r.x);
+    wb[d] = bspline_w((float)d + r.y);
+    wc[d] = bspline_w((float)d + r.z);
   }
-  float wc = bspline_w((float)iz + tc);
-  unsigned gcz = ((unsigned)gc + nc - iz) % nc;
-  float dz = qi * wc;
 #pragma unroll
-  for (int da = 0; da < SPME_P; ++da) {
-    unsigned gaa = ((unsigned)ga + na - da) % na;
-    float dzda = dz * wa[da];
+  for (int iz = 0; iz < SPME_P; ++iz) {
+    int oc = rlc - iz;
+    if ((unsigned)oc >= bc) continue;
+    float dz = r.w * wc[iz];
 #pragma unroll
-    for (int db = 0; db < SPME_P; ++db) {
-      float v = dzda * wb[db];
-      long long vf = __float2ll_rn(v * 4294967296.0f);
-      if (vf != 0) {
-        unsigned gbb = ((unsigned)gb + nb - db) % nb;
-        atomicAdd(rho + ((size_t)gaa * nb + gbb) * nc + gcz, (unsigned long long)vf);
+    for (int da = 0; da < SPME_P; ++da) {
+      int oa = rla - da;
+      if ((unsigned)oa >= ba) continue;
+      float dzda = dz * wa[da];
+#pragma unroll
+      for (int db = 0; db < SPME_P; ++db) {
+        int ob = rlb - db;
+        if ((unsigned)ob >= bb) continue;
+        float v = dzda * wb[db];
+        long long vf = __float2ll_rn(v * 4294967296.0f);
+        if (vf != 0)
+          atomicAdd(&acc[((unsigned)oa * bb + (unsigned)ob) * bc + (unsigned)oc],
+                    (unsigned long long)vf);
       }
     }
   }
+}
+
+__global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
+    const float4* __restrict__ spl, const unsigned* __restrict__ sg, unsigned na, unsigned nb,
+    unsigned nc, unsigned ba, unsigned bb, unsigned bc, const unsigned* __restrict__ voxel_rank,
+    const unsigned* __restrict__ voxel_off, unsigned nva, unsigned nvb, unsigned nvc, int mva,
+    int mvb, int mvc, unsigned long long* __restrict__ rho) {
+  __shared__ unsigned long long acc[SPREAD_BRICK_MAX];
+  __shared__ unsigned wstart[SPREAD_WIN_MAX];
+  __shared__ unsigned wpre[SPREAD_WIN_MAX];
+  __shared__ unsigned part[SPREAD_THREADS];
+  unsigned t = threadIdx.x;
+  unsigned nbb = nb / bb, nbc = nc / bc;
+  unsigned bi = blockIdx.x;
+  unsigned bic = bi % nbc, bib = (bi / nbc) % nbb, bia = bi / (nbc * nbb);
+  int c0a = (int)(bia * ba), c0b = (int)(bib * bb), c0c = (int)(bic * bc);
+  unsigned vol = ba * bb * bc;
+  for (unsigned k = t; k < vol; k += SPREAD_THREADS) acc[k] = 0ull;
+  // voxel window per axis (periodic), clamped to one full period
+  int vla = (int)(((long long)c0a * nva) / na) - mva;
+  int vha = (int)((((long long)c0a + ba + SPME_P - 1) * nva - 1) / na) + mva;
+  int vlb = (int)(((long long)c0b * nvb) / nb) - mvb;
+  int vhb = (int)((((long long)c0b + bb + SPME_P - 1) * nvb - 1) / nb) + mvb;
+  int vlc = (int)(((long long)c0c * nvc) / nc) - mvc;
+  int vhc = (int)((((long long)c0c + bc + SPME_P - 1) * nvc - 1) / nc) + mvc;
+  unsigned wa_n = (unsigned)min(vha - vla + 1, (int)nva);
+  unsigned wb_n = (unsigned)min(vhb - vlb + 1, (int)nvb);
+  unsigned wc_n = (unsigned)min(vhc - vlc + 1, (int)nvc);
+  unsigned nwin = wa_n * wb_n * wc_n;
+  bool flat = nwin <= SPREAD_WIN_MAX;  // uniform over the workgroup
+  __syncthreads();  // acc cleared before any accumulation
+  unsigned per = (nwin + SPREAD_THREADS - 1) / SPREAD_THREADS;
+  unsigned my = 0;
+  for (unsigned j = 0; j < per; ++j) {
+    // thread t owns the contiguous window range [t*per, t*per + per)
+    unsigned k = t * per + j;
+    if (k >= nwin) break;
+    unsigned kc = k % wc_n, kab = k / wc_n;
+    unsigned kb = kab % wb_n, ka = kab / wb_n;
+    unsigned va = (unsigned)(((vla + (int)ka) % (int)nva + (int)nva) % (int)nva);
+    unsigned vb = (unsigned)(((vlb + (int)kb) % (int)nvb + (int)nvb) % (int)nvb);
+    unsigned vc = (unsigned)(((vlc + (int)kc) % (int)nvc + (int)nvc) % (int)nvc);
+    unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
+    unsigned s0 = voxel_off[r], s1 = voxel_off[r + 1];
+    if (flat) {
+      wstart[k] = s0;
+      wpre[k] = s1 - s0;  // count for now, prefix after the scan
+    } else {
+      for (unsigned s = s0; s < s1; ++s)
+        spread_atom(spl, sg, s, na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
+    }
+    my += s1 - s0;
+  }
+  __syncthreads();
+  if (flat) {
+    // exclusive scan of the per-thread totals, then of each thread's range
+    part[t] = my;
+    __syncthreads();
+    for (unsigned off = 1; off < SPREAD_THREADS; off <<= 1) {
+      unsigned u = (t >= off) ? part[t - off] : 0u;
+      __syncthreads();
+      part[t] += u;
+      __syncthreads();
+    }
+    unsigned total = part[SPREAD_THREADS - 1];
+    unsigned run = part[t] - my;
+    for (unsigned j = 0; j < per; ++j) {
+      unsigned k = t * per + j;
+      if (k >= nwin) break;
+      unsigned c = wpre[k];
+      wpre[k] = run;
+      run += c;
+    }
+    __syncthreads();
+    for (unsigned f = t; f < total; f += SPREAD_THREADS) {
+      // last window entry whose prefix is <= f
+      unsigned lo = 0, hi = nwin;
+      while (hi - lo > 1) {
+        unsigned mid = (lo + hi) >> 1;
+        if (wpre[mid] <= f) lo = mid; else hi = mid;
+      }
+      spread_atom(spl, sg, wstart[lo] + (f - wpre[lo]), na, nb, nc, c0a, c0b, c0c, ba, bb, bc,
+                  acc);
+    }
+    __syncthreads();
+  }
+  for (unsigned k = t; k < vol; k += SPREAD_THREADS) {
+    unsigned kc = k % bc, kab = k / bc;
+    unsigned kb = kab % bb, ka = kab / bb;
+    rho[((size_t)(c0a + ka) * nb + (c0b + kb)) * nc + (c0c + kc)] = acc[k];
+  }
+}
+
+unsigned hd_spread_bricks(unsigned na, unsigned nb, unsigned nc, unsigned brick[3]) {
+  // up to 8 x 16 x 16 cells; smaller grids keep at least 8 per axis or the
+  // whole axis
+  brick[0] = na < 8 ? na : 8;
+  brick[1] = nb < 16 ? nb : 16;
+  brick[2] = nc < 16 ? nc : 16;
+  return (na / brick[0]) * (nb / brick[1]) * (nc / brick[2]);
 }
 
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, size_t count) {
@@ -391,12 +526,16 @@ static unsigned ilog2u(unsigned v) {
 }
 
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
-  size_t M = (size_t)a.na * a.nb * a.nc;
   unsigned nch = a.nc / 2 + 1;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1024), dim3(256), 0, st, (unsigned long long*)a.rho_fixed, M);
+  unsigned brick[3];
+  unsigned nbricks = hd_spread_bricks(a.na, a.nb, a.nc, brick);
   if (a.n > 0)
-    hipLaunchKernelGGL(k_spread, dim3((a.n * SPME_P + 255) / 256), dim3(256), 0, st, a.tile_posq,
-                       a.lattice, a.na, a.nb, a.nc, (unsigned long long*)a.rho_fixed, a.n);
+    hipLaunchKernelGGL(k_spline_setup, dim3((a.n + 255) / 256), dim3(256), 0, st, a.tile_posq,
+                       a.lattice, a.na, a.nb, a.nc, a.spl, a.sg, a.n);
+  hipLaunchKernelGGL(k_spread, dim3(nbricks), dim3(SPREAD_THREADS), 0, st, a.spl, a.sg, a.na,
+                     a.nb, a.nc, brick[0], brick[1], brick[2], a.voxel_rank, a.voxel_off,
+                     a.nv[0], a.nv[1], a.nv[2], a.mv[0], a.mv[1], a.mv[2],
+                     (unsigned long long*)a.rho_fixed);
   unsigned nrows = a.na * a.nb;
   unsigned h = a.nc / 2;
   unsigned rows = FFT_TILE / h;
