@@ -46,6 +46,8 @@ SIGNATURES = {
                              C.c_uint64]),
     "hd_set_crescale": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_uint64]),
+    "hd_set_langevin": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64]),
+    "hd_get_langevin_counter": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
     "hd_setup": (C.c_int, [_ctx]),
     "hd_force_eval": (C.c_int, [_ctx, C.c_int]),
     "hd_constrain_velocities": (C.c_int, [_ctx]),

@@ -112,6 +112,19 @@ struct HdIntegrateArgs {
 };
 void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a);
 
+// Langevin BAOAB half-steps (kernels/langevin.cu:13-80).  Uses the fields
+// n, dt, posq, ix..iz, vx..vz, fx..fz, mass, lattice, snap of `a`.
+struct HdLangevinArgs {
+  float alpha, kt;                    // exp(-gamma dt) and k_B T, f32
+  unsigned seed_lo, seed_hi;          // Philox key
+  unsigned long long* counter;        // device draw counter
+};
+// B + A: kick with the current forces, then x += v dt/2 (advances the
+// draw counter by one for the O step that follows)
+void hd_launch_lan_kick_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
+// O + A: Ornstein-Uhlenbeck velocity update, then x += v dt/2
+void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
+
 struct HdSettleArgs {
   unsigned n_groups;
   const unsigned* groups;  // 3 per group: O, H1, H2

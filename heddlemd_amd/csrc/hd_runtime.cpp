@@ -75,6 +75,9 @@ struct hd_ctx {
   bool have_baro = false;
   double baro_p = 0, baro_tau = 0, baro_kappa = 0, baro_kt = 0;
   uint64_t baro_seed = 0;
+  bool have_lan = false;  // Langevin BAOAB instead of velocity Verlet
+  double lan_friction = 0, lan_kt = 0;
+  uint64_t lan_seed = 0;
   bool use_graph = true;
   bool timing = false;
   bool ready = false;
@@ -100,6 +103,7 @@ struct hd_ctx {
   float* baro_mu = nullptr;
   double* baro_diag = nullptr;
   unsigned long long* baro_counter = nullptr;
+  unsigned long long* lan_counter = nullptr;
   // device: neighbour list
   unsigned *voxel_rank = nullptr, *atom_key = nullptr, *voxel_count = nullptr,
            *voxel_off = nullptr, *voxel_cursor = nullptr, *voxel_part = nullptr;
@@ -472,6 +476,119 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   }
 }
 
+HdIntegrateArgs integrate_args(hd_ctx* c) {
+  HdIntegrateArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = (unsigned)c->n;
+  ia.dt = (float)c->dt;
+  ia.posq = c->posq;
+  ia.ix = c->ix;
+  ia.iy = c->iy;
+  ia.iz = c->iz;
+  ia.vx = c->vx;
+  ia.vy = c->vy;
+  ia.vz = c->vz;
+  ia.fx = c->fx;
+  ia.fy = c->fy;
+  ia.fz = c->fz;
+  ia.mass = c->mass;
+  ia.lattice = c->d_lattice;
+  ia.snap = c->h_groups.empty() ? nullptr : c->snap;
+  return ia;
+}
+
+HdSettleArgs settle_args(hd_ctx* c) {
+  HdSettleArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.n_groups = (unsigned)(c->h_groups.size() / 3);
+  sa.groups = c->d_groups;
+  double total = c->m_o + 2.0 * c->m_h;
+  double h = sqrt(c->d_oh * c->d_oh - 0.25 * c->d_hh * c->d_hh);
+  sa.ra = (float)((2.0 * c->m_h / total) * h);
+  sa.rb = (float)((c->m_o / total) * h);
+  sa.rc = (float)(c->d_hh * 0.5);
+  sa.m_o = (float)c->m_o;
+  sa.m_h = (float)c->m_h;
+  sa.dt = (float)c->dt;
+  sa.posq = c->posq;
+  sa.snap = c->snap;
+  sa.vx = c->vx;
+  sa.vy = c->vy;
+  sa.vz = c->vz;
+  sa.w_cons = c->w_cons;
+  sa.lattice = c->d_lattice;
+  return sa;
+}
+
+void enqueue_crescale(hd_ctx* c, bool timed) {
+  // C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): K of the
+  // constrained, not yet kicked velocities (scal[3]) and W of this
+  // evaluation (scal[2]); positions are scaled after the kick
+  HdBarostatArgs b;
+  memset(&b, 0, sizeof(b));
+  b.seed_lo = (unsigned)(c->baro_seed & 0xFFFFFFFFull);
+  b.seed_hi = (unsigned)(c->baro_seed >> 32);
+  b.pressure = c->baro_p;
+  b.tau = c->baro_tau;
+  b.compressibility = c->baro_kappa;
+  b.kt = c->baro_kt;
+  b.dt = c->dt;
+  b.mu3_min = 1e-18;
+  stage_begin(c, "c_rescale", timed);
+  hd_launch_crescale(c->stream, c->scal + 3, c->scal + 2, c->d_lattice, c->baro_mu, c->baro_diag,
+                     c->baro_counter, b, c->posq, (unsigned)c->n);
+  stage_end(c, timed);
+}
+
+// Langevin BAOAB (src/integrator/langevin_baoab.rs plan B, A, O, A, F, B;
+// the last B is the fused post-force kick).  With SETTLE the constraint
+// hooks of run_step wrap each drift (snapshot before, positions after,
+// with the full dt as the reference passes it) and the velocity
+// constraint follows the force evaluation, before the fused kick
+// (src/integrator/mod.rs:344-429); it commutes with the force
+// evaluation, so it is issued before it.
+void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
+  hipStream_t st = c->stream;
+  unsigned n = (unsigned)c->n;
+  bool settle = !c->h_groups.empty();
+  HdIntegrateArgs ia = integrate_args(c);
+  HdLangevinArgs la;
+  la.alpha = expf(-(float)c->lan_friction * (float)c->dt);
+  la.kt = (float)c->lan_kt;
+  la.seed_lo = (unsigned)(c->lan_seed & 0xFFFFFFFFull);
+  la.seed_hi = (unsigned)(c->lan_seed >> 32);
+  la.counter = c->lan_counter;
+  HdSettleArgs sa = settle_args(c);
+  stage_begin(c, "langevin_kick_drift", timed);
+  hd_launch_lan_kick_drift(st, ia, la);
+  stage_end(c, timed);
+  if (settle) {
+    stage_begin(c, "settle_positions", timed);
+    hd_launch_settle_positions(st, sa);
+    stage_end(c, timed);
+  }
+  stage_begin(c, "langevin_ou_drift", timed);
+  hd_launch_lan_ou_drift(st, ia, la);
+  stage_end(c, timed);
+  if (settle) {
+    stage_begin(c, "settle_positions", timed);
+    hd_launch_settle_positions(st, sa);
+    stage_end(c, timed);
+    stage_begin(c, "settle_velocities", timed);
+    hd_launch_settle_velocities(st, sa, true);
+    stage_end(c, timed);
+  }
+  if (c->have_baro) {
+    hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
+                 0.f, nullptr);
+    ev = true;
+  }
+  enqueue_forces(c, ev, true, timed);
+  if (c->have_baro) enqueue_crescale(c, timed);
+  if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
+                       nullptr, 0.f, nullptr);
+}
+
 void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   // ev may be forced on below (a barostat consumes the virial every step)
   hipStream_t st = c->stream;
@@ -484,6 +601,10 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
     stage_end(c, timed);
   }
   bool settle = !c->h_groups.empty();
+  if (c->have_lan) {
+    enqueue_langevin_step(c, ev, timed);
+    return;
+  }
   HdIntegrateArgs ia;
   memset(&ia, 0, sizeof(ia));
   ia.n = n;
@@ -547,25 +668,7 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   }
   if (c->have_baro) ev = true;
   enqueue_forces(c, ev, true, timed);
-  if (c->have_baro) {
-    // C-rescale after the thermostat (src/runner.rs:2664-2668): K of the
-    // constrained, not yet kicked velocities and W of this evaluation;
-    // positions are scaled after the kick (post-force fragment order).
-    HdBarostatArgs b;
-    memset(&b, 0, sizeof(b));
-    b.seed_lo = (unsigned)(c->baro_seed & 0xFFFFFFFFull);
-    b.seed_hi = (unsigned)(c->baro_seed >> 32);
-    b.pressure = c->baro_p;
-    b.tau = c->baro_tau;
-    b.compressibility = c->baro_kappa;
-    b.kt = c->baro_kt;
-    b.dt = c->dt;
-    b.mu3_min = 1e-18;
-    stage_begin(c, "c_rescale", timed);
-    hd_launch_crescale(st, c->scal + 3, c->scal + 2, c->d_lattice, c->baro_mu, c->baro_diag,
-                       c->baro_counter, b, c->posq, n);
-    stage_end(c, timed);
-  }
+  if (c->have_baro) enqueue_crescale(c, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
@@ -781,6 +884,7 @@ int hd_set_integrator(hd_ctx* c, double dt, int lossless) {
     drop_graphs(c);
     return HD_OK;
   }
+  if (lossless && c->have_lan) return fail(c, HD_ERR_UNSUPPORTED, "Langevin BAOAB has no lossless mode");
   c->dt = dt;
   c->lossless = lossless != 0;
   return HD_OK;
@@ -807,6 +911,8 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
     c->have_nhc = false;
     return HD_OK;
   }
+  if (c->have_lan)
+    return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: no Nose-Hoover chain");
   if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
       !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
     return fail(c, HD_ERR_ARG, "invalid Nose-Hoover chain parameters");
@@ -829,6 +935,35 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
   c->nhc_yoshida = yoshida;
   c->nhc_resp = n_resp;
   c->nhc_nc = n_constraints;
+  return HD_OK;
+}
+
+int hd_set_langevin(hd_ctx* c, double friction, double kt, uint64_t seed) {
+  if (!c) return HD_ERR_ARG;
+  if (!(friction > 0) || !(kt > 0) || !std::isfinite(friction) || !std::isfinite(kt))
+    return fail(c, HD_ERR_ARG, "Langevin friction and temperature must be finite and > 0");
+  if (c->have_nhc)
+    return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: remove the Nose-Hoover chain");
+  if (c->lossless) return fail(c, HD_ERR_UNSUPPORTED, "Langevin BAOAB has no lossless mode");
+  if (c->ready) {
+    if (!c->have_lan)
+      return fail(c, HD_ERR_STATE, "the integrator kind is fixed at hd_setup");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  c->have_lan = true;
+  c->lan_friction = friction;
+  c->lan_kt = kt;
+  c->lan_seed = seed;
+  return HD_OK;
+}
+
+int hd_get_langevin_counter(hd_ctx* c, uint64_t* counter) {
+  if (!c || !c->ready || !counter) return fail(c, HD_ERR_STATE, "not set up");
+  unsigned long long v = 0;
+  HIPCHK(c, hipMemcpyAsync(&v, c->lan_counter, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *counter = v;
   return HD_OK;
 }
 
@@ -949,6 +1084,7 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->baro_mu, 1);
   DALLOC(c, c->baro_diag, 4);
   DALLOC(c, c->baro_counter, 1);
+  DALLOC(c, c->lan_counter, 1);
   DALLOC(c, c->voxel_rank, c->nvox); DALLOC(c, c->atom_key, n);
   DALLOC(c, c->voxel_count, c->nvox); DALLOC(c, c->voxel_off, c->nvox + 1);
   DALLOC(c, c->voxel_cursor, c->nvox);

@@ -614,6 +614,76 @@ __device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, 
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+// ---------------------------------------------------------------------
+// Langevin BAOAB (src/integrator/langevin_baoab.rs: B, A, O, A, F, B with
+// the last B fused into the force finalize).  B + A and O + A are fused
+// pairs of the reference's vv_kick / lan_drift_half / lan_ou_step kernels
+// (integrate.cu, langevin.cu:13-80) with the same f32 operations in the
+// same order.  The draw counter lives on the device: the B + A kernel
+// advances it and the O + A kernel of the same step uses counter - 1, so a
+// captured step graph replays with fresh draws.
+// ---------------------------------------------------------------------
+template <bool SNAP>
+__global__ void k_lan_kick_drift(HdIntegrateArgs a, HdLangevinArgs l) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) l.counter[0] += 1ull;
+  if (i >= a.n) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  float m = a.mass[i];
+  float ax = a.fx[i] / m, ay = a.fy[i] / m, az = a.fz[i] / m;
+  float half_dt = a.dt * 0.5f;
+  float vx = a.vx[i] + ax * half_dt;
+  float vy = a.vy[i] + ay * half_dt;
+  float vz = a.vz[i] + az * half_dt;
+  float4 pq = a.posq[i];
+  if (SNAP) a.snap[i] = pq;
+  float px = pq.x + vx * half_dt, py = pq.y + vy * half_dt, pz = pq.z + vz * half_dt;
+  int ka, kb, kc;
+  hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+  a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+  a.posq[i] = make_float4(px, py, pz, pq.w);
+  a.ix[i] += ka; a.iy[i] += kb; a.iz[i] += kc;
+}
+
+template <bool SNAP>
+__global__ void k_lan_ou_drift(HdIntegrateArgs a, HdLangevinArgs l) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned long long ctr = l.counter[0] - 1ull;
+  unsigned clo = (unsigned)(ctr & 0xFFFFFFFFull), chi = (unsigned)(ctr >> 32);
+  float m = a.mass[i];
+  float sigma = sqrtf((1.0f - l.alpha * l.alpha) * l.kt / m);
+  float xi0 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 0u);
+  float xi1 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 1u);
+  float xi2 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 2u);
+  float vx = l.alpha * a.vx[i] + sigma * xi0;
+  float vy = l.alpha * a.vy[i] + sigma * xi1;
+  float vz = l.alpha * a.vz[i] + sigma * xi2;
+  float half_dt = a.dt * 0.5f;
+  float4 pq = a.posq[i];
+  if (SNAP) a.snap[i] = pq;
+  float px = pq.x + vx * half_dt, py = pq.y + vy * half_dt, pz = pq.z + vz * half_dt;
+  int ka, kb, kc;
+  hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+  a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+  a.posq[i] = make_float4(px, py, pz, pq.w);
+  a.ix[i] += ka; a.iy[i] += kb; a.iz[i] += kc;
+}
+
+void hd_launch_lan_kick_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l) {
+  dim3 g(a.n ? (a.n + 255) / 256 : 1), b(256);
+  if (a.snap) hipLaunchKernelGGL((k_lan_kick_drift<true>), g, b, 0, st, a, l);
+  else hipLaunchKernelGGL((k_lan_kick_drift<false>), g, b, 0, st, a, l);
+}
+
+void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l) {
+  if (a.n == 0) return;
+  dim3 g((a.n + 255) / 256), b(256);
+  if (a.snap) hipLaunchKernelGGL((k_lan_ou_drift<true>), g, b, 0, st, a, l);
+  else hipLaunchKernelGGL((k_lan_ou_drift<false>), g, b, 0, st, a, l);
+}
+
 // One thread: P = (2K + W)/(3V), mu^3 = 1 - kappa (dt/tau)(P0 - P)
 // + sqrt(2 kappa kT dt/(tau V)) r, clamped, lattice *= (f32) mu; the
 // draw counter advances by one.  K and W are read from FP64 scalars.

@@ -101,6 +101,16 @@ class Engine:
                                             float(compressibility), float(kt), int(seed)),
                   "hd_set_crescale")
 
+    def set_langevin(self, friction, kt, seed):
+        self._chk(self._lib.hd_set_langevin(self._ctx, float(friction), float(kt), int(seed)),
+                  "hd_set_langevin")
+
+    def langevin_counter(self):
+        n = C.c_uint64(0)
+        self._chk(self._lib.hd_get_langevin_counter(self._ctx, C.byref(n)),
+                  "hd_get_langevin_counter")
+        return int(n.value)
+
     def setup(self):
         self._chk(self._lib.hd_setup(self._ctx), "hd_setup")
 
@@ -215,8 +225,9 @@ class Engine:
 
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
-               velocities=None, crescale=None, device=0):
-    """Configure an Engine for a ``systems.System``."""
+               velocities=None, crescale=None, langevin=None, device=0):
+    """Configure an Engine for a ``systems.System``; ``langevin`` =
+    dict(friction, kt, seed) selects the BAOAB integrator."""
     e = Engine(device)
     e.set_system(system.posq, system.mass, system.types, system.excl_key)
     e.set_lattice(system.lattice)
@@ -232,6 +243,8 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_settle(system.water_groups, d_oh, d_hh, m_o, m_h)
     if dt is not None:
         e.set_integrator(dt, lossless)
+    if langevin is not None:
+        e.set_langevin(**langevin)
     if nhc_kt is not None:
         e.set_nhc(nhc_kt, nhc_tau, n_constraints=system.n_constraints)
     if crescale is not None:

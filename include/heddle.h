@@ -81,6 +81,16 @@ int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);
 int hd_set_nhc(hd_ctx *ctx, double kt, double tau, int chain, int yoshida, int n_resp,
                uint64_t n_constraints);
 
+/* Langevin BAOAB integrator (src/integrator/langevin_baoab.rs,
+ * kernels/langevin.cu): friction gamma (1/time), k_B*T and the Philox seed
+ * of the Ornstein-Uhlenbeck draws.  Replaces velocity Verlet for the whole
+ * context and owns the thermostat (no Nose-Hoover chain alongside); the
+ * draw counter (one per step) lives on the device.  After hd_setup it can
+ * only be re-parameterised. */
+int hd_set_langevin(hd_ctx *ctx, double friction, double kt, uint64_t seed);
+/* Number of Ornstein-Uhlenbeck draws made so far (steps taken). */
+int hd_get_langevin_counter(hd_ctx *ctx, uint64_t *counter);
+
 /* C-rescale barostat (src/integrator/c_rescale_barostat.rs,
  * kernels/barostat.cu:144-232): target pressure, tau, isothermal
  * compressibility (atomic units), k_B*T of the noise term and the Philox

@@ -690,6 +690,105 @@ void or_vv_kick_f32(int n, float *vx, float *vy, float *vz, const float *fx,
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Philox-4x32-10 and the Langevin BAOAB sub-steps                     */
+/* (kernels/philox.cuh:20-172, kernels/langevin.cu:13-80,              */
+/*  src/integrator/langevin_baoab.rs plan: B, A, O, A, F, B)            */
+/* ------------------------------------------------------------------ */
+
+void or_philox4x32_10(uint32_t k0, uint32_t k1, const uint32_t *ctr, uint32_t *out)
+{
+    /* Salmon et al. SC11: ten rounds, multipliers 0xD2511F53/0xCD9E8D57,
+     * Weyl key increments 0x9E3779B9/0xBB67AE85 (philox.cuh:20-60) */
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)c0 * 0xD2511F53u;
+        uint64_t p2 = (uint64_t)c2 * 0xCD9E8D57u;
+        uint32_t n0 = (uint32_t)(p2 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p2;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double or_philox_gaussian(uint32_t seed_lo, uint32_t seed_hi, uint32_t ctr_lo,
+                          uint32_t ctr_hi, uint32_t pid, uint32_t axis)
+{
+    /* philox.cuh philox_gaussian: Box-Muller on the first two lanes, each
+     * mapped to the open interval by (x + 0.5) * 2^-32, in double */
+    uint32_t ctr[4] = {ctr_lo, ctr_hi, pid, axis}, o[4];
+    or_philox4x32_10(seed_lo, seed_hi, ctr, o);
+    double u1 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+    double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+    double r = sqrt(-2.0 * log(u1));
+    return r * cos(6.283185307179586 * u2);
+}
+
+float or_lan_alpha(float friction, float dt)
+{
+    /* alpha = exp(-gamma dt), f32 on the host (langevin_baoab.rs, "O") */
+    return expf(-friction * dt);
+}
+
+void or_lan_kick_drift_half_f32(int n, float *posq, int *img, float *vx, float *vy,
+                                float *vz, const float *fx, const float *fy,
+                                const float *fz, const float *mass, const float *L,
+                                float dt)
+{
+    /* B: vv_kick (integrate.cu) with the current forces, then A:
+     * lan_drift_half (langevin.cu:13-50) x += v dt/2 and wrap */
+    for (int i = 0; i < n; ++i) {
+        float m = mass[i];
+        float ax = fx[i] / m, ay = fy[i] / m, az = fz[i] / m;
+        float half_dt = dt * 0.5f;
+        float v0 = vx[i] + ax * half_dt;
+        float v1 = vy[i] + ay * half_dt;
+        float v2 = vz[i] + az * half_dt;
+        vx[i] = v0; vy[i] = v1; vz[i] = v2;
+        float px = posq[4 * i] + v0 * half_dt;
+        float py = posq[4 * i + 1] + v1 * half_dt;
+        float pz = posq[4 * i + 2] + v2 * half_dt;
+        int ka, kb, kc;
+        wrap_with_image_f(&px, &py, &pz, &ka, &kb, &kc, L);
+        posq[4 * i] = px; posq[4 * i + 1] = py; posq[4 * i + 2] = pz;
+        img[3 * i] += ka; img[3 * i + 1] += kb; img[3 * i + 2] += kc;
+    }
+}
+
+void or_lan_ou_drift_half_f32(int n, float *posq, int *img, float *vx, float *vy,
+                              float *vz, const float *mass, const float *L, float dt,
+                              float alpha, float kt, uint64_t seed, uint64_t counter)
+{
+    /* O: lan_ou_step (langevin.cu:54-80) v = alpha v + sigma xi with
+     * sigma = sqrt((1 - alpha^2) kT / m) in f32 and xi the double
+     * Box-Muller draw rounded to f32; then A: x += v dt/2 and wrap */
+    uint32_t slo = (uint32_t)(seed & 0xFFFFFFFFu), shi = (uint32_t)(seed >> 32);
+    uint32_t clo = (uint32_t)(counter & 0xFFFFFFFFu), chi = (uint32_t)(counter >> 32);
+    float half_dt = dt * 0.5f;
+    for (int i = 0; i < n; ++i) {
+        float m = mass[i];
+        float sigma = sqrtf((1.0f - alpha * alpha) * kt / m);
+        float xi0 = (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 0u);
+        float xi1 = (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 1u);
+        float xi2 = (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 2u);
+        float v0 = alpha * vx[i] + sigma * xi0;
+        float v1 = alpha * vy[i] + sigma * xi1;
+        float v2 = alpha * vz[i] + sigma * xi2;
+        vx[i] = v0; vy[i] = v1; vz[i] = v2;
+        float px = posq[4 * i] + v0 * half_dt;
+        float py = posq[4 * i + 1] + v1 * half_dt;
+        float pz = posq[4 * i + 2] + v2 * half_dt;
+        int ka, kb, kc;
+        wrap_with_image_f(&px, &py, &pz, &ka, &kb, &kc, L);
+        posq[4 * i] = px; posq[4 * i + 1] = py; posq[4 * i + 2] = pz;
+        img[3 * i] += ka; img[3 * i + 1] += kb; img[3 * i + 2] += kc;
+    }
+}
+
 static inline void comp_add(float *hi, double *lo, double delta)
 {
     double ext = (double)*hi + *lo + delta;

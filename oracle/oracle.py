@@ -56,6 +56,16 @@ def lib():
         L.or_spme_recip.argtypes = [C.c_int, _f32p, _f64p, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]
+        L.or_philox4x32_10.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p]
+        L.or_lan_alpha.restype = C.c_float
+        L.or_lan_alpha.argtypes = [C.c_float, C.c_float]
+        L.or_philox_gaussian.restype = C.c_double
+        L.or_philox_gaussian.argtypes = [C.c_uint32] * 6
+        L.or_lan_kick_drift_half_f32.argtypes = [C.c_int, _f32p, _i32p, _f32p, _f32p, _f32p,
+                                                 _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.or_lan_ou_drift_half_f32.argtypes = [C.c_int, _f32p, _i32p, _f32p, _f32p, _f32p,
+                                               _f32p, _f32p, C.c_float, C.c_float, C.c_float,
+                                               C.c_uint64, C.c_uint64]
         L.or_vv_kick_drift_f32.argtypes = [C.c_int, _f32p, _i32p, _f32p, _f32p, _f32p,
                                            _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
         L.or_vv_kick_f32.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
@@ -207,6 +217,42 @@ def vv_kick_drift_f32(posq, img, v, f, mass, lattice, dt):
 def vv_kick_f32(v, f, mass, dt):
     n = v.shape[1]
     lib().or_vv_kick_f32(n, v[0], v[1], v[2], f[0], f[1], f[2], mass, np.float32(dt))
+
+
+def philox4x32_10(key, ctr):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(int(key[0]), int(key[1]), np.asarray(ctr, np.uint32), out)
+    return [int(x) for x in out]
+
+
+def philox_gaussian(seed, counter, pid, axis):
+    """Standard normal draw of (seed, draw counter, particle, axis)
+    (philox.cuh philox_gaussian), in double."""
+    return lib().or_philox_gaussian(seed & 0xFFFFFFFF, seed >> 32, counter & 0xFFFFFFFF,
+                                    counter >> 32, pid, axis)
+
+
+def lan_kick_drift_half(posq, img, v, f, mass, lattice, dt):
+    """Langevin B + A (integrate.cu vv_kick, langevin.cu lan_drift_half), f32."""
+    n = posq.shape[0]
+    lib().or_lan_kick_drift_half_f32(n, posq.reshape(-1), img.reshape(-1), v[0], v[1], v[2],
+                                     f[0], f[1], f[2], mass,
+                                     np.ascontiguousarray(lattice, np.float32), np.float32(dt))
+
+
+def lan_ou_drift_half(posq, img, v, mass, lattice, dt, alpha, kt, seed, counter):
+    """Langevin O + A (langevin.cu lan_ou_step, lan_drift_half), f32."""
+    n = posq.shape[0]
+    lib().or_lan_ou_drift_half_f32(n, posq.reshape(-1), img.reshape(-1), v[0], v[1], v[2],
+                                   mass, np.ascontiguousarray(lattice, np.float32),
+                                   np.float32(dt), np.float32(alpha), np.float32(kt), seed,
+                                   counter)
+
+
+def lan_alpha(friction, dt):
+    """exp(-gamma dt) in f32, as the reference computes it on the host
+    (langevin_baoab.rs execute, Custom "O")."""
+    return np.float32(lib().or_lan_alpha(np.float32(friction), np.float32(dt)))
 
 
 def vv_kick_drift_lossless(posq, img, v, xlo, vlo, f, mass, lattice, dt):

@@ -261,3 +261,74 @@ def test_crescale_move_matches_formula():
     assert b["mu"] == pytest.approx(float(np.float32(mu3 ** (1 / 3))), rel=1e-6)
     assert L1[0] == pytest.approx(float(np.float32(L0[0]) * np.float32(b["mu"])), rel=1e-7)
     assert b["draws"] == 1
+
+
+def test_langevin_baoab_step_matches_oracle(oracle):
+    # one BAOAB step of jittered argon: B + A with the initial forces, O + A
+    # with draw counter 0, then the fused B with the new forces; f32
+    # operations in the reference order on both sides (the Box-Muller
+    # log/cos are FP64 library calls on each side, so a last-bit difference
+    # of a draw is tolerated)
+    say("case: argon 10000 Langevin BAOAB one step")
+    s = _jittered_argon(seed=9)
+    rng = np.random.default_rng(10)
+    v0 = rng.normal(0, 2e-4, size=(3, s.n)).astype(np.float32)
+    dt = np.float32(U.time(5e-15))
+    kt = U.temperature(120.0)
+    gamma = 1.0 / U.time(1e-12)
+    e = engine_for(s, dt=float(dt), velocities=v0, langevin=dict(friction=gamma, kt=kt, seed=77))
+    e.use_graph(False)
+    e.force_eval(scalars=False)
+    f0 = e.forces().T.copy()
+    posq0, img0 = e.positions(images=True)
+    e.step(1)
+    posq1, img1 = e.positions(images=True)
+    v1 = e.velocities()
+    f1 = e.forces().T.copy()
+    assert e.langevin_counter() == 1
+    e.step(3)
+    assert e.langevin_counter() == 4
+    e.close()
+    p = posq0.copy()
+    img = img0.T.copy()
+    v = v0.copy()
+    L32 = s.lattice.astype(np.float32)
+    alpha = oracle.lan_alpha(gamma, dt)
+    oracle.lan_kick_drift_half(p, img, v, f0, s.mass, L32, dt)
+    oracle.lan_ou_drift_half(p, img, v, s.mass, L32, dt, alpha, np.float32(kt), 77, 0)
+    oracle.vv_kick_f32(v, f1, s.mass, dt)
+    assert np.array_equal(img.T, img1)
+    dp = np.abs(p[:, :3] - posq1[:, :3]).max()
+    dv = np.abs(v - v1).max() / np.abs(v).max()
+    say(f"  max |dx| {dp:.3e} Bohr, max |dv|/max|v| {dv:.3e}")
+    assert dp <= 1e-5
+    assert dv <= 1e-6
+
+
+def test_langevin_water_settle_thermalises():
+    # rigid SPC water under BAOAB + SETTLE: geometry stays rigid and the
+    # kinetic temperature approaches the bath from a cold start
+    say("case: water 8192 Langevin + SETTLE, 400 steps")
+    s = systems.config_water_8192()
+    kt = U.temperature(300.0)
+    e = engine_for(s, dt=U.time(1e-15),
+                   langevin=dict(friction=1.0 / U.time(5e-14), kt=kt, seed=3))
+    e.constrain_velocities()
+    e.force_eval(scalars=False)
+    e.step(400, scalars_last=True)
+    sc = e.scalars()
+    p, _ = e.positions(images=True)
+    e.close()
+    ndof = 3 * s.n - s.n_constraints - 3
+    T = 2 * sc["ke"] / ndof * U.TEMPERATURE_AU_TO_K
+    say(f"  T = {T:.1f} K")
+    # the lattice start releases ~0.6 Eh per molecule over a few ps; without
+    # the bath the same run passes 1000 K
+    assert 150.0 < T < 600.0
+    d_oh, d_hh = s.water_geom
+    L = s.lattice[0]
+    g = s.water_groups
+    for a, b, d in ((0, 1, d_oh), (0, 2, d_oh), (1, 2, d_hh)):
+        r = p[g[:, a], :3].astype(np.float64) - p[g[:, b], :3]
+        r -= L * np.floor(r / L + 0.5)
+        assert np.abs(np.linalg.norm(r, axis=1) / d - 1).max() < 2e-5

@@ -146,3 +146,82 @@ def test_nhc_chain_conserves_extended_energy(oracle):
     assert h1 == pytest.approx(h0, rel=1e-5)  # Trotter splitting error O(dt^3)
     # and it relaxes toward the target temperature
     assert abs(k / (0.5 * st.g_dof * kt) - 1.0) < 0.7
+
+
+# ---------------------------------------------------------------------
+# Langevin BAOAB (kernels/langevin.cu, kernels/philox.cuh,
+# src/integrator/langevin_baoab.rs)
+# ---------------------------------------------------------------------
+
+def test_oracle_philox_known_answer_vectors(oracle):
+    # Random123 kat_vectors for philox4x32_10 pin the C restatement
+    assert oracle.philox4x32_10((0, 0), (0, 0, 0, 0)) == \
+        [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox4x32_10((0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF,) * 4) == \
+        [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox4x32_10((0xA4093822, 0x299F31D0),
+                                (0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344)) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_oracle_gaussian_is_box_muller_of_the_first_two_lanes(oracle):
+    # Box-Muller on (x + 0.5) 2^-32 of lanes 0 and 1, in double
+    for seed, ctr, pid, axis in ((0, 0, 0, 0), (42, 7, 123, 2), (2**40 + 5, 2**33 + 1, 99999, 1)):
+        o = oracle.philox4x32_10((seed & 0xFFFFFFFF, seed >> 32),
+                                 (ctr & 0xFFFFFFFF, ctr >> 32, pid, axis))
+        u1 = (o[0] + 0.5) / 2**32
+        u2 = (o[1] + 0.5) / 2**32
+        want = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        assert oracle.philox_gaussian(seed, ctr, pid, axis) == want
+
+
+def test_ou_step_variance_and_independence(oracle):
+    # from v = 0 one O step gives v ~ N(0, (1 - alpha^2) kT / m) per axis;
+    # distinct axes and particles are uncorrelated draws
+    n = 40000
+    posq = np.zeros((n, 4), np.float32)
+    img = np.zeros((n, 3), np.int32)
+    v = np.zeros((3, n), np.float32)
+    m = np.full(n, 7.0e4, np.float32)
+    kt, gamma, dt = 9.5e-4, 1.0 / 4134.0, 82.68
+    alpha = oracle.lan_alpha(gamma, dt)
+    L = np.array([1e4, 1e4, 1e4, 0, 0, 0], np.float32)
+    oracle.lan_ou_drift_half(posq, img, v, m, L, dt, alpha, kt, seed=11, counter=3)
+    var = (1.0 - float(alpha) ** 2) * kt / 7.0e4
+    for k in range(3):
+        s2 = float(np.mean(v[k].astype(np.float64) ** 2))
+        assert s2 == pytest.approx(var, rel=4 * math.sqrt(2.0 / n))
+    c = np.corrcoef(v.astype(np.float64))
+    assert np.all(np.abs(c[np.triu_indices(3, 1)]) < 5.0 / math.sqrt(n))
+    # the half drift moved positions by v dt/2
+    assert np.allclose(posq[:, 0], v[0] * np.float32(dt * 0.5), rtol=1e-6)
+
+
+def test_baoab_harmonic_configurational_temperature(oracle):
+    # BAOAB samples the exact configurational distribution of a harmonic
+    # oscillator (Leimkuhler & Matthews 2013): <x^2> = kT / k at any stable dt
+    n, kt, k, m = 2000, 1.0e-3, 2.0e-4, 1.0e3
+    omega = math.sqrt(k / m)
+    dt = 0.5 / omega
+    gamma = 0.5 * omega
+    alpha = oracle.lan_alpha(gamma, dt)
+    posq = np.zeros((n, 4), np.float32)
+    img = np.zeros((n, 3), np.int32)
+    v = np.zeros((3, n), np.float32)
+    mass = np.full(n, m, np.float32)
+    L = np.array([1e6, 1e6, 1e6, 0, 0, 0], np.float32)
+
+    def force(p):
+        return (np.float32(-k) * p[:, :3]).T.astype(np.float32).copy()
+
+    f = force(posq)
+    acc, cnt = 0.0, 0
+    for step in range(1200):
+        oracle.lan_kick_drift_half(posq, img, v, f, mass, L, dt)
+        oracle.lan_ou_drift_half(posq, img, v, mass, L, dt, alpha, kt, seed=5, counter=step)
+        f = force(posq)
+        oracle.vv_kick_f32(v, f, mass, dt)
+        if step >= 200:
+            acc += float(np.mean(posq[:, :3].astype(np.float64) ** 2))
+            cnt += 1
+    assert acc / cnt == pytest.approx(kt / k, rel=0.03)
