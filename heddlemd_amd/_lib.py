@@ -22,6 +22,7 @@ _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _ctx = C.c_void_p
 
@@ -47,6 +48,9 @@ SIGNATURES = {
     "hd_set_crescale": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_uint64]),
     "hd_set_langevin": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64]),
+    "hd_set_shake": (C.c_int, [_ctx, C.c_uint64, _u32p, _u32p, _u32p, _u8p, _u8p, _f64p]),
+    "hd_set_shake_tolerance": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int]),
+    "hd_get_constraint_stats": (C.c_int, [_ctx, _u64p]),
     "hd_get_langevin_counter": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
     "hd_setup": (C.c_int, [_ctx]),
     "hd_force_eval": (C.c_int, [_ctx, C.c_int]),

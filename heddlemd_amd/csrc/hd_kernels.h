@@ -138,6 +138,31 @@ struct HdSettleArgs {
 void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a);
 void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool accumulate_virial);
 
+// General SHAKE / RATTLE groups (kernels/shake.cu): atoms goff[g] ..
+// goff[g+1] of gatoms, constraints coff[g] .. coff[g+1] with group-local
+// indices packed (i | j << 8) and r2 = (f32 d)^2.  One thread per group;
+// its atoms and constraints live in LDS at a fixed per-thread stride.
+#define HD_SHAKE_MAX_ATOMS 64
+#define HD_SHAKE_MAX_CONS 64
+struct HdShakeArgs {
+  unsigned n_groups, max_atoms, max_cons, threads;  // threads per block
+  const unsigned *goff, *gatoms, *coff, *cij;
+  const float* cr2;
+  float4* posq;
+  const float4* snap;
+  float *vx, *vy, *vz;
+  const float* mass;
+  float* w_cons;
+  const float* lattice;
+  float dt, tol, rattle_tol;
+  int max_iter;
+  unsigned* stats;  // [sum sweeps, max sweeps, unconverged] for SHAKE then RATTLE
+};
+// LDS bytes per thread of a group with the given maxima
+unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons);
+void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a);
+void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool accumulate_virial);
+
 struct HdNhcArgs {
   int chain, n_yoshida, n_resp;
   double w[7];

@@ -129,6 +129,15 @@ struct hd_ctx {
   float* factors = nullptr;  // [0] pre, [1] post
   double* nhc_state = nullptr;
   unsigned* d_groups = nullptr;
+  // general SHAKE / RATTLE groups
+  std::vector<uint32_t> h_sh_goff, h_sh_atoms, h_sh_coff, h_sh_cij;
+  std::vector<float> h_sh_r2;
+  unsigned sh_max_atoms = 0, sh_max_cons = 0, sh_threads = 0;
+  double sh_tol = 3.57e-6, sh_rattle_tol = 8.63e-17;
+  int sh_max_iter = 32;
+  unsigned *d_sh_goff = nullptr, *d_sh_atoms = nullptr, *d_sh_coff = nullptr,
+           *d_sh_cij = nullptr, *d_sh_stats = nullptr;
+  float* d_sh_r2 = nullptr;
 
   // graphs
   hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
@@ -350,6 +359,9 @@ void stage_end(hd_ctx* c, bool on) {
 
 // Force evaluation; kick = fuse the post-force half kick with factor
 // factors[1].  Enqueue only.
+bool has_shake(const hd_ctx* c) { return c->h_sh_goff.size() > 1; }
+bool has_constraints(const hd_ctx* c) { return !c->h_groups.empty() || has_shake(c); }
+
 void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
@@ -450,7 +462,7 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   fa.alpha = c->alpha;
   fa.w_share = c->w_share;
   fa.lattice = c->d_lattice;
-  fa.w_cons = c->h_groups.empty() ? nullptr : c->w_cons;
+  fa.w_cons = has_constraints(c) ? c->w_cons : nullptr;
   fa.fx = c->fx;
   fa.fy = c->fy;
   fa.fz = c->fz;
@@ -476,6 +488,66 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   }
 }
 
+HdSettleArgs settle_args(hd_ctx* c);
+
+HdShakeArgs shake_args(hd_ctx* c, float dt) {
+  HdShakeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.n_groups = (unsigned)(c->h_sh_goff.size() - 1);
+  a.max_atoms = c->sh_max_atoms;
+  a.max_cons = c->sh_max_cons;
+  a.threads = c->sh_threads;
+  a.goff = c->d_sh_goff;
+  a.gatoms = c->d_sh_atoms;
+  a.coff = c->d_sh_coff;
+  a.cij = c->d_sh_cij;
+  a.cr2 = c->d_sh_r2;
+  a.posq = c->posq;
+  a.snap = c->snap;
+  a.vx = c->vx;
+  a.vy = c->vy;
+  a.vz = c->vz;
+  a.mass = c->mass;
+  a.w_cons = c->w_cons;
+  a.lattice = c->d_lattice;
+  a.dt = dt;
+  a.tol = (float)c->sh_tol;
+  a.rattle_tol = (float)c->sh_rattle_tol;
+  a.max_iter = c->sh_max_iter;
+  a.stats = c->d_sh_stats;
+  return a;
+}
+
+// constraint hooks of run_step (src/integrator/mod.rs:344-429) for both
+// constraint kinds: positions after a drift, velocities after the kick
+void enqueue_constrain_positions(hd_ctx* c, bool timed) {
+  if (!c->h_groups.empty()) {
+    HdSettleArgs sa = settle_args(c);
+    stage_begin(c, "settle_positions", timed);
+    hd_launch_settle_positions(c->stream, sa);
+    stage_end(c, timed);
+  }
+  if (has_shake(c)) {
+    stage_begin(c, "shake_positions", timed);
+    hd_launch_shake_positions(c->stream, shake_args(c, (float)c->dt));
+    stage_end(c, timed);
+  }
+}
+
+void enqueue_constrain_velocities(hd_ctx* c, bool timed) {
+  if (!c->h_groups.empty()) {
+    HdSettleArgs sa = settle_args(c);
+    stage_begin(c, "settle_velocities", timed);
+    hd_launch_settle_velocities(c->stream, sa, true);
+    stage_end(c, timed);
+  }
+  if (has_shake(c)) {
+    stage_begin(c, "rattle_velocities", timed);
+    hd_launch_rattle_velocities(c->stream, shake_args(c, (float)c->dt), true);
+    stage_end(c, timed);
+  }
+}
+
 HdIntegrateArgs integrate_args(hd_ctx* c) {
   HdIntegrateArgs ia;
   memset(&ia, 0, sizeof(ia));
@@ -493,7 +565,7 @@ HdIntegrateArgs integrate_args(hd_ctx* c) {
   ia.fz = c->fz;
   ia.mass = c->mass;
   ia.lattice = c->d_lattice;
-  ia.snap = c->h_groups.empty() ? nullptr : c->snap;
+  ia.snap = has_constraints(c) ? c->snap : nullptr;
   return ia;
 }
 
@@ -550,7 +622,6 @@ void enqueue_crescale(hd_ctx* c, bool timed) {
 void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   hipStream_t st = c->stream;
   unsigned n = (unsigned)c->n;
-  bool settle = !c->h_groups.empty();
   HdIntegrateArgs ia = integrate_args(c);
   HdLangevinArgs la;
   la.alpha = expf(-(float)c->lan_friction * (float)c->dt);
@@ -558,26 +629,15 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   la.seed_lo = (unsigned)(c->lan_seed & 0xFFFFFFFFull);
   la.seed_hi = (unsigned)(c->lan_seed >> 32);
   la.counter = c->lan_counter;
-  HdSettleArgs sa = settle_args(c);
   stage_begin(c, "langevin_kick_drift", timed);
   hd_launch_lan_kick_drift(st, ia, la);
   stage_end(c, timed);
-  if (settle) {
-    stage_begin(c, "settle_positions", timed);
-    hd_launch_settle_positions(st, sa);
-    stage_end(c, timed);
-  }
+  enqueue_constrain_positions(c, timed);
   stage_begin(c, "langevin_ou_drift", timed);
   hd_launch_lan_ou_drift(st, ia, la);
   stage_end(c, timed);
-  if (settle) {
-    stage_begin(c, "settle_positions", timed);
-    hd_launch_settle_positions(st, sa);
-    stage_end(c, timed);
-    stage_begin(c, "settle_velocities", timed);
-    hd_launch_settle_velocities(st, sa, true);
-    stage_end(c, timed);
-  }
+  enqueue_constrain_positions(c, timed);
+  enqueue_constrain_velocities(c, timed);
   if (c->have_baro) {
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
                  0.f, nullptr);
@@ -600,7 +660,6 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
                  (float)c->dt, c->factors + 0);
     stage_end(c, timed);
   }
-  bool settle = !c->h_groups.empty();
   if (c->have_lan) {
     enqueue_langevin_step(c, ev, timed);
     return;
@@ -626,37 +685,12 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   ia.mass = c->mass;
   ia.lattice = c->d_lattice;
   ia.factor = c->have_nhc ? c->factors + 0 : nullptr;
-  ia.snap = settle ? c->snap : nullptr;
+  ia.snap = has_constraints(c) ? c->snap : nullptr;
   stage_begin(c, "vv_kick_drift", timed);
   hd_launch_kick_drift(st, ia);
   stage_end(c, timed);
-  if (settle) {
-    HdSettleArgs sa;
-    memset(&sa, 0, sizeof(sa));
-    sa.n_groups = (unsigned)(c->h_groups.size() / 3);
-    sa.groups = c->d_groups;
-    double total = c->m_o + 2.0 * c->m_h;
-    double h = sqrt(c->d_oh * c->d_oh - 0.25 * c->d_hh * c->d_hh);
-    sa.ra = (float)((2.0 * c->m_h / total) * h);
-    sa.rb = (float)((c->m_o / total) * h);
-    sa.rc = (float)(c->d_hh * 0.5);
-    sa.m_o = (float)c->m_o;
-    sa.m_h = (float)c->m_h;
-    sa.dt = (float)c->dt;
-    sa.posq = c->posq;
-    sa.snap = c->snap;
-    sa.vx = c->vx;
-    sa.vy = c->vy;
-    sa.vz = c->vz;
-    sa.w_cons = c->w_cons;
-    sa.lattice = c->d_lattice;
-    stage_begin(c, "settle_positions", timed);
-    hd_launch_settle_positions(st, sa);
-    stage_end(c, timed);
-    stage_begin(c, "settle_velocities", timed);
-    hd_launch_settle_velocities(st, sa, true);
-    stage_end(c, timed);
-  }
+  enqueue_constrain_positions(c, timed);
+  enqueue_constrain_velocities(c, timed);
   if (c->have_nhc) {
     stage_begin(c, "nhc_post", timed);
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
@@ -868,6 +902,80 @@ int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, d
   c->d_hh = d_hh;
   c->m_o = m_o;
   c->m_h = m_h;
+  return HD_OK;
+}
+
+static void drop_graphs(hd_ctx* c);
+
+int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* gatoms,
+                 const uint32_t* coff, const uint8_t* ci, const uint8_t* cj, const double* d) {
+  if (!c || (ng && (!goff || !gatoms || !coff || !ci || !cj || !d))) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_shake after hd_setup");
+  if (ng && (goff[0] != 0 || coff[0] != 0))
+    return fail(c, HD_ERR_ARG, "SHAKE group offsets must start at 0");
+  std::vector<char> seen(c->n, 0);
+  for (uint32_t a : c->h_groups) seen[a] = 1;
+  unsigned ma = 0, mc = 0;
+  for (uint64_t g = 0; g < ng; ++g) {
+    if (goff[g + 1] <= goff[g] || coff[g + 1] < coff[g])
+      return fail(c, HD_ERR_ARG, "SHAKE group %llu has decreasing offsets", (unsigned long long)g);
+    unsigned na = goff[g + 1] - goff[g], nc = coff[g + 1] - coff[g];
+    if (na > HD_SHAKE_MAX_ATOMS || nc > HD_SHAKE_MAX_CONS)
+      return fail(c, HD_ERR_UNSUPPORTED, "SHAKE group %llu: %u atoms / %u constraints exceed %d / %d",
+                  (unsigned long long)g, na, nc, HD_SHAKE_MAX_ATOMS, HD_SHAKE_MAX_CONS);
+    for (unsigned k = goff[g]; k < goff[g + 1]; ++k) {
+      if (gatoms[k] >= c->n) return fail(c, HD_ERR_ARG, "SHAKE atom %u out of range", gatoms[k]);
+      if (seen[gatoms[k]]) return fail(c, HD_ERR_ARG, "atom %u in two constraint groups", gatoms[k]);
+      seen[gatoms[k]] = 1;
+    }
+    for (unsigned k = coff[g]; k < coff[g + 1]; ++k)
+      if (ci[k] >= na || cj[k] >= na || ci[k] == cj[k] || !(d[k] > 0))
+        return fail(c, HD_ERR_ARG, "SHAKE constraint %u of group %llu is invalid", k,
+                    (unsigned long long)g);
+    ma = std::max(ma, na);
+    mc = std::max(mc, nc);
+  }
+  c->h_sh_goff.assign(goff, goff + (ng ? ng + 1 : 0));
+  c->h_sh_atoms.assign(gatoms, gatoms + (ng ? goff[ng] : 0));
+  c->h_sh_coff.assign(coff, coff + (ng ? ng + 1 : 0));
+  unsigned ncons = ng ? coff[ng] : 0;
+  c->h_sh_cij.resize(ncons);
+  c->h_sh_r2.resize(ncons);
+  for (unsigned k = 0; k < ncons; ++k) {
+    c->h_sh_cij[k] = (uint32_t)ci[k] | ((uint32_t)cj[k] << 8);
+    float df = (float)d[k];  // r2 = (Real d)^2 (shake.rs:301-302)
+    c->h_sh_r2[k] = df * df;
+  }
+  c->sh_max_atoms = ma;
+  c->sh_max_cons = mc;
+  // threads per block: the largest power of two whose LDS fits in 48 KiB
+  unsigned per = hd_shake_lds_per_thread(std::max(ma, 1u), std::max(mc, 1u));
+  unsigned t = 256;
+  while (t > 1 && t * per > 48u * 1024u) t >>= 1;
+  c->sh_threads = t;
+  return HD_OK;
+}
+
+int hd_set_shake_tolerance(hd_ctx* c, double tol, double rattle_tol, int max_iter) {
+  if (!c || !(tol > 0) || !(rattle_tol > 0) || max_iter < 1) return HD_ERR_ARG;
+  if (c->ready) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  c->sh_tol = tol;
+  c->sh_rattle_tol = rattle_tol;
+  c->sh_max_iter = max_iter;
+  return HD_OK;
+}
+
+int hd_get_constraint_stats(hd_ctx* c, uint64_t* out) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  unsigned st[6] = {0, 0, 0, 0, 0, 0};
+  if (c->d_sh_stats) {
+    HIPCHK(c, hipMemcpyAsync(st, c->d_sh_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  for (int k = 0; k < 6; ++k) out[k] = st[k];
   return HD_OK;
 }
 
@@ -1125,6 +1233,21 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->d_groups, c->h_groups.size());
     if ((r = upload(c, c->d_groups, c->h_groups.data(), c->h_groups.size()))) return r;
   }
+  if (has_shake(c)) {
+    DALLOC(c, c->d_sh_goff, c->h_sh_goff.size());
+    DALLOC(c, c->d_sh_atoms, c->h_sh_atoms.size());
+    DALLOC(c, c->d_sh_coff, c->h_sh_coff.size());
+    DALLOC(c, c->d_sh_cij, c->h_sh_cij.size());
+    DALLOC(c, c->d_sh_r2, c->h_sh_r2.size());
+    DALLOC(c, c->d_sh_stats, 6);
+    if ((r = upload(c, c->d_sh_goff, c->h_sh_goff.data(), c->h_sh_goff.size()))) return r;
+    if ((r = upload(c, c->d_sh_atoms, c->h_sh_atoms.data(), c->h_sh_atoms.size()))) return r;
+    if ((r = upload(c, c->d_sh_coff, c->h_sh_coff.data(), c->h_sh_coff.size()))) return r;
+    if (!c->h_sh_cij.empty()) {
+      if ((r = upload(c, c->d_sh_cij, c->h_sh_cij.data(), c->h_sh_cij.size()))) return r;
+      if ((r = upload(c, c->d_sh_r2, c->h_sh_r2.data(), c->h_sh_r2.size()))) return r;
+    }
+  }
   if (c->have_spme) {
     size_t M = (size_t)c->grid[0] * c->grid[1] * c->grid[2];
     size_t Mc = (size_t)c->grid[0] * c->grid[1] * (c->grid[2] / 2 + 1);
@@ -1211,7 +1334,15 @@ int hd_force_eval(hd_ctx* c, int want_scalars) {
 
 int hd_constrain_velocities(hd_ctx* c) {
   if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_constrain_velocities before hd_setup");
-  if (c->h_groups.empty()) return HD_OK;
+  if (has_shake(c)) {
+    // Constraint::apply_initial_velocity_projection (shake.rs): RATTLE, dt = 0
+    hd_launch_rattle_velocities(c->stream, shake_args(c, 0.f), false);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (c->h_groups.empty()) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HD_OK;
+  }
   HdSettleArgs sa;
   memset(&sa, 0, sizeof(sa));
   sa.n_groups = (unsigned)(c->h_groups.size() / 3);
@@ -1251,6 +1382,9 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
     enqueue_forces(c, false, false, false);
     c->forces_valid = true;
   }
+  // constraint iteration counters cover this call's steps
+  if (c->d_sh_stats)
+    HIPCHK(c, hipMemsetAsync(c->d_sh_stats, 0, 6 * sizeof(unsigned), c->stream));
   for (int s = 0; s < n_steps; ++s) {
     bool ev = scalars_last && s == n_steps - 1;
     if (c->use_graph) {

@@ -615,6 +615,219 @@ __device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, 
 }
 
 // ---------------------------------------------------------------------
+// SHAKE / RATTLE for general constraint groups (kernels/shake.cu:96-490,
+// src/integrator/shake.rs): one thread per group, Gauss-Seidel sweeps in
+// topology order with the reference's f32 operation order; the group's
+// atoms and constraints are staged in LDS at a fixed per-thread stride,
+// which lifts the reference's register-bound caps (8 atoms, 12
+// constraints) to HD_SHAKE_MAX_ATOMS / HD_SHAKE_MAX_CONS so that whole
+// polymer chains can be one group.
+// ---------------------------------------------------------------------
+unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons) {
+  return 4u * (11u * max_atoms + 5u * max_cons);
+}
+
+__device__ __forceinline__ void shake_min_image_to(float ax, float ay, float az, float& bx,
+                                                   float& by, float& bz, const HdLattice& L) {
+  float dx = bx - ax, dy = by - ay, dz = bz - az;
+  hd_min_image(dx, dy, dz, L);
+  bx = ax + dx; by = ay + dy; bz = az + dz;
+}
+
+__device__ __forceinline__ void shake_com(const float* x, const float* y, const float* z,
+                                          const float* m, unsigned n, float& cx, float& cy,
+                                          float& cz) {
+  float total = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+  for (unsigned a = 0; a < n; ++a) {
+    sx += m[a] * x[a]; sy += m[a] * y[a]; sz += m[a] * z[a];
+    total += m[a];
+  }
+  float inv = total > 0.f ? 1.0f / total : 0.f;
+  cx = sx * inv; cy = sy * inv; cz = sz * inv;
+}
+
+__device__ __forceinline__ void shake_stats(unsigned* st, unsigned it, int max_iter) {
+  atomicAdd(&st[0], it);
+  atomicMax(&st[1], it);
+  if ((int)it >= max_iter) atomicAdd(&st[2], 1u);
+}
+
+__global__ void k_shake_positions(HdShakeArgs a) {
+  extern __shared__ float sh[];
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  const unsigned MA = a.max_atoms, MC = a.max_cons;
+  float* base = sh + (size_t)threadIdx.x * (11u * MA + 5u * MC);
+  float *x0 = base, *y0 = x0 + MA, *z0 = y0 + MA;
+  float *xu = z0 + MA, *yu = xu + MA, *zu = yu + MA;
+  float *xc = zu + MA, *yc = xc + MA, *zc = yc + MA;
+  float *m = zc + MA, *im = m + MA;
+  float *gx = im + MA, *gy = gx + MC, *gz = gy + MC, *ip = gz + MC;
+  unsigned* cij = (unsigned*)(ip + MC);
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+  for (unsigned k = 0; k < na; ++k) {
+    unsigned i = a.gatoms[a0 + k];
+    float4 p = a.posq[i], q = a.snap[i];
+    xu[k] = p.x; yu[k] = p.y; zu[k] = p.z;
+    x0[k] = q.x; y0[k] = q.y; z0[k] = q.z;
+    float mk = a.mass[i];
+    m[k] = mk;
+    im[k] = mk > 0.f ? 1.0f / mk : 0.f;
+  }
+  for (unsigned k = 1; k < na; ++k) {
+    shake_min_image_to(x0[0], y0[0], z0[0], x0[k], y0[k], z0[k], L);
+    shake_min_image_to(xu[0], yu[0], zu[0], xu[k], yu[k], zu[k], L);
+  }
+  for (unsigned k = 0; k < na; ++k) { xc[k] = xu[k]; yc[k] = yu[k]; zc[k] = zu[k]; }
+  for (unsigned k = 0; k < nc; ++k) {
+    unsigned pk = a.cij[c0 + k];
+    unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
+    cij[k] = pk;
+    gx[k] = x0[li] - x0[lj];
+    gy[k] = y0[li] - y0[lj];
+    gz[k] = z0[li] - z0[lj];
+    ip[k] = im[li] + im[lj];
+  }
+  int it = 0;
+  for (; it < a.max_iter; ++it) {
+    bool converged = true;
+    for (unsigned k = 0; k < nc; ++k) {
+      unsigned pk = cij[k];
+      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
+      float dx = xc[li] - xc[lj], dy = yc[li] - yc[lj], dz = zc[li] - zc[lj];
+      float dist2 = dx * dx + dy * dy + dz * dz;
+      float sigma = dist2 - a.cr2[c0 + k];
+      if (fabsf(sigma) > a.tol) {
+        converged = false;
+        float ddot = dx * gx[k] + dy * gy[k] + dz * gz[k];
+        float denom = 2.0f * ddot * ip[k];
+        if (denom == 0.f) continue;
+        float lambda = sigma / denom;
+        float ki = lambda * im[li], kj = lambda * im[lj];
+        xc[li] -= ki * gx[k]; yc[li] -= ki * gy[k]; zc[li] -= ki * gz[k];
+        xc[lj] += kj * gx[k]; yc[lj] += kj * gy[k]; zc[lj] += kj * gz[k];
+      }
+    }
+    if (converged) break;
+  }
+  if (a.stats) shake_stats(a.stats, (unsigned)it, a.max_iter);
+  float cxg, cyg, czg;
+  shake_com(xc, yc, zc, m, na, cxg, cyg, czg);
+  float inv_dt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
+  float inv_dt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+  for (unsigned k = 0; k < na; ++k) {
+    unsigned i = a.gatoms[a0 + k];
+    float dxg = xc[k] - xu[k], dyg = yc[k] - yu[k], dzg = zc[k] - zu[k];
+    a.vx[i] += dxg * inv_dt; a.vy[i] += dyg * inv_dt; a.vz[i] += dzg * inv_dt;
+    float4 p = a.posq[i];
+    if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
+    else { p.x += dxg; p.y += dyg; p.z += dzg; }
+    a.posq[i] = p;
+    float rx = xc[k] - cxg, ry = yc[k] - cyg, rz = zc[k] - czg;
+    float scale = m[k] * inv_dt2;
+    a.w_cons[i] = scale * (dxg * rx + dyg * ry + dzg * rz);
+  }
+}
+
+template <bool ACC>
+__global__ void k_rattle_velocities(HdShakeArgs a) {
+  extern __shared__ float sh[];
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  const unsigned MA = a.max_atoms, MC = a.max_cons;
+  float* base = sh + (size_t)threadIdx.x * (11u * MA + 5u * MC);
+  float *px = base, *py = px + MA, *pz = py + MA;
+  float *v0 = pz + MA, *v1 = v0 + MA, *v2 = v1 + MA;
+  float *dv0 = v2 + MA, *dv1 = dv0 + MA, *dv2 = dv1 + MA;
+  float *m = dv2 + MA, *im = m + MA;
+  float *dx = im + MA, *dy = dx + MC, *dz = dy + MC, *d2 = dz + MC;
+  float* ipc = d2 + MC;  // 1/m_i + 1/m_j per constraint (5th constraint slot)
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+  for (unsigned k = 0; k < na; ++k) {
+    unsigned i = a.gatoms[a0 + k];
+    float4 p = a.posq[i];
+    px[k] = p.x; py[k] = p.y; pz[k] = p.z;
+    v0[k] = a.vx[i]; v1[k] = a.vy[i]; v2[k] = a.vz[i];
+    dv0[k] = dv1[k] = dv2[k] = 0.f;
+    float mk = a.mass[i];
+    m[k] = mk;
+    im[k] = mk > 0.f ? 1.0f / mk : 0.f;
+  }
+  for (unsigned k = 1; k < na; ++k)
+    shake_min_image_to(px[0], py[0], pz[0], px[k], py[k], pz[k], L);
+  for (unsigned k = 0; k < nc; ++k) {
+    unsigned pk = a.cij[c0 + k];
+    unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
+    dx[k] = px[li] - px[lj];
+    dy[k] = py[li] - py[lj];
+    dz[k] = pz[li] - pz[lj];
+    d2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+    ipc[k] = im[li] + im[lj];
+  }
+  int it = 0;
+  for (; it < a.max_iter; ++it) {
+    bool converged = true;
+    for (unsigned k = 0; k < nc; ++k) {
+      unsigned pk = a.cij[c0 + k];
+      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
+      float vxr = v0[li] - v0[lj], vyr = v1[li] - v1[lj], vzr = v2[li] - v2[lj];
+      float vrel = vxr * dx[k] + vyr * dy[k] + vzr * dz[k];
+      if (fabsf(vrel) > a.rattle_tol) {
+        converged = false;
+        float denom = d2[k] * ipc[k];
+        if (denom == 0.f) continue;
+        float mu = vrel / denom;
+        float ki = mu * im[li], kj = mu * im[lj];
+        v0[li] -= ki * dx[k]; v1[li] -= ki * dy[k]; v2[li] -= ki * dz[k];
+        v0[lj] += kj * dx[k]; v1[lj] += kj * dy[k]; v2[lj] += kj * dz[k];
+        dv0[li] -= ki * dx[k]; dv1[li] -= ki * dy[k]; dv2[li] -= ki * dz[k];
+        dv0[lj] += kj * dx[k]; dv1[lj] += kj * dy[k]; dv2[lj] += kj * dz[k];
+      }
+    }
+    if (converged) break;
+  }
+  if (a.stats) shake_stats(a.stats + 3, (unsigned)it, a.max_iter);
+  if (ACC && a.dt > 0.f) {
+    float inv_dt = 1.0f / a.dt;
+    float cx, cy, cz;
+    shake_com(px, py, pz, m, na, cx, cy, cz);
+    for (unsigned k = 0; k < na; ++k) {
+      float rx = px[k] - cx, ry = py[k] - cy, rz = pz[k] - cz;
+      float scale = m[k] * inv_dt;
+      unsigned i = a.gatoms[a0 + k];
+      a.w_cons[i] += scale * (dv0[k] * rx + dv1[k] * ry + dv2[k] * rz);
+    }
+  }
+  for (unsigned k = 0; k < na; ++k) {
+    unsigned i = a.gatoms[a0 + k];
+    a.vx[i] = v0[k]; a.vy[i] = v1[k]; a.vz[i] = v2[k];
+  }
+}
+
+void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a) {
+  if (a.n_groups == 0) return;
+  unsigned T = a.threads;
+  size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
+  hipLaunchKernelGGL(k_shake_positions, dim3((a.n_groups + T - 1) / T), dim3(T), lds, st, a);
+}
+
+void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool acc) {
+  if (a.n_groups == 0) return;
+  unsigned T = a.threads;
+  size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
+  if (acc)
+    hipLaunchKernelGGL(k_rattle_velocities<true>, dim3((a.n_groups + T - 1) / T), dim3(T), lds,
+                       st, a);
+  else
+    hipLaunchKernelGGL(k_rattle_velocities<false>, dim3((a.n_groups + T - 1) / T), dim3(T), lds,
+                       st, a);
+}
+
+// ---------------------------------------------------------------------
 // Langevin BAOAB (src/integrator/langevin_baoab.rs: B, A, O, A, F, B with
 // the last B fused into the force finalize).  B + A and O + A are fused
 // pairs of the reference's vv_kick / lan_drift_half / lan_ou_step kernels

@@ -101,6 +101,33 @@ class Engine:
                                             float(compressibility), float(kt), int(seed)),
                   "hd_set_crescale")
 
+    def set_shake(self, groups):
+        """groups: list of (atom_ids, [(local_i, local_j, d), ...])."""
+        goff, gat, coff, ci, cj, d = [0], [], [0], [], [], []
+        for atoms, cons in groups:
+            gat.extend(int(a) for a in atoms)
+            goff.append(len(gat))
+            for a, b, dist in cons:
+                ci.append(a)
+                cj.append(b)
+                d.append(float(dist))
+            coff.append(len(ci))
+        self._chk(self._lib.hd_set_shake(self._ctx, len(groups), np.asarray(goff, np.uint32),
+                                         np.asarray(gat, np.uint32), np.asarray(coff, np.uint32),
+                                         np.asarray(ci, np.uint8), np.asarray(cj, np.uint8),
+                                         np.asarray(d, np.float64)), "hd_set_shake")
+
+    def set_shake_tolerance(self, tol, rattle_tol, max_iter):
+        self._chk(self._lib.hd_set_shake_tolerance(self._ctx, float(tol), float(rattle_tol),
+                                                   int(max_iter)), "hd_set_shake_tolerance")
+
+    def constraint_stats(self):
+        out = np.zeros(6, np.uint64)
+        self._chk(self._lib.hd_get_constraint_stats(self._ctx, out), "hd_get_constraint_stats")
+        keys = ["shake_sweeps", "shake_max", "shake_capped", "rattle_sweeps", "rattle_max",
+                "rattle_capped"]
+        return dict(zip(keys, (int(x) for x in out)))
+
     def set_langevin(self, friction, kt, seed):
         self._chk(self._lib.hd_set_langevin(self._ctx, float(friction), float(kt), int(seed)),
                   "hd_set_langevin")
@@ -236,6 +263,8 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_spme(system.spme)
     if r_skin is not None:
         e.set_neighbor(r_skin)
+    if getattr(system, "shake_groups", None) is not None:
+        e.set_shake(system.shake_groups)
     if system.water_groups is not None:
         d_oh, d_hh = system.water_geom
         m_o = float(system.mass[system.water_groups[0, 0]])

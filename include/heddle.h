@@ -73,6 +73,24 @@ int hd_set_neighbor(hd_ctx *ctx, double r_skin);
  * groups = 3 * n_groups atom ids (O, H1, H2). */
 int hd_set_settle(hd_ctx *ctx, uint64_t n_groups, const uint32_t *groups, double d_oh,
                   double d_hh, double m_o, double m_h);
+/* General SHAKE / RATTLE constraint groups (src/integrator/shake.rs,
+ * kernels/shake.cu): group g holds atoms group_atoms[group_offsets[g] ..
+ * group_offsets[g+1]) and constraints con_offsets[g] .. con_offsets[g+1]
+ * between group-local atoms con_i/con_j at distance con_d.  Up to 64 atoms
+ * and 64 constraints per group (the reference caps groups at 8 atoms and
+ * 12 constraints, shake.rs:21-22); an atom belongs to at most one
+ * constraint group (SETTLE or SHAKE). */
+int hd_set_shake(hd_ctx *ctx, uint64_t n_groups, const uint32_t *group_offsets,
+                 const uint32_t *group_atoms, const uint32_t *con_offsets, const uint8_t *con_i,
+                 const uint8_t *con_j, const double *con_d);
+/* SHAKE tolerance on |r^2 - d^2| (Bohr^2), RATTLE tolerance on |v_rel . r|
+ * and the sweep cap; the defaults are the reference's 3.57e-6, 8.63e-17
+ * and 32 (shake.cu:251-256, 446). */
+int hd_set_shake_tolerance(hd_ctx *ctx, double tol, double rattle_tol, int max_iter);
+/* Sweep counts of the last hd_step call (or of hd_constrain_velocities):
+ * out[0..2] SHAKE total sweeps over all groups and calls, the largest
+ * count, groups that hit the cap; out[3..5] the same for RATTLE. */
+int hd_get_constraint_stats(hd_ctx *ctx, uint64_t out[6]);
 /* Velocity Verlet (src/integrator/velocity_verlet.rs); lossless = the
  * compensated (f32, f64) mode. */
 int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);

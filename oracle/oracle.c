@@ -76,6 +76,16 @@ static inline void cart_to_frac_f(float x, float y, float z, const float *L,
     *sa = (x - *sb * L[3] - *sc * L[4]) / L[0];
 }
 
+static inline void min_image_f(float *dx, float *dy, float *dz, const float *L)
+{
+    float sa, sb, sc;
+    cart_to_frac_f(*dx, *dy, *dz, L, &sa, &sb, &sc);
+    float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+    *dx -= ka * L[0] + kb * L[3] + kc * L[4];
+    *dy -= kb * L[1] + kc * L[5];
+    *dz -= kc * L[2];
+}
+
 static inline void wrap_with_image_f(float *x, float *y, float *z, int *ka_o,
                                      int *kb_o, int *kc_o, const float *L)
 {
@@ -687,6 +697,191 @@ void or_vv_kick_f32(int n, float *vx, float *vy, float *vz, const float *fx,
         vx[i] += ax * half_dt;
         vy[i] += ay * half_dt;
         vz[i] += az * half_dt;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* SHAKE / RATTLE (kernels/shake.cu:96-490, src/integrator/shake.rs)     */
+/* Groups: atoms goff[g]..goff[g+1] of gatoms, constraints coff[g]..     */
+/* coff[g+1] with group-local atom indices ci/cj and r2 = (f32 d)^2.     */
+/* ------------------------------------------------------------------ */
+
+static void min_image_to_f(float ax, float ay, float az, float *bx, float *by,
+                           float *bz, const float *L)
+{
+    /* shake.cu:27-41 */
+    float dx = *bx - ax, dy = *by - ay, dz = *bz - az;
+    min_image_f(&dx, &dy, &dz, L);
+    *bx = ax + dx; *by = ay + dy; *bz = az + dz;
+}
+
+static void weighted_com_f(const float *x, const float *y, const float *z, const float *m,
+                           int n, float *cx, float *cy, float *cz)
+{
+    /* shake.cu:70-91 */
+    float total = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int a = 0; a < n; ++a) {
+        sx += m[a] * x[a]; sy += m[a] * y[a]; sz += m[a] * z[a];
+        total += m[a];
+    }
+    float inv = total > 0.0f ? 1.0f / total : 0.0f;
+    *cx = sx * inv; *cy = sy * inv; *cz = sz * inv;
+}
+
+#define OR_SHAKE_MAXA 64
+#define OR_SHAKE_MAXC 64
+
+void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
+                            const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
+                            const float *cr2, float *posq, float *vx, float *vy, float *vz,
+                            const float *snap, const float *mass, const float *L, float dt,
+                            float tol, int max_iter, float *w_cons, int *iters)
+{
+    /* shake_positions (shake.cu:96-330): Gauss-Seidel sweeps in topology
+     * order with gradients from the pre-drift snapshot, in the image of
+     * atom 0; delta write-back; position-level constraint virial */
+    for (int g = 0; g < n_groups; ++g) {
+        int na = (int)(goff[g + 1] - goff[g]), nc = (int)(coff[g + 1] - coff[g]);
+        const uint32_t *at = gatoms + goff[g];
+        float x0[OR_SHAKE_MAXA], y0[OR_SHAKE_MAXA], z0[OR_SHAKE_MAXA];
+        float xu[OR_SHAKE_MAXA], yu[OR_SHAKE_MAXA], zu[OR_SHAKE_MAXA];
+        float xc[OR_SHAKE_MAXA], yc[OR_SHAKE_MAXA], zc[OR_SHAKE_MAXA];
+        float m[OR_SHAKE_MAXA], im[OR_SHAKE_MAXA];
+        for (int a = 0; a < na; ++a) {
+            uint32_t i = at[a];
+            xu[a] = posq[4 * i]; yu[a] = posq[4 * i + 1]; zu[a] = posq[4 * i + 2];
+            x0[a] = snap[4 * i]; y0[a] = snap[4 * i + 1]; z0[a] = snap[4 * i + 2];
+            m[a] = mass[i];
+            im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
+        }
+        for (int a = 1; a < na; ++a) {
+            min_image_to_f(x0[0], y0[0], z0[0], &x0[a], &y0[a], &z0[a], L);
+            min_image_to_f(xu[0], yu[0], zu[0], &xu[a], &yu[a], &zu[a], L);
+        }
+        for (int a = 0; a < na; ++a) { xc[a] = xu[a]; yc[a] = yu[a]; zc[a] = zu[a]; }
+        float gx[OR_SHAKE_MAXC], gy[OR_SHAKE_MAXC], gz[OR_SHAKE_MAXC], ip[OR_SHAKE_MAXC];
+        const uint8_t *li = ci + coff[g], *lj = cj + coff[g];
+        const float *r2 = cr2 + coff[g];
+        for (int k = 0; k < nc; ++k) {
+            gx[k] = x0[li[k]] - x0[lj[k]];
+            gy[k] = y0[li[k]] - y0[lj[k]];
+            gz[k] = z0[li[k]] - z0[lj[k]];
+            ip[k] = im[li[k]] + im[lj[k]];
+        }
+        int it = 0;
+        for (; it < max_iter; ++it) {
+            int converged = 1;
+            for (int k = 0; k < nc; ++k) {
+                int a = li[k], b = lj[k];
+                float dx = xc[a] - xc[b], dy = yc[a] - yc[b], dz = zc[a] - zc[b];
+                float dist2 = dx * dx + dy * dy + dz * dz;
+                float sigma = dist2 - r2[k];
+                if (fabsf(sigma) > tol) {
+                    converged = 0;
+                    float ddot = dx * gx[k] + dy * gy[k] + dz * gz[k];
+                    float denom = 2.0f * ddot * ip[k];
+                    if (denom == 0.0f) continue;
+                    float lambda = sigma / denom;
+                    float ki = lambda * im[a], kj = lambda * im[b];
+                    xc[a] -= ki * gx[k]; yc[a] -= ki * gy[k]; zc[a] -= ki * gz[k];
+                    xc[b] += kj * gx[k]; yc[b] += kj * gy[k]; zc[b] += kj * gz[k];
+                }
+            }
+            if (converged) break;
+        }
+        if (iters) iters[g] = it;
+        float cxg, cyg, czg;
+        weighted_com_f(xc, yc, zc, m, na, &cxg, &cyg, &czg);
+        float inv_dt = dt != 0.0f ? 1.0f / dt : 0.0f;
+        float inv_dt2 = dt != 0.0f ? 1.0f / (dt * dt) : 0.0f;
+        for (int a = 0; a < na; ++a) {
+            uint32_t i = at[a];
+            float dxg = xc[a] - xu[a], dyg = yc[a] - yu[a], dzg = zc[a] - zu[a];
+            vx[i] += dxg * inv_dt; vy[i] += dyg * inv_dt; vz[i] += dzg * inv_dt;
+            if (a == 0) {
+                posq[4 * i] = xc[0]; posq[4 * i + 1] = yc[0]; posq[4 * i + 2] = zc[0];
+            } else {
+                posq[4 * i] += dxg; posq[4 * i + 1] += dyg; posq[4 * i + 2] += dzg;
+            }
+            float rx = xc[a] - cxg, ry = yc[a] - cyg, rz = zc[a] - czg;
+            float scale = m[a] * inv_dt2;
+            w_cons[i] = scale * (dxg * rx + dyg * ry + dzg * rz);
+        }
+    }
+}
+
+void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
+                              const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
+                              const float *posq, float *vx, float *vy, float *vz,
+                              const float *mass, const float *L, float dt, float tol,
+                              int max_iter, float *w_cons, int *iters)
+{
+    /* rattle_velocities (shake.cu:336-490): Gauss-Seidel on the relative
+     * velocity along each bond; with dt > 0 the velocity-level virial is
+     * added to the position-level one */
+    for (int g = 0; g < n_groups; ++g) {
+        int na = (int)(goff[g + 1] - goff[g]), nc = (int)(coff[g + 1] - coff[g]);
+        const uint32_t *at = gatoms + goff[g];
+        float px[OR_SHAKE_MAXA], py[OR_SHAKE_MAXA], pz[OR_SHAKE_MAXA];
+        float v0[OR_SHAKE_MAXA], v1[OR_SHAKE_MAXA], v2[OR_SHAKE_MAXA];
+        float dv0[OR_SHAKE_MAXA], dv1[OR_SHAKE_MAXA], dv2[OR_SHAKE_MAXA];
+        float m[OR_SHAKE_MAXA], im[OR_SHAKE_MAXA];
+        for (int a = 0; a < na; ++a) {
+            uint32_t i = at[a];
+            px[a] = posq[4 * i]; py[a] = posq[4 * i + 1]; pz[a] = posq[4 * i + 2];
+            v0[a] = vx[i]; v1[a] = vy[i]; v2[a] = vz[i];
+            dv0[a] = dv1[a] = dv2[a] = 0.0f;
+            m[a] = mass[i];
+            im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
+        }
+        for (int a = 1; a < na; ++a)
+            min_image_to_f(px[0], py[0], pz[0], &px[a], &py[a], &pz[a], L);
+        const uint8_t *li = ci + coff[g], *lj = cj + coff[g];
+        float dx[OR_SHAKE_MAXC], dy[OR_SHAKE_MAXC], dz[OR_SHAKE_MAXC], d2[OR_SHAKE_MAXC],
+            ip[OR_SHAKE_MAXC];
+        for (int k = 0; k < nc; ++k) {
+            dx[k] = px[li[k]] - px[lj[k]];
+            dy[k] = py[li[k]] - py[lj[k]];
+            dz[k] = pz[li[k]] - pz[lj[k]];
+            d2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+            ip[k] = im[li[k]] + im[lj[k]];
+        }
+        int it = 0;
+        for (; it < max_iter; ++it) {
+            int converged = 1;
+            for (int k = 0; k < nc; ++k) {
+                int a = li[k], b = lj[k];
+                float vxr = v0[a] - v0[b], vyr = v1[a] - v1[b], vzr = v2[a] - v2[b];
+                float vrel = vxr * dx[k] + vyr * dy[k] + vzr * dz[k];
+                if (fabsf(vrel) > tol) {
+                    converged = 0;
+                    float denom = d2[k] * ip[k];
+                    if (denom == 0.0f) continue;
+                    float mu = vrel / denom;
+                    float ki = mu * im[a], kj = mu * im[b];
+                    v0[a] -= ki * dx[k]; v1[a] -= ki * dy[k]; v2[a] -= ki * dz[k];
+                    v0[b] += kj * dx[k]; v1[b] += kj * dy[k]; v2[b] += kj * dz[k];
+                    dv0[a] -= ki * dx[k]; dv1[a] -= ki * dy[k]; dv2[a] -= ki * dz[k];
+                    dv0[b] += kj * dx[k]; dv1[b] += kj * dy[k]; dv2[b] += kj * dz[k];
+                }
+            }
+            if (converged) break;
+        }
+        if (iters) iters[g] = it;
+        if (dt > 0.0f) {
+            float inv_dt = 1.0f / dt;
+            float cx, cy, cz;
+            weighted_com_f(px, py, pz, m, na, &cx, &cy, &cz);
+            for (int a = 0; a < na; ++a) {
+                float rx = px[a] - cx, ry = py[a] - cy, rz = pz[a] - cz;
+                float scale = m[a] * inv_dt;
+                w_cons[at[a]] += scale * (dv0[a] * rx + dv1[a] * ry + dv2[a] * rz);
+            }
+        }
+        for (int a = 0; a < na; ++a) {
+            uint32_t i = at[a];
+            vx[i] = v0[a]; vy[i] = v1[a]; vz[i] = v2[a];
+        }
     }
 }
 

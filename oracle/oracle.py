@@ -56,6 +56,12 @@ def lib():
         L.or_spme_recip.argtypes = [C.c_int, _f32p, _f64p, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]
+        L.or_shake_positions_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, _f32p,
+                                             _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                             C.c_float, C.c_float, C.c_int, _f32p, _i32p]
+        L.or_rattle_velocities_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p,
+                                               _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                               C.c_float, C.c_float, C.c_int, _f32p, _i32p]
         L.or_philox4x32_10.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p]
         L.or_lan_alpha.restype = C.c_float
         L.or_lan_alpha.argtypes = [C.c_float, C.c_float]
@@ -217,6 +223,63 @@ def vv_kick_drift_f32(posq, img, v, f, mass, lattice, dt):
 def vv_kick_f32(v, f, mass, dt):
     n = v.shape[1]
     lib().or_vv_kick_f32(n, v[0], v[1], v[2], f[0], f[1], f[2], mass, np.float32(dt))
+
+
+SHAKE_TOL = 3.57e-6     # |r^2 - d^2| in Bohr^2 (shake.cu:251-256)
+RATTLE_TOL = 8.63e-17   # |v_rel . r| (shake.cu:446)
+SHAKE_MAX_ITER = 32
+
+
+class ShakeGroups:
+    """Constraint groups in the reference's flattened layout
+    (src/integrator/shake.rs:250-345): per group a run of atom ids and a
+    run of constraints (group-local i, j, r2 = (f32 d)^2)."""
+
+    def __init__(self, groups):
+        # groups: list of (atom_ids, [(local_i, local_j, d), ...])
+        goff, gat, coff, ci, cj, r2 = [0], [], [0], [], [], []
+        for atoms, cons in groups:
+            gat.extend(int(a) for a in atoms)
+            goff.append(len(gat))
+            for a, b, d in cons:
+                ci.append(a)
+                cj.append(b)
+                df = np.float32(d)
+                r2.append(df * df)
+            coff.append(len(ci))
+        self.goff = np.asarray(goff, np.uint32)
+        self.atoms = np.asarray(gat, np.uint32)
+        self.coff = np.asarray(coff, np.uint32)
+        self.ci = np.asarray(ci, np.uint8)
+        self.cj = np.asarray(cj, np.uint8)
+        self.r2 = np.asarray(r2, np.float32)
+        self.n = len(groups)
+
+
+def shake_positions(G, posq, v, snap, mass, lattice, dt, tol=SHAKE_TOL, max_iter=SHAKE_MAX_ITER):
+    """In-place SHAKE (shake.cu shake_positions), f32; returns (w_cons
+    per atom, sweeps per group)."""
+    w = np.zeros(posq.shape[0], np.float32)
+    iters = np.zeros(max(G.n, 1), np.int32)
+    lib().or_shake_positions_f32(G.n, G.goff, G.atoms, G.coff, G.ci, G.cj, G.r2,
+                                 posq.reshape(-1), v[0], v[1], v[2], snap.reshape(-1), mass,
+                                 np.ascontiguousarray(lattice, np.float32), np.float32(dt),
+                                 np.float32(tol), int(max_iter), w, iters)
+    return w, iters[:G.n]
+
+
+def rattle_velocities(G, posq, v, mass, lattice, dt, w_cons=None, tol=RATTLE_TOL,
+                      max_iter=SHAKE_MAX_ITER):
+    """In-place RATTLE (shake.cu rattle_velocities), f32; adds the
+    velocity-level virial to w_cons when dt > 0; returns sweeps per group."""
+    if w_cons is None:
+        w_cons = np.zeros(posq.shape[0], np.float32)
+    iters = np.zeros(max(G.n, 1), np.int32)
+    lib().or_rattle_velocities_f32(G.n, G.goff, G.atoms, G.coff, G.ci, G.cj,
+                                   posq.reshape(-1), v[0], v[1], v[2], mass,
+                                   np.ascontiguousarray(lattice, np.float32), np.float32(dt),
+                                   np.float32(tol), int(max_iter), w_cons, iters)
+    return iters[:G.n]
 
 
 def philox4x32_10(key, ctr):

@@ -332,3 +332,115 @@ def test_langevin_water_settle_thermalises():
         r = p[g[:, a], :3].astype(np.float64) - p[g[:, b], :3]
         r -= L * np.floor(r / L + 0.5)
         assert np.abs(np.linalg.norm(r, axis=1) / d - 1).max() < 2e-5
+
+
+def _vv_constrained_step_oracle(oracle, s, G, posq0, img0, v0, f0, f1, dt, w=None):
+    """The reference VV step with SHAKE hooks (run_step,
+    src/integrator/mod.rs:344-429): kick + drift with a snapshot, SHAKE,
+    RATTLE (dt), then the fused kick with the new forces."""
+    p = posq0.copy()
+    img = img0.T.copy()
+    v = v0.copy()
+    L32 = s.lattice.astype(np.float32)
+    snap = p.copy()
+    oracle.vv_kick_drift_f32(p, img, v, f0, s.mass, L32, dt)
+    wc, it_s = oracle.shake_positions(G, p, v, snap, s.mass, L32, dt)
+    it_r = oracle.rattle_velocities(G, p, v, s.mass, L32, dt, w_cons=wc)
+    oracle.vv_kick_f32(v, f1, s.mass, dt)
+    return p, img, v, it_s, it_r
+
+
+def test_shake_water_step_bit_exact(oracle):
+    # SPC water with general SHAKE groups in place of SETTLE: one step on
+    # the GPU equals the f32 restatement bit for bit (same operation order,
+    # correctly rounded division on both sides)
+    say("case: water 8192 SHAKE/RATTLE one VV step")
+    s = systems.water_as_shake(systems.config_water_8192())
+    kt = U.temperature(300.0)
+    from heddlemd_amd.rng import maxwell_boltzmann
+    v0 = maxwell_boltzmann(s.mass, kt, seed=21, n_constraints=s.n_constraints)
+    dt = np.float32(U.time(2e-15))
+    e = engine_for(s, dt=float(dt), velocities=v0)
+    e.use_graph(False)
+    e.constrain_velocities()
+    v0 = e.velocities()
+    e.force_eval(scalars=False)
+    f0 = e.forces().T.copy()
+    posq0, img0 = e.positions(images=True)
+    e.step(1)
+    posq1, img1 = e.positions(images=True)
+    v1 = e.velocities()
+    f1 = e.forces().T.copy()
+    st = e.constraint_stats()
+    e.close()
+    G = oracle.ShakeGroups(s.shake_groups)
+    p, img, v, it_s, it_r = _vv_constrained_step_oracle(oracle, s, G, posq0, img0, v0, f0, f1,
+                                                        dt)
+    say(f"  sweeps: shake max {st['shake_max']} (oracle {it_s.max()}), rattle max "
+        f"{st['rattle_max']} (oracle {it_r.max()})")
+    assert st["shake_sweeps"] == int(it_s.sum())
+    assert st["rattle_sweeps"] == int(it_r.sum())
+    assert np.array_equal(img.T, img1)
+    assert np.array_equal(p, posq1)
+    assert np.array_equal(v, v1)
+
+
+def test_shake_polymer_chains_step_bit_exact(oracle):
+    # 50-bead chains as single SHAKE groups (49 coupled constraints each,
+    # beyond the reference's per-group caps), force-free so only the
+    # integrator and the constraints act
+    say("case: 24 chains x 50 beads SHAKE step")
+    rng = np.random.default_rng(23)
+    n_ch, n_b, d = 24, 50, 2.89
+    L = 120.0
+    pos = []
+    for c in range(n_ch):
+        x = rng.uniform(-L / 4, L / 4, size=3)
+        chain = [x]
+        for k in range(1, n_b):
+            u = rng.normal(size=3)
+            chain.append(chain[-1] + d * u / np.linalg.norm(u))
+        pos.extend(chain)
+    pos = np.asarray(pos)
+    n = n_ch * n_b
+    posq = np.zeros((n, 4), np.float32)
+    posq[:, :3] = systems.wrap_into_box(pos, [L, L, L])
+    lj = systems.LJTable(ntypes=1, sigma=np.array([[7.46]]), epsilon=np.array([[0.0]]),
+                         cutoff=np.array([[18.9]]), r_switch=np.array([[18.9]]))
+    s = systems.System(name="chains", posq=posq, mass=np.full(n, 25520.0, np.float32),
+                       types=np.zeros(n, np.uint32),
+                       lattice=np.array([L, L, L, 0, 0, 0], np.float64), lj=lj,
+                       excl_key=np.repeat(np.arange(n_ch, dtype=np.uint32), n_b),
+                       shake_groups=[(list(range(c * n_b, (c + 1) * n_b)),
+                                      [(k, k + 1, d) for k in range(n_b - 1)])
+                                     for c in range(n_ch)])
+    v0 = rng.normal(0, 2.2e-4, size=(3, n)).astype(np.float32)
+    dt = np.float32(U.time(2e-15))
+    e = engine_for(s, dt=float(dt), velocities=v0)
+    e.set_shake_tolerance(5e-5, 1e-8, 200)
+    e.use_graph(False)
+    e.force_eval(scalars=False)
+    f0 = e.forces().T.copy()
+    posq0, img0 = e.positions(images=True)
+    e.step(1)
+    posq1, img1 = e.positions(images=True)
+    v1 = e.velocities()
+    f1 = e.forces().T.copy()
+    st = e.constraint_stats()
+    e.close()
+    assert np.abs(f0).max() == 0.0 and np.abs(f1).max() == 0.0
+    G = oracle.ShakeGroups(s.shake_groups)
+    p = posq0.copy()
+    img = img0.T.copy()
+    v = v0.copy()
+    L32 = s.lattice.astype(np.float32)
+    snap = p.copy()
+    oracle.vv_kick_drift_f32(p, img, v, f0, s.mass, L32, dt)
+    wc, it_s = oracle.shake_positions(G, p, v, snap, s.mass, L32, dt, tol=5e-5, max_iter=200)
+    it_r = oracle.rattle_velocities(G, p, v, s.mass, L32, dt, w_cons=wc, tol=1e-8, max_iter=200)
+    oracle.vv_kick_f32(v, f1, s.mass, dt)
+    say(f"  shake sweeps max {st['shake_max']} (oracle {it_s.max()}), capped {st['shake_capped']}")
+    assert st["shake_capped"] == 0 and it_s.max() < 200
+    assert st["shake_sweeps"] == int(it_s.sum())
+    assert np.array_equal(p, posq1)
+    assert np.array_equal(v, v1)
