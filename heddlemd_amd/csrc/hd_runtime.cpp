@@ -1684,4 +1684,18 @@ int hd_selftest_dpp(hd_ctx* c, int32_t* out) {
   return HD_OK;
 }
 
+int hd_selftest_mfma(hd_ctx* c, float* out) {
+  if (!c || !out) return HD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  float* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, 1024 * sizeof(float)));
+  hd_launch_selftest_mfma(c->stream, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, 1024 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "selftest: %s", hipGetErrorString(e));
+  return HD_OK;
+}
+
 }  // extern "C"

@@ -235,6 +235,11 @@ class Engine:
                   "hd_get_spme_grids")
         return rho.reshape(grid), v.reshape(grid)
 
+    def selftest_mfma(self):
+        out = np.zeros(1024, np.float32)
+        self._chk(self._lib.hd_selftest_mfma(self._ctx, out), "hd_selftest_mfma")
+        return out.reshape(64, 16)
+
     def selftest_dpp(self):
         out = np.zeros(64, np.int32)
         self._chk(self._lib.hd_selftest_dpp(self._ctx, out), "hd_selftest_dpp")

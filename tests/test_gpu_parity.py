@@ -50,6 +50,24 @@ def test_dpp_rotation_lane_layout():
     assert out.tolist() == [7 * ((l + 1) % 64) + 3 for l in range(64)]
 
 
+def test_mfma_f16_lane_layout():
+    # the split-precision DFT relies on this A/B/D lane layout of
+    # v_mfma_f32_32x32x8_f16 (exact small-integer products)
+    say("case: mfma selftest")
+    from heddlemd_amd.engine import Engine
+    e = Engine(0)
+    out = e.selftest_mfma()
+    e.close()
+    A = np.array([[(8 * i + k) % 61 - 30 for k in range(8)] for i in range(32)], np.float64)
+    B = np.array([[(32 * k + j) % 23 - 11 for j in range(32)] for k in range(8)], np.float64)
+    D = A @ B
+    want = np.zeros((64, 16))
+    for l in range(64):
+        for v in range(16):
+            want[l, v] = D[8 * (v // 4) + 4 * (l // 32) + v % 4, l % 32]
+    assert np.array_equal(out.astype(np.float64), want)
+
+
 def test_argon_forces_energy_virial(oracle):
     say("case: argon 10000 forces")
     s = _jittered_argon()
