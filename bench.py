@@ -114,10 +114,50 @@ def cpu_baseline(system, args):
     O.vv_kick_f32(v, f, system.mass, dt)
     t = time.perf_counter() - t0
     ns_day = args.dt_fs * 1e-6 * 86400.0 / t
+    ref = {"F": F + Fr, "Fr": Fr, "e_recip": er, "w_recip": wr}
     return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
             "sample": f"1 MD step (pair forces + SPME 128^3 + VV update) of the same "
                       f"{system.n}-atom state on 1 host thread of {os.cpu_count()} "
-                      f"({platform.processor() or platform.machine()}): {t:.2f} s"}
+                      f"({platform.processor() or platform.machine()}): {t:.2f} s"}, ref
+
+
+def precision_check(system, ref):
+    """Force / reciprocal-energy / virial error of both reciprocal transforms
+    (FP32 radix FFT, FP16 split matrix-core DFT) against the FP64 oracle on
+    the benchmark's initial state."""
+    import numpy as np
+    from heddlemd_amd.engine import engine_for
+
+    out = {}
+    F = ref["F"]
+    frms = float(np.sqrt(np.mean(np.sum(F ** 2, axis=1))))
+    for mode, name in ((0, "fp32_fft"), (1, "fp16_split_dft")):
+        e = engine_for(system, dt=1.0)
+        e.set_spme_dft(mode)
+        e.force_eval(scalars=True)
+        f = e.forces().astype(np.float64)
+        sc = e.scalars()
+        e.close()
+        d = np.linalg.norm(f - F, axis=1)
+        out[name] = {"force_err_rms_rel": float(np.sqrt(np.mean(d ** 2)) / frms),
+                     "force_err_max_rel": float(d.max() / frms),
+                     "e_recip_rel": float(abs(sc["e_recip"] / ref["e_recip"] - 1.0)),
+                     "w_recip_rel": float(abs(sc["w_recip"] / ref["w_recip"] - 1.0))}
+    return out
+
+
+def traffic_from_profile(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 pmc
+    summary (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "water196k_step_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("| `" + kernel_prefix):
+                    return float(line.rstrip().rstrip("|").split("|")[-1]) * 1e6
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -137,6 +177,8 @@ def main():
     ap.add_argument("--no-barostat", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--no-split-dft", action="store_true",
+                    help="skip the split-precision matrix-core DFT pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (the reference kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counter readings to bytes) and exit")
@@ -235,7 +277,9 @@ def main():
         result["roofline"] = {
             "kernel": "k_pair (fused LJ + real-space Ewald, packed wave64 list)",
             "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic_from_profile("void k_pair<true"),
+            "traffic_source": "profiles/water196k_step_traffic.md (rocprofv3 --pmc, calibrated)",
             "useful_pairs": pairs, "flops_per_pair": FLOPS_PER_PAIR,
             "mean_launch_ms": round(pair_ms, 4),
             "timing": f"hipEvents on the context stream over {args.steps} non-graph steps "
@@ -243,9 +287,36 @@ def main():
         }
         result["stages_ms"] = {k: round(v, 4) for k, v in stages}
         log(f"stages: {result['stages_ms']}")
+    # ---- precision form: the same production steps with the split-precision
+    # matrix-core DFT in place of the FP32 radix FFT ---------------------------
+    if not args.no_split_dft:
+        e.use_graph(True)
+        e.set_spme_dft(1)
+        e.step(args.warmup, scalars_last=True)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(args.steps, scalars_last=True)
+        e.sync()
+        el3 = time.perf_counter() - t0
+        e.use_graph(False)
+        e.set_timing(True)
+        e.step(min(args.steps, 100), scalars_last=True)
+        e.sync()
+        e.set_timing(False)
+        st3 = dict(e.stage_times())
+        e.set_spme_dft(0)
+        result["split_dft"] = {
+            "ns_per_day": round(args.dt_fs * 1e-6 * (args.steps / el3) * 86400.0, 3),
+            "ms_per_step": round(el3 / args.steps * 1e3, 4),
+            "spme_recip_ms": round(st3.get("spme_recip", float("nan")), 4),
+            "fp32_fft_spme_recip_ms": round(result.get("stages_ms", {}).get("spme_recip", float("nan")), 4),
+        }
+        log(f"split-precision DFT: {result['split_dft']}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle restatement) ...")
-        result["cpu_baseline"] = cpu_baseline(system, args)
+        result["cpu_baseline"], ref = cpu_baseline(system, args)
+        log("precision check against the FP64 oracle ...")
+        result["precision"] = precision_check(system, ref)
     e.close()
     print(json.dumps(result), flush=True)
 

@@ -39,6 +39,7 @@ SIGNATURES = {
     "hd_set_lj": (C.c_int, [_ctx, C.c_int, _f64p, _f64p, _f64p, _f64p]),
     "hd_set_spme": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.c_double]),
+    "hd_set_spme_dft": (C.c_int, [_ctx, C.c_int]),
     "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
     "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
                                 C.c_double]),

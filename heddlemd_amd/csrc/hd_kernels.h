@@ -69,6 +69,32 @@ struct HdSpmeArgs {
   int mv[3];  // voxel margin per axis: >= half the skin in voxel units, plus one
 };
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
+// the spread half of hd_launch_spme (spline records + charge grid) alone
+void hd_launch_spme_spread(hipStream_t st, const HdSpmeArgs& a);
+void hd_launch_recip_reduce(hipStream_t st, const HdSpmeArgs& a, unsigned n_partials);
+
+// Split-precision FP16 matrix-core DFT of the reciprocal grid (k_dft.hip)
+struct HdDftArgs {
+  unsigned na, nb, nc, nch;
+  const long long* rho;
+  float2* cgrid;
+  float* vgrid;
+  const float *bfa, *bfb, *bfc;
+  const float* lattice;
+  float kc, alpha;
+  double2* partials;
+};
+// DFT matrices, FP16 high and 2^12-scaled low parts, [M][K] row-major:
+// c-axis forward (M = 2 nch rounded up to 32, K = nc), c-axis inverse
+// (M = nc, K = 2 nch rounded up to 8), b and a axes forward / inverse
+// (M = K = 2n, the [[Cr, -Ci], [Ci, Cr]] block form)
+struct HdDftMats {
+  const _Float16 *cf_hi, *cf_lo, *ci_hi, *ci_lo;
+  unsigned cf_m, cf_k, ci_m, ci_k;
+  const _Float16 *bf_hi, *bf_lo, *bi_hi, *bi_lo, *af_hi, *af_lo, *ai_hi, *ai_lo;
+};
+void hd_launch_dft(hipStream_t st, const HdDftArgs& a, const HdDftMats& m);
+unsigned hd_dft_partials_count(unsigned nb, unsigned nc);
 unsigned hd_spme_partials_count(unsigned nb, unsigned nc);
 // brick shape of the owner-computes spread; returns the number of bricks
 unsigned hd_spread_bricks(unsigned na, unsigned nb, unsigned nc, unsigned brick[3]);

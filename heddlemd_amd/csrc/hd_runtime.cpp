@@ -117,6 +117,9 @@ struct hd_ctx {
   float2* cgrid = nullptr;
   float* vgrid = nullptr;
   float4* spl = nullptr;  // SPME spline records per slot
+  int dft_mode = 0;        // 0 FP32 radix FFT, 1 split-precision FP16 MFMA DFT
+  _Float16* dft_mats = nullptr;
+  HdDftMats dft;
   unsigned* sg = nullptr;
   float2 *tw_a = nullptr, *tw_b = nullptr, *tw_c = nullptr;
   float *bf_a = nullptr, *bf_b = nullptr, *bf_c = nullptr;
@@ -439,7 +442,29 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
       sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] / c->lattice[d]) + 1;
     }
     stage_begin(c, "spme_recip", timed);
-    hd_launch_spme(st, sa);
+    if (c->dft_mode == 1) {
+      hd_launch_spme_spread(st, sa);
+      HdDftArgs da;
+      memset(&da, 0, sizeof(da));
+      da.na = c->grid[0];
+      da.nb = c->grid[1];
+      da.nc = c->grid[2];
+      da.nch = c->grid[2] / 2 + 1;
+      da.rho = c->rho_fixed;
+      da.cgrid = c->cgrid;
+      da.vgrid = c->vgrid;
+      da.bfa = c->bf_a;
+      da.bfb = c->bf_b;
+      da.bfc = c->bf_c;
+      da.lattice = c->d_lattice;
+      da.kc = (float)c->kc;
+      da.alpha = (float)c->alpha;
+      da.partials = c->recip_partials;
+      hd_launch_dft(st, da, c->dft);
+      hd_launch_recip_reduce(st, sa, hd_dft_partials_count(c->grid[1], c->grid[2]));
+    } else {
+      hd_launch_spme(st, sa);
+    }
     stage_end(c, timed);
   }
   HdFinalizeArgs fa;
@@ -907,6 +932,100 @@ int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, d
 
 static void drop_graphs(hd_ctx* c);
 
+// Split one DFT matrix into FP16 high and 2^12-scaled low parts.
+static void dft_split_host(const std::vector<double>& w, _Float16* hi, _Float16* lo) {
+  for (size_t k = 0; k < w.size(); ++k) {
+    float x = (float)w[k];
+    _Float16 h = (_Float16)x;
+    hi[k] = h;
+    lo[k] = (_Float16)((x - (float)h) * 4096.0f);
+  }
+}
+
+// DFT matrices of the split-precision path (layouts in hd_kernels.h)
+static int build_dft_mats(hd_ctx* c) {
+  if (c->dft_mats) return HD_OK;
+  unsigned na = c->grid[0], nb = c->grid[1], nc = c->grid[2], nch = nc / 2 + 1;
+  if (2 * na > 256 || 2 * nb > 256 || nc > 256)
+    return fail(c, HD_ERR_UNSUPPORTED, "the matrix-core DFT handles axes up to 128 points");
+  unsigned cf_m = (2 * nch + 31) / 32 * 32, cf_k = nc;
+  unsigned ci_m = nc, ci_k = (2 * nch + 7) / 8 * 8;
+  size_t sz_cf = (size_t)cf_m * cf_k, sz_ci = (size_t)ci_m * ci_k;
+  size_t sz_b = (size_t)(2 * nb) * (2 * nb), sz_a = (size_t)(2 * na) * (2 * na);
+  size_t total = 2 * (sz_cf + sz_ci + 2 * sz_b + 2 * sz_a);
+  std::vector<_Float16> h(total);
+  const double two_pi = 2.0 * M_PI;
+  std::vector<double> w;
+  size_t off = 0;
+  auto put = [&](const std::vector<double>& m, const _Float16** hi, const _Float16** lo) {
+    dft_split_host(m, h.data() + off, h.data() + off + m.size());
+    *hi = (const _Float16*)(uintptr_t)off;  // offsets for now, pointers after upload
+    *lo = (const _Float16*)(uintptr_t)(off + m.size());
+    off += 2 * m.size();
+  };
+  // c-axis forward: rows k' < nch -> cos, nch <= m < 2 nch -> -sin
+  w.assign(sz_cf, 0.0);
+  for (unsigned m = 0; m < 2 * nch; ++m)
+    for (unsigned x = 0; x < nc; ++x) {
+      unsigned k = m < nch ? m : m - nch;
+      double th = two_pi * (double)((k * x) % nc) / nc;
+      w[(size_t)m * cf_k + x] = m < nch ? cos(th) : -sin(th);
+    }
+  put(w, &c->dft.cf_hi, &c->dft.cf_lo);
+  // c-axis inverse over the Hermitian half spectrum
+  w.assign(sz_ci, 0.0);
+  for (unsigned x = 0; x < nc; ++x)
+    for (unsigned k = 0; k < 2 * nch; ++k) {
+      unsigned kk = k < nch ? k : k - nch;
+      double wt = (kk == 0 || 2 * kk == nc) ? 1.0 : 2.0;
+      double th = two_pi * (double)((kk * x) % nc) / nc;
+      w[(size_t)x * ci_k + k] = k < nch ? wt * cos(th) : -wt * sin(th);
+    }
+  put(w, &c->dft.ci_hi, &c->dft.ci_lo);
+  // complex axes, [[Cr, -Ci], [Ci, Cr]] with C = exp(sign 2 pi i m k / n)
+  auto cplx = [&](unsigned n, double sign) {
+    std::vector<double> m2((size_t)(2 * n) * (2 * n), 0.0);
+    for (unsigned m = 0; m < n; ++m)
+      for (unsigned k = 0; k < n; ++k) {
+        double th = two_pi * (double)((m * k) % n) / n;
+        double cr = cos(th), ci = sign * sin(th);
+        m2[(size_t)m * 2 * n + k] = cr;
+        m2[(size_t)m * 2 * n + n + k] = -ci;
+        m2[(size_t)(n + m) * 2 * n + k] = ci;
+        m2[(size_t)(n + m) * 2 * n + n + k] = cr;
+      }
+    return m2;
+  };
+  put(cplx(nb, -1.0), &c->dft.bf_hi, &c->dft.bf_lo);
+  put(cplx(nb, +1.0), &c->dft.bi_hi, &c->dft.bi_lo);
+  put(cplx(na, -1.0), &c->dft.af_hi, &c->dft.af_lo);
+  put(cplx(na, +1.0), &c->dft.ai_hi, &c->dft.ai_lo);
+  c->dft.cf_m = cf_m; c->dft.cf_k = cf_k; c->dft.ci_m = ci_m; c->dft.ci_k = ci_k;
+  DALLOC(c, c->dft_mats, total);
+  int r = upload(c, c->dft_mats, h.data(), total);
+  if (r) return r;
+  const _Float16** ptrs[] = {&c->dft.cf_hi, &c->dft.cf_lo, &c->dft.ci_hi, &c->dft.ci_lo,
+                             &c->dft.bf_hi, &c->dft.bf_lo, &c->dft.bi_hi, &c->dft.bi_lo,
+                             &c->dft.af_hi, &c->dft.af_lo, &c->dft.ai_hi, &c->dft.ai_lo};
+  for (auto p : ptrs) *p = c->dft_mats + (size_t)(uintptr_t)(*p);
+  return HD_OK;
+}
+
+int hd_set_spme_dft(hd_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 1) return HD_ERR_ARG;
+  if (!c->have_spme) return fail(c, HD_ERR_STATE, "no SPME configured");
+  if (c->ready) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    if (mode == 1) {
+      int r = build_dft_mats(c);
+      if (r) return r;
+    }
+  }
+  c->dft_mode = mode;
+  return HD_OK;
+}
+
 int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* gatoms,
                  const uint32_t* coff, const uint8_t* ci, const uint8_t* cj, const double* d) {
   if (!c || (ng && (!goff || !gatoms || !coff || !ci || !cj || !d))) return HD_ERR_ARG;
@@ -1266,7 +1385,9 @@ int hd_setup(hd_ctx* c) {
     if ((r = upload(c, c->bf_a, ba.data(), ba.size()))) return r;
     if ((r = upload(c, c->bf_b, bb.data(), bb.size()))) return r;
     if ((r = upload(c, c->bf_c, bc.data(), bc.size()))) return r;
-    DALLOC(c, c->recip_partials, hd_spme_partials_count(c->grid[1], c->grid[2]));
+    DALLOC(c, c->recip_partials, std::max(hd_spme_partials_count(c->grid[1], c->grid[2]),
+                                          hd_dft_partials_count(c->grid[1], c->grid[2])));
+    if (c->dft_mode == 1 && (r = build_dft_mats(c))) return r;
   }
   if (c->have_nhc) {
     int m = c->nhc_chain;

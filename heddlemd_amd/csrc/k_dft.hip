@@ -42,3 +42,251 @@ __global__ void k_selftest_mfma(float* out) {
 void hd_launch_selftest_mfma(hipStream_t st, float* out) {
   hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, st, out);
 }
+
+// ---------------------------------------------------------------------
+// DFT passes as GEMMs.  A workgroup takes DFT_NL lines (the GEMM's N),
+// stages them in LDS as FP16 high/low parts of the line scaled by 2^-s
+// (s chosen from the tile's largest magnitude), and its four waves each
+// produce 32-row tiles of Y = W X (M rows; K = input length):
+//   main = W_hi X_hi,  corr = W_hi X_lo' + W_lo' X_hi,
+//   Y = (main + corr 2^-12) 2^s,
+// where the low parts are stored scaled by 2^12 (X_lo' = 2^12 (x - x_hi),
+// W_lo' = 2^12 (W - W_hi)) so that they stay in FP16's normal range.
+//
+// Passes over the (na, nb, nch = nc/2 + 1) complex grid, forward sign -1,
+// inverse +1, unnormalised (the radix FFT's convention):
+//   C_FWD: rows of the i64 spread grid (real, length nc) -> 2 nch outputs
+//          (real parts, then imaginary parts);
+//   CPLX:  complex lines along b or a, [re; im] stacked (K = M = 2n) with
+//          the block matrix [[Cr, -Ci], [Ci, Cr]];
+//   A_CONV: forward a-axis, influence multiply with the energy / virial
+//          partials, inverse a-axis, in one workgroup pass;
+//   C_INV: Hermitian half spectrum (K = 2 nch) -> nc real values.
+// ---------------------------------------------------------------------
+#define DFT_NL 32
+#define DFT_THREADS 256
+#define DFT_KMAX 256
+#define DFT_KP (DFT_KMAX + 8)  // LDS line pitch in halves
+#define DFT_LO_SCALE 4096.0f
+#define DFT_LO_UNSCALE (1.0f / 4096.0f)
+
+enum { DFT_C_FWD = 0, DFT_CPLX_B = 1, DFT_CPLX_A = 2, DFT_A_CONV = 3, DFT_C_INV = 4 };
+
+struct DftMat {
+  const _Float16 *hi, *lo;  // [M][K] row-major
+  unsigned M, K;
+};
+
+// element (line, k) of the pass input
+template <int MODE>
+__device__ __forceinline__ float dft_load(const HdDftArgs& a, unsigned line, unsigned k) {
+  if (MODE == DFT_C_FWD) {
+    if (line >= a.na * a.nb || k >= a.nc) return 0.f;
+    long long v = a.rho[(size_t)line * a.nc + k];
+    return (float)v * (1.0f / 4294967296.0f);
+  } else if (MODE == DFT_C_INV) {
+    if (line >= a.na * a.nb || k >= 2 * a.nch) return 0.f;
+    unsigned kk = k < a.nch ? k : k - a.nch;
+    float2 v = a.cgrid[(size_t)line * a.nch + kk];
+    return k < a.nch ? v.x : v.y;
+  } else {
+    unsigned n = (MODE == DFT_CPLX_B) ? a.nb : a.na;
+    unsigned nl = ((MODE == DFT_CPLX_B) ? a.na : a.nb) * a.nch;
+    if (line >= nl || k >= 2 * n) return 0.f;
+    unsigned o = line / a.nch, cc = line - o * a.nch, e = k < n ? k : k - n;
+    size_t idx = (MODE == DFT_CPLX_B) ? ((size_t)o * a.nb + e) * a.nch + cc
+                                      : ((size_t)e * a.nb + o) * a.nch + cc;
+    float2 v = a.cgrid[idx];
+    return k < n ? v.x : v.y;
+  }
+}
+
+// stage s_x[line][k] (f32, already in LDS) as FP16 high/low parts with a
+// common power-of-two scale; returns the exponent s
+__device__ __forceinline__ int dft_split(const float* s_x, _Float16* xh, _Float16* xl,
+                                         unsigned K, unsigned* s_max) {
+  unsigned t = threadIdx.x;
+  if (t == 0) *s_max = 0u;
+  __syncthreads();
+  unsigned mx = 0u;
+  for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
+    unsigned line = e / K, k = e - line * K;
+    mx = max(mx, __float_as_uint(fabsf(s_x[line * DFT_KMAX + k])));
+  }
+  atomicMax(s_max, mx);
+  __syncthreads();
+  float vmax = __uint_as_float(*s_max);
+  int ex = 0;
+  if (vmax > 0.f) (void)frexpf(vmax, &ex);
+  int s = ex - 14;  // |x| 2^-s < 2^14
+  for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
+    unsigned line = e / K, k = e - line * K;
+    float x = ldexpf(s_x[line * DFT_KMAX + k], -s);
+    _Float16 h = (_Float16)x;
+    _Float16 l = (_Float16)((x - (float)h) * DFT_LO_SCALE);
+    xh[line * DFT_KP + k] = h;
+    xl[line * DFT_KP + k] = l;
+  }
+  __syncthreads();
+  return s;
+}
+
+// one 32-row tile of Y = W X for the staged lines; acc in D layout
+__device__ __forceinline__ void dft_tile(const DftMat& w, unsigned mt, const _Float16* xh,
+                                         const _Float16* xl, hd_float16v& main,
+                                         hd_float16v& corr) {
+  unsigned lane = threadIdx.x & 63u;
+  unsigned row = mt * 32u + (lane & 31u);
+  unsigned kq = 4u * (lane >> 5);
+  unsigned line = lane & 31u;
+  for (int v = 0; v < 16; ++v) { main[v] = 0.f; corr[v] = 0.f; }
+  const _Float16* wh = w.hi + (size_t)row * w.K;
+  const _Float16* wl = w.lo + (size_t)row * w.K;
+  for (unsigned k0 = 0; k0 < w.K; k0 += 8u) {
+    hd_half4 ah = *reinterpret_cast<const hd_half4*>(wh + k0 + kq);
+    hd_half4 al = *reinterpret_cast<const hd_half4*>(wl + k0 + kq);
+    hd_half4 bh = *reinterpret_cast<const hd_half4*>(xh + line * DFT_KP + k0 + kq);
+    hd_half4 bl = *reinterpret_cast<const hd_half4*>(xl + line * DFT_KP + k0 + kq);
+    main = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh, main, 0, 0, 0);
+    corr = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl, corr, 0, 0, 0);
+    corr = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh, corr, 0, 0, 0);
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void dft_store(const HdDftArgs& a, unsigned line, unsigned m,
+                                          float y) {
+  if (MODE == DFT_C_FWD) {
+    if (line >= a.na * a.nb || m >= 2 * a.nch) return;
+    float* p = reinterpret_cast<float*>(a.cgrid + (size_t)line * a.nch + (m < a.nch ? m : m - a.nch));
+    p[m < a.nch ? 0 : 1] = y;
+  } else if (MODE == DFT_C_INV) {
+    if (line >= a.na * a.nb || m >= a.nc) return;
+    a.vgrid[(size_t)line * a.nc + m] = y;
+  } else {
+    bool along_b = (MODE == DFT_CPLX_B);
+    unsigned n = along_b ? a.nb : a.na;
+    unsigned nl = (along_b ? a.na : a.nb) * a.nch;
+    if (line >= nl || m >= 2 * n) return;
+    unsigned o = line / a.nch, cc = line - o * a.nch, e = m < n ? m : m - n;
+    size_t idx = along_b ? ((size_t)o * a.nb + e) * a.nch + cc : ((size_t)e * a.nb + o) * a.nch + cc;
+    float* p = reinterpret_cast<float*>(a.cgrid + idx);
+    p[m < n ? 0 : 1] = y;
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(DFT_THREADS) k_dft(HdDftArgs a, DftMat w, DftMat w2) {
+  __shared__ float s_x[DFT_NL * DFT_KMAX];
+  __shared__ _Float16 xh[DFT_NL * DFT_KP], xl[DFT_NL * DFT_KP];
+  __shared__ unsigned s_max;
+  __shared__ double red_e[DFT_THREADS], red_w[DFT_THREADS];
+  unsigned t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+  unsigned line0 = blockIdx.x * DFT_NL;
+  unsigned K = w.K;
+  // coalesced staging: consecutive threads take consecutive lines
+  for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
+    unsigned ln = e % DFT_NL, k = e / DFT_NL;
+    s_x[ln * DFT_KMAX + k] = dft_load<MODE>(a, line0 + ln, k);
+  }
+  __syncthreads();
+  int s = dft_split(s_x, xh, xl, K, &s_max);
+  float sc = ldexpf(1.0f, s);
+  unsigned ntiles = w.M / 32u;
+  for (unsigned mt = wave; mt < ntiles; mt += 4u) {
+    hd_float16v main, corr;
+    dft_tile(w, mt, xh, xl, main, corr);
+    for (unsigned v = 0; v < 16; ++v) {
+      unsigned m = mt * 32u + mfma_d_row(lane, v);
+      float y = (main[v] + corr[v] * DFT_LO_UNSCALE) * sc;
+      if (MODE == DFT_A_CONV) s_x[(lane & 31u) * DFT_KMAX + m] = y;
+      else dft_store<MODE>(a, line0 + (lane & 31u), m, y);
+    }
+  }
+  if (MODE != DFT_A_CONV) return;
+  __syncthreads();
+  // influence multiply on the forward a-spectrum of the staged lines
+  // (spme_recip.cu:34-115, 296-342), energy/virial partials in FP64
+  {
+    HdLattice L = hd_load_lattice(a.lattice);
+    float v_box = L.lx * L.ly * L.lz;
+    float four_alpha2 = 4.0f * a.alpha * a.alpha;
+    const float pi_r = 3.14159265358979f;
+    float four_pi = 4.0f * pi_r, two_pi = 2.0f * pi_r;
+    float pref = a.kc / v_box;
+    float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
+    float ra_x = ilx, ra_y = -L.xy * ilx * ily, ra_z = (L.xy * L.yz - L.xz * L.ly) * ilx * ily * ilz;
+    float rb_y = ily, rb_z = -L.yz * ily * ilz, rc_z = ilz;
+    unsigned na = a.na, nl = a.nb * a.nch;
+    double se = 0.0, sw = 0.0;
+    for (unsigned e = t; e < DFT_NL * na; e += DFT_THREADS) {
+      unsigned ln = e / na, ka = e - ln * na;
+      unsigned line = line0 + ln;
+      if (line >= nl) continue;
+      unsigned b = line / a.nch, kcc = line - b * a.nch;
+      float ma = (ka <= na / 2) ? (float)ka : (float)ka - (float)na;
+      float mb = (b <= a.nb / 2) ? (float)b : (float)b - (float)a.nb;
+      float mc = (kcc <= a.nc / 2) ? (float)kcc : (float)kcc - (float)a.nc;
+      float kx = two_pi * (ma * ra_x);
+      float ky = two_pi * (ma * ra_y + mb * rb_y);
+      float kz = two_pi * (ma * ra_z + mb * rb_z + mc * rc_z);
+      float k2 = kx * kx + ky * ky + kz * kz;
+      float g = 0.f, vf = 0.f;
+      if (k2 != 0.f) {
+        g = pref * (four_pi / k2) * expf(-k2 / four_alpha2) * a.bfa[ka] * a.bfb[b] * a.bfc[kcc];
+        vf = g * (1.0f - k2 / (2.0f * a.alpha * a.alpha));
+      }
+      float yr = s_x[ln * DFT_KMAX + ka], yi = s_x[ln * DFT_KMAX + na + ka];
+      float rsq = yr * yr + yi * yi;
+      double hw = (kcc == 0 || 2 * kcc == a.nc) ? 1.0 : 2.0;
+      se += hw * (double)g * (double)rsq;
+      sw += hw * (double)vf * (double)rsq;
+      s_x[ln * DFT_KMAX + ka] = g * yr;
+      s_x[ln * DFT_KMAX + na + ka] = g * yi;
+    }
+    red_e[t] = se;
+    red_w[t] = sw;
+    __syncthreads();
+    for (unsigned st = 1; st < DFT_THREADS; st <<= 1) {
+      if ((t % (2 * st)) == 0) {
+        red_e[t] += red_e[t + st];
+        red_w[t] += red_w[t + st];
+      }
+      __syncthreads();
+    }
+    if (t == 0) a.partials[blockIdx.x] = make_double2(red_e[0], red_w[0]);
+  }
+  s = dft_split(s_x, xh, xl, w2.K, &s_max);
+  sc = ldexpf(1.0f, s);
+  ntiles = w2.M / 32u;
+  for (unsigned mt = wave; mt < ntiles; mt += 4u) {
+    hd_float16v main, corr;
+    dft_tile(w2, mt, xh, xl, main, corr);
+    for (unsigned v = 0; v < 16; ++v) {
+      unsigned m = mt * 32u + mfma_d_row(lane, v);
+      dft_store<DFT_CPLX_A>(a, line0 + (lane & 31u), m, (main[v] + corr[v] * DFT_LO_UNSCALE) * sc);
+    }
+  }
+}
+
+unsigned hd_dft_partials_count(unsigned nb, unsigned nc) {
+  return (nb * (nc / 2 + 1) + DFT_NL - 1) / DFT_NL;
+}
+
+void hd_launch_dft(hipStream_t st, const HdDftArgs& a, const HdDftMats& m) {
+  unsigned rows = a.na * a.nb;
+  unsigned lb = a.na * a.nch, la = a.nb * a.nch;
+  DftMat cf = {m.cf_hi, m.cf_lo, m.cf_m, m.cf_k}, ci = {m.ci_hi, m.ci_lo, m.ci_m, m.ci_k};
+  DftMat bf = {m.bf_hi, m.bf_lo, 2 * a.nb, 2 * a.nb}, bi = {m.bi_hi, m.bi_lo, 2 * a.nb, 2 * a.nb};
+  DftMat af = {m.af_hi, m.af_lo, 2 * a.na, 2 * a.na}, ai = {m.ai_hi, m.ai_lo, 2 * a.na, 2 * a.na};
+  hipLaunchKernelGGL(k_dft<DFT_C_FWD>, dim3((rows + DFT_NL - 1) / DFT_NL), dim3(DFT_THREADS), 0,
+                     st, a, cf, cf);
+  hipLaunchKernelGGL(k_dft<DFT_CPLX_B>, dim3((lb + DFT_NL - 1) / DFT_NL), dim3(DFT_THREADS), 0,
+                     st, a, bf, bf);
+  hipLaunchKernelGGL(k_dft<DFT_A_CONV>, dim3((la + DFT_NL - 1) / DFT_NL), dim3(DFT_THREADS), 0,
+                     st, a, af, ai);
+  hipLaunchKernelGGL(k_dft<DFT_CPLX_B>, dim3((lb + DFT_NL - 1) / DFT_NL), dim3(DFT_THREADS), 0,
+                     st, a, bi, bi);
+  hipLaunchKernelGGL(k_dft<DFT_C_INV>, dim3((rows + DFT_NL - 1) / DFT_NL), dim3(DFT_THREADS), 0,
+                     st, a, ci, ci);
+}

@@ -525,8 +525,7 @@ static unsigned ilog2u(unsigned v) {
   return r;
 }
 
-void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
-  unsigned nch = a.nc / 2 + 1;
+void hd_launch_spme_spread(hipStream_t st, const HdSpmeArgs& a) {
   unsigned brick[3];
   unsigned nbricks = hd_spread_bricks(a.na, a.nb, a.nc, brick);
   if (a.n > 0)
@@ -536,6 +535,16 @@ void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
                      a.nb, a.nc, brick[0], brick[1], brick[2], a.voxel_rank, a.voxel_off,
                      a.nv[0], a.nv[1], a.nv[2], a.mv[0], a.mv[1], a.mv[2],
                      (unsigned long long*)a.rho_fixed);
+}
+
+void hd_launch_recip_reduce(hipStream_t st, const HdSpmeArgs& a, unsigned n_partials) {
+  hipLaunchKernelGGL(k_recip_reduce, dim3(1), dim3(256), 0, st, a.partials, n_partials, a.recip,
+                     a.w_share, a.n);
+}
+
+void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
+  unsigned nch = a.nc / 2 + 1;
+  hd_launch_spme_spread(st, a);
   unsigned nrows = a.na * a.nb;
   unsigned h = a.nc / 2;
   unsigned rows = FFT_TILE / h;

@@ -101,6 +101,10 @@ class Engine:
                                             float(compressibility), float(kt), int(seed)),
                   "hd_set_crescale")
 
+    def set_spme_dft(self, mode):
+        """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT."""
+        self._chk(self._lib.hd_set_spme_dft(self._ctx, int(mode)), "hd_set_spme_dft")
+
     def set_shake(self, groups):
         """groups: list of (atom_ids, [(local_i, local_j, d), ...])."""
         goff, gat, coff, ci, cj, d = [0], [], [0], [], [], []

@@ -67,6 +67,12 @@ int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilo
  * two >= 8; order must be 4. */
 int hd_set_spme(hd_ctx *ctx, double alpha, double r_cut, int na, int nb, int nc, int order,
                 double k_c);
+/* Reciprocal-space transform: mode 0 = FP32 radix FFT in LDS (default),
+ * mode 1 = split-precision FP16 matrix-core DFT (each 1D stage a GEMM with
+ * high/low FP16 operands accumulated in FP32; axes up to 128 points).  The
+ * reference uses cuFFT in FP32 (src/gpu/cufft.rs); mode 1 has no
+ * counterpart there.  May be switched between phases. */
+int hd_set_spme_dft(hd_ctx *ctx, int mode);
 /* Cell-list neighbour skin (src/io/config.rs:1232, default 0.3 * r_cut). */
 int hd_set_neighbor(hd_ctx *ctx, double r_skin);
 /* Rigid three-site water via SETTLE (src/integrator/settle.rs):

@@ -462,3 +462,34 @@ def test_shake_polymer_chains_step_bit_exact(oracle):
     assert st["shake_sweeps"] == int(it_s.sum())
     assert np.array_equal(p, posq1)
     assert np.array_equal(v, v1)
+
+
+def test_split_precision_dft_matches_fp64(oracle):
+    # the FP16 split matrix-core DFT path against the FP64 oracle, next to
+    # the FP32 radix FFT path on the same state: potential grid, reciprocal
+    # energy and virial, and total forces
+    say("case: water 8192 split-precision DFT")
+    s = systems.config_water_8192()
+    out = {}
+    for mode in (0, 1):
+        e = _water_engine(s)
+        e.set_spme_dft(mode)
+        e.force_eval(scalars=True)
+        out[mode] = (e.forces(), e.scalars(), e.spme_grids(s.spme.grid)[1])
+        e.close()
+    Fp, Ep, Wp, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice,
+                                          coul=dict(alpha=s.spme.alpha, r_cut=s.spme.r_cut),
+                                          excl=s.exclusion_csr(), band=1e-6)
+    Fr, Er, Wr, er, wr, V = oracle.spme_recip(s.posq, s.lattice, s.spme.grid, 4, s.spme.alpha,
+                                              want_v=True)
+    for mode, name in ((0, "fp32 fft"), (1, "fp16 split dft")):
+        f, sc, v_g = out[mode]
+        rms, mx = _force_errors(f, Fp + Fr, band)
+        frr = np.abs(Fr).max()
+        v_err = float(np.abs(v_g - V).max() / np.abs(V).max())
+        say(f"  {name}: force rms {rms:.3e} max {mx:.3e} (of F_rms), V err {v_err:.3e}, "
+            f"E_recip rel {abs(sc['e_recip'] / er - 1):.3e}, W_recip rel {abs(sc['w_recip'] / wr - 1):.3e}")
+        assert rms <= 5e-6 and mx <= 5e-5
+        assert v_err <= 2e-5
+        assert sc["e_recip"] == pytest.approx(er, rel=2e-6)
+        assert sc["w_recip"] == pytest.approx(wr, rel=2e-6)
