@@ -946,8 +946,8 @@ static void dft_split_host(const std::vector<double>& w, _Float16* hi, _Float16*
 static int build_dft_mats(hd_ctx* c) {
   if (c->dft_mats) return HD_OK;
   unsigned na = c->grid[0], nb = c->grid[1], nc = c->grid[2], nch = nc / 2 + 1;
-  if (2 * na > 256 || 2 * nb > 256 || nc > 256)
-    return fail(c, HD_ERR_UNSUPPORTED, "the matrix-core DFT handles axes up to 128 points");
+  if (2 * na > 256 || 2 * nb > 256 || nc > 256 || na < 32 || nb < 32 || nc < 32)
+    return fail(c, HD_ERR_UNSUPPORTED, "the matrix-core DFT handles axes of 32 to 128 points");
   unsigned cf_m = (2 * nch + 31) / 32 * 32, cf_k = nc;
   unsigned ci_m = nc, ci_k = (2 * nch + 7) / 8 * 8;
   size_t sz_cf = (size_t)cf_m * cf_k, sz_ci = (size_t)ci_m * ci_k;

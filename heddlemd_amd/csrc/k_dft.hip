@@ -66,7 +66,8 @@ void hd_launch_selftest_mfma(hipStream_t st, float* out) {
 #define DFT_NL 32
 #define DFT_THREADS 256
 #define DFT_KMAX 256
-#define DFT_KP (DFT_KMAX + 8)  // LDS line pitch in halves
+#define DFT_KP (DFT_KMAX + 4)  // FP16 line pitch (halves): 130 dwords, 2-way at most
+#define DFT_XP (DFT_KMAX + 1)  // f32 line pitch: odd, so column walks hit distinct banks
 #define DFT_LO_SCALE 4096.0f
 #define DFT_LO_UNSCALE (1.0f / 4096.0f)
 
@@ -77,27 +78,56 @@ struct DftMat {
   unsigned M, K;
 };
 
-// element (line, k) of the pass input
+// Global index (complex units) of element e of line `line` for the passes
+// along b (lines = (a, c')) and along a (lines = (b, c')).
+template <bool ALONG_B>
+__device__ __forceinline__ size_t dft_cidx(const HdDftArgs& a, unsigned line, unsigned e) {
+  unsigned o = line / a.nch, cc = line - o * a.nch;
+  return ALONG_B ? ((size_t)o * a.nb + e) * a.nch + cc : ((size_t)e * a.nb + o) * a.nch + cc;
+}
+
+// Stage the workgroup's lines into s_x[line][k] (f32, zero beyond the
+// valid range), with coalesced global reads: rows for the c-axis passes,
+// consecutive c' for the b / a passes (real and imaginary parts of one
+// complex element land in rows k and n + k).
 template <int MODE>
-__device__ __forceinline__ float dft_load(const HdDftArgs& a, unsigned line, unsigned k) {
+__device__ __forceinline__ void dft_stage(const HdDftArgs& a, unsigned line0, unsigned K,
+                                          float* s_x) {
+  unsigned t = threadIdx.x;
   if (MODE == DFT_C_FWD) {
-    if (line >= a.na * a.nb || k >= a.nc) return 0.f;
-    long long v = a.rho[(size_t)line * a.nc + k];
-    return (float)v * (1.0f / 4294967296.0f);
+    unsigned nrow = a.na * a.nb;
+    for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
+      unsigned ln = e / K, k = e - ln * K;
+      float v = 0.f;
+      if (line0 + ln < nrow && k < a.nc)
+        v = (float)a.rho[(size_t)(line0 + ln) * a.nc + k] * (1.0f / 4294967296.0f);
+      s_x[ln * DFT_XP + k] = v;
+    }
   } else if (MODE == DFT_C_INV) {
-    if (line >= a.na * a.nb || k >= 2 * a.nch) return 0.f;
-    unsigned kk = k < a.nch ? k : k - a.nch;
-    float2 v = a.cgrid[(size_t)line * a.nch + kk];
-    return k < a.nch ? v.x : v.y;
+    unsigned nrow = a.na * a.nb;
+    for (unsigned e = t; e < DFT_NL * a.nch; e += DFT_THREADS) {
+      unsigned ln = e / a.nch, k = e - ln * a.nch;
+      float2 v = make_float2(0.f, 0.f);
+      if (line0 + ln < nrow) v = a.cgrid[(size_t)(line0 + ln) * a.nch + k];
+      s_x[ln * DFT_XP + k] = v.x;
+      s_x[ln * DFT_XP + a.nch + k] = v.y;
+    }
+    unsigned pad = K - 2 * a.nch;
+    for (unsigned e = t; e < DFT_NL * pad; e += DFT_THREADS) {
+      unsigned ln = e / pad, k = 2 * a.nch + (e - ln * pad);
+      s_x[ln * DFT_XP + k] = 0.f;
+    }
   } else {
-    unsigned n = (MODE == DFT_CPLX_B) ? a.nb : a.na;
-    unsigned nl = ((MODE == DFT_CPLX_B) ? a.na : a.nb) * a.nch;
-    if (line >= nl || k >= 2 * n) return 0.f;
-    unsigned o = line / a.nch, cc = line - o * a.nch, e = k < n ? k : k - n;
-    size_t idx = (MODE == DFT_CPLX_B) ? ((size_t)o * a.nb + e) * a.nch + cc
-                                      : ((size_t)e * a.nb + o) * a.nch + cc;
-    float2 v = a.cgrid[idx];
-    return k < n ? v.x : v.y;
+    constexpr bool along_b = (MODE == DFT_CPLX_B);
+    unsigned n = along_b ? a.nb : a.na;
+    unsigned nl = (along_b ? a.na : a.nb) * a.nch;
+    for (unsigned e = t; e < DFT_NL * n; e += DFT_THREADS) {
+      unsigned ln = e % DFT_NL, kk = e / DFT_NL;
+      float2 v = make_float2(0.f, 0.f);
+      if (line0 + ln < nl) v = a.cgrid[dft_cidx<along_b>(a, line0 + ln, kk)];
+      s_x[ln * DFT_XP + kk] = v.x;
+      s_x[ln * DFT_XP + n + kk] = v.y;
+    }
   }
 }
 
@@ -111,9 +141,10 @@ __device__ __forceinline__ int dft_split(const float* s_x, _Float16* xh, _Float1
   unsigned mx = 0u;
   for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
     unsigned line = e / K, k = e - line * K;
-    mx = max(mx, __float_as_uint(fabsf(s_x[line * DFT_KMAX + k])));
+    mx = max(mx, __float_as_uint(fabsf(s_x[line * DFT_XP + k])));
   }
-  atomicMax(s_max, mx);
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off));
+  if ((t & 63u) == 0) atomicMax(s_max, mx);
   __syncthreads();
   float vmax = __uint_as_float(*s_max);
   int ex = 0;
@@ -121,7 +152,7 @@ __device__ __forceinline__ int dft_split(const float* s_x, _Float16* xh, _Float1
   int s = ex - 14;  // |x| 2^-s < 2^14
   for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
     unsigned line = e / K, k = e - line * K;
-    float x = ldexpf(s_x[line * DFT_KMAX + k], -s);
+    float x = ldexpf(s_x[line * DFT_XP + k], -s);
     _Float16 h = (_Float16)x;
     _Float16 l = (_Float16)((x - (float)h) * DFT_LO_SCALE);
     xh[line * DFT_KP + k] = h;
@@ -131,7 +162,10 @@ __device__ __forceinline__ int dft_split(const float* s_x, _Float16* xh, _Float1
   return s;
 }
 
-// one 32-row tile of Y = W X for the staged lines; acc in D layout
+// one 32-row tile of Y = W X for the staged lines; acc in D layout.  The
+// operand fragments of DFT_KB consecutive K-steps are loaded before their
+// MFMAs are issued, so the W reads (L2) overlap instead of serialising.
+#define DFT_KB 8
 __device__ __forceinline__ void dft_tile(const DftMat& w, unsigned mt, const _Float16* xh,
                                          const _Float16* xl, hd_float16v& main,
                                          hd_float16v& corr) {
@@ -140,70 +174,120 @@ __device__ __forceinline__ void dft_tile(const DftMat& w, unsigned mt, const _Fl
   unsigned kq = 4u * (lane >> 5);
   unsigned line = lane & 31u;
   for (int v = 0; v < 16; ++v) { main[v] = 0.f; corr[v] = 0.f; }
-  const _Float16* wh = w.hi + (size_t)row * w.K;
-  const _Float16* wl = w.lo + (size_t)row * w.K;
-  for (unsigned k0 = 0; k0 < w.K; k0 += 8u) {
-    hd_half4 ah = *reinterpret_cast<const hd_half4*>(wh + k0 + kq);
-    hd_half4 al = *reinterpret_cast<const hd_half4*>(wl + k0 + kq);
-    hd_half4 bh = *reinterpret_cast<const hd_half4*>(xh + line * DFT_KP + k0 + kq);
-    hd_half4 bl = *reinterpret_cast<const hd_half4*>(xl + line * DFT_KP + k0 + kq);
+  const _Float16* wh = w.hi + (size_t)row * w.K + kq;
+  const _Float16* wl = w.lo + (size_t)row * w.K + kq;
+  const _Float16* bxh = xh + line * DFT_KP + kq;
+  const _Float16* bxl = xl + line * DFT_KP + kq;
+  unsigned k0 = 0;
+  for (; k0 + 8u * DFT_KB <= w.K; k0 += 8u * DFT_KB) {
+    hd_half4 ah[DFT_KB], al[DFT_KB], bh[DFT_KB], bl[DFT_KB];
+#pragma unroll
+    for (int u = 0; u < DFT_KB; ++u) {
+      ah[u] = *reinterpret_cast<const hd_half4*>(wh + k0 + 8u * u);
+      al[u] = *reinterpret_cast<const hd_half4*>(wl + k0 + 8u * u);
+    }
+#pragma unroll
+    for (int u = 0; u < DFT_KB; ++u) {
+      bh[u] = *reinterpret_cast<const hd_half4*>(bxh + k0 + 8u * u);
+      bl[u] = *reinterpret_cast<const hd_half4*>(bxl + k0 + 8u * u);
+    }
+#pragma unroll
+    for (int u = 0; u < DFT_KB; ++u) {
+      main = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[u], bh[u], main, 0, 0, 0);
+      corr = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[u], bl[u], corr, 0, 0, 0);
+      corr = __builtin_amdgcn_mfma_f32_32x32x8f16(al[u], bh[u], corr, 0, 0, 0);
+    }
+  }
+  for (; k0 < w.K; k0 += 8u) {
+    hd_half4 ah = *reinterpret_cast<const hd_half4*>(wh + k0);
+    hd_half4 al = *reinterpret_cast<const hd_half4*>(wl + k0);
+    hd_half4 bh = *reinterpret_cast<const hd_half4*>(bxh + k0);
+    hd_half4 bl = *reinterpret_cast<const hd_half4*>(bxl + k0);
     main = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh, main, 0, 0, 0);
     corr = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl, corr, 0, 0, 0);
     corr = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh, corr, 0, 0, 0);
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void dft_store(const HdDftArgs& a, unsigned line, unsigned m,
-                                          float y) {
-  if (MODE == DFT_C_FWD) {
-    if (line >= a.na * a.nb || m >= 2 * a.nch) return;
-    float* p = reinterpret_cast<float*>(a.cgrid + (size_t)line * a.nch + (m < a.nch ? m : m - a.nch));
-    p[m < a.nch ? 0 : 1] = y;
-  } else if (MODE == DFT_C_INV) {
-    if (line >= a.na * a.nb || m >= a.nc) return;
-    a.vgrid[(size_t)line * a.nc + m] = y;
-  } else {
-    bool along_b = (MODE == DFT_CPLX_B);
-    unsigned n = along_b ? a.nb : a.na;
-    unsigned nl = (along_b ? a.na : a.nb) * a.nch;
-    if (line >= nl || m >= 2 * n) return;
-    unsigned o = line / a.nch, cc = line - o * a.nch, e = m < n ? m : m - n;
-    size_t idx = along_b ? ((size_t)o * a.nb + e) * a.nch + cc : ((size_t)e * a.nb + o) * a.nch + cc;
-    float* p = reinterpret_cast<float*>(a.cgrid + idx);
-    p[m < n ? 0 : 1] = y;
+// complex passes: wave w owns the real-part tile w and the imaginary-part
+// tile w + n/32 of the same frequencies, so each lane holds whole complex
+// outputs; with TO_LDS they go to s_x (rows k and n + k), else to the grid
+template <bool ALONG_B, bool TO_LDS>
+__device__ __forceinline__ void dft_cplx_gemm(const HdDftArgs& a, const DftMat& w,
+                                              unsigned line0, unsigned n, const _Float16* xh,
+                                              const _Float16* xl, float sc, float* s_x) {
+  unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  unsigned half = n / 32u;
+  unsigned nl = (ALONG_B ? a.na : a.nb) * a.nch;
+  for (unsigned mt = wave; mt < half; mt += 4u) {
+    hd_float16v mr, cr, mi, ci;
+    dft_tile(w, mt, xh, xl, mr, cr);
+    dft_tile(w, mt + half, xh, xl, mi, ci);
+    unsigned ln = lane & 31u;
+    for (unsigned v = 0; v < 16; ++v) {
+      unsigned m = mt * 32u + mfma_d_row(lane, v);
+      float yr = (mr[v] + cr[v] * DFT_LO_UNSCALE) * sc;
+      float yi = (mi[v] + ci[v] * DFT_LO_UNSCALE) * sc;
+      if (TO_LDS) {
+        s_x[ln * DFT_XP + m] = yr;
+        s_x[ln * DFT_XP + n + m] = yi;
+      } else if (line0 + ln < nl) {
+        a.cgrid[dft_cidx<ALONG_B>(a, line0 + ln, m)] = make_float2(yr, yi);
+      }
+    }
   }
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(DFT_THREADS) k_dft(HdDftArgs a, DftMat w, DftMat w2) {
-  __shared__ float s_x[DFT_NL * DFT_KMAX];
+  __shared__ float s_x[DFT_NL * DFT_XP];
   __shared__ _Float16 xh[DFT_NL * DFT_KP], xl[DFT_NL * DFT_KP];
   __shared__ unsigned s_max;
   __shared__ double red_e[DFT_THREADS], red_w[DFT_THREADS];
   unsigned t = threadIdx.x, wave = t >> 6, lane = t & 63u;
   unsigned line0 = blockIdx.x * DFT_NL;
-  unsigned K = w.K;
-  // coalesced staging: consecutive threads take consecutive lines
-  for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
-    unsigned ln = e % DFT_NL, k = e / DFT_NL;
-    s_x[ln * DFT_KMAX + k] = dft_load<MODE>(a, line0 + ln, k);
-  }
+  dft_stage<MODE>(a, line0, w.K, s_x);
   __syncthreads();
-  int s = dft_split(s_x, xh, xl, K, &s_max);
+  int s = dft_split(s_x, xh, xl, w.K, &s_max);
   float sc = ldexpf(1.0f, s);
-  unsigned ntiles = w.M / 32u;
-  for (unsigned mt = wave; mt < ntiles; mt += 4u) {
-    hd_float16v main, corr;
-    dft_tile(w, mt, xh, xl, main, corr);
-    for (unsigned v = 0; v < 16; ++v) {
-      unsigned m = mt * 32u + mfma_d_row(lane, v);
-      float y = (main[v] + corr[v] * DFT_LO_UNSCALE) * sc;
-      if (MODE == DFT_A_CONV) s_x[(lane & 31u) * DFT_KMAX + m] = y;
-      else dft_store<MODE>(a, line0 + (lane & 31u), m, y);
-    }
+  if (MODE == DFT_CPLX_B) {
+    dft_cplx_gemm<true, false>(a, w, line0, a.nb, xh, xl, sc, s_x);
+    return;
   }
-  if (MODE != DFT_A_CONV) return;
+  if (MODE == DFT_CPLX_A) {
+    dft_cplx_gemm<false, false>(a, w, line0, a.na, xh, xl, sc, s_x);
+    return;
+  }
+  if (MODE == DFT_C_FWD || MODE == DFT_C_INV) {
+    // result tiles through LDS, then row-contiguous stores
+    unsigned ntiles = w.M / 32u;
+    for (unsigned mt = wave; mt < ntiles; mt += 4u) {
+      hd_float16v main, corr;
+      dft_tile(w, mt, xh, xl, main, corr);
+      for (unsigned v = 0; v < 16; ++v) {
+        unsigned m = mt * 32u + mfma_d_row(lane, v);
+        s_x[(lane & 31u) * DFT_XP + m] = (main[v] + corr[v] * DFT_LO_UNSCALE) * sc;
+      }
+    }
+    __syncthreads();
+    unsigned nrow = a.na * a.nb;
+    if (MODE == DFT_C_FWD) {
+      for (unsigned e = t; e < DFT_NL * a.nch; e += DFT_THREADS) {
+        unsigned ln = e / a.nch, k = e - ln * a.nch;
+        if (line0 + ln < nrow)
+          a.cgrid[(size_t)(line0 + ln) * a.nch + k] =
+              make_float2(s_x[ln * DFT_XP + k], s_x[ln * DFT_XP + a.nch + k]);
+      }
+    } else {
+      for (unsigned e = t; e < DFT_NL * a.nc; e += DFT_THREADS) {
+        unsigned ln = e / a.nc, c = e - ln * a.nc;
+        if (line0 + ln < nrow) a.vgrid[(size_t)(line0 + ln) * a.nc + c] = s_x[ln * DFT_XP + c];
+      }
+    }
+    return;
+  }
+  // DFT_A_CONV: forward a-axis into LDS, influence, inverse a-axis
+  dft_cplx_gemm<false, true>(a, w, line0, a.na, xh, xl, sc, s_x);
   __syncthreads();
   // influence multiply on the forward a-spectrum of the staged lines
   // (spme_recip.cu:34-115, 296-342), energy/virial partials in FP64
@@ -236,13 +320,13 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft(HdDftArgs a, DftMat w, DftM
         g = pref * (four_pi / k2) * expf(-k2 / four_alpha2) * a.bfa[ka] * a.bfb[b] * a.bfc[kcc];
         vf = g * (1.0f - k2 / (2.0f * a.alpha * a.alpha));
       }
-      float yr = s_x[ln * DFT_KMAX + ka], yi = s_x[ln * DFT_KMAX + na + ka];
+      float yr = s_x[ln * DFT_XP + ka], yi = s_x[ln * DFT_XP + na + ka];
       float rsq = yr * yr + yi * yi;
       double hw = (kcc == 0 || 2 * kcc == a.nc) ? 1.0 : 2.0;
       se += hw * (double)g * (double)rsq;
       sw += hw * (double)vf * (double)rsq;
-      s_x[ln * DFT_KMAX + ka] = g * yr;
-      s_x[ln * DFT_KMAX + na + ka] = g * yi;
+      s_x[ln * DFT_XP + ka] = g * yr;
+      s_x[ln * DFT_XP + na + ka] = g * yi;
     }
     red_e[t] = se;
     red_w[t] = sw;
@@ -258,15 +342,7 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft(HdDftArgs a, DftMat w, DftM
   }
   s = dft_split(s_x, xh, xl, w2.K, &s_max);
   sc = ldexpf(1.0f, s);
-  ntiles = w2.M / 32u;
-  for (unsigned mt = wave; mt < ntiles; mt += 4u) {
-    hd_float16v main, corr;
-    dft_tile(w2, mt, xh, xl, main, corr);
-    for (unsigned v = 0; v < 16; ++v) {
-      unsigned m = mt * 32u + mfma_d_row(lane, v);
-      dft_store<DFT_CPLX_A>(a, line0 + (lane & 31u), m, (main[v] + corr[v] * DFT_LO_UNSCALE) * sc);
-    }
-  }
+  dft_cplx_gemm<false, false>(a, w2, line0, a.na, xh, xl, sc, s_x);
 }
 
 unsigned hd_dft_partials_count(unsigned nb, unsigned nc) {
