@@ -146,6 +146,62 @@ def precision_check(system, ref):
     return out
 
 
+def batched_replicas(args):
+    """Batched-replica workload: uneven argon and SPC water replicas with
+    their own neighbour lists, SPME grids and Langevin thermostats, timed
+    as looped single-replica launches (hd_step per replica) and as one
+    graph launch per step for all replicas (hd_batch_step)."""
+    from heddlemd_amd import systems, units as U
+    from heddlemd_amd.engine import Batch, engine_for
+    from heddlemd_amd.rng import maxwell_boltzmann
+
+    specs = systems.replica_specs(count=args.replicas)
+    t0 = time.time()
+    engines = []
+    for r, sp in enumerate(specs):
+        s = systems.replica_system(sp)
+        kt = U.temperature(sp["temperature_K"])
+        v = maxwell_boltzmann(s.mass, kt, seed=100 + r, n_constraints=s.n_constraints)
+        e = engine_for(s, dt=U.time(1e-15), velocities=v,
+                       langevin=dict(friction=10.0 / U.time(1e-12), kt=kt, seed=1000 + r))
+        e.constrain_velocities()
+        engines.append((e, kt, s.n))
+    atoms = sum(n for _, _, n in engines)
+    log(f"batched: {len(engines)} replicas, {atoms} atoms, set-up {time.time() - t0:.1f} s")
+    batch = Batch([e for e, _, _ in engines])
+    batch.step(args.batch_equil)  # relax the lattice starts at 1 fs, strong friction
+    for r, (e, kt, _) in enumerate(engines):
+        e.set_integrator(U.time(args.dt_fs * 1e-15))
+        e.set_langevin(1.0 / U.time(1e-12), kt, 1000 + r)
+    batch.step(args.warmup)
+    lanes, tune = batch.autotune(steps=10)
+    log(f"batched: graph shape autotuned to lanes={lanes} "
+        f"({', '.join(f'{k}: {v * 100:.1f} ms/step' for k, v in tune.items())})")
+    for e, _, _ in engines:
+        e.step(2)  # capture each replica's own step graph before timing
+    t0 = time.perf_counter()
+    for e, _, _ in engines:
+        e.step(args.batch_steps)
+    t_loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch.step(args.batch_steps)
+    t_batch = time.perf_counter() - t0
+    batch.close()
+    for e, _, _ in engines:
+        e.close()
+    out = {"replicas": len(engines), "atoms": atoms, "steps": args.batch_steps,
+           "dt_fs": args.dt_fs,
+           "looped_atom_steps_per_s": round(atoms * args.batch_steps / t_loop, 1),
+           "batched_atom_steps_per_s": round(atoms * args.batch_steps / t_batch, 1),
+           "speedup": round(t_loop / t_batch, 3),
+           "graph_lanes": lanes,
+           "workload": "half LJ argon (2.1e28 m^-3) / half SPC water (1 g/cm^3), atom counts "
+                       "log-uniform in [256, 8192] (seed 11), 280-380 K Langevin (1/ps), "
+                       "SPME grids 16^3-64^3"}
+    log(f"batched: {out}")
+    return out
+
+
 def traffic_from_profile(kernel_prefix):
     """HBM bytes per launch of a kernel from the committed rocprofv3 pmc
     summary (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)."""
@@ -177,6 +233,10 @@ def main():
     ap.add_argument("--no-barostat", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-replica pass")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-equil", type=int, default=100)
     ap.add_argument("--no-split-dft", action="store_true",
                     help="skip the split-precision matrix-core DFT pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -312,12 +372,14 @@ def main():
             "fp32_fft_spme_recip_ms": round(result.get("stages_ms", {}).get("spme_recip", float("nan")), 4),
         }
         log(f"split-precision DFT: {result['split_dft']}")
+    e.close()
+    if not args.no_batched:
+        result["batched"] = batched_replicas(args)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle restatement) ...")
         result["cpu_baseline"], ref = cpu_baseline(system, args)
         log("precision check against the FP64 oracle ...")
         result["precision"] = precision_check(system, ref)
-    e.close()
     print(json.dumps(result), flush=True)
 
 

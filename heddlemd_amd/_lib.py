@@ -40,6 +40,11 @@ SIGNATURES = {
     "hd_set_spme": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.c_double]),
     "hd_set_spme_dft": (C.c_int, [_ctx, C.c_int]),
+    "hd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "hd_batch_destroy": (None, [C.c_void_p]),
+    "hd_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hd_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int]),
+    "hd_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
     "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
                                 C.c_double]),

@@ -144,6 +144,7 @@ struct hd_ctx {
 
   // graphs
   hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
+  unsigned graph_gen = 0;  // bumped whenever captured work goes stale
 
   std::vector<Stage> stages;  // event pool; the first n_stage_used are live
   size_t n_stage_used = 0;
@@ -1241,6 +1242,7 @@ static int alloc_entries(hd_ctx* c, unsigned cap) {
 }
 
 static void drop_graphs(hd_ctx* c) {
+  ++c->graph_gen;
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
   if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
   c->graph_f = c->graph_fev = nullptr;
@@ -1802,6 +1804,162 @@ int hd_selftest_dpp(hd_ctx* c, int32_t* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(c, HD_ERR_HIP, "selftest: %s", hipGetErrorString(e));
+  return HD_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched replicas: one hipGraph per step for many independent contexts.
+// The graph forks from the batch's stream into every replica's stream
+// (event record / wait), captures that replica's step there, and joins
+// back, so a replay is one launch and the replicas' branches are
+// independent nodes the runtime may run side by side.  Every replica runs
+// exactly the kernels of its own hd_step, on its own buffers, so its
+// trajectory is byte-identical to a solo run.
+// ---------------------------------------------------------------------
+struct hd_batch {
+  std::vector<hd_ctx*> ctx;
+  std::vector<unsigned> gen;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int device = 0;
+  int lanes = 0;  // parallel chains in the step graph (0 = one per replica)
+  char err[512] = {0};
+};
+
+static int bfail(hd_batch* b, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(b->err, sizeof(b->err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define BCHK(b, expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) return bfail((b), HD_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+static void batch_drop(hd_batch* b) {
+  for (auto& g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+int hd_batch_create(hd_ctx* const* ctxs, int n, hd_batch** out) {
+  if (!out || n < 1 || !ctxs) return HD_ERR_ARG;
+  *out = nullptr;
+  hd_batch* b = new hd_batch();
+  for (int i = 0; i < n; ++i) {
+    hd_ctx* c = ctxs[i];
+    if (!c || !c->ready) {
+      delete b;
+      return HD_ERR_STATE;
+    }
+    if (i > 0 && c->device != b->device) {
+      delete b;
+      return HD_ERR_ARG;
+    }
+    b->device = c->device;
+    b->ctx.push_back(c);
+  }
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return HD_ERR_HIP;
+  }
+  b->join.resize(n, nullptr);
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess) {
+      hd_batch_destroy(b);
+      return HD_ERR_HIP;
+    }
+  b->gen.assign(n, ~0u);
+  *out = b;
+  return HD_OK;
+}
+
+void hd_batch_destroy(hd_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop(b);
+  for (auto e : b->join)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* hd_batch_last_error(const hd_batch* b) { return b ? b->err : "null batch"; }
+
+int hd_batch_set_lanes(hd_batch* b, int lanes) {
+  if (!b || lanes < 0) return HD_ERR_ARG;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  batch_drop(b);
+  b->lanes = lanes;
+  return HD_OK;
+}
+
+static int batch_capture(hd_batch* b, bool ev, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  BCHK(b, hipEventRecord(b->fork, b->stream));
+  // replicas form `lanes` chains: replica i follows replica i - lanes
+  size_t n = b->ctx.size();
+  size_t lanes = (b->lanes <= 0 || (size_t)b->lanes >= n) ? n : (size_t)b->lanes;
+  for (size_t i = 0; i < n; ++i) {
+    hd_ctx* c = b->ctx[i];
+    BCHK(b, hipStreamWaitEvent(c->stream, i < lanes ? b->fork : b->join[i - lanes], 0));
+    enqueue_step(c, ev, false);
+    BCHK(b, hipEventRecord(b->join[i], c->stream));
+    if (i + lanes >= n) BCHK(b, hipStreamWaitEvent(b->stream, b->join[i], 0));
+  }
+  BCHK(b, hipStreamEndCapture(b->stream, &g));
+  BCHK(b, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  return HD_OK;
+}
+
+int hd_batch_step(hd_batch* b, int n_steps, int scalars_last) {
+  if (!b || n_steps < 0) return HD_ERR_ARG;
+  BCHK(b, hipSetDevice(b->device));
+  // forces of the current positions must exist before the first kick
+  bool stale = false;
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    hd_ctx* c = b->ctx[i];
+    if (!c->forces_valid) {
+      enqueue_forces(c, false, false, false);
+      c->forces_valid = true;
+      BCHK(b, hipStreamSynchronize(c->stream));
+    }
+    if (c->d_sh_stats) BCHK(b, hipMemsetAsync(c->d_sh_stats, 0, 6 * sizeof(unsigned), c->stream));
+    if (b->gen[i] != c->graph_gen) stale = true;
+  }
+  if (stale) {
+    batch_drop(b);
+    for (size_t i = 0; i < b->ctx.size(); ++i) b->gen[i] = b->ctx[i]->graph_gen;
+  }
+  for (auto c : b->ctx) BCHK(b, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n_steps; ++s) {
+    bool ev = scalars_last && s == n_steps - 1;
+    hipGraphExec_t* gx = &b->graph[ev ? 1 : 0];
+    if (!*gx) {
+      int r = batch_capture(b, ev, gx);
+      if (r) return r;
+    }
+    BCHK(b, hipGraphLaunch(*gx, b->stream));
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    int r = check_status(b->ctx[i]);
+    if (r) return bfail(b, r, "replica %zu: %s", i, b->ctx[i]->err.c_str());
+  }
   return HD_OK;
 }
 

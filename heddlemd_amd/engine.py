@@ -291,3 +291,60 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_velocities(velocities)
     e.setup()
     return e
+
+
+class Batch:
+    """Many set-up Engines advanced by one graph launch per step
+    (hd_batch_*); the engines stay owned by the caller."""
+
+    def __init__(self, engines):
+        self._lib = engines[0]._lib
+        self._engines = list(engines)
+        arr = (C.c_void_p * len(engines))(*[e._ctx.value for e in engines])
+        h = C.c_void_p()
+        rc = self._lib.hd_batch_create(arr, len(engines), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"hd_batch_create failed ({rc})")
+        self._h = h
+
+    def step(self, n_steps, scalars_last=False):
+        rc = self._lib.hd_batch_step(self._h, int(n_steps), int(bool(scalars_last)))
+        if rc != 0:
+            msg = self._lib.hd_batch_last_error(self._h).decode()
+            raise RuntimeError(f"hd_batch_step failed ({rc}): {msg}")
+
+    def set_lanes(self, lanes):
+        rc = self._lib.hd_batch_set_lanes(self._h, int(lanes))
+        if rc != 0:
+            raise RuntimeError(f"hd_batch_set_lanes failed ({rc})")
+
+    def autotune(self, steps=10, candidates=(0, 64, 32, 16, 8, 4, 2)):
+        """Time `steps` steps per graph shape and keep the fastest; the
+        shape does not change any replica's arithmetic (the steps advance
+        the replicas, so run this in a warm-up phase)."""
+        import time
+        best = None
+        timings = {}
+        for lanes in candidates:
+            self.set_lanes(lanes)
+            self.step(1)  # capture
+            t0 = time.perf_counter()
+            self.step(steps)
+            dt = time.perf_counter() - t0
+            timings[lanes] = dt
+            if best is None or dt < timings[best]:
+                best = lanes
+        self.set_lanes(best)
+        return best, timings
+
+    def close(self):
+        if self._h:
+            self._lib.hd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

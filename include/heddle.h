@@ -142,6 +142,24 @@ int hd_negate_velocities(hd_ctx *ctx);
 int hd_use_graph(hd_ctx *ctx, int enable);
 int hd_sync(hd_ctx *ctx);
 
+/* Batched replicas.  A batch advances many independent, set-up contexts
+ * (same device) with one hipGraph launch per step: the graph forks into
+ * every context's stream, runs that context's own step there and joins,
+ * so each replica's trajectory is byte-identical to its solo hd_step run.
+ * hd_batch_step synchronises at its end and checks every replica's
+ * neighbour status (errors name the replica).  The contexts stay owned by
+ * the caller and must outlive the batch.  No counterpart in the reference,
+ * which runs one system per process (src/runner.rs). */
+typedef struct hd_batch hd_batch;
+int hd_batch_create(hd_ctx *const *ctxs, int n, hd_batch **out);
+void hd_batch_destroy(hd_batch *batch);
+int hd_batch_step(hd_batch *batch, int n_steps, int scalars_last);
+/* Shape of the step graph: `lanes` parallel chains of replicas (replica i
+ * runs after replica i - lanes); 0 = every replica its own branch.  Does
+ * not change any replica's arithmetic. */
+int hd_batch_set_lanes(hd_batch *batch, int lanes);
+const char *hd_batch_last_error(const hd_batch *batch);
+
 /* Downloads (caller-owned host arrays of n entries; posq has 4n). */
 int hd_get_positions(hd_ctx *ctx, float *posq, int32_t *ix, int32_t *iy, int32_t *iz);
 int hd_get_velocities(hd_ctx *ctx, float *vx, float *vy, float *vz);

@@ -493,3 +493,44 @@ def test_split_precision_dft_matches_fp64(oracle):
         assert v_err <= 2e-5
         assert sc["e_recip"] == pytest.approx(er, rel=2e-6)
         assert sc["w_recip"] == pytest.approx(wr, rel=2e-6)
+
+
+def _replica_engine(spec, r):
+    from heddlemd_amd.rng import maxwell_boltzmann
+    s = systems.replica_system(spec)
+    kt = U.temperature(spec["temperature_K"])
+    v = maxwell_boltzmann(s.mass, kt, seed=100 + r, n_constraints=s.n_constraints)
+    e = engine_for(s, dt=U.time(1e-15), velocities=v,
+                   langevin=dict(friction=1.0 / U.time(1e-13), kt=kt, seed=1000 + r))
+    e.constrain_velocities()
+    return e
+
+
+def test_batched_replicas_byte_identical_to_solo_runs():
+    # uneven argon and water replicas advanced by one graph launch per step
+    # end in exactly the state of their own single-replica runs
+    say("case: 6 replicas, batched vs solo, 30 steps")
+    from heddlemd_amd.engine import Batch
+    specs = [s for s in systems.replica_specs(count=256)][::43][:6]
+    say("  " + ", ".join(f"{s['kind']}:{s['n_atoms']}" for s in specs))
+    solo = []
+    for r, sp in enumerate(specs):
+        e = _replica_engine(sp, r)
+        e.step(30, scalars_last=True)
+        solo.append((e.positions(images=True), e.velocities(), e.scalars()))
+        e.close()
+    engines = [_replica_engine(sp, r) for r, sp in enumerate(specs)]
+    b = Batch(engines)
+    b.step(10)
+    b.step(20, scalars_last=True)
+    b.close()
+    for r, e in enumerate(engines):
+        (p, i), v, sc = e.positions(images=True), e.velocities(), e.scalars()
+        (ps, is_), vs, scs = solo[r]
+        assert np.isfinite(p).all()
+        assert p.tobytes() == ps.tobytes(), f"replica {r} positions"
+        assert i.tobytes() == is_.tobytes()
+        assert v.tobytes() == vs.tobytes(), f"replica {r} velocities"
+        assert sc == scs
+        assert e.langevin_counter() == 30
+        e.close()
