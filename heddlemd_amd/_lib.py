@@ -45,6 +45,8 @@ SIGNATURES = {
     "hd_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hd_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "hd_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "hd_set_exclusion_span": (C.c_int, [_ctx, C.c_uint32]),
+    "hd_set_angles": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),
     "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
     "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
                                 C.c_double]),

@@ -42,11 +42,16 @@ struct HdPairArgs {
   const float* lattice;
   const float4* tile_posq;
   const unsigned *tile_type, *tile_key;
+  unsigned excl_span;  // pairs with |key_i - key_j| <= excl_span are excluded
   const float4 *center, *half;
   const unsigned *entry_atoms, *entry_count;
   long long *fpi, *fpi_ew, *fpj, *fpj_ew;
 };
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev);
+void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,
+                      const unsigned* ijk, const float* kth, const float* th0,
+                      const unsigned* slot_of, long long* fpj, long long* fpj_ew,
+                      unsigned n_angles, bool ev);
 
 struct HdSpmeArgs {
   unsigned n, na, nb, nc;

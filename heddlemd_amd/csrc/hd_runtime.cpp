@@ -64,6 +64,11 @@ struct hd_ctx {
   double alpha = 0, coul_rc = 0, kc = 1.0;
   int grid[3] = {0, 0, 0};
   double r_skin = -1.0;
+  unsigned excl_span = 0;
+  std::vector<uint32_t> h_angles;  // i, j, k per angle
+  std::vector<float> h_ang_k, h_ang_t0;
+  unsigned* d_angles = nullptr;
+  float *d_ang_k = nullptr, *d_ang_t0 = nullptr;
   std::vector<uint32_t> h_groups;
   double d_oh = 0, d_hh = 0, m_o = 0, m_h = 0;
   double dt = 0;
@@ -398,6 +403,7 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   pa.tile_posq = c->tile_posq;
   pa.tile_type = c->tile_type;
   pa.tile_key = c->tile_key;
+  pa.excl_span = c->excl_span;
   pa.center = c->center;
   pa.half = c->half;
   pa.entry_atoms = c->entry_atoms;
@@ -408,6 +414,9 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   pa.fpj_ew = c->fpj_ew;
   stage_begin(c, "pair_force", timed);
   hd_launch_pair(st, pa, ev);
+  if (!c->h_ang_k.empty())
+    hd_launch_angles(st, c->posq, c->d_lattice, c->d_angles, c->d_ang_k, c->d_ang_t0, c->slot_of,
+                     c->fpj, c->fpj_ew, (unsigned)c->h_ang_k.size(), ev);
   stage_end(c, timed);
   if (c->have_spme) {
     HdSpmeArgs sa;
@@ -931,6 +940,30 @@ int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, d
   return HD_OK;
 }
 
+int hd_set_angles(hd_ctx* c, uint64_t n_angles, const uint32_t* ijk, const double* k_theta,
+                  const double* theta0) {
+  if (!c || (n_angles && (!ijk || !k_theta || !theta0))) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_angles after hd_setup");
+  for (uint64_t a = 0; a < 3 * n_angles; ++a)
+    if (ijk[a] >= c->n) return fail(c, HD_ERR_ARG, "angle atom %u out of range", ijk[a]);
+  c->h_angles.assign(ijk, ijk + 3 * n_angles);
+  c->h_ang_k.resize(n_angles);
+  c->h_ang_t0.resize(n_angles);
+  for (uint64_t a = 0; a < n_angles; ++a) {
+    c->h_ang_k[a] = (float)k_theta[a];
+    c->h_ang_t0[a] = (float)theta0[a];
+  }
+  return HD_OK;
+}
+
+int hd_set_exclusion_span(hd_ctx* c, uint32_t span) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_exclusion_span after hd_setup");
+  if (span > (1u << 20)) return fail(c, HD_ERR_ARG, "exclusion span %u too large", span);
+  c->excl_span = span;
+  return HD_OK;
+}
+
 static void drop_graphs(hd_ctx* c);
 
 // Split one DFT matrix into FP16 high and 2^12-scaled low parts.
@@ -1354,6 +1387,14 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->d_groups, c->h_groups.size());
     if ((r = upload(c, c->d_groups, c->h_groups.data(), c->h_groups.size()))) return r;
   }
+  if (!c->h_ang_k.empty()) {
+    DALLOC(c, c->d_angles, c->h_angles.size());
+    DALLOC(c, c->d_ang_k, c->h_ang_k.size());
+    DALLOC(c, c->d_ang_t0, c->h_ang_t0.size());
+    if ((r = upload(c, c->d_angles, c->h_angles.data(), c->h_angles.size()))) return r;
+    if ((r = upload(c, c->d_ang_k, c->h_ang_k.data(), c->h_ang_k.size()))) return r;
+    if ((r = upload(c, c->d_ang_t0, c->h_ang_t0.data(), c->h_ang_t0.size()))) return r;
+  }
   if (has_shake(c)) {
     DALLOC(c, c->d_sh_goff, c->h_sh_goff.size());
     DALLOC(c, c->d_sh_atoms, c->h_sh_atoms.size());
@@ -1724,6 +1765,7 @@ int hd_count_pairs(hd_ctx* c, uint64_t* out) {
   pa.lattice = c->d_lattice;
   pa.tile_posq = c->tile_posq;
   pa.tile_key = c->tile_key;
+  pa.excl_span = c->excl_span;
   pa.entry_atoms = c->entry_atoms;
   pa.entry_count = c->entry_count;
   if (e == hipSuccess) {

@@ -676,9 +676,14 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     m[k] = mk;
     im[k] = mk > 0.f ? 1.0f / mk : 0.f;
   }
+  // one image for the whole group: relative to atom 0 (shake.cu:213-219)
+  // for groups within the reference's 8-atom cap, relative to the previous
+  // atom for longer groups (a chain longer than half the box would fold
+  // back onto atom 0's image)
   for (unsigned k = 1; k < na; ++k) {
-    shake_min_image_to(x0[0], y0[0], z0[0], x0[k], y0[k], z0[k], L);
-    shake_min_image_to(xu[0], yu[0], zu[0], xu[k], yu[k], zu[k], L);
+    unsigned r = na > 8u ? k - 1 : 0u;
+    shake_min_image_to(x0[r], y0[r], z0[r], x0[k], y0[k], z0[k], L);
+    shake_min_image_to(xu[r], yu[r], zu[r], xu[k], yu[k], zu[k], L);
   }
   for (unsigned k = 0; k < na; ++k) { xc[k] = xu[k]; yc[k] = yu[k]; zc[k] = zu[k]; }
   for (unsigned k = 0; k < nc; ++k) {
@@ -757,8 +762,10 @@ __global__ void k_rattle_velocities(HdShakeArgs a) {
     m[k] = mk;
     im[k] = mk > 0.f ? 1.0f / mk : 0.f;
   }
-  for (unsigned k = 1; k < na; ++k)
-    shake_min_image_to(px[0], py[0], pz[0], px[k], py[k], pz[k], L);
+  for (unsigned k = 1; k < na; ++k) {
+    unsigned r = na > 8u ? k - 1 : 0u;
+    shake_min_image_to(px[r], py[r], pz[r], px[k], py[k], pz[k], L);
+  }
   for (unsigned k = 0; k < nc; ++k) {
     unsigned pk = a.cij[c0 + k];
     unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;

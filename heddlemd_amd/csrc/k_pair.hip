@@ -19,10 +19,12 @@
 //     clearing pass); only the j-side uses i64 atomics;
 //   * the LJ type-pair row of the i-atom is held in registers (<= 4 types)
 //     and selected per pair, so the inner loop touches no memory;
-//   * exclusions by molecule key: a pair is excluded when both atoms carry
-//     the same key (every intra-molecule pair of a rigid group, as the
-//     reference derives them, src/forces/topology.rs:1225-1245); the atom's
-//     self pair is excluded the same way;
+//   * exclusions by key: a pair is excluded when the atoms' keys differ by
+//     at most a span.  Span 0 with one key per molecule excludes every
+//     intra-molecule pair of a rigid group, as the reference derives them
+//     (src/forces/topology.rs:1225-1245); keys chain * stride + bead index
+//     with span 2 exclude the 1-2 and 1-3 pairs of linear chains.  The
+//     atom's self pair is excluded the same way;
 //   * when the box is orthorhombic and wide enough, every j-atom is shifted
 //     once per entry to the image nearest the i-block centre, so the inner
 //     loop needs no minimum-image arithmetic (the single-periodic-copy path,
@@ -45,6 +47,7 @@ struct PairParams {
   float kc, alpha, coul_rc2;  // coul_rc2 < 0 disables the Coulomb term
   float rmax2;
   float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
+  unsigned excl_span;  // |key_i - key_j| <= excl_span excludes the pair
 };
 
 // The i-atom's row of the LJ type-pair table, held in VGPRs for the whole
@@ -165,7 +168,8 @@ __device__ __forceinline__ void pair_entries(
       float ec = qq * erfc_ar * inv_r;
       float fc = qq * inv_r2 * (erfc_ar * inv_r + two_a_sqrtpi * gauss);
       bool c_on = r2 <= coul_rc2;
-      bool ok = (jkey != ikey);
+      // excluded when |key_j - key_i| <= span (span 0: equal molecule keys)
+      bool ok = (jkey - ikey + P.excl_span) > 2u * P.excl_span;
       if (!SPC) ok = ok & (max(jkey, ikey) < HD_SENTINEL - 1u);
       float factor = (ok & lj_on) ? flj : 0.f;
       factor += (ok & c_on) ? fc : 0.f;
@@ -274,11 +278,88 @@ __global__ void __launch_bounds__(256) k_pair(
   }
 }
 
+// Harmonic angles (src/forces/angle.rs:258-313 functor, composed kernel
+// src/forces/jit_composed.rs:2300-2345): one thread per angle i-j-k with
+// minimum-image bond vectors, theta = atan2(|r_ij||r_kj| sin, dot),
+// F_i, F_k from g = -k (theta - theta_0) / sin theta, F_j = -(F_i + F_k),
+// energy and virial split in thirds.  The three forces are folded into the
+// j-side fixed-point accumulators at 2^48 (integer sums, so the total is
+// independent of the order in which angles land).
+__global__ void k_angles(const float4* __restrict__ posq, const float* __restrict__ lattice,
+                         const unsigned* __restrict__ ijk, const float* __restrict__ kth,
+                         const float* __restrict__ th0, const unsigned* __restrict__ slot_of,
+                         unsigned long long* __restrict__ fpj,
+                         unsigned long long* __restrict__ fpj_ew, unsigned n_angles, int ev) {
+  unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_angles) return;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned ai = ijk[3 * m], aj = ijk[3 * m + 1], ak = ijk[3 * m + 2];
+  float4 pi = posq[ai], pj = posq[aj], pk = posq[ak];
+  float dxi = pi.x - pj.x, dyi = pi.y - pj.y, dzi = pi.z - pj.z;
+  float dxk = pk.x - pj.x, dyk = pk.y - pj.y, dzk = pk.z - pj.z;
+  hd_min_image(dxi, dyi, dzi, L);
+  hd_min_image(dxk, dyk, dzk, L);
+  float dij2 = dxi * dxi + dyi * dyi + dzi * dzi;
+  float dkj2 = dxk * dxk + dyk * dyk + dzk * dzk;
+  float fix = 0.f, fiy = 0.f, fiz = 0.f, fkx = 0.f, fky = 0.f, fkz = 0.f, u = 0.f, w = 0.f;
+  if (dij2 != 0.f && dkj2 != 0.f) {
+    float dij = sqrtf(dij2), dkj = sqrtf(dkj2);
+    float inv_dd = 1.0f / (dij * dkj);
+    float dot = dxi * dxk + dyi * dyk + dzi * dzk;
+    float c = fminf(fmaxf(dot * inv_dd, -1.0f), 1.0f);
+    float s2 = 1.0f - c * c;
+    float sn = sqrtf(s2 > 0.f ? s2 : 0.f);
+    if (sn >= 1.0e-7f) {
+      float theta = atan2f(dij * dkj * sn, dot);
+      float k = kth[m], dth = theta - th0[m];
+      float g = -k * dth / sn;
+      float ii = 1.0f / dij2, kk = 1.0f / dkj2;
+      fix = g * (c * ii * dxi - inv_dd * dxk);
+      fiy = g * (c * ii * dyi - inv_dd * dyk);
+      fiz = g * (c * ii * dzi - inv_dd * dzk);
+      fkx = g * (c * kk * dxk - inv_dd * dxi);
+      fky = g * (c * kk * dyk - inv_dd * dyi);
+      fkz = g * (c * kk * dzk - inv_dd * dzi);
+      u = 0.5f * k * dth * dth;
+      w = (dxi * fix + dyi * fiy + dzi * fiz) + (dxk * fkx + dyk * fky + dzk * fkz);
+    }
+  }
+  float fjx = -(fix + fkx), fjy = -(fiy + fky), fjz = -(fiz + fkz);
+  unsigned si = slot_of[ai], sj = slot_of[aj], sk = slot_of[ak];
+  const float f3[3][3] = {{fix, fiy, fiz}, {fjx, fjy, fjz}, {fkx, fky, fkz}};
+  const unsigned sl[3] = {si, sj, sk};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      long long v = hd_to_fixed48(f3[a][d]);
+      if (v) atomicAdd(fpj + (size_t)sl[a] * 4 + d, (unsigned long long)v);
+    }
+  if (ev) {
+    long long e3 = hd_to_fixed48(u * (1.0f / 3.0f)), w3 = hd_to_fixed48(w * (1.0f / 3.0f));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (e3) atomicAdd(fpj_ew + (size_t)sl[a] * 2, (unsigned long long)e3);
+      if (w3) atomicAdd(fpj_ew + (size_t)sl[a] * 2 + 1, (unsigned long long)w3);
+    }
+  }
+}
+
+void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,
+                      const unsigned* ijk, const float* kth, const float* th0,
+                      const unsigned* slot_of, long long* fpj, long long* fpj_ew,
+                      unsigned n_angles, bool ev) {
+  if (n_angles == 0) return;
+  hipLaunchKernelGGL(k_angles, dim3((n_angles + 255) / 256), dim3(256), 0, st, posq, lattice,
+                     ijk, kth, th0, slot_of, (unsigned long long*)fpj,
+                     (unsigned long long*)fpj_ew, n_angles, ev ? 1 : 0);
+}
+
 // Diagnostic: number of distinct pairs of the current list with
 // r^2 <= r_max^2 that are neither excluded nor padding (the "useful" pairs
 // the pair kernel evaluates).  One wave per i-block, plain loops.
 __global__ void __launch_bounds__(64) k_count_pairs(
-    float rmax2, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
+    float rmax2, unsigned excl_span, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
     const unsigned* __restrict__ tile_key, const unsigned* __restrict__ entry_atoms,
     const unsigned* __restrict__ entry_count, unsigned cap, unsigned n,
     unsigned long long* __restrict__ out) {
@@ -296,7 +377,7 @@ __global__ void __launch_bounds__(64) k_count_pairs(
       unsigned js = ents[(size_t)e * 64 + t];
       if (js >= n) continue;
       if ((js >> 6) == b && js <= islot) continue;
-      if (tile_key[js] == ikey) continue;
+      if (tile_key[js] - ikey + excl_span <= 2u * excl_span) continue;
       float4 pj = tile_posq[js];
       float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       hd_min_image(dx, dy, dz, L);
@@ -308,7 +389,7 @@ __global__ void __launch_bounds__(64) k_count_pairs(
 void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out) {
   unsigned nblocks = a.npad / 64;
   hipLaunchKernelGGL(k_count_pairs, dim3(nblocks), dim3(64), 0, st, (float)(a.rmax * a.rmax),
-                     a.lattice, a.tile_posq, a.tile_key, a.entry_atoms, a.entry_count, a.cap,
+                     a.excl_span, a.lattice, a.tile_posq, a.tile_key, a.entry_atoms, a.entry_count, a.cap,
                      a.n, out);
 }
 
@@ -358,6 +439,7 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
   P.coul_rc2 = a.coul_rc > 0.0 ? (float)(a.coul_rc * a.coul_rc) : -1.0f;
   P.rmax2 = (float)(a.rmax * a.rmax);
   P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
+  P.excl_span = a.excl_span;
   if (ev) launch_pair_ev<true>(st, a, P, any_switch);
   else launch_pair_ev<false>(st, a, P, any_switch);
 }

@@ -105,6 +105,17 @@ class Engine:
         """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT."""
         self._chk(self._lib.hd_set_spme_dft(self._ctx, int(mode)), "hd_set_spme_dft")
 
+    def set_angles(self, ijk, k_theta, theta0):
+        ijk = np.ascontiguousarray(ijk, np.uint32).reshape(-1)
+        n = ijk.size // 3
+        self._chk(self._lib.hd_set_angles(self._ctx, n, ijk,
+                                          np.ascontiguousarray(k_theta, np.float64),
+                                          np.ascontiguousarray(theta0, np.float64)),
+                  "hd_set_angles")
+
+    def set_exclusion_span(self, span):
+        self._chk(self._lib.hd_set_exclusion_span(self._ctx, int(span)), "hd_set_exclusion_span")
+
     def set_shake(self, groups):
         """groups: list of (atom_ids, [(local_i, local_j, d), ...])."""
         goff, gat, coff, ci, cj, d = [0], [], [0], [], [], []
@@ -274,6 +285,11 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_neighbor(r_skin)
     if getattr(system, "shake_groups", None) is not None:
         e.set_shake(system.shake_groups)
+    if getattr(system, "angles", None) is not None:
+        ijk, kth, th0 = system.angles
+        e.set_angles(ijk, kth, th0)
+    if getattr(system, "excl_span", 0):
+        e.set_exclusion_span(system.excl_span)
     if system.water_groups is not None:
         d_oh, d_hh = system.water_geom
         m_o = float(system.mass[system.water_groups[0, 0]])

@@ -53,6 +53,17 @@ const char *hd_last_error(const hd_ctx *ctx);
  * NULL for "no exclusions". */
 int hd_set_system(hd_ctx *ctx, uint64_t n, const float *posq, const float *mass,
                   const uint32_t *type, const uint32_t *excl_key);
+/* Harmonic angles (src/forces/angle.rs): atoms ijk[3a..3a+2] (j the apex),
+ * force constant k_theta[a] (energy / rad^2) and rest angle theta0[a]
+ * (rad).  Before hd_setup. */
+int hd_set_angles(hd_ctx *ctx, uint64_t n_angles, const uint32_t *ijk, const double *k_theta,
+                  const double *theta0);
+/* Exclusion rule: a pair is excluded from the LJ and real-space terms when
+ * |excl_key_i - excl_key_j| <= span (default 0: equal keys, i.e. whole
+ * molecules).  Keys chain * stride + bead with span 2 give the 1-2 / 1-3
+ * exclusions of linear chains (the CSR lists of src/forces/topology.rs
+ * for these topologies).  Before hd_setup. */
+int hd_set_exclusion_span(hd_ctx *ctx, uint32_t span);
 /* Box (src/pbc.rs SimulationBox lattice). */
 int hd_set_lattice(hd_ctx *ctx, const double lattice[6]);
 int hd_set_velocities(hd_ctx *ctx, const float *vx, const float *vy, const float *vz);

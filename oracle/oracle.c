@@ -701,6 +701,56 @@ void or_vv_kick_f32(int n, float *vx, float *vy, float *vz, const float *fx,
 }
 
 /* ------------------------------------------------------------------ */
+/* Harmonic angles (src/forces/angle.rs:258-313; per-atom thirds of the  */
+/* energy and virial, src/forces/jit_composed.rs:2300-2345)              */
+/* ------------------------------------------------------------------ */
+
+void or_angle_forces(int n_angles, const uint32_t *ijk, const double *kth, const double *th0,
+                     const float *posq, const double *L, double *F, double *E, double *W)
+{
+    for (int m = 0; m < n_angles; ++m) {
+        uint32_t i = ijk[3 * m], j = ijk[3 * m + 1], k = ijk[3 * m + 2];
+        double dxi = (double)posq[4 * i] - posq[4 * j];
+        double dyi = (double)posq[4 * i + 1] - posq[4 * j + 1];
+        double dzi = (double)posq[4 * i + 2] - posq[4 * j + 2];
+        double dxk = (double)posq[4 * k] - posq[4 * j];
+        double dyk = (double)posq[4 * k + 1] - posq[4 * j + 1];
+        double dzk = (double)posq[4 * k + 2] - posq[4 * j + 2];
+        or_min_image(&dxi, &dyi, &dzi, L);
+        or_min_image(&dxk, &dyk, &dzk, L);
+        double dij2 = dxi * dxi + dyi * dyi + dzi * dzi;
+        double dkj2 = dxk * dxk + dyk * dyk + dzk * dzk;
+        if (dij2 == 0.0 || dkj2 == 0.0) continue;
+        double dij = sqrt(dij2), dkj = sqrt(dkj2);
+        double inv_dd = 1.0 / (dij * dkj);
+        double dot = dxi * dxk + dyi * dyk + dzi * dzk;
+        double c = dot * inv_dd;
+        if (c > 1.0) c = 1.0;
+        if (c < -1.0) c = -1.0;
+        double s2 = 1.0 - c * c;
+        double sn = sqrt(s2 > 0.0 ? s2 : 0.0);
+        if (sn < 1.0e-7) continue;
+        double theta = atan2(dij * dkj * sn, dot);
+        double dth = theta - th0[m];
+        double g = -kth[m] * dth / sn;
+        double ii = 1.0 / dij2, kk = 1.0 / dkj2;
+        double fi[3] = {g * (c * ii * dxi - inv_dd * dxk), g * (c * ii * dyi - inv_dd * dyk),
+                        g * (c * ii * dzi - inv_dd * dzk)};
+        double fk[3] = {g * (c * kk * dxk - inv_dd * dxi), g * (c * kk * dyk - inv_dd * dyi),
+                        g * (c * kk * dzk - inv_dd * dzi)};
+        double u = 0.5 * kth[m] * dth * dth;
+        double w = dxi * fi[0] + dyi * fi[1] + dzi * fi[2] + dxk * fk[0] + dyk * fk[1] + dzk * fk[2];
+        for (int d = 0; d < 3; ++d) {
+            F[3 * i + d] += fi[d];
+            F[3 * k + d] += fk[d];
+            F[3 * j + d] -= fi[d] + fk[d];
+        }
+        E[i] += u / 3.0; E[j] += u / 3.0; E[k] += u / 3.0;
+        W[i] += w / 3.0; W[j] += w / 3.0; W[k] += w / 3.0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* SHAKE / RATTLE (kernels/shake.cu:96-490, src/integrator/shake.rs)     */
 /* Groups: atoms goff[g]..goff[g+1] of gatoms, constraints coff[g]..     */
 /* coff[g+1] with group-local atom indices ci/cj and r2 = (f32 d)^2.     */
@@ -729,6 +779,7 @@ static void weighted_com_f(const float *x, const float *y, const float *z, const
 }
 
 #define OR_SHAKE_MAXA 64
+#define OR_SHAKE_REF_ATOMS 8  /* MAX_GROUP_ATOMS, src/integrator/shake.rs:21 */
 #define OR_SHAKE_MAXC 64
 
 void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
@@ -754,9 +805,14 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             m[a] = mass[i];
             im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
         }
+        /* one image for the whole group: relative to atom 0 (shake.cu:213-219)
+         * for groups within the reference's 8-atom cap, relative to the
+         * previous atom for longer groups (chains longer than half the box
+         * would otherwise fold back onto atom 0's image) */
         for (int a = 1; a < na; ++a) {
-            min_image_to_f(x0[0], y0[0], z0[0], &x0[a], &y0[a], &z0[a], L);
-            min_image_to_f(xu[0], yu[0], zu[0], &xu[a], &yu[a], &zu[a], L);
+            int r = na > OR_SHAKE_REF_ATOMS ? a - 1 : 0;
+            min_image_to_f(x0[r], y0[r], z0[r], &x0[a], &y0[a], &z0[a], L);
+            min_image_to_f(xu[r], yu[r], zu[r], &xu[a], &yu[a], &zu[a], L);
         }
         for (int a = 0; a < na; ++a) { xc[a] = xu[a]; yc[a] = yu[a]; zc[a] = zu[a]; }
         float gx[OR_SHAKE_MAXC], gy[OR_SHAKE_MAXC], gz[OR_SHAKE_MAXC], ip[OR_SHAKE_MAXC];
@@ -834,8 +890,10 @@ void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t
             m[a] = mass[i];
             im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
         }
-        for (int a = 1; a < na; ++a)
-            min_image_to_f(px[0], py[0], pz[0], &px[a], &py[a], &pz[a], L);
+        for (int a = 1; a < na; ++a) {
+            int r = na > OR_SHAKE_REF_ATOMS ? a - 1 : 0;
+            min_image_to_f(px[r], py[r], pz[r], &px[a], &py[a], &pz[a], L);
+        }
         const uint8_t *li = ci + coff[g], *lj = cj + coff[g];
         float dx[OR_SHAKE_MAXC], dy[OR_SHAKE_MAXC], dz[OR_SHAKE_MAXC], d2[OR_SHAKE_MAXC],
             ip[OR_SHAKE_MAXC];

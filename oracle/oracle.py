@@ -56,6 +56,8 @@ def lib():
         L.or_spme_recip.argtypes = [C.c_int, _f32p, _f64p, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_double, C.c_double, _f64p, _f64p, _f64p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]
+        L.or_angle_forces.argtypes = [C.c_int, _u32p, _f64p, _f64p, _f32p, _f64p, _f64p,
+                                      _f64p, _f64p]
         L.or_shake_positions_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, _f32p,
                                              _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                              C.c_float, C.c_float, C.c_int, _f32p, _i32p]
@@ -223,6 +225,20 @@ def vv_kick_drift_f32(posq, img, v, f, mass, lattice, dt):
 def vv_kick_f32(v, f, mass, dt):
     n = v.shape[1]
     lib().or_vv_kick_f32(n, v[0], v[1], v[2], f[0], f[1], f[2], mass, np.float32(dt))
+
+
+def angle_forces(posq, lattice, ijk, k_theta, theta0):
+    """Harmonic angle forces (n,3), per-atom energy and virial thirds, FP64
+    (src/forces/angle.rs:258-313)."""
+    n = posq.shape[0]
+    ijk = np.ascontiguousarray(ijk, np.uint32).reshape(-1)
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(n)
+    lib().or_angle_forces(ijk.size // 3, ijk, np.ascontiguousarray(k_theta, np.float64),
+                          np.ascontiguousarray(theta0, np.float64), np.ascontiguousarray(posq),
+                          np.ascontiguousarray(lattice, np.float64), F, E, W)
+    return F.reshape(n, 3), E, W
 
 
 SHAKE_TOL = 3.57e-6     # |r^2 - d^2| in Bohr^2 (shake.cu:251-256)

@@ -176,3 +176,41 @@ def test_pair_set_matches_brute_force(oracle):
     got = set(map(tuple, pairs.tolist())) | set(map(tuple, band.tolist()))
     assert want - got == set()
     assert set(map(tuple, pairs.tolist())) <= want
+
+
+def test_angle_forces_are_energy_gradient(oracle):
+    # harmonic angle (src/forces/angle.rs:258-313): F = -grad E by central
+    # differences, Newton's third law, zero force at the rest angle
+    from heddlemd_amd import systems
+    s = systems.polymer_melt(n_chains=2, n_beads=6, seed=3)
+    rng = np.random.default_rng(0)
+    posq = s.posq.copy()
+    posq[:, :3] += rng.normal(0, 0.2, size=(s.n, 3)).astype(np.float32)
+    ijk, kth, th0 = s.angles
+    F, E, W = oracle.angle_forces(posq, s.lattice, ijk, kth, th0)
+    assert np.allclose(F.sum(0), 0.0, atol=1e-12)
+    h = 1.0 / 1024
+    for a in range(s.n):
+        for d in range(3):
+            pp = posq.copy()
+            pm = posq.copy()
+            pp[a, d] += np.float32(h)
+            pm[a, d] -= np.float32(h)
+            ep = oracle.angle_forces(pp, s.lattice, ijk, kth, th0)[1].sum()
+            em = oracle.angle_forces(pm, s.lattice, ijk, kth, th0)[1].sum()
+            dx = float(pp[a, d]) - float(pm[a, d])
+            assert -(ep - em) / dx == pytest.approx(F[a, d], rel=1e-4, abs=1e-9)
+    # at the rest geometry (the generator's exact 114 degrees) the force vanishes
+    F0, E0, _ = oracle.angle_forces(s.posq, s.lattice, ijk, kth, th0)
+    assert np.abs(F0).max() < 1e-5 * np.abs(F).max()
+
+
+def test_exclusion_span_gives_1_2_and_1_3_pairs():
+    from heddlemd_amd import systems
+    s = systems.polymer_melt(n_chains=3, n_beads=7, seed=4)
+    off, part, _ = s.exclusion_csr()
+    for c in range(3):
+        for b in range(7):
+            i = 7 * c + b
+            want = sorted(7 * c + x for x in range(max(0, b - 2), min(7, b + 3)) if x != b)
+            assert part[off[i]:off[i + 1]].tolist() == want
