@@ -60,6 +60,25 @@ SIGNATURES = {
     "hd_set_shake_tolerance": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int]),
     "hd_get_constraint_stats": (C.c_int, [_ctx, _u64p]),
     "hd_get_langevin_counter": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
+    "hd_set_shake_predictor": (C.c_int, [_ctx, _vp]),
+    "hd_get_shake_multipliers": (C.c_int, [_ctx, _vp, _vp]),
+    "hd_shake_log_start": (C.c_int, [_ctx, C.c_uint64]),
+    "hd_shake_log_stop": (C.c_int, [_ctx]),
+    "hd_shake_log_size": (C.c_int, [_ctx, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "hd_shake_log_read": (C.c_int, [_ctx, C.c_uint64, C.c_uint64, _f32p, _f32p]),
+    "hd_mlp_param_count": (C.c_int, []),
+    "hd_mlp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "hd_mlp_destroy": (None, [C.c_void_p]),
+    "hd_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "hd_mlp_set_params": (C.c_int, [C.c_void_p, _f32p]),
+    "hd_mlp_get_params": (C.c_int, [C.c_void_p, _f32p]),
+    "hd_mlp_predict": (C.c_int, [C.c_void_p, _f32p, C.c_uint64, _f32p]),
+    "hd_mlp_gradient": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_uint64, _f32p,
+                                  C.POINTER(C.c_double)]),
+    "hd_mlp_train": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_uint64, C.c_int, C.c_int,
+                               C.c_float, C.c_uint64, _vp]),
+    "hd_mlp_train_log": (C.c_int, [C.c_void_p, _ctx, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                   _vp]),
     "hd_setup": (C.c_int, [_ctx]),
     "hd_force_eval": (C.c_int, [_ctx, C.c_int]),
     "hd_constrain_velocities": (C.c_int, [_ctx]),

@@ -188,11 +188,76 @@ struct HdShakeArgs {
   float dt, tol, rattle_tol;
   int max_iter;
   unsigned* stats;  // [sum sweeps, max sweeps, unconverged] for SHAKE then RATTLE
+  // multipliers in the dimensionless form nu = lambda (1/m_i + 1/m_j): the
+  // fraction of the snapshot bond vector that the constraint moves
+  const float* lam0;  // per-constraint nu applied before the sweeps (or null)
+  float* lam_out;     // per-constraint total nu of this call (or null)
+  const unsigned* cgrp;  // constraint -> group (feature kernel)
+  float* feat;           // n_cons x HD_SHAKE_NFEAT predictor inputs (feature kernel)
+  unsigned n_cons;
 };
 // LDS bytes per thread of a group with the given maxima
 unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons);
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a);
 void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool accumulate_virial);
+// Predictor inputs of every constraint (HD_SHAKE_NFEAT floats each) into
+// a.feat, from posq (after the drift) and snap.
+#define HD_SHAKE_NFEAT 8
+void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a);
+
+// Multiplier predictor: an 8 -> 64 -> 64 -> 1 tanh network on the scaled
+// features, predicting the correction to the first-order multiplier:
+//   u = x / s (the nu1 and rho columns; kappa unscaled),
+//   h1 = tanh(W1 u + b1), h2 = tanh(W2 h1 + b2), o = w3.h2 + b3,
+//   nu = x0 + s o.
+// Parameter pack (floats): W1[64][8], b1[64], W2[64][64], b2[64], w3[64],
+// b3, s.  Training minimises the mean of (o - (t - x0) / s)^2.
+#define HD_MLP_IN 8
+#define HD_MLP_H 64
+#define HD_MLP_W1 0
+#define HD_MLP_B1 (HD_MLP_W1 + HD_MLP_H * HD_MLP_IN)
+#define HD_MLP_W2 (HD_MLP_B1 + HD_MLP_H)
+#define HD_MLP_B2 (HD_MLP_W2 + HD_MLP_H * HD_MLP_H)
+#define HD_MLP_W3 (HD_MLP_B2 + HD_MLP_H)
+#define HD_MLP_B3 (HD_MLP_W3 + HD_MLP_H)
+#define HD_MLP_NTRAIN (HD_MLP_B3 + 1)
+#define HD_MLP_SCALE HD_MLP_NTRAIN
+#define HD_MLP_NPARAM (HD_MLP_NTRAIN + 1)
+#define HD_MLP_PSTRIDE 4864  // floats per block partial: gradient, then the loss sum
+#define HD_MLP_SAMPLES 64    // samples per block of the gradient kernel
+// nu[c] = prediction for feature row c
+void hd_launch_mlp_predict(hipStream_t st, const float* params, const float* feat, unsigned n,
+                           float* nu);
+// Gradient partials of one mini-batch: samples order(first + q), q < count,
+// with order(m) = (mul * m + add) mod n; block b writes the gradient sum of
+// its 64 samples and their loss sum to partials[b * HD_MLP_PSTRIDE ...].
+// inv_count scales the gradient (1 / samples in the mini-batch).
+// Returns the number of blocks.
+unsigned hd_launch_mlp_grad(hipStream_t st, const float* params, const float* x,
+                            const float* t, uint64_t n, uint64_t first, unsigned count,
+                            uint64_t mul, uint64_t add, float inv_count, float* partials);
+// Sums the partials of nb blocks in block order; grad_out (or null)
+// receives the gradient; with adam non-null, one Adam step:
+//   m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr_t m / (sqrt(v) + eps)
+// (lr_t carries the bias correction); loss_acc[0] += the loss sum.
+struct HdAdam {
+  float b1, b2, lr_t, eps;
+  float *m, *v;
+};
+void hd_launch_mlp_reduce(hipStream_t st, const float* partials, unsigned nb, float* params,
+                          float* grad_out, const HdAdam* adam, double* loss_acc);
+// params[HD_MLP_SCALE] = rms of column 0 of x (n rows), 1 if that is 0.
+void hd_launch_mlp_scale(hipStream_t st, const float* x, uint64_t n, float* params);
+// Appends this step's rows (feat, nu) at slot counter[0] of a ring of cap
+// slots of n rows each and advances the counter; a full ring drops rows.
+void hd_launch_shake_log(hipStream_t st, const float* feat, const float* nu, unsigned n,
+                         float* log_x, float* log_t, unsigned long long* counter,
+                         unsigned long long cap);
+// Device view of a context's SHAKE log (synchronises the context stream):
+// rows = logged steps x constraints (host runtime, used by the trainer).
+struct hd_ctx;
+extern "C" int hd_ctx_shake_log_view(hd_ctx* c, int* device, const float** x,
+                                     const float** t, uint64_t* rows);
 
 struct HdNhcArgs {
   int chain, n_yoshida, n_resp;

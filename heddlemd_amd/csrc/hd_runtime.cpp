@@ -146,6 +146,14 @@ struct hd_ctx {
   unsigned *d_sh_goff = nullptr, *d_sh_atoms = nullptr, *d_sh_coff = nullptr,
            *d_sh_cij = nullptr, *d_sh_stats = nullptr;
   float* d_sh_r2 = nullptr;
+  // SHAKE multiplier predictor and the step log of plain-SHAKE solutions
+  unsigned* d_sh_cgrp = nullptr;  // constraint -> group
+  float *d_sh_feat = nullptr, *d_sh_nu0 = nullptr, *d_sh_nu = nullptr;
+  float* d_mlp = nullptr;  // predictor parameters (HD_MLP_NPARAM floats)
+  bool sh_predict = false, sh_logging = false;
+  float *d_log_x = nullptr, *d_log_t = nullptr;
+  unsigned long long* d_log_ctr = nullptr;
+  unsigned long long log_cap = 0;  // steps the log holds
 
   // graphs
   hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
@@ -550,6 +558,11 @@ HdShakeArgs shake_args(hd_ctx* c, float dt) {
   a.rattle_tol = (float)c->sh_rattle_tol;
   a.max_iter = c->sh_max_iter;
   a.stats = c->d_sh_stats;
+  a.n_cons = (unsigned)c->h_sh_cij.size();
+  a.cgrp = c->d_sh_cgrp;
+  a.feat = c->d_sh_feat;
+  a.lam0 = c->sh_predict ? c->d_sh_nu0 : nullptr;
+  a.lam_out = c->d_sh_nu;
   return a;
 }
 
@@ -563,9 +576,20 @@ void enqueue_constrain_positions(hd_ctx* c, bool timed) {
     stage_end(c, timed);
   }
   if (has_shake(c)) {
+    HdShakeArgs sa = shake_args(c, (float)c->dt);
+    if (c->sh_predict || c->sh_logging) {
+      stage_begin(c, "shake_predict", timed);
+      hd_launch_shake_features(c->stream, sa);
+      if (c->sh_predict)
+        hd_launch_mlp_predict(c->stream, c->d_mlp, c->d_sh_feat, sa.n_cons, c->d_sh_nu0);
+      stage_end(c, timed);
+    }
     stage_begin(c, "shake_positions", timed);
-    hd_launch_shake_positions(c->stream, shake_args(c, (float)c->dt));
+    hd_launch_shake_positions(c->stream, sa);
     stage_end(c, timed);
+    if (c->sh_logging)
+      hd_launch_shake_log(c->stream, c->d_sh_feat, c->d_sh_nu, sa.n_cons, c->d_log_x,
+                          c->d_log_t, c->d_log_ctr, c->log_cap);
   }
 }
 
@@ -1134,6 +1158,150 @@ int hd_get_constraint_stats(hd_ctx* c, uint64_t* out) {
 
 static void drop_graphs(hd_ctx* c);
 
+// ---- SHAKE multiplier predictor and step log -------------------------
+
+static int ensure_shake_features(hd_ctx* c) {
+  size_t nc = c->h_sh_cij.size();
+  if (!c->d_sh_feat) DALLOC(c, c->d_sh_feat, nc * HD_SHAKE_NFEAT);
+  if (!c->d_sh_nu0) DALLOC(c, c->d_sh_nu0, nc);
+  return HD_OK;
+}
+
+int hd_set_shake_predictor(hd_ctx* c, const float* params) {
+  if (!c) return HD_ERR_ARG;
+  if (!c->ready || !has_shake(c)) return fail(c, HD_ERR_STATE, "no SHAKE groups set up");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (params) {
+    for (int k = 0; k < HD_MLP_NPARAM; ++k)
+      if (!std::isfinite(params[k])) return fail(c, HD_ERR_ARG, "predictor parameter %d not finite", k);
+    int r = ensure_shake_features(c);
+    if (r) return r;
+    if (!c->d_mlp) DALLOC(c, c->d_mlp, HD_MLP_NPARAM);
+    if ((r = upload(c, c->d_mlp, params, HD_MLP_NPARAM))) return r;
+  }
+  bool on = params != nullptr;
+  if (on != c->sh_predict) {
+    c->sh_predict = on;
+    drop_graphs(c);
+  }
+  return HD_OK;
+}
+
+int hd_get_shake_multipliers(hd_ctx* c, float* nu0, float* nu) {
+  if (!c) return HD_ERR_ARG;
+  if (!c->ready || !has_shake(c)) return fail(c, HD_ERR_STATE, "no SHAKE groups set up");
+  size_t nc = c->h_sh_cij.size();
+  HIPCHK(c, hipSetDevice(c->device));
+  if (nu0) {
+    if (c->d_sh_nu0)
+      HIPCHK(c, hipMemcpyAsync(nu0, c->d_sh_nu0, nc * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+    else
+      memset(nu0, 0, nc * sizeof(float));
+  }
+  if (nu)
+    HIPCHK(c, hipMemcpyAsync(nu, c->d_sh_nu, nc * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_shake_log_start(hd_ctx* c, uint64_t max_steps) {
+  if (!c || max_steps == 0) return HD_ERR_ARG;
+  if (!c->ready || !has_shake(c)) return fail(c, HD_ERR_STATE, "no SHAKE groups set up");
+  size_t nc = c->h_sh_cij.size();
+  if ((double)max_steps * (double)nc >= 4294967296.0)
+    return fail(c, HD_ERR_ARG, "log of %llu steps x %zu constraints exceeds 2^32 rows",
+                (unsigned long long)max_steps, nc);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int r = ensure_shake_features(c);
+  if (r) return r;
+  if (c->log_cap != max_steps) {
+    dfree(c, c->d_log_x);
+    dfree(c, c->d_log_t);
+    c->d_log_x = c->d_log_t = nullptr;
+    c->log_cap = 0;
+    DALLOC(c, c->d_log_x, (size_t)max_steps * nc * HD_SHAKE_NFEAT);
+    DALLOC(c, c->d_log_t, (size_t)max_steps * nc);
+    c->log_cap = max_steps;
+  }
+  if (!c->d_log_ctr) DALLOC(c, c->d_log_ctr, 1);
+  HIPCHK(c, hipMemsetAsync(c->d_log_ctr, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->sh_logging) {
+    c->sh_logging = true;
+    drop_graphs(c);
+  }
+  return HD_OK;
+}
+
+int hd_shake_log_stop(hd_ctx* c) {
+  if (!c) return HD_ERR_ARG;
+  if (c->sh_logging) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->sh_logging = false;
+    drop_graphs(c);
+  }
+  return HD_OK;
+}
+
+static int log_steps(hd_ctx* c, uint64_t* steps) {
+  unsigned long long k = 0;
+  if (c->d_log_ctr) {
+    HIPCHK(c, hipMemcpyAsync(&k, c->d_log_ctr, sizeof(k), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  *steps = k;
+  return HD_OK;
+}
+
+int hd_shake_log_size(hd_ctx* c, uint64_t* steps, uint64_t* rows_per_step) {
+  if (!c || !steps) return HD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = log_steps(c, steps);
+  if (r) return r;
+  if (rows_per_step) *rows_per_step = c->h_sh_cij.size();
+  return HD_OK;
+}
+
+int hd_shake_log_read(hd_ctx* c, uint64_t first_step, uint64_t n_steps, float* x, float* t) {
+  if (!c || (n_steps && (!x || !t))) return HD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  uint64_t have = 0;
+  int r = log_steps(c, &have);
+  if (r) return r;
+  if (first_step + n_steps > have)
+    return fail(c, HD_ERR_ARG, "log holds %llu steps, asked for %llu..%llu",
+                (unsigned long long)have, (unsigned long long)first_step,
+                (unsigned long long)(first_step + n_steps));
+  size_t nc = c->h_sh_cij.size();
+  if (n_steps) {
+    HIPCHK(c, hipMemcpyAsync(x, c->d_log_x + first_step * nc * HD_SHAKE_NFEAT,
+                             n_steps * nc * HD_SHAKE_NFEAT * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(t, c->d_log_t + first_step * nc, n_steps * nc * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return HD_OK;
+}
+
+int hd_ctx_shake_log_view(hd_ctx* c, int* device, const float** x, const float** t,
+                          uint64_t* rows) {
+  if (!c || !c->d_log_ctr) return fail(c, HD_ERR_STATE, "no SHAKE log (hd_shake_log_start)");
+  HIPCHK(c, hipSetDevice(c->device));
+  uint64_t steps = 0;
+  int r = log_steps(c, &steps);
+  if (r) return r;
+  *device = c->device;
+  *x = c->d_log_x;
+  *t = c->d_log_t;
+  *rows = steps * c->h_sh_cij.size();
+  return HD_OK;
+}
+
 int hd_set_integrator(hd_ctx* c, double dt, int lossless) {
   if (!c || !(dt > 0)) return HD_ERR_ARG;
   if (c->ready) {
@@ -1402,6 +1570,14 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->d_sh_cij, c->h_sh_cij.size());
     DALLOC(c, c->d_sh_r2, c->h_sh_r2.size());
     DALLOC(c, c->d_sh_stats, 6);
+    {
+      std::vector<uint32_t> cg(c->h_sh_cij.size());
+      for (size_t g = 0; g + 1 < c->h_sh_coff.size(); ++g)
+        for (uint32_t k = c->h_sh_coff[g]; k < c->h_sh_coff[g + 1]; ++k) cg[k] = (uint32_t)g;
+      DALLOC(c, c->d_sh_cgrp, cg.size());
+      DALLOC(c, c->d_sh_nu, cg.size());
+      if (!cg.empty() && (r = upload(c, c->d_sh_cgrp, cg.data(), cg.size()))) return r;
+    }
     if ((r = upload(c, c->d_sh_goff, c->h_sh_goff.data(), c->h_sh_goff.size()))) return r;
     if ((r = upload(c, c->d_sh_atoms, c->h_sh_atoms.data(), c->h_sh_atoms.size()))) return r;
     if ((r = upload(c, c->d_sh_coff, c->h_sh_coff.data(), c->h_sh_coff.size()))) return r;

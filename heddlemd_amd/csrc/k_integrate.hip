@@ -624,7 +624,7 @@ __device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, 
 // polymer chains can be one group.
 // ---------------------------------------------------------------------
 unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons) {
-  return 4u * (11u * max_atoms + 5u * max_cons);
+  return 4u * (11u * max_atoms + 6u * max_cons);
 }
 
 __device__ __forceinline__ void shake_min_image_to(float ax, float ay, float az, float& bx,
@@ -657,13 +657,14 @@ __global__ void k_shake_positions(HdShakeArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   const unsigned MA = a.max_atoms, MC = a.max_cons;
-  float* base = sh + (size_t)threadIdx.x * (11u * MA + 5u * MC);
+  float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
   float *x0 = base, *y0 = x0 + MA, *z0 = y0 + MA;
   float *xu = z0 + MA, *yu = xu + MA, *zu = yu + MA;
   float *xc = zu + MA, *yc = xc + MA, *zc = yc + MA;
   float *m = zc + MA, *im = m + MA;
   float *gx = im + MA, *gy = gx + MC, *gz = gy + MC, *ip = gz + MC;
   unsigned* cij = (unsigned*)(ip + MC);
+  float* lam = ip + 2u * MC;  // accumulated multiplier per constraint
   HdLattice L = hd_load_lattice(a.lattice);
   unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
   unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
@@ -685,6 +686,20 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     shake_min_image_to(x0[r], y0[r], z0[r], x0[k], y0[k], z0[k], L);
     shake_min_image_to(xu[r], yu[r], zu[r], xu[k], yu[k], zu[k], L);
   }
+  // longer groups are solved relative to their middle atom: |r|^2 - d^2
+  // then resolves to the f32 spacing of the group's extent instead of that
+  // of its position in the box (at 200 Bohr the latter is ~1e-4 Bohr^2,
+  // above a 5e-5 tolerance)
+  float ox = 0.f, oy = 0.f, oz = 0.f;
+  if (na > 8u) {
+    unsigned mid = na / 2;
+    float sx = x0[mid], sy = y0[mid], sz = z0[mid];
+    ox = xu[mid]; oy = yu[mid]; oz = zu[mid];
+    for (unsigned k = 0; k < na; ++k) {
+      x0[k] -= sx; y0[k] -= sy; z0[k] -= sz;
+      xu[k] -= ox; yu[k] -= oy; zu[k] -= oz;
+    }
+  }
   for (unsigned k = 0; k < na; ++k) { xc[k] = xu[k]; yc[k] = yu[k]; zc[k] = zu[k]; }
   for (unsigned k = 0; k < nc; ++k) {
     unsigned pk = a.cij[c0 + k];
@@ -694,6 +709,20 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     gy[k] = y0[li] - y0[lj];
     gz[k] = z0[li] - z0[lj];
     ip[k] = im[li] + im[lj];
+    lam[k] = 0.f;
+  }
+  if (a.lam0) {
+    // warm start: move along every constraint gradient by a predicted
+    // multiplier, in topology order, before the Gauss-Seidel sweeps
+    for (unsigned k = 0; k < nc; ++k) {
+      unsigned pk = cij[k];
+      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
+      float lambda = ip[k] > 0.f ? a.lam0[c0 + k] / ip[k] : 0.f;
+      float ki = lambda * im[li], kj = lambda * im[lj];
+      xc[li] -= ki * gx[k]; yc[li] -= ki * gy[k]; zc[li] -= ki * gz[k];
+      xc[lj] += kj * gx[k]; yc[lj] += kj * gy[k]; zc[lj] += kj * gz[k];
+      lam[k] = lambda;
+    }
   }
   int it = 0;
   for (; it < a.max_iter; ++it) {
@@ -713,11 +742,14 @@ __global__ void k_shake_positions(HdShakeArgs a) {
         float ki = lambda * im[li], kj = lambda * im[lj];
         xc[li] -= ki * gx[k]; yc[li] -= ki * gy[k]; zc[li] -= ki * gz[k];
         xc[lj] += kj * gx[k]; yc[lj] += kj * gy[k]; zc[lj] += kj * gz[k];
+        lam[k] += lambda;
       }
     }
     if (converged) break;
   }
   if (a.stats) shake_stats(a.stats, (unsigned)it, a.max_iter);
+  if (a.lam_out)
+    for (unsigned k = 0; k < nc; ++k) a.lam_out[c0 + k] = lam[k] * ip[k];
   float cxg, cyg, czg;
   shake_com(xc, yc, zc, m, na, cxg, cyg, czg);
   float inv_dt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
@@ -727,7 +759,8 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     float dxg = xc[k] - xu[k], dyg = yc[k] - yu[k], dzg = zc[k] - zu[k];
     a.vx[i] += dxg * inv_dt; a.vy[i] += dyg * inv_dt; a.vz[i] += dzg * inv_dt;
     float4 p = a.posq[i];
-    if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
+    if (k == 0 && na > 8u) { p.x = xc[0] + ox; p.y = yc[0] + oy; p.z = zc[0] + oz; }
+    else if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
     else { p.x += dxg; p.y += dyg; p.z += dzg; }
     a.posq[i] = p;
     float rx = xc[k] - cxg, ry = yc[k] - cyg, rz = zc[k] - czg;
@@ -742,7 +775,7 @@ __global__ void k_rattle_velocities(HdShakeArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   const unsigned MA = a.max_atoms, MC = a.max_cons;
-  float* base = sh + (size_t)threadIdx.x * (11u * MA + 5u * MC);
+  float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
   float *px = base, *py = px + MA, *pz = py + MA;
   float *v0 = pz + MA, *v1 = v0 + MA, *v2 = v1 + MA;
   float *dv0 = v2 + MA, *dv1 = dv0 + MA, *dv2 = dv1 + MA;
@@ -813,6 +846,90 @@ __global__ void k_rattle_velocities(HdShakeArgs a) {
     unsigned i = a.gatoms[a0 + k];
     a.vx[i] = v0[k]; a.vy[i] = v1[k]; a.vz[i] = v2[k];
   }
+}
+
+// Inputs of the SHAKE multiplier predictor, one thread per constraint,
+// all dimensionless.  For constraint k between atoms i, j with snapshot
+// bond g and unconstrained bond r (both minimum-imaged):
+//   nu1_k    = (|r|^2 - d^2) / (2 g.r), the first Gauss-Seidel multiplier;
+//   kappa_kl = c_kl (r_k.g_l) / (ip_l g_k.r_k), the coupling of the
+//              linearised system nu_k + sum_l kappa_kl nu_l = nu1_k, where
+//              c_kl = +-1/m of the atom that k and l share (0 if none) and
+//              ip = 1/m_i + 1/m_j;
+//   rho_k    = g.r / d^2 - 1.
+// Row k: [nu1_k, nu1_k-1, nu1_k+1, nu1_k-2, nu1_k+2, kappa_k,k-1,
+// kappa_k,k+1, rho_k], neighbours in the group's constraint order, zero
+// past the group's ends.
+struct ShakeBond {
+  float gx, gy, gz, rx, ry, rz, imi, imj, ip, gr, nu1;
+  unsigned li, lj;
+};
+
+__device__ __forceinline__ ShakeBond shake_bond(const HdShakeArgs& a, const HdLattice& L,
+                                                unsigned a0, unsigned c) {
+  ShakeBond b;
+  unsigned pk = a.cij[c];
+  b.li = pk & 0xFFu;
+  b.lj = (pk >> 8) & 0xFFu;
+  unsigned i = a.gatoms[a0 + b.li], j = a.gatoms[a0 + b.lj];
+  float4 si = a.snap[i], sj = a.snap[j], pi = a.posq[i], pj = a.posq[j];
+  b.gx = si.x - sj.x; b.gy = si.y - sj.y; b.gz = si.z - sj.z;
+  hd_min_image(b.gx, b.gy, b.gz, L);
+  b.rx = pi.x - pj.x; b.ry = pi.y - pj.y; b.rz = pi.z - pj.z;
+  hd_min_image(b.rx, b.ry, b.rz, L);
+  float mi = a.mass[i], mj = a.mass[j];
+  b.imi = mi > 0.f ? 1.0f / mi : 0.f;
+  b.imj = mj > 0.f ? 1.0f / mj : 0.f;
+  b.ip = b.imi + b.imj;
+  b.gr = b.gx * b.rx + b.gy * b.ry + b.gz * b.rz;
+  float sigma = (b.rx * b.rx + b.ry * b.ry + b.rz * b.rz) - a.cr2[c];
+  b.nu1 = b.gr != 0.f ? sigma / (2.0f * b.gr) : 0.f;
+  return b;
+}
+
+__device__ __forceinline__ float shake_coupling(const ShakeBond& k, const ShakeBond& l) {
+  float c = 0.f;
+  if (k.li == l.li) c += l.imi;
+  if (k.li == l.lj) c -= l.imj;
+  if (k.lj == l.li) c -= l.imi;
+  if (k.lj == l.lj) c += l.imj;
+  float den = l.ip * k.gr;
+  if (c == 0.f || den == 0.f) return 0.f;
+  float num = c * (k.rx * l.gx + k.ry * l.gy + k.rz * l.gz);
+  return num / den;
+}
+
+__global__ void k_shake_features(HdShakeArgs a) {
+  unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_cons) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned g = a.cgrp[c];
+  unsigned a0 = a.goff[g], cb = a.coff[g], ce = a.coff[g + 1];
+  ShakeBond k = shake_bond(a, L, a0, c);
+  float f[HD_SHAKE_NFEAT];
+  f[0] = k.nu1;
+  f[1] = f[2] = f[3] = f[4] = f[5] = f[6] = 0.f;
+  if (c >= cb + 1) {
+    ShakeBond l = shake_bond(a, L, a0, c - 1);
+    f[1] = l.nu1;
+    f[5] = shake_coupling(k, l);
+  }
+  if (c + 1 < ce) {
+    ShakeBond l = shake_bond(a, L, a0, c + 1);
+    f[2] = l.nu1;
+    f[6] = shake_coupling(k, l);
+  }
+  if (c >= cb + 2) f[3] = shake_bond(a, L, a0, c - 2).nu1;
+  if (c + 2 < ce) f[4] = shake_bond(a, L, a0, c + 2).nu1;
+  f[7] = k.gr / a.cr2[c] - 1.0f;
+  float4* out = reinterpret_cast<float4*>(a.feat + (size_t)c * HD_SHAKE_NFEAT);
+  out[0] = make_float4(f[0], f[1], f[2], f[3]);
+  out[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a) {
+  if (a.n_cons == 0) return;
+  hipLaunchKernelGGL(k_shake_features, dim3((a.n_cons + 255) / 256), dim3(256), 0, st, a);
 }
 
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a) {

@@ -1,0 +1,317 @@
+// Learned initial multipliers for SHAKE: inference of the 8 -> 64 -> 64 -> 1
+// predictor over the per-constraint feature rows, the step log that
+// collects (features, converged multiplier) rows from plain SHAKE, and the
+// training kernels (per-block gradient partials of 64 samples, an ordered
+// reduction with an Adam step).  The reference has no learned component;
+// the model layout is documented beside HD_MLP_* in hd_kernels.h.
+//
+// Every sum runs in a fixed order (samples within a block, blocks within
+// the reduction), so training is bit-reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include "hd_kernels.h"
+
+namespace {
+
+constexpr int F = HD_MLP_IN;
+constexpr int H = HD_MLP_H;
+constexpr int NS = HD_MLP_SAMPLES;
+constexpr int HP = H + 1;  // padded LDS row pitch (conflict-free column reads)
+constexpr int QS = NS / 4; // samples per wave of a 256-thread block
+
+// nu1 columns (0..4) and rho (7) in units of the scale; the couplings
+// (5, 6) are O(1) already
+__device__ __forceinline__ float mlp_in(int f, float x, float si) {
+  return (f == 5 || f == 6) ? x : x * si;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ P,
+                                                     const float* __restrict__ feat, unsigned n,
+                                                     float* __restrict__ nu) {
+  __shared__ float w[HD_MLP_NPARAM];
+  for (unsigned i = threadIdx.x; i < HD_MLP_NPARAM; i += blockDim.x) w[i] = P[i];
+  __syncthreads();
+  unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const float4* xr = reinterpret_cast<const float4*>(feat + (size_t)c * F);
+  float4 xa = xr[0], xb = xr[1];
+  float x[F] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+  float s = w[HD_MLP_SCALE];
+  float si = s > 0.f ? 1.0f / s : 0.f;
+  float u[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) u[f] = mlp_in(f, x[f], si);
+  float h1[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    float acc = w[HD_MLP_B1 + j];
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc += w[HD_MLP_W1 + j * F + f] * u[f];
+    h1[j] = tanhf(acc);
+  }
+  float o = w[HD_MLP_B3];
+  for (int j = 0; j < H; ++j) {
+    float acc = w[HD_MLP_B2 + j];
+#pragma unroll
+    for (int i = 0; i < H; ++i) acc += w[HD_MLP_W2 + j * H + i] * h1[i];
+    o += w[HD_MLP_W3 + j] * tanhf(acc);
+  }
+  nu[c] = x[0] + s * o;
+}
+
+// One block = 64 samples, 256 threads (4 waves).  Forward activations stay
+// in LDS and are overwritten in place by their gradients; lane = hidden
+// unit throughout, so every LDS row access is contiguous or a broadcast.
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P,
+                                                  const float* __restrict__ X,
+                                                  const float* __restrict__ T, uint64_t n,
+                                                  uint64_t first, unsigned count, uint64_t mul,
+                                                  uint64_t add, float inv_count,
+                                                  float* __restrict__ part) {
+  __shared__ float w[HD_MLP_NPARAM];
+  __shared__ float w2t[H * HP];  // W2 transposed: w2t[i * HP + j] = W2[j][i]
+  __shared__ float u[NS][F + 1];
+  __shared__ float tgt[NS];      // scaled target, then dL/do
+  __shared__ float sq[NS];       // squared residual
+  __shared__ float h1[NS * HP];  // tanh of layer 1, then dL/d(pre-activation 1)
+  __shared__ float h2[NS * HP];  // tanh of layer 2, then dL/d(pre-activation 2)
+  __shared__ unsigned valid[NS];
+  const unsigned t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  for (unsigned i = t; i < HD_MLP_NPARAM; i += 256) w[i] = P[i];
+  for (unsigned e = t; e < H * H; e += 256) {
+    unsigned j = e / H, i = e % H;
+    w2t[i * HP + j] = P[HD_MLP_W2 + e];
+  }
+  const float s = P[HD_MLP_SCALE];
+  const float si = s > 0.f ? 1.0f / s : 0.f;
+  if (t < NS) {
+    unsigned q = blockIdx.x * NS + t;
+    unsigned ok = q < count ? 1u : 0u;
+    float xr[F] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float tr = 0.f;
+    if (ok) {
+      uint64_t idx = (mul * (first + q) + add) % n;
+      const float4* p4 = reinterpret_cast<const float4*>(X + idx * F);
+      float4 xa = p4[0], xb = p4[1];
+      xr[0] = xa.x; xr[1] = xa.y; xr[2] = xa.z; xr[3] = xa.w;
+      xr[4] = xb.x; xr[5] = xb.y; xr[6] = xb.z; xr[7] = xb.w;
+      tr = (T[idx] - xr[0]) * si;
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) u[t][f] = mlp_in(f, xr[f], si);
+    tgt[t] = tr;
+    valid[t] = ok;
+  }
+  __syncthreads();
+  // layer 1: unit = lane, samples wv + 4 q
+  {
+    float wr[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) wr[f] = w[HD_MLP_W1 + lane * F + f];
+    float b = w[HD_MLP_B1 + lane];
+    for (int q = 0; q < QS; ++q) {
+      unsigned sm = wv + 4u * q;
+      float acc = b;
+#pragma unroll
+      for (int f = 0; f < F; ++f) acc += wr[f] * u[sm][f];
+      h1[sm * HP + lane] = tanhf(acc);
+    }
+  }
+  __syncthreads();
+  // layer 2
+  {
+    float acc[QS];
+    float b = w[HD_MLP_B2 + lane];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) acc[q] = b;
+    for (int i = 0; i < H; ++i) {
+      float wji = w2t[i * HP + lane];
+#pragma unroll
+      for (int q = 0; q < QS; ++q) acc[q] += wji * h1[(wv + 4u * q) * HP + i];
+    }
+#pragma unroll
+    for (int q = 0; q < QS; ++q) h2[(wv + 4u * q) * HP + lane] = tanhf(acc[q]);
+  }
+  __syncthreads();
+  // output, residual, dL/do
+  if (t < NS) {
+    float o = w[HD_MLP_B3];
+    for (int j = 0; j < H; ++j) o += w[HD_MLP_W3 + j] * h2[t * HP + j];
+    float r = o - tgt[t];
+    tgt[t] = valid[t] ? 2.0f * r * inv_count : 0.f;
+    sq[t] = valid[t] ? r * r : 0.f;
+  }
+  __syncthreads();
+  float* out = part + (size_t)blockIdx.x * HD_MLP_PSTRIDE;
+  if (t < H) {
+    float g = 0.f;
+    for (int sm = 0; sm < NS; ++sm) g += tgt[sm] * h2[sm * HP + t];
+    out[HD_MLP_W3 + t] = g;
+  } else if (t == H) {
+    float g = 0.f, l = 0.f;
+    for (int sm = 0; sm < NS; ++sm) { g += tgt[sm]; l += sq[sm]; }
+    out[HD_MLP_B3] = g;
+    out[HD_MLP_NTRAIN] = l;
+  }
+  __syncthreads();
+  // dL/d(pre-activation 2) in place of h2
+  {
+    float w3 = w[HD_MLP_W3 + lane];
+    for (int q = 0; q < QS; ++q) {
+      unsigned sm = wv + 4u * q;
+      float hv = h2[sm * HP + lane];
+      h2[sm * HP + lane] = tgt[sm] * w3 * (1.0f - hv * hv);
+    }
+  }
+  __syncthreads();
+  // gW2[j][i] = sum_s D2[s][j] h1[s][i]: i = lane, j = 16 wv + q
+  {
+    float acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+    for (int sm = 0; sm < NS; ++sm) {
+      float hv = h1[sm * HP + lane];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] += h2[sm * HP + 16u * wv + q] * hv;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) out[HD_MLP_W2 + (16u * wv + q) * H + lane] = acc[q];
+    if (wv == 0) {
+      float g = 0.f;
+      for (int sm = 0; sm < NS; ++sm) g += h2[sm * HP + lane];
+      out[HD_MLP_B2 + lane] = g;
+    }
+  }
+  __syncthreads();
+  // dL/d(pre-activation 1) in place of h1: unit i = lane
+  {
+    float acc[QS];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) acc[q] = 0.f;
+    for (int j = 0; j < H; ++j) {
+      float wji = w[HD_MLP_W2 + j * H + lane];
+#pragma unroll
+      for (int q = 0; q < QS; ++q) acc[q] += h2[(wv + 4u * q) * HP + j] * wji;
+    }
+#pragma unroll
+    for (int q = 0; q < QS; ++q) {
+      unsigned sm = wv + 4u * q;
+      float hv = h1[sm * HP + lane];
+      h1[sm * HP + lane] = acc[q] * (1.0f - hv * hv);
+    }
+  }
+  __syncthreads();
+  // gW1[i][f] = sum_s D1[s][i] u[s][f]: i = lane, f = 2 wv, 2 wv + 1
+  {
+    float g0 = 0.f, g1 = 0.f;
+    unsigned f0 = 2u * wv;
+    for (int sm = 0; sm < NS; ++sm) {
+      float d = h1[sm * HP + lane];
+      g0 += d * u[sm][f0];
+      g1 += d * u[sm][f0 + 1];
+    }
+    out[HD_MLP_W1 + lane * F + f0] = g0;
+    out[HD_MLP_W1 + lane * F + f0 + 1] = g1;
+    if (wv == 0) {
+      float g = 0.f;
+      for (int sm = 0; sm < NS; ++sm) g += h1[sm * HP + lane];
+      out[HD_MLP_B1 + lane] = g;
+    }
+  }
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, unsigned nb, float* __restrict__ P,
+                             float* __restrict__ grad_out, HdAdam adam, int use_adam,
+                             double* __restrict__ loss_acc) {
+  unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < HD_MLP_NTRAIN) {
+    float g = 0.f;
+    for (unsigned b = 0; b < nb; ++b) g += part[(size_t)b * HD_MLP_PSTRIDE + p];
+    if (grad_out) grad_out[p] = g;
+    if (use_adam) {
+      float m = adam.b1 * adam.m[p] + (1.0f - adam.b1) * g;
+      float v = adam.b2 * adam.v[p] + (1.0f - adam.b2) * g * g;
+      adam.m[p] = m;
+      adam.v[p] = v;
+      P[p] -= adam.lr_t * m / (sqrtf(v) + adam.eps);
+    }
+  } else if (p == HD_MLP_NTRAIN && loss_acc) {
+    double l = 0.0;
+    for (unsigned b = 0; b < nb; ++b) l += (double)part[(size_t)b * HD_MLP_PSTRIDE + p];
+    loss_acc[0] += l;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_mlp_scale(const float* __restrict__ x, uint64_t n,
+                                                    float* __restrict__ P) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (uint64_t i = threadIdx.x; i < n; i += 1024) {
+    double v = x[i * F];
+    acc += v * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (unsigned h = 512; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double ms = n ? red[0] / (double)n : 0.0;
+    P[HD_MLP_SCALE] = ms > 0.0 ? (float)sqrt(ms) : 1.0f;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_shake_log(const float* __restrict__ feat,
+                                                    const float* __restrict__ nu, unsigned n,
+                                                    float* __restrict__ lx, float* __restrict__ lt,
+                                                    unsigned long long* counter,
+                                                    unsigned long long cap) {
+  unsigned long long slot = counter[0];
+  if (slot >= cap) return;  // uniform over the block
+  const float4* src = reinterpret_cast<const float4*>(feat);
+  float4* dst = reinterpret_cast<float4*>(lx + slot * (unsigned long long)n * F);
+  for (unsigned i = threadIdx.x; i < n * (F / 4); i += 1024) dst[i] = src[i];
+  float* dt = lt + slot * (unsigned long long)n;
+  for (unsigned i = threadIdx.x; i < n; i += 1024) dt[i] = nu[i];
+  __syncthreads();
+  if (threadIdx.x == 0) counter[0] = slot + 1;
+}
+
+}  // namespace
+
+void hd_launch_mlp_predict(hipStream_t st, const float* params, const float* feat, unsigned n,
+                           float* nu) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mlp_predict, dim3((n + 255) / 256), dim3(256), 0, st, params, feat, n, nu);
+}
+
+unsigned hd_launch_mlp_grad(hipStream_t st, const float* params, const float* x,
+                            const float* t, uint64_t n, uint64_t first, unsigned count,
+                            uint64_t mul, uint64_t add, float inv_count, float* partials) {
+  unsigned nb = (count + NS - 1) / NS;
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, st, params, x, t, n, first, count, mul,
+                     add, inv_count, partials);
+  return nb;
+}
+
+void hd_launch_mlp_reduce(hipStream_t st, const float* partials, unsigned nb, float* params,
+                          float* grad_out, const HdAdam* adam, double* loss_acc) {
+  HdAdam a{};
+  if (adam) a = *adam;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((HD_MLP_NTRAIN + 1 + 255) / 256), dim3(256), 0, st,
+                     partials, nb, params, grad_out, a, adam ? 1 : 0, loss_acc);
+}
+
+void hd_launch_mlp_scale(hipStream_t st, const float* x, uint64_t n, float* params) {
+  hipLaunchKernelGGL(k_mlp_scale, dim3(1), dim3(1024), 0, st, x, n, params);
+}
+
+void hd_launch_shake_log(hipStream_t st, const float* feat, const float* nu, unsigned n,
+                         float* log_x, float* log_t, unsigned long long* counter,
+                         unsigned long long cap) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_shake_log, dim3(1), dim3(1024), 0, st, feat, nu, n, log_x, log_t, counter,
+                     cap);
+}

@@ -29,6 +29,7 @@ class Engine:
         if rc != 0:
             raise HeddleError(f"hd_ctx_create({device}) failed with {rc}")
         self.n = 0
+        self._n_constraints_shake = 0
         self._keep = []
 
     def close(self):
@@ -131,6 +132,7 @@ class Engine:
                                          np.asarray(gat, np.uint32), np.asarray(coff, np.uint32),
                                          np.asarray(ci, np.uint8), np.asarray(cj, np.uint8),
                                          np.asarray(d, np.float64)), "hd_set_shake")
+        self._n_constraints_shake = len(ci)
 
     def set_shake_tolerance(self, tol, rattle_tol, max_iter):
         self._chk(self._lib.hd_set_shake_tolerance(self._ctx, float(tol), float(rattle_tol),
@@ -142,6 +144,53 @@ class Engine:
         keys = ["shake_sweeps", "shake_max", "shake_capped", "rattle_sweeps", "rattle_max",
                 "rattle_capped"]
         return dict(zip(keys, (int(x) for x in out)))
+
+    def set_shake_predictor(self, params):
+        """Warm-start every SHAKE solve from the predictor with these
+        parameters (a ShakePredictor or its parameter array); None turns
+        prediction off."""
+        if params is None:
+            ptr = None
+        else:
+            if isinstance(params, ShakePredictor):
+                params = params.params
+            params = np.ascontiguousarray(params, np.float32)
+            if params.size != self._lib.hd_mlp_param_count():
+                raise ValueError("predictor parameter count mismatch")
+            ptr = params.ctypes.data
+        self._chk(self._lib.hd_set_shake_predictor(self._ctx, ptr), "hd_set_shake_predictor")
+
+    def shake_multipliers(self):
+        """(predicted nu0, converged nu) of the last SHAKE solve."""
+        n = int(self._n_constraints_shake)
+        nu0 = np.zeros(n, np.float32)
+        nu = np.zeros(n, np.float32)
+        self._chk(self._lib.hd_get_shake_multipliers(self._ctx, nu0.ctypes.data, nu.ctypes.data),
+                  "hd_get_shake_multipliers")
+        return nu0, nu
+
+    def shake_log_start(self, max_steps):
+        self._chk(self._lib.hd_shake_log_start(self._ctx, int(max_steps)), "hd_shake_log_start")
+
+    def shake_log_stop(self):
+        self._chk(self._lib.hd_shake_log_stop(self._ctx), "hd_shake_log_stop")
+
+    def shake_log_size(self):
+        steps, rows = C.c_uint64(0), C.c_uint64(0)
+        self._chk(self._lib.hd_shake_log_size(self._ctx, C.byref(steps), C.byref(rows)),
+                  "hd_shake_log_size")
+        return int(steps.value), int(rows.value)
+
+    def shake_log(self, first_step=0, n_steps=None):
+        """Logged (features [rows, 8], converged nu [rows])."""
+        steps, rows = self.shake_log_size()
+        if n_steps is None:
+            n_steps = steps - first_step
+        x = np.zeros((n_steps * rows, 8), np.float32)
+        t = np.zeros(n_steps * rows, np.float32)
+        self._chk(self._lib.hd_shake_log_read(self._ctx, int(first_step), int(n_steps), x, t),
+                  "hd_shake_log_read")
+        return x, t
 
     def set_langevin(self, friction, kt, seed):
         self._chk(self._lib.hd_set_langevin(self._ctx, float(friction), float(kt), int(seed)),
@@ -307,6 +356,80 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_velocities(velocities)
     e.setup()
     return e
+
+
+class ShakePredictor:
+    """The SHAKE multiplier predictor (hd_mlp_*): an 8 -> 64 -> 64 -> 1
+    tanh network trained on the device."""
+
+    def __init__(self, device=0, seed=1):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        rc = self._lib.hd_mlp_create(int(device), int(seed), C.byref(h))
+        if rc != 0:
+            raise _lib.HeddleError(f"hd_mlp_create failed ({rc})")
+        self._h = h
+        self.n_params = self._lib.hd_mlp_param_count()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._lib.hd_mlp_last_error(self._h).decode()
+            raise _lib.HeddleError(f"{what} failed ({rc}): {msg}")
+
+    @property
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._chk(self._lib.hd_mlp_get_params(self._h, p), "hd_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("parameter count mismatch")
+        self._chk(self._lib.hd_mlp_set_params(self._h, p), "hd_mlp_set_params")
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        nu = np.zeros(x.shape[0], np.float32)
+        self._chk(self._lib.hd_mlp_predict(self._h, x, x.shape[0], nu), "hd_mlp_predict")
+        return nu
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n_params - 1, np.float32)
+        loss = C.c_double(0)
+        self._chk(self._lib.hd_mlp_gradient(self._h, x, t, x.shape[0], g, C.byref(loss)),
+                  "hd_mlp_gradient")
+        return float(loss.value), g
+
+    def train(self, x, t, epochs=1, batch=1024, lr=1e-3, seed=7):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = np.zeros(epochs, np.float64)
+        self._chk(self._lib.hd_mlp_train(self._h, x, t, x.shape[0], int(epochs), int(batch),
+                                         float(lr), int(seed), loss.ctypes.data),
+                  "hd_mlp_train")
+        return loss
+
+    def train_on_log(self, engine, epochs=1, batch=1024, lr=1e-3, seed=7):
+        loss = np.zeros(epochs, np.float64)
+        self._chk(self._lib.hd_mlp_train_log(self._h, engine._ctx, int(epochs), int(batch),
+                                             float(lr), int(seed), loss.ctypes.data),
+                  "hd_mlp_train_log")
+        return loss
+
+    def close(self):
+        if self._h:
+            self._lib.hd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

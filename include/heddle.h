@@ -171,6 +171,65 @@ int hd_batch_step(hd_batch *batch, int n_steps, int scalars_last);
 int hd_batch_set_lanes(hd_batch *batch, int lanes);
 const char *hd_batch_last_error(const hd_batch *batch);
 
+/* Learned initial multipliers for SHAKE.  No counterpart in the reference
+ * (whose SHAKE starts every solve from zero, shake.cu:251-330).
+ *
+ * Per constraint the device forms 8 dimensionless features: the first
+ * Gauss-Seidel multiplier nu1 = (|r|^2 - d^2) / (2 g.r) of the constraint
+ * and of its neighbours k-1, k+1, k-2, k+2 in the group's constraint
+ * order, the linearised couplings to k-1 and k+1, and g.r / d^2 - 1
+ * (g: snapshot bond, r: unconstrained bond).  Multipliers are exchanged in
+ * the form nu = lambda (1/m_i + 1/m_j).  A predictor (hd_mlp) maps the
+ * features to nu; SHAKE then starts from the predicted nu and runs its
+ * usual sweeps to the same tolerance.
+ *
+ * hd_set_shake_predictor: params = HD_MLP_NPARAM floats (hd_mlp_get_params)
+ * or NULL to switch prediction off.  Needs SHAKE groups and hd_setup. */
+int hd_set_shake_predictor(hd_ctx *ctx, const float *params);
+/* Predicted (nu0, zero without a predictor) and converged (nu) multipliers
+ * of the last SHAKE solve, one per constraint in hd_set_shake order; either
+ * pointer may be NULL. */
+int hd_get_shake_multipliers(hd_ctx *ctx, float *nu0, float *nu);
+/* Step log of SHAKE solves, device-resident: every step appends its
+ * feature rows and converged nu until max_steps steps are held (later steps
+ * are not logged).  Start resets the log. */
+int hd_shake_log_start(hd_ctx *ctx, uint64_t max_steps);
+int hd_shake_log_stop(hd_ctx *ctx);
+int hd_shake_log_size(hd_ctx *ctx, uint64_t *steps, uint64_t *rows_per_step);
+/* Copies logged steps first_step .. first_step + n_steps - 1: x gets
+ * 8 floats per row, t one. */
+int hd_shake_log_read(hd_ctx *ctx, uint64_t first_step, uint64_t n_steps, float *x, float *t);
+
+/* The predictor: an 8 -> 64 -> 64 -> 1 tanh network (4,801 trained
+ * parameters plus the feature scale s) on one device.  Scaled inputs
+ * u = x / s (features 0-4 and 7) and x (5, 6); prediction
+ * nu = x0 + s (w3 . tanh(W2 tanh(W1 u + b1) + b2) + b3).  Parameter pack:
+ * W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3, s.  Training
+ * minimises the mean of ((nu - t) / s)^2 with Adam (0.9, 0.999, 1e-8);
+ * s <= 0 is replaced by the rms of column 0 of the training rows.
+ * Training is deterministic: fixed-order sums, seeded row order. */
+typedef struct hd_mlp hd_mlp;
+int hd_mlp_param_count(void);
+int hd_mlp_create(int device, uint64_t seed, hd_mlp **out);
+void hd_mlp_destroy(hd_mlp *mlp);
+const char *hd_mlp_last_error(const hd_mlp *mlp);
+/* Setting parameters also resets the Adam state. */
+int hd_mlp_set_params(hd_mlp *mlp, const float *params);
+int hd_mlp_get_params(hd_mlp *mlp, float *params);
+/* nu[i] for the feature rows x[8 i .. 8 i + 7]. */
+int hd_mlp_predict(hd_mlp *mlp, const float *x, uint64_t n, float *nu);
+/* Gradient of the loss over all n rows (HD_MLP_NTRAIN = param_count - 1
+ * floats, the scale excluded) and the loss itself; no update. */
+int hd_mlp_gradient(hd_mlp *mlp, const float *x, const float *t, uint64_t n, float *grad,
+                    double *loss);
+/* Adam over `epochs` passes of mini-batches of `batch` rows (a multiple of
+ * 64); loss[e] = mean loss during epoch e (may be NULL). */
+int hd_mlp_train(hd_mlp *mlp, const float *x, const float *t, uint64_t n, int epochs, int batch,
+                 float lr, uint64_t seed, double *loss);
+/* The same on a context's SHAKE log, read in place on the device. */
+int hd_mlp_train_log(hd_mlp *mlp, hd_ctx *ctx, int epochs, int batch, float lr, uint64_t seed,
+                     double *loss);
+
 /* Downloads (caller-owned host arrays of n entries; posq has 4n). */
 int hd_get_positions(hd_ctx *ctx, float *posq, int32_t *ix, int32_t *iy, int32_t *iz);
 int hd_get_velocities(hd_ctx *ctx, float *vx, float *vy, float *vz);

@@ -786,7 +786,8 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
                             const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
                             const float *cr2, float *posq, float *vx, float *vy, float *vz,
                             const float *snap, const float *mass, const float *L, float dt,
-                            float tol, int max_iter, float *w_cons, int *iters)
+                            float tol, int max_iter, float *w_cons, int *iters,
+                            const float *nu0, float *nu_out)
 {
     /* shake_positions (shake.cu:96-330): Gauss-Seidel sweeps in topology
      * order with gradients from the pre-drift snapshot, in the image of
@@ -814,15 +815,42 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             min_image_to_f(x0[r], y0[r], z0[r], &x0[a], &y0[a], &z0[a], L);
             min_image_to_f(xu[r], yu[r], zu[r], &xu[a], &yu[a], &zu[a], L);
         }
+        /* longer groups are solved relative to their middle atom, so that
+         * |r|^2 - d^2 resolves to the f32 spacing of the group's extent */
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+        if (na > OR_SHAKE_REF_ATOMS) {
+            int mid = na / 2;
+            float sx = x0[mid], sy = y0[mid], sz = z0[mid];
+            ox = xu[mid]; oy = yu[mid]; oz = zu[mid];
+            for (int a = 0; a < na; ++a) {
+                x0[a] -= sx; y0[a] -= sy; z0[a] -= sz;
+                xu[a] -= ox; yu[a] -= oy; zu[a] -= oz;
+            }
+        }
         for (int a = 0; a < na; ++a) { xc[a] = xu[a]; yc[a] = yu[a]; zc[a] = zu[a]; }
         float gx[OR_SHAKE_MAXC], gy[OR_SHAKE_MAXC], gz[OR_SHAKE_MAXC], ip[OR_SHAKE_MAXC];
         const uint8_t *li = ci + coff[g], *lj = cj + coff[g];
         const float *r2 = cr2 + coff[g];
+        float lam[OR_SHAKE_MAXC];
         for (int k = 0; k < nc; ++k) {
             gx[k] = x0[li[k]] - x0[lj[k]];
             gy[k] = y0[li[k]] - y0[lj[k]];
             gz[k] = z0[li[k]] - z0[lj[k]];
             ip[k] = im[li[k]] + im[lj[k]];
+            lam[k] = 0.0f;
+        }
+        if (nu0) {
+            /* warm start (no reference counterpart): one move along every
+             * constraint gradient by the given nu = lambda (1/m_i + 1/m_j),
+             * in topology order, before the sweeps */
+            for (int k = 0; k < nc; ++k) {
+                int a = li[k], b = lj[k];
+                float lambda = ip[k] > 0.0f ? nu0[coff[g] + k] / ip[k] : 0.0f;
+                float ki = lambda * im[a], kj = lambda * im[b];
+                xc[a] -= ki * gx[k]; yc[a] -= ki * gy[k]; zc[a] -= ki * gz[k];
+                xc[b] += kj * gx[k]; yc[b] += kj * gy[k]; zc[b] += kj * gz[k];
+                lam[k] = lambda;
+            }
         }
         int it = 0;
         for (; it < max_iter; ++it) {
@@ -841,11 +869,14 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
                     float ki = lambda * im[a], kj = lambda * im[b];
                     xc[a] -= ki * gx[k]; yc[a] -= ki * gy[k]; zc[a] -= ki * gz[k];
                     xc[b] += kj * gx[k]; yc[b] += kj * gy[k]; zc[b] += kj * gz[k];
+                    lam[k] += lambda;
                 }
             }
             if (converged) break;
         }
         if (iters) iters[g] = it;
+        if (nu_out)
+            for (int k = 0; k < nc; ++k) nu_out[coff[g] + k] = lam[k] * ip[k];
         float cxg, cyg, czg;
         weighted_com_f(xc, yc, zc, m, na, &cxg, &cyg, &czg);
         float inv_dt = dt != 0.0f ? 1.0f / dt : 0.0f;
@@ -854,7 +885,10 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             uint32_t i = at[a];
             float dxg = xc[a] - xu[a], dyg = yc[a] - yu[a], dzg = zc[a] - zu[a];
             vx[i] += dxg * inv_dt; vy[i] += dyg * inv_dt; vz[i] += dzg * inv_dt;
-            if (a == 0) {
+            if (a == 0 && na > OR_SHAKE_REF_ATOMS) {
+                posq[4 * i] = xc[0] + ox; posq[4 * i + 1] = yc[0] + oy;
+                posq[4 * i + 2] = zc[0] + oz;
+            } else if (a == 0) {
                 posq[4 * i] = xc[0]; posq[4 * i + 1] = yc[0]; posq[4 * i + 2] = zc[0];
             } else {
                 posq[4 * i] += dxg; posq[4 * i + 1] += dyg; posq[4 * i + 2] += dzg;
@@ -862,6 +896,87 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             float rx = xc[a] - cxg, ry = yc[a] - cyg, rz = zc[a] - czg;
             float scale = m[a] * inv_dt2;
             w_cons[i] = scale * (dxg * rx + dyg * ry + dzg * rz);
+        }
+    }
+}
+
+/* Predictor inputs per constraint (no reference counterpart), f32 in the
+ * device's operation order: for constraint k with snapshot bond g and
+ * unconstrained bond r (each minimum-imaged), nu1 = (|r|^2 - d^2)/(2 g.r);
+ * coupling kappa_kl = c_kl (r_k.g_l) / (ip_l g_k.r_k) with c_kl the signed
+ * inverse mass of the atom shared by k and l; rho = g.r/d^2 - 1.
+ * Row: nu1_k, nu1_k-1, nu1_k+1, nu1_k-2, nu1_k+2, kappa_k,k-1, kappa_k,k+1,
+ * rho_k (zero past the group's ends). */
+typedef struct {
+    float gx, gy, gz, rx, ry, rz, imi, imj, ip, gr, nu1;
+    int li, lj;
+} or_bond;
+
+static or_bond or_shake_bond(const uint32_t *at, const uint8_t *ci, const uint8_t *cj,
+                             const float *cr2, const float *posq, const float *snap,
+                             const float *mass, const float *L, int c)
+{
+    or_bond b;
+    b.li = ci[c];
+    b.lj = cj[c];
+    uint32_t i = at[b.li], j = at[b.lj];
+    b.gx = snap[4 * i] - snap[4 * j];
+    b.gy = snap[4 * i + 1] - snap[4 * j + 1];
+    b.gz = snap[4 * i + 2] - snap[4 * j + 2];
+    min_image_f(&b.gx, &b.gy, &b.gz, L);
+    b.rx = posq[4 * i] - posq[4 * j];
+    b.ry = posq[4 * i + 1] - posq[4 * j + 1];
+    b.rz = posq[4 * i + 2] - posq[4 * j + 2];
+    min_image_f(&b.rx, &b.ry, &b.rz, L);
+    float mi = mass[i], mj = mass[j];
+    b.imi = mi > 0.0f ? 1.0f / mi : 0.0f;
+    b.imj = mj > 0.0f ? 1.0f / mj : 0.0f;
+    b.ip = b.imi + b.imj;
+    b.gr = b.gx * b.rx + b.gy * b.ry + b.gz * b.rz;
+    float sigma = (b.rx * b.rx + b.ry * b.ry + b.rz * b.rz) - cr2[c];
+    b.nu1 = b.gr != 0.0f ? sigma / (2.0f * b.gr) : 0.0f;
+    return b;
+}
+
+static float or_coupling(const or_bond *k, const or_bond *l)
+{
+    float c = 0.0f;
+    if (k->li == l->li) c += l->imi;
+    if (k->li == l->lj) c -= l->imj;
+    if (k->lj == l->li) c -= l->imi;
+    if (k->lj == l->lj) c += l->imj;
+    float den = l->ip * k->gr;
+    if (c == 0.0f || den == 0.0f) return 0.0f;
+    float num = c * (k->rx * l->gx + k->ry * l->gy + k->rz * l->gz);
+    return num / den;
+}
+
+void or_shake_features_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
+                           const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
+                           const float *cr2, const float *posq, const float *snap,
+                           const float *mass, const float *L, float *feat)
+{
+    for (int g = 0; g < n_groups; ++g) {
+        const uint32_t *at = gatoms + goff[g];
+        int cb = (int)coff[g], ce = (int)coff[g + 1];
+        for (int c = cb; c < ce; ++c) {
+            float *f = feat + 8 * (size_t)c;
+            or_bond k = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c);
+            for (int q = 0; q < 8; ++q) f[q] = 0.0f;
+            f[0] = k.nu1;
+            if (c >= cb + 1) {
+                or_bond l = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c - 1);
+                f[1] = l.nu1;
+                f[5] = or_coupling(&k, &l);
+            }
+            if (c + 1 < ce) {
+                or_bond l = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c + 1);
+                f[2] = l.nu1;
+                f[6] = or_coupling(&k, &l);
+            }
+            if (c >= cb + 2) f[3] = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c - 2).nu1;
+            if (c + 2 < ce) f[4] = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c + 2).nu1;
+            f[7] = k.gr / cr2[c] - 1.0f;
         }
     }
 }

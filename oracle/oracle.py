@@ -60,7 +60,10 @@ def lib():
                                       _f64p, _f64p]
         L.or_shake_positions_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, _f32p,
                                              _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
-                                             C.c_float, C.c_float, C.c_int, _f32p, _i32p]
+                                             C.c_float, C.c_float, C.c_int, _f32p, _i32p,
+                                             C.c_void_p, C.c_void_p]
+        L.or_shake_features_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, _f32p,
+                                            _f32p, _f32p, _f32p, _f32p, _f32p]
         L.or_rattle_velocities_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p,
                                                _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                                C.c_float, C.c_float, C.c_int, _f32p, _i32p]
@@ -272,16 +275,39 @@ class ShakeGroups:
         self.n = len(groups)
 
 
-def shake_positions(G, posq, v, snap, mass, lattice, dt, tol=SHAKE_TOL, max_iter=SHAKE_MAX_ITER):
+def shake_positions(G, posq, v, snap, mass, lattice, dt, tol=SHAKE_TOL, max_iter=SHAKE_MAX_ITER,
+                    nu0=None, nu_out=None):
     """In-place SHAKE (shake.cu shake_positions), f32; returns (w_cons
-    per atom, sweeps per group)."""
+    per atom, sweeps per group).  nu0 (one per constraint) warm-starts the
+    solve with multipliers nu = lambda (1/m_i + 1/m_j); nu_out receives
+    the total nu of each constraint."""
     w = np.zeros(posq.shape[0], np.float32)
     iters = np.zeros(max(G.n, 1), np.int32)
+    p0 = None
+    if nu0 is not None:
+        nu0 = np.ascontiguousarray(nu0, np.float32)
+        p0 = nu0.ctypes.data
+    p1 = None
+    if nu_out is not None:
+        assert nu_out.dtype == np.float32 and nu_out.flags.c_contiguous
+        p1 = nu_out.ctypes.data
     lib().or_shake_positions_f32(G.n, G.goff, G.atoms, G.coff, G.ci, G.cj, G.r2,
                                  posq.reshape(-1), v[0], v[1], v[2], snap.reshape(-1), mass,
                                  np.ascontiguousarray(lattice, np.float32), np.float32(dt),
-                                 np.float32(tol), int(max_iter), w, iters)
+                                 np.float32(tol), int(max_iter), w, iters, p0, p1)
     return w, iters[:G.n]
+
+
+def shake_features(G, posq, snap, mass, lattice):
+    """Predictor inputs per constraint (8 floats per row), f32 in the
+    device's operation order (no reference counterpart)."""
+    out = np.zeros((len(G.r2), 8), np.float32)
+    lib().or_shake_features_f32(G.n, G.goff, G.atoms, G.coff, G.ci, G.cj, G.r2,
+                                np.ascontiguousarray(posq, np.float32).reshape(-1),
+                                np.ascontiguousarray(snap, np.float32).reshape(-1),
+                                np.ascontiguousarray(mass, np.float32),
+                                np.ascontiguousarray(lattice, np.float32), out.reshape(-1))
+    return out
 
 
 def rattle_velocities(G, posq, v, mass, lattice, dt, w_cons=None, tol=RATTLE_TOL,
@@ -420,3 +446,73 @@ def kinetic_energy(v, mass):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+# ---------------------------------------------------------------------
+# SHAKE multiplier predictor (no reference counterpart): the 8 -> 64 ->
+# 64 -> 1 tanh network in FP64, forward and analytic backward.
+# Parameter pack: W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3, s.
+MLP_IN, MLP_H = 8, 64
+MLP_NTRAIN = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1
+MLP_NPARAM = MLP_NTRAIN + 1
+
+
+def mlp_unpack(p):
+    p = np.asarray(p, np.float64)
+    o = 0
+    W1 = p[o:o + MLP_H * MLP_IN].reshape(MLP_H, MLP_IN); o += MLP_H * MLP_IN
+    b1 = p[o:o + MLP_H]; o += MLP_H
+    W2 = p[o:o + MLP_H * MLP_H].reshape(MLP_H, MLP_H); o += MLP_H * MLP_H
+    b2 = p[o:o + MLP_H]; o += MLP_H
+    w3 = p[o:o + MLP_H]; o += MLP_H
+    b3 = p[o]
+    s = p[o + 1]
+    return W1, b1, W2, b2, w3, b3, s
+
+
+def mlp_inputs(x, s):
+    """Scaled inputs: the nu1 columns (0-4) and rho (7) divided by s, the
+    couplings (5, 6) as they are."""
+    x = np.asarray(x, np.float64).reshape(-1, MLP_IN)
+    si = 1.0 / s if s > 0 else 0.0
+    u = x * si
+    u[:, 5:7] = x[:, 5:7]
+    return u
+
+
+def mlp_forward(p, x):
+    """Predicted nu = x0 + s o(x) per row."""
+    W1, b1, W2, b2, w3, b3, s = mlp_unpack(p)
+    x = np.asarray(x, np.float64).reshape(-1, MLP_IN)
+    u = mlp_inputs(x, s)
+    h1 = np.tanh(u @ W1.T + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    o = h2 @ w3 + b3
+    return x[:, 0] + s * o
+
+
+def mlp_loss_grad(p, x, t):
+    """Mean of (o - (t - x0)/s)^2 over the rows and its gradient with
+    respect to the trained parameters (the pack without s)."""
+    W1, b1, W2, b2, w3, b3, s = mlp_unpack(p)
+    x = np.asarray(x, np.float64).reshape(-1, MLP_IN)
+    t = np.asarray(t, np.float64)
+    n = x.shape[0]
+    si = 1.0 / s if s > 0 else 0.0
+    u = mlp_inputs(x, s)
+    h1 = np.tanh(u @ W1.T + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    o = h2 @ w3 + b3
+    r = o - (t - x[:, 0]) * si
+    loss = float(np.mean(r * r))
+    d3 = 2.0 * r / n
+    gw3 = h2.T @ d3
+    gb3 = d3.sum()
+    D2 = np.outer(d3, w3) * (1.0 - h2 * h2)
+    gW2 = D2.T @ h1
+    gb2 = D2.sum(axis=0)
+    D1 = (D2 @ W2) * (1.0 - h1 * h1)
+    gW1 = D1.T @ u
+    gb1 = D1.sum(axis=0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
+    return loss, g
