@@ -171,12 +171,22 @@ void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool acc
 
 // General SHAKE / RATTLE groups (kernels/shake.cu): atoms goff[g] ..
 // goff[g+1] of gatoms, constraints coff[g] .. coff[g+1] with group-local
-// indices packed (i | j << 8) and r2 = (f32 d)^2.  One thread per group;
-// its atoms and constraints live in LDS at a fixed per-thread stride.
+// indices packed (i | j << 8) and r2 = (f32 d)^2.  Groups within the
+// reference's 8-atom cap: one thread per group, its atoms and constraints
+// in LDS at a fixed per-thread stride.  Longer groups: one wave per group,
+// constraints swept colour by colour.
 #define HD_SHAKE_MAX_ATOMS 64
 #define HD_SHAKE_MAX_CONS 64
+#define HD_SHAKE_REF_ATOMS 8   // MAX_GROUP_ATOMS (src/integrator/shake.rs:21)
+#define HD_SHAKE_MAX_COLORS 16
 struct HdShakeArgs {
+  // groups of at most 8 atoms (the reference's cap) run one per thread with
+  // max_atoms / max_cons over those groups; longer groups one per wave
   unsigned n_groups, max_atoms, max_cons, threads;  // threads per block
+  const unsigned *slist, *llist;  // short and long group ids
+  unsigned n_short, n_long;
+  const unsigned char* color;  // constraint colour (long groups)
+  unsigned n_colors;           // colours used by the long groups
   const unsigned *goff, *gatoms, *coff, *cij;
   const float* cr2;
   float4* posq;
@@ -248,8 +258,8 @@ void hd_launch_mlp_reduce(hipStream_t st, const float* partials, unsigned nb, fl
                           float* grad_out, const HdAdam* adam, double* loss_acc);
 // params[HD_MLP_SCALE] = rms of column 0 of x (n rows), 1 if that is 0.
 void hd_launch_mlp_scale(hipStream_t st, const float* x, uint64_t n, float* params);
-// Appends this step's rows (feat, nu) at slot counter[0] of a ring of cap
-// slots of n rows each and advances the counter; a full ring drops rows.
+// Appends this step's rows (feat, nu) at the slot it claims by advancing
+// counter[0]; slots at or beyond cap (n rows each) are not written.
 void hd_launch_shake_log(hipStream_t st, const float* feat, const float* nu, unsigned n,
                          float* log_x, float* log_t, unsigned long long* counter,
                          unsigned long long cap);

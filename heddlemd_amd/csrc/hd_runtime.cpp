@@ -140,6 +140,11 @@ struct hd_ctx {
   // general SHAKE / RATTLE groups
   std::vector<uint32_t> h_sh_goff, h_sh_atoms, h_sh_coff, h_sh_cij;
   std::vector<float> h_sh_r2;
+  std::vector<uint32_t> h_sh_short, h_sh_long;  // group ids by kernel
+  std::vector<uint8_t> h_sh_color;              // constraint colours (long groups)
+  unsigned sh_n_colors = 0;
+  unsigned *d_sh_short = nullptr, *d_sh_long = nullptr;
+  uint8_t* d_sh_color = nullptr;
   unsigned sh_max_atoms = 0, sh_max_cons = 0, sh_threads = 0;
   double sh_tol = 3.57e-6, sh_rattle_tol = 8.63e-17;
   int sh_max_iter = 32;
@@ -194,8 +199,12 @@ int dalloc(hd_ctx* c, T** p, size_t count) {
   hipError_t e = hipMalloc((void**)p, bytes);
   if (e != hipSuccess)
     return fail(c, HD_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-  e = hipMemset(*p, 0, bytes);
-  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+  // zeroed on the context stream and waited for: a null-stream memset is not
+  // ordered with the context's non-blocking stream and could land after
+  // kernels that already use the buffer
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "zeroing a buffer failed: %s", hipGetErrorString(e));
   c->allocs.push_back((void*)*p);
   return HD_OK;
 }
@@ -558,6 +567,12 @@ HdShakeArgs shake_args(hd_ctx* c, float dt) {
   a.rattle_tol = (float)c->sh_rattle_tol;
   a.max_iter = c->sh_max_iter;
   a.stats = c->d_sh_stats;
+  a.slist = c->d_sh_short;
+  a.llist = c->d_sh_long;
+  a.n_short = (unsigned)c->h_sh_short.size();
+  a.n_long = (unsigned)c->h_sh_long.size();
+  a.color = c->d_sh_color;
+  a.n_colors = c->sh_n_colors;
   a.n_cons = (unsigned)c->h_sh_cij.size();
   a.cgrp = c->d_sh_cgrp;
   a.feat = c->d_sh_feat;
@@ -1093,6 +1108,9 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
   std::vector<char> seen(c->n, 0);
   for (uint32_t a : c->h_groups) seen[a] = 1;
   unsigned ma = 0, mc = 0;
+  std::vector<uint32_t> shortg, longg;
+  std::vector<uint8_t> color(ng ? coff[ng] : 0, 0);
+  unsigned ncol = 0;
   for (uint64_t g = 0; g < ng; ++g) {
     if (goff[g + 1] <= goff[g] || coff[g + 1] < coff[g])
       return fail(c, HD_ERR_ARG, "SHAKE group %llu has decreasing offsets", (unsigned long long)g);
@@ -1109,8 +1127,28 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
       if (ci[k] >= na || cj[k] >= na || ci[k] == cj[k] || !(d[k] > 0))
         return fail(c, HD_ERR_ARG, "SHAKE constraint %u of group %llu is invalid", k,
                     (unsigned long long)g);
-    ma = std::max(ma, na);
-    mc = std::max(mc, nc);
+    if (na <= HD_SHAKE_REF_ATOMS) {
+      shortg.push_back((uint32_t)g);
+      ma = std::max(ma, na);
+      mc = std::max(mc, nc);
+      continue;
+    }
+    // greedy colouring in topology order: the lowest colour that no earlier
+    // constraint on either atom holds
+    longg.push_back((uint32_t)g);
+    uint32_t used[HD_SHAKE_MAX_ATOMS] = {0};
+    for (unsigned k = coff[g]; k < coff[g + 1]; ++k) {
+      uint32_t busy = used[ci[k]] | used[cj[k]];
+      unsigned q = 0;
+      while (busy & (1u << q)) ++q;
+      if (q >= HD_SHAKE_MAX_COLORS)
+        return fail(c, HD_ERR_UNSUPPORTED, "SHAKE group %llu needs more than %d constraint colours",
+                    (unsigned long long)g, HD_SHAKE_MAX_COLORS);
+      color[k] = (uint8_t)q;
+      used[ci[k]] |= 1u << q;
+      used[cj[k]] |= 1u << q;
+      ncol = std::max(ncol, q + 1);
+    }
   }
   c->h_sh_goff.assign(goff, goff + (ng ? ng + 1 : 0));
   c->h_sh_atoms.assign(gatoms, gatoms + (ng ? goff[ng] : 0));
@@ -1123,6 +1161,10 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
     float df = (float)d[k];  // r2 = (Real d)^2 (shake.rs:301-302)
     c->h_sh_r2[k] = df * df;
   }
+  c->h_sh_short = shortg;
+  c->h_sh_long = longg;
+  c->h_sh_color = color;
+  c->sh_n_colors = ncol;
   c->sh_max_atoms = ma;
   c->sh_max_cons = mc;
   // threads per block: the largest power of two whose LDS fits in 48 KiB
@@ -1162,6 +1204,7 @@ static void drop_graphs(hd_ctx* c);
 
 static int ensure_shake_features(hd_ctx* c) {
   size_t nc = c->h_sh_cij.size();
+  if (!c->d_sh_feat || !c->d_sh_nu0) drop_graphs(c);
   if (!c->d_sh_feat) DALLOC(c, c->d_sh_feat, nc * HD_SHAKE_NFEAT);
   if (!c->d_sh_nu0) DALLOC(c, c->d_sh_nu0, nc);
   return HD_OK;
@@ -1177,7 +1220,10 @@ int hd_set_shake_predictor(hd_ctx* c, const float* params) {
       if (!std::isfinite(params[k])) return fail(c, HD_ERR_ARG, "predictor parameter %d not finite", k);
     int r = ensure_shake_features(c);
     if (r) return r;
-    if (!c->d_mlp) DALLOC(c, c->d_mlp, HD_MLP_NPARAM);
+    if (!c->d_mlp) {
+      drop_graphs(c);
+      DALLOC(c, c->d_mlp, HD_MLP_NPARAM);
+    }
     if ((r = upload(c, c->d_mlp, params, HD_MLP_NPARAM))) return r;
   }
   bool on = params != nullptr;
@@ -1218,7 +1264,11 @@ int hd_shake_log_start(hd_ctx* c, uint64_t max_steps) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int r = ensure_shake_features(c);
   if (r) return r;
+  bool fresh = !c->sh_logging;
   if (c->log_cap != max_steps) {
+    // captured step graphs hold the old log pointers: drop them with the
+    // buffers
+    drop_graphs(c);
     dfree(c, c->d_log_x);
     dfree(c, c->d_log_t);
     c->d_log_x = c->d_log_t = nullptr;
@@ -1227,13 +1277,14 @@ int hd_shake_log_start(hd_ctx* c, uint64_t max_steps) {
     DALLOC(c, c->d_log_t, (size_t)max_steps * nc);
     c->log_cap = max_steps;
   }
-  if (!c->d_log_ctr) DALLOC(c, c->d_log_ctr, 1);
+  if (!c->d_log_ctr) {
+    drop_graphs(c);
+    DALLOC(c, c->d_log_ctr, 1);
+  }
   HIPCHK(c, hipMemsetAsync(c->d_log_ctr, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (!c->sh_logging) {
-    c->sh_logging = true;
-    drop_graphs(c);
-  }
+  c->sh_logging = true;
+  if (fresh) drop_graphs(c);
   return HD_OK;
 }
 
@@ -1253,7 +1304,7 @@ static int log_steps(hd_ctx* c, uint64_t* steps) {
     HIPCHK(c, hipMemcpyAsync(&k, c->d_log_ctr, sizeof(k), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  *steps = k;
+  *steps = std::min<unsigned long long>(k, c->log_cap);  // the counter runs on past a full log
   return HD_OK;
 }
 
@@ -1578,6 +1629,18 @@ int hd_setup(hd_ctx* c) {
       DALLOC(c, c->d_sh_nu, cg.size());
       if (!cg.empty() && (r = upload(c, c->d_sh_cgrp, cg.data(), cg.size()))) return r;
     }
+    DALLOC(c, c->d_sh_short, c->h_sh_short.size());
+    DALLOC(c, c->d_sh_long, c->h_sh_long.size());
+    DALLOC(c, c->d_sh_color, c->h_sh_color.size());
+    if (!c->h_sh_short.empty() &&
+        (r = upload(c, c->d_sh_short, c->h_sh_short.data(), c->h_sh_short.size())))
+      return r;
+    if (!c->h_sh_long.empty() &&
+        (r = upload(c, c->d_sh_long, c->h_sh_long.data(), c->h_sh_long.size())))
+      return r;
+    if (!c->h_sh_color.empty() &&
+        (r = upload(c, c->d_sh_color, c->h_sh_color.data(), c->h_sh_color.size())))
+      return r;
     if ((r = upload(c, c->d_sh_goff, c->h_sh_goff.data(), c->h_sh_goff.size()))) return r;
     if ((r = upload(c, c->d_sh_atoms, c->h_sh_atoms.data(), c->h_sh_atoms.size()))) return r;
     if ((r = upload(c, c->d_sh_coff, c->h_sh_coff.data(), c->h_sh_coff.size()))) return r;

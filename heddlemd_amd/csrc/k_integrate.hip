@@ -652,10 +652,13 @@ __device__ __forceinline__ void shake_stats(unsigned* st, unsigned it, int max_i
   if ((int)it >= max_iter) atomicAdd(&st[2], 1u);
 }
 
+// Groups within the reference's 8-atom cap: one thread per group, the
+// reference's sequential sweep.
 __global__ void k_shake_positions(HdShakeArgs a) {
   extern __shared__ float sh[];
-  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_groups) return;
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_short) return;
+  unsigned g = a.slist[t];
   const unsigned MA = a.max_atoms, MC = a.max_cons;
   float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
   float *x0 = base, *y0 = x0 + MA, *z0 = y0 + MA;
@@ -678,27 +681,9 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     im[k] = mk > 0.f ? 1.0f / mk : 0.f;
   }
   // one image for the whole group: relative to atom 0 (shake.cu:213-219)
-  // for groups within the reference's 8-atom cap, relative to the previous
-  // atom for longer groups (a chain longer than half the box would fold
-  // back onto atom 0's image)
   for (unsigned k = 1; k < na; ++k) {
-    unsigned r = na > 8u ? k - 1 : 0u;
-    shake_min_image_to(x0[r], y0[r], z0[r], x0[k], y0[k], z0[k], L);
-    shake_min_image_to(xu[r], yu[r], zu[r], xu[k], yu[k], zu[k], L);
-  }
-  // longer groups are solved relative to their middle atom: |r|^2 - d^2
-  // then resolves to the f32 spacing of the group's extent instead of that
-  // of its position in the box (at 200 Bohr the latter is ~1e-4 Bohr^2,
-  // above a 5e-5 tolerance)
-  float ox = 0.f, oy = 0.f, oz = 0.f;
-  if (na > 8u) {
-    unsigned mid = na / 2;
-    float sx = x0[mid], sy = y0[mid], sz = z0[mid];
-    ox = xu[mid]; oy = yu[mid]; oz = zu[mid];
-    for (unsigned k = 0; k < na; ++k) {
-      x0[k] -= sx; y0[k] -= sy; z0[k] -= sz;
-      xu[k] -= ox; yu[k] -= oy; zu[k] -= oz;
-    }
+    shake_min_image_to(x0[0], y0[0], z0[0], x0[k], y0[k], z0[k], L);
+    shake_min_image_to(xu[0], yu[0], zu[0], xu[k], yu[k], zu[k], L);
   }
   for (unsigned k = 0; k < na; ++k) { xc[k] = xu[k]; yc[k] = yu[k]; zc[k] = zu[k]; }
   for (unsigned k = 0; k < nc; ++k) {
@@ -759,8 +744,7 @@ __global__ void k_shake_positions(HdShakeArgs a) {
     float dxg = xc[k] - xu[k], dyg = yc[k] - yu[k], dzg = zc[k] - zu[k];
     a.vx[i] += dxg * inv_dt; a.vy[i] += dyg * inv_dt; a.vz[i] += dzg * inv_dt;
     float4 p = a.posq[i];
-    if (k == 0 && na > 8u) { p.x = xc[0] + ox; p.y = yc[0] + oy; p.z = zc[0] + oz; }
-    else if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
+    if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
     else { p.x += dxg; p.y += dyg; p.z += dzg; }
     a.posq[i] = p;
     float rx = xc[k] - cxg, ry = yc[k] - cyg, rz = zc[k] - czg;
@@ -772,8 +756,9 @@ __global__ void k_shake_positions(HdShakeArgs a) {
 template <bool ACC>
 __global__ void k_rattle_velocities(HdShakeArgs a) {
   extern __shared__ float sh[];
-  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_groups) return;
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_short) return;
+  unsigned g = a.slist[t];
   const unsigned MA = a.max_atoms, MC = a.max_cons;
   float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
   float *px = base, *py = px + MA, *pz = py + MA;
@@ -795,10 +780,7 @@ __global__ void k_rattle_velocities(HdShakeArgs a) {
     m[k] = mk;
     im[k] = mk > 0.f ? 1.0f / mk : 0.f;
   }
-  for (unsigned k = 1; k < na; ++k) {
-    unsigned r = na > 8u ? k - 1 : 0u;
-    shake_min_image_to(px[r], py[r], pz[r], px[k], py[k], pz[k], L);
-  }
+  for (unsigned k = 1; k < na; ++k) shake_min_image_to(px[0], py[0], pz[0], px[k], py[k], pz[k], L);
   for (unsigned k = 0; k < nc; ++k) {
     unsigned pk = a.cij[c0 + k];
     unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
@@ -845,6 +827,238 @@ __global__ void k_rattle_velocities(HdShakeArgs a) {
   for (unsigned k = 0; k < na; ++k) {
     unsigned i = a.gatoms[a0 + k];
     a.vx[i] = v0[k]; a.vy[i] = v1[k]; a.vz[i] = v2[k];
+  }
+}
+
+// Groups beyond the reference's 8-atom cap (chains): one wave per group.
+// Lane k owns atom k and constraint k (a group holds at most 64 of each).
+//   * Frame: lane k takes the bond vector d_k = minimg(x_k - x_(k-1)), an
+//     inclusive wave scan gives S_k, and the group is solved in
+//     coordinates S_k - S_mid relative to the middle atom, so a chain longer
+//     than half the box does not fold back and |r|^2 - d^2 resolves to the
+//     f32 spacing of the group's extent rather than of its place in the box.
+//   * Sweeps: the host colours the constraints so that constraints of one
+//     colour share no atom (a chain alternates two colours); a sweep visits
+//     the colours in order and the lanes of one colour update their
+//     disjoint atoms at once, which is a sequential Gauss-Seidel sweep in
+//     (colour, index) order.
+//   * Every atom moves by its delta; centre-of-mass sums are xor butterflies
+//     read from lane 0.
+// The CPU oracle restates the scan and butterfly orders.
+
+// inclusive scan over the wave: lane k gets v_0 + ... + v_k, grouped by
+// the offsets 1, 2, 4, ..., 32
+__device__ __forceinline__ float chain_scan(float v, unsigned lane) {
+#pragma unroll
+  for (unsigned off = 1; off < 64; off <<= 1) {
+    float u = __shfl_up(v, off);
+    if (lane >= off) v = u + v;
+  }
+  return v;
+}
+
+// sum over the wave as lane 0 of an xor butterfly, broadcast
+__device__ __forceinline__ float chain_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+  return __shfl(v, 0);
+}
+
+// the group frame of the wrapped coordinates (x, y, z) of lane k's atom
+__device__ __forceinline__ void chain_frame(float& x, float& y, float& z, unsigned lane,
+                                            unsigned na, const HdLattice& L) {
+  float px = __shfl_up(x, 1u), py = __shfl_up(y, 1u), pz = __shfl_up(z, 1u);
+  float dx = 0.f, dy = 0.f, dz = 0.f;
+  if (lane >= 1 && lane < na) {
+    dx = x - px; dy = y - py; dz = z - pz;
+    hd_min_image(dx, dy, dz, L);
+  }
+  dx = chain_scan(dx, lane);
+  dy = chain_scan(dy, lane);
+  dz = chain_scan(dz, lane);
+  unsigned mid = na / 2;
+  x = dx - __shfl(dx, (int)mid);
+  y = dy - __shfl(dy, (int)mid);
+  z = dz - __shfl(dz, (int)mid);
+}
+
+__device__ __forceinline__ void chain_com(float x, float y, float z, float m, float& cx,
+                                          float& cy, float& cz) {
+  float total = chain_sum(m);
+  float inv = total > 0.f ? 1.0f / total : 0.f;
+  cx = chain_sum(m * x) * inv;
+  cy = chain_sum(m * y) * inv;
+  cz = chain_sum(m * z) * inv;
+}
+
+__global__ __launch_bounds__(64) void k_shake_positions_long(HdShakeArgs a) {
+  __shared__ float x0[64], y0[64], z0[64], xc[64], yc[64], zc[64], ims[64];
+  const unsigned lane = threadIdx.x;
+  const unsigned g = a.llist[blockIdx.x];
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+  unsigned ia = 0;
+  float ux = 0.f, uy = 0.f, uz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, mk = 0.f, imk = 0.f;
+  if (lane < na) {
+    ia = a.gatoms[a0 + lane];
+    float4 p = a.posq[ia], q = a.snap[ia];
+    ux = p.x; uy = p.y; uz = p.z;
+    sx = q.x; sy = q.y; sz = q.z;
+    mk = a.mass[ia];
+    imk = mk > 0.f ? 1.0f / mk : 0.f;
+  }
+  chain_frame(sx, sy, sz, lane, na, L);
+  chain_frame(ux, uy, uz, lane, na, L);
+  x0[lane] = sx; y0[lane] = sy; z0[lane] = sz;
+  xc[lane] = ux; yc[lane] = uy; zc[lane] = uz;
+  ims[lane] = imk;
+  __syncthreads();
+  unsigned li = 0, lj = 0, col = 0xFFFFFFFFu;
+  float gx = 0.f, gy = 0.f, gz = 0.f, ip = 0.f, r2 = 0.f, lam = 0.f;
+  if (lane < nc) {
+    unsigned pk = a.cij[c0 + lane];
+    li = pk & 0xFFu;
+    lj = (pk >> 8) & 0xFFu;
+    col = a.color[c0 + lane];
+    gx = x0[li] - x0[lj];
+    gy = y0[li] - y0[lj];
+    gz = z0[li] - z0[lj];
+    ip = ims[li] + ims[lj];
+    r2 = a.cr2[c0 + lane];
+  }
+  const float imi = ims[li], imj = ims[lj];
+  if (a.lam0) {
+    for (unsigned q = 0; q < a.n_colors; ++q) {
+      if (col == q) {
+        float lambda = ip > 0.f ? a.lam0[c0 + lane] / ip : 0.f;
+        float ki = lambda * imi, kj = lambda * imj;
+        xc[li] -= ki * gx; yc[li] -= ki * gy; zc[li] -= ki * gz;
+        xc[lj] += kj * gx; yc[lj] += kj * gy; zc[lj] += kj * gz;
+        lam = lambda;
+      }
+      __syncthreads();
+    }
+  }
+  int it = 0;
+  for (; it < a.max_iter; ++it) {
+    int bad = 0;
+    for (unsigned q = 0; q < a.n_colors; ++q) {
+      if (col == q) {
+        float dx = xc[li] - xc[lj], dy = yc[li] - yc[lj], dz = zc[li] - zc[lj];
+        float dist2 = dx * dx + dy * dy + dz * dz;
+        float sigma = dist2 - r2;
+        if (fabsf(sigma) > a.tol) {
+          bad = 1;
+          float ddot = dx * gx + dy * gy + dz * gz;
+          float denom = 2.0f * ddot * ip;
+          if (denom != 0.f) {
+            float lambda = sigma / denom;
+            float ki = lambda * imi, kj = lambda * imj;
+            xc[li] -= ki * gx; yc[li] -= ki * gy; zc[li] -= ki * gz;
+            xc[lj] += kj * gx; yc[lj] += kj * gy; zc[lj] += kj * gz;
+            lam += lambda;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(bad)) break;
+  }
+  if (lane == 0 && a.stats) shake_stats(a.stats, (unsigned)it, a.max_iter);
+  if (a.lam_out && lane < nc) a.lam_out[c0 + lane] = lam * ip;
+  const float cxl = xc[lane], cyl = yc[lane], czl = zc[lane];
+  float cmx, cmy, cmz;
+  chain_com(cxl, cyl, czl, mk, cmx, cmy, cmz);
+  if (lane < na) {
+    float inv_dt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
+    float inv_dt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+    float dxg = cxl - ux, dyg = cyl - uy, dzg = czl - uz;
+    a.vx[ia] += dxg * inv_dt; a.vy[ia] += dyg * inv_dt; a.vz[ia] += dzg * inv_dt;
+    float4 p = a.posq[ia];
+    p.x += dxg; p.y += dyg; p.z += dzg;
+    a.posq[ia] = p;
+    float rx = cxl - cmx, ry = cyl - cmy, rz = czl - cmz;
+    float scale = mk * inv_dt2;
+    a.w_cons[ia] = scale * (dxg * rx + dyg * ry + dzg * rz);
+  }
+}
+
+template <bool ACC>
+__global__ __launch_bounds__(64) void k_rattle_velocities_long(HdShakeArgs a) {
+  __shared__ float px[64], py[64], pz[64], v0[64], v1[64], v2[64];
+  __shared__ float dv0[64], dv1[64], dv2[64], ims[64];
+  const unsigned lane = threadIdx.x;
+  const unsigned g = a.llist[blockIdx.x];
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+  unsigned ia = 0;
+  float x = 0.f, y = 0.f, z = 0.f, mk = 0.f, imk = 0.f;
+  v0[lane] = v1[lane] = v2[lane] = 0.f;
+  if (lane < na) {
+    ia = a.gatoms[a0 + lane];
+    float4 p = a.posq[ia];
+    x = p.x; y = p.y; z = p.z;
+    v0[lane] = a.vx[ia]; v1[lane] = a.vy[ia]; v2[lane] = a.vz[ia];
+    mk = a.mass[ia];
+    imk = mk > 0.f ? 1.0f / mk : 0.f;
+  }
+  chain_frame(x, y, z, lane, na, L);
+  px[lane] = x; py[lane] = y; pz[lane] = z;
+  dv0[lane] = dv1[lane] = dv2[lane] = 0.f;
+  ims[lane] = imk;
+  __syncthreads();
+  unsigned li = 0, lj = 0, col = 0xFFFFFFFFu;
+  float dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, ipc = 0.f;
+  if (lane < nc) {
+    unsigned pk = a.cij[c0 + lane];
+    li = pk & 0xFFu;
+    lj = (pk >> 8) & 0xFFu;
+    col = a.color[c0 + lane];
+    dx = px[li] - px[lj];
+    dy = py[li] - py[lj];
+    dz = pz[li] - pz[lj];
+    d2 = dx * dx + dy * dy + dz * dz;
+    ipc = ims[li] + ims[lj];
+  }
+  const float imi = ims[li], imj = ims[lj];
+  int it = 0;
+  for (; it < a.max_iter; ++it) {
+    int bad = 0;
+    for (unsigned q = 0; q < a.n_colors; ++q) {
+      if (col == q) {
+        float vxr = v0[li] - v0[lj], vyr = v1[li] - v1[lj], vzr = v2[li] - v2[lj];
+        float vrel = vxr * dx + vyr * dy + vzr * dz;
+        if (fabsf(vrel) > a.rattle_tol) {
+          bad = 1;
+          float denom = d2 * ipc;
+          if (denom != 0.f) {
+            float mu = vrel / denom;
+            float ki = mu * imi, kj = mu * imj;
+            v0[li] -= ki * dx; v1[li] -= ki * dy; v2[li] -= ki * dz;
+            v0[lj] += kj * dx; v1[lj] += kj * dy; v2[lj] += kj * dz;
+            dv0[li] -= ki * dx; dv1[li] -= ki * dy; dv2[li] -= ki * dz;
+            dv0[lj] += kj * dx; dv1[lj] += kj * dy; dv2[lj] += kj * dz;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(bad)) break;
+  }
+  if (lane == 0 && a.stats) shake_stats(a.stats + 3, (unsigned)it, a.max_iter);
+  if (ACC && a.dt > 0.f) {
+    float cx, cy, cz;
+    chain_com(x, y, z, mk, cx, cy, cz);
+    if (lane < na) {
+      float rx = x - cx, ry = y - cy, rz = z - cz;
+      float scale = mk * (1.0f / a.dt);
+      a.w_cons[ia] += scale * (dv0[lane] * rx + dv1[lane] * ry + dv2[lane] * rz);
+    }
+  }
+  if (lane < na) {
+    a.vx[ia] = v0[lane]; a.vy[ia] = v1[lane]; a.vz[ia] = v2[lane];
   }
 }
 
@@ -933,22 +1147,31 @@ void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a) {
 }
 
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a) {
-  if (a.n_groups == 0) return;
-  unsigned T = a.threads;
-  size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
-  hipLaunchKernelGGL(k_shake_positions, dim3((a.n_groups + T - 1) / T), dim3(T), lds, st, a);
+  if (a.n_short) {
+    unsigned T = a.threads;
+    size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
+    hipLaunchKernelGGL(k_shake_positions, dim3((a.n_short + T - 1) / T), dim3(T), lds, st, a);
+  }
+  if (a.n_long)
+    hipLaunchKernelGGL(k_shake_positions_long, dim3(a.n_long), dim3(64), 0, st, a);
 }
 
 void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool acc) {
-  if (a.n_groups == 0) return;
-  unsigned T = a.threads;
-  size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
-  if (acc)
-    hipLaunchKernelGGL(k_rattle_velocities<true>, dim3((a.n_groups + T - 1) / T), dim3(T), lds,
-                       st, a);
-  else
-    hipLaunchKernelGGL(k_rattle_velocities<false>, dim3((a.n_groups + T - 1) / T), dim3(T), lds,
-                       st, a);
+  if (a.n_short) {
+    unsigned T = a.threads;
+    size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
+    dim3 grid((a.n_short + T - 1) / T);
+    if (acc)
+      hipLaunchKernelGGL(k_rattle_velocities<true>, grid, dim3(T), lds, st, a);
+    else
+      hipLaunchKernelGGL(k_rattle_velocities<false>, grid, dim3(T), lds, st, a);
+  }
+  if (a.n_long) {
+    if (acc)
+      hipLaunchKernelGGL(k_rattle_velocities_long<true>, dim3(a.n_long), dim3(64), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_rattle_velocities_long<false>, dim3(a.n_long), dim3(64), 0, st, a);
+  }
 }
 
 // ---------------------------------------------------------------------

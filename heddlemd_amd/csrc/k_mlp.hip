@@ -25,38 +25,54 @@ __device__ __forceinline__ float mlp_in(int f, float x, float si) {
   return (f == 5 || f == 6) ? x : x * si;
 }
 
+// Inference: lane j of a wave is hidden unit j (its rows of W1 and W2 in
+// registers) and the wave walks PREDICT_ROWS feature rows: layer 1 per
+// lane, layer 2 from layer 1 broadcast through LDS, the output as a
+// butterfly sum over the lanes.
+constexpr int PREDICT_ROWS = 8;
+
 __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ P,
                                                      const float* __restrict__ feat, unsigned n,
                                                      float* __restrict__ nu) {
-  __shared__ float w[HD_MLP_NPARAM];
-  for (unsigned i = threadIdx.x; i < HD_MLP_NPARAM; i += blockDim.x) w[i] = P[i];
-  __syncthreads();
-  unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const float4* xr = reinterpret_cast<const float4*>(feat + (size_t)c * F);
-  float4 xa = xr[0], xb = xr[1];
-  float x[F] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
-  float s = w[HD_MLP_SCALE];
-  float si = s > 0.f ? 1.0f / s : 0.f;
-  float u[F];
+  __shared__ float hsh[4][H];
+  const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  float w1[F], w2[H];
 #pragma unroll
-  for (int f = 0; f < F; ++f) u[f] = mlp_in(f, x[f], si);
-  float h1[H];
+  for (int f = 0; f < F; ++f) w1[f] = P[HD_MLP_W1 + lane * F + f];
+  const float4* w2r = reinterpret_cast<const float4*>(P + HD_MLP_W2 + lane * H);
 #pragma unroll
-  for (int j = 0; j < H; ++j) {
-    float acc = w[HD_MLP_B1 + j];
-#pragma unroll
-    for (int f = 0; f < F; ++f) acc += w[HD_MLP_W1 + j * F + f] * u[f];
-    h1[j] = tanhf(acc);
+  for (int q = 0; q < H / 4; ++q) {
+    float4 v = w2r[q];
+    w2[4 * q] = v.x; w2[4 * q + 1] = v.y; w2[4 * q + 2] = v.z; w2[4 * q + 3] = v.w;
   }
-  float o = w[HD_MLP_B3];
-  for (int j = 0; j < H; ++j) {
-    float acc = w[HD_MLP_B2 + j];
+  const float b1 = P[HD_MLP_B1 + lane], b2 = P[HD_MLP_B2 + lane], w3 = P[HD_MLP_W3 + lane];
+  const float b3 = P[HD_MLP_B3], s = P[HD_MLP_SCALE];
+  const float si = s > 0.f ? 1.0f / s : 0.f;
+  const unsigned row0 = (blockIdx.x * 4u + wv) * PREDICT_ROWS;
+  for (int q = 0; q < PREDICT_ROWS; ++q) {
+    unsigned c = row0 + q;
+    bool ok = c < n;  // uniform over the wave
+    float x[F] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+      const float4* xr = reinterpret_cast<const float4*>(feat + (size_t)c * F);
+      float4 xa = xr[0], xb = xr[1];
+      x[0] = xa.x; x[1] = xa.y; x[2] = xa.z; x[3] = xa.w;
+      x[4] = xb.x; x[5] = xb.y; x[6] = xb.z; x[7] = xb.w;
+    }
+    float acc = b1;
 #pragma unroll
-    for (int i = 0; i < H; ++i) acc += w[HD_MLP_W2 + j * H + i] * h1[i];
-    o += w[HD_MLP_W3 + j] * tanhf(acc);
+    for (int f = 0; f < F; ++f) acc += w1[f] * mlp_in(f, x[f], si);
+    hsh[wv][lane] = tanhf(acc);
+    __syncthreads();
+    acc = b2;
+#pragma unroll
+    for (int i = 0; i < H; ++i) acc += w2[i] * hsh[wv][i];
+    float o = w3 * tanhf(acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
+    if (ok && lane == 0) nu[c] = x[0] + s * (o + b3);
+    __syncthreads();
   }
-  nu[c] = x[0] + s * o;
 }
 
 // One block = 64 samples, 256 threads (4 waves).  Forward activations stay
@@ -267,15 +283,18 @@ __global__ __launch_bounds__(1024) void k_shake_log(const float* __restrict__ fe
                                                     float* __restrict__ lx, float* __restrict__ lt,
                                                     unsigned long long* counter,
                                                     unsigned long long cap) {
-  unsigned long long slot = counter[0];
+  // the slot is claimed with a device-scope atomic, so the count never
+  // depends on which XCD's cache served the previous step's value
+  __shared__ unsigned long long claimed;
+  if (threadIdx.x == 0) claimed = atomicAdd(counter, 1ull);
+  __syncthreads();
+  const unsigned long long slot = claimed;
   if (slot >= cap) return;  // uniform over the block
   const float4* src = reinterpret_cast<const float4*>(feat);
   float4* dst = reinterpret_cast<float4*>(lx + slot * (unsigned long long)n * F);
   for (unsigned i = threadIdx.x; i < n * (F / 4); i += 1024) dst[i] = src[i];
   float* dt = lt + slot * (unsigned long long)n;
   for (unsigned i = threadIdx.x; i < n; i += 1024) dt[i] = nu[i];
-  __syncthreads();
-  if (threadIdx.x == 0) counter[0] = slot + 1;
 }
 
 }  // namespace
@@ -283,7 +302,9 @@ __global__ __launch_bounds__(1024) void k_shake_log(const float* __restrict__ fe
 void hd_launch_mlp_predict(hipStream_t st, const float* params, const float* feat, unsigned n,
                            float* nu) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mlp_predict, dim3((n + 255) / 256), dim3(256), 0, st, params, feat, n, nu);
+  const unsigned rows_per_block = 4u * PREDICT_ROWS;
+  hipLaunchKernelGGL(k_mlp_predict, dim3((n + rows_per_block - 1) / rows_per_block), dim3(256), 0,
+                     st, params, feat, n, nu);
 }
 
 unsigned hd_launch_mlp_grad(hipStream_t st, const float* params, const float* x,

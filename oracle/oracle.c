@@ -782,6 +782,98 @@ static void weighted_com_f(const float *x, const float *y, const float *z, const
 #define OR_SHAKE_REF_ATOMS 8  /* MAX_GROUP_ATOMS, src/integrator/shake.rs:21 */
 #define OR_SHAKE_MAXC 64
 
+/* Frame of a group beyond the 8-atom cap (no reference counterpart): the
+ * bond vectors d_k = minimg(x_k - x_(k-1)) (d_0 = 0), an inclusive
+ * Hillis-Steele scan over 64 lanes (offsets 1, 2, ..., 32; lane k adds lane
+ * k - off) giving S_k, and coordinates S_k - S_mid relative to the middle
+ * atom mid = na / 2.  The device computes the same sums with wave shuffles. */
+static void or_chain_frame(int na, float *x, float *y, float *z, const float *L)
+{
+    float sx[64], sy[64], sz[64], tx[64], ty[64], tz[64];
+    for (int k = 0; k < 64; ++k) {
+        sx[k] = sy[k] = sz[k] = 0.0f;
+        if (k >= 1 && k < na) {
+            float dx = x[k] - x[k - 1], dy = y[k] - y[k - 1], dz = z[k] - z[k - 1];
+            min_image_f(&dx, &dy, &dz, L);
+            sx[k] = dx; sy[k] = dy; sz[k] = dz;
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        for (int k = 0; k < 64; ++k) {
+            tx[k] = sx[k]; ty[k] = sy[k]; tz[k] = sz[k];
+            if (k >= off) {
+                tx[k] = sx[k - off] + sx[k];
+                ty[k] = sy[k - off] + sy[k];
+                tz[k] = sz[k - off] + sz[k];
+            }
+        }
+        for (int k = 0; k < 64; ++k) { sx[k] = tx[k]; sy[k] = ty[k]; sz[k] = tz[k]; }
+    }
+    int mid = na / 2;
+    for (int k = 0; k < na; ++k) {
+        x[k] = sx[k] - sx[mid];
+        y[k] = sy[k] - sy[mid];
+        z[k] = sz[k] - sz[mid];
+    }
+}
+
+/* Sum of v[0..n-1] as lane 0 of a 64-lane xor butterfly (offsets 32, 16,
+ * ..., 1; lane k adds lane k ^ off) over zero-padded lanes. */
+static float or_lane0_sum(const float *v, int n)
+{
+    float a[64], t[64];
+    for (int k = 0; k < 64; ++k) a[k] = k < n ? v[k] : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int k = 0; k < 64; ++k) t[k] = a[k] + a[k ^ off];
+        for (int k = 0; k < 64; ++k) a[k] = t[k];
+    }
+    return a[0];
+}
+
+/* Mass-weighted centre of a long group in the device's butterfly order. */
+static void or_long_com(const float *x, const float *y, const float *z, const float *m, int n,
+                        float *cx, float *cy, float *cz)
+{
+    float px[64], py[64], pz[64];
+    for (int a = 0; a < n; ++a) { px[a] = m[a] * x[a]; py[a] = m[a] * y[a]; pz[a] = m[a] * z[a]; }
+    float total = or_lane0_sum(m, n);
+    float inv = total > 0.0f ? 1.0f / total : 0.0f;
+    *cx = or_lane0_sum(px, n) * inv;
+    *cy = or_lane0_sum(py, n) * inv;
+    *cz = or_lane0_sum(pz, n) * inv;
+}
+
+/* Sweep order of a group's constraints.  Groups within the reference's
+ * 8-atom cap: topology order.  Longer groups: greedy colouring in topology
+ * order (each constraint takes the lowest colour no earlier constraint on
+ * either of its atoms holds), then colour by colour, topology order within
+ * a colour; constraints of one colour share no atom, which lets the device
+ * run each colour's updates at once. */
+static void or_sweep_order(int na, int nc, const uint8_t *li, const uint8_t *lj, int *order)
+{
+    if (na <= OR_SHAKE_REF_ATOMS) {
+        for (int k = 0; k < nc; ++k) order[k] = k;
+        return;
+    }
+    int color[OR_SHAKE_MAXC];
+    uint32_t used[OR_SHAKE_MAXA];
+    for (int a = 0; a < na; ++a) used[a] = 0;
+    int ncol = 0;
+    for (int k = 0; k < nc; ++k) {
+        uint32_t busy = used[li[k]] | used[lj[k]];
+        int q = 0;
+        while (busy & (1u << q)) ++q;
+        color[k] = q;
+        used[li[k]] |= 1u << q;
+        used[lj[k]] |= 1u << q;
+        if (q + 1 > ncol) ncol = q + 1;
+    }
+    int s = 0;
+    for (int q = 0; q < ncol; ++q)
+        for (int k = 0; k < nc; ++k)
+            if (color[k] == q) order[s++] = k;
+}
+
 void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
                             const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
                             const float *cr2, float *posq, float *vx, float *vy, float *vz,
@@ -791,7 +883,9 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
 {
     /* shake_positions (shake.cu:96-330): Gauss-Seidel sweeps in topology
      * order with gradients from the pre-drift snapshot, in the image of
-     * atom 0; delta write-back; position-level constraint virial */
+     * atom 0; delta write-back; position-level constraint virial.  Groups
+     * beyond the reference's 8-atom cap: bond-by-bond image, coordinates
+     * relative to the middle atom, sweeps in colour order (or_sweep_order) */
     for (int g = 0; g < n_groups; ++g) {
         int na = (int)(goff[g + 1] - goff[g]), nc = (int)(coff[g + 1] - coff[g]);
         const uint32_t *at = gatoms + goff[g];
@@ -806,25 +900,19 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             m[a] = mass[i];
             im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
         }
-        /* one image for the whole group: relative to atom 0 (shake.cu:213-219)
-         * for groups within the reference's 8-atom cap, relative to the
-         * previous atom for longer groups (chains longer than half the box
-         * would otherwise fold back onto atom 0's image) */
-        for (int a = 1; a < na; ++a) {
-            int r = na > OR_SHAKE_REF_ATOMS ? a - 1 : 0;
-            min_image_to_f(x0[r], y0[r], z0[r], &x0[a], &y0[a], &z0[a], L);
-            min_image_to_f(xu[r], yu[r], zu[r], &xu[a], &yu[a], &zu[a], L);
-        }
-        /* longer groups are solved relative to their middle atom, so that
-         * |r|^2 - d^2 resolves to the f32 spacing of the group's extent */
-        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-        if (na > OR_SHAKE_REF_ATOMS) {
-            int mid = na / 2;
-            float sx = x0[mid], sy = y0[mid], sz = z0[mid];
-            ox = xu[mid]; oy = yu[mid]; oz = zu[mid];
-            for (int a = 0; a < na; ++a) {
-                x0[a] -= sx; y0[a] -= sy; z0[a] -= sz;
-                xu[a] -= ox; yu[a] -= oy; zu[a] -= oz;
+        /* one image for the whole group: relative to atom 0
+         * (shake.cu:213-219); longer groups in the chain frame, bond by bond
+         * relative to the middle atom (a chain longer than half the box
+         * would fold back onto atom 0's image, and |r|^2 - d^2 resolves to
+         * the f32 spacing of the group's extent instead of the box's) */
+        const int lg = na > OR_SHAKE_REF_ATOMS;
+        if (lg) {
+            or_chain_frame(na, x0, y0, z0, L);
+            or_chain_frame(na, xu, yu, zu, L);
+        } else {
+            for (int a = 1; a < na; ++a) {
+                min_image_to_f(x0[0], y0[0], z0[0], &x0[a], &y0[a], &z0[a], L);
+                min_image_to_f(xu[0], yu[0], zu[0], &xu[a], &yu[a], &zu[a], L);
             }
         }
         for (int a = 0; a < na; ++a) { xc[a] = xu[a]; yc[a] = yu[a]; zc[a] = zu[a]; }
@@ -839,11 +927,14 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
             ip[k] = im[li[k]] + im[lj[k]];
             lam[k] = 0.0f;
         }
+        int order[OR_SHAKE_MAXC];
+        or_sweep_order(na, nc, li, lj, order);
         if (nu0) {
             /* warm start (no reference counterpart): one move along every
              * constraint gradient by the given nu = lambda (1/m_i + 1/m_j),
-             * in topology order, before the sweeps */
-            for (int k = 0; k < nc; ++k) {
+             * in sweep order, before the sweeps */
+            for (int s = 0; s < nc; ++s) {
+                int k = order[s];
                 int a = li[k], b = lj[k];
                 float lambda = ip[k] > 0.0f ? nu0[coff[g] + k] / ip[k] : 0.0f;
                 float ki = lambda * im[a], kj = lambda * im[b];
@@ -855,7 +946,8 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
         int it = 0;
         for (; it < max_iter; ++it) {
             int converged = 1;
-            for (int k = 0; k < nc; ++k) {
+            for (int s = 0; s < nc; ++s) {
+                int k = order[s];
                 int a = li[k], b = lj[k];
                 float dx = xc[a] - xc[b], dy = yc[a] - yc[b], dz = zc[a] - zc[b];
                 float dist2 = dx * dx + dy * dy + dz * dz;
@@ -878,17 +970,18 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
         if (nu_out)
             for (int k = 0; k < nc; ++k) nu_out[coff[g] + k] = lam[k] * ip[k];
         float cxg, cyg, czg;
-        weighted_com_f(xc, yc, zc, m, na, &cxg, &cyg, &czg);
+        if (lg)
+            or_long_com(xc, yc, zc, m, na, &cxg, &cyg, &czg);
+        else
+            weighted_com_f(xc, yc, zc, m, na, &cxg, &cyg, &czg);
         float inv_dt = dt != 0.0f ? 1.0f / dt : 0.0f;
         float inv_dt2 = dt != 0.0f ? 1.0f / (dt * dt) : 0.0f;
         for (int a = 0; a < na; ++a) {
             uint32_t i = at[a];
             float dxg = xc[a] - xu[a], dyg = yc[a] - yu[a], dzg = zc[a] - zu[a];
             vx[i] += dxg * inv_dt; vy[i] += dyg * inv_dt; vz[i] += dzg * inv_dt;
-            if (a == 0 && na > OR_SHAKE_REF_ATOMS) {
-                posq[4 * i] = xc[0] + ox; posq[4 * i + 1] = yc[0] + oy;
-                posq[4 * i + 2] = zc[0] + oz;
-            } else if (a == 0) {
+            /* long groups: every atom moves by its delta */
+            if (a == 0 && !lg) {
                 posq[4 * i] = xc[0]; posq[4 * i + 1] = yc[0]; posq[4 * i + 2] = zc[0];
             } else {
                 posq[4 * i] += dxg; posq[4 * i + 1] += dyg; posq[4 * i + 2] += dzg;
@@ -1005,10 +1098,12 @@ void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t
             m[a] = mass[i];
             im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
         }
-        for (int a = 1; a < na; ++a) {
-            int r = na > OR_SHAKE_REF_ATOMS ? a - 1 : 0;
-            min_image_to_f(px[r], py[r], pz[r], &px[a], &py[a], &pz[a], L);
-        }
+        const int lg = na > OR_SHAKE_REF_ATOMS;
+        if (lg)
+            or_chain_frame(na, px, py, pz, L);
+        else
+            for (int a = 1; a < na; ++a)
+                min_image_to_f(px[0], py[0], pz[0], &px[a], &py[a], &pz[a], L);
         const uint8_t *li = ci + coff[g], *lj = cj + coff[g];
         float dx[OR_SHAKE_MAXC], dy[OR_SHAKE_MAXC], dz[OR_SHAKE_MAXC], d2[OR_SHAKE_MAXC],
             ip[OR_SHAKE_MAXC];
@@ -1019,10 +1114,13 @@ void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t
             d2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
             ip[k] = im[li[k]] + im[lj[k]];
         }
+        int order[OR_SHAKE_MAXC];
+        or_sweep_order(na, nc, li, lj, order);
         int it = 0;
         for (; it < max_iter; ++it) {
             int converged = 1;
-            for (int k = 0; k < nc; ++k) {
+            for (int s = 0; s < nc; ++s) {
+                int k = order[s];
                 int a = li[k], b = lj[k];
                 float vxr = v0[a] - v0[b], vyr = v1[a] - v1[b], vzr = v2[a] - v2[b];
                 float vrel = vxr * dx[k] + vyr * dy[k] + vzr * dz[k];
@@ -1044,7 +1142,10 @@ void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t
         if (dt > 0.0f) {
             float inv_dt = 1.0f / dt;
             float cx, cy, cz;
-            weighted_com_f(px, py, pz, m, na, &cx, &cy, &cz);
+            if (lg)
+                or_long_com(px, py, pz, m, na, &cx, &cy, &cz);
+            else
+                weighted_com_f(px, py, pz, m, na, &cx, &cy, &cz);
             for (int a = 0; a < na; ++a) {
                 float rx = px[a] - cx, ry = py[a] - cy, rz = pz[a] - cz;
                 float scale = m[a] * inv_dt;

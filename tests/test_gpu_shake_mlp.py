@@ -214,6 +214,17 @@ def test_shake_log_in_graph_replays_fills_and_stops():
     e.shake_log_stop()
     e.step(1)
     assert e.shake_log_size()[0] == 3
+    # a restart with a different size re-allocates the log: the step graphs
+    # must pick up the new buffers
+    for cap in (4, 2):
+        e.shake_log_start(cap)
+        e.step(cap + 1)
+        steps, rows = e.shake_log_size()
+        assert steps == cap
+        x, t = e.shake_log()
+        t = t.reshape(cap, rows)
+        assert (np.abs(t).max(axis=1) > 0).all()
+        assert (np.abs(x).reshape(cap, -1).max(axis=1) > 0).all()
     e.close()
 
 
