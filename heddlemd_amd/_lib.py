@@ -48,6 +48,7 @@ SIGNATURES = {
     "hd_set_exclusion_span": (C.c_int, [_ctx, C.c_uint32]),
     "hd_set_angles": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),
     "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
+    "hd_set_pair_list": (C.c_int, [_ctx, C.c_int]),
     "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
                                 C.c_double]),
     "hd_set_integrator": (C.c_int, [_ctx, C.c_double, C.c_int]),

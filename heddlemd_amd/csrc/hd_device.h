@@ -108,6 +108,23 @@ __device__ __forceinline__ void hd_rel_to_center(float x, float y, float z, floa
   rz = (float)dz;
 }
 
+// The same with the reciprocal box lengths (FP64) precomputed: a multiply
+// instead of a correctly rounded FP64 division per axis.  The nearest image
+// can differ from hd_rel_to_center only for a separation within an ulp of
+// half a box length, far beyond any search radius.
+__device__ __forceinline__ void hd_rel_to_center_rcp(float x, float y, float z, float cx,
+                                                     float cy, float cz, const HdLattice& L,
+                                                     double ilx, double ily, double ilz,
+                                                     float& rx, float& ry, float& rz) {
+  double dx = (double)x - (double)cx, dy = (double)y - (double)cy, dz = (double)z - (double)cz;
+  dx -= (double)L.lx * floor(dx * ilx + 0.5);
+  dy -= (double)L.ly * floor(dy * ily + 0.5);
+  dz -= (double)L.lz * floor(dz * ilz + 0.5);
+  rx = (float)dx;
+  ry = (float)dy;
+  rz = (float)dz;
+}
+
 // SPME spline origin of one atom: wrap into the primary cell, fractional
 // coordinate + 1/2, u = s * n, primary bin floor(u) (mod n) and offset
 // t = u - floor(u) (the geometry of kernels/spme_spread_gather.cuh:90-126).

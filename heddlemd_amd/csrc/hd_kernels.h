@@ -24,7 +24,14 @@ struct HdNeighborArgs {
   float4 *center, *half;
   unsigned *entry_atoms, *entry_count, *max_count;
   unsigned *status, *rebuild_count;
+  // atom-list mode: entry_atoms holds per-block rows of per-atom neighbour
+  // lists (row k = k-th neighbour of each of the block's 64 atoms) and
+  // entry_count the block's row count; exclusions are applied at build
+  int atom_lists;
+  unsigned excl_span;
+  unsigned long long* block_mask;  // nblocks x hd_block_pair_chunks(nblocks) candidate masks
 };
+unsigned hd_block_pair_chunks(unsigned nblocks);
 
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a);
 // number of chunk sums of the voxel-count scan (the buffer needs one more)
@@ -46,6 +53,8 @@ struct HdPairArgs {
   const float4 *center, *half;
   const unsigned *entry_atoms, *entry_count;
   long long *fpi, *fpi_ew, *fpj, *fpj_ew;
+  int atom_lists;  // per-atom full lists (no j side) instead of the packed cluster list
+  int ortho;       // no box tilt
 };
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev);
 void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,

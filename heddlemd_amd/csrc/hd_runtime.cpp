@@ -65,6 +65,7 @@ struct hd_ctx {
   int grid[3] = {0, 0, 0};
   double r_skin = -1.0;
   unsigned excl_span = 0;
+  int atom_lists = 0;  // 1: per-atom full neighbour lists instead of the packed cluster list
   std::vector<uint32_t> h_angles;  // i, j, k per angle
   std::vector<float> h_ang_k, h_ang_t0;
   unsigned* d_angles = nullptr;
@@ -115,6 +116,7 @@ struct hd_ctx {
   unsigned *sorted_id = nullptr, *slot_of = nullptr, *tile_type = nullptr, *tile_key = nullptr;
   float4 *tile_posq = nullptr, *center = nullptr, *half = nullptr;
   unsigned *entry_atoms = nullptr, *entry_count = nullptr;
+  unsigned long long* block_mask = nullptr;  // block-pair candidate masks
   unsigned* ctrl = nullptr;  // [0] status, [1] rebuild count, [2] max entries
   long long *fpi = nullptr, *fpi_ew = nullptr, *fpj = nullptr, *fpj_ew = nullptr;
   // device: SPME
@@ -341,7 +343,14 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   a.max_count = c->ctrl + 2;
   a.status = c->ctrl;
   a.rebuild_count = c->ctrl + 1;
+  a.atom_lists = c->atom_lists;
+  a.excl_span = c->excl_span;
+  a.block_mask = c->block_mask;
   return a;
+}
+
+bool box_ortho(const hd_ctx* c) {
+  return c->lattice[3] == 0.0 && c->lattice[4] == 0.0 && c->lattice[5] == 0.0;
 }
 
 HdNhcArgs nhc_args(hd_ctx* c) {
@@ -429,6 +438,8 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   pa.fpi_ew = c->fpi_ew;
   pa.fpj = c->fpj;
   pa.fpj_ew = c->fpj_ew;
+  pa.atom_lists = c->atom_lists;
+  pa.ortho = box_ortho(c) ? 1 : 0;
   stage_begin(c, "pair_force", timed);
   hd_launch_pair(st, pa, ev);
   if (!c->h_ang_k.empty())
@@ -1493,6 +1504,49 @@ static int alloc_entries(hd_ctx* c, unsigned cap) {
   return HD_OK;
 }
 
+// Probe the list size, then size the per-block regions with headroom.
+static int size_list(hd_ctx* c) {
+  int r;
+  if ((r = alloc_entries(c, 64))) return r;
+  if ((r = build_list(c))) return r;
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  unsigned need = ctrl[2];
+  unsigned cap = std::max(4u, (unsigned)ceil(need * 1.5) + 2);
+  if (cap != c->cap) {
+    if ((r = alloc_entries(c, cap))) return r;
+    if ((r = build_list(c))) return r;
+  }
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  if (ctrl[0] & 8u) return fail(c, HD_ERR_OVERFLOW, "neighbour list overflow while sizing");
+  unsigned zero = 0;
+  HIPCHK(c, hipMemcpy(c->ctrl, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  return HD_OK;
+}
+
+static void drop_graphs(hd_ctx* c);
+
+int hd_set_pair_list(hd_ctx* c, int mode) {
+  if (!c || (mode != 0 && mode != 1)) return HD_ERR_ARG;
+  if (!c->ready) {
+    c->atom_lists = mode;
+    return HD_OK;
+  }
+  if (mode == c->atom_lists) return HD_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->atom_lists = mode;
+  // the rebuild count is kept across the switch
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+  int r = size_list(c);
+  if (r) return r;
+  HIPCHK(c, hipMemcpy(c->ctrl + 1, &ctrl[1], sizeof(unsigned), hipMemcpyHostToDevice));
+  c->forces_valid = false;
+  return HD_OK;
+}
+
 static void drop_graphs(hd_ctx* c) {
   ++c->graph_gen;
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
@@ -1575,6 +1629,7 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->tile_posq, c->npad);
   DALLOC(c, c->center, c->nblocks); DALLOC(c, c->half, c->nblocks);
   DALLOC(c, c->entry_count, c->nblocks);
+  DALLOC(c, c->block_mask, (size_t)c->nblocks * hd_block_pair_chunks(c->nblocks));
   DALLOC(c, c->ctrl, 4);
   DALLOC(c, c->fpi, (size_t)c->npad * 4); DALLOC(c, c->fpj, (size_t)c->npad * 4);
   DALLOC(c, c->fpi_ew, (size_t)c->npad * 2); DALLOC(c, c->fpj_ew, (size_t)c->npad * 2);
@@ -1681,21 +1736,7 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->nhc_state, 3 * m);
     if ((r = upload(c, c->nhc_state, st.data(), st.size()))) return r;
   }
-  // probe the list size, then size the entry regions with headroom
-  if ((r = alloc_entries(c, 64))) return r;
-  if ((r = build_list(c))) return r;
-  unsigned ctrl[4];
-  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
-  unsigned need = ctrl[2];
-  unsigned cap = std::max(4u, (unsigned)ceil(need * 1.5) + 2);
-  if (cap != c->cap) {
-    if ((r = alloc_entries(c, cap))) return r;
-    if ((r = build_list(c))) return r;
-  }
-  HIPCHK(c, hipMemcpy(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
-  if (ctrl[0] & 8u) return fail(c, HD_ERR_OVERFLOW, "neighbour list overflow at setup");
-  unsigned zero = 0;
-  HIPCHK(c, hipMemcpy(c->ctrl, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  if ((r = size_list(c))) return r;
   c->ready = true;
   return HD_OK;
 }
@@ -1964,7 +2005,19 @@ int hd_get_pairs(hd_ctx* c, uint32_t* out, uint64_t cap, uint64_t* n_pairs) {
   HIPCHK(c, hipMemcpy(sid.data(), c->sorted_id, sid.size() * 4, hipMemcpyDeviceToHost));
   std::vector<uint64_t> pairs;
   unsigned n = (unsigned)c->n;
-  for (unsigned b = 0; b < c->nblocks; ++b) {
+  if (c->atom_lists) {
+    // row k of block b: the k-th neighbour of each of the block's atoms
+    for (unsigned b = 0; b < c->nblocks; ++b)
+      for (unsigned k = 0; k < cnt[b]; ++k)
+        for (unsigned l = 0; l < 64; ++l) {
+          unsigned js = ents[((size_t)b * c->cap + k) * 64 + l], is = b * 64 + l;
+          if (js >= n || is >= n) continue;
+          uint32_t iid = sid[is], jid = sid[js];
+          uint32_t lo = std::min(iid, jid), hi = std::max(iid, jid);
+          pairs.push_back(((uint64_t)lo << 32) | hi);
+        }
+  }
+  for (unsigned b = 0; b < c->nblocks && !c->atom_lists; ++b) {
     for (unsigned e = 0; e < cnt[b]; ++e)
       for (unsigned l = 0; l < 64; ++l) {
         unsigned js = ents[((size_t)b * c->cap + e) * 64 + l];
@@ -2007,6 +2060,8 @@ int hd_count_pairs(hd_ctx* c, uint64_t* out) {
   pa.excl_span = c->excl_span;
   pa.entry_atoms = c->entry_atoms;
   pa.entry_count = c->entry_count;
+  pa.atom_lists = c->atom_lists;
+  pa.ortho = box_ortho(c) ? 1 : 0;
   if (e == hipSuccess) {
     hd_launch_tile_gather(c->stream, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
                           c->fpj_ew, (unsigned)c->n, c->npad);

@@ -297,20 +297,117 @@ __device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, f
   return ex * ex + ey * ey + ez * ez;
 }
 
-// Build the packed list: one wave per i-block.  For every j-block >= b
-// (in increasing order) whose box lies within r_search, each lane tests
-// its j-atom against the 64 i-atoms; j-atoms with at least one hit are
-// appended, in lane order, to a staging buffer flushed 64 at a time.
-// The entry layout is therefore a pure function of the positions.
+// Block-pair candidates: bit (jb % 64) of mask[b * nchunks + jb / 64] is
+// set when j-block jb's box lies within r_search of block b's (jb >= b only
+// for the half list).  One wave per (b, chunk of 64 j-blocks), so the box
+// tests of a rebuild run at full width instead of as a dependent loop inside
+// each block's wave.
+__global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ center,
+                                                     const float4* __restrict__ half,
+                                                     const float* __restrict__ lattice, float rs2,
+                                                     unsigned long long* __restrict__ mask,
+                                                     unsigned nblocks, unsigned nchunks, int full,
+                                                     const unsigned* status) {
+  if (!nb_rebuild_due(status)) return;
+  unsigned wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (wv >= nblocks * nchunks) return;
+  unsigned b = wv / nchunks, c = wv % nchunks;
+  unsigned jb = c * 64 + lane;
+  bool cand = false;
+  if (jb < nblocks && (full || jb >= b)) {
+    HdLattice L = hd_load_lattice(lattice);
+    cand = nb_box_dist2(center[b], half[b], center[jb], half[jb], L) <= rs2;
+  }
+  unsigned long long m = __ballot(cand);
+  if (lane == 0) mask[wv] = m;
+}
+
+#define NB_CAND_MAX 1024
+
+// The candidate j-blocks of block b in increasing order, from the chunk
+// masks, into an LDS list (at most NB_CAND_MAX; *too_many flags more).
+__device__ __forceinline__ unsigned nb_collect(const unsigned long long* __restrict__ mask,
+                                               unsigned b, unsigned nchunks, unsigned lane,
+                                               unsigned* cand, bool* too_many) {
+  unsigned ncand = 0;
+  for (unsigned c0 = 0; c0 < nchunks; c0 += 64) {
+    unsigned c = c0 + lane;
+    unsigned long long mk = c < nchunks ? mask[(size_t)b * nchunks + c] : 0ull;
+    unsigned cnt = __popcll(mk);
+    unsigned inc = cnt;
+#pragma unroll
+    for (unsigned off = 1; off < 64; off <<= 1) {
+      unsigned v = __shfl_up(inc, off);
+      if (lane >= off) inc += v;
+    }
+    unsigned base = ncand + inc - cnt;
+    while (mk) {
+      unsigned bit = __builtin_ctzll(mk);
+      mk &= mk - 1;
+      if (base < NB_CAND_MAX) cand[base] = c * 64 + bit;
+      ++base;
+    }
+    ncand += __shfl(inc, 63);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (ncand > NB_CAND_MAX) {
+    *too_many = true;
+    ncand = NB_CAND_MAX;
+  }
+  return ncand;
+}
+
+// Build the packed list: one wave per i-block.  The candidate j-blocks
+// (>= b, increasing order) are streamed through the wave one block per step,
+// the next block's atoms loaded while the current one is handled.  A j-atom
+// farther than r_search from the i-block's box cannot hit any i-atom; the
+// others are queued in LDS, and every time 64 are queued each lane tests
+// one queued j-atom against the 64 i-atoms.  Queued j-atoms with at least
+// one hit are appended, in queue order, to a staging buffer flushed 64 at a
+// time.  The entries are therefore the hit j-atoms in (j-block, slot) order:
+// a pure function of the positions.
 #define NB_WAVES 4
+
+// one pass of the 64 x 64 test over the queued j-atoms (lane = queue
+// position); returns whether this lane's j-atom hits any i-atom
+__device__ __forceinline__ bool nb_test_queued(const float4* ipos, float4 pj, unsigned jslot,
+                                               bool jvalid, bool rel, unsigned b, unsigned ni,
+                                               float rs2, const HdLattice& L) {
+  bool hit = false;
+  if (!jvalid) return false;
+  if (rel) {
+#pragma unroll 8
+    for (unsigned m = 0; m < 64; ++m) {
+      float4 a = ipos[m];
+      float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
+      float r2 = dx * dx + dy * dy + dz * dz;
+      hit |= (r2 <= rs2) & (m < ni) & ((b * 64 + m) != jslot);
+    }
+  } else {
+#pragma unroll 8
+    for (unsigned m = 0; m < 64; ++m) {
+      float4 a = ipos[m];
+      float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
+      hd_min_image(dx, dy, dz, L);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      hit |= (r2 <= rs2) & (m < ni) & ((b * 64 + m) != jslot);
+    }
+  }
+  return hit;
+}
+
 __global__ void __launch_bounds__(256) k_build_entries(
     const float4* __restrict__ tile_posq, const float4* __restrict__ center,
     const float4* __restrict__ half, const float* __restrict__ lattice, float rs2,
+    const unsigned long long* __restrict__ mask, unsigned nchunks,
     unsigned* __restrict__ entry_atoms, unsigned* __restrict__ entry_count, unsigned cap,
     unsigned* __restrict__ max_count, unsigned* status, unsigned n, unsigned nblocks) {
   if (!nb_rebuild_due(status)) return;
   __shared__ float4 ipos[NB_WAVES][64];
   __shared__ unsigned buf[NB_WAVES][128];
+  __shared__ float4 qpos[NB_WAVES][128];
+  __shared__ unsigned qslot[NB_WAVES][128];
+  __shared__ unsigned cand[NB_WAVES][NB_CAND_MAX];
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned b = blockIdx.x * NB_WAVES + w;
   if (b >= nblocks) return;
@@ -335,57 +432,73 @@ __global__ void __launch_bounds__(256) k_build_entries(
     r.w = 0.f;
     ipos[w][lane] = r;
   }
+  bool too_many = false;
+  unsigned ncand = nb_collect(mask, b, nchunks, lane, cand[w], &too_many);
   __builtin_amdgcn_wave_barrier();
-  unsigned nbuf = 0, nent = 0;
+  const double ilx = 1.0 / (double)L.lx, ily = 1.0 / (double)L.ly, ilz = 1.0 / (double)L.lz;
+  unsigned nbuf = 0, nent = 0, nq = 0;
   unsigned* out = entry_atoms + (size_t)b * cap * 64;
-  for (unsigned jb0 = b; jb0 < nblocks; jb0 += 64) {
-    unsigned jbl = jb0 + lane;
-    bool cand = false;
-    if (jbl < nblocks) cand = nb_box_dist2(cb, hb, center[jbl], half[jbl], L) <= rs2;
-    unsigned long long cmask = __ballot(cand);
-    while (cmask) {
-      unsigned bit = __builtin_ctzll(cmask);
-      cmask &= cmask - 1;
-      unsigned jb = jb0 + bit;
+  float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (ncand) pn = tile_posq[cand[w][0] * 64 + lane];
+  for (unsigned c = 0; c <= ncand; ++c) {
+    bool last = c == ncand;
+    if (!last) {
+      unsigned jb = cand[w][c];
       unsigned jslot = jb * 64 + lane;
       bool jvalid = jslot < n;
-      float4 pj = tile_posq[jslot];
-      bool hit = false;
-      if (jvalid) {
-        if (rel) {
-          float jx, jy, jz;
-          hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, jx, jy, jz);
-          for (unsigned m = 0; m < ni; ++m) {
-            float4 a = ipos[w][m];
-            float dx = a.x - jx, dy = a.y - jy, dz = a.z - jz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            hit |= (r2 <= rs2) & ((b * 64 + m) != jslot);
-          }
-        } else {
-          for (unsigned m = 0; m < ni; ++m) {
-            float4 a = ipos[w][m];
-            float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
-            hd_min_image(dx, dy, dz, L);
-            float r2 = dx * dx + dy * dy + dz * dz;
-            hit |= (r2 <= rs2) & ((b * 64 + m) != jslot);
-          }
-        }
+      float4 pj = pn;
+      if (c + 1 < ncand) pn = tile_posq[cand[w][c + 1] * 64 + lane];
+      // distance from the j-atom to the i-block's box
+      float dx, dy, dz;
+      if (rel) {
+        hd_rel_to_center_rcp(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, ilx, ily, ilz, dx, dy, dz);
+        pj.x = dx; pj.y = dy; pj.z = dz;
+      } else {
+        dx = pj.x - cb.x; dy = pj.y - cb.y; dz = pj.z - cb.z;
+        hd_min_image(dx, dy, dz, L);
       }
+      float ex = fmaxf(fabsf(dx) - hb.x, 0.f), ey = fmaxf(fabsf(dy) - hb.y, 0.f),
+            ez = fmaxf(fabsf(dz) - hb.z, 0.f);
+      bool near = jvalid && (ex * ex + ey * ey + ez * ez <= rs2);
+      unsigned long long nmask = __ballot(near);
+      if (near) {
+        unsigned pos = nq + __popcll(nmask & hd_lane_mask_lt());
+        qpos[w][pos] = pj;
+        qslot[w][pos] = jslot;
+      }
+      nq += __popcll(nmask);
+      __builtin_amdgcn_wave_barrier();
+    }
+    // test 64 queued atoms at a time (the remainder after the last block)
+    while (nq >= 64 || (last && nq > 0)) {
+      bool qvalid = lane < nq;
+      float4 qp = qpos[w][lane];
+      unsigned qs = qvalid ? qslot[w][lane] : HD_SENTINEL;
+      bool hit = nb_test_queued(ipos[w], qp, qs, qvalid, rel, b, ni, rs2, L);
+      unsigned rem = nq > 64 ? nq - 64 : 0;
+      float4 rp = qpos[w][64 + (lane < rem ? lane : 0)];
+      unsigned rs = qslot[w][64 + (lane < rem ? lane : 0)];
+      __builtin_amdgcn_wave_barrier();
+      if (lane < rem) {
+        qpos[w][lane] = rp;
+        qslot[w][lane] = rs;
+      }
+      nq = rem;
       unsigned long long hmask = __ballot(hit);
       if (hit) {
         unsigned pos = nbuf + __popcll(hmask & hd_lane_mask_lt());
-        buf[w][pos] = jslot;
+        buf[w][pos] = qs;
       }
       nbuf += __popcll(hmask);
       __builtin_amdgcn_wave_barrier();
       if (nbuf >= 64) {
         if (nent < cap) out[(size_t)nent * 64 + lane] = buf[w][lane];
         ++nent;
-        unsigned rem = nbuf - 64;
-        unsigned v = (lane < rem) ? buf[w][64 + lane] : 0u;
+        unsigned brem = nbuf - 64;
+        unsigned v = (lane < brem) ? buf[w][64 + lane] : 0u;
         __builtin_amdgcn_wave_barrier();
-        if (lane < rem) buf[w][lane] = v;
-        nbuf = rem;
+        if (lane < brem) buf[w][lane] = v;
+        nbuf = brem;
         __builtin_amdgcn_wave_barrier();
       }
     }
@@ -397,8 +510,107 @@ __global__ void __launch_bounds__(256) k_build_entries(
   if (lane == 0) {
     entry_count[b] = min(nent, cap);
     atomicMax(max_count, nent);
-    if (nent > cap) atomicOr(status, NB_STATUS_OVERFLOW);
+    if (nent > cap || too_many) atomicOr(status, NB_STATUS_OVERFLOW);
     else if (4 * nent > 3 * cap) atomicOr(status, NB_STATUS_HIGHWATER);
+  }
+}
+
+// Per-atom full lists (the atom-list mode): one wave per i-block, lane =
+// i-atom.  The wave reads its candidate j-blocks (every j-block whose box
+// lies within r_search, not only j >= i) into an LDS list, then stages the
+// candidates' atoms in LDS one block at a time, loading the next block
+// while it tests the current one.  Each lane appends the j-atoms within
+// r_search of its own atom that are neither itself nor excluded by the key
+// rule.  Row k of block b holds the k-th neighbour of every lane
+// (column-major, so the pair kernel's index loads are coalesced); rows past
+// a lane's count up to the block's longest list hold HD_SENTINEL.  Each
+// pair appears once in each atom's list, so the pair kernel needs no j-side
+// accumulation.  The order is j-block by j-block, then j-slot: the lists
+// are a pure function of the positions.
+
+__device__ __forceinline__ void alist_stage(const float4* __restrict__ tile_posq,
+                                            const unsigned* __restrict__ tile_key, unsigned jb,
+                                            unsigned lane, unsigned n, float4& p, unsigned& key) {
+  unsigned jslot = jb * 64 + lane;
+  bool jvalid = jslot < n;
+  p = tile_posq[jvalid ? jslot : 0u];
+  key = jvalid ? tile_key[jslot] : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_build_alist(
+    const float4* __restrict__ tile_posq, const unsigned* __restrict__ tile_key,
+    unsigned excl_span, const float4* __restrict__ center, const float4* __restrict__ half,
+    const float* __restrict__ lattice, float rs2, const unsigned long long* __restrict__ mask,
+    unsigned nchunks, unsigned* __restrict__ rows_out, unsigned* __restrict__ row_count,
+    unsigned cap, unsigned* __restrict__ max_count, unsigned* status, unsigned n,
+    unsigned nblocks) {
+  if (!nb_rebuild_due(status)) return;
+  __shared__ float4 jpos[NB_WAVES][64];
+  __shared__ unsigned jkey[NB_WAVES][64];
+  __shared__ unsigned cand[NB_WAVES][NB_CAND_MAX];
+  unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned b = blockIdx.x * NB_WAVES + w;
+  if (b >= nblocks) return;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned islot = b * 64 + lane;
+  bool ivalid = islot < n;
+  float4 pi = tile_posq[ivalid ? islot : 0u];
+  unsigned ikey = ivalid ? tile_key[islot] : 0u;
+  float4 cb = center[b], hb = half[b];
+  float rsearch = sqrtf(rs2);
+  // far from every periodic boundary (orthorhombic box): i and j relative
+  // to the block centre, the periodic shift applied once per staged atom
+  bool rel = (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) && (0.5f * L.lx - hb.x >= rsearch) &&
+             (0.5f * L.ly - hb.y >= rsearch) && (0.5f * L.lz - hb.z >= rsearch);
+  float ix = pi.x, iy = pi.y, iz = pi.z;
+  if (rel) hd_rel_to_center(pi.x, pi.y, pi.z, cb.x, cb.y, cb.z, L, ix, iy, iz);
+  bool too_many = false;
+  unsigned ncand = nb_collect(mask, b, nchunks, lane, cand[w], &too_many);
+  const double ilx = 1.0 / (double)L.lx, ily = 1.0 / (double)L.ly, ilz = 1.0 / (double)L.lz;
+  unsigned cnt = 0;
+  unsigned* out = rows_out + (size_t)b * cap * 64;
+  float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned kn = 0;
+  if (ncand) alist_stage(tile_posq, tile_key, cand[w][0], lane, n, pn, kn);
+  for (unsigned c = 0; c < ncand; ++c) {
+    unsigned jb = cand[w][c];
+    float4 pj = pn;
+    if (rel) {
+      float jx, jy, jz;
+      hd_rel_to_center_rcp(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, ilx, ily, ilz, jx, jy, jz);
+      pj.x = jx; pj.y = jy; pj.z = jz;
+    }
+    jpos[w][lane] = pj;
+    jkey[w][lane] = kn;
+    __builtin_amdgcn_wave_barrier();
+    if (c + 1 < ncand) alist_stage(tile_posq, tile_key, cand[w][c + 1], lane, n, pn, kn);
+    unsigned nj = min(64u, n - jb * 64);
+    if (ivalid) {
+#pragma unroll 8
+      for (unsigned m = 0; m < 64; ++m) {
+        float4 a = jpos[w][m];
+        float dx = ix - a.x, dy = iy - a.y, dz = iz - a.z;
+        if (!rel) hd_min_image(dx, dy, dz, L);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        unsigned js = jb * 64 + m;
+        bool ok = (jkey[w][m] - ikey + excl_span) > 2u * excl_span;
+        if (r2 <= rs2 && ok && js != islot && m < nj) {
+          if (cnt < cap) out[(size_t)cnt * 64 + lane] = js;
+          ++cnt;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  unsigned mx = cnt;
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off));
+  unsigned rows = min(mx, cap);
+  for (unsigned k = cnt; k < rows; ++k) out[(size_t)k * 64 + lane] = HD_SENTINEL;
+  if (lane == 0) {
+    row_count[b] = rows;
+    atomicMax(max_count, mx);
+    if (mx > cap || too_many) atomicOr(status, NB_STATUS_OVERFLOW);
+    else if (4 * mx > 3 * cap) atomicOr(status, NB_STATUS_HIGHWATER);
   }
 }
 
@@ -463,13 +675,25 @@ void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* s
                      lattice, tile_posq, fpj, fpj_ew, n, npad);
 }
 
+unsigned hd_block_pair_chunks(unsigned nblocks) { return cdiv(nblocks, 64); }
+
 void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
   unsigned nblocks = a.npad / 64;
+  unsigned nchunks = hd_block_pair_chunks(nblocks);
   hipLaunchKernelGGL(k_block_bbox, dim3(cdiv(nblocks * 64, 256)), dim3(256), 0, st, a.tile_posq,
                      a.center, a.half, a.status, a.n, nblocks);
-  hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0, st,
-                     a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.entry_atoms,
-                     a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
+  hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(nblocks * nchunks * 64, 256)), dim3(256), 0, st,
+                     a.center, a.half, a.lattice, a.rs2, a.block_mask, nblocks, nchunks,
+                     a.atom_lists, a.status);
+  if (a.atom_lists)
+    hipLaunchKernelGGL(k_build_alist, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0, st,
+                       a.tile_posq, a.tile_key, a.excl_span, a.center, a.half, a.lattice, a.rs2,
+                       a.block_mask, nchunks, a.entry_atoms, a.entry_count, a.cap, a.max_count,
+                       a.status, a.n, nblocks);
+  else
+    hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0,
+                       st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask, nchunks,
+                       a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
   unsigned gridN = min(cdiv(a.n, 256), 2048u);
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,

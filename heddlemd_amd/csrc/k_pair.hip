@@ -48,6 +48,7 @@ struct PairParams {
   float rmax2;
   float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
   unsigned excl_span;  // |key_i - key_j| <= excl_span excludes the pair
+  int ortho;           // the box has no tilt
 };
 
 // The i-atom's row of the LJ type-pair table, held in VGPRs for the whole
@@ -278,6 +279,164 @@ __global__ void __launch_bounds__(256) k_pair(
   }
 }
 
+// Atom-list mode: one workgroup of 4 waves per 64-atom i-block, lane =
+// i-atom; wave w walks rows w, w + 4, ... of the block's list region (row
+// k = the k-th neighbour of every lane: coalesced index loads, gathered j
+// positions).  Each wave issues the loads of ALIST_U rows before it
+// evaluates them, so ALIST_U gathers per lane are in flight at once.
+// Every pair is evaluated from both sides, so there is no j-side partial
+// and no atomic: the i-side f32 sums are folded into i64 fixed point at
+// 2^48 every 64 neighbours (the reference's per-entry fold), combined over
+// the 4 waves in LDS (integer sums, so the order does not matter) and
+// stored once.  Energy and virial are halved per side as in the packed
+// kernel.  Exclusions were applied when the lists were built.  The
+// displacement takes a per-pair minimum image (ORTHO: one rint per axis).
+#define ALIST_U 4
+
+template <bool EV, int NT, bool SW, bool ORTHO>
+__device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice& L, float ilx,
+                                           float ily, float ilz, const LjRow<NT>& R, float4 pi,
+                                           float kqi, float4 pj, unsigned jtype, float& efx,
+                                           float& efy, float& efz, float& ee, float& ew) {
+  const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
+  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  if (ORTHO) {
+    dx -= L.lx * rintf(dx * ilx);
+    dy -= L.ly * rintf(dy * ily);
+    dz -= L.lz * rintf(dz * ilz);
+  } else {
+    hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
+  }
+  float r2 = dx * dx + dy * dy + dz * dz;
+  float inv_r = __builtin_amdgcn_rsqf(r2);
+  float inv_r2 = inv_r * inv_r;
+  float ir6 = inv_r2 * inv_r2 * inv_r2;
+  float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+  float elj = 0.f;
+  if (EV || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+  if (SW) {
+    float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
+    float tau = fmaxf((r2 - rs2) * idl, 0.0f);
+    float omt = 1.0f - tau;
+    float sw = omt * omt * (1.0f + 2.0f * tau);
+    float chain = 12.0f * tau * omt * idl;
+    flj = sw * flj + chain * elj;
+    elj = sw * elj;
+  }
+  bool lj_on = r2 <= pick<NT>(R.cut2, jtype);
+  float r = r2 * inv_r;
+  float qq = kqi * pj.w;
+  float ar = P.alpha * r;
+  float gauss = __expf(-(ar * ar));
+  float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+  float erfc_ar =
+      (0.254829592f +
+       (-0.284496736f + (1.421413741f + (-1.453152027f + 1.061405429f * tt) * tt) * tt) * tt) *
+      tt * gauss;
+  float ec = qq * erfc_ar * inv_r;
+  float fc = qq * inv_r2 * (erfc_ar * inv_r + two_a_sqrtpi * gauss);
+  bool c_on = r2 <= P.coul_rc2;
+  float factor = lj_on ? flj : 0.f;
+  factor += c_on ? fc : 0.f;
+  efx += factor * dx;
+  efy += factor * dy;
+  efz += factor * dz;
+  if (EV) {
+    float energy = lj_on ? elj : 0.f;
+    energy += c_on ? ec : 0.f;
+    ee += 0.5f * energy;
+    ew += 0.5f * factor * r2;
+  }
+}
+
+template <bool EV, int NT, bool SW, bool ORTHO>
+__global__ void __launch_bounds__(256) k_pair_alist(
+    PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
+    const unsigned* __restrict__ tile_type, const unsigned* __restrict__ rows_in,
+    const unsigned* __restrict__ row_count, unsigned cap, unsigned n, unsigned nblocks,
+    long long* __restrict__ fpi, long long* __restrict__ fpi_ew) {
+  __shared__ long long red[PAIR_WAVES][5][64];
+  unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned b = blockIdx.x;
+  HdLattice L = hd_load_lattice(lattice);
+  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
+  unsigned islot = b * 64 + lane;
+  bool ivalid = islot < n;
+  float4 pi = tile_posq[ivalid ? islot : 0u];
+  unsigned itype = ivalid ? tile_type[islot] : 0u;
+  LjRow<NT> R = load_row<NT>(P, itype, ivalid);
+  const float kqi = P.kc * (ivalid ? pi.w : 0.f);
+  const unsigned rows = row_count[b];
+  const unsigned* list = rows_in + (size_t)b * cap * 64 + lane;
+  long long ax = 0, ay = 0, az = 0, ae = 0, aw = 0;
+  float efx = 0.f, efy = 0.f, efz = 0.f, ee = 0.f, ew = 0.f;
+  unsigned since_fold = 0;
+  for (unsigned k = w; k < rows; k += PAIR_WAVES * ALIST_U) {
+    unsigned js[ALIST_U];
+#pragma unroll
+    for (int u = 0; u < ALIST_U; ++u) {
+      unsigned kk = k + u * PAIR_WAVES;
+      js[u] = kk < rows ? list[(size_t)kk * 64] : HD_SENTINEL;
+    }
+    float4 pj[ALIST_U];
+    unsigned tj[ALIST_U];
+#pragma unroll
+    for (int u = 0; u < ALIST_U; ++u) {
+      unsigned jsafe = js[u] < n ? js[u] : 0u;
+      pj[u] = tile_posq[jsafe];
+      tj[u] = (NT > 1) ? tile_type[jsafe] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < ALIST_U; ++u)
+      if (js[u] < n)
+        alist_pair<EV, NT, SW, ORTHO>(P, L, ilx, ily, ilz, R, pi, kqi, pj[u], tj[u], efx, efy,
+                                      efz, ee, ew);
+    since_fold += ALIST_U;
+    if (since_fold >= 64) {
+      ax += hd_to_fixed48(efx);
+      ay += hd_to_fixed48(efy);
+      az += hd_to_fixed48(efz);
+      if (EV) {
+        ae += hd_to_fixed48(ee);
+        aw += hd_to_fixed48(ew);
+      }
+      efx = efy = efz = ee = ew = 0.f;
+      since_fold = 0;
+    }
+  }
+  ax += hd_to_fixed48(efx);
+  ay += hd_to_fixed48(efy);
+  az += hd_to_fixed48(efz);
+  if (EV) {
+    ae += hd_to_fixed48(ee);
+    aw += hd_to_fixed48(ew);
+  }
+  red[w][0][lane] = ax;
+  red[w][1][lane] = ay;
+  red[w][2][lane] = az;
+  if (EV) {
+    red[w][3][lane] = ae;
+    red[w][4][lane] = aw;
+  }
+  __syncthreads();
+  if (w == 0 && ivalid) {
+    long long sm[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < PAIR_WAVES; ++q) {
+      sm[0] += red[q][0][lane];
+      sm[1] += red[q][1][lane];
+      sm[2] += red[q][2][lane];
+      if (EV) {
+        sm[3] += red[q][3][lane];
+        sm[4] += red[q][4][lane];
+      }
+    }
+    reinterpret_cast<longlong2*>(fpi)[2 * islot] = make_longlong2(sm[0], sm[1]);
+    reinterpret_cast<longlong2*>(fpi)[2 * islot + 1] = make_longlong2(sm[2], 0);
+    if (EV) reinterpret_cast<longlong2*>(fpi_ew)[islot] = make_longlong2(sm[3], sm[4]);
+  }
+}
+
 // Harmonic angles (src/forces/angle.rs:258-313 functor, composed kernel
 // src/forces/jit_composed.rs:2300-2345): one thread per angle i-j-k with
 // minimum-image bond vectors, theta = atan2(|r_ij||r_kj| sin, dot),
@@ -386,16 +545,57 @@ __global__ void __launch_bounds__(64) k_count_pairs(
   atomicAdd(out, cnt);
 }
 
+// Atom-list mode: each pair is in both atoms' lists; count it from the
+// lower slot.
+__global__ void __launch_bounds__(64) k_count_pairs_alist(
+    float rmax2, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
+    const unsigned* __restrict__ rows_in, const unsigned* __restrict__ row_count, unsigned cap,
+    unsigned n, unsigned long long* __restrict__ out) {
+  unsigned b = blockIdx.x, lane = threadIdx.x;
+  unsigned islot = b * 64 + lane;
+  if (islot >= n) return;
+  HdLattice L = hd_load_lattice(lattice);
+  float4 pi = tile_posq[islot];
+  const unsigned* list = rows_in + (size_t)b * cap * 64 + lane;
+  unsigned long long cnt = 0;
+  for (unsigned k = 0; k < row_count[b]; ++k) {
+    unsigned js = list[(size_t)k * 64];
+    if (js >= n || js <= islot) continue;
+    float4 pj = tile_posq[js];
+    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    hd_min_image(dx, dy, dz, L);
+    if (dx * dx + dy * dy + dz * dz <= rmax2) ++cnt;
+  }
+  atomicAdd(out, cnt);
+}
+
 void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out) {
   unsigned nblocks = a.npad / 64;
-  hipLaunchKernelGGL(k_count_pairs, dim3(nblocks), dim3(64), 0, st, (float)(a.rmax * a.rmax),
-                     a.excl_span, a.lattice, a.tile_posq, a.tile_key, a.entry_atoms, a.entry_count, a.cap,
-                     a.n, out);
+  if (a.atom_lists)
+    hipLaunchKernelGGL(k_count_pairs_alist, dim3(nblocks), dim3(64), 0, st,
+                       (float)(a.rmax * a.rmax), a.lattice, a.tile_posq, a.entry_atoms,
+                       a.entry_count, a.cap, a.n, out);
+  else
+    hipLaunchKernelGGL(k_count_pairs, dim3(nblocks), dim3(64), 0, st, (float)(a.rmax * a.rmax),
+                       a.excl_span, a.lattice, a.tile_posq, a.tile_key, a.entry_atoms,
+                       a.entry_count, a.cap, a.n, out);
 }
 
 template <bool EV, int NT, bool SW>
 static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams& P) {
   unsigned nblocks = a.npad / 64;
+  if (a.atom_lists) {
+    dim3 grid(nblocks);
+    if (P.ortho)
+      hipLaunchKernelGGL((k_pair_alist<EV, NT, SW, true>), grid, dim3(64 * PAIR_WAVES), 0, st, P,
+                         a.lattice, a.tile_posq, a.tile_type, a.entry_atoms, a.entry_count,
+                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew);
+    else
+      hipLaunchKernelGGL((k_pair_alist<EV, NT, SW, false>), grid, dim3(64 * PAIR_WAVES), 0, st,
+                         P, a.lattice, a.tile_posq, a.tile_type, a.entry_atoms, a.entry_count,
+                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew);
+    return;
+  }
   hipLaunchKernelGGL((k_pair<EV, NT, SW>), dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P,
                      a.lattice, a.tile_posq, a.tile_type, a.tile_key, a.center, a.half,
                      a.entry_atoms, a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
@@ -440,6 +640,7 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
   P.rmax2 = (float)(a.rmax * a.rmax);
   P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
   P.excl_span = a.excl_span;
+  P.ortho = a.ortho;
   if (ev) launch_pair_ev<true>(st, a, P, any_switch);
   else launch_pair_ev<false>(st, a, P, any_switch);
 }

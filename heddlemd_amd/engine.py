@@ -117,6 +117,11 @@ class Engine:
     def set_exclusion_span(self, span):
         self._chk(self._lib.hd_set_exclusion_span(self._ctx, int(span)), "hd_set_exclusion_span")
 
+    def set_pair_list(self, mode):
+        """0: packed cluster list (half, j-side atomics); 1: per-atom full
+        lists."""
+        self._chk(self._lib.hd_set_pair_list(self._ctx, int(mode)), "hd_set_pair_list")
+
     def set_shake(self, groups):
         """groups: list of (atom_ids, [(local_i, local_j, d), ...])."""
         goff, gat, coff, ci, cj, d = [0], [], [0], [], [], []
@@ -321,10 +326,13 @@ class Engine:
 
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
-               velocities=None, crescale=None, langevin=None, device=0):
+               velocities=None, crescale=None, langevin=None, device=0, pair_list=None):
     """Configure an Engine for a ``systems.System``; ``langevin`` =
-    dict(friction, kt, seed) selects the BAOAB integrator."""
+    dict(friction, kt, seed) selects the BAOAB integrator; ``pair_list``
+    selects the neighbour-list form (hd_set_pair_list)."""
     e = Engine(device)
+    if pair_list is not None:
+        e.set_pair_list(pair_list)
     e.set_system(system.posq, system.mass, system.types, system.excl_key)
     e.set_lattice(system.lattice)
     e.set_lj(system.lj)

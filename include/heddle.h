@@ -108,6 +108,13 @@ int hd_set_shake_tolerance(hd_ctx *ctx, double tol, double rattle_tol, int max_i
  * out[0..2] SHAKE total sweeps over all groups and calls, the largest
  * count, groups that hit the cap; out[3..5] the same for RATTLE. */
 int hd_get_constraint_stats(hd_ctx *ctx, uint64_t out[6]);
+/* Pair list form.  0: the packed 64-atom cluster list of the reference's
+ * neighbour pipeline (kernels/neighbor.cu:416-631), half list with j-side
+ * fixed-point atomics.  1: per-atom full lists, each pair listed from both
+ * atoms and evaluated from both sides with no atomics (no counterpart in
+ * the reference).  Both give the same forces to f32 rounding.  May be
+ * changed after hd_setup (the list is rebuilt). */
+int hd_set_pair_list(hd_ctx *ctx, int mode);
 /* Velocity Verlet (src/integrator/velocity_verlet.rs); lossless = the
  * compensated (f32, f64) mode. */
 int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);
