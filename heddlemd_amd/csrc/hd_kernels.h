@@ -226,7 +226,7 @@ void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a);
 
 // Multiplier predictor: an 8 -> 64 -> 64 -> 1 tanh network on the scaled
 // features, predicting the correction to the first-order multiplier:
-//   u = x / s (the nu1 and rho columns; kappa unscaled),
+//   u = x / s (columns 0-4 and rho; the couplings 5, 6 unscaled),
 //   h1 = tanh(W1 u + b1), h2 = tanh(W2 h1 + b2), o = w3.h2 + b3,
 //   nu = x0 + s o.
 // Parameter pack (floats): W1[64][8], b1[64], W2[64][64], b2[64], w3[64],

@@ -1070,10 +1070,12 @@ __global__ __launch_bounds__(64) void k_rattle_velocities_long(HdShakeArgs a) {
 //              linearised system nu_k + sum_l kappa_kl nu_l = nu1_k, where
 //              c_kl = +-1/m of the atom that k and l share (0 if none) and
 //              ip = 1/m_i + 1/m_j;
-//   rho_k    = g.r / d^2 - 1.
-// Row k: [nu1_k, nu1_k-1, nu1_k+1, nu1_k-2, nu1_k+2, kappa_k,k-1,
-// kappa_k,k+1, rho_k], neighbours in the group's constraint order, zero
-// past the group's ends.
+//   rho_k    = g.r / d^2 - 1;
+//   nuprev_k = the converged nu of the constraint in the previous solve
+//              (a.lam_out before this step's SHAKE overwrites it).
+// Row k: [nu1_k, nu1_k-1, nu1_k+1, nuprev_k, nu1_k-2 + nu1_k+2,
+// kappa_k,k-1, kappa_k,k+1, rho_k], neighbours in the group's constraint
+// order, zero past the group's ends.
 struct ShakeBond {
   float gx, gy, gz, rx, ry, rz, imi, imj, ip, gr, nu1;
   unsigned li, lj;
@@ -1133,8 +1135,10 @@ __global__ void k_shake_features(HdShakeArgs a) {
     f[2] = l.nu1;
     f[6] = shake_coupling(k, l);
   }
-  if (c >= cb + 2) f[3] = shake_bond(a, L, a0, c - 2).nu1;
-  if (c + 2 < ce) f[4] = shake_bond(a, L, a0, c + 2).nu1;
+  f[3] = a.lam_out ? a.lam_out[c] : 0.f;
+  float f2l = c >= cb + 2 ? shake_bond(a, L, a0, c - 2).nu1 : 0.f;
+  float f2r = c + 2 < ce ? shake_bond(a, L, a0, c + 2).nu1 : 0.f;
+  f[4] = f2l + f2r;
   f[7] = k.gr / a.cr2[c] - 1.0f;
   float4* out = reinterpret_cast<float4*>(a.feat + (size_t)c * HD_SHAKE_NFEAT);
   out[0] = make_float4(f[0], f[1], f[2], f[3]);

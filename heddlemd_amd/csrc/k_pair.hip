@@ -174,15 +174,16 @@ __device__ __forceinline__ void pair_entries(
       if (!SPC) ok = ok & (max(jkey, ikey) < HD_SENTINEL - 1u);
       float factor = (ok & lj_on) ? flj : 0.f;
       factor += (ok & c_on) ? fc : 0.f;
-      float fx = factor * dx, fy = factor * dy, fz = factor * dz;
-      efx += fx; efy += fy; efz += fz;
-      jfx -= fx; jfy -= fy; jfz -= fz;
+      // i and j sides as fused multiply-adds of +-factor * d
+      float nfactor = -factor;
+      efx = fmaf(factor, dx, efx); efy = fmaf(factor, dy, efy); efz = fmaf(factor, dz, efz);
+      jfx = fmaf(nfactor, dx, jfx); jfy = fmaf(nfactor, dy, jfy); jfz = fmaf(nfactor, dz, jfz);
       if (EV) {
         float energy = (ok & lj_on) ? elj : 0.f;
         energy += (ok & c_on) ? ec : 0.f;
-        float he = 0.5f * energy, hw = 0.5f * factor * r2;
-        ee += he; ew += hw;
-        je += he; jw += hw;
+        float fr2 = factor * r2;
+        ee = fmaf(0.5f, energy, ee); ew = fmaf(0.5f, fr2, ew);
+        je = fmaf(0.5f, energy, je); jw = fmaf(0.5f, fr2, jw);
       }
       pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
       jkey = hd_rot1_u(jkey);

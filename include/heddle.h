@@ -183,9 +183,11 @@ const char *hd_batch_last_error(const hd_batch *batch);
  *
  * Per constraint the device forms 8 dimensionless features: the first
  * Gauss-Seidel multiplier nu1 = (|r|^2 - d^2) / (2 g.r) of the constraint
- * and of its neighbours k-1, k+1, k-2, k+2 in the group's constraint
- * order, the linearised couplings to k-1 and k+1, and g.r / d^2 - 1
- * (g: snapshot bond, r: unconstrained bond).  Multipliers are exchanged in
+ * and of its neighbours k-1 and k+1 in the group's constraint order, the
+ * constraint's converged multiplier of the previous solve, the sum of nu1
+ * over the second neighbours k-2 and k+2, the linearised couplings to k-1
+ * and k+1, and g.r / d^2 - 1 (g: snapshot bond, r: unconstrained bond).
+ * Multipliers are exchanged in
  * the form nu = lambda (1/m_i + 1/m_j).  A predictor (hd_mlp) maps the
  * features to nu; SHAKE then starts from the predicted nu and runs its
  * usual sweeps to the same tolerance.

@@ -997,9 +997,10 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
  * device's operation order: for constraint k with snapshot bond g and
  * unconstrained bond r (each minimum-imaged), nu1 = (|r|^2 - d^2)/(2 g.r);
  * coupling kappa_kl = c_kl (r_k.g_l) / (ip_l g_k.r_k) with c_kl the signed
- * inverse mass of the atom shared by k and l; rho = g.r/d^2 - 1.
- * Row: nu1_k, nu1_k-1, nu1_k+1, nu1_k-2, nu1_k+2, kappa_k,k-1, kappa_k,k+1,
- * rho_k (zero past the group's ends). */
+ * inverse mass of the atom shared by k and l; rho = g.r/d^2 - 1; nuprev =
+ * the previous solve's converged nu (nu_prev, or 0).
+ * Row: nu1_k, nu1_k-1, nu1_k+1, nuprev_k, nu1_k-2 + nu1_k+2, kappa_k,k-1,
+ * kappa_k,k+1, rho_k (zero past the group's ends). */
 typedef struct {
     float gx, gy, gz, rx, ry, rz, imi, imj, ip, gr, nu1;
     int li, lj;
@@ -1047,7 +1048,7 @@ static float or_coupling(const or_bond *k, const or_bond *l)
 void or_shake_features_f32(int n_groups, const uint32_t *goff, const uint32_t *gatoms,
                            const uint32_t *coff, const uint8_t *ci, const uint8_t *cj,
                            const float *cr2, const float *posq, const float *snap,
-                           const float *mass, const float *L, float *feat)
+                           const float *mass, const float *L, const float *nu_prev, float *feat)
 {
     for (int g = 0; g < n_groups; ++g) {
         const uint32_t *at = gatoms + goff[g];
@@ -1067,8 +1068,12 @@ void or_shake_features_f32(int n_groups, const uint32_t *goff, const uint32_t *g
                 f[2] = l.nu1;
                 f[6] = or_coupling(&k, &l);
             }
-            if (c >= cb + 2) f[3] = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c - 2).nu1;
-            if (c + 2 < ce) f[4] = or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c + 2).nu1;
+            f[3] = nu_prev ? nu_prev[c] : 0.0f;
+            float f2l = c >= cb + 2 ? or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c - 2).nu1
+                                    : 0.0f;
+            float f2r = c + 2 < ce ? or_shake_bond(at, ci, cj, cr2, posq, snap, mass, L, c + 2).nu1
+                                   : 0.0f;
+            f[4] = f2l + f2r;
             f[7] = k.gr / cr2[c] - 1.0f;
         }
     }

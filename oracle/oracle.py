@@ -63,7 +63,7 @@ def lib():
                                              C.c_float, C.c_float, C.c_int, _f32p, _i32p,
                                              C.c_void_p, C.c_void_p]
         L.or_shake_features_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, _f32p,
-                                            _f32p, _f32p, _f32p, _f32p, _f32p]
+                                            _f32p, _f32p, _f32p, _f32p, C.c_void_p, _f32p]
         L.or_rattle_velocities_f32.argtypes = [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p,
                                                _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                                C.c_float, C.c_float, C.c_int, _f32p, _i32p]
@@ -298,15 +298,20 @@ def shake_positions(G, posq, v, snap, mass, lattice, dt, tol=SHAKE_TOL, max_iter
     return w, iters[:G.n]
 
 
-def shake_features(G, posq, snap, mass, lattice):
+def shake_features(G, posq, snap, mass, lattice, nu_prev=None):
     """Predictor inputs per constraint (8 floats per row), f32 in the
-    device's operation order (no reference counterpart)."""
+    device's operation order (no reference counterpart); nu_prev = the
+    previous solve's converged multipliers (zeros when None)."""
     out = np.zeros((len(G.r2), 8), np.float32)
+    pp = None
+    if nu_prev is not None:
+        nu_prev = np.ascontiguousarray(nu_prev, np.float32)
+        pp = nu_prev.ctypes.data
     lib().or_shake_features_f32(G.n, G.goff, G.atoms, G.coff, G.ci, G.cj, G.r2,
                                 np.ascontiguousarray(posq, np.float32).reshape(-1),
                                 np.ascontiguousarray(snap, np.float32).reshape(-1),
                                 np.ascontiguousarray(mass, np.float32),
-                                np.ascontiguousarray(lattice, np.float32), out.reshape(-1))
+                                np.ascontiguousarray(lattice, np.float32), pp, out.reshape(-1))
     return out
 
 

@@ -156,7 +156,18 @@ def test_shake_log_and_warm_started_step_match_oracle(oracle):
     assert e.shake_log_size() == (1, len(G.r2))
     x_log, t_log = e.shake_log()
     _, nu_dev = e.shake_multipliers()
+    # a second step: its rows carry the first step's multipliers
+    posq1, img1 = e.positions(images=True)
+    v1 = e.velocities()
+    e.step(1)
+    x2, t2 = e.shake_log(1, 1)
     e.close()
+    p2, v2, snap2, L32 = _pre_shake_state(oracle, s, posq1, img1, v1,
+                                          np.zeros_like(f0), dt)
+    feat2 = oracle.shake_features(G, p2, snap2, s.mass, L32, nu_prev=t_log)
+    say(f"  step 2 features: {np.sum(feat2 != x2)} differ")
+    assert np.array_equal(feat2, x2)
+    assert np.array_equal(x2[:, 3], t_log)
     p, v, snap, L32 = _pre_shake_state(oracle, s, posq0, img0, v0, f0, dt)
     feat = oracle.shake_features(G, p, snap, s.mass, L32)
     say(f"  features: {np.sum(feat != x_log)} of {feat.size} differ, "

@@ -64,7 +64,8 @@ def test_converged_nu_of_one_bond_is_the_quadratic_root(oracle):
 
 def test_features_of_a_uniformly_stretched_straight_chain(oracle):
     # collinear beads stretched by (1 + e): nu1 = ((1+e)^2 - 1) / (2 (1+e)),
-    # couplings to both neighbours -1/2, rho = e, zero past the ends
+    # couplings to both neighbours -1/2, rho = e, zero past the ends; the
+    # previous multipliers are passed through
     n, e = 12, 0.01
     snap = np.zeros((n, 4), np.float32)
     snap[:, 0] = np.arange(n) * D_BOND
@@ -72,7 +73,8 @@ def test_features_of_a_uniformly_stretched_straight_chain(oracle):
     posq[:, 0] *= np.float32(1 + e)
     G = _groups(oracle, 1, n)
     mass = np.full(n, M_BEAD, np.float32)
-    f = oracle.shake_features(G, posq, snap, mass, L_BIG).astype(np.float64)
+    prev = np.linspace(-0.01, 0.02, n - 1).astype(np.float32)
+    f = oracle.shake_features(G, posq, snap, mass, L_BIG, nu_prev=prev).astype(np.float64)
     nu1 = ((1 + e) ** 2 - 1) / (2 * (1 + e))
     nc = n - 1
     # |r|^2 - d^2 cancels ~2 of f32's 7 digits at these coordinates
@@ -80,8 +82,10 @@ def test_features_of_a_uniformly_stretched_straight_chain(oracle):
     assert np.allclose(f[:, 0], nu1, rtol=tol)
     assert np.allclose(f[1:, 1], nu1, rtol=tol) and f[0, 1] == 0
     assert np.allclose(f[:-1, 2], nu1, rtol=tol) and f[-1, 2] == 0
-    assert np.allclose(f[2:, 3], nu1, rtol=tol) and (f[:2, 3] == 0).all()
-    assert np.allclose(f[:-2, 4], nu1, rtol=tol) and (f[-2:, 4] == 0).all()
+    assert np.array_equal(f[:, 3], prev.astype(np.float64))
+    # second neighbours summed: two inside, one within two of an end
+    assert np.allclose(f[2:-2, 4], 2 * nu1, rtol=tol)
+    assert np.allclose(f[[0, 1, -2, -1], 4], nu1, rtol=tol)
     assert np.allclose(f[1:, 5], -0.5, rtol=1e-5) and f[0, 5] == 0
     assert np.allclose(f[:-1, 6], -0.5, rtol=1e-5) and f[nc - 1, 6] == 0
     assert np.allclose(f[:, 7], e, rtol=1e-3)
