@@ -56,7 +56,9 @@ struct HdPairArgs {
   int atom_lists;  // per-atom full lists (no j side) instead of the packed cluster list
   int ortho;       // no box tilt
 };
-void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev);
+// ev: 0 forces only, 1 + per-atom energy and virial, 2 + per-atom virial
+// only (no pair energy is accumulated)
+void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev);
 void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,
                       const unsigned* ijk, const float* kth, const float* th0,
                       const unsigned* slot_of, long long* fpj, long long* fpj_ew,

@@ -397,7 +397,10 @@ void stage_end(hd_ctx* c, bool on) {
 bool has_shake(const hd_ctx* c) { return c->h_sh_goff.size() > 1; }
 bool has_constraints(const hd_ctx* c) { return !c->h_groups.empty() || has_shake(c); }
 
-void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
+// ev: 0 forces only; 1 energy and virial into scal[1], scal[2]; 2 virial
+// only (what a barostat consumes on steps that log nothing), reduced into
+// the scratch pair scal[6], scal[7] so the last logged energy stays intact
+void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
   stage_begin(c, "neighbor_update", timed);
@@ -441,7 +444,7 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   pa.atom_lists = c->atom_lists;
   pa.ortho = box_ortho(c) ? 1 : 0;
   stage_begin(c, "pair_force", timed);
-  hd_launch_pair(st, pa, ev);
+  hd_launch_pair(st, pa, c->atom_lists && ev == 2 ? 1 : ev);
   if (!c->h_ang_k.empty())
     hd_launch_angles(st, c->posq, c->d_lattice, c->d_angles, c->d_ang_k, c->d_ang_t0, c->slot_of,
                      c->fpj, c->fpj_ew, (unsigned)c->h_ang_k.size(), ev);
@@ -546,7 +549,8 @@ void enqueue_forces(hd_ctx* c, bool ev, bool kick, bool timed) {
   stage_end(c, timed);
   if (ev) {
     stage_begin(c, "reduce_scalars", timed);
-    hd_launch_sum2(st, (unsigned)c->n, c->pe, c->vir, c->red_partials, c->scal + 1);
+    hd_launch_sum2(st, (unsigned)c->n, c->pe, c->vir, c->red_partials,
+                   c->scal + (ev == 1 ? 1 : 6));
     stage_end(c, timed);
   }
 }
@@ -677,10 +681,11 @@ HdSettleArgs settle_args(hd_ctx* c) {
   return sa;
 }
 
-void enqueue_crescale(hd_ctx* c, bool timed) {
+void enqueue_crescale(hd_ctx* c, int ev, bool timed) {
   // C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): K of the
   // constrained, not yet kicked velocities (scal[3]) and W of this
-  // evaluation (scal[2]); positions are scaled after the kick
+  // evaluation (scal[2], or scal[7] on a virial-only evaluation);
+  // positions are scaled after the kick
   HdBarostatArgs b;
   memset(&b, 0, sizeof(b));
   b.seed_lo = (unsigned)(c->baro_seed & 0xFFFFFFFFull);
@@ -692,8 +697,8 @@ void enqueue_crescale(hd_ctx* c, bool timed) {
   b.dt = c->dt;
   b.mu3_min = 1e-18;
   stage_begin(c, "c_rescale", timed);
-  hd_launch_crescale(c->stream, c->scal + 3, c->scal + 2, c->d_lattice, c->baro_mu, c->baro_diag,
-                     c->baro_counter, b, c->posq, (unsigned)c->n);
+  hd_launch_crescale(c->stream, c->scal + 3, c->scal + (ev == 1 ? 2 : 7), c->d_lattice,
+                     c->baro_mu, c->baro_diag, c->baro_counter, b, c->posq, (unsigned)c->n);
   stage_end(c, timed);
 }
 
@@ -723,19 +728,18 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   stage_end(c, timed);
   enqueue_constrain_positions(c, timed);
   enqueue_constrain_velocities(c, timed);
-  if (c->have_baro) {
+  if (c->have_baro)
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
                  0.f, nullptr);
-    ev = true;
-  }
-  enqueue_forces(c, ev, true, timed);
-  if (c->have_baro) enqueue_crescale(c, timed);
+  // a barostat consumes the virial every step; the energies only when logged
+  int mode = ev ? 1 : (c->have_baro ? 2 : 0);
+  enqueue_forces(c, mode, true, timed);
+  if (c->have_baro) enqueue_crescale(c, mode, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
 
 void enqueue_step(hd_ctx* c, bool ev, bool timed) {
-  // ev may be forced on below (a barostat consumes the virial every step)
   hipStream_t st = c->stream;
   unsigned n = (unsigned)c->n;
   HdNhcArgs nhc = nhc_args(c);
@@ -785,9 +789,10 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
                  0.f, nullptr);
   }
-  if (c->have_baro) ev = true;
-  enqueue_forces(c, ev, true, timed);
-  if (c->have_baro) enqueue_crescale(c, timed);
+  // a barostat consumes the virial every step; the energies only when logged
+  int mode = ev ? 1 : (c->have_baro ? 2 : 0);
+  enqueue_forces(c, mode, true, timed);
+  if (c->have_baro) enqueue_crescale(c, mode, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }

@@ -103,7 +103,7 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
 // j-side partial sums travel with them, arriving back in the loading lane
 // after 64 steps.  SPC: coordinates are relative to the block centre (no
 // per-pair minimum image); otherwise general-cell minimum image per pair.
-template <bool EV, int NT, bool SW, bool SPC>
+template <int EV, int NT, bool SW, bool SPC>
 __device__ __forceinline__ void pair_entries(
     const PairParams& P, const HdLattice& L, unsigned b, unsigned w, unsigned lane, float pix,
     float piy, float piz, float qi, unsigned ikey, const LjRow<NT>& R, float4 cb,
@@ -145,7 +145,7 @@ __device__ __forceinline__ void pair_entries(
       float ir6 = inv_r2 * inv_r2 * inv_r2;
       float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
       float elj = 0.f;
-      if (EV || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+      if (EV == 1 || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
       if (SW) {
         float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
         float tau = fmaxf((r2 - rs2) * idl, 0.0f);
@@ -178,18 +178,23 @@ __device__ __forceinline__ void pair_entries(
       float nfactor = -factor;
       efx = fmaf(factor, dx, efx); efy = fmaf(factor, dy, efy); efz = fmaf(factor, dz, efz);
       jfx = fmaf(nfactor, dx, jfx); jfy = fmaf(nfactor, dy, jfy); jfz = fmaf(nfactor, dz, jfz);
-      if (EV) {
+      if (EV == 1) {
         float energy = (ok & lj_on) ? elj : 0.f;
         energy += (ok & c_on) ? ec : 0.f;
+        ee = fmaf(0.5f, energy, ee);
+        je = fmaf(0.5f, energy, je);
+      }
+      if (EV) {
         float fr2 = factor * r2;
-        ee = fmaf(0.5f, energy, ee); ew = fmaf(0.5f, fr2, ew);
-        je = fmaf(0.5f, energy, je); jw = fmaf(0.5f, fr2, jw);
+        ew = fmaf(0.5f, fr2, ew);
+        jw = fmaf(0.5f, fr2, jw);
       }
       pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
       jkey = hd_rot1_u(jkey);
       if (NT > 1) jtype = hd_rot1_u(jtype);
       jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
-      if (EV) { je = hd_rot1(je); jw = hd_rot1(jw); }
+      if (EV == 1) je = hd_rot1(je);
+      if (EV) jw = hd_rot1(jw);
     }
     ai_fx += hd_to_fixed48(efx);
     ai_fy += hd_to_fixed48(efy);
@@ -208,14 +213,14 @@ __device__ __forceinline__ void pair_entries(
       if (vz) atomicAdd(d + 2, (unsigned long long)vz);
       if (EV) {
         long long ve = hd_to_fixed48(je), vw = hd_to_fixed48(jw);
-        if (ve) atomicAdd(fpj_ew + (size_t)jslot * 2, (unsigned long long)ve);
+        if (EV == 1 && ve) atomicAdd(fpj_ew + (size_t)jslot * 2, (unsigned long long)ve);
         if (vw) atomicAdd(fpj_ew + (size_t)jslot * 2 + 1, (unsigned long long)vw);
       }
     }
   }
 }
 
-template <bool EV, int NT, bool SW>
+template <int EV, int NT, bool SW>
 __global__ void __launch_bounds__(256) k_pair(
     PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
     const unsigned* __restrict__ tile_type, const unsigned* __restrict__ tile_key,
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(256) k_pair(
 // displacement takes a per-pair minimum image (ORTHO: one rint per axis).
 #define ALIST_U 4
 
-template <bool EV, int NT, bool SW, bool ORTHO>
+template <int EV, int NT, bool SW, bool ORTHO>
 __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice& L, float ilx,
                                            float ily, float ilz, const LjRow<NT>& R, float4 pi,
                                            float kqi, float4 pj, unsigned jtype, float& efx,
@@ -314,7 +319,7 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
   float ir6 = inv_r2 * inv_r2 * inv_r2;
   float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
   float elj = 0.f;
-  if (EV || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+  if (EV == 1 || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
   if (SW) {
     float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
     float tau = fmaxf((r2 - rs2) * idl, 0.0f);
@@ -342,15 +347,15 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
   efx += factor * dx;
   efy += factor * dy;
   efz += factor * dz;
-  if (EV) {
+  if (EV == 1) {
     float energy = lj_on ? elj : 0.f;
     energy += c_on ? ec : 0.f;
     ee += 0.5f * energy;
-    ew += 0.5f * factor * r2;
   }
+  if (EV) ew += 0.5f * factor * r2;
 }
 
-template <bool EV, int NT, bool SW, bool ORTHO>
+template <int EV, int NT, bool SW, bool ORTHO>
 __global__ void __launch_bounds__(256) k_pair_alist(
     PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
     const unsigned* __restrict__ tile_type, const unsigned* __restrict__ rows_in,
@@ -582,7 +587,7 @@ void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long lo
                        a.entry_count, a.cap, a.n, out);
 }
 
-template <bool EV, int NT, bool SW>
+template <int EV, int NT, bool SW>
 static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams& P) {
   unsigned nblocks = a.npad / 64;
   if (a.atom_lists) {
@@ -603,20 +608,20 @@ static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams&
                      (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
 }
 
-template <bool EV, int NT>
+template <int EV, int NT>
 static void launch_pair_nt(hipStream_t st, const HdPairArgs& a, const PairParams& P, bool sw) {
   if (sw) launch_pair_t<EV, NT, true>(st, a, P);
   else launch_pair_t<EV, NT, false>(st, a, P);
 }
 
-template <bool EV>
+template <int EV>
 static void launch_pair_ev(hipStream_t st, const HdPairArgs& a, const PairParams& P, bool sw) {
   if (P.ntypes <= 1) launch_pair_nt<EV, 1>(st, a, P, sw);
   else if (P.ntypes == 2) launch_pair_nt<EV, 2>(st, a, P, sw);
   else launch_pair_nt<EV, 4>(st, a, P, sw);
 }
 
-void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
+void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
   PairParams P;
   memset(&P, 0, sizeof(P));
   P.ntypes = a.ntypes;
@@ -642,6 +647,7 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, bool ev) {
   P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
   P.excl_span = a.excl_span;
   P.ortho = a.ortho;
-  if (ev) launch_pair_ev<true>(st, a, P, any_switch);
-  else launch_pair_ev<false>(st, a, P, any_switch);
+  if (ev == 1) launch_pair_ev<1>(st, a, P, any_switch);
+  else if (ev == 2) launch_pair_ev<2>(st, a, P, any_switch);
+  else launch_pair_ev<0>(st, a, P, any_switch);
 }

@@ -252,11 +252,19 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, size_t count) {
 // ---------------------------------------------------------------------
 // Radix-2 complex FFT over `nlines` lines of length n held contiguously in
 // LDS, input already in bit-reversed order.  tw[k] = exp(-2 pi i k / twn),
-// k < twn/2; sign = -1 forward, +1 inverse (unnormalised).
+// k < twn/2, staged in LDS by the caller (fft_stage_twiddles) so that no
+// butterfly waits on a global load; sign = -1 forward, +1 inverse
+// (unnormalised).
 // ---------------------------------------------------------------------
+#define FFT_TW_MAX 256  // twn / 2 for the largest axis (nc <= 512)
+
+__device__ __forceinline__ void fft_stage_twiddles(float2* stw, const float2* __restrict__ tw,
+                                                   unsigned twn) {
+  for (unsigned i = threadIdx.x; i < twn / 2; i += blockDim.x) stw[i] = tw[i];
+}
 __device__ __forceinline__ void lds_fft(float2* d, unsigned n, unsigned log2n, unsigned nlines,
-                                        unsigned lstride, int sign,
-                                        const float2* __restrict__ tw, unsigned twn) {
+                                        unsigned lstride, int sign, const float2* tw,
+                                        unsigned twn) {
   unsigned hn = n >> 1;
   unsigned nbf = nlines * hn;
   for (unsigned s = 1; s <= log2n; ++s) {
@@ -283,7 +291,7 @@ __device__ __forceinline__ void lds_fft(float2* d, unsigned n, unsigned log2n, u
 // Decimation-in-frequency twin: natural-order input, bit-reversed output.
 __device__ __forceinline__ void lds_fft_dif(float2* d, unsigned n, unsigned log2n,
                                             unsigned nlines, unsigned lstride, int sign,
-                                            const float2* __restrict__ tw, unsigned twn) {
+                                            const float2* tw, unsigned twn) {
   unsigned hn = n >> 1;
   unsigned nbf = nlines * hn;
   for (unsigned s = log2n; s >= 1; --s) {
@@ -319,18 +327,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
     const long long* __restrict__ rho, float2* __restrict__ cgrid, unsigned nrows, unsigned nc,
     unsigned log2h, const float2* __restrict__ twc) {
   __shared__ float2 d[FFT_TILE];
+  __shared__ float2 stw[FFT_TW_MAX];
   unsigned h = nc >> 1, nch = h + 1;
   unsigned rows = FFT_TILE / h;
   unsigned r0 = blockIdx.x * rows;
   unsigned nr = min(rows, nrows - r0);
   const float inv = 1.0f / 4294967296.0f;
+  fft_stage_twiddles(stw, twc, nc);
   for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
     unsigned r = e / h, m = e - r * h;
     longlong2 v = reinterpret_cast<const longlong2*>(rho + (size_t)(r0 + r) * nc)[m];
     d[r * h + bitrev(m, log2h)] = make_float2((float)v.x * inv, (float)v.y * inv);
   }
   __syncthreads();
-  lds_fft(d, h, log2h, nr, h, -1, twc, nc);
+  lds_fft(d, h, log2h, nr, h, -1, stw, nc);
   for (unsigned e = threadIdx.x; e < nr * nch; e += blockDim.x) {
     unsigned r = e / nch, k = e - r * nch;
     float2 zk = d[r * h + (k % h)];
@@ -339,7 +349,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
     float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
     // O = (Z[k] - conj(Z[h-k])) / (2i)
     float2 O = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
-    float2 w = (k < h) ? twc[k] : make_float2(-1.f, 0.f);  // exp(-2 pi i k / nc)
+    float2 w = (k < h) ? stw[k] : make_float2(-1.f, 0.f);  // exp(-2 pi i k / nc)
     float2 wO = make_float2(w.x * O.x - w.y * O.y, w.x * O.y + w.y * O.x);
     cgrid[(size_t)(r0 + r) * nch + k] = make_float2(E.x + wO.x, E.y + wO.y);
   }
@@ -351,10 +361,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_inv(
     const float2* __restrict__ cgrid, float* __restrict__ vgrid, unsigned nrows, unsigned nc,
     unsigned log2h, const float2* __restrict__ twc) {
   __shared__ float2 d[FFT_TILE];
+  __shared__ float2 stw[FFT_TW_MAX];
   unsigned h = nc >> 1, nch = h + 1;
   unsigned rows = FFT_TILE / h;
   unsigned r0 = blockIdx.x * rows;
   unsigned nr = min(rows, nrows - r0);
+  fft_stage_twiddles(stw, twc, nc);
+  __syncthreads();
   for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
     unsigned r = e / h, k = e - r * h;
     const float2* row = cgrid + (size_t)(r0 + r) * nch;
@@ -362,14 +375,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_inv(
     xc.y = -xc.y;  // conj(X[h-k])
     float2 A = make_float2(xk.x + xc.x, xk.y + xc.y);
     float2 B = make_float2(xk.x - xc.x, xk.y - xc.y);
-    float2 w = twc[k];
+    float2 w = stw[k];
     w.y = -w.y;  // exp(+2 pi i k / nc)
     float2 wB = make_float2(w.x * B.x - w.y * B.y, w.x * B.y + w.y * B.x);
     // Z' = A + i * wB
     d[r * h + bitrev(k, log2h)] = make_float2(A.x - wB.y, A.y + wB.x);
   }
   __syncthreads();
-  lds_fft(d, h, log2h, nr, h, +1, twc, nc);
+  lds_fft(d, h, log2h, nr, h, +1, stw, nc);
   for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
     unsigned r = e / h, m = e - r * h;
     reinterpret_cast<float2*>(vgrid + (size_t)(r0 + r) * nc)[m] = d[r * h + m];
@@ -408,13 +421,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgri
                                                        unsigned log2b, int sign,
                                                        const float2* __restrict__ twb) {
   __shared__ float2 d[FFT_CW * 257];
+  __shared__ float2 stw[FFT_TW_MAX];
   unsigned nchunk = (nch + FFT_CW - 1) / FFT_CW;
   unsigned a = blockIdx.x / nchunk, c0 = (blockIdx.x - a * nchunk) * FFT_CW;
   unsigned ncol = min(FFT_CW, nch - c0);
   size_t base = (size_t)a * nb * nch + c0;
+  fft_stage_twiddles(stw, twb, nb);
   strided_load(d, cgrid, base, nb, log2b, nch, ncol, true);
   __syncthreads();
-  lds_fft(d, nb, log2b, FFT_CW, nb + 1, sign, twb, nb);
+  lds_fft(d, nb, log2b, FFT_CW, nb + 1, sign, stw, nb);
   strided_store(d, cgrid, base, nb, log2b, nch, ncol, false);
 }
 
@@ -430,6 +445,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
     const float* __restrict__ bfb, const float* __restrict__ bfc,
     const float* __restrict__ lattice, float kc, float alpha, double2* __restrict__ partials) {
   __shared__ float2 d[FFT_CW * 257];
+  __shared__ float2 stw[FFT_TW_MAX];
   __shared__ double red_e[FFT_THREADS], red_w[FFT_THREADS];
   unsigned nchunk = (nch + FFT_CW - 1) / FFT_CW;
   unsigned b = blockIdx.x / nchunk, c0 = (blockIdx.x - b * nchunk) * FFT_CW;
@@ -437,9 +453,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
   size_t base = (size_t)b * nch + c0;
   size_t estride = (size_t)nb * nch;
   unsigned ls = na + 1;
+  fft_stage_twiddles(stw, twa, na);
   strided_load(d, cgrid, base, na, log2a, estride, ncol, true);
   __syncthreads();
-  lds_fft(d, na, log2a, FFT_CW, ls, -1, twa, na);
+  lds_fft(d, na, log2a, FFT_CW, ls, -1, stw, na);
   HdLattice L = hd_load_lattice(lattice);
   float v_box = L.lx * L.ly * L.lz;
   float four_alpha2 = 4.0f * alpha * alpha;
@@ -484,7 +501,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = make_double2(red_e[0], red_w[0]);
-  lds_fft_dif(d, na, log2a, FFT_CW, ls, +1, twa, na);
+  lds_fft_dif(d, na, log2a, FFT_CW, ls, +1, stw, na);
   strided_store(d, cgrid, base, na, log2a, estride, ncol, true);
 }
 

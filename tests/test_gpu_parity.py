@@ -281,6 +281,41 @@ def test_crescale_move_matches_formula():
     assert b["draws"] == 1
 
 
+def test_barostat_virial_only_steps_match_full_scalar_steps():
+    # between logged steps a barostat run evaluates only the virial; the
+    # virial (and so every box move) is the one a full energy+virial step
+    # gives, and the last logged energy survives the virial-only steps
+    say("case: water 8192 C-rescale, 6 steps virial-only vs 6 full-scalar steps")
+    s = systems.config_water_8192()
+    kt = U.temperature(300.0)
+    baro = dict(pressure=U.pressure(1e5), tau=U.time(1e-12),
+                compressibility=U.inv_pressure(4.5e-10), kt=kt, seed=13)
+    from heddlemd_amd.rng import maxwell_boltzmann
+    v = maxwell_boltzmann(s.mass, kt, seed=4, n_constraints=s.n_constraints)
+    out = []
+    for per_step in (False, True):
+        e = _water_engine(s, crescale=baro, velocities=v, nhc_kt=kt, nhc_tau=U.time(1e-13))
+        e.constrain_velocities()
+        e.force_eval(scalars=True)
+        pe0 = e.scalars()["pe"]
+        if per_step:
+            for _ in range(6):
+                e.step(1, scalars_last=True)
+        else:
+            e.step(6, scalars_last=False)
+            assert e.scalars()["pe"] == pe0
+            e.step(1, scalars_last=True)
+        if per_step:
+            e.step(1, scalars_last=True)
+        p, _ = e.positions(images=True)
+        out.append((p, e.velocities(), e.lattice(), e.scalars()))
+        e.close()
+    (p0, v0, l0, s0), (p1, v1, l1, s1) = out
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+    assert s0["pe"] == s1["pe"] and s0["virial"] == s1["virial"]
+
+
 def test_langevin_baoab_step_matches_oracle(oracle):
     # one BAOAB step of jittered argon: B + A with the initial forces, O + A
     # with draw counter 0, then the fused B with the new forces; f32
