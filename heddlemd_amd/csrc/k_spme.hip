@@ -262,14 +262,46 @@ __device__ __forceinline__ void fft_stage_twiddles(float2* stw, const float2* __
                                                    unsigned twn) {
   for (unsigned i = threadIdx.x; i < twn / 2; i += blockDim.x) stw[i] = tw[i];
 }
+__device__ __forceinline__ float2 cmul(float2 b, float2 w) {
+  return make_float2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
+}
+
+// Two radix-2 stages per LDS pass (a radix-4 unit of 4 elements per thread:
+// the same butterflies in the same order, half the passes and barriers),
+// a last radix-2 pass when log2n is odd.
 __device__ __forceinline__ void lds_fft(float2* d, unsigned n, unsigned log2n, unsigned nlines,
                                         unsigned lstride, int sign, const float2* tw,
                                         unsigned twn) {
-  unsigned hn = n >> 1;
-  unsigned nbf = nlines * hn;
-  for (unsigned s = 1; s <= log2n; ++s) {
+  unsigned s = 1;
+  for (; s + 1 <= log2n; s += 2) {
+    unsigned h = 1u << (s - 1);         // half of stage s
+    unsigned t1 = twn >> s, t2 = twn >> (s + 1);
+    unsigned qn = n >> 2;                // radix-4 units per line
+    unsigned nu = nlines * qn;
+    for (unsigned idx = threadIdx.x; idx < nu; idx += blockDim.x) {
+      unsigned line = idx >> (log2n - 2), u = idx & (qn - 1);
+      unsigned j = u & (h - 1);
+      unsigned i0 = ((u >> (s - 1)) << (s + 1)) + j;
+      float2 w1 = tw[j * t1], w2 = tw[j * t2], w3 = tw[(j + h) * t2];
+      if (sign > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+      float2* row = d + line * lstride;
+      float2 a0 = row[i0], a1 = row[i0 + h], a2 = row[i0 + 2 * h], a3 = row[i0 + 3 * h];
+      float2 b1 = cmul(a1, w1), b3 = cmul(a3, w1);
+      float2 y0 = make_float2(a0.x + b1.x, a0.y + b1.y), y1 = make_float2(a0.x - b1.x, a0.y - b1.y);
+      float2 y2 = make_float2(a2.x + b3.x, a2.y + b3.y), y3 = make_float2(a2.x - b3.x, a2.y - b3.y);
+      float2 c2 = cmul(y2, w2), c3 = cmul(y3, w3);
+      row[i0] = make_float2(y0.x + c2.x, y0.y + c2.y);
+      row[i0 + 2 * h] = make_float2(y0.x - c2.x, y0.y - c2.y);
+      row[i0 + h] = make_float2(y1.x + c3.x, y1.y + c3.y);
+      row[i0 + 3 * h] = make_float2(y1.x - c3.x, y1.y - c3.y);
+    }
+    __syncthreads();
+  }
+  if (s == log2n) {
     unsigned half = 1u << (s - 1);
     unsigned tstr = twn >> s;
+    unsigned hn = n >> 1;
+    unsigned nbf = nlines * hn;
     for (unsigned idx = threadIdx.x; idx < nbf; idx += blockDim.x) {
       unsigned line = idx >> (log2n - 1);
       unsigned k = idx & (hn - 1);
@@ -279,8 +311,7 @@ __device__ __forceinline__ void lds_fft(float2* d, unsigned n, unsigned log2n, u
       float2 w = tw[j * tstr];
       if (sign > 0) w.y = -w.y;
       float2* row = d + line * lstride;
-      float2 a = row[i0], b = row[i1];
-      float2 bw = make_float2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
+      float2 a = row[i0], bw = cmul(row[i1], w);
       row[i0] = make_float2(a.x + bw.x, a.y + bw.y);
       row[i1] = make_float2(a.x - bw.x, a.y - bw.y);
     }
@@ -288,28 +319,50 @@ __device__ __forceinline__ void lds_fft(float2* d, unsigned n, unsigned log2n, u
   }
 }
 
-// Decimation-in-frequency twin: natural-order input, bit-reversed output.
+// Decimation-in-frequency twin: natural-order input, bit-reversed output;
+// stages log2n .. 1, two per pass (the radix-2 DIF butterflies in order).
 __device__ __forceinline__ void lds_fft_dif(float2* d, unsigned n, unsigned log2n,
                                             unsigned nlines, unsigned lstride, int sign,
                                             const float2* tw, unsigned twn) {
-  unsigned hn = n >> 1;
-  unsigned nbf = nlines * hn;
-  for (unsigned s = log2n; s >= 1; --s) {
-    unsigned half = 1u << (s - 1);
-    unsigned tstr = twn >> s;
+  unsigned s = log2n;
+  for (; s >= 2; s -= 2) {
+    unsigned H = 1u << (s - 2);          // half of stage s - 1
+    unsigned t1 = twn >> s, t2 = twn >> (s - 1);
+    unsigned qn = n >> 2;
+    unsigned nu = nlines * qn;
+    for (unsigned idx = threadIdx.x; idx < nu; idx += blockDim.x) {
+      unsigned line = idx >> (log2n - 2), u = idx & (qn - 1);
+      unsigned j = u & (H - 1);
+      unsigned i0 = ((u >> (s - 2)) << s) + j;
+      float2 w1 = tw[j * t1], w1b = tw[(j + H) * t1], w2 = tw[j * t2];
+      if (sign > 0) { w1.y = -w1.y; w1b.y = -w1b.y; w2.y = -w2.y; }
+      float2* row = d + line * lstride;
+      float2 x0 = row[i0], x1 = row[i0 + H], x2 = row[i0 + 2 * H], x3 = row[i0 + 3 * H];
+      float2 y0 = make_float2(x0.x + x2.x, x0.y + x2.y);
+      float2 y2 = cmul(make_float2(x0.x - x2.x, x0.y - x2.y), w1);
+      float2 y1 = make_float2(x1.x + x3.x, x1.y + x3.y);
+      float2 y3 = cmul(make_float2(x1.x - x3.x, x1.y - x3.y), w1b);
+      row[i0] = make_float2(y0.x + y1.x, y0.y + y1.y);
+      row[i0 + H] = cmul(make_float2(y0.x - y1.x, y0.y - y1.y), w2);
+      row[i0 + 2 * H] = make_float2(y2.x + y3.x, y2.y + y3.y);
+      row[i0 + 3 * H] = cmul(make_float2(y2.x - y3.x, y2.y - y3.y), w2);
+    }
+    __syncthreads();
+  }
+  if (s == 1) {
+    unsigned hn = n >> 1;
+    unsigned nbf = nlines * hn;
     for (unsigned idx = threadIdx.x; idx < nbf; idx += blockDim.x) {
       unsigned line = idx >> (log2n - 1);
       unsigned k = idx & (hn - 1);
-      unsigned j = k & (half - 1);
-      unsigned i0 = ((k >> (s - 1)) << s) + j;
-      unsigned i1 = i0 + half;
-      float2 w = tw[j * tstr];
+      unsigned i0 = k << 1, i1 = i0 + 1;
+      float2 w = tw[0];
       if (sign > 0) w.y = -w.y;
       float2* row = d + line * lstride;
       float2 a = row[i0], b = row[i1];
       float2 dlt = make_float2(a.x - b.x, a.y - b.y);
       row[i0] = make_float2(a.x + b.x, a.y + b.y);
-      row[i1] = make_float2(dlt.x * w.x - dlt.y * w.y, dlt.x * w.y + dlt.y * w.x);
+      row[i1] = cmul(dlt, w);
     }
     __syncthreads();
   }
