@@ -33,8 +33,14 @@ METRIC = "ns/day (196k-atom SPC SPME) & batched atom-steps/s vs plain, same erro
 # MI355X published peaks used for the roofline fraction
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# Flops per useful pair (LJ + erfc-Coulomb, transcendental = 1 flop)
+# Flops per useful pair of the production pair kernel (LJ + Hastings-erfc
+# Coulomb force, i and j force sums, virial; a transcendental or an FMA's
+# multiply-add counts once per operation), a conservative count
 FLOPS_PER_PAIR = 55.0
+# The production pair kernel: virial only (a barostat step that logs
+# nothing), two LJ types, no switch
+PAIR_KERNEL = "void k_pair<2, 2, false>"
+
 
 
 def log(msg):
@@ -435,10 +441,11 @@ def main():
         flops = pairs * FLOPS_PER_PAIR
         achieved = flops / (pair_ms * 1e-3) / 1e12
         result["roofline"] = {
-            "kernel": "k_pair (fused LJ + real-space Ewald, packed wave64 list)",
+            "kernel": "k_pair<2, 2, false> (fused LJ + real-space Ewald forces and virial, "
+                      "packed wave64 cluster list; FP32 vector ALU, no matrix-core form)",
             "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": traffic_from_profile("void k_pair<true"),
+            "traffic": traffic_from_profile(PAIR_KERNEL),
             "traffic_source": "profiles/water196k_step_traffic.md (rocprofv3 --pmc, calibrated)",
             "useful_pairs": pairs, "flops_per_pair": FLOPS_PER_PAIR,
             "mean_launch_ms": round(pair_ms, 4),
