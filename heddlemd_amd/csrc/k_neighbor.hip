@@ -299,9 +299,11 @@ __device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, f
 
 // Block-pair candidates: bit (jb % 64) of mask[b * nchunks + jb / 64] is
 // set when j-block jb's box lies within r_search of block b's (jb >= b only
-// for the half list).  One wave per (b, chunk of 64 j-blocks), so the box
-// tests of a rebuild run at full width instead of as a dependent loop inside
-// each block's wave.
+// for the half list).  Each wave handles (b, chunk of 64 j-blocks) items in
+// a grid-stride loop, so the box tests of a rebuild run at full width
+// instead of as a dependent loop inside each block's wave, while the launch
+// stays small on the steps where the rebuild gate closes it at once.
+#define NB_PAIR_WAVES 4096
 __global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ center,
                                                      const float4* __restrict__ half,
                                                      const float* __restrict__ lattice, float rs2,
@@ -309,17 +311,19 @@ __global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ 
                                                      unsigned nblocks, unsigned nchunks, int full,
                                                      const unsigned* status) {
   if (!nb_rebuild_due(status)) return;
-  unsigned wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  if (wv >= nblocks * nchunks) return;
-  unsigned b = wv / nchunks, c = wv % nchunks;
-  unsigned jb = c * 64 + lane;
-  bool cand = false;
-  if (jb < nblocks && (full || jb >= b)) {
-    HdLattice L = hd_load_lattice(lattice);
-    cand = nb_box_dist2(center[b], half[b], center[jb], half[jb], L) <= rs2;
+  unsigned lane = threadIdx.x & 63;
+  unsigned nwaves = gridDim.x * (blockDim.x >> 6);
+  HdLattice L = hd_load_lattice(lattice);
+  for (unsigned wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; wv < nblocks * nchunks;
+       wv += nwaves) {
+    unsigned b = wv / nchunks, c = wv % nchunks;
+    unsigned jb = c * 64 + lane;
+    bool cand = false;
+    if (jb < nblocks && (full || jb >= b))
+      cand = nb_box_dist2(center[b], half[b], center[jb], half[jb], L) <= rs2;
+    unsigned long long m = __ballot(cand);
+    if (lane == 0) mask[wv] = m;
   }
-  unsigned long long m = __ballot(cand);
-  if (lane == 0) mask[wv] = m;
 }
 
 #define NB_CAND_MAX 1024
@@ -643,9 +647,14 @@ static inline unsigned cdiv(unsigned a, unsigned b) { return (a + b - 1) / b; }
 
 unsigned hd_voxel_scan_parts(unsigned nvox) { return cdiv(nvox, NB_SCAN_CHUNK); }
 
+// Kernels gated by the rebuild flag run grid-stride loops over a modest grid
+// (NB_GATED_WGS workgroups): on the steps where the gate closes them they
+// retire quickly, and on rebuild steps the loops cover the work.
+#define NB_GATED_WGS 512u
+
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
   const unsigned T = 256;
-  unsigned gridN = min(cdiv(a.n, T), 2048u);
+  unsigned gridN = min(cdiv(a.n, T), NB_GATED_WGS);
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st, a.posq,
                      a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
@@ -660,10 +669,10 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
                      a.voxel_part, a.voxel_off, a.voxel_cursor, a.status, a.nvox, nparts);
   hipLaunchKernelGGL(k_voxel_scatter, dim3(gridN), dim3(T), 0, st, a.atom_key, a.voxel_off,
                      a.voxel_cursor, a.sorted_id, a.status, a.n);
-  unsigned gridV = min(cdiv(a.nvox, T), 2048u);
+  unsigned gridV = min(cdiv(a.nvox, T), NB_GATED_WGS);
   hipLaunchKernelGGL(k_voxel_sort, dim3(gridV), dim3(T), 0, st, a.voxel_off, a.voxel_count,
                      a.sorted_id, a.slot_of, a.status, a.nvox);
-  unsigned gridP = min(cdiv(a.npad, T), 2048u);
+  unsigned gridP = min(cdiv(a.npad, T), NB_GATED_WGS);
   hipLaunchKernelGGL(k_slot_static, dim3(gridP), dim3(T), 0, st, a.sorted_id, a.type, a.key,
                      a.tile_type, a.tile_key, a.status, a.n, a.npad);
 }
@@ -682,7 +691,8 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
   unsigned nchunks = hd_block_pair_chunks(nblocks);
   hipLaunchKernelGGL(k_block_bbox, dim3(cdiv(nblocks * 64, 256)), dim3(256), 0, st, a.tile_posq,
                      a.center, a.half, a.status, a.n, nblocks);
-  hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(nblocks * nchunks * 64, 256)), dim3(256), 0, st,
+  unsigned pair_waves = min(nblocks * nchunks, (unsigned)NB_PAIR_WAVES);
+  hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(pair_waves * 64, 256)), dim3(256), 0, st,
                      a.center, a.half, a.lattice, a.rs2, a.block_mask, nblocks, nchunks,
                      a.atom_lists, a.status);
   if (a.atom_lists)
@@ -694,7 +704,7 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
     hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0,
                        st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask, nchunks,
                        a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
-  unsigned gridN = min(cdiv(a.n, 256), 2048u);
+  unsigned gridN = min(cdiv(a.n, 256), NB_GATED_WGS);
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,
                      a.lattice, a.ref_lattice, a.status, a.n);
