@@ -184,7 +184,7 @@ __device__ __forceinline__ void pair_entries(
         ee = fmaf(0.5f, energy, ee);
         je = fmaf(0.5f, energy, je);
       }
-      if (EV) {
+      if (EV && !SPC) {
         float fr2 = factor * r2;
         ew = fmaf(0.5f, fr2, ew);
         jw = fmaf(0.5f, fr2, jw);
@@ -194,7 +194,17 @@ __device__ __forceinline__ void pair_entries(
       if (NT > 1) jtype = hd_rot1_u(jtype);
       jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
       if (EV == 1) je = hd_rot1(je);
-      if (EV) jw = hd_rot1(jw);
+      if (EV && !SPC) jw = hd_rot1(jw);
+    }
+    if (EV && SPC) {
+      // the entry's pair virial sum_ij F_ij . (r_i - r_j) split by atom: both
+      // positions are relative to the block centre (j in the image nearest
+      // it), so the i side is r_i . F_i(entry) and the j side r_j . F_j(entry)
+      // (j partials are back in their loading lanes, and so are the j
+      // positions); pairs inside the block are evaluated from both sides and
+      // their j sides dropped, which again leaves F_ij . r_ij once
+      ew = pix * efx + piy * efy + piz * efz;
+      jw = pjx * jfx + pjy * jfy + pjz * jfz;
     }
     ai_fx += hd_to_fixed48(efx);
     ai_fy += hd_to_fixed48(efy);

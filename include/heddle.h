@@ -243,6 +243,11 @@ int hd_mlp_train_log(hd_mlp *mlp, hd_ctx *ctx, int epochs, int batch, float lr, 
 int hd_get_positions(hd_ctx *ctx, float *posq, int32_t *ix, int32_t *iy, int32_t *iz);
 int hd_get_velocities(hd_ctx *ctx, float *vx, float *vy, float *vz);
 int hd_get_forces(hd_ctx *ctx, float *fx, float *fy, float *fz);
+/* Per-atom energy and virial of the last scalar evaluation.  Pair energies
+ * are split half and half between the two atoms (the reference's
+ * convention); the pair virial is split as r_i . F_i over pairs measured
+ * from the i-block's centre when the box has no tilt (the same total, a
+ * different split), half and half otherwise. */
 int hd_get_per_atom(hd_ctx *ctx, float *pe, float *virial);
 int hd_get_lattice(hd_ctx *ctx, double lattice[6]);
 /* Scalars of the last scalar evaluation, FP64:
