@@ -498,10 +498,10 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
       bspline_wd((float)d + ta, wa[d], dwa[d]);
       bspline_wd((float)d + tb, wb[d], dwb[d]);
       bspline_wd((float)d + tc, wc[d], dwc[d]);
-      int g;
-      g = ga0 - d; gia[d] = ((g % (int)a.na) + (int)a.na) % (int)a.na;
-      g = gb0 - d; gib[d] = ((g % (int)a.nb) + (int)a.nb) % (int)a.nb;
-      g = gc0 - d; gic[d] = ((g % (int)a.nc) + (int)a.nc) % (int)a.nc;
+      // periodic wrap; the grid sides are powers of two
+      gia[d] = (ga0 - d) & ((int)a.na - 1);
+      gib[d] = (gb0 - d) & ((int)a.nb - 1);
+      gic[d] = (gc0 - d) & ((int)a.nc - 1);
     }
     float phi = 0.f, gra = 0.f, grb = 0.f, grc = 0.f;
 #pragma unroll

@@ -104,10 +104,11 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   if (g == SPME_NO_CELL) return;
   int ga = (int)(g & 1023u), gb = (int)((g >> 10) & 1023u), gc = (int)(g >> 20);
   // brick-relative index of the stencil's first cell per axis; cell
-  // (g - d) mod n lies in the brick when 0 <= rel - d < extent
-  int rla = ((ga - c0a) % (int)na + (int)na) % (int)na;
-  int rlb = ((gb - c0b) % (int)nb + (int)nb) % (int)nb;
-  int rlc = ((gc - c0c) % (int)nc + (int)nc) % (int)nc;
+  // (g - d) mod n lies in the brick when 0 <= rel - d < extent (the grid
+  // sides are powers of two, so mod n is a mask)
+  int rla = (ga - c0a) & ((int)na - 1);
+  int rlb = (gb - c0b) & ((int)nb - 1);
+  int rlc = (gc - c0c) & ((int)nc - 1);
   if (rla >= (int)ba + SPME_P - 1 || rlb >= (int)bb + SPME_P - 1 ||
       rlc >= (int)bc + SPME_P - 1)
     return;
@@ -180,9 +181,10 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
     if (k >= nwin) break;
     unsigned kc = k % wc_n, kab = k / wc_n;
     unsigned kb = kab % wb_n, ka = kab / wb_n;
-    unsigned va = (unsigned)(((vla + (int)ka) % (int)nva + (int)nva) % (int)nva);
-    unsigned vb = (unsigned)(((vlb + (int)kb) % (int)nvb + (int)nvb) % (int)nvb);
-    unsigned vc = (unsigned)(((vlc + (int)kc) % (int)nvc + (int)nvc) % (int)nvc);
+    // periodic wrap; the voxel grid sides are powers of two
+    unsigned va = (unsigned)((vla + (int)ka) & ((int)nva - 1));
+    unsigned vb = (unsigned)((vlb + (int)kb) & ((int)nvb - 1));
+    unsigned vc = (unsigned)((vlc + (int)kc) & ((int)nvc - 1));
     unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
     unsigned s0 = voxel_off[r], s1 = voxel_off[r + 1];
     if (flat) {
