@@ -49,7 +49,17 @@ struct PairParams {
   float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
   unsigned excl_span;  // |key_i - key_j| <= excl_span excludes the pair
   int ortho;           // the box has no tilt
+  // one cutoff for every interacting type pair and Coulomb, no switch
+  int uniform;
+  float cg;  // -alpha^2 log2(e): exp(-(alpha r)^2) = exp2(cg r^2)
+  float pa;  // 0.3275911 alpha, the Hastings t = 1 / (1 + pa r)
 };
+
+// 32768-bit map of the i-block's keys widened by the exclusion span, used
+// to skip the per-pair key test in entries with no excludable j-atom; with
+// a wider span every entry runs the test
+#define PAIR_KEYMAP_WORDS 1024
+#define PAIR_KEYMAP_SPAN 8
 
 // The i-atom's row of the LJ type-pair table, held in VGPRs for the whole
 // kernel.  NT = table width (1, 2 or 4).  Each value passes through an
@@ -98,19 +108,76 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
 #define PAIR_FAR_I 1.0e18f
 #define PAIR_FAR_J -1.0e18f
 
+// The 64 steps of one entry on the shifted path when one cutoff serves LJ
+// and Coulomb and there is no switch.  Same terms as the general loop,
+// regrouped for fewer instructions per pair: exp(-(alpha r)^2) as
+// exp2(cg r^2); the Ewald force factor erfc(alpha r)/r^3 +
+// 2 alpha/sqrt(pi) exp(-(alpha r)^2)/r^2 as gauss * (poly(t) t / r +
+// 2 alpha/sqrt(pi)) / r^2; the LJ and Coulomb factors summed before the
+// shared 1/r^2 and a single cutoff select.  The force arithmetic is the same
+// for every EV, so energy steps move atoms exactly as force-only steps do.
+// XK = false drops the key test (no j-atom of the entry can be excluded
+// against the block).
+template <int EV, int NT, bool XK>
+__device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float piy, float piz,
+                                         unsigned ikey, unsigned span, float kqi, float rc2,
+                                         float cg, float pa, float k2, float& pjx, float& pjy,
+                                         float& pjz, float& qj, unsigned& jkey, unsigned& jtype,
+                                         float& efx, float& efy, float& efz, float& jfx,
+                                         float& jfy, float& jfz, float& ee, float& je) {
+#pragma unroll 4
+  for (int t = 0; t < 64; ++t) {
+    float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float inv_r = __builtin_amdgcn_rsqf(r2);
+    float inv_r2 = inv_r * inv_r;
+    float ir6 = inv_r2 * inv_r2 * inv_r2;
+    float lj = ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+    float r = r2 * inv_r;
+    float gauss = __builtin_amdgcn_exp2f(cg * r2);
+    float tt = __builtin_amdgcn_rcpf(fmaf(pa, r, 1.0f));
+    float poly =
+        (0.254829592f +
+         (-0.284496736f + (1.421413741f + (-1.453152027f + 1.061405429f * tt) * tt) * tt) * tt) *
+        tt;
+    float ci = gauss * fmaf(poly, inv_r, k2);
+    float qq = kqi * qj;
+    float factor = inv_r2 * fmaf(qq, ci, lj);
+    bool on = r2 <= rc2;
+    if (XK) on = on & ((jkey - ikey + span) > 2u * span);
+    factor = on ? factor : 0.f;
+    float nfactor = -factor;
+    efx = fmaf(factor, dx, efx); efy = fmaf(factor, dy, efy); efz = fmaf(factor, dz, efz);
+    jfx = fmaf(nfactor, dx, jfx); jfy = fmaf(nfactor, dy, jfy); jfz = fmaf(nfactor, dz, jfz);
+    if (EV == 1) {
+      // LJ energy (lj.rs:237-247) and qq erfc(alpha r) / r (spme.rs:744-770)
+      float elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+      float energy = on ? fmaf(qq * gauss, poly * inv_r, elj) : 0.f;
+      ee = fmaf(0.5f, energy, ee);
+      je = fmaf(0.5f, energy, je);
+    }
+    pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
+    if (XK) jkey = hd_rot1_u(jkey);
+    if (NT > 1) jtype = hd_rot1_u(jtype);
+    jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
+    if (EV == 1) je = hd_rot1(je);
+  }
+}
+
 // One wave's share of the entries of block b.  Lane = i-atom; the 64
 // j-atoms of an entry rotate through the wave with DPP wave_rol:1 and the
 // j-side partial sums travel with them, arriving back in the loading lane
 // after 64 steps.  SPC: coordinates are relative to the block centre (no
 // per-pair minimum image); otherwise general-cell minimum image per pair.
-template <int EV, int NT, bool SW, bool SPC>
+template <int EV, int NT, bool SW, bool SPC, bool UC>
 __device__ __forceinline__ void pair_entries(
     const PairParams& P, const HdLattice& L, unsigned b, unsigned w, unsigned lane, float pix,
     float piy, float piz, float qi, unsigned ikey, const LjRow<NT>& R, float4 cb,
     const float4* __restrict__ tile_posq, const unsigned* __restrict__ tile_type,
     const unsigned* __restrict__ tile_key, const unsigned* __restrict__ ents, unsigned nent,
     unsigned n, unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
-    long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w) {
+    long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w,
+    const unsigned* keymap) {
   const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
   const float alpha = P.alpha, coul_rc2 = P.coul_rc2;
   const float kqi = P.kc * qi;
@@ -134,6 +201,19 @@ __device__ __forceinline__ void pair_entries(
     }
     float jfx = 0.f, jfy = 0.f, jfz = 0.f, je = 0.f, jw = 0.f;
     float efx = 0.f, efy = 0.f, efz = 0.f, ee = 0.f, ew = 0.f;
+    if constexpr (UC) {
+      // a j-atom can be excluded only when its key is in the block's map
+      unsigned word = keymap[(jkey >> 5) & (PAIR_KEYMAP_WORDS - 1u)];
+      bool hit = jvalid && ((word >> (jkey & 31u)) & 1u);
+      if (P.excl_span > PAIR_KEYMAP_SPAN || __any(hit))
+        uc_steps<EV, NT, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
+                               two_a_sqrtpi, pjx, pjy, pjz, qj, jkey, jtype, efx, efy, efz,
+                               jfx, jfy, jfz, ee, je);
+      else
+        uc_steps<EV, NT, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
+                                two_a_sqrtpi, pjx, pjy, pjz, qj, jkey, jtype, efx, efy, efz,
+                                jfx, jfy, jfz, ee, je);
+    } else {
 #pragma unroll 4
     for (int t = 0; t < 64; ++t) {
       float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
@@ -196,6 +276,7 @@ __device__ __forceinline__ void pair_entries(
       if (EV == 1) je = hd_rot1(je);
       if (EV && !SPC) jw = hd_rot1(jw);
     }
+    }
     if (EV && SPC) {
       // the entry's pair virial sum_ij F_ij . (r_i - r_j) split by atom: both
       // positions are relative to the block centre (j in the image nearest
@@ -239,6 +320,7 @@ __global__ void __launch_bounds__(256) k_pair(
     unsigned cap, unsigned n, long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
     unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
   __shared__ long long red[PAIR_WAVES][5][64];
+  __shared__ unsigned keymap[PAIR_KEYMAP_WORDS];
   unsigned b = blockIdx.x;
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HdLattice L = hd_load_lattice(lattice);
@@ -261,14 +343,31 @@ __global__ void __launch_bounds__(256) k_pair(
   unsigned nent = entry_count[b];
   const unsigned* ents = entry_atoms + (size_t)b * cap * 64;
   long long ax = 0, ay = 0, az = 0, ae = 0, aw = 0;
-  if (spc)
-    pair_entries<EV, NT, SW, true>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb, tile_posq,
-                                   tile_type, tile_key, ents, nent, n, fpj, fpj_ew, ax, ay, az,
-                                   ae, aw);
+  // the regrouped loop: one cutoff, no switch
+  constexpr bool UCOK = !SW;
+  bool uc = UCOK && spc && P.uniform;  // uniform over the workgroup
+  if (uc) {
+    if constexpr (UCOK) {
+      for (unsigned k = threadIdx.x; k < PAIR_KEYMAP_WORDS; k += 64 * PAIR_WAVES) keymap[k] = 0u;
+      __syncthreads();
+      if (w == 0 && ivalid && P.excl_span <= PAIR_KEYMAP_SPAN)
+        for (unsigned d = 0; d <= 2u * P.excl_span; ++d) {
+          unsigned key = ikey - P.excl_span + d;
+          atomicOr(&keymap[(key >> 5) & (PAIR_KEYMAP_WORDS - 1u)], 1u << (key & 31u));
+        }
+      __syncthreads();
+      pair_entries<EV, NT, SW, true, true>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
+                                           tile_posq, tile_type, tile_key, ents, nent, n, fpj,
+                                           fpj_ew, ax, ay, az, ae, aw, keymap);
+    }
+  } else if (spc)
+    pair_entries<EV, NT, SW, true, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
+                                          tile_posq, tile_type, tile_key, ents, nent, n, fpj,
+                                          fpj_ew, ax, ay, az, ae, aw, keymap);
   else
-    pair_entries<EV, NT, SW, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb, tile_posq,
-                                    tile_type, tile_key, ents, nent, n, fpj, fpj_ew, ax, ay, az,
-                                    ae, aw);
+    pair_entries<EV, NT, SW, false, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
+                                           tile_posq, tile_type, tile_key, ents, nent, n, fpj,
+                                           fpj_ew, ax, ay, az, ae, aw, keymap);
   red[w][0][lane] = ax;
   red[w][1][lane] = ay;
   red[w][2][lane] = az;
@@ -657,6 +756,12 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
   P.spc_margin = (float)(a.rmax + 0.5 * a.r_skin);
   P.excl_span = a.excl_span;
   P.ortho = a.ortho;
+  bool uniform = P.coul_rc2 > 0.f && !any_switch;
+  for (int k = 0; k < a.ntypes * a.ntypes; ++k)
+    if (!(P.c6[k] == 0.f && P.c12[k] == 0.f) && P.ljcut2[k] != P.coul_rc2) uniform = false;
+  P.uniform = uniform ? 1 : 0;
+  P.cg = (float)(-a.alpha * a.alpha * 1.4426950408889634);
+  P.pa = (float)(0.3275911 * a.alpha);
   if (ev == 1) launch_pair_ev<1>(st, a, P, any_switch);
   else if (ev == 2) launch_pair_ev<2>(st, a, P, any_switch);
   else launch_pair_ev<0>(st, a, P, any_switch);
