@@ -61,6 +61,22 @@ struct PairParams {
 #define PAIR_KEYMAP_WORDS 1024
 #define PAIR_KEYMAP_SPAN 8
 
+// One wave's copy of the current entry's j-atoms for the regrouped loop,
+// written twice over so that step t reads record lane + t with no wrap:
+// positions (block-centre relative) and charges, types, keys.
+struct PairJStage {
+  float4 pq[128];
+  unsigned type[128];
+  unsigned key[128];
+};
+
+// Orders one wave's LDS writes before its lanes read each other's records.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The i-atom's row of the LJ type-pair table, held in VGPRs for the whole
 // kernel.  NT = table width (1, 2 or 4).  Each value passes through an
 // empty asm that pins it to a register: otherwise the compiler folds the
@@ -109,7 +125,10 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
 #define PAIR_FAR_J -1.0e18f
 
 // The 64 steps of one entry on the shifted path when one cutoff serves LJ
-// and Coulomb and there is no switch.  Same terms as the general loop,
+// and Coulomb and there is no switch.  Lane l meets j-atom (l + t) mod 64 at
+// step t, read from the wave's LDS stage (jb, jt, jk point at record l); the
+// j-side sums still rotate with DPP, so they meet the same j-atoms and are
+// back in the loading lane after 64 steps.  Same terms as the general loop,
 // regrouped for fewer instructions per pair: exp(-(alpha r)^2) as
 // exp2(cg r^2); the Ewald force factor erfc(alpha r)/r^3 +
 // 2 alpha/sqrt(pi) exp(-(alpha r)^2)/r^2 as gauss * (poly(t) t / r +
@@ -121,13 +140,16 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
 template <int EV, int NT, bool XK>
 __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float piy, float piz,
                                          unsigned ikey, unsigned span, float kqi, float rc2,
-                                         float cg, float pa, float k2, float& pjx, float& pjy,
-                                         float& pjz, float& qj, unsigned& jkey, unsigned& jtype,
-                                         float& efx, float& efy, float& efz, float& jfx,
-                                         float& jfy, float& jfz, float& ee, float& je) {
-#pragma unroll 4
+                                         float cg, float pa, float k2, const float4* jb,
+                                         const unsigned* jt, const unsigned* jk, float& efx,
+                                         float& efy, float& efz, float& jfx, float& jfy,
+                                         float& jfz, float& ee, float& je) {
+#pragma unroll 8
   for (int t = 0; t < 64; ++t) {
-    float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
+    float4 pq = jb[t];
+    unsigned jtype = (NT > 1) ? jt[t] : 0u;
+    float qj = pq.w;
+    float dx = pix - pq.x, dy = piy - pq.y, dz = piz - pq.z;
     float r2 = dx * dx + dy * dy + dz * dz;
     float inv_r = __builtin_amdgcn_rsqf(r2);
     float inv_r2 = inv_r * inv_r;
@@ -144,7 +166,7 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
     float qq = kqi * qj;
     float factor = inv_r2 * fmaf(qq, ci, lj);
     bool on = r2 <= rc2;
-    if (XK) on = on & ((jkey - ikey + span) > 2u * span);
+    if (XK) on = on & ((jk[t] - ikey + span) > 2u * span);
     factor = on ? factor : 0.f;
     float nfactor = -factor;
     efx = fmaf(factor, dx, efx); efy = fmaf(factor, dy, efy); efz = fmaf(factor, dz, efz);
@@ -156,9 +178,6 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
       ee = fmaf(0.5f, energy, ee);
       je = fmaf(0.5f, energy, je);
     }
-    pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
-    if (XK) jkey = hd_rot1_u(jkey);
-    if (NT > 1) jtype = hd_rot1_u(jtype);
     jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
     if (EV == 1) je = hd_rot1(je);
   }
@@ -177,7 +196,7 @@ __device__ __forceinline__ void pair_entries(
     const unsigned* __restrict__ tile_key, const unsigned* __restrict__ ents, unsigned nent,
     unsigned n, unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
     long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w,
-    const unsigned* keymap) {
+    const unsigned* keymap, PairJStage* stage) {
   const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
   const float alpha = P.alpha, coul_rc2 = P.coul_rc2;
   const float kqi = P.kc * qi;
@@ -205,14 +224,22 @@ __device__ __forceinline__ void pair_entries(
       // a j-atom can be excluded only when its key is in the block's map
       unsigned word = keymap[(jkey >> 5) & (PAIR_KEYMAP_WORDS - 1u)];
       bool hit = jvalid && ((word >> (jkey & 31u)) & 1u);
+      PairJStage& S = stage[w];
+      float4 rec = make_float4(pjx, pjy, pjz, qj);
+      S.pq[lane] = rec; S.pq[lane + 64] = rec;
+      S.type[lane] = jtype; S.type[lane + 64] = jtype;
+      S.key[lane] = jkey; S.key[lane + 64] = jkey;
+      wave_lds_sync();
       if (P.excl_span > PAIR_KEYMAP_SPAN || __any(hit))
         uc_steps<EV, NT, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                               two_a_sqrtpi, pjx, pjy, pjz, qj, jkey, jtype, efx, efy, efz,
-                               jfx, jfy, jfz, ee, je);
+                               two_a_sqrtpi, &S.pq[lane], &S.type[lane], &S.key[lane], efx, efy,
+                               efz, jfx, jfy, jfz, ee, je);
       else
         uc_steps<EV, NT, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                                two_a_sqrtpi, pjx, pjy, pjz, qj, jkey, jtype, efx, efy, efz,
-                                jfx, jfy, jfz, ee, je);
+                                two_a_sqrtpi, &S.pq[lane], &S.type[lane], &S.key[lane], efx,
+                                efy, efz, jfx, jfy, jfz, ee, je);
+      // every lane's reads of this entry precede the next entry's writes
+      wave_lds_sync();
     } else {
 #pragma unroll 4
     for (int t = 0; t < 64; ++t) {
@@ -321,6 +348,7 @@ __global__ void __launch_bounds__(256) k_pair(
     unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
   __shared__ long long red[PAIR_WAVES][5][64];
   __shared__ unsigned keymap[PAIR_KEYMAP_WORDS];
+  __shared__ PairJStage stage[PAIR_WAVES];
   unsigned b = blockIdx.x;
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HdLattice L = hd_load_lattice(lattice);
@@ -358,16 +386,16 @@ __global__ void __launch_bounds__(256) k_pair(
       __syncthreads();
       pair_entries<EV, NT, SW, true, true>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                            tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                           fpj_ew, ax, ay, az, ae, aw, keymap);
+                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage);
     }
   } else if (spc)
     pair_entries<EV, NT, SW, true, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                           tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                          fpj_ew, ax, ay, az, ae, aw, keymap);
+                                          fpj_ew, ax, ay, az, ae, aw, keymap, stage);
   else
     pair_entries<EV, NT, SW, false, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                            tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                           fpj_ew, ax, ay, az, ae, aw, keymap);
+                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage);
   red[w][0][lane] = ax;
   red[w][1][lane] = ay;
   red[w][2][lane] = az;
