@@ -63,9 +63,13 @@ struct PairParams {
 
 // One wave's copy of the current entry's j-atoms for the regrouped loop,
 // written twice over so that step t reads record lane + t with no wrap:
-// positions (block-centre relative) and charges, types, keys.
+// positions (block-centre relative) and charges; the LJ force coefficients
+// (12 c12, 6 c6) of the j-atom against each i type, so a lane reads its own
+// type's row with no per-pair select; types (energy steps) and keys.
+template <int NT>
 struct PairJStage {
   float4 pq[128];
+  float2 lj[NT][128];
   unsigned type[128];
   unsigned key[128];
 };
@@ -141,20 +145,21 @@ template <int EV, int NT, bool XK>
 __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float piy, float piz,
                                          unsigned ikey, unsigned span, float kqi, float rc2,
                                          float cg, float pa, float k2, const float4* jb,
-                                         const unsigned* jt, const unsigned* jk, float& efx,
+                                         const float2* jl, const unsigned* jt,
+                                         const unsigned* jk, float& efx,
                                          float& efy, float& efz, float& jfx, float& jfy,
                                          float& jfz, float& ee, float& je) {
 #pragma unroll 8
   for (int t = 0; t < 64; ++t) {
     float4 pq = jb[t];
-    unsigned jtype = (NT > 1) ? jt[t] : 0u;
+    float2 ljc = (NT > 1) ? jl[t] : make_float2(R.f12[0], R.f6[0]);
     float qj = pq.w;
     float dx = pix - pq.x, dy = piy - pq.y, dz = piz - pq.z;
     float r2 = dx * dx + dy * dy + dz * dz;
     float inv_r = __builtin_amdgcn_rsqf(r2);
     float inv_r2 = inv_r * inv_r;
     float ir6 = inv_r2 * inv_r2 * inv_r2;
-    float lj = ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+    float lj = ir6 * (ljc.x * ir6 - ljc.y);
     float r = r2 * inv_r;
     float gauss = __builtin_amdgcn_exp2f(cg * r2);
     float tt = __builtin_amdgcn_rcpf(fmaf(pa, r, 1.0f));
@@ -173,6 +178,7 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
     jfx = fmaf(nfactor, dx, jfx); jfy = fmaf(nfactor, dy, jfy); jfz = fmaf(nfactor, dz, jfz);
     if (EV == 1) {
       // LJ energy (lj.rs:237-247) and qq erfc(alpha r) / r (spme.rs:744-770)
+      unsigned jtype = (NT > 1) ? jt[t] : 0u;
       float elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
       float energy = on ? fmaf(qq * gauss, poly * inv_r, elj) : 0.f;
       ee = fmaf(0.5f, energy, ee);
@@ -196,7 +202,7 @@ __device__ __forceinline__ void pair_entries(
     const unsigned* __restrict__ tile_key, const unsigned* __restrict__ ents, unsigned nent,
     unsigned n, unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
     long long& ai_fx, long long& ai_fy, long long& ai_fz, long long& ai_e, long long& ai_w,
-    const unsigned* keymap, PairJStage* stage) {
+    const unsigned* keymap, PairJStage<NT>* stage, unsigned itype) {
   const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
   const float alpha = P.alpha, coul_rc2 = P.coul_rc2;
   const float kqi = P.kc * qi;
@@ -224,19 +230,35 @@ __device__ __forceinline__ void pair_entries(
       // a j-atom can be excluded only when its key is in the block's map
       unsigned word = keymap[(jkey >> 5) & (PAIR_KEYMAP_WORDS - 1u)];
       bool hit = jvalid && ((word >> (jkey & 31u)) & 1u);
-      PairJStage& S = stage[w];
+      PairJStage<NT>& S = stage[w];
       float4 rec = make_float4(pjx, pjy, pjz, qj);
       S.pq[lane] = rec; S.pq[lane + 64] = rec;
-      S.type[lane] = jtype; S.type[lane + 64] = jtype;
+      if (NT > 1) {
+        // the same products load_row forms for an i-atom of type ti
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) {
+          float f12[NT], f6[NT];
+#pragma unroll
+          for (int u = 0; u < NT; ++u) {
+            int k = (ti < P.ntypes && u < P.ntypes) ? ti * P.ntypes + u : 0;
+            f12[u] = 12.0f * P.c12[k];
+            f6[u] = 6.0f * P.c6[k];
+          }
+          float2 c = make_float2(pick<NT>(f12, jtype), pick<NT>(f6, jtype));
+          S.lj[ti][lane] = c; S.lj[ti][lane + 64] = c;
+        }
+      }
+      if (EV == 1) { S.type[lane] = jtype; S.type[lane + 64] = jtype; }
       S.key[lane] = jkey; S.key[lane + 64] = jkey;
       wave_lds_sync();
+      const float2* jl = &S.lj[(NT > 1) ? itype : 0u][lane];
       if (P.excl_span > PAIR_KEYMAP_SPAN || __any(hit))
         uc_steps<EV, NT, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                               two_a_sqrtpi, &S.pq[lane], &S.type[lane], &S.key[lane], efx, efy,
-                               efz, jfx, jfy, jfz, ee, je);
+                               two_a_sqrtpi, &S.pq[lane], jl, &S.type[lane], &S.key[lane], efx,
+                               efy, efz, jfx, jfy, jfz, ee, je);
       else
         uc_steps<EV, NT, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                                two_a_sqrtpi, &S.pq[lane], &S.type[lane], &S.key[lane], efx,
+                                two_a_sqrtpi, &S.pq[lane], jl, &S.type[lane], &S.key[lane], efx,
                                 efy, efz, jfx, jfy, jfz, ee, je);
       // every lane's reads of this entry precede the next entry's writes
       wave_lds_sync();
@@ -348,7 +370,7 @@ __global__ void __launch_bounds__(256) k_pair(
     unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
   __shared__ long long red[PAIR_WAVES][5][64];
   __shared__ unsigned keymap[PAIR_KEYMAP_WORDS];
-  __shared__ PairJStage stage[PAIR_WAVES];
+  __shared__ PairJStage<NT> stage[PAIR_WAVES];
   unsigned b = blockIdx.x;
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HdLattice L = hd_load_lattice(lattice);
@@ -386,16 +408,16 @@ __global__ void __launch_bounds__(256) k_pair(
       __syncthreads();
       pair_entries<EV, NT, SW, true, true>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                            tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage);
+                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage, itype);
     }
   } else if (spc)
     pair_entries<EV, NT, SW, true, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                           tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                          fpj_ew, ax, ay, az, ae, aw, keymap, stage);
+                                          fpj_ew, ax, ay, az, ae, aw, keymap, stage, itype);
   else
     pair_entries<EV, NT, SW, false, false>(P, L, b, w, lane, pix, piy, piz, qi, ikey, R, cb,
                                            tile_posq, tile_type, tile_key, ents, nent, n, fpj,
-                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage);
+                                           fpj_ew, ax, ay, az, ae, aw, keymap, stage, itype);
   red[w][0][lane] = ax;
   red[w][1][lane] = ay;
   red[w][2][lane] = az;
