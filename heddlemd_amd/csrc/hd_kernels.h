@@ -30,6 +30,7 @@ struct HdNeighborArgs {
   int atom_lists;
   unsigned excl_span;
   unsigned long long* block_mask;  // nblocks x hd_block_pair_chunks(nblocks) candidate masks
+  unsigned* block_order;           // blocks by entry count, heaviest first (rebuild steps)
 };
 unsigned hd_block_pair_chunks(unsigned nblocks);
 
@@ -55,6 +56,7 @@ struct HdPairArgs {
   long long *fpi, *fpi_ew, *fpj, *fpj_ew;
   int atom_lists;  // per-atom full lists (no j side) instead of the packed cluster list
   int ortho;       // no box tilt
+  const unsigned* block_order;  // workgroup -> block for the packed-list kernel
 };
 // ev: 0 forces only, 1 + per-atom energy and virial, 2 + per-atom virial
 // only (no pair energy is accumulated)

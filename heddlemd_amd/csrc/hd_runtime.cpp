@@ -117,6 +117,7 @@ struct hd_ctx {
   float4 *tile_posq = nullptr, *center = nullptr, *half = nullptr;
   unsigned *entry_atoms = nullptr, *entry_count = nullptr;
   unsigned long long* block_mask = nullptr;  // block-pair candidate masks
+  unsigned* block_order = nullptr;           // pair workgroup -> block
   unsigned* ctrl = nullptr;  // [0] status, [1] rebuild count, [2] max entries
   long long *fpi = nullptr, *fpi_ew = nullptr, *fpj = nullptr, *fpj_ew = nullptr;
   // device: SPME
@@ -346,6 +347,7 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   a.atom_lists = c->atom_lists;
   a.excl_span = c->excl_span;
   a.block_mask = c->block_mask;
+  a.block_order = c->block_order;
   return a;
 }
 
@@ -437,6 +439,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
   pa.half = c->half;
   pa.entry_atoms = c->entry_atoms;
   pa.entry_count = c->entry_count;
+  pa.block_order = c->block_order;
   pa.fpi = c->fpi;
   pa.fpi_ew = c->fpi_ew;
   pa.fpj = c->fpj;
@@ -1635,6 +1638,15 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->center, c->nblocks); DALLOC(c, c->half, c->nblocks);
   DALLOC(c, c->entry_count, c->nblocks);
   DALLOC(c, c->block_mask, (size_t)c->nblocks * hd_block_pair_chunks(c->nblocks));
+  DALLOC(c, c->block_order, c->nblocks);
+  {
+    // identity until the first rebuild orders the blocks
+    std::vector<unsigned> ident(c->nblocks);
+    for (unsigned b = 0; b < c->nblocks; ++b) ident[b] = b;
+    if (hipMemcpy(c->block_order, ident.data(), ident.size() * sizeof(unsigned),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, HD_ERR_HIP, "block order upload failed");
+  }
   DALLOC(c, c->ctrl, 4);
   DALLOC(c, c->fpi, (size_t)c->npad * 4); DALLOC(c, c->fpj, (size_t)c->npad * 4);
   DALLOC(c, c->fpi_ew, (size_t)c->npad * 2); DALLOC(c, c->fpj_ew, (size_t)c->npad * 2);
@@ -2065,6 +2077,7 @@ int hd_count_pairs(hd_ctx* c, uint64_t* out) {
   pa.excl_span = c->excl_span;
   pa.entry_atoms = c->entry_atoms;
   pa.entry_count = c->entry_count;
+  pa.block_order = c->block_order;
   pa.atom_lists = c->atom_lists;
   pa.ortho = box_ortho(c) ? 1 : 0;
   if (e == hipSuccess) {

@@ -632,9 +632,40 @@ __global__ void k_rebuild_finish(const float4* __restrict__ posq, float4* __rest
     ref[i] = posq[i];
 }
 
-__global__ void k_rebuild_clear(unsigned* status, unsigned* rebuild_count) {
-  unsigned s = *status;
-  if (s & (NB_STATUS_REBUILD | NB_STATUS_FORCE)) {
+// Clears the rebuild request and, on rebuild steps, orders the blocks by
+// entry count, heaviest first, as the pair kernel's workgroup order: block
+// costs vary about twofold, and a heavy block dispatched last runs alone at
+// the end of the launch.  One workgroup: a histogram of the counts, suffix
+// sums, a scatter (the order among equal counts is arbitrary; the forces do
+// not depend on it, their sums being integer).
+#define NB_ORDER_BINS 1024u
+__global__ void __launch_bounds__(NB_ORDER_BINS) k_rebuild_clear(
+    unsigned* status, unsigned* rebuild_count, const unsigned* __restrict__ entry_count,
+    unsigned nblocks, unsigned* __restrict__ order) {
+  __shared__ unsigned hist[NB_ORDER_BINS], cur[NB_ORDER_BINS];
+  unsigned s = *status;  // read by every thread before thread 0 rewrites it
+  if (!(s & (NB_STATUS_REBUILD | NB_STATUS_FORCE))) return;
+  unsigned t = threadIdx.x;
+  hist[t] = 0u;
+  __syncthreads();
+  for (unsigned i = t; i < nblocks; i += NB_ORDER_BINS)
+    atomicAdd(&hist[min(entry_count[i], NB_ORDER_BINS - 1u)], 1u);
+  __syncthreads();
+  // hist[c] <- number of blocks whose count is >= c
+  for (unsigned d = 1; d < NB_ORDER_BINS; d <<= 1) {
+    unsigned v = (t + d < NB_ORDER_BINS) ? hist[t + d] : 0u;
+    __syncthreads();
+    hist[t] += v;
+    __syncthreads();
+  }
+  // the blocks with count c take positions [hist[c + 1], hist[c])
+  cur[t] = (t + 1 < NB_ORDER_BINS) ? hist[t + 1] : 0u;
+  __syncthreads();
+  for (unsigned i = t; i < nblocks; i += NB_ORDER_BINS) {
+    unsigned pos = atomicAdd(&cur[min(entry_count[i], NB_ORDER_BINS - 1u)], 1u);
+    order[pos] = i;
+  }
+  if (t == 0) {
     *status = s & ~(NB_STATUS_REBUILD | NB_STATUS_FORCE);
     *rebuild_count += 1u;
   }
@@ -708,5 +739,6 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
   if (gridN == 0) gridN = 1;
   hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,
                      a.lattice, a.ref_lattice, a.status, a.n);
-  hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(1), 0, st, a.status, a.rebuild_count);
+  hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(NB_ORDER_BINS), 0, st, a.status,
+                     a.rebuild_count, a.entry_count, nblocks, a.block_order);
 }

@@ -366,12 +366,13 @@ __global__ void __launch_bounds__(256) k_pair(
     const unsigned* __restrict__ tile_type, const unsigned* __restrict__ tile_key,
     const float4* __restrict__ center, const float4* __restrict__ half,
     const unsigned* __restrict__ entry_atoms, const unsigned* __restrict__ entry_count,
-    unsigned cap, unsigned n, long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
+    const unsigned* __restrict__ block_order, unsigned cap, unsigned n,
+    long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
     unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
   __shared__ long long red[PAIR_WAVES][5][64];
   __shared__ unsigned keymap[PAIR_KEYMAP_WORDS];
   __shared__ PairJStage<NT> stage[PAIR_WAVES];
-  unsigned b = blockIdx.x;
+  unsigned b = block_order[blockIdx.x];  // heaviest blocks first
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HdLattice L = hd_load_lattice(lattice);
   unsigned islot = b * 64 + lane;
@@ -763,7 +764,7 @@ static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams&
   }
   hipLaunchKernelGGL((k_pair<EV, NT, SW>), dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P,
                      a.lattice, a.tile_posq, a.tile_type, a.tile_key, a.center, a.half,
-                     a.entry_atoms, a.entry_count, a.cap, a.n, a.fpi, a.fpi_ew,
+                     a.entry_atoms, a.entry_count, a.block_order, a.cap, a.n, a.fpi, a.fpi_ew,
                      (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
 }
 
