@@ -123,6 +123,24 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
   return r;
 }
 
+// The j-atom's offset from the block centre in the image nearest it, as
+// hd_rel_to_center computes it (FP64 difference less a whole number of box
+// lengths, rounded once), with the image count taken from an f32 product
+// with the reciprocal box length instead of an FP64 division.  On the
+// shifted path every staged j-atom lies within the search radius plus the
+// block half-extent of the centre, at least the skin away from half a box
+// length, so the f32 count is the FP64 one.
+__device__ __forceinline__ void pair_rel_to_center(float x, float y, float z, float4 cb,
+                                                   const HdLattice& L, float ilx, float ily,
+                                                   float ilz, float& rx, float& ry, float& rz) {
+  double nx = (double)rintf((x - cb.x) * ilx);
+  double ny = (double)rintf((y - cb.y) * ily);
+  double nz = (double)rintf((z - cb.z) * ilz);
+  rx = (float)(((double)x - (double)cb.x) - (double)L.lx * nx);
+  ry = (float)(((double)y - (double)cb.y) - (double)L.ly * ny);
+  rz = (float)(((double)z - (double)cb.z) - (double)L.lz * nz);
+}
+
 // Far-away coordinates for padding lanes on the shifted path: any pair
 // with one of them fails both cutoffs, and |d|^2 stays finite (< 1e37).
 #define PAIR_FAR_I 1.0e18f
@@ -217,7 +235,7 @@ __device__ __forceinline__ void pair_entries(
     bool jin = jvalid && (jslot >> 6) == b;
     float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = jvalid ? pj.w : 0.f;
     if (SPC) {
-      hd_rel_to_center(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, pjx, pjy, pjz);
+      pair_rel_to_center(pj.x, pj.y, pj.z, cb, L, ilx, ily, ilz, pjx, pjy, pjz);
       if (!jvalid) pjx = pjy = pjz = PAIR_FAR_J;
     } else {
       // validity travels in the key: padding lanes can never pass the
