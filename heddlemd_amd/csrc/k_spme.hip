@@ -93,7 +93,19 @@ __global__ void k_spline_setup(const float4* __restrict__ tile_posq,
 // that the threads share them evenly.
 #define SPREAD_THREADS 256
 #define SPREAD_BRICK_MAX 2048  // cells per brick (16 KiB of i64)
-#define SPREAD_WIN_MAX 3072    // voxels per window held in LDS
+#define SPREAD_WIN_MAX 3072    // voxels per window listed through LDS
+#define SPREAD_PER_MAX (SPREAD_WIN_MAX / SPREAD_THREADS)
+#define SPREAD_ATOM_MAX 4096   // window atoms listed in LDS
+
+// round(x) as i64 for an integer-valued or rounded f32 x, the value
+// __float2ll_rn gives: one conversion through i32 when |x| < 2^31 (every
+// contribution below 0.5 e at the 2^32 scale), the general conversion
+// otherwise.
+__device__ __forceinline__ long long spread_fixed(float x) {
+  float r = rintf(x);
+  if (fabsf(r) < 2147483648.0f) return (long long)(int)r;
+  return __float2ll_rn(r);
+}
 
 __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
                                             const unsigned* __restrict__ sg, unsigned s,
@@ -101,6 +113,7 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
                                             int c0b, int c0c, unsigned ba, unsigned bb,
                                             unsigned bc, unsigned long long* acc) {
   unsigned g = sg[s];
+  float4 r = spl[s];  // loaded with the origin, not after its test
   if (g == SPME_NO_CELL) return;
   int ga = (int)(g & 1023u), gb = (int)((g >> 10) & 1023u), gc = (int)(g >> 20);
   // brick-relative index of the stencil's first cell per axis; cell
@@ -112,7 +125,6 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   if (rla >= (int)ba + SPME_P - 1 || rlb >= (int)bb + SPME_P - 1 ||
       rlc >= (int)bc + SPME_P - 1)
     return;
-  float4 r = spl[s];
   float wa[SPME_P], wb[SPME_P], wc[SPME_P];
 #pragma unroll
   for (int d = 0; d < SPME_P; ++d) {
@@ -124,7 +136,9 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   for (int iz = 0; iz < SPME_P; ++iz) {
     int oc = rlc - iz;
     if ((unsigned)oc >= bc) continue;
-    float dz = r.w * wc[iz];
+    // the 2^32 scale is a power of two: applying it to q w_c first gives
+    // the same products as applying it last
+    float dz = (r.w * wc[iz]) * 4294967296.0f;
 #pragma unroll
     for (int da = 0; da < SPME_P; ++da) {
       int oa = rla - da;
@@ -134,8 +148,7 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
       for (int db = 0; db < SPME_P; ++db) {
         int ob = rlb - db;
         if ((unsigned)ob >= bb) continue;
-        float v = dzda * wb[db];
-        long long vf = __float2ll_rn(v * 4294967296.0f);
+        long long vf = spread_fixed(dzda * wb[db]);
         if (vf != 0)
           atomicAdd(&acc[((unsigned)oa * bb + (unsigned)ob) * bc + (unsigned)oc],
                     (unsigned long long)vf);
@@ -150,8 +163,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
     const unsigned* __restrict__ voxel_off, unsigned nva, unsigned nvb, unsigned nvc, int mva,
     int mvb, int mvc, unsigned long long* __restrict__ rho) {
   __shared__ unsigned long long acc[SPREAD_BRICK_MAX];
-  __shared__ unsigned wstart[SPREAD_WIN_MAX];
-  __shared__ unsigned wpre[SPREAD_WIN_MAX];
+  __shared__ unsigned wslot[SPREAD_ATOM_MAX];
   __shared__ unsigned part[SPREAD_THREADS];
   unsigned t = threadIdx.x;
   unsigned nbb = nb / bb, nbc = nc / bc;
@@ -174,61 +186,65 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
   bool flat = nwin <= SPREAD_WIN_MAX;  // uniform over the workgroup
   __syncthreads();  // acc cleared before any accumulation
   unsigned per = (nwin + SPREAD_THREADS - 1) / SPREAD_THREADS;
+  // thread t owns the contiguous window range [t*per, t*per + per): the
+  // slot range of each of its voxels, kept in registers
+  unsigned vs0[SPREAD_PER_MAX], vcnt[SPREAD_PER_MAX];
   unsigned my = 0;
-  for (unsigned j = 0; j < per; ++j) {
-    // thread t owns the contiguous window range [t*per, t*per + per)
+#pragma unroll
+  for (unsigned j = 0; j < SPREAD_PER_MAX; ++j) {
     unsigned k = t * per + j;
-    if (k >= nwin) break;
-    unsigned kc = k % wc_n, kab = k / wc_n;
-    unsigned kb = kab % wb_n, ka = kab / wb_n;
-    // periodic wrap; the voxel grid sides are powers of two
-    unsigned va = (unsigned)((vla + (int)ka) & ((int)nva - 1));
-    unsigned vb = (unsigned)((vlb + (int)kb) & ((int)nvb - 1));
-    unsigned vc = (unsigned)((vlc + (int)kc) & ((int)nvc - 1));
-    unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
-    unsigned s0 = voxel_off[r], s1 = voxel_off[r + 1];
-    if (flat) {
-      wstart[k] = s0;
-      wpre[k] = s1 - s0;  // count for now, prefix after the scan
-    } else {
-      for (unsigned s = s0; s < s1; ++s)
-        spread_atom(spl, sg, s, na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
+    vs0[j] = 0u;
+    vcnt[j] = 0u;
+    if (flat && j < per && k < nwin) {
+      unsigned kc = k % wc_n, kab = k / wc_n;
+      unsigned kb = kab % wb_n, ka = kab / wb_n;
+      // periodic wrap; the voxel grid sides are powers of two
+      unsigned va = (unsigned)((vla + (int)ka) & ((int)nva - 1));
+      unsigned vb = (unsigned)((vlb + (int)kb) & ((int)nvb - 1));
+      unsigned vc = (unsigned)((vlc + (int)kc) & ((int)nvc - 1));
+      unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
+      vs0[j] = voxel_off[r];
+      vcnt[j] = voxel_off[r + 1] - vs0[j];
+      my += vcnt[j];
     }
-    my += s1 - s0;
   }
+  // exclusive scan of the per-thread totals
+  part[t] = my;
   __syncthreads();
-  if (flat) {
-    // exclusive scan of the per-thread totals, then of each thread's range
-    part[t] = my;
+  for (unsigned off = 1; off < SPREAD_THREADS; off <<= 1) {
+    unsigned u = (t >= off) ? part[t - off] : 0u;
     __syncthreads();
-    for (unsigned off = 1; off < SPREAD_THREADS; off <<= 1) {
-      unsigned u = (t >= off) ? part[t - off] : 0u;
-      __syncthreads();
-      part[t] += u;
-      __syncthreads();
-    }
-    unsigned total = part[SPREAD_THREADS - 1];
+    part[t] += u;
+    __syncthreads();
+  }
+  unsigned total = part[SPREAD_THREADS - 1];
+  bool listed = flat && total <= SPREAD_ATOM_MAX;  // uniform over the workgroup
+  if (listed) {
+    // the window's atoms in window order, then shared evenly
     unsigned run = part[t] - my;
+#pragma unroll
+    for (unsigned j = 0; j < SPREAD_PER_MAX; ++j)
+      for (unsigned i = 0; i < vcnt[j]; ++i) wslot[run++] = vs0[j] + i;
+    __syncthreads();
+    for (unsigned f = t; f < total; f += SPREAD_THREADS)
+      spread_atom(spl, sg, wslot[f], na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
+  } else {
+    // a window too wide to list: every thread walks its own voxels
     for (unsigned j = 0; j < per; ++j) {
       unsigned k = t * per + j;
       if (k >= nwin) break;
-      unsigned c = wpre[k];
-      wpre[k] = run;
-      run += c;
+      unsigned kc = k % wc_n, kab = k / wc_n;
+      unsigned kb = kab % wb_n, ka = kab / wb_n;
+      unsigned va = (unsigned)((vla + (int)ka) & ((int)nva - 1));
+      unsigned vb = (unsigned)((vlb + (int)kb) & ((int)nvb - 1));
+      unsigned vc = (unsigned)((vlc + (int)kc) & ((int)nvc - 1));
+      unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
+      unsigned s0 = voxel_off[r], s1 = voxel_off[r + 1];
+      for (unsigned sl = s0; sl < s1; ++sl)
+        spread_atom(spl, sg, sl, na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
     }
-    __syncthreads();
-    for (unsigned f = t; f < total; f += SPREAD_THREADS) {
-      // last window entry whose prefix is <= f
-      unsigned lo = 0, hi = nwin;
-      while (hi - lo > 1) {
-        unsigned mid = (lo + hi) >> 1;
-        if (wpre[mid] <= f) lo = mid; else hi = mid;
-      }
-      spread_atom(spl, sg, wstart[lo] + (f - wpre[lo]), na, nb, nc, c0a, c0b, c0c, ba, bb, bc,
-                  acc);
-    }
-    __syncthreads();
   }
+  __syncthreads();
   for (unsigned k = t; k < vol; k += SPREAD_THREADS) {
     unsigned kc = k % bc, kab = k / bc;
     unsigned kb = kab % bb, ka = kab / bb;
