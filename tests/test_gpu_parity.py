@@ -571,6 +571,39 @@ def test_batched_replicas_byte_identical_to_solo_runs():
         e.close()
 
 
+def test_charged_chains_real_space_key_map(oracle):
+    # chains with alternating charges under SPME: one cutoff for LJ and
+    # Coulomb sends every block through the regrouped pair loop, and the
+    # 1-2 / 1-3 exclusions (key span 2) through its per-block key map, both
+    # in entries that hold excludable atoms and in entries that skip the
+    # key test
+    import dataclasses
+    say("case: charged polymer 40 x 50, SPME 32^3, key span 2")
+    s = systems.polymer_melt(n_chains=40, n_beads=50)
+    posq = s.posq.copy()
+    posq[:, 3] = np.where(np.arange(s.n) % 2 == 0, 0.25, -0.25).astype(np.float32)
+    rc = float(s.lj.cutoff[0, 0])
+    spme = systems.SpmeParams(alpha=systems.ewald_alpha(rc, 1e-5), r_cut=rc, grid=(32, 32, 32),
+                              order=4)
+    s = dataclasses.replace(s, posq=posq, spme=spme)
+    e = engine_for(s)
+    e.force_eval(scalars=True)
+    f = e.forces()
+    sc = e.scalars()
+    e.close()
+    Fp, Ep, Wp, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice,
+                                          coul=dict(alpha=spme.alpha, r_cut=spme.r_cut),
+                                          excl=s.exclusion_csr(), band=1e-6)
+    Fr, Er, Wr, er, wr = oracle.spme_recip(s.posq, s.lattice, spme.grid, 4, spme.alpha)
+    Fa, Ea, Wa = oracle.angle_forces(s.posq, s.lattice, *s.angles)
+    rms, mx = _force_errors(f, Fp + Fr + Fa, band)
+    say(f"  forces rms {rms:.3e} max {mx:.3e}")
+    assert rms <= 5e-6 and mx <= 1e-4
+    assert sc["pe"] == pytest.approx((Ep + Er + Ea).sum(), rel=1e-5)
+    assert sc["virial"] == pytest.approx((Wp + Wr + Wa).sum(), rel=1e-5,
+                                         abs=1e-6 * np.abs(Wp).sum())
+
+
 def test_polymer_melt_forces_and_shake_dynamics(oracle):
     # united-atom melt: LJ with 1-2 / 1-3 exclusions (key span), harmonic
     # angles folded into the fixed-point accumulators, then NHC dynamics
