@@ -371,6 +371,25 @@ __device__ __forceinline__ unsigned nb_collect(const unsigned long long* __restr
 // time.  The entries are therefore the hit j-atoms in (j-block, slot) order:
 // a pure function of the positions.
 #define NB_WAVES 4
+#define NB_CHUNK 4  // candidate blocks whose loads are in flight together
+
+// A j-atom's offset from the i-block centre in the image nearest it, as
+// hd_rel_to_center computes it (FP64 difference less whole box lengths,
+// rounded once), with the image count from an f32 product with the
+// reciprocal box length.  The counts can differ only for an offset within
+// an ulp of half a box length, and such an atom is rejected by the box
+// test in either image (the block lies at least r_search inside the half
+// box on this path).
+__device__ __forceinline__ void nb_rel_to_center(float x, float y, float z, float4 cb,
+                                                 const HdLattice& L, float ilx, float ily,
+                                                 float ilz, float& rx, float& ry, float& rz) {
+  double nx = (double)rintf((x - cb.x) * ilx);
+  double ny = (double)rintf((y - cb.y) * ily);
+  double nz = (double)rintf((z - cb.z) * ilz);
+  rx = (float)(((double)x - (double)cb.x) - (double)L.lx * nx);
+  ry = (float)(((double)y - (double)cb.y) - (double)L.ly * ny);
+  rz = (float)(((double)z - (double)cb.z) - (double)L.lz * nz);
+}
 
 // one pass of the 64 x 64 test over the queued j-atoms (lane = queue
 // position); returns whether this lane's j-atom hits any i-atom
@@ -439,41 +458,11 @@ __global__ void __launch_bounds__(256) k_build_entries(
   bool too_many = false;
   unsigned ncand = nb_collect(mask, b, nchunks, lane, cand[w], &too_many);
   __builtin_amdgcn_wave_barrier();
-  const double ilx = 1.0 / (double)L.lx, ily = 1.0 / (double)L.ly, ilz = 1.0 / (double)L.lz;
+  const float filx = 1.0f / L.lx, fily = 1.0f / L.ly, filz = 1.0f / L.lz;
   unsigned nbuf = 0, nent = 0, nq = 0;
   unsigned* out = entry_atoms + (size_t)b * cap * 64;
-  float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (ncand) pn = tile_posq[cand[w][0] * 64 + lane];
-  for (unsigned c = 0; c <= ncand; ++c) {
-    bool last = c == ncand;
-    if (!last) {
-      unsigned jb = cand[w][c];
-      unsigned jslot = jb * 64 + lane;
-      bool jvalid = jslot < n;
-      float4 pj = pn;
-      if (c + 1 < ncand) pn = tile_posq[cand[w][c + 1] * 64 + lane];
-      // distance from the j-atom to the i-block's box
-      float dx, dy, dz;
-      if (rel) {
-        hd_rel_to_center_rcp(pj.x, pj.y, pj.z, cb.x, cb.y, cb.z, L, ilx, ily, ilz, dx, dy, dz);
-        pj.x = dx; pj.y = dy; pj.z = dz;
-      } else {
-        dx = pj.x - cb.x; dy = pj.y - cb.y; dz = pj.z - cb.z;
-        hd_min_image(dx, dy, dz, L);
-      }
-      float ex = fmaxf(fabsf(dx) - hb.x, 0.f), ey = fmaxf(fabsf(dy) - hb.y, 0.f),
-            ez = fmaxf(fabsf(dz) - hb.z, 0.f);
-      bool near = jvalid && (ex * ex + ey * ey + ez * ez <= rs2);
-      unsigned long long nmask = __ballot(near);
-      if (near) {
-        unsigned pos = nq + __popcll(nmask & hd_lane_mask_lt());
-        qpos[w][pos] = pj;
-        qslot[w][pos] = jslot;
-      }
-      nq += __popcll(nmask);
-      __builtin_amdgcn_wave_barrier();
-    }
-    // test 64 queued atoms at a time (the remainder after the last block)
+  // test 64 queued atoms at a time (the remainder after the last block)
+  auto drain = [&](bool last) {
     while (nq >= 64 || (last && nq > 0)) {
       bool qvalid = lane < nq;
       float4 qp = qpos[w][lane];
@@ -481,11 +470,11 @@ __global__ void __launch_bounds__(256) k_build_entries(
       bool hit = nb_test_queued(ipos[w], qp, qs, qvalid, rel, b, ni, rs2, L);
       unsigned rem = nq > 64 ? nq - 64 : 0;
       float4 rp = qpos[w][64 + (lane < rem ? lane : 0)];
-      unsigned rs = qslot[w][64 + (lane < rem ? lane : 0)];
+      unsigned rsl = qslot[w][64 + (lane < rem ? lane : 0)];
       __builtin_amdgcn_wave_barrier();
       if (lane < rem) {
         qpos[w][lane] = rp;
-        qslot[w][lane] = rs;
+        qslot[w][lane] = rsl;
       }
       nq = rem;
       unsigned long long hmask = __ballot(hit);
@@ -506,7 +495,52 @@ __global__ void __launch_bounds__(256) k_build_entries(
         __builtin_amdgcn_wave_barrier();
       }
     }
+  };
+  // the candidate blocks' atoms are loaded NB_CHUNK blocks at a time, the
+  // next group while the current one is queued and tested
+  float4 pn[NB_CHUNK];
+#pragma unroll
+  for (unsigned u = 0; u < NB_CHUNK; ++u)
+    pn[u] = (u < ncand) ? tile_posq[cand[w][u] * 64 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (unsigned c0 = 0; c0 < ncand; c0 += NB_CHUNK) {
+    float4 pc[NB_CHUNK];
+#pragma unroll
+    for (unsigned u = 0; u < NB_CHUNK; ++u) {
+      pc[u] = pn[u];
+      unsigned cn = c0 + NB_CHUNK + u;
+      if (cn < ncand) pn[u] = tile_posq[cand[w][cn] * 64 + lane];
+    }
+#pragma unroll
+    for (unsigned u = 0; u < NB_CHUNK; ++u) {
+      if (c0 + u >= ncand) break;
+      unsigned jb = cand[w][c0 + u];
+      unsigned jslot = jb * 64 + lane;
+      bool jvalid = jslot < n;
+      float4 pj = pc[u];
+      // distance from the j-atom to the i-block's box
+      float dx, dy, dz;
+      if (rel) {
+        nb_rel_to_center(pj.x, pj.y, pj.z, cb, L, filx, fily, filz, dx, dy, dz);
+        pj.x = dx; pj.y = dy; pj.z = dz;
+      } else {
+        dx = pj.x - cb.x; dy = pj.y - cb.y; dz = pj.z - cb.z;
+        hd_min_image(dx, dy, dz, L);
+      }
+      float ex = fmaxf(fabsf(dx) - hb.x, 0.f), ey = fmaxf(fabsf(dy) - hb.y, 0.f),
+            ez = fmaxf(fabsf(dz) - hb.z, 0.f);
+      bool near = jvalid && (ex * ex + ey * ey + ez * ez <= rs2);
+      unsigned long long nmask = __ballot(near);
+      if (near) {
+        unsigned pos = nq + __popcll(nmask & hd_lane_mask_lt());
+        qpos[w][pos] = pj;
+        qslot[w][pos] = jslot;
+      }
+      nq += __popcll(nmask);
+      __builtin_amdgcn_wave_barrier();
+      drain(false);
+    }
   }
+  drain(true);
   if (nbuf > 0) {
     if (nent < cap) out[(size_t)nent * 64 + lane] = (lane < nbuf) ? buf[w][lane] : HD_SENTINEL;
     ++nent;
