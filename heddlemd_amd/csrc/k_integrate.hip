@@ -466,8 +466,55 @@ __device__ __forceinline__ void bspline_wd(float x, float& w, float& dw) {
 
 template <bool EV, bool GATHER, bool KICK, bool LOSSLESS>
 __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
-  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  // With the SPME gather four lanes share a slot: lane q of the quad reads
+  // the a-plane da = q of the 4x4x4 stencil (16 grid values), so four
+  // times as many reads are in flight; the quad's partial sums are combined
+  // and its first lane finishes the slot.
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned s = GATHER ? (t >> 2) : t;
+  unsigned q = GATHER ? (t & 3u) : 0u;
+  if (s >= a.n) return;  // whole quads
+  unsigned gcell = GATHER ? a.sg[s] : 0xFFFFFFFFu;
+  float phi = 0.f, gra = 0.f, grb = 0.f, grc = 0.f, qi = 0.f;
+  if (GATHER && gcell != 0xFFFFFFFFu) {
+    // spline record of this slot (k_spline_setup): (t_a, t_b, t_c, q) and
+    // the packed origin cell; uncharged atoms have none and no SPME terms
+    float4 rec = a.spl[s];
+    qi = rec.w;
+    int ga0 = (int)(gcell & 1023u), gb0 = (int)((gcell >> 10) & 1023u), gc0 = (int)(gcell >> 20);
+    float wa, dwa, wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
+    int gib[GATHER_P], gic[GATHER_P];
+    bspline_wd((float)q + rec.x, wa, dwa);
+#pragma unroll
+    for (int d = 0; d < GATHER_P; ++d) {
+      bspline_wd((float)d + rec.y, wb[d], dwb[d]);
+      bspline_wd((float)d + rec.z, wc[d], dwc[d]);
+      // periodic wrap; the grid sides are powers of two
+      gib[d] = (gb0 - d) & ((int)a.nb - 1);
+      gic[d] = (gc0 - d) & ((int)a.nc - 1);
+    }
+    int gia = (ga0 - (int)q) & ((int)a.na - 1);
+#pragma unroll
+    for (int db = 0; db < GATHER_P; ++db) {
+      const float* row = a.vgrid + ((size_t)gia * a.nb + gib[db]) * a.nc;
+#pragma unroll
+      for (int dc = 0; dc < GATHER_P; ++dc) {
+        float v = row[gic[dc]];
+        phi += v * wa * wb[db] * wc[dc];
+        gra += v * dwa * wb[db] * wc[dc];
+        grb += v * wa * dwb[db] * wc[dc];
+        grc += v * wa * wb[db] * dwc[dc];
+      }
+    }
+  }
+  if (GATHER) {
+    // planes 0 + 1 and 2 + 3, then the two halves
+    phi += __shfl_xor(phi, 1); gra += __shfl_xor(gra, 1);
+    grb += __shfl_xor(grb, 1); grc += __shfl_xor(grc, 1);
+    phi += __shfl_xor(phi, 2); gra += __shfl_xor(gra, 2);
+    grb += __shfl_xor(grb, 2); grc += __shfl_xor(grc, 2);
+    if (q != 0) return;
+  }
   unsigned id = a.sorted_id[s];
   const longlong2* fi = reinterpret_cast<const longlong2*>(a.fpi) + 2 * s;
   const longlong2* fj = reinterpret_cast<const longlong2*>(a.fpj) + 2 * s;
@@ -482,43 +529,8 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     pe = hd_from_fixed48(ie.x + je.x);
     vir = hd_from_fixed48(ie.y + je.y);
   }
-  unsigned gcell = GATHER ? a.sg[s] : 0xFFFFFFFFu;
   if (GATHER && gcell != 0xFFFFFFFFu) {
-    // spline record of this slot (k_spline_setup): (t_a, t_b, t_c, q) and
-    // the packed origin cell; uncharged atoms have none and no SPME terms
     HdLattice L = hd_load_lattice(a.lattice);
-    float4 rec = a.spl[s];
-    float qi = rec.w;
-    float ta = rec.x, tb = rec.y, tc = rec.z;
-    int ga0 = (int)(gcell & 1023u), gb0 = (int)((gcell >> 10) & 1023u), gc0 = (int)(gcell >> 20);
-    float wa[GATHER_P], dwa[GATHER_P], wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
-    int gia[GATHER_P], gib[GATHER_P], gic[GATHER_P];
-#pragma unroll
-    for (int d = 0; d < GATHER_P; ++d) {
-      bspline_wd((float)d + ta, wa[d], dwa[d]);
-      bspline_wd((float)d + tb, wb[d], dwb[d]);
-      bspline_wd((float)d + tc, wc[d], dwc[d]);
-      // periodic wrap; the grid sides are powers of two
-      gia[d] = (ga0 - d) & ((int)a.na - 1);
-      gib[d] = (gb0 - d) & ((int)a.nb - 1);
-      gic[d] = (gc0 - d) & ((int)a.nc - 1);
-    }
-    float phi = 0.f, gra = 0.f, grb = 0.f, grc = 0.f;
-#pragma unroll
-    for (int da = 0; da < GATHER_P; ++da) {
-#pragma unroll
-      for (int db = 0; db < GATHER_P; ++db) {
-        const float* row = a.vgrid + ((size_t)gia[da] * a.nb + gib[db]) * a.nc;
-#pragma unroll
-        for (int dc = 0; dc < GATHER_P; ++dc) {
-          float v = row[gic[dc]];
-          phi += v * wa[da] * wb[db] * wc[dc];
-          gra += v * dwa[da] * wb[db] * wc[dc];
-          grb += v * wa[da] * dwb[db] * wc[dc];
-          grc += v * wa[da] * wb[db] * dwc[dc];
-        }
-      }
-    }
     float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
     float bax = ilx, bay = -L.xy * ilx * ily, baz = (L.xy * L.yz - L.xz * L.ly) * ilx * ily * ilz;
     float bby = ily, bbz = -L.yz * ily * ilz, bcz = ilz;
@@ -570,8 +582,9 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
 
 void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev) {
   if (!a.n) return;
-  dim3 g((a.n + 255) / 256), b(256);
   bool ga = a.vgrid != nullptr, k = a.kick, ll = a.vxlo != nullptr;
+  // four lanes per slot with the SPME gather
+  dim3 g(((ga ? 4u : 1u) * a.n + 255) / 256), b(256);
 #define HD_FIN(E, G, K, LL) \
   if (ev == E && ga == G && k == K && ll == LL) { \
     hipLaunchKernelGGL((k_finalize<E, G, K, LL>), g, b, 0, st, a); \
