@@ -49,8 +49,11 @@ struct PairParams {
   float spc_margin;  // minimum (L/2 - half_extent) that enables the shifted path
   unsigned excl_span;  // |key_i - key_j| <= excl_span excludes the pair
   int ortho;           // the box has no tilt
-  // one cutoff for every interacting type pair and Coulomb, no switch
+  // 1: one cutoff for every interacting type pair and Coulomb, no switch;
+  // 2: LJ only (no Coulomb), one cutoff for every interacting type pair, no
+  // switch; 0: neither (the general loop)
   int uniform;
+  float uc_rc2;  // the shared cutoff^2 of modes 1 and 2
   float cg;  // -alpha^2 log2(e): exp(-(alpha r)^2) = exp2(cg r^2)
   float pa;  // 0.3275911 alpha, the Hastings t = 1 / (1 + pa r)
 };
@@ -159,7 +162,7 @@ __device__ __forceinline__ void pair_rel_to_center(float x, float y, float z, fl
 // for every EV, so energy steps move atoms exactly as force-only steps do.
 // XK = false drops the key test (no j-atom of the entry can be excluded
 // against the block).
-template <int EV, int NT, bool XK>
+template <int EV, int NT, bool XK, bool COUL>
 __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float piy, float piz,
                                          unsigned ikey, unsigned span, float kqi, float rc2,
                                          float cg, float pa, float k2, const float4* jb,
@@ -178,16 +181,21 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
     float inv_r2 = inv_r * inv_r;
     float ir6 = inv_r2 * inv_r2 * inv_r2;
     float lj = ir6 * (ljc.x * ir6 - ljc.y);
-    float r = r2 * inv_r;
-    float gauss = __builtin_amdgcn_exp2f(cg * r2);
-    float tt = __builtin_amdgcn_rcpf(fmaf(pa, r, 1.0f));
-    float poly =
-        (0.254829592f +
-         (-0.284496736f + (1.421413741f + (-1.453152027f + 1.061405429f * tt) * tt) * tt) * tt) *
-        tt;
-    float ci = gauss * fmaf(poly, inv_r, k2);
-    float qq = kqi * qj;
-    float factor = inv_r2 * fmaf(qq, ci, lj);
+    float qq = 0.f, gauss = 0.f, poly = 0.f, factor;
+    if (COUL) {
+      float r = r2 * inv_r;
+      gauss = __builtin_amdgcn_exp2f(cg * r2);
+      float tt = __builtin_amdgcn_rcpf(fmaf(pa, r, 1.0f));
+      poly =
+          (0.254829592f +
+           (-0.284496736f + (1.421413741f + (-1.453152027f + 1.061405429f * tt) * tt) * tt) * tt) *
+          tt;
+      float ci = gauss * fmaf(poly, inv_r, k2);
+      qq = kqi * qj;
+      factor = inv_r2 * fmaf(qq, ci, lj);
+    } else {
+      factor = inv_r2 * lj;
+    }
     bool on = r2 <= rc2;
     if (XK) on = on & ((jk[t] - ikey + span) > 2u * span);
     factor = on ? factor : 0.f;
@@ -198,7 +206,7 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
       // LJ energy (lj.rs:237-247) and qq erfc(alpha r) / r (spme.rs:744-770)
       unsigned jtype = (NT > 1) ? jt[t] : 0u;
       float elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
-      float energy = on ? fmaf(qq * gauss, poly * inv_r, elj) : 0.f;
+      float energy = on ? (COUL ? fmaf(qq * gauss, poly * inv_r, elj) : elj) : 0.f;
       ee = fmaf(0.5f, energy, ee);
       je = fmaf(0.5f, energy, je);
     }
@@ -270,14 +278,28 @@ __device__ __forceinline__ void pair_entries(
       S.key[lane] = jkey; S.key[lane + 64] = jkey;
       wave_lds_sync();
       const float2* jl = &S.lj[(NT > 1) ? itype : 0u][lane];
-      if (P.excl_span > PAIR_KEYMAP_SPAN || __any(hit))
-        uc_steps<EV, NT, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                               two_a_sqrtpi, &S.pq[lane], jl, &S.type[lane], &S.key[lane], efx,
-                               efy, efz, jfx, jfy, jfz, ee, je);
-      else
-        uc_steps<EV, NT, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, coul_rc2, P.cg, P.pa,
-                                two_a_sqrtpi, &S.pq[lane], jl, &S.type[lane], &S.key[lane], efx,
-                                efy, efz, jfx, jfy, jfz, ee, je);
+      bool xk = P.excl_span > PAIR_KEYMAP_SPAN || __any(hit);
+      const unsigned* jt = &S.type[lane];
+      const unsigned* jk = &S.key[lane];
+      if (P.uniform == 2) {
+        if (xk)
+          uc_steps<EV, NT, true, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, P.uc_rc2, P.cg,
+                                        P.pa, two_a_sqrtpi, &S.pq[lane], jl, jt, jk, efx, efy,
+                                        efz, jfx, jfy, jfz, ee, je);
+        else
+          uc_steps<EV, NT, false, false>(R, pix, piy, piz, ikey, P.excl_span, kqi, P.uc_rc2,
+                                         P.cg, P.pa, two_a_sqrtpi, &S.pq[lane], jl, jt, jk, efx,
+                                         efy, efz, jfx, jfy, jfz, ee, je);
+      } else {
+        if (xk)
+          uc_steps<EV, NT, true, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, P.uc_rc2, P.cg,
+                                       P.pa, two_a_sqrtpi, &S.pq[lane], jl, jt, jk, efx, efy, efz,
+                                       jfx, jfy, jfz, ee, je);
+        else
+          uc_steps<EV, NT, false, true>(R, pix, piy, piz, ikey, P.excl_span, kqi, P.uc_rc2, P.cg,
+                                        P.pa, two_a_sqrtpi, &S.pq[lane], jl, jt, jk, efx, efy,
+                                        efz, jfx, jfy, jfz, ee, je);
+      }
       // every lane's reads of this entry precede the next entry's writes
       wave_lds_sync();
     } else {
@@ -828,7 +850,17 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
   bool uniform = P.coul_rc2 > 0.f && !any_switch;
   for (int k = 0; k < a.ntypes * a.ntypes; ++k)
     if (!(P.c6[k] == 0.f && P.c12[k] == 0.f) && P.ljcut2[k] != P.coul_rc2) uniform = false;
-  P.uniform = uniform ? 1 : 0;
+  // LJ only: the interacting type pairs share one cutoff
+  float ljc = -1.f;
+  bool lj_uniform = !(P.coul_rc2 > 0.f) && !any_switch;
+  for (int k = 0; k < a.ntypes * a.ntypes; ++k) {
+    if (P.c6[k] == 0.f && P.c12[k] == 0.f) continue;
+    if (ljc < 0.f) ljc = P.ljcut2[k];
+    else if (P.ljcut2[k] != ljc) lj_uniform = false;
+  }
+  if (!(ljc > 0.f)) lj_uniform = false;
+  P.uniform = uniform ? 1 : (lj_uniform ? 2 : 0);
+  P.uc_rc2 = uniform ? P.coul_rc2 : ljc;
   P.cg = (float)(-a.alpha * a.alpha * 1.4426950408889634);
   P.pa = (float)(0.3275911 * a.alpha);
   if (ev == 1) launch_pair_ev<1>(st, a, P, any_switch);
