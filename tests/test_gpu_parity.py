@@ -86,6 +86,40 @@ def test_argon_forces_energy_virial(oracle):
     assert pe.sum() == pytest.approx(E.sum(), rel=1e-5)
 
 
+def _argon_parity(s, tag, oracle):
+    e = engine_for(s, dt=U.time(5e-15))
+    e.force_eval(scalars=True)
+    f = e.forces()
+    sc = e.scalars()
+    e.close()
+    F, E, W, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice, band=1e-6)
+    rms, mx = _force_errors(f, F, band)
+    say(f"{tag} rms {rms:.3e} max {mx:.3e}")
+    assert rms <= 5e-6 and mx <= 1e-4
+    assert sc["pe"] == pytest.approx(E.sum(), rel=1e-5)
+    assert sc["virial"] == pytest.approx(W.sum(), rel=1e-5)
+
+
+def test_switched_lj_forces_general_loop(oracle):
+    # the reference's cubic switch from 0.9 r_c: the general pair loop
+    # (shifted images, per-type select, switch polynomial)
+    say("case: argon 10000 forces, switched LJ")
+    s = _jittered_argon()
+    s.lj.r_switch[:] = 0.9 * s.lj.cutoff
+    _argon_parity(s, "switched argon", oracle)
+
+
+def test_narrow_box_forces_minimum_image_loop(oracle):
+    # a 2.38 nm box leaves less than the search radius between a block and
+    # half the box: the general loop with a minimum image per pair
+    say("case: argon 343 in a 2.38 nm box (per-pair minimum image)")
+    s = systems.argon_lattice(n_atoms=343, n_side=7, box_m=7 * 3.40e-10)
+    rng = np.random.default_rng(3)
+    p = s.posq[:, :3].astype(np.float64) + rng.normal(0, 0.3, size=(s.n, 3))
+    s.posq[:, :3] = systems.wrap_into_box(p, s.lattice).astype(np.float32)
+    _argon_parity(s, "narrow argon", oracle)
+
+
 def test_packed_list_covers_pair_set(oracle):
     say("case: argon pair set")
     s = _jittered_argon(seed=3)
