@@ -50,6 +50,13 @@ struct Stage {
 struct hd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream for the SPME pipeline, forked from and joined back into
+  // `stream` inside each force evaluation (default priority)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // off while a batch captures this context's step: the batch graph already
+  // runs the replicas side by side, and per-replica forks slowed it
+  bool spme_fork = true;
   std::string err;
 
   // configuration
@@ -394,6 +401,67 @@ void stage_end(hd_ctx* c, bool on) {
   (void)hipEventRecord(c->stages[c->n_stage_used - 1].b, c->stream);
 }
 
+// The SPME reciprocal pipeline of one force evaluation, on stream sst.
+static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed) {
+  HdSpmeArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.n = (unsigned)c->n;
+  sa.na = c->grid[0];
+  sa.nb = c->grid[1];
+  sa.nc = c->grid[2];
+  sa.kc = c->kc;
+  sa.alpha = c->alpha;
+  sa.lattice = c->d_lattice;
+  sa.tile_posq = c->tile_posq;
+  sa.rho_fixed = c->rho_fixed;
+  sa.cgrid = c->cgrid;
+  sa.vgrid = c->vgrid;
+  sa.spl = c->spl;
+  sa.sg = c->sg;
+  sa.tw_a = c->tw_a;
+  sa.tw_b = c->tw_b;
+  sa.tw_c = c->tw_c;
+  sa.bf_a = c->bf_a;
+  sa.bf_b = c->bf_b;
+  sa.bf_c = c->bf_c;
+  sa.partials = c->recip_partials;
+  sa.recip = c->recip;
+  sa.w_share = c->w_share;
+  sa.voxel_rank = c->voxel_rank;
+  sa.voxel_off = c->voxel_off;
+  for (int d = 0; d < 3; ++d) {
+    sa.nv[d] = c->nv[d];
+    // atoms move at most r_skin/2 between rebuilds; 20 % slack for box
+    // changes since set-up, plus one voxel for the binning's rounding
+    sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] / c->lattice[d]) + 1;
+  }
+  stage_begin(c, "spme_recip", timed);
+  if (c->dft_mode == 1) {
+    hd_launch_spme_spread(sst, sa);
+    HdDftArgs da;
+    memset(&da, 0, sizeof(da));
+    da.na = c->grid[0];
+    da.nb = c->grid[1];
+    da.nc = c->grid[2];
+    da.nch = c->grid[2] / 2 + 1;
+    da.rho = c->rho_fixed;
+    da.cgrid = c->cgrid;
+    da.vgrid = c->vgrid;
+    da.bfa = c->bf_a;
+    da.bfb = c->bf_b;
+    da.bfc = c->bf_c;
+    da.lattice = c->d_lattice;
+    da.kc = (float)c->kc;
+    da.alpha = (float)c->alpha;
+    da.partials = c->recip_partials;
+    hd_launch_dft(sst, da, c->dft);
+    hd_launch_recip_reduce(sst, sa, hd_dft_partials_count(c->grid[1], c->grid[2]));
+  } else {
+    hd_launch_spme(sst, sa);
+  }
+  stage_end(c, timed);
+}
+
 // Force evaluation; kick = fuse the post-force half kick with factor
 // factors[1].  Enqueue only.
 bool has_shake(const hd_ctx* c) { return c->h_sh_goff.size() > 1; }
@@ -412,6 +480,18 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
   hd_launch_tile_gather(st, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
                         ev ? c->fpj_ew : nullptr, (unsigned)c->n, c->npad);
   stage_end(c, timed);
+  // The SPME pipeline reads the tile positions, the lattice and the voxel
+  // tables, all final here, and nothing on the build and pair path writes
+  // what it reads: it runs on the second stream, concurrent with them, and
+  // joins before the finalize.  The stage-timing pass keeps one stream so
+  // that every stage is timed alone.
+  bool fork = c->have_spme && !timed && c->spme_fork;
+  if (fork) {
+    (void)hipEventRecord(c->ev_fork, st);
+    (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    enqueue_spme(c, c->side, false);
+    (void)hipEventRecord(c->ev_join, c->side);
+  }
   stage_begin(c, "neighbor_build", timed);
   hd_launch_neighbor_build(st, na);
   stage_end(c, timed);
@@ -452,65 +532,10 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
     hd_launch_angles(st, c->posq, c->d_lattice, c->d_angles, c->d_ang_k, c->d_ang_t0, c->slot_of,
                      c->fpj, c->fpj_ew, (unsigned)c->h_ang_k.size(), ev);
   stage_end(c, timed);
-  if (c->have_spme) {
-    HdSpmeArgs sa;
-    memset(&sa, 0, sizeof(sa));
-    sa.n = (unsigned)c->n;
-    sa.na = c->grid[0];
-    sa.nb = c->grid[1];
-    sa.nc = c->grid[2];
-    sa.kc = c->kc;
-    sa.alpha = c->alpha;
-    sa.lattice = c->d_lattice;
-    sa.tile_posq = c->tile_posq;
-    sa.rho_fixed = c->rho_fixed;
-    sa.cgrid = c->cgrid;
-    sa.vgrid = c->vgrid;
-    sa.spl = c->spl;
-    sa.sg = c->sg;
-    sa.tw_a = c->tw_a;
-    sa.tw_b = c->tw_b;
-    sa.tw_c = c->tw_c;
-    sa.bf_a = c->bf_a;
-    sa.bf_b = c->bf_b;
-    sa.bf_c = c->bf_c;
-    sa.partials = c->recip_partials;
-    sa.recip = c->recip;
-    sa.w_share = c->w_share;
-    sa.voxel_rank = c->voxel_rank;
-    sa.voxel_off = c->voxel_off;
-    for (int d = 0; d < 3; ++d) {
-      sa.nv[d] = c->nv[d];
-      // atoms move at most r_skin/2 between rebuilds; 20 % slack for box
-      // changes since set-up, plus one voxel for the binning's rounding
-      sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] / c->lattice[d]) + 1;
-    }
-    stage_begin(c, "spme_recip", timed);
-    if (c->dft_mode == 1) {
-      hd_launch_spme_spread(st, sa);
-      HdDftArgs da;
-      memset(&da, 0, sizeof(da));
-      da.na = c->grid[0];
-      da.nb = c->grid[1];
-      da.nc = c->grid[2];
-      da.nch = c->grid[2] / 2 + 1;
-      da.rho = c->rho_fixed;
-      da.cgrid = c->cgrid;
-      da.vgrid = c->vgrid;
-      da.bfa = c->bf_a;
-      da.bfb = c->bf_b;
-      da.bfc = c->bf_c;
-      da.lattice = c->d_lattice;
-      da.kc = (float)c->kc;
-      da.alpha = (float)c->alpha;
-      da.partials = c->recip_partials;
-      hd_launch_dft(st, da, c->dft);
-      hd_launch_recip_reduce(st, sa, hd_dft_partials_count(c->grid[1], c->grid[2]));
-    } else {
-      hd_launch_spme(st, sa);
-    }
-    stage_end(c, timed);
-  }
+  if (fork)
+    (void)hipStreamWaitEvent(st, c->ev_join, 0);
+  else if (c->have_spme)
+    enqueue_spme(c, st, timed);
   HdFinalizeArgs fa;
   memset(&fa, 0, sizeof(fa));
   fa.n = (unsigned)c->n;
@@ -828,7 +853,13 @@ int hd_ctx_create(int device, hd_ctx** out) {
     delete c;
     return HD_ERR_HIP;
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HD_ERR_HIP;
   }
@@ -840,6 +871,7 @@ void hd_ctx_destroy(hd_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
   if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
   for (auto& s : c->stages) {
@@ -847,6 +879,9 @@ void hd_ctx_destroy(hd_ctx* c) {
     (void)hipEventDestroy(s.b);
   }
   for (void* p : c->allocs) (void)hipFree(p);
+  (void)hipEventDestroy(c->ev_fork);
+  (void)hipEventDestroy(c->ev_join);
+  (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2270,7 +2305,9 @@ static int batch_capture(hd_batch* b, bool ev, hipGraphExec_t* out) {
   for (size_t i = 0; i < n; ++i) {
     hd_ctx* c = b->ctx[i];
     BCHK(b, hipStreamWaitEvent(c->stream, i < lanes ? b->fork : b->join[i - lanes], 0));
+    c->spme_fork = false;
     enqueue_step(c, ev, false);
+    c->spme_fork = true;
     BCHK(b, hipEventRecord(b->join[i], c->stream));
     if (i + lanes >= n) BCHK(b, hipStreamWaitEvent(b->stream, b->join[i], 0));
   }
