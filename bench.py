@@ -10,11 +10,14 @@ Untimed set-up: Maxwell-Boltzmann velocities (ChaCha8 seed 7), an
 equilibration phase with a tight thermostat (the lattice start is far
 from equilibrium), then `--warmup` steps at the production settings.
 Timed: exactly `--steps` steps replayed as captured hipGraphs, bracketed by
-device synchronisation.  Afterwards a second pass of the same number of
-steps runs without graphs with hipEvents around every stage (same stream),
-which gives the per-kernel durations used for the roofline entry.
+device synchronisation (each step's SPME pipeline runs on a second stream
+beside the neighbour build and the pair kernel).  Afterwards a second pass
+of the same number of steps runs without graphs, every stage alone on one
+stream with hipEvents around it, which gives the per-kernel durations used
+for the roofline entry.
 
-Prints one JSON line (the driver's contract).
+Prints one JSON line: metric, value, unit, the run's shape and the
+roofline and CPU-baseline objects.
 """
 from __future__ import annotations
 
