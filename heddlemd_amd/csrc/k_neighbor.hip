@@ -55,15 +55,21 @@ __global__ void k_disp_check(const float4* __restrict__ posq, const float4* __re
 }
 
 // Voxel key = curve rank of the atom's fractional voxel; histogram.
+// Also records the positions and lattice of this rebuild, against which the
+// displacement check measures (positions do not change within a force
+// evaluation, so the copy can be taken here, with the keys).
 __global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __restrict__ lattice,
                              const unsigned* __restrict__ voxel_rank, unsigned nva, unsigned nvb,
                              unsigned nvc, unsigned* __restrict__ atom_key,
-                             unsigned* __restrict__ voxel_count, const unsigned* status,
+                             unsigned* __restrict__ voxel_count, float4* __restrict__ ref,
+                             float* __restrict__ ref_lattice, const unsigned* status,
                              unsigned n) {
   if (!nb_rebuild_due(status)) return;
   HdLattice L = hd_load_lattice(lattice);
+  if (blockIdx.x == 0 && threadIdx.x < 6) ref_lattice[threadIdx.x] = lattice[threadIdx.x];
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 p = posq[i];
+    ref[i] = p;
     float x = p.x, y = p.y, z = p.z;
     int ia, ib, ic;
     hd_wrap_with_image(x, y, z, ia, ib, ic, L);
@@ -82,9 +88,12 @@ __global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __res
 }
 
 // Exclusive scan of voxel_count -> voxel_off (and cleared scatter
-// cursors) in three passes over chunks of NB_SCAN_CHUNK voxels: chunk sums,
-// a one-workgroup scan of the chunk sums, then a workgroup scan per chunk.
+// cursors) over chunks of NB_SCAN_CHUNK voxels: chunk sums, then a
+// workgroup scan per chunk.  Up to NB_SCAN_FUSED_MAX chunks (16 M atoms),
+// each workgroup sums the chunk sums before its own; beyond that a
+// one-workgroup pass turns them into offsets first (k_voxel_scan_parts).
 #define NB_SCAN_CHUNK 2048u  // 256 threads x 8 voxels
+#define NB_SCAN_FUSED_MAX 4096u
 
 __device__ __forceinline__ unsigned nb_block_excl_scan256(unsigned v, unsigned* tmp,
                                                           unsigned& total) {
@@ -120,6 +129,7 @@ __global__ void __launch_bounds__(256) k_voxel_scan_reduce(const unsigned* __res
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
+
 __global__ void __launch_bounds__(1024) k_voxel_scan_parts(unsigned* __restrict__ part,
                                                            const unsigned* status,
                                                            unsigned nparts) {
@@ -143,7 +153,6 @@ __global__ void __launch_bounds__(1024) k_voxel_scan_parts(unsigned* __restrict_
   }
   if (t == 0) part[nparts] = carry;
 }
-
 __global__ void __launch_bounds__(256) k_voxel_scan_apply(const unsigned* __restrict__ count,
                                                           const unsigned* __restrict__ part,
                                                           unsigned* __restrict__ voxel_off,
@@ -161,8 +170,23 @@ __global__ void __launch_bounds__(256) k_voxel_scan_apply(const unsigned* __rest
     c[k] = (b0 + k < nvox) ? count[b0 + k] : 0u;
     s += c[k];
   }
+  unsigned before, all;
+  if (nparts <= NB_SCAN_FUSED_MAX) {
+    // offset of this chunk: the sum of the chunk sums before it; chunk 0
+    // also sums them all, for voxel_off[nvox]
+    unsigned lim = blockIdx.x == 0 ? nparts : blockIdx.x;
+    unsigned o = 0;
+    for (unsigned k = threadIdx.x; k < lim; k += 256u) o += part[k];
+    (void)nb_block_excl_scan256(o, tmp, before);
+    all = before;
+    if (blockIdx.x == 0) before = 0u;
+  } else {
+    // offsets from k_voxel_scan_parts
+    before = part[blockIdx.x];
+    all = part[nparts];
+  }
   unsigned total;
-  unsigned run = part[blockIdx.x] + nb_block_excl_scan256(s, tmp, total);
+  unsigned run = before + nb_block_excl_scan256(s, tmp, total);
 #pragma unroll
   for (unsigned k = 0; k < per; ++k) {
     if (b0 + k < nvox) {
@@ -171,7 +195,7 @@ __global__ void __launch_bounds__(256) k_voxel_scan_apply(const unsigned* __rest
     }
     run += c[k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) voxel_off[nvox] = part[nparts];
+  if (blockIdx.x == 0 && threadIdx.x == 0) voxel_off[nvox] = all;
 }
 
 __global__ void k_voxel_scatter(const unsigned* __restrict__ atom_key,
@@ -654,17 +678,6 @@ __global__ void __launch_bounds__(256) k_build_alist(
 
 // End of a rebuild: remember the reference positions and clear the
 // rebuild request bits (sticky overflow/high-water bits stay for the host).
-__global__ void k_rebuild_finish(const float4* __restrict__ posq, float4* __restrict__ ref,
-                                 const float* __restrict__ lattice, float* __restrict__ ref_lattice,
-                                 unsigned* status, unsigned n) {
-  if (!nb_rebuild_due(status)) return;
-  if (blockIdx.x == 0 && threadIdx.x < 6) ref_lattice[threadIdx.x] = lattice[threadIdx.x];
-  // every thread observed the request before the first write below: the
-  // clearing is done by the last workgroup after a grid-wide fence is not
-  // possible, so it is done in a separate one-thread kernel.
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    ref[i] = posq[i];
-}
 
 // Clears the rebuild request and, on rebuild steps, orders the blocks by
 // entry count, heaviest first, as the pair kernel's workgroup order: block
@@ -724,12 +737,14 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
   hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st, a.posq,
                      a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
   hipLaunchKernelGGL(k_voxel_keys, dim3(gridN), dim3(T), 0, st, a.posq, a.lattice, a.voxel_rank,
-                     a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.status, a.n);
+                     a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.ref_pos,
+                     a.ref_lattice, a.status, a.n);
   unsigned nparts = hd_voxel_scan_parts(a.nvox);
   hipLaunchKernelGGL(k_voxel_scan_reduce, dim3(nparts), dim3(256), 0, st, a.voxel_count,
                      a.voxel_part, a.status, a.nvox);
-  hipLaunchKernelGGL(k_voxel_scan_parts, dim3(1), dim3(1024), 0, st, a.voxel_part, a.status,
-                     nparts);
+  if (nparts > NB_SCAN_FUSED_MAX)
+    hipLaunchKernelGGL(k_voxel_scan_parts, dim3(1), dim3(1024), 0, st, a.voxel_part, a.status,
+                       nparts);
   hipLaunchKernelGGL(k_voxel_scan_apply, dim3(nparts), dim3(256), 0, st, a.voxel_count,
                      a.voxel_part, a.voxel_off, a.voxel_cursor, a.status, a.nvox, nparts);
   hipLaunchKernelGGL(k_voxel_scatter, dim3(gridN), dim3(T), 0, st, a.atom_key, a.voxel_off,
@@ -769,10 +784,6 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
     hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0,
                        st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask, nchunks,
                        a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
-  unsigned gridN = min(cdiv(a.n, 256), NB_GATED_WGS);
-  if (gridN == 0) gridN = 1;
-  hipLaunchKernelGGL(k_rebuild_finish, dim3(gridN), dim3(256), 0, st, a.posq, a.ref_pos,
-                     a.lattice, a.ref_lattice, a.status, a.n);
   hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(NB_ORDER_BINS), 0, st, a.status,
                      a.rebuild_count, a.entry_count, nblocks, a.block_order);
 }
