@@ -41,6 +41,10 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
 void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
                            const float* lattice, float4* tile_posq, long long* fpj,
                            long long* fpj_ew, unsigned n, unsigned npad);
+// the same, and on rebuild steps the slot types and keys and the blocks'
+// bounding boxes (which the neighbour build then uses)
+void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long long* fpj,
+                                   long long* fpj_ew);
 
 struct HdPairArgs {
   unsigned n, npad, cap;

@@ -477,8 +477,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
   hd_launch_neighbor_update(st, na);
   stage_end(c, timed);
   stage_begin(c, "tile_gather", timed);
-  hd_launch_tile_gather(st, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
-                        ev ? c->fpj_ew : nullptr, (unsigned)c->n, c->npad);
+  hd_launch_tile_gather_rebuild(st, na, c->fpj, ev ? c->fpj_ew : nullptr);
   stage_end(c, timed);
   // The SPME pipeline reads the tile positions, the lattice and the voxel
   // tables, all final here, and nothing on the build and pair path writes
@@ -1531,8 +1530,7 @@ static int build_list(hd_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->ctrl + 2, &zero, sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
   HdNeighborArgs na = nb_args(c);
   hd_launch_neighbor_update(c->stream, na);
-  hd_launch_tile_gather(c->stream, c->posq, c->sorted_id, c->d_lattice, c->tile_posq, c->fpj,
-                        c->fpj_ew, (unsigned)c->n, c->npad);
+  hd_launch_tile_gather_rebuild(c->stream, na, c->fpj, c->fpj_ew);
   hd_launch_neighbor_build(c->stream, na);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));

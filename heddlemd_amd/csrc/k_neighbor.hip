@@ -238,36 +238,28 @@ __global__ void k_voxel_sort(const unsigned* __restrict__ voxel_off,
 
 // Per-slot static data refreshed at a rebuild: type and exclusion key in
 // slot order, padding lanes marked with sentinels.
-__global__ void k_slot_static(const unsigned* __restrict__ sorted_id,
-                              const unsigned* __restrict__ type, const unsigned* __restrict__ key,
-                              unsigned* __restrict__ tile_type, unsigned* __restrict__ tile_key,
-                              const unsigned* status, unsigned n, unsigned npad) {
-  if (!nb_rebuild_due(status)) return;
-  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < npad;
-       s += gridDim.x * blockDim.x) {
-    if (s < n) {
-      unsigned id = sorted_id[s];
-      tile_type[s] = type[id];
-      tile_key[s] = key[id];
-    } else {
-      tile_type[s] = 0u;
-      tile_key[s] = HD_SENTINEL - 1u;
-    }
-  }
-}
 
 // Every step: gather positions into slot order (wrapped into the primary
 // cell) and clear the j-side fixed-point accumulators of the pair kernel.
+// Slot-order copy of the positions (wrapped into the cell) and clearing of
+// the j-side accumulators, every step.  On rebuild steps (status given and
+// the request set) it also fills the slots' types and keys and each 64-slot
+// block's bounding box (one wave per block), from the same positions.
 __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* __restrict__ sorted_id,
                               const float* __restrict__ lattice, float4* __restrict__ tile_posq,
                               long long* __restrict__ fpj, long long* __restrict__ fpj_ew,
-                              unsigned n, unsigned npad) {
+                              unsigned n, unsigned npad, const unsigned* status,
+                              const unsigned* __restrict__ type, const unsigned* __restrict__ key,
+                              unsigned* __restrict__ tile_type, unsigned* __restrict__ tile_key,
+                              float4* __restrict__ center, float4* __restrict__ half) {
   unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= npad) return;
+  if (s >= npad) return;  // whole waves: npad is a multiple of 64
   float4 out;
+  unsigned id = 0u;
   if (s < n) {
     HdLattice L = hd_load_lattice(lattice);
-    float4 p = posq[sorted_id[s]];
+    id = sorted_id[s];
+    float4 p = posq[id];
     int a, b, c;
     hd_wrap_with_image(p.x, p.y, p.z, a, b, c, L);
     out = p;
@@ -279,22 +271,12 @@ __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* _
   reinterpret_cast<longlong2*>(fpj)[2 * s] = z;
   reinterpret_cast<longlong2*>(fpj)[2 * s + 1] = z;
   if (fpj_ew) reinterpret_cast<longlong2*>(fpj_ew)[s] = z;
-}
-
-// Per-block axis-aligned bounding box: centre and half extents (one wave
-// per block).  Padding lanes are ignored.
-__global__ void k_block_bbox(const float4* __restrict__ tile_posq, float4* __restrict__ center,
-                             float4* __restrict__ half, const unsigned* status, unsigned n,
-                             unsigned nblocks) {
-  if (!nb_rebuild_due(status)) return;
-  unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  unsigned lane = threadIdx.x & 63;
-  if (wave >= nblocks) return;
-  unsigned s = wave * 64 + lane;
+  if (!status || !nb_rebuild_due(status)) return;
   bool v = s < n;
-  float4 p = tile_posq[s];
-  float lox = v ? p.x : 3.0e38f, loy = v ? p.y : 3.0e38f, loz = v ? p.z : 3.0e38f;
-  float hix = v ? p.x : -3.0e38f, hiy = v ? p.y : -3.0e38f, hiz = v ? p.z : -3.0e38f;
+  tile_type[s] = v ? type[id] : 0u;
+  tile_key[s] = v ? key[id] : HD_SENTINEL - 1u;
+  float lox = v ? out.x : 3.0e38f, loy = v ? out.y : 3.0e38f, loz = v ? out.z : 3.0e38f;
+  float hix = v ? out.x : -3.0e38f, hiy = v ? out.y : -3.0e38f, hiz = v ? out.z : -3.0e38f;
   for (int off = 32; off > 0; off >>= 1) {
     lox = fminf(lox, __shfl_xor(lox, off));
     loy = fminf(loy, __shfl_xor(loy, off));
@@ -303,11 +285,15 @@ __global__ void k_block_bbox(const float4* __restrict__ tile_posq, float4* __res
     hiy = fmaxf(hiy, __shfl_xor(hiy, off));
     hiz = fmaxf(hiz, __shfl_xor(hiz, off));
   }
-  if (lane == 0) {
-    center[wave] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
-    half[wave] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
+  if ((threadIdx.x & 63u) == 0) {
+    unsigned blk = s >> 6;
+    center[blk] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
+    half[blk] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
   }
 }
+
+// Per-block axis-aligned bounding box: centre and half extents (one wave
+// per block).  Padding lanes are ignored.
 
 // Closest approach of two boxes under the minimum image (per axis; exact
 // for orthorhombic cells), as kernels/neighbor.cu:364-388.
@@ -752,16 +738,21 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
   unsigned gridV = min(cdiv(a.nvox, T), NB_GATED_WGS);
   hipLaunchKernelGGL(k_voxel_sort, dim3(gridV), dim3(T), 0, st, a.voxel_off, a.voxel_count,
                      a.sorted_id, a.slot_of, a.status, a.nvox);
-  unsigned gridP = min(cdiv(a.npad, T), NB_GATED_WGS);
-  hipLaunchKernelGGL(k_slot_static, dim3(gridP), dim3(T), 0, st, a.sorted_id, a.type, a.key,
-                     a.tile_type, a.tile_key, a.status, a.n, a.npad);
 }
 
 void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
                            const float* lattice, float4* tile_posq, long long* fpj,
                            long long* fpj_ew, unsigned n, unsigned npad) {
   hipLaunchKernelGGL(k_tile_gather, dim3(cdiv(npad, 256)), dim3(256), 0, st, posq, sorted_id,
-                     lattice, tile_posq, fpj, fpj_ew, n, npad);
+                     lattice, tile_posq, fpj, fpj_ew, n, npad, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr);
+}
+
+void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long long* fpj,
+                                   long long* fpj_ew) {
+  hipLaunchKernelGGL(k_tile_gather, dim3(cdiv(a.npad, 256)), dim3(256), 0, st, a.posq,
+                     a.sorted_id, a.lattice, a.tile_posq, fpj, fpj_ew, a.n, a.npad, a.status,
+                     a.type, a.key, a.tile_type, a.tile_key, a.center, a.half);
 }
 
 unsigned hd_block_pair_chunks(unsigned nblocks) { return cdiv(nblocks, 64); }
@@ -769,8 +760,6 @@ unsigned hd_block_pair_chunks(unsigned nblocks) { return cdiv(nblocks, 64); }
 void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
   unsigned nblocks = a.npad / 64;
   unsigned nchunks = hd_block_pair_chunks(nblocks);
-  hipLaunchKernelGGL(k_block_bbox, dim3(cdiv(nblocks * 64, 256)), dim3(256), 0, st, a.tile_posq,
-                     a.center, a.half, a.status, a.n, nblocks);
   unsigned pair_waves = min(nblocks * nchunks, (unsigned)NB_PAIR_WAVES);
   hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(pair_waves * 64, 256)), dim3(256), 0, st,
                      a.center, a.half, a.lattice, a.rs2, a.block_mask, nblocks, nchunks,
