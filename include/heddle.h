@@ -8,8 +8,11 @@
  * src/integrator/constraint.rs:43).  This header is the boundary that
  * replaces that binding: plain pointers and sizes, no C++ or torch types,
  * host arrays owned by the caller, device memory owned by the context, all
- * work ordered on one HIP stream per context.  A context is not thread
- * safe; separate contexts are independent.
+ * calls ordered on one HIP stream per context (inside a force evaluation
+ * the SPME pipeline forks onto a second stream of the context and joins
+ * back before the forces are finalized, so callers see one ordered
+ * stream).  A context is not thread safe; separate contexts are
+ * independent.
  *
  * Units: Hartree atomic units (Bohr, electron mass, e, Hartree, atomic
  * time; k_B = k_C = 1) as the reference uses internally
