@@ -143,8 +143,14 @@ struct HdFinalizeArgs {
   double *vxlo, *vylo, *vzlo;  // lossless residuals (nullptr = f32 mode)
   const float* mass;
   const float* factor;  // thermostat factor applied after the kick
+  // per-workgroup FP64 sums of the per-atom energy and virial (nullptr =
+  // none), 2 * hd_finalize_blocks() doubles
+  double* partials;
 };
+unsigned hd_finalize_blocks(unsigned n, bool gather);
 void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev);
+// sum of np (a, b) partial pairs into out[0], out[1], in fixed order
+void hd_launch_sum2_final(hipStream_t st, const double* partials, unsigned np, double* out);
 
 struct HdIntegrateArgs {
   unsigned n;

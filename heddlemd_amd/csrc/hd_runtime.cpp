@@ -143,6 +143,7 @@ struct hd_ctx {
   float* w_share = nullptr;
   // device: scalars / thermostat
   double* red_partials = nullptr;
+  double* fin_partials = nullptr;  // finalize's per-workgroup energy/virial sums
   double* scal = nullptr;  // [0] ke, [1] pe, [2] vir, [3] ke scratch
   float* factors = nullptr;  // [0] pre, [1] post
   double* nhc_state = nullptr;
@@ -571,13 +572,14 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
   fa.vzlo = c->lossless ? c->vzlo : nullptr;
   fa.mass = c->mass;
   fa.factor = c->have_nhc ? c->factors + 1 : nullptr;
+  fa.partials = ev ? c->fin_partials : nullptr;
   stage_begin(c, kick ? "finalize_kick" : "finalize", timed);
   hd_launch_finalize(st, fa, ev);
   stage_end(c, timed);
   if (ev) {
     stage_begin(c, "reduce_scalars", timed);
-    hd_launch_sum2(st, (unsigned)c->n, c->pe, c->vir, c->red_partials,
-                   c->scal + (ev == 1 ? 1 : 6));
+    hd_launch_sum2_final(st, c->fin_partials, hd_finalize_blocks((unsigned)c->n, c->have_spme),
+                         c->scal + (ev == 1 ? 1 : 6));
     stage_end(c, timed);
   }
 }
@@ -1684,6 +1686,7 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->fpi, (size_t)c->npad * 4); DALLOC(c, c->fpj, (size_t)c->npad * 4);
   DALLOC(c, c->fpi_ew, (size_t)c->npad * 2); DALLOC(c, c->fpj_ew, (size_t)c->npad * 2);
   DALLOC(c, c->red_partials, 2 * HD_REDUCE_BLOCKS);
+  DALLOC(c, c->fin_partials, 2 * (size_t)hd_finalize_blocks((unsigned)c->n, true));
   DALLOC(c, c->scal, 8);
   DALLOC(c, c->factors, 2);
   DALLOC(c, c->recip, 2);

@@ -396,6 +396,10 @@ __global__ void __launch_bounds__(256) k_sum2_final(const double* __restrict__ p
   }
 }
 
+void hd_launch_sum2_final(hipStream_t st, const double* partials, unsigned np, double* out) {
+  hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(256), 0, st, partials, np, out);
+}
+
 void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
                     double* out) {
   hipLaunchKernelGGL(k_sum2_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, a, b,
@@ -470,11 +474,16 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
   // the a-plane da = q of the 4x4x4 stencil (16 grid values), so four
   // times as many reads are in flight; the quad's partial sums are combined
   // and its first lane finishes the slot.
+  //
+  // With a.partials set (energy or virial launches), the workgroup also sums
+  // its slots' final per-atom energy and virial in FP64, in slot order, and
+  // writes them to partials[2 * blockIdx.x + {0, 1}]; every thread reaches
+  // that reduction.
   unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned s = GATHER ? (t >> 2) : t;
   unsigned q = GATHER ? (t & 3u) : 0u;
-  if (s >= a.n) return;  // whole quads
-  unsigned gcell = GATHER ? a.sg[s] : 0xFFFFFFFFu;
+  bool live = s < a.n;  // whole quads
+  unsigned gcell = (GATHER && live) ? a.sg[s] : 0xFFFFFFFFu;
   float phi = 0.f, gra = 0.f, grb = 0.f, grc = 0.f, qi = 0.f;
   if (GATHER && gcell != 0xFFFFFFFFu) {
     // spline record of this slot (k_spline_setup): (t_a, t_b, t_c, q) and
@@ -513,8 +522,9 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     grb += __shfl_xor(grb, 1); grc += __shfl_xor(grc, 1);
     phi += __shfl_xor(phi, 2); gra += __shfl_xor(gra, 2);
     grb += __shfl_xor(grb, 2); grc += __shfl_xor(grc, 2);
-    if (q != 0) return;
   }
+  float pe = 0.f, vir = 0.f;
+  if (live && q == 0) {
   unsigned id = a.sorted_id[s];
   const longlong2* fi = reinterpret_cast<const longlong2*>(a.fpi) + 2 * s;
   const longlong2* fj = reinterpret_cast<const longlong2*>(a.fpj) + 2 * s;
@@ -522,7 +532,6 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
   float fx = hd_from_fixed48(i01.x + j01.x);
   float fy = hd_from_fixed48(i01.y + j01.y);
   float fz = hd_from_fixed48(i2x.x + j2x.x);
-  float pe = 0.f, vir = 0.f;
   if (EV) {
     longlong2 ie = reinterpret_cast<const longlong2*>(a.fpi_ew)[s];
     longlong2 je = reinterpret_cast<const longlong2*>(a.fpj_ew)[s];
@@ -578,13 +587,28 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
       a.vz[id] = (a.vz[id] + az * half_dt) * f;
     }
   }
+  }
+  if (EV && a.partials) {
+    __shared__ double sh[256];
+    double sp = block_tree_sum((double)pe, sh);
+    __syncthreads();
+    double sv = block_tree_sum((double)vir, sh);
+    if (threadIdx.x == 0) {
+      a.partials[2 * blockIdx.x] = sp;
+      a.partials[2 * blockIdx.x + 1] = sv;
+    }
+  }
+}
+
+unsigned hd_finalize_blocks(unsigned n, bool gather) {
+  // four lanes per slot with the SPME gather
+  return ((gather ? 4u : 1u) * n + 255) / 256;
 }
 
 void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev) {
   if (!a.n) return;
   bool ga = a.vgrid != nullptr, k = a.kick, ll = a.vxlo != nullptr;
-  // four lanes per slot with the SPME gather
-  dim3 g(((ga ? 4u : 1u) * a.n + 255) / 256), b(256);
+  dim3 g(hd_finalize_blocks(a.n, ga)), b(256);
 #define HD_FIN(E, G, K, LL) \
   if (ev == E && ga == G && k == K && ll == LL) { \
     hipLaunchKernelGGL((k_finalize<E, G, K, LL>), g, b, 0, st, a); \
