@@ -304,9 +304,6 @@ struct HdNhcArgs {
 void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
                   const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
                   float dt, float* factor);
-// Fixed-order FP64 sums of per-atom arrays a and b into out[0], out[1].
-void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
-                    double* out);
 struct HdBarostatArgs {
   unsigned seed_lo, seed_hi;
   double pressure, tau, compressibility, kt, dt, mu3_min;

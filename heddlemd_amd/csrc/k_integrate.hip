@@ -359,25 +359,6 @@ void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, 
                      ke, na, nhc ? 1 : 0, dt, factor);
 }
 
-__global__ void __launch_bounds__(256) k_sum2_partials(unsigned n, const float* __restrict__ a,
-                                                       const float* __restrict__ b,
-                                                       double* __restrict__ partials) {
-  __shared__ double sh[256];
-  unsigned chunk = (n + gridDim.x - 1) / gridDim.x;
-  unsigned b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, n);
-  double sa = 0.0, sb = 0.0;
-  for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-    sa += (double)a[i];
-    sb += (double)b[i];
-  }
-  double ta = block_tree_sum(sa, sh);
-  __syncthreads();
-  double tb = block_tree_sum(sb, sh);
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = ta;
-    partials[2 * blockIdx.x + 1] = tb;
-  }
-}
 
 __global__ void __launch_bounds__(256) k_sum2_final(const double* __restrict__ partials,
                                                     unsigned np, double* __restrict__ out) {
@@ -400,13 +381,6 @@ void hd_launch_sum2_final(hipStream_t st, const double* partials, unsigned np, d
   hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(256), 0, st, partials, np, out);
 }
 
-void hd_launch_sum2(hipStream_t st, unsigned n, const float* a, const float* b, double* partials,
-                    double* out) {
-  hipLaunchKernelGGL(k_sum2_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, a, b,
-                     partials);
-  hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(256), 0, st, partials,
-                     (unsigned)HD_REDUCE_BLOCKS, out);
-}
 
 // Time reversal: v -> -v including the compensated residuals (lossless
 // mode), so a following run retraces the trajectory
