@@ -299,10 +299,15 @@ __global__ void __launch_bounds__(256) k_ke_partials(unsigned n, const float* __
 }
 
 // One Nose-Hoover chain sub-step (nose_hoover_chain.rs:109-180), FP64.
+// The forward sweep's scale for link j reads pxi[j + 1] before that link is
+// updated again, i.e. the value the backward sweep left: its exp is the
+// backward sweep's, reused (chain length <= 16).
 __device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, double dt,
                                double k, double g_dof, double kt) {
+  double sc[16];
   for (int j = m - 1; j >= 0; --j) {
     double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
+    sc[j] = s;
     pxi[j] *= s;
     double gj = (j == 0) ? k - g_dof * kt : pxi[j - 1] * pxi[j - 1] / q[j - 1] - kt;
     pxi[j] += dt / 4.0 * gj;
@@ -312,7 +317,7 @@ __device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, 
   k *= factor * factor;
   for (int j = 0; j < m; ++j) xi[j] += dt / 2.0 * pxi[j] / q[j];
   for (int j = 0; j < m; ++j) {
-    double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
+    double s = sc[j];
     pxi[j] *= s;
     double gj = (j == 0) ? k - g_dof * kt : pxi[j - 1] * pxi[j - 1] / q[j - 1] - kt;
     pxi[j] += dt / 4.0 * gj;
