@@ -34,7 +34,8 @@ struct HdNeighborArgs {
 };
 unsigned hd_block_pair_chunks(unsigned nblocks);
 
-void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a);
+// check = false when the displacement check already ran this step (KE pass)
+void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a, bool check = true);
 // number of chunk sums of the voxel-count scan (the buffer needs one more)
 unsigned hd_voxel_scan_parts(unsigned nvox);
 void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
@@ -301,9 +302,27 @@ struct HdNhcArgs {
 // KE reduction in FP64 (fixed chunking and tree) into ke[0]; when nhc is
 // non-null also advances the chain by one half step and writes the
 // cumulative velocity factor (f32) to factor[0].
+// Neighbour status word bits (device-side rebuild request and list state).
+#define NB_STATUS_REBUILD 1u
+#define NB_STATUS_HIGHWATER 2u
+#define NB_STATUS_OVERFLOW 8u
+#define NB_STATUS_FORCE 0x100u
+
+// Optional neighbour displacement check folded into the KE pass (the
+// positions final for the step): sets NB_STATUS_REBUILD in *status when
+// an atom has moved more than half_skin (plus the lattice change) since the
+// last rebuild, as k_disp_check does.
+struct HdDispArgs {
+  const float4* posq;
+  const float4* ref;
+  const float* lattice;
+  const float* ref_lattice;
+  float half_skin;
+  unsigned* status;
+};
 void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
                   const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
-                  float dt, float* factor);
+                  float dt, float* factor, const HdDispArgs* disp = nullptr);
 struct HdBarostatArgs {
   unsigned seed_lo, seed_hi;
   double pressure, tau, compressibility, kt, dt, mu3_min;

@@ -471,11 +471,13 @@ bool has_constraints(const hd_ctx* c) { return !c->h_groups.empty() || has_shake
 // ev: 0 forces only; 1 energy and virial into scal[1], scal[2]; 2 virial
 // only (what a barostat consumes on steps that log nothing), reduced into
 // the scratch pair scal[6], scal[7] so the last logged energy stays intact
-void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed) {
+// disp_done: the displacement check already ran on this step's final
+// positions (folded into the step's last KE pass)
+void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = false) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
   stage_begin(c, "neighbor_update", timed);
-  hd_launch_neighbor_update(st, na);
+  hd_launch_neighbor_update(st, na, !disp_done);
   stage_end(c, timed);
   stage_begin(c, "tile_gather", timed);
   hd_launch_tile_gather_rebuild(st, na, c->fpj, ev ? c->fpj_ew : nullptr);
@@ -809,18 +811,28 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   stage_end(c, timed);
   enqueue_constrain_positions(c, timed);
   enqueue_constrain_velocities(c, timed);
+  // the positions are final for the step: the KE pass also runs the
+  // neighbour displacement check
+  HdDispArgs disp;
+  disp.posq = c->posq;
+  disp.ref = c->ref_pos;
+  disp.lattice = c->d_lattice;
+  disp.ref_lattice = c->ref_lattice;
+  disp.half_skin = (float)(0.5 * c->r_skin);
+  disp.status = c->ctrl;
+  bool disp_done = c->have_nhc || c->have_baro;
   if (c->have_nhc) {
     stage_begin(c, "nhc_post", timed);
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
-                 (float)c->dt, c->factors + 1);
+                 (float)c->dt, c->factors + 1, &disp);
     stage_end(c, timed);
   } else if (c->have_baro) {
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
-                 0.f, nullptr);
+                 0.f, nullptr, &disp);
   }
   // a barostat consumes the virial every step; the energies only when logged
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
-  enqueue_forces(c, mode, true, timed);
+  enqueue_forces(c, mode, true, timed, disp_done);
   if (c->have_baro) enqueue_crescale(c, mode, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);

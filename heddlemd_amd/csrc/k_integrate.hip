@@ -285,15 +285,32 @@ __global__ void __launch_bounds__(256) k_ke_partials(unsigned n, const float* __
                                                      const float* __restrict__ vy,
                                                      const float* __restrict__ vz,
                                                      const float* __restrict__ mass,
-                                                     double* __restrict__ partials) {
+                                                     double* __restrict__ partials,
+                                                     HdDispArgs d) {
   __shared__ double sh[256];
   unsigned chunk = (n + gridDim.x - 1) / gridDim.x;
   unsigned b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, n);
   double s = 0.0;
+  bool over = false;
+  HdLattice L;
+  float dl = 0.f;
+  if (d.status) {
+    L = hd_load_lattice(d.lattice);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dl = fmaxf(dl, fabsf(d.lattice[k] - d.ref_lattice[k]));
+  }
   for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
     double x = vx[i], y = vy[i], z = vz[i];
     s += 0.5 * (double)mass[i] * (x * x + y * y + z * z);
+    if (d.status) {
+      // the displacement test of k_disp_check
+      float4 p = d.posq[i], r = d.ref[i];
+      float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+      hd_min_image(dx, dy, dz, L);
+      over |= sqrtf(dx * dx + dy * dy + dz * dz) + 1.7320508f * dl > d.half_skin;
+    }
   }
+  if (d.status && __any(over) && (threadIdx.x & 63) == 0) atomicOr(d.status, NB_STATUS_REBUILD);
   double t = block_tree_sum(s, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
@@ -354,9 +371,12 @@ __global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ par
 
 void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
                   const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
-                  float dt, float* factor) {
+                  float dt, float* factor, const HdDispArgs* disp) {
+  HdDispArgs d;
+  memset(&d, 0, sizeof(d));
+  if (disp) d = *disp;
   hipLaunchKernelGGL(k_ke_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, vx, vy, vz,
-                     mass, partials);
+                     mass, partials, d);
   HdNhcArgs na;
   memset(&na, 0, sizeof(na));
   if (nhc) na = *nhc;

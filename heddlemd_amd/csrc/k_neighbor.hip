@@ -21,10 +21,6 @@
 #include "hd_device.h"
 #include "hd_kernels.h"
 
-#define NB_STATUS_REBUILD 1u
-#define NB_STATUS_HIGHWATER 2u
-#define NB_STATUS_OVERFLOW 8u
-#define NB_STATUS_FORCE 0x100u
 
 __device__ __forceinline__ bool nb_rebuild_due(const unsigned* status) {
   return (*(volatile const unsigned*)status & (NB_STATUS_REBUILD | NB_STATUS_FORCE)) != 0u;
@@ -716,12 +712,13 @@ unsigned hd_voxel_scan_parts(unsigned nvox) { return cdiv(nvox, NB_SCAN_CHUNK); 
 // retire quickly, and on rebuild steps the loops cover the work.
 #define NB_GATED_WGS 512u
 
-void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a) {
+void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a, bool check) {
   const unsigned T = 256;
   unsigned gridN = min(cdiv(a.n, T), NB_GATED_WGS);
   if (gridN == 0) gridN = 1;
-  hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st, a.posq,
-                     a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
+  if (check)
+    hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st,
+                       a.posq, a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
   hipLaunchKernelGGL(k_voxel_keys, dim3(gridN), dim3(T), 0, st, a.posq, a.lattice, a.voxel_rank,
                      a.nv[0], a.nv[1], a.nv[2], a.atom_key, a.voxel_count, a.ref_pos,
                      a.ref_lattice, a.status, a.n);
