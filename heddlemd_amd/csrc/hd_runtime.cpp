@@ -173,6 +173,9 @@ struct hd_ctx {
 
   // graphs
   hipGraphExec_t graph_f = nullptr, graph_fev = nullptr;
+  // HD_GRAPH_STEPS consecutive steps without scalars in one graph: one
+  // launch, and no gap between graphs, per that many steps
+  hipGraphExec_t graph_fk = nullptr;
   unsigned graph_gen = 0;  // bumped whenever captured work goes stale
 
   std::vector<Stage> stages;  // event pool; the first n_stage_used are live
@@ -887,6 +890,7 @@ void hd_ctx_destroy(hd_ctx* c) {
   (void)hipStreamSynchronize(c->side);
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
   if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
+  if (c->graph_fk) (void)hipGraphExecDestroy(c->graph_fk);
   for (auto& s : c->stages) {
     (void)hipEventDestroy(s.a);
     (void)hipEventDestroy(s.b);
@@ -1606,7 +1610,8 @@ static void drop_graphs(hd_ctx* c) {
   ++c->graph_gen;
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
   if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
-  c->graph_f = c->graph_fev = nullptr;
+  if (c->graph_fk) (void)hipGraphExecDestroy(c->graph_fk);
+  c->graph_f = c->graph_fev = c->graph_fk = nullptr;
 }
 
 int hd_setup(hd_ctx* c) {
@@ -1871,10 +1876,13 @@ int hd_constrain_velocities(hd_ctx* c) {
   return HD_OK;
 }
 
-static int capture(hd_ctx* c, bool ev, hipGraphExec_t* out) {
+#define HD_GRAPH_STEPS 8
+
+// nsteps steps, the last with scalars when ev
+static int capture(hd_ctx* c, bool ev, hipGraphExec_t* out, int nsteps = 1) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(c, ev, false);
+  for (int k = 0; k < nsteps; ++k) enqueue_step(c, ev && k == nsteps - 1, false);
   HIPCHK(c, hipStreamEndCapture(c->stream, &g));
   HIPCHK(c, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
@@ -1896,6 +1904,17 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
     HIPCHK(c, hipMemsetAsync(c->d_sh_stats, 0, 6 * sizeof(unsigned), c->stream));
   for (int s = 0; s < n_steps; ++s) {
     bool ev = scalars_last && s == n_steps - 1;
+    // whole groups of steps without scalars as one graph launch
+    int plain_left = n_steps - s - (scalars_last ? 1 : 0);
+    if (c->use_graph && plain_left >= HD_GRAPH_STEPS) {
+      if (!c->graph_fk) {
+        int r = capture(c, false, &c->graph_fk, HD_GRAPH_STEPS);
+        if (r) return r;
+      }
+      HIPCHK(c, hipGraphLaunch(c->graph_fk, c->stream));
+      s += HD_GRAPH_STEPS - 1;
+      continue;
+    }
     if (c->use_graph) {
       hipGraphExec_t* gx = ev ? &c->graph_fev : &c->graph_f;
       if (!*gx) {
