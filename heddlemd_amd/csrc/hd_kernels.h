@@ -299,9 +299,6 @@ struct HdNhcArgs {
   double kt, g_dof;
   double* state;  // xi[chain], pxi[chain], q[chain]
 };
-// KE reduction in FP64 (fixed chunking and tree) into ke[0]; when nhc is
-// non-null also advances the chain by one half step and writes the
-// cumulative velocity factor (f32) to factor[0].
 // Neighbour status word bits (device-side rebuild request and list state).
 #define NB_STATUS_REBUILD 1u
 #define NB_STATUS_HIGHWATER 2u
@@ -320,6 +317,9 @@ struct HdDispArgs {
   float half_skin;
   unsigned* status;
 };
+// KE reduction in FP64 (fixed chunking and tree) into ke[0]; when nhc is
+// non-null also advances the chain by one half step and writes the
+// cumulative velocity factor (f32) to factor[0].
 void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, const float* vz,
                   const float* mass, double* partials, double* ke, const HdNhcArgs* nhc,
                   float dt, float* factor, const HdDispArgs* disp = nullptr);

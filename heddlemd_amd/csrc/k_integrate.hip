@@ -318,10 +318,16 @@ __global__ void __launch_bounds__(256) k_ke_partials(unsigned n, const float* __
 // One Nose-Hoover chain sub-step (nose_hoover_chain.rs:109-180), FP64.
 // The forward sweep's scale for link j reads pxi[j + 1] before that link is
 // updated again, i.e. the value the backward sweep left: its exp is the
-// backward sweep's, reused (chain length <= 16).
-__device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, double dt,
-                               double k, double g_dof, double kt) {
-  double sc[16];
+// backward sweep's, reused (chain length <= 16).  M > 0 fixes the chain
+// length at compile time, so the unrolled sweeps keep the chain in
+// registers; M == 0 takes the runtime m.
+template <int M>
+__device__ __forceinline__ double nhc_sub_step(int m_rt, double* xi, double* pxi,
+                                               const double* q, double dt, double k,
+                                               double g_dof, double kt) {
+  const int m = M > 0 ? M : m_rt;
+  double sc[M > 0 ? M : 16];
+#pragma unroll
   for (int j = m - 1; j >= 0; --j) {
     double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
     sc[j] = s;
@@ -332,7 +338,9 @@ __device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, 
   }
   double factor = exp(-dt / 2.0 * pxi[0] / q[0]);
   k *= factor * factor;
+#pragma unroll
   for (int j = 0; j < m; ++j) xi[j] += dt / 2.0 * pxi[j] / q[j];
+#pragma unroll
   for (int j = 0; j < m; ++j) {
     double s = sc[j];
     pxi[j] *= s;
@@ -341,6 +349,43 @@ __device__ double nhc_sub_step(int m, double* xi, double* pxi, const double* q, 
     pxi[j] *= s;
   }
   return factor;
+}
+
+// One chain half step: n_yoshida x n_resp sub-steps; returns the
+// cumulative velocity factor
+template <int M>
+__device__ __forceinline__ double nhc_half_step(const HdNhcArgs& nhc, float dt, double k,
+                                                double* xi, double* pxi, const double* q) {
+  double cum = 1.0;
+  for (int y = 0; y < nhc.n_yoshida; ++y)
+    for (int r = 0; r < nhc.n_resp; ++r) {
+      double delta = nhc.w[y] * (double)dt / (2.0 * nhc.n_resp);
+      double f = nhc_sub_step<M>(nhc.chain, xi, pxi, q, delta, 2.0 * k, nhc.g_dof, nhc.kt);
+      cum *= f;
+      k *= f * f;
+    }
+  return cum;
+}
+
+// The same with the chain state (xi, pxi, q) copied into registers for the
+// half step and xi, pxi stored back: the sweeps' loads and stores otherwise
+// go to memory one after another through the state buffer
+template <int M>
+__device__ double nhc_half_step_local(const HdNhcArgs& nhc, float dt, double k) {
+  double xi[M], pxi[M], q[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    xi[j] = nhc.state[j];
+    pxi[j] = nhc.state[M + j];
+    q[j] = nhc.state[2 * M + j];
+  }
+  double cum = nhc_half_step<M>(nhc, dt, k, xi, pxi, q);
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    nhc.state[j] = xi[j];
+    nhc.state[M + j] = pxi[j];
+  }
+  return cum;
 }
 
 __global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ partials,
@@ -354,18 +399,18 @@ __global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ par
   if (threadIdx.x != 0) return;
   ke[0] = k;
   if (!use_nhc) return;
-  int m = nhc.chain;
-  double* xi = nhc.state;
-  double* pxi = nhc.state + m;
-  const double* q = nhc.state + 2 * m;
-  double cum = 1.0;
-  for (int y = 0; y < nhc.n_yoshida; ++y)
-    for (int r = 0; r < nhc.n_resp; ++r) {
-      double delta = nhc.w[y] * (double)dt / (2.0 * nhc.n_resp);
-      double f = nhc_sub_step(m, xi, pxi, q, delta, 2.0 * k, nhc.g_dof, nhc.kt);
-      cum *= f;
-      k *= f * f;
+  double cum;
+  switch (nhc.chain) {
+    case 1: cum = nhc_half_step_local<1>(nhc, dt, k); break;
+    case 2: cum = nhc_half_step_local<2>(nhc, dt, k); break;
+    case 3: cum = nhc_half_step_local<3>(nhc, dt, k); break;
+    case 4: cum = nhc_half_step_local<4>(nhc, dt, k); break;
+    case 5: cum = nhc_half_step_local<5>(nhc, dt, k); break;
+    default: {
+      int m = nhc.chain;
+      cum = nhc_half_step<0>(nhc, dt, k, nhc.state, nhc.state + m, nhc.state + 2 * m);
     }
+  }
   factor[0] = (float)cum;
 }
 

@@ -533,6 +533,49 @@ def test_shake_polymer_chains_step_bit_exact(oracle):
     assert np.array_equal(v, v1)
 
 
+def test_nose_hoover_chain_matches_oracle(oracle):
+    # force-free gas (epsilon 0) started at 1.7x the bath temperature under
+    # a 20 fs chain: each step is the pre-kick chain half step (factor on
+    # v), the drift, and the post-force half step (factor on v), so the
+    # velocities and the chain state follow the oracle's chain alone.  The
+    # kinetic energies are FP64 sums in different orders on the two sides,
+    # so a factor may round to a neighbouring f32 value: velocities within
+    # 1e-6 relative, chain term and KE within 1e-9 relative.
+    say("case: argon 4000 force-free NHC 60 steps")
+    s = _jittered_argon(n_atoms=4000, seed=3)
+    s.lj.epsilon[:] = 0.0
+    kt = U.temperature(300.0)
+    tau = U.time(2e-14)
+    dt = np.float32(U.time(2e-15))
+    rng = np.random.default_rng(17)
+    v0 = (rng.normal(size=(3, s.n)) * np.sqrt(1.7 * kt / s.mass)).astype(np.float32)
+    steps = 60
+    e = engine_for(s, dt=float(dt), velocities=v0, nhc_kt=kt, nhc_tau=tau)
+    e.force_eval(scalars=False)
+    e.step(steps, scalars_last=True)
+    v1 = e.velocities()
+    sc = e.scalars()
+    assert np.abs(e.forces()).max() == 0.0
+    e.close()
+    st = oracle.NHCState(s.n, 0, kt, tau)
+    v = v0.copy()
+    for _ in range(steps):
+        f, _ = st.half_step(float(dt), oracle.kinetic_energy(v, s.mass))
+        v = v * np.float32(f)
+        f, _ = st.half_step(float(dt), oracle.kinetic_energy(v, s.mass))
+        v = v * np.float32(f)
+    k_ref = oracle.kinetic_energy(v, s.mass)
+    k0 = oracle.kinetic_energy(v0, s.mass)
+    same = np.mean(v1 == v)
+    say(f"  KE {k0:.6e} -> {sc['ke']:.6e} (oracle {k_ref:.6e}); chain {sc['nhc_chain']:.9e} "
+        f"(oracle {st.conserved_extra():.9e}); identical velocity words {same:.4f}")
+    # the chain moved the gas well away from its start
+    assert abs(k_ref / k0 - 1.0) > 0.1
+    assert np.all(np.abs(v1 - v) <= 1e-6 * np.abs(v))
+    assert sc["ke"] == pytest.approx(k_ref, rel=1e-9)
+    assert sc["nhc_chain"] == pytest.approx(st.conserved_extra(), rel=1e-9)
+
+
 def test_split_precision_dft_matches_fp64(oracle):
     # the FP16 split matrix-core DFT path against the FP64 oracle, next to
     # the FP32 radix FFT path on the same state: potential grid, reciprocal
