@@ -332,6 +332,14 @@ struct HdBarostatArgs {
 void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float* lattice,
                         float* mu, double* diag, unsigned long long* counter,
                         const HdBarostatArgs& b, float4* posq, unsigned n);
+// The scalar reduction of hd_launch_sum2_final whose last thread also
+// computes the C-rescale mu from k[0] and the reduced virial (out[1]);
+// hd_launch_scale_positions then completes the barostat step
+void hd_launch_sum2_final_crescale(hipStream_t st, const double* partials, unsigned np,
+                                   double* out, const double* k, float* lattice, float* mu,
+                                   double* diag, unsigned long long* counter,
+                                   const HdBarostatArgs& b);
+void hd_launch_scale_positions(hipStream_t st, float4* posq, const float* mu, unsigned n);
 // Pairs within r_c (both atoms valid, not excluded) in the current list.
 void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out);
 void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count);

@@ -476,7 +476,12 @@ bool has_constraints(const hd_ctx* c) { return !c->h_groups.empty() || has_shake
 // the scratch pair scal[6], scal[7] so the last logged energy stays intact
 // disp_done: the displacement check already ran on this step's final
 // positions (folded into the step's last KE pass)
-void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = false) {
+HdBarostatArgs baro_args(hd_ctx* c);
+
+// baro_mu: a C-rescale step follows (ev != 0): the scalar reduction also
+// computes its mu, and enqueue_crescale is called with mu_done
+void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = false,
+                    bool baro_mu = false) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
   stage_begin(c, "neighbor_update", timed);
@@ -583,8 +588,16 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   stage_end(c, timed);
   if (ev) {
     stage_begin(c, "reduce_scalars", timed);
-    hd_launch_sum2_final(st, c->fin_partials, hd_finalize_blocks((unsigned)c->n, c->have_spme),
-                         c->scal + (ev == 1 ? 1 : 6));
+    unsigned np = hd_finalize_blocks((unsigned)c->n, c->have_spme);
+    if (baro_mu) {
+      // C-rescale: K of the constrained, not yet kicked velocities (scal[3])
+      HdBarostatArgs b = baro_args(c);
+      hd_launch_sum2_final_crescale(st, c->fin_partials, np, c->scal + (ev == 1 ? 1 : 6),
+                                    c->scal + 3, c->d_lattice, c->baro_mu, c->baro_diag,
+                                    c->baro_counter, b);
+    } else {
+      hd_launch_sum2_final(st, c->fin_partials, np, c->scal + (ev == 1 ? 1 : 6));
+    }
     stage_end(c, timed);
   }
 }
@@ -715,11 +728,7 @@ HdSettleArgs settle_args(hd_ctx* c) {
   return sa;
 }
 
-void enqueue_crescale(hd_ctx* c, int ev, bool timed) {
-  // C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): K of the
-  // constrained, not yet kicked velocities (scal[3]) and W of this
-  // evaluation (scal[2], or scal[7] on a virial-only evaluation);
-  // positions are scaled after the kick
+HdBarostatArgs baro_args(hd_ctx* c) {
   HdBarostatArgs b;
   memset(&b, 0, sizeof(b));
   b.seed_lo = (unsigned)(c->baro_seed & 0xFFFFFFFFull);
@@ -730,9 +739,23 @@ void enqueue_crescale(hd_ctx* c, int ev, bool timed) {
   b.kt = c->baro_kt;
   b.dt = c->dt;
   b.mu3_min = 1e-18;
+  return b;
+}
+
+// mu_done: the scalar reduction of this evaluation has computed mu
+void enqueue_crescale(hd_ctx* c, int ev, bool timed, bool mu_done = false) {
+  // C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): K of the
+  // constrained, not yet kicked velocities (scal[3]) and W of this
+  // evaluation (scal[2], or scal[7] on a virial-only evaluation);
+  // positions are scaled after the kick
   stage_begin(c, "c_rescale", timed);
-  hd_launch_crescale(c->stream, c->scal + 3, c->scal + (ev == 1 ? 2 : 7), c->d_lattice,
-                     c->baro_mu, c->baro_diag, c->baro_counter, b, c->posq, (unsigned)c->n);
+  if (mu_done) {
+    hd_launch_scale_positions(c->stream, c->posq, c->baro_mu, (unsigned)c->n);
+  } else {
+    HdBarostatArgs b = baro_args(c);
+    hd_launch_crescale(c->stream, c->scal + 3, c->scal + (ev == 1 ? 2 : 7), c->d_lattice,
+                       c->baro_mu, c->baro_diag, c->baro_counter, b, c->posq, (unsigned)c->n);
+  }
   stage_end(c, timed);
 }
 
@@ -767,8 +790,8 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
                  0.f, nullptr);
   // a barostat consumes the virial every step; the energies only when logged
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
-  enqueue_forces(c, mode, true, timed);
-  if (c->have_baro) enqueue_crescale(c, mode, timed);
+  enqueue_forces(c, mode, true, timed, false, c->have_baro);
+  if (c->have_baro) enqueue_crescale(c, mode, timed, true);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
@@ -835,8 +858,8 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   }
   // a barostat consumes the virial every step; the energies only when logged
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
-  enqueue_forces(c, mode, true, timed, disp_done);
-  if (c->have_baro) enqueue_crescale(c, mode, timed);
+  enqueue_forces(c, mode, true, timed, disp_done, c->have_baro);
+  if (c->have_baro) enqueue_crescale(c, mode, timed, true);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }

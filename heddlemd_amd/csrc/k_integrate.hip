@@ -326,8 +326,9 @@ __device__ __forceinline__ double nhc_sub_step(int m_rt, double* xi, double* pxi
                                                const double* q, double dt, double k,
                                                double g_dof, double kt) {
   const int m = M > 0 ? M : m_rt;
+  constexpr int UNROLL = M > 0 ? 16 : 1;  // full unroll of the fixed lengths only
   double sc[M > 0 ? M : 16];
-#pragma unroll
+#pragma unroll UNROLL
   for (int j = m - 1; j >= 0; --j) {
     double s = (j == m - 1) ? 1.0 : exp(-dt / 8.0 * pxi[j + 1] / q[j + 1]);
     sc[j] = s;
@@ -338,9 +339,9 @@ __device__ __forceinline__ double nhc_sub_step(int m_rt, double* xi, double* pxi
   }
   double factor = exp(-dt / 2.0 * pxi[0] / q[0]);
   k *= factor * factor;
-#pragma unroll
+#pragma unroll UNROLL
   for (int j = 0; j < m; ++j) xi[j] += dt / 2.0 * pxi[j] / q[j];
-#pragma unroll
+#pragma unroll UNROLL
   for (int j = 0; j < m; ++j) {
     double s = sc[j];
     pxi[j] *= s;
@@ -1332,14 +1333,15 @@ void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLa
 // One thread: P = (2K + W)/(3V), mu^3 = 1 - kappa (dt/tau)(P0 - P)
 // + sqrt(2 kappa kT dt/(tau V)) r, clamped, lattice *= (f32) mu; the
 // draw counter advances by one.  K and W are read from FP64 scalars.
-__global__ void k_crescale_mu(const double* __restrict__ k, const double* __restrict__ w,
-                              float* lattice, float* mu_out, double* diag,
-                              unsigned long long* counter, HdBarostatArgs b) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One thread: mu from K and W, the lattice scaled, the diagnostics and
+// the draw counter advanced
+__device__ void crescale_mu_update(double k, double w, float* lattice, float* mu_out,
+                                   double* diag, unsigned long long* counter,
+                                   const HdBarostatArgs& b) {
   unsigned long long ctr = *counter;
   double lx = lattice[0], ly = lattice[1], lz = lattice[2];
   double v_pre = lx * ly * lz;
-  double p = (2.0 * k[0] + w[0]) / (3.0 * v_pre);
+  double p = (2.0 * k + w) / (3.0 * v_pre);
   double r = philox_gaussian_f64(b.seed_lo, b.seed_hi, (unsigned)(ctr & 0xFFFFFFFFull),
                                  (unsigned)(ctr >> 32), 0u, 0u);
   double det = -b.compressibility * (b.dt / b.tau) * (b.pressure - p);
@@ -1355,6 +1357,36 @@ __global__ void k_crescale_mu(const double* __restrict__ k, const double* __rest
   diag[1] = v_post;
   diag[2] += b.pressure * (v_post - v_pre);
   *counter = ctr + 1ull;
+}
+
+__global__ void k_crescale_mu(const double* __restrict__ k, const double* __restrict__ w,
+                              float* lattice, float* mu_out, double* diag,
+                              unsigned long long* counter, HdBarostatArgs b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  crescale_mu_update(k[0], w[0], lattice, mu_out, diag, counter, b);
+}
+
+// k_sum2_final's reduction, then its thread 0 runs the C-rescale update
+// with the virial it has just reduced (the same value k_crescale_mu would
+// read from out[1])
+__global__ void __launch_bounds__(256) k_sum2_final_crescale(
+    const double* __restrict__ partials, unsigned np, double* __restrict__ out,
+    const double* __restrict__ k, float* lattice, float* mu_out, double* diag,
+    unsigned long long* counter, HdBarostatArgs b) {
+  __shared__ double sh[256];
+  double sa = 0.0, sb = 0.0;
+  for (unsigned i = threadIdx.x; i < np; i += blockDim.x) {
+    sa += partials[2 * i];
+    sb += partials[2 * i + 1];
+  }
+  double ta = block_tree_sum(sa, sh);
+  __syncthreads();
+  double tb = block_tree_sum(sb, sh);
+  if (threadIdx.x == 0) {
+    out[0] = ta;
+    out[1] = tb;
+    crescale_mu_update(k[0], tb, lattice, mu_out, diag, counter, b);
+  }
 }
 
 __global__ void k_scale_positions(float4* __restrict__ posq, const float* __restrict__ mu,
@@ -1373,8 +1405,20 @@ void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float*
                         float* mu, double* diag, unsigned long long* counter,
                         const HdBarostatArgs& b, float4* posq, unsigned n) {
   hipLaunchKernelGGL(k_crescale_mu, dim3(1), dim3(64), 0, st, k, w, lattice, mu, diag, counter, b);
+  hd_launch_scale_positions(st, posq, mu, n);
+}
+
+void hd_launch_scale_positions(hipStream_t st, float4* posq, const float* mu, unsigned n) {
   if (n)
     hipLaunchKernelGGL(k_scale_positions, dim3((n + 255) / 256), dim3(256), 0, st, posq, mu, n);
+}
+
+void hd_launch_sum2_final_crescale(hipStream_t st, const double* partials, unsigned np,
+                                   double* out, const double* k, float* lattice, float* mu,
+                                   double* diag, unsigned long long* counter,
+                                   const HdBarostatArgs& b) {
+  hipLaunchKernelGGL(k_sum2_final_crescale, dim3(1), dim3(256), 0, st, partials, np, out, k,
+                     lattice, mu, diag, counter, b);
 }
 
 // In-bounds check of the DPP wave_rol:1 lane layout: out[l] = value of
