@@ -327,14 +327,10 @@ struct HdBarostatArgs {
   unsigned seed_lo, seed_hi;
   double pressure, tau, compressibility, kt, dt, mu3_min;
 };
-// C-rescale: mu from K, W (FP64 scalars) and a Philox normal; scales the
-// device lattice and then every position by mu.
-void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float* lattice,
-                        float* mu, double* diag, unsigned long long* counter,
-                        const HdBarostatArgs& b, float4* posq, unsigned n);
-// The scalar reduction of hd_launch_sum2_final whose last thread also
-// computes the C-rescale mu from k[0] and the reduced virial (out[1]);
-// hd_launch_scale_positions then completes the barostat step
+// C-rescale: the scalar reduction of hd_launch_sum2_final whose last
+// thread also computes mu from K = k[0], W = the reduced virial (out[1])
+// and a Philox normal, and scales the device lattice by mu;
+// hd_launch_scale_positions then scales every position by mu
 void hd_launch_sum2_final_crescale(hipStream_t st, const double* partials, unsigned np,
                                    double* out, const double* k, float* lattice, float* mu,
                                    double* diag, unsigned long long* counter,

@@ -479,7 +479,7 @@ bool has_constraints(const hd_ctx* c) { return !c->h_groups.empty() || has_shake
 HdBarostatArgs baro_args(hd_ctx* c);
 
 // baro_mu: a C-rescale step follows (ev != 0): the scalar reduction also
-// computes its mu, and enqueue_crescale is called with mu_done
+// computes its mu, and enqueue_crescale then scales the positions
 void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = false,
                     bool baro_mu = false) {
   hipStream_t st = c->stream;
@@ -742,20 +742,13 @@ HdBarostatArgs baro_args(hd_ctx* c) {
   return b;
 }
 
-// mu_done: the scalar reduction of this evaluation has computed mu
-void enqueue_crescale(hd_ctx* c, int ev, bool timed, bool mu_done = false) {
-  // C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): K of the
-  // constrained, not yet kicked velocities (scal[3]) and W of this
-  // evaluation (scal[2], or scal[7] on a virial-only evaluation);
-  // positions are scaled after the kick
+// C-rescale (src/runner.rs:2664-2668, barostat.cu:144-232): mu comes from
+// K of the constrained, not yet kicked velocities (scal[3]) and W of this
+// evaluation, computed by the scalar reduction of enqueue_forces (baro_mu);
+// positions are scaled after the kick
+void enqueue_crescale(hd_ctx* c, bool timed) {
   stage_begin(c, "c_rescale", timed);
-  if (mu_done) {
-    hd_launch_scale_positions(c->stream, c->posq, c->baro_mu, (unsigned)c->n);
-  } else {
-    HdBarostatArgs b = baro_args(c);
-    hd_launch_crescale(c->stream, c->scal + 3, c->scal + (ev == 1 ? 2 : 7), c->d_lattice,
-                       c->baro_mu, c->baro_diag, c->baro_counter, b, c->posq, (unsigned)c->n);
-  }
+  hd_launch_scale_positions(c->stream, c->posq, c->baro_mu, (unsigned)c->n);
   stage_end(c, timed);
 }
 
@@ -791,7 +784,7 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   // a barostat consumes the virial every step; the energies only when logged
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
   enqueue_forces(c, mode, true, timed, false, c->have_baro);
-  if (c->have_baro) enqueue_crescale(c, mode, timed, true);
+  if (c->have_baro) enqueue_crescale(c, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
@@ -859,7 +852,7 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   // a barostat consumes the virial every step; the energies only when logged
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
   enqueue_forces(c, mode, true, timed, disp_done, c->have_baro);
-  if (c->have_baro) enqueue_crescale(c, mode, timed, true);
+  if (c->have_baro) enqueue_crescale(c, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }

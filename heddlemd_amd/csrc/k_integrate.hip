@@ -1359,16 +1359,8 @@ __device__ void crescale_mu_update(double k, double w, float* lattice, float* mu
   *counter = ctr + 1ull;
 }
 
-__global__ void k_crescale_mu(const double* __restrict__ k, const double* __restrict__ w,
-                              float* lattice, float* mu_out, double* diag,
-                              unsigned long long* counter, HdBarostatArgs b) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  crescale_mu_update(k[0], w[0], lattice, mu_out, diag, counter, b);
-}
-
 // k_sum2_final's reduction, then its thread 0 runs the C-rescale update
-// with the virial it has just reduced (the same value k_crescale_mu would
-// read from out[1])
+// (mu from K = k[0] and the virial it has just reduced)
 __global__ void __launch_bounds__(256) k_sum2_final_crescale(
     const double* __restrict__ partials, unsigned np, double* __restrict__ out,
     const double* __restrict__ k, float* lattice, float* mu_out, double* diag,
@@ -1399,13 +1391,6 @@ __global__ void k_scale_positions(float4* __restrict__ posq, const float* __rest
   p.y *= f;
   p.z *= f;
   posq[i] = p;
-}
-
-void hd_launch_crescale(hipStream_t st, const double* k, const double* w, float* lattice,
-                        float* mu, double* diag, unsigned long long* counter,
-                        const HdBarostatArgs& b, float4* posq, unsigned n) {
-  hipLaunchKernelGGL(k_crescale_mu, dim3(1), dim3(64), 0, st, k, w, lattice, mu, diag, counter, b);
-  hd_launch_scale_positions(st, posq, mu, n);
 }
 
 void hd_launch_scale_positions(hipStream_t st, float4* posq, const float* mu, unsigned n) {
