@@ -19,6 +19,15 @@ struct HdLattice {
   float lx, ly, lz, xy, xz, yz;
 };
 
+// Workgroup index remap for the 8 XCDs of MI355X, which receive
+// workgroups in turn (b to XCD b % 8): the result gives XCD x the
+// contiguous range [x * per, (x + 1) * per) with per = nb / 8, and leaves
+// the nb % 8 tail in place.  A bijection on [0, nb).
+__device__ __forceinline__ unsigned hd_xcd_block(unsigned b, unsigned nb) {
+  unsigned per = nb >> 3;
+  return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
+}
+
 __device__ __forceinline__ HdLattice hd_load_lattice(const float* L) {
   HdLattice l;
   l.lx = L[0]; l.ly = L[1]; l.lz = L[2];

@@ -522,9 +522,14 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
   //
   // With a.partials set (energy or virial launches), the workgroup also sums
   // its slots' final per-atom energy and virial in FP64, in slot order, and
-  // writes them to partials[2 * blockIdx.x + {0, 1}]; every thread reaches
-  // that reduction.
-  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  // writes them to partials[2 * wg + {0, 1}]; every thread reaches that
+  // reduction.
+  //
+  // Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b % 8);
+  // wg gives each XCD one contiguous range of slot blocks (atoms close in
+  // space), so the grid values a range gathers stay in that XCD's L2.
+  unsigned wg = hd_xcd_block(blockIdx.x, gridDim.x);
+  unsigned t = wg * blockDim.x + threadIdx.x;
   unsigned s = GATHER ? (t >> 2) : t;
   unsigned q = GATHER ? (t & 3u) : 0u;
   bool live = s < a.n;  // whole quads
@@ -639,8 +644,8 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     __syncthreads();
     double sv = block_tree_sum((double)vir, sh);
     if (threadIdx.x == 0) {
-      a.partials[2 * blockIdx.x] = sp;
-      a.partials[2 * blockIdx.x + 1] = sv;
+      a.partials[2 * wg] = sp;
+      a.partials[2 * wg + 1] = sv;
     }
   }
 }
