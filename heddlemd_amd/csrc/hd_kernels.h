@@ -191,6 +191,10 @@ struct HdSettleArgs {
   const float* lattice;
 };
 void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a);
+// hd_launch_kick_drift (f32 state) followed by hd_launch_settle_positions in
+// one kernel; valid when the SETTLE groups hold every atom
+void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
+                                 const HdSettleArgs& a);
 void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool accumulate_virial);
 
 // General SHAKE / RATTLE groups (kernels/shake.cu): atoms goff[g] ..

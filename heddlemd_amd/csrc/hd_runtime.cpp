@@ -825,10 +825,18 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   ia.lattice = c->d_lattice;
   ia.factor = c->have_nhc ? c->factors + 0 : nullptr;
   ia.snap = has_constraints(c) ? c->snap : nullptr;
-  stage_begin(c, "vv_kick_drift", timed);
-  hd_launch_kick_drift(st, ia);
-  stage_end(c, timed);
-  enqueue_constrain_positions(c, timed);
+  if (!c->lossless && !has_shake(c) && c->h_groups.size() == (size_t)c->n) {
+    // every atom in a SETTLE water (3 indices per water, disjoint): the
+    // drift and SETTLE in one kernel
+    stage_begin(c, "vv_kick_drift_settle", timed);
+    hd_launch_kick_drift_settle(st, ia, settle_args(c));
+    stage_end(c, timed);
+  } else {
+    stage_begin(c, "vv_kick_drift", timed);
+    hd_launch_kick_drift(st, ia);
+    stage_end(c, timed);
+    enqueue_constrain_positions(c, timed);
+  }
   enqueue_constrain_velocities(c, timed);
   // the positions are final for the step: the KE pass also runs the
   // neighbour displacement check

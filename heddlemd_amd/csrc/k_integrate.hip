@@ -81,13 +81,13 @@ void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a) {
 // ---------------------------------------------------------------------
 __device__ __forceinline__ float settle_csqrt(float x) { return sqrtf(x > 0.f ? x : 0.f); }
 
-__global__ void k_settle_positions(HdSettleArgs a) {
-  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_groups) return;
-  HdLattice L = hd_load_lattice(a.lattice);
-  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
-  float4 p[3] = {a.posq[idx[0]], a.posq[idx[1]], a.posq[idx[2]]};
-  float4 q[3] = {a.snap[idx[0]], a.snap[idx[1]], a.snap[idx[2]]};
+// SETTLE of one water on positions: p the unconstrained positions, q the
+// snapshot before the drift; the velocities are read from memory, or with
+// VREG taken from vr (the values the drift has just computed)
+template <bool VREG>
+__device__ __forceinline__ void settle_group(const HdSettleArgs& a, const HdLattice& L,
+                                             const unsigned (&idx)[3], const float4 (&p)[3],
+                                             const float4 (&q)[3], const float (&vr)[3][3]) {
   float cur[3][3] = {{p[0].x, p[0].y, p[0].z}, {p[1].x, p[1].y, p[1].z}, {p[2].x, p[2].y, p[2].z}};
   float snp[3][3] = {{q[0].x, q[0].y, q[0].z}, {q[1].x, q[1].y, q[1].z}, {q[2].x, q[2].y, q[2].z}};
   float xb0 = snp[1][0] - snp[0][0], yb0 = snp[1][1] - snp[0][1], zb0 = snp[1][2] - snp[0][2];
@@ -166,12 +166,73 @@ __global__ void k_settle_positions(HdSettleArgs a) {
     float c1 = (com[1] + rcom[k][1]) - unc[k][1];
     float c2 = (com[2] + rcom[k][2]) - unc[k][2];
     unsigned id = idx[k];
-    a.vx[id] += c0 * idt;
-    a.vy[id] += c1 * idt;
-    a.vz[id] += c2 * idt;
+    float vx = VREG ? vr[k][0] : a.vx[id];
+    float vy = VREG ? vr[k][1] : a.vy[id];
+    float vz = VREG ? vr[k][2] : a.vz[id];
+    a.vx[id] = vx + c0 * idt;
+    a.vy[id] = vy + c1 * idt;
+    a.vz[id] = vz + c2 * idt;
     a.posq[id] = make_float4(cur[k][0] + c0, cur[k][1] + c1, cur[k][2] + c2, p[k].w);
     a.w_cons[id] = mat[k] * idt2 * (c0 * rcom[k][0] + c1 * rcom[k][1] + c2 * rcom[k][2]);
   }
+}
+
+__global__ void k_settle_positions(HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
+  float4 p[3] = {a.posq[idx[0]], a.posq[idx[1]], a.posq[idx[2]]};
+  float4 q[3] = {a.snap[idx[0]], a.snap[idx[1]], a.snap[idx[2]]};
+  float vr[3][3] = {};
+  settle_group<false>(a, L, idx, p, q, vr);
+}
+
+// VV rescale + kick + drift + wrap fused with SETTLE on positions, for a
+// system whose atoms all belong to SETTLE groups (no SHAKE, f32 state): one
+// thread per water runs k_kick_drift<false, true>'s arithmetic on its three
+// atoms and then k_settle_positions' on the results held in registers, so
+// the drifted positions and velocities and the snapshot skip the round trip
+// through memory.  The same operations in the same order as the two
+// kernels: bit-identical results.
+__global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  float f = ia.factor ? ia.factor[0] : 1.0f;
+  float half_dt = ia.dt * 0.5f;
+  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
+  float4 p[3], q[3];
+  float vr[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned i = idx[k];
+    float m = ia.mass[i];
+    float ax = ia.fx[i] / m, ay = ia.fy[i] / m, az = ia.fz[i] / m;
+    float4 pq = ia.posq[i];
+    q[k] = pq;
+    float vx = ia.vx[i] * f, vy = ia.vy[i] * f, vz = ia.vz[i] * f;
+    int nx = ia.ix[i], ny = ia.iy[i], nz = ia.iz[i];
+    vx = vx + ax * half_dt;
+    vy = vy + ay * half_dt;
+    vz = vz + az * half_dt;
+    float px = pq.x + vx * ia.dt;
+    float py = pq.y + vy * ia.dt;
+    float pz = pq.z + vz * ia.dt;
+    int ka, kb, kc;
+    hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+    ia.ix[i] = nx + ka; ia.iy[i] = ny + kb; ia.iz[i] = nz + kc;
+    p[k] = make_float4(px, py, pz, pq.w);
+    vr[k][0] = vx; vr[k][1] = vy; vr[k][2] = vz;
+  }
+  settle_group<true>(a, L, idx, p, q, vr);
+}
+
+void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
+                                 const HdSettleArgs& a) {
+  if (!a.n_groups) return;
+  hipLaunchKernelGGL(k_kick_drift_settle, dim3((a.n_groups + 127) / 128), dim3(128), 0, st, ia,
+                     a);
 }
 
 template <bool ACC>

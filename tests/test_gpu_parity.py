@@ -229,6 +229,51 @@ def test_repeat_runs_and_graph_replay_are_byte_identical():
     assert sa == sb == sc
 
 
+def test_settle_water_step_matches_oracle(oracle):
+    # one VV step of 8192 SETTLE waters (every atom in a water, so the drift
+    # and SETTLE run as one kernel): the f32 kick + drift is bit-exact, and
+    # SETTLE on positions and velocities follows the FP64 oracle to f32
+    # resolution (2e-5 Bohr on positions ~ 60 Bohr; the velocity resets are
+    # position resets / dt, so 4e-5 / dt on velocities)
+    say("case: water 8192 SETTLE step against the oracle")
+    s = systems.config_water_8192()
+    rng = np.random.default_rng(29)
+    v0 = rng.normal(0, 2e-4, size=(3, s.n)).astype(np.float32)
+    dt = np.float32(U.time(2e-15))
+    e = engine_for(s, dt=float(dt), velocities=v0)
+    e.use_graph(False)
+    e.force_eval(scalars=False)
+    f0 = e.forces().T.copy()
+    posq0, img0 = e.positions(images=True)
+    e.step(1)
+    posq1, img1 = e.positions(images=True)
+    v1 = e.velocities()
+    f1 = e.forces().T.copy()
+    e.close()
+    p = posq0.copy()
+    img = img0.T.copy()
+    v = v0.copy()
+    L32 = s.lattice.astype(np.float32)
+    oracle.vv_kick_drift_f32(p, img, v, f0, s.mass, L32, dt)
+    assert np.array_equal(img.T, img1)
+    d_oh, d_hh = s.water_geom
+    m_o, m_h = float(s.mass[s.water_groups[0, 0]]), float(s.mass[s.water_groups[0, 1]])
+    geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
+    pos = p[:, :3].astype(np.float64)
+    vel = v.T.astype(np.float64).copy()
+    oracle.settle_positions(s.water_groups, pos, vel, posq0[:, :3].astype(np.float64), geom,
+                            m_o, m_h, s.lattice, float(dt))
+    oracle.settle_velocities(s.water_groups, pos, vel, m_o, m_h, s.lattice, float(dt))
+    vel += 0.5 * float(dt) * f1.T.astype(np.float64) / s.mass[:, None].astype(np.float64)
+    dp = np.abs(posq1[:, :3] - pos).max()
+    dv = np.abs(v1.T - vel).max()
+    say(f"  max |dp| {dp:.2e} Bohr, max |dv| {dv:.2e} (|v| ~ {np.abs(vel).max():.1e})")
+    # SETTLE moved the atoms by far more than the tolerance
+    assert np.abs(p[:, :3] - pos).max() > 1e-3
+    assert dp <= 2e-5
+    assert dv <= 4e-5 / float(dt)
+
+
 def test_settle_keeps_water_rigid():
     say("case: water 8192 rigidity, 40 steps")
     s = systems.config_water_8192()
