@@ -191,8 +191,9 @@ struct HdSettleArgs {
   const float* lattice;
 };
 void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a);
-// hd_launch_kick_drift (f32 state) followed by hd_launch_settle_positions in
-// one kernel; valid when the SETTLE groups hold every atom
+// hd_launch_kick_drift (f32 state), hd_launch_settle_positions and
+// hd_launch_settle_velocities (with the virial) in one kernel; valid when
+// the SETTLE groups hold every atom
 void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
                                  const HdSettleArgs& a);
 void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool accumulate_virial);

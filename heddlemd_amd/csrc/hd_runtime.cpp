@@ -827,7 +827,7 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   ia.snap = has_constraints(c) ? c->snap : nullptr;
   if (!c->lossless && !has_shake(c) && c->h_groups.size() == (size_t)c->n) {
     // every atom in a SETTLE water (3 indices per water, disjoint): the
-    // drift and SETTLE in one kernel
+    // drift and SETTLE on positions and velocities in one kernel
     stage_begin(c, "vv_kick_drift_settle", timed);
     hd_launch_kick_drift_settle(st, ia, settle_args(c));
     stage_end(c, timed);
@@ -836,8 +836,8 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
     hd_launch_kick_drift(st, ia);
     stage_end(c, timed);
     enqueue_constrain_positions(c, timed);
+    enqueue_constrain_velocities(c, timed);
   }
-  enqueue_constrain_velocities(c, timed);
   // the positions are final for the step: the KE pass also runs the
   // neighbour displacement check
   HdDispArgs disp;

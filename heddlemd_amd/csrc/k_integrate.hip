@@ -83,11 +83,16 @@ __device__ __forceinline__ float settle_csqrt(float x) { return sqrtf(x > 0.f ? 
 
 // SETTLE of one water on positions: p the unconstrained positions, q the
 // snapshot before the drift; the velocities are read from memory, or with
-// VREG taken from vr (the values the drift has just computed)
-template <bool VREG>
+// VREG taken from vr (the values the drift has just computed).  The new
+// positions, velocities and constraint virials are stored, and also
+// returned in po, vo, wo; without VSTORE the velocities and virials are
+// only returned (a velocity stage that follows stores them)
+template <bool VREG, bool VSTORE>
 __device__ __forceinline__ void settle_group(const HdSettleArgs& a, const HdLattice& L,
                                              const unsigned (&idx)[3], const float4 (&p)[3],
-                                             const float4 (&q)[3], const float (&vr)[3][3]) {
+                                             const float4 (&q)[3], const float (&vr)[3][3],
+                                             float4 (&po)[3], float (&vo)[3][3],
+                                             float (&wo)[3]) {
   float cur[3][3] = {{p[0].x, p[0].y, p[0].z}, {p[1].x, p[1].y, p[1].z}, {p[2].x, p[2].y, p[2].z}};
   float snp[3][3] = {{q[0].x, q[0].y, q[0].z}, {q[1].x, q[1].y, q[1].z}, {q[2].x, q[2].y, q[2].z}};
   float xb0 = snp[1][0] - snp[0][0], yb0 = snp[1][1] - snp[0][1], zb0 = snp[1][2] - snp[0][2];
@@ -169,11 +174,18 @@ __device__ __forceinline__ void settle_group(const HdSettleArgs& a, const HdLatt
     float vx = VREG ? vr[k][0] : a.vx[id];
     float vy = VREG ? vr[k][1] : a.vy[id];
     float vz = VREG ? vr[k][2] : a.vz[id];
-    a.vx[id] = vx + c0 * idt;
-    a.vy[id] = vy + c1 * idt;
-    a.vz[id] = vz + c2 * idt;
-    a.posq[id] = make_float4(cur[k][0] + c0, cur[k][1] + c1, cur[k][2] + c2, p[k].w);
-    a.w_cons[id] = mat[k] * idt2 * (c0 * rcom[k][0] + c1 * rcom[k][1] + c2 * rcom[k][2]);
+    vo[k][0] = vx + c0 * idt;
+    vo[k][1] = vy + c1 * idt;
+    vo[k][2] = vz + c2 * idt;
+    po[k] = make_float4(cur[k][0] + c0, cur[k][1] + c1, cur[k][2] + c2, p[k].w);
+    wo[k] = mat[k] * idt2 * (c0 * rcom[k][0] + c1 * rcom[k][1] + c2 * rcom[k][2]);
+    a.posq[id] = po[k];
+    if (VSTORE) {
+      a.vx[id] = vo[k][0];
+      a.vy[id] = vo[k][1];
+      a.vz[id] = vo[k][2];
+      a.w_cons[id] = wo[k];
+    }
   }
 }
 
@@ -184,17 +196,25 @@ __global__ void k_settle_positions(HdSettleArgs a) {
   unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
   float4 p[3] = {a.posq[idx[0]], a.posq[idx[1]], a.posq[idx[2]]};
   float4 q[3] = {a.snap[idx[0]], a.snap[idx[1]], a.snap[idx[2]]};
-  float vr[3][3] = {};
-  settle_group<false>(a, L, idx, p, q, vr);
+  float vr[3][3] = {}, vo[3][3], wo[3];
+  float4 po[3];
+  settle_group<false, true>(a, L, idx, p, q, vr, po, vo, wo);
 }
 
-// VV rescale + kick + drift + wrap fused with SETTLE on positions, for a
-// system whose atoms all belong to SETTLE groups (no SHAKE, f32 state): one
-// thread per water runs k_kick_drift<false, true>'s arithmetic on its three
-// atoms and then k_settle_positions' on the results held in registers, so
-// the drifted positions and velocities and the snapshot skip the round trip
-// through memory.  The same operations in the same order as the two
-// kernels: bit-identical results.
+template <bool ACC, bool REG>
+__device__ __forceinline__ void settle_vel_group(const HdSettleArgs& a, const HdLattice& L,
+                                                 const unsigned (&ids)[3],
+                                                 const float4 (&pin)[3],
+                                                 const float (&vin)[3][3], const float (&win)[3]);
+
+// VV rescale + kick + drift + wrap fused with SETTLE on positions and then
+// on velocities (with the constraint virial), for a system whose atoms all
+// belong to SETTLE groups (no SHAKE, f32 state): one thread per water runs
+// k_kick_drift<false, true>'s arithmetic on its three atoms, then
+// k_settle_positions' and k_settle_velocities<true>'s on the values held
+// in registers, so the intermediate positions, velocities, virials and the
+// snapshot skip the round trips through memory.  The same operations in
+// the same order as the three kernels: bit-identical results.
 __global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
@@ -225,7 +245,10 @@ __global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
     p[k] = make_float4(px, py, pz, pq.w);
     vr[k][0] = vx; vr[k][1] = vy; vr[k][2] = vz;
   }
-  settle_group<true>(a, L, idx, p, q, vr);
+  float4 po[3];
+  float vo[3][3], wo[3];
+  settle_group<true, false>(a, L, idx, p, q, vr, po, vo, wo);
+  settle_vel_group<true, true>(a, L, idx, po, vo, wo);
 }
 
 void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
@@ -235,13 +258,17 @@ void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
                      a);
 }
 
-template <bool ACC>
-__global__ void k_settle_velocities(HdSettleArgs a) {
-  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_groups) return;
-  HdLattice L = hd_load_lattice(a.lattice);
-  unsigned i0 = a.groups[3 * g], i1 = a.groups[3 * g + 1], i2 = a.groups[3 * g + 2];
-  float4 p0 = a.posq[i0], p1 = a.posq[i1], p2 = a.posq[i2];
+// SETTLE of one water on velocities; with REG the positions, velocities
+// and constraint virials come from pin, vin, win (the values the position
+// stage has just computed) instead of memory
+template <bool ACC, bool REG>
+__device__ __forceinline__ void settle_vel_group(const HdSettleArgs& a, const HdLattice& L,
+                                                 const unsigned (&ids)[3],
+                                                 const float4 (&pin)[3],
+                                                 const float (&vin)[3][3], const float (&win)[3]) {
+  unsigned i0 = ids[0], i1 = ids[1], i2 = ids[2];
+  float4 p0 = REG ? pin[0] : a.posq[i0], p1 = REG ? pin[1] : a.posq[i1];
+  float4 p2 = REG ? pin[2] : a.posq[i2];
   float px[3] = {p0.x, p1.x, p2.x}, py[3] = {p0.y, p1.y, p2.y}, pz[3] = {p0.z, p1.z, p2.z};
   for (int k = 1; k < 3; ++k) {
     float dx = px[k] - px[0], dy = py[k] - py[0], dz = pz[k] - pz[0];
@@ -250,9 +277,13 @@ __global__ void k_settle_velocities(HdSettleArgs a) {
   }
   float m_o = a.m_o, m_h = a.m_h;
   float io = m_o > 0.f ? 1.0f / m_o : 0.f, ih = m_h > 0.f ? 1.0f / m_h : 0.f;
-  float vx[3] = {a.vx[i0], a.vx[i1], a.vx[i2]};
-  float vy[3] = {a.vy[i0], a.vy[i1], a.vy[i2]};
-  float vz[3] = {a.vz[i0], a.vz[i1], a.vz[i2]};
+  float vx[3], vy[3], vz[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    vx[k] = REG ? vin[k][0] : a.vx[ids[k]];
+    vy[k] = REG ? vin[k][1] : a.vy[ids[k]];
+    vz[k] = REG ? vin[k][2] : a.vz[ids[k]];
+  }
   float r1x = px[0] - px[1], r1y = py[0] - py[1], r1z = pz[0] - pz[1];
   float r2x = px[0] - px[2], r2y = py[0] - py[2], r2z = pz[0] - pz[2];
   float r3x = px[1] - px[2], r3y = py[1] - py[2], r3z = pz[1] - pz[2];
@@ -291,7 +322,6 @@ __global__ void k_settle_velocities(HdSettleArgs a) {
       {(g1 * r1x + g2 * r2x) * io, (g1 * r1y + g2 * r2y) * io, (g1 * r1z + g2 * r2z) * io},
       {(-g1 * r1x + g3 * r3x) * ih, (-g1 * r1y + g3 * r3y) * ih, (-g1 * r1z + g3 * r3z) * ih},
       {(-g2 * r2x - g3 * r3x) * ih, (-g2 * r2y - g3 * r3y) * ih, (-g2 * r2z - g3 * r3z) * ih}};
-  unsigned ids[3] = {i0, i1, i2};
   for (int k = 0; k < 3; ++k) {
     a.vx[ids[k]] = vx[k] + dv[k][0];
     a.vy[ids[k]] = vy[k] + dv[k][1];
@@ -307,10 +337,25 @@ __global__ void k_settle_velocities(HdSettleArgs a) {
     float ma[3] = {m_o, m_h, m_h};
     for (int k = 0; k < 3; ++k) {
       float sc = ma[k] * idt;
-      a.w_cons[ids[k]] += sc * (dv[k][0] * (px[k] - cx) + dv[k][1] * (py[k] - cy) +
-                               dv[k][2] * (pz[k] - cz));
+      float w0 = REG ? win[k] : a.w_cons[ids[k]];
+      a.w_cons[ids[k]] = w0 + sc * (dv[k][0] * (px[k] - cx) + dv[k][1] * (py[k] - cy) +
+                                    dv[k][2] * (pz[k] - cz));
     }
+  } else if (REG) {
+    // the position stage left these to this stage
+    for (int k = 0; k < 3; ++k) a.w_cons[ids[k]] = win[k];
   }
+}
+
+template <bool ACC>
+__global__ void k_settle_velocities(HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned ids[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
+  float4 pin[3] = {};
+  float vin[3][3] = {}, win[3] = {};
+  settle_vel_group<ACC, false>(a, L, ids, pin, vin, win);
 }
 
 void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a) {
