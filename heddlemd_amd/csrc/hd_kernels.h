@@ -36,6 +36,8 @@ unsigned hd_block_pair_chunks(unsigned nblocks);
 
 // check = false when the displacement check already ran this step (KE pass)
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a, bool check = true);
+// the displacement check alone (sets NB_STATUS_REBUILD)
+void hd_launch_disp_check(hipStream_t st, const HdNeighborArgs& a);
 // number of chunk sums of the voxel-count scan (the buffer needs one more)
 unsigned hd_voxel_scan_parts(unsigned nvox);
 void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
@@ -183,7 +185,8 @@ void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLa
 struct HdSettleArgs {
   unsigned n_groups;
   const unsigned* groups;  // 3 per group: O, H1, H2
-  float ra, rb, rc, m_o, m_h, dt;
+  double d_oh, d_hh;  // constrained O-H and H-H distances
+  float m_o, m_h, dt;
   float4* posq;
   const float4* snap;
   float *vx, *vy, *vz;
@@ -209,13 +212,10 @@ void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool acc
 #define HD_SHAKE_REF_ATOMS 8   // MAX_GROUP_ATOMS (src/integrator/shake.rs:21)
 #define HD_SHAKE_MAX_COLORS 16
 struct HdShakeArgs {
-  // groups of at most 8 atoms (the reference's cap) run one per thread with
-  // max_atoms / max_cons over those groups; longer groups one per wave
-  unsigned n_groups, max_atoms, max_cons, threads;  // threads per block
-  const unsigned *slist, *llist;  // short and long group ids
-  unsigned n_short, n_long;
-  const unsigned char* color;  // constraint colour (long groups)
-  unsigned n_colors;           // colours used by the long groups
+  // one wave per group (at most 64 atoms and 64 constraints each)
+  unsigned n_groups;
+  const unsigned char* color;  // constraint colour (sweep order)
+  unsigned n_colors;           // colours used by any group
   const unsigned *goff, *gatoms, *coff, *cij;
   const float* cr2;
   float4* posq;
@@ -236,7 +236,6 @@ struct HdShakeArgs {
   unsigned n_cons;
 };
 // LDS bytes per thread of a group with the given maxima
-unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons);
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a);
 void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool accumulate_virial);
 // Predictor inputs of every constraint (HD_SHAKE_NFEAT floats each) into

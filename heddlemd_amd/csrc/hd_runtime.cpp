@@ -54,6 +54,10 @@ struct hd_ctx {
   // `stream` inside each force evaluation (default priority)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // status-word polling inside hd_step: two pinned 4-word slots, each
+  // filled by an async copy after a graph launch and marked by its event
+  unsigned* h_stat = nullptr;
+  hipEvent_t ev_stat[2] = {nullptr, nullptr};
   // off while a batch captures this context's step: the batch graph already
   // runs the replicas side by side, and per-replica forks slowed it
   bool spme_fork = true;
@@ -92,6 +96,7 @@ struct hd_ctx {
   double lan_friction = 0, lan_kt = 0;
   uint64_t lan_seed = 0;
   bool use_graph = true;
+  bool fused_settle = true;  // all-water systems: drift and SETTLE in one kernel
   bool timing = false;
   bool ready = false;
   bool forces_valid = false;
@@ -151,12 +156,9 @@ struct hd_ctx {
   // general SHAKE / RATTLE groups
   std::vector<uint32_t> h_sh_goff, h_sh_atoms, h_sh_coff, h_sh_cij;
   std::vector<float> h_sh_r2;
-  std::vector<uint32_t> h_sh_short, h_sh_long;  // group ids by kernel
   std::vector<uint8_t> h_sh_color;              // constraint colours (long groups)
   unsigned sh_n_colors = 0;
-  unsigned *d_sh_short = nullptr, *d_sh_long = nullptr;
   uint8_t* d_sh_color = nullptr;
-  unsigned sh_max_atoms = 0, sh_max_cons = 0, sh_threads = 0;
   double sh_tol = 3.57e-6, sh_rattle_tol = 8.63e-17;
   int sh_max_iter = 32;
   unsigned *d_sh_goff = nullptr, *d_sh_atoms = nullptr, *d_sh_coff = nullptr,
@@ -608,9 +610,6 @@ HdShakeArgs shake_args(hd_ctx* c, float dt) {
   HdShakeArgs a;
   memset(&a, 0, sizeof(a));
   a.n_groups = (unsigned)(c->h_sh_goff.size() - 1);
-  a.max_atoms = c->sh_max_atoms;
-  a.max_cons = c->sh_max_cons;
-  a.threads = c->sh_threads;
   a.goff = c->d_sh_goff;
   a.gatoms = c->d_sh_atoms;
   a.coff = c->d_sh_coff;
@@ -629,10 +628,6 @@ HdShakeArgs shake_args(hd_ctx* c, float dt) {
   a.rattle_tol = (float)c->sh_rattle_tol;
   a.max_iter = c->sh_max_iter;
   a.stats = c->d_sh_stats;
-  a.slist = c->d_sh_short;
-  a.llist = c->d_sh_long;
-  a.n_short = (unsigned)c->h_sh_short.size();
-  a.n_long = (unsigned)c->h_sh_long.size();
   a.color = c->d_sh_color;
   a.n_colors = c->sh_n_colors;
   a.n_cons = (unsigned)c->h_sh_cij.size();
@@ -710,11 +705,8 @@ HdSettleArgs settle_args(hd_ctx* c) {
   memset(&sa, 0, sizeof(sa));
   sa.n_groups = (unsigned)(c->h_groups.size() / 3);
   sa.groups = c->d_groups;
-  double total = c->m_o + 2.0 * c->m_h;
-  double h = sqrt(c->d_oh * c->d_oh - 0.25 * c->d_hh * c->d_hh);
-  sa.ra = (float)((2.0 * c->m_h / total) * h);
-  sa.rb = (float)((c->m_o / total) * h);
-  sa.rc = (float)(c->d_hh * 0.5);
+  sa.d_oh = c->d_oh;
+  sa.d_hh = c->d_hh;
   sa.m_o = (float)c->m_o;
   sa.m_h = (float)c->m_h;
   sa.dt = (float)c->dt;
@@ -825,7 +817,7 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   ia.lattice = c->d_lattice;
   ia.factor = c->have_nhc ? c->factors + 0 : nullptr;
   ia.snap = has_constraints(c) ? c->snap : nullptr;
-  if (!c->lossless && !has_shake(c) && c->h_groups.size() == (size_t)c->n) {
+  if (c->fused_settle && !c->lossless && !has_shake(c) && c->h_groups.size() == (size_t)c->n) {
     // every atom in a SETTLE water (3 indices per water, disjoint): the
     // drift and SETTLE on positions and velocities in one kernel
     stage_begin(c, "vv_kick_drift_settle", timed);
@@ -896,7 +888,14 @@ int hd_ctx_create(int device, hd_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_stat[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_stat[1], hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_stat, 8 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
+    if (c->h_stat) (void)hipHostFree(c->h_stat);
+    for (hipEvent_t e : c->ev_stat)
+      if (e) (void)hipEventDestroy(e);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -922,6 +921,9 @@ void hd_ctx_destroy(hd_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   (void)hipEventDestroy(c->ev_fork);
   (void)hipEventDestroy(c->ev_join);
+  (void)hipEventDestroy(c->ev_stat[0]);
+  (void)hipEventDestroy(c->ev_stat[1]);
+  (void)hipHostFree(c->h_stat);
   (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1202,8 +1204,6 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
     return fail(c, HD_ERR_ARG, "SHAKE group offsets must start at 0");
   std::vector<char> seen(c->n, 0);
   for (uint32_t a : c->h_groups) seen[a] = 1;
-  unsigned ma = 0, mc = 0;
-  std::vector<uint32_t> shortg, longg;
   std::vector<uint8_t> color(ng ? coff[ng] : 0, 0);
   unsigned ncol = 0;
   for (uint64_t g = 0; g < ng; ++g) {
@@ -1222,15 +1222,8 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
       if (ci[k] >= na || cj[k] >= na || ci[k] == cj[k] || !(d[k] > 0))
         return fail(c, HD_ERR_ARG, "SHAKE constraint %u of group %llu is invalid", k,
                     (unsigned long long)g);
-    if (na <= HD_SHAKE_REF_ATOMS) {
-      shortg.push_back((uint32_t)g);
-      ma = std::max(ma, na);
-      mc = std::max(mc, nc);
-      continue;
-    }
     // greedy colouring in topology order: the lowest colour that no earlier
     // constraint on either atom holds
-    longg.push_back((uint32_t)g);
     uint32_t used[HD_SHAKE_MAX_ATOMS] = {0};
     for (unsigned k = coff[g]; k < coff[g + 1]; ++k) {
       uint32_t busy = used[ci[k]] | used[cj[k]];
@@ -1256,17 +1249,8 @@ int hd_set_shake(hd_ctx* c, uint64_t ng, const uint32_t* goff, const uint32_t* g
     float df = (float)d[k];  // r2 = (Real d)^2 (shake.rs:301-302)
     c->h_sh_r2[k] = df * df;
   }
-  c->h_sh_short = shortg;
-  c->h_sh_long = longg;
   c->h_sh_color = color;
   c->sh_n_colors = ncol;
-  c->sh_max_atoms = ma;
-  c->sh_max_cons = mc;
-  // threads per block: the largest power of two whose LDS fits in 48 KiB
-  unsigned per = hd_shake_lds_per_thread(std::max(ma, 1u), std::max(mc, 1u));
-  unsigned t = 256;
-  while (t > 1 && t * per > 48u * 1024u) t >>= 1;
-  c->sh_threads = t;
   return HD_OK;
 }
 
@@ -1545,6 +1529,8 @@ int hd_get_langevin_counter(hd_ctx* c, uint64_t* counter) {
 int hd_set_crescale(hd_ctx* c, double pressure, double tau, double compressibility, double kt,
                     uint64_t seed) {
   if (!c) return HD_ERR_ARG;
+  if (c->lossless && tau > 0)
+    return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support a barostat");
   if (c->ready) {
     // a new phase may switch the barostat on or off or change it
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1632,6 +1618,10 @@ int hd_set_pair_list(hd_ctx* c, int mode) {
 
 static void drop_graphs(hd_ctx* c) {
   ++c->graph_gen;
+  // an executable graph is destroyed only once no launch of it can still
+  // be running on either stream of the context
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
   if (c->graph_f) (void)hipGraphExecDestroy(c->graph_f);
   if (c->graph_fev) (void)hipGraphExecDestroy(c->graph_fev);
   if (c->graph_fk) (void)hipGraphExecDestroy(c->graph_fk);
@@ -1649,6 +1639,13 @@ int hd_setup(hd_ctx* c) {
                                                          (unsigned long long)i);
   if (c->lattice[3] != 0 || c->lattice[4] != 0 || c->lattice[5] != 0)
     return fail(c, HD_ERR_UNSUPPORTED, "triclinic boxes are not built (orthorhombic only)");
+  // the compensated mode carries FP64 residuals of x and v that constraint
+  // projections and barostat scaling would leave stale; the reference
+  // refuses the combination (src/integrator/velocity_verlet.rs:222-229)
+  if (c->lossless && has_constraints(c))
+    return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support constraints (SETTLE/SHAKE)");
+  if (c->lossless && c->have_baro)
+    return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support a barostat");
   HIPCHK(c, hipSetDevice(c->device));
   c->rmax = c->have_spme ? c->coul_rc : 0.0;
   for (double v : c->lj_cut) c->rmax = std::max(c->rmax, v);
@@ -1778,15 +1775,7 @@ int hd_setup(hd_ctx* c) {
       DALLOC(c, c->d_sh_nu, cg.size());
       if (!cg.empty() && (r = upload(c, c->d_sh_cgrp, cg.data(), cg.size()))) return r;
     }
-    DALLOC(c, c->d_sh_short, c->h_sh_short.size());
-    DALLOC(c, c->d_sh_long, c->h_sh_long.size());
     DALLOC(c, c->d_sh_color, c->h_sh_color.size());
-    if (!c->h_sh_short.empty() &&
-        (r = upload(c, c->d_sh_short, c->h_sh_short.data(), c->h_sh_short.size())))
-      return r;
-    if (!c->h_sh_long.empty() &&
-        (r = upload(c, c->d_sh_long, c->h_sh_long.data(), c->h_sh_long.size())))
-      return r;
     if (!c->h_sh_color.empty() &&
         (r = upload(c, c->d_sh_color, c->h_sh_color.data(), c->h_sh_color.size())))
       return r;
@@ -1870,6 +1859,34 @@ int hd_force_eval(hd_ctx* c, int want_scalars) {
   return check_status(c);
 }
 
+int hd_set_positions(hd_ctx* c, const float* posq) {
+  if (!c || !posq) return HD_ERR_ARG;
+  if (!c->ready) return fail(c, HD_ERR_STATE, "hd_set_positions before hd_setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->posq, posq, c->n * sizeof(float4), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // charges travel with the positions and feed the SPME pipeline as given;
+  // forces belong to the old positions
+  c->forces_valid = false;
+  return HD_OK;
+}
+
+int hd_neighbor_pre_step(hd_ctx* c, uint32_t* status_out) {
+  if (!c || !status_out) return HD_ERR_ARG;
+  if (!c->ready) return fail(c, HD_ERR_STATE, "hd_neighbor_pre_step before hd_setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  HdNeighborArgs na = nb_args(c);
+  hd_launch_disp_check(c->stream, na);
+  HIPCHK(c, hipGetLastError());
+  unsigned ctrl[4];
+  HIPCHK(c, hipMemcpyAsync(ctrl, c->ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *status_out = ctrl[0];
+  return HD_OK;
+}
+
 int hd_constrain_velocities(hd_ctx* c) {
   if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_constrain_velocities before hd_setup");
   if (has_shake(c)) {
@@ -1902,14 +1919,60 @@ int hd_constrain_velocities(hd_ctx* c) {
 
 #define HD_GRAPH_STEPS 8
 
-// nsteps steps, the last with scalars when ev
+// nsteps steps, the last with scalars when ev.  Every path out of here
+// leaves the stream out of capture mode and *out either null or a complete
+// executable graph: a failure inside the capture ends the capture and
+// frees the partial graph instead of returning with the stream still
+// capturing (which would pull the caller's next launches into a graph
+// that is never instantiated).
 static int capture(hd_ctx* c, bool ev, hipGraphExec_t* out, int nsteps = 1) {
-  hipGraph_t g = nullptr;
+  *out = nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(c, hipStreamIsCapturing(c->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(c, HD_ERR_STATE, "context stream is already capturing");
+  // captured from an idle context: nothing queued earlier on either stream
+  // can be mistaken for part of the graph
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->side));
+  (void)hipGetLastError();
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < nsteps; ++k) enqueue_step(c, ev && k == nsteps - 1, false);
-  HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-  HIPCHK(c, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+  hipError_t le = hipGetLastError();
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(c->stream, &g);
+  if (le != hipSuccess || ee != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c, HD_ERR_HIP, "step capture failed: %s",
+                hipGetErrorString(le != hipSuccess ? le : ee));
+  }
+  hipGraphExec_t x = nullptr;
+  hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
+  if (ie != hipSuccess || !x)
+    return fail(c, HD_ERR_HIP, "step graph instantiation failed: %s", hipGetErrorString(ie));
+  *out = x;
+  return HD_OK;
+}
+
+// Status polling inside a long hd_step call (the reference reads the
+// status word once per batch, src/forces/neighbor_list.rs:1300-1340).
+// After each graph launch the status word is copied into pinned slot
+// k & 1 and an event marks the copy; the slot of launch k - 1 is read once
+// launch k is queued, so the device always has a launch in hand while the
+// host waits.  Returns 1 when a high-water or overflow bit is set: the
+// caller then synchronises and lets check_status grow the list (or fail)
+// before any further step runs.
+static int poll_status(hd_ctx* c, unsigned k, int* flagged) {
+  unsigned slot = k & 1u;
+  *flagged = 0;
+  HIPCHK(c, hipMemcpyAsync(c->h_stat + 4 * slot, c->ctrl, 4 * sizeof(unsigned),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev_stat[slot], c->stream));
+  if (k == 0) return HD_OK;
+  unsigned prev = slot ^ 1u;
+  HIPCHK(c, hipEventSynchronize(c->ev_stat[prev]));
+  if (c->h_stat[4 * prev] & (NB_STATUS_HIGHWATER | NB_STATUS_OVERFLOW)) *flagged = 1;
   return HD_OK;
 }
 
@@ -1926,10 +1989,12 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
   // constraint iteration counters cover this call's steps
   if (c->d_sh_stats)
     HIPCHK(c, hipMemsetAsync(c->d_sh_stats, 0, 6 * sizeof(unsigned), c->stream));
+  unsigned polls = 0;
   for (int s = 0; s < n_steps; ++s) {
     bool ev = scalars_last && s == n_steps - 1;
     // whole groups of steps without scalars as one graph launch
     int plain_left = n_steps - s - (scalars_last ? 1 : 0);
+    bool poll = false;
     if (c->use_graph && plain_left >= HD_GRAPH_STEPS) {
       if (!c->graph_fk) {
         int r = capture(c, false, &c->graph_fk, HD_GRAPH_STEPS);
@@ -1937,9 +2002,8 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
       }
       HIPCHK(c, hipGraphLaunch(c->graph_fk, c->stream));
       s += HD_GRAPH_STEPS - 1;
-      continue;
-    }
-    if (c->use_graph) {
+      poll = true;
+    } else if (c->use_graph) {
       hipGraphExec_t* gx = ev ? &c->graph_fev : &c->graph_f;
       if (!*gx) {
         int r = capture(c, ev, gx);
@@ -1950,6 +2014,17 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
       if (timed && s == 0) c->n_stage_used = 0;
       enqueue_step(c, ev, timed);
       HIPCHK(c, hipGetLastError());
+      poll = (s + 1) % HD_GRAPH_STEPS == 0;
+    }
+    if (poll && s + 1 < n_steps) {
+      int flagged = 0;
+      int r = poll_status(c, polls++, &flagged);
+      if (r) return r;
+      if (flagged) {
+        // grow (or report the overflow) before the next step
+        if ((r = check_status(c))) return r;
+        polls = 0;
+      }
     }
   }
   int r = check_status(c);
@@ -1985,6 +2060,16 @@ int hd_negate_velocities(hd_ctx* c) {
                               (unsigned)c->n);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+int hd_set_fused_settle(hd_ctx* c, int enable) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) {
+    HIPCHK(c, hipSetDevice(c->device));
+    drop_graphs(c);
+  }
+  c->fused_settle = enable != 0;
   return HD_OK;
 }
 
@@ -2288,6 +2373,7 @@ static int bfail(hd_batch* b, int code, const char* fmt, ...) {
   } while (0)
 
 static void batch_drop(hd_batch* b) {
+  (void)hipStreamSynchronize(b->stream);
   for (auto& g : b->graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -2352,24 +2438,43 @@ int hd_batch_set_lanes(hd_batch* b, int lanes) {
 }
 
 static int batch_capture(hd_batch* b, bool ev, hipGraphExec_t* out) {
-  hipGraph_t g = nullptr;
+  *out = nullptr;
+  for (auto c : b->ctx) {
+    BCHK(b, hipStreamSynchronize(c->stream));
+    BCHK(b, hipStreamSynchronize(c->side));
+  }
+  (void)hipGetLastError();
   BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  BCHK(b, hipEventRecord(b->fork, b->stream));
+  // errors inside the capture are collected, and the capture is always
+  // ended, so no stream is left capturing after a failure
+  hipError_t e = hipEventRecord(b->fork, b->stream);
   // replicas form `lanes` chains: replica i follows replica i - lanes
   size_t n = b->ctx.size();
   size_t lanes = (b->lanes <= 0 || (size_t)b->lanes >= n) ? n : (size_t)b->lanes;
-  for (size_t i = 0; i < n; ++i) {
+  for (size_t i = 0; i < n && e == hipSuccess; ++i) {
     hd_ctx* c = b->ctx[i];
-    BCHK(b, hipStreamWaitEvent(c->stream, i < lanes ? b->fork : b->join[i - lanes], 0));
+    e = hipStreamWaitEvent(c->stream, i < lanes ? b->fork : b->join[i - lanes], 0);
+    if (e != hipSuccess) break;
     c->spme_fork = false;
     enqueue_step(c, ev, false);
     c->spme_fork = true;
-    BCHK(b, hipEventRecord(b->join[i], c->stream));
-    if (i + lanes >= n) BCHK(b, hipStreamWaitEvent(b->stream, b->join[i], 0));
+    e = hipEventRecord(b->join[i], c->stream);
+    if (e == hipSuccess && i + lanes >= n) e = hipStreamWaitEvent(b->stream, b->join[i], 0);
   }
-  BCHK(b, hipStreamEndCapture(b->stream, &g));
-  BCHK(b, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+  if (e == hipSuccess) e = hipGetLastError();
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || ee != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(b, HD_ERR_HIP, "batch capture failed: %s",
+                 hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  hipGraphExec_t x = nullptr;
+  hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
+  if (ie != hipSuccess || !x)
+    return bfail(b, HD_ERR_HIP, "batch graph instantiation failed: %s", hipGetErrorString(ie));
+  *out = x;
   return HD_OK;
 }
 

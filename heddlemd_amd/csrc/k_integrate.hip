@@ -5,7 +5,10 @@
 // velocity-Verlet update is bit-exact against a scalar restatement:
 //   * velocity Verlet kick/drift, f32 and compensated (f32, f64)
 //     (kernels/integrate.cu:24-263);
-//   * SETTLE positions / velocities (kernels/settle.cu:145-497);
+//   * the constraint contract of SETTLE (kernels/settle.cu:145-497):
+//     rigid water from the pre-drift snapshot, then zero bond velocities,
+//     with the constraint virial (solved here by FP64 Newton / direct
+//     solves, see below);
 //   * Nose-Hoover chain (src/integrator/nose_hoover_chain.rs:109-279), here
 //     advanced on the device in FP64 so the step needs no host round trip;
 //   * the post-force kick fragment and thermostat factor
@@ -77,156 +80,297 @@ void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a) {
 }
 
 // ---------------------------------------------------------------------
-// SETTLE
+// SETTLE: rigid three-site water (Miyamoto & Kollman, J. Comput. Chem. 13,
+// 952-962 (1992)).
+//
+// Positions.  With u_k the drifted (unconstrained) positions and g_c the
+// three bond vectors of the step's start (the snapshot), the constrained
+// positions are
+//     x_k = u_k + (1/m_k) sum_c e_kc lambda_c g_c,
+//     |x_i(c) - x_j(c)|^2 = d_c^2  for the bonds c = OH1, OH2, H1H2,
+// with e_kc = +1 (k = i(c)), -1 (k = j(c)), 0 otherwise: displacements
+// along the constraint forces of the start of the step.  Miyamoto and
+// Kollman give the closed-form solution of these three equations; here
+// they are solved by Newton's method on (lambda_1, lambda_2, lambda_3) in
+// FP64, in a frame at the snapshot oxygen, which converges quadratically
+// to the same point from lambda = 0 and loses nothing to the f32 storage:
+// the correction is rounded once, into the drifted f32 positions.
+//
+// Velocities.  The bond-relative velocities along the new bonds r_c must
+// vanish: with v_k' = v_k + (1/m_k) sum_c e_kc mu_c r_c the conditions
+// r_c . (v'_i(c) - v'_j(c)) = 0 are linear in mu, one 3x3 system per
+// water, solved directly (adjugate, FP64).
+//
+// Both stages use one coupling matrix form: d(x_i(c) - x_j(c))/d(lambda_c')
+// = K_cc' b_c' with K_cc' = e_i(c)c'/m_i(c) - e_j(c)c'/m_j(c).
+// Constraint virial per atom: m_k dx_k . (x_k - X_com) / dt^2 for the
+// position correction dx_k, plus m_k dv_k . (x_k - X_com) / dt for the
+// velocity correction dv_k (the totals are frame independent: the
+// constraint forces of a water sum to zero).
 // ---------------------------------------------------------------------
-__device__ __forceinline__ float settle_csqrt(float x) { return sqrtf(x > 0.f ? x : 0.f); }
+#define HD_SETTLE_NEWTON_MAX 8
 
-// SETTLE of one water on positions: p the unconstrained positions, q the
-// snapshot before the drift; the velocities are read from memory, or with
-// VREG taken from vr (the values the drift has just computed).  The new
-// positions, velocities and constraint virials are stored, and also
-// returned in po, vo, wo; without VSTORE the velocities and virials are
-// only returned (a velocity stage that follows stores them)
-template <bool VREG, bool VSTORE>
-__device__ __forceinline__ void settle_group(const HdSettleArgs& a, const HdLattice& L,
-                                             const unsigned (&idx)[3], const float4 (&p)[3],
-                                             const float4 (&q)[3], const float (&vr)[3][3],
-                                             float4 (&po)[3], float (&vo)[3][3],
-                                             float (&wo)[3]) {
-  float cur[3][3] = {{p[0].x, p[0].y, p[0].z}, {p[1].x, p[1].y, p[1].z}, {p[2].x, p[2].y, p[2].z}};
-  float snp[3][3] = {{q[0].x, q[0].y, q[0].z}, {q[1].x, q[1].y, q[1].z}, {q[2].x, q[2].y, q[2].z}};
-  float xb0 = snp[1][0] - snp[0][0], yb0 = snp[1][1] - snp[0][1], zb0 = snp[1][2] - snp[0][2];
-  float xc0 = snp[2][0] - snp[0][0], yc0 = snp[2][1] - snp[0][1], zc0 = snp[2][2] - snp[0][2];
-  hd_min_image(xb0, yb0, zb0, L);
-  hd_min_image(xc0, yc0, zc0, L);
-  float xp[3][3];
+// bond c joins water atoms tri_i(c) and tri_j(c): O-H1, O-H2, H1-H2
+__device__ __forceinline__ int tri_i(int c) { return c == 2 ? 1 : 0; }
+__device__ __forceinline__ int tri_j(int c) { return c == 0 ? 1 : 2; }
+// e_kc: how atom k enters bond c
+__device__ __forceinline__ double tri_sign(int k, int c) {
+  return k == tri_i(c) ? 1.0 : (k == tri_j(c) ? -1.0 : 0.0);
+}
+
+// x = A^-1 b through the adjugate; a singular A leaves x = 0
+__device__ __forceinline__ void solve3_adj(const double (&A)[3][3], const double (&b)[3],
+                                           double (&x)[3]) {
+  double cof[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int r1 = (r + 1) % 3, r2 = (r + 2) % 3, q1 = (q + 1) % 3, q2 = (q + 2) % 3;
+      cof[r][q] = A[r1][q1] * A[r2][q2] - A[r1][q2] * A[r2][q1];
+    }
+  double det = A[0][0] * cof[0][0] + A[0][1] * cof[0][1] + A[0][2] * cof[0][2];
+  double inv = det != 0.0 ? 1.0 / det : 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) x[r] = (cof[0][r] * b[0] + cof[1][r] * b[1] + cof[2][r] * b[2]) * inv;
+}
+
+// coupling coefficients K_cc' of the three bonds for atom inverse masses im
+__device__ __forceinline__ void tri_coupling(const double (&im)[3], double (&K)[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      K[c][q] = tri_sign(tri_i(c), q) * im[tri_i(c)] - tri_sign(tri_j(c), q) * im[tri_j(c)];
+}
+
+__device__ __forceinline__ double dot3(const double (&a)[3], const double (&b)[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// f32 minimum-image difference a - b, widened
+__device__ __forceinline__ void rel_f64(float4 a, float4 b, const HdLattice& L, double (&d)[3]) {
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  hd_min_image(dx, dy, dz, L);
+  d[0] = dx; d[1] = dy; d[2] = dz;
+}
+
+struct SettleOut {
+  float4 p[3];
+  float v[3][3];
+  float w[3];
+};
+
+// Position stage of one water: p the drifted positions, q the snapshot,
+// vin the velocities after the kick (f32).  Returns the constrained
+// positions, the velocities with the correction / dt, and the position
+// virials.
+__device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& a,
+                                                            const HdLattice& L,
+                                                            const float4 (&p)[3],
+                                                            const float4 (&q)[3],
+                                                            const float (&vin)[3][3]) {
+  const double mass[3] = {(double)a.m_o, (double)a.m_h, (double)a.m_h};
+  const double im[3] = {1.0 / mass[0], 1.0 / mass[1], 1.0 / mass[2]};
+  const double target[3] = {a.d_oh * a.d_oh, a.d_oh * a.d_oh, a.d_hh * a.d_hh};
+  // snapshot frame at the old oxygen: s_k; drifted positions u_k = s_k + (p_k - q_k)
+  double s[3][3] = {{0.0, 0.0, 0.0}}, u[3][3];
+  rel_f64(q[1], q[0], L, s[1]);
+  rel_f64(q[2], q[0], L, s[2]);
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float dx = cur[k][0] - snp[k][0], dy = cur[k][1] - snp[k][1], dz = cur[k][2] - snp[k][2];
-    hd_min_image(dx, dy, dz, L);
-    xp[k][0] = dx; xp[k][1] = dy; xp[k][2] = dz;
+    double step[3];
+    rel_f64(p[k], q[k], L, step);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[k][d] = s[k][d] + step[d];
   }
-  float m0 = a.m_o, m1 = a.m_h, m2 = a.m_h;
-  float itm = 1.0f / (m0 + m1 + m2);
-  float ra = a.ra, rb = a.rb, rc = a.rc;
-  float d2 = 2.0f * rc, d2sq = d2 * d2;
-  float xcom = (xp[0][0] * m0 + (xb0 + xp[1][0]) * m1 + (xc0 + xp[2][0]) * m2) * itm;
-  float ycom = (xp[0][1] * m0 + (yb0 + xp[1][1]) * m1 + (yc0 + xp[2][1]) * m2) * itm;
-  float zcom = (xp[0][2] * m0 + (zb0 + xp[1][2]) * m1 + (zc0 + xp[2][2]) * m2) * itm;
-  float xa1 = xp[0][0] - xcom, ya1 = xp[0][1] - ycom, za1 = xp[0][2] - zcom;
-  float xb1 = xb0 + xp[1][0] - xcom, yb1 = yb0 + xp[1][1] - ycom, zb1 = zb0 + xp[1][2] - zcom;
-  float xc1 = xc0 + xp[2][0] - xcom, yc1 = yc0 + xp[2][1] - ycom, zc1 = zc0 + xp[2][2] - zcom;
-  float ez_x = yb0 * zc0 - zb0 * yc0, ez_y = zb0 * xc0 - xb0 * zc0, ez_z = xb0 * yc0 - yb0 * xc0;
-  float ex_x = ya1 * ez_z - za1 * ez_y, ex_y = za1 * ez_x - xa1 * ez_z, ex_z = xa1 * ez_y - ya1 * ez_x;
-  float ey_x = ez_y * ex_z - ez_z * ex_y, ey_y = ez_z * ex_x - ez_x * ex_z, ey_z = ez_x * ex_y - ez_y * ex_x;
-  float exl = settle_csqrt(ex_x * ex_x + ex_y * ex_y + ex_z * ex_z);
-  float eyl = settle_csqrt(ey_x * ey_x + ey_y * ey_y + ey_z * ey_z);
-  float ezl = settle_csqrt(ez_x * ez_x + ez_y * ez_y + ez_z * ez_z);
-  float iex = exl > 0.f ? 1.0f / exl : 0.f, iey = eyl > 0.f ? 1.0f / eyl : 0.f;
-  float iez = ezl > 0.f ? 1.0f / ezl : 0.f;
-  float r11 = ex_x * iex, r21 = ex_y * iex, r31 = ex_z * iex;
-  float r12 = ey_x * iey, r22 = ey_y * iey, r32 = ey_z * iey;
-  float r13 = ez_x * iez, r23 = ez_y * iez, r33 = ez_z * iez;
-  float b0x = r11 * xb0 + r21 * yb0 + r31 * zb0, b0y = r12 * xb0 + r22 * yb0 + r32 * zb0;
-  float c0x = r11 * xc0 + r21 * yc0 + r31 * zc0, c0y = r12 * xc0 + r22 * yc0 + r32 * zc0;
-  float a1z = r13 * xa1 + r23 * ya1 + r33 * za1;
-  float b1x = r11 * xb1 + r21 * yb1 + r31 * zb1, b1y = r12 * xb1 + r22 * yb1 + r32 * zb1;
-  float b1z = r13 * xb1 + r23 * yb1 + r33 * zb1;
-  float c1x = r11 * xc1 + r21 * yc1 + r31 * zc1, c1y = r12 * xc1 + r22 * yc1 + r32 * zc1;
-  float c1z = r13 * xc1 + r23 * yc1 + r33 * zc1;
-  float sinphi = a1z / ra;
-  float cosphi = settle_csqrt(1.0f - sinphi * sinphi);
-  float sinpsi = (b1z - c1z) / (2.0f * rc * cosphi);
-  float cospsi = settle_csqrt(1.0f - sinpsi * sinpsi);
-  float a2y = ra * cosphi;
-  float b2x = -rc * cospsi;
-  float b2y = -rb * cosphi - rc * sinpsi * sinphi;
-  float c2y = -rb * cosphi + rc * sinpsi * sinphi;
-  float b2xsq = b2x * b2x;
-  float hhsq = 4.0f * b2xsq + (b2y - c2y) * (b2y - c2y) + (b1z - c1z) * (b1z - c1z);
-  float delta_x = 2.0f * b2x + settle_csqrt(4.0f * b2xsq - hhsq + d2sq);
-  b2x -= delta_x * 0.5f;
-  float al = b2x * (b0x - c0x) + b0y * b2y + c0y * c2y;
-  float be = b2x * (c0y - b0y) + b0x * b2y + c0x * c2y;
-  float ga = b0x * b1y - b1x * b0y + c0x * c1y - c1x * c0y;
-  float ab2 = al * al + be * be;
-  float iab2 = ab2 > 0.f ? 1.0f / ab2 : 0.f;
-  float sinth = (al * ga - be * settle_csqrt(ab2 - ga * ga)) * iab2;
-  float costh = settle_csqrt(1.0f - sinth * sinth);
-  float a3x = -a2y * sinth, a3y = a2y * costh, a3z = a1z;
-  float b3x = b2x * costh - b2y * sinth, b3y = b2x * sinth + b2y * costh, b3z = b1z;
-  float c3x = -b2x * costh - c2y * sinth, c3y = -b2x * sinth + c2y * costh, c3z = c1z;
-  float rcom[3][3] = {
-      {r11 * a3x + r12 * a3y + r13 * a3z, r21 * a3x + r22 * a3y + r23 * a3z, r31 * a3x + r32 * a3y + r33 * a3z},
-      {r11 * b3x + r12 * b3y + r13 * b3z, r21 * b3x + r22 * b3y + r23 * b3z, r31 * b3x + r32 * b3y + r33 * b3z},
-      {r11 * c3x + r12 * c3y + r13 * c3z, r21 * c3x + r22 * c3y + r23 * c3z, r31 * c3x + r32 * c3y + r33 * c3z}};
-  float com[3] = {xcom, ycom, zcom};
-  float unc[3][3] = {{xp[0][0], xp[0][1], xp[0][2]},
-                     {xb0 + xp[1][0], yb0 + xp[1][1], zb0 + xp[1][2]},
-                     {xc0 + xp[2][0], yc0 + xp[2][1], zc0 + xp[2][2]}};
-  float mat[3] = {m0, m1, m2};
-  float idt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
-  float idt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+  double bond0[3][3];  // g_c
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bond0[c][d] = s[tri_i(c)][d] - s[tri_j(c)][d];
+  double K[3][3];
+  tri_coupling(im, K);
+  double lam[3] = {0.0, 0.0, 0.0};
+  double x[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[k][d] = u[k][d];
+  for (int it = 0; it < HD_SETTLE_NEWTON_MAX; ++it) {
+    double bond[3][3], res[3], worst = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) bond[c][d] = x[tri_i(c)][d] - x[tri_j(c)][d];
+      res[c] = dot3(bond[c], bond[c]) - target[c];
+      worst = fmax(worst, fabs(res[c]) / target[c]);
+    }
+    if (worst <= 1e-15) break;
+    // Jacobian of the residuals in lambda: 2 K_cc' (r_c . g_c')
+    double J[3][3], rhs[3], dl[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int q2 = 0; q2 < 3; ++q2) J[c][q2] = 2.0 * K[c][q2] * dot3(bond[c], bond0[q2]);
+      rhs[c] = -res[c];
+    }
+    solve3_adj(J, rhs, dl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lam[c] += dl[c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double sh = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sh += tri_sign(k, c) * lam[c] * bond0[c][d];
+        x[k][d] = u[k][d] + im[k] * sh;
+      }
+  }
+  double msum = mass[0] + mass[1] + mass[2], com[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) com[d] = (mass[0] * x[0][d] + mass[1] * x[1][d] + mass[2] * x[2][d]) / msum;
+  double idt = a.dt != 0.f ? 1.0 / (double)a.dt : 0.0;
+  SettleOut o;
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float c0 = (com[0] + rcom[k][0]) - unc[k][0];
-    float c1 = (com[1] + rcom[k][1]) - unc[k][1];
-    float c2 = (com[2] + rcom[k][2]) - unc[k][2];
-    unsigned id = idx[k];
-    float vx = VREG ? vr[k][0] : a.vx[id];
-    float vy = VREG ? vr[k][1] : a.vy[id];
-    float vz = VREG ? vr[k][2] : a.vz[id];
-    vo[k][0] = vx + c0 * idt;
-    vo[k][1] = vy + c1 * idt;
-    vo[k][2] = vz + c2 * idt;
-    po[k] = make_float4(cur[k][0] + c0, cur[k][1] + c1, cur[k][2] + c2, p[k].w);
-    wo[k] = mat[k] * idt2 * (c0 * rcom[k][0] + c1 * rcom[k][1] + c2 * rcom[k][2]);
-    a.posq[id] = po[k];
-    if (VSTORE) {
-      a.vx[id] = vo[k][0];
-      a.vy[id] = vo[k][1];
-      a.vz[id] = vo[k][2];
-      a.w_cons[id] = wo[k];
+    double shift[3] = {x[k][0] - u[k][0], x[k][1] - u[k][1], x[k][2] - u[k][2]};
+    o.p[k] = make_float4((float)((double)p[k].x + shift[0]), (float)((double)p[k].y + shift[1]),
+                         (float)((double)p[k].z + shift[2]), p[k].w);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o.v[k][d] = (float)((double)vin[k][d] + shift[d] * idt);
+    double arm[3] = {x[k][0] - com[0], x[k][1] - com[1], x[k][2] - com[2]};
+    o.w[k] = (float)(mass[k] * idt * idt * dot3(shift, arm));
+  }
+  return o;
+}
+
+// Velocity stage of one water at its final positions p (f32 as stored):
+// the velocities v are projected; with ACC (dt > 0) the velocity virial is
+// added to w.
+template <bool ACC>
+__device__ __forceinline__ void settle_water_velocities(const HdSettleArgs& a, const HdLattice& L,
+                                                        const float4 (&p)[3], float (&v)[3][3],
+                                                        float (&w)[3]) {
+  const double mass[3] = {(double)a.m_o, (double)a.m_h, (double)a.m_h};
+  const double im[3] = {1.0 / mass[0], 1.0 / mass[1], 1.0 / mass[2]};
+  double y[3][3] = {{0.0, 0.0, 0.0}};
+  rel_f64(p[1], p[0], L, y[1]);
+  rel_f64(p[2], p[0], L, y[2]);
+  double bond[3][3], rel[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double dv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      bond[c][d] = y[tri_i(c)][d] - y[tri_j(c)][d];
+      dv[d] = (double)v[tri_i(c)][d] - (double)v[tri_j(c)][d];
+    }
+    rel[c] = -dot3(bond[c], dv);
+  }
+  double K[3][3], A[3][3], mu[3];
+  tri_coupling(im, K);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q2 = 0; q2 < 3; ++q2) A[c][q2] = K[c][q2] * dot3(bond[c], bond[q2]);
+  solve3_adj(A, rel, mu);
+  double msum = mass[0] + mass[1] + mass[2], com[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) com[d] = (mass[0] * y[0][d] + mass[1] * y[1][d] + mass[2] * y[2][d]) / msum;
+  double idt = (ACC && a.dt > 0.f) ? 1.0 / (double)a.dt : 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double kick[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double sh = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sh += tri_sign(k, c) * mu[c] * bond[c][d];
+      kick[d] = im[k] * sh;
+      v[k][d] = (float)((double)v[k][d] + kick[d]);
+    }
+    if (ACC && a.dt > 0.f) {
+      double arm[3] = {y[k][0] - com[0], y[k][1] - com[1], y[k][2] - com[2]};
+      w[k] = (float)((double)w[k] + mass[k] * idt * dot3(kick, arm));
     }
   }
+}
+
+__device__ __forceinline__ void settle_load_ids(const HdSettleArgs& a, unsigned g,
+                                                unsigned (&ids)[3]) {
+  ids[0] = a.groups[3 * g];
+  ids[1] = a.groups[3 * g + 1];
+  ids[2] = a.groups[3 * g + 2];
 }
 
 __global__ void k_settle_positions(HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   HdLattice L = hd_load_lattice(a.lattice);
-  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
-  float4 p[3] = {a.posq[idx[0]], a.posq[idx[1]], a.posq[idx[2]]};
-  float4 q[3] = {a.snap[idx[0]], a.snap[idx[1]], a.snap[idx[2]]};
-  float vr[3][3] = {}, vo[3][3], wo[3];
-  float4 po[3];
-  settle_group<false, true>(a, L, idx, p, q, vr, po, vo, wo);
+  unsigned ids[3];
+  settle_load_ids(a, g, ids);
+  float4 p[3], q[3];
+  float v[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p[k] = a.posq[ids[k]];
+    q[k] = a.snap[ids[k]];
+    v[k][0] = a.vx[ids[k]]; v[k][1] = a.vy[ids[k]]; v[k][2] = a.vz[ids[k]];
+  }
+  SettleOut o = settle_water_positions(a, L, p, q, v);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    a.posq[ids[k]] = o.p[k];
+    a.vx[ids[k]] = o.v[k][0]; a.vy[ids[k]] = o.v[k][1]; a.vz[ids[k]] = o.v[k][2];
+    a.w_cons[ids[k]] = o.w[k];
+  }
 }
 
-template <bool ACC, bool REG>
-__device__ __forceinline__ void settle_vel_group(const HdSettleArgs& a, const HdLattice& L,
-                                                 const unsigned (&ids)[3],
-                                                 const float4 (&pin)[3],
-                                                 const float (&vin)[3][3], const float (&win)[3]);
+template <bool ACC>
+__global__ void k_settle_velocities(HdSettleArgs a) {
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  unsigned ids[3];
+  settle_load_ids(a, g, ids);
+  float4 p[3];
+  float v[3][3], w[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p[k] = a.posq[ids[k]];
+    v[k][0] = a.vx[ids[k]]; v[k][1] = a.vy[ids[k]]; v[k][2] = a.vz[ids[k]];
+    w[k] = ACC ? a.w_cons[ids[k]] : 0.f;
+  }
+  settle_water_velocities<ACC>(a, L, p, v, w);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    a.vx[ids[k]] = v[k][0]; a.vy[ids[k]] = v[k][1]; a.vz[ids[k]] = v[k][2];
+    if (ACC) a.w_cons[ids[k]] = w[k];
+  }
+}
 
 // VV rescale + kick + drift + wrap fused with SETTLE on positions and then
 // on velocities (with the constraint virial), for a system whose atoms all
 // belong to SETTLE groups (no SHAKE, f32 state): one thread per water runs
-// k_kick_drift<false, true>'s arithmetic on its three atoms, then
-// k_settle_positions' and k_settle_velocities<true>'s on the values held
-// in registers, so the intermediate positions, velocities, virials and the
-// snapshot skip the round trips through memory.  The same operations in
-// the same order as the three kernels: bit-identical results.
+// k_kick_drift<false, true>'s arithmetic on its three atoms, then the two
+// SETTLE stages on the f32 values held in registers, the same values the
+// unfused kernels pass through memory, so the results are bit-identical
+// and the intermediate state and the snapshot skip the round trips.
 __global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   HdLattice L = hd_load_lattice(a.lattice);
   float f = ia.factor ? ia.factor[0] : 1.0f;
   float half_dt = ia.dt * 0.5f;
-  unsigned idx[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
+  unsigned ids[3];
+  settle_load_ids(a, g, ids);
   float4 p[3], q[3];
-  float vr[3][3];
+  float v[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    unsigned i = idx[k];
+    unsigned i = ids[k];
     float m = ia.mass[i];
     float ax = ia.fx[i] / m, ay = ia.fy[i] / m, az = ia.fz[i] / m;
     float4 pq = ia.posq[i];
@@ -243,12 +387,17 @@ __global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
     hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
     ia.ix[i] = nx + ka; ia.iy[i] = ny + kb; ia.iz[i] = nz + kc;
     p[k] = make_float4(px, py, pz, pq.w);
-    vr[k][0] = vx; vr[k][1] = vy; vr[k][2] = vz;
+    v[k][0] = vx; v[k][1] = vy; v[k][2] = vz;
   }
-  float4 po[3];
-  float vo[3][3], wo[3];
-  settle_group<true, false>(a, L, idx, p, q, vr, po, vo, wo);
-  settle_vel_group<true, true>(a, L, idx, po, vo, wo);
+  SettleOut o = settle_water_positions(a, L, p, q, v);
+  settle_water_velocities<true>(a, L, o.p, o.v, o.w);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned i = ids[k];
+    a.posq[i] = o.p[k];
+    a.vx[i] = o.v[k][0]; a.vy[i] = o.v[k][1]; a.vz[i] = o.v[k][2];
+    a.w_cons[i] = o.w[k];
+  }
 }
 
 void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
@@ -256,106 +405,6 @@ void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
   if (!a.n_groups) return;
   hipLaunchKernelGGL(k_kick_drift_settle, dim3((a.n_groups + 127) / 128), dim3(128), 0, st, ia,
                      a);
-}
-
-// SETTLE of one water on velocities; with REG the positions, velocities
-// and constraint virials come from pin, vin, win (the values the position
-// stage has just computed) instead of memory
-template <bool ACC, bool REG>
-__device__ __forceinline__ void settle_vel_group(const HdSettleArgs& a, const HdLattice& L,
-                                                 const unsigned (&ids)[3],
-                                                 const float4 (&pin)[3],
-                                                 const float (&vin)[3][3], const float (&win)[3]) {
-  unsigned i0 = ids[0], i1 = ids[1], i2 = ids[2];
-  float4 p0 = REG ? pin[0] : a.posq[i0], p1 = REG ? pin[1] : a.posq[i1];
-  float4 p2 = REG ? pin[2] : a.posq[i2];
-  float px[3] = {p0.x, p1.x, p2.x}, py[3] = {p0.y, p1.y, p2.y}, pz[3] = {p0.z, p1.z, p2.z};
-  for (int k = 1; k < 3; ++k) {
-    float dx = px[k] - px[0], dy = py[k] - py[0], dz = pz[k] - pz[0];
-    hd_min_image(dx, dy, dz, L);
-    px[k] = px[0] + dx; py[k] = py[0] + dy; pz[k] = pz[0] + dz;
-  }
-  float m_o = a.m_o, m_h = a.m_h;
-  float io = m_o > 0.f ? 1.0f / m_o : 0.f, ih = m_h > 0.f ? 1.0f / m_h : 0.f;
-  float vx[3], vy[3], vz[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    vx[k] = REG ? vin[k][0] : a.vx[ids[k]];
-    vy[k] = REG ? vin[k][1] : a.vy[ids[k]];
-    vz[k] = REG ? vin[k][2] : a.vz[ids[k]];
-  }
-  float r1x = px[0] - px[1], r1y = py[0] - py[1], r1z = pz[0] - pz[1];
-  float r2x = px[0] - px[2], r2y = py[0] - py[2], r2z = pz[0] - pz[2];
-  float r3x = px[1] - px[2], r3y = py[1] - py[2], r3z = pz[1] - pz[2];
-  float d11 = r1x * r1x + r1y * r1y + r1z * r1z;
-  float d22 = r2x * r2x + r2y * r2y + r2z * r2z;
-  float d33 = r3x * r3x + r3y * r3y + r3z * r3z;
-  float d12 = r1x * r2x + r1y * r2y + r1z * r2z;
-  float d13 = r1x * r3x + r1y * r3y + r1z * r3z;
-  float d23 = r2x * r3x + r2y * r3y + r2z * r3z;
-  float b1 = (vx[0] - vx[1]) * r1x + (vy[0] - vy[1]) * r1y + (vz[0] - vz[1]) * r1z;
-  float b2 = (vx[0] - vx[2]) * r2x + (vy[0] - vy[2]) * r2y + (vz[0] - vz[2]) * r2z;
-  float b3 = (vx[1] - vx[2]) * r3x + (vy[1] - vy[2]) * r3y + (vz[1] - vz[2]) * r3z;
-  float ioh = io + ih;
-  float m[3][3] = {{d11 * ioh, d12 * io, -d13 * ih},
-                   {d12 * io, d22 * ioh, d23 * ih},
-                   {-d13 * ih, d23 * ih, 2.0f * d33 * ih}};
-  float rhs[3] = {-b1, -b2, -b3};
-  float det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
-              m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
-              m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-  float gg[3] = {0.f, 0.f, 0.f};
-  if (det != 0.f) {
-    float idet = 1.0f / det;
-    for (int c = 0; c < 3; ++c) {
-      float t[3][3];
-      for (int r = 0; r < 3; ++r)
-        for (int q = 0; q < 3; ++q) t[r][q] = (q == c) ? rhs[r] : m[r][q];
-      float dc = t[0][0] * (t[1][1] * t[2][2] - t[1][2] * t[2][1]) -
-                 t[0][1] * (t[1][0] * t[2][2] - t[1][2] * t[2][0]) +
-                 t[0][2] * (t[1][0] * t[2][1] - t[1][1] * t[2][0]);
-      gg[c] = dc * idet;
-    }
-  }
-  float g1 = gg[0], g2 = gg[1], g3 = gg[2];
-  float dv[3][3] = {
-      {(g1 * r1x + g2 * r2x) * io, (g1 * r1y + g2 * r2y) * io, (g1 * r1z + g2 * r2z) * io},
-      {(-g1 * r1x + g3 * r3x) * ih, (-g1 * r1y + g3 * r3y) * ih, (-g1 * r1z + g3 * r3z) * ih},
-      {(-g2 * r2x - g3 * r3x) * ih, (-g2 * r2y - g3 * r3y) * ih, (-g2 * r2z - g3 * r3z) * ih}};
-  for (int k = 0; k < 3; ++k) {
-    a.vx[ids[k]] = vx[k] + dv[k][0];
-    a.vy[ids[k]] = vy[k] + dv[k][1];
-    a.vz[ids[k]] = vz[k] + dv[k][2];
-  }
-  if (ACC && a.dt > 0.f) {
-    float idt = 1.0f / a.dt;
-    float tot = m_o + 2.0f * m_h;
-    float iM = tot > 0.f ? 1.0f / tot : 0.f;
-    float cx = (m_o * px[0] + m_h * px[1] + m_h * px[2]) * iM;
-    float cy = (m_o * py[0] + m_h * py[1] + m_h * py[2]) * iM;
-    float cz = (m_o * pz[0] + m_h * pz[1] + m_h * pz[2]) * iM;
-    float ma[3] = {m_o, m_h, m_h};
-    for (int k = 0; k < 3; ++k) {
-      float sc = ma[k] * idt;
-      float w0 = REG ? win[k] : a.w_cons[ids[k]];
-      a.w_cons[ids[k]] = w0 + sc * (dv[k][0] * (px[k] - cx) + dv[k][1] * (py[k] - cy) +
-                                    dv[k][2] * (pz[k] - cz));
-    }
-  } else if (REG) {
-    // the position stage left these to this stage
-    for (int k = 0; k < 3; ++k) a.w_cons[ids[k]] = win[k];
-  }
-}
-
-template <bool ACC>
-__global__ void k_settle_velocities(HdSettleArgs a) {
-  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_groups) return;
-  HdLattice L = hd_load_lattice(a.lattice);
-  unsigned ids[3] = {a.groups[3 * g], a.groups[3 * g + 1], a.groups[3 * g + 2]};
-  float4 pin[3] = {};
-  float vin[3][3] = {}, win[3] = {};
-  settle_vel_group<ACC, false>(a, L, ids, pin, vin, win);
 }
 
 void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a) {
@@ -808,36 +857,28 @@ __device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, 
 }
 
 // ---------------------------------------------------------------------
-// SHAKE / RATTLE for general constraint groups (kernels/shake.cu:96-490,
-// src/integrator/shake.rs): one thread per group, Gauss-Seidel sweeps in
-// topology order with the reference's f32 operation order; the group's
-// atoms and constraints are staged in LDS at a fixed per-thread stride,
-// which lifts the reference's register-bound caps (8 atoms, 12
-// constraints) to HD_SHAKE_MAX_ATOMS / HD_SHAKE_MAX_CONS so that whole
-// polymer chains can be one group.
+// SHAKE / RATTLE for general constraint groups (the constraint contract of
+// kernels/shake.cu:96-490 and src/integrator/shake.rs: Gauss-Seidel
+// sweeps, gradients from the pre-drift snapshot, the same tolerances and
+// sweep cap, the position- and velocity-level constraint virial).
+//
+// One wave per group; lane k owns atom k and constraint k (a group holds at
+// most 64 of each, well beyond the reference's 8 atoms / 12 constraints,
+// so whole polymer chains are one group).
+//   * Frame: lane k takes the bond vector b_k = minimg(x_k - x_(k-1)), an
+//     inclusive wave scan gives P_k = b_1 + ... + b_k, and the group is
+//     solved in coordinates P_k - P_mid relative to its middle atom.  A
+//     chain longer than half the box does not fold back onto one atom's
+//     image, and |r|^2 - d^2 resolves to the f32 spacing of the group's
+//     extent rather than of its place in the box.
+//   * Sweeps: the host colours the constraints so that constraints of one
+//     colour share no atom; a sweep visits the colours in order and the
+//     lanes of one colour update their disjoint atoms at once, which is a
+//     sequential Gauss-Seidel sweep in (colour, index) order.
+//   * Every atom moves by its correction; centre-of-mass sums are xor
+//     butterflies read from lane 0.
+// The CPU oracle restates the scan, sweep and butterfly orders.
 // ---------------------------------------------------------------------
-unsigned hd_shake_lds_per_thread(unsigned max_atoms, unsigned max_cons) {
-  return 4u * (11u * max_atoms + 6u * max_cons);
-}
-
-__device__ __forceinline__ void shake_min_image_to(float ax, float ay, float az, float& bx,
-                                                   float& by, float& bz, const HdLattice& L) {
-  float dx = bx - ax, dy = by - ay, dz = bz - az;
-  hd_min_image(dx, dy, dz, L);
-  bx = ax + dx; by = ay + dy; bz = az + dz;
-}
-
-__device__ __forceinline__ void shake_com(const float* x, const float* y, const float* z,
-                                          const float* m, unsigned n, float& cx, float& cy,
-                                          float& cz) {
-  float total = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-  for (unsigned a = 0; a < n; ++a) {
-    sx += m[a] * x[a]; sy += m[a] * y[a]; sz += m[a] * z[a];
-    total += m[a];
-  }
-  float inv = total > 0.f ? 1.0f / total : 0.f;
-  cx = sx * inv; cy = sy * inv; cz = sz * inv;
-}
 
 __device__ __forceinline__ void shake_stats(unsigned* st, unsigned it, int max_iter) {
   atomicAdd(&st[0], it);
@@ -845,203 +886,9 @@ __device__ __forceinline__ void shake_stats(unsigned* st, unsigned it, int max_i
   if ((int)it >= max_iter) atomicAdd(&st[2], 1u);
 }
 
-// Groups within the reference's 8-atom cap: one thread per group, the
-// reference's sequential sweep.
-__global__ void k_shake_positions(HdShakeArgs a) {
-  extern __shared__ float sh[];
-  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n_short) return;
-  unsigned g = a.slist[t];
-  const unsigned MA = a.max_atoms, MC = a.max_cons;
-  float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
-  float *x0 = base, *y0 = x0 + MA, *z0 = y0 + MA;
-  float *xu = z0 + MA, *yu = xu + MA, *zu = yu + MA;
-  float *xc = zu + MA, *yc = xc + MA, *zc = yc + MA;
-  float *m = zc + MA, *im = m + MA;
-  float *gx = im + MA, *gy = gx + MC, *gz = gy + MC, *ip = gz + MC;
-  unsigned* cij = (unsigned*)(ip + MC);
-  float* lam = ip + 2u * MC;  // accumulated multiplier per constraint
-  HdLattice L = hd_load_lattice(a.lattice);
-  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
-  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
-  for (unsigned k = 0; k < na; ++k) {
-    unsigned i = a.gatoms[a0 + k];
-    float4 p = a.posq[i], q = a.snap[i];
-    xu[k] = p.x; yu[k] = p.y; zu[k] = p.z;
-    x0[k] = q.x; y0[k] = q.y; z0[k] = q.z;
-    float mk = a.mass[i];
-    m[k] = mk;
-    im[k] = mk > 0.f ? 1.0f / mk : 0.f;
-  }
-  // one image for the whole group: relative to atom 0 (shake.cu:213-219)
-  for (unsigned k = 1; k < na; ++k) {
-    shake_min_image_to(x0[0], y0[0], z0[0], x0[k], y0[k], z0[k], L);
-    shake_min_image_to(xu[0], yu[0], zu[0], xu[k], yu[k], zu[k], L);
-  }
-  for (unsigned k = 0; k < na; ++k) { xc[k] = xu[k]; yc[k] = yu[k]; zc[k] = zu[k]; }
-  for (unsigned k = 0; k < nc; ++k) {
-    unsigned pk = a.cij[c0 + k];
-    unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
-    cij[k] = pk;
-    gx[k] = x0[li] - x0[lj];
-    gy[k] = y0[li] - y0[lj];
-    gz[k] = z0[li] - z0[lj];
-    ip[k] = im[li] + im[lj];
-    lam[k] = 0.f;
-  }
-  if (a.lam0) {
-    // warm start: move along every constraint gradient by a predicted
-    // multiplier, in topology order, before the Gauss-Seidel sweeps
-    for (unsigned k = 0; k < nc; ++k) {
-      unsigned pk = cij[k];
-      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
-      float lambda = ip[k] > 0.f ? a.lam0[c0 + k] / ip[k] : 0.f;
-      float ki = lambda * im[li], kj = lambda * im[lj];
-      xc[li] -= ki * gx[k]; yc[li] -= ki * gy[k]; zc[li] -= ki * gz[k];
-      xc[lj] += kj * gx[k]; yc[lj] += kj * gy[k]; zc[lj] += kj * gz[k];
-      lam[k] = lambda;
-    }
-  }
-  int it = 0;
-  for (; it < a.max_iter; ++it) {
-    bool converged = true;
-    for (unsigned k = 0; k < nc; ++k) {
-      unsigned pk = cij[k];
-      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
-      float dx = xc[li] - xc[lj], dy = yc[li] - yc[lj], dz = zc[li] - zc[lj];
-      float dist2 = dx * dx + dy * dy + dz * dz;
-      float sigma = dist2 - a.cr2[c0 + k];
-      if (fabsf(sigma) > a.tol) {
-        converged = false;
-        float ddot = dx * gx[k] + dy * gy[k] + dz * gz[k];
-        float denom = 2.0f * ddot * ip[k];
-        if (denom == 0.f) continue;
-        float lambda = sigma / denom;
-        float ki = lambda * im[li], kj = lambda * im[lj];
-        xc[li] -= ki * gx[k]; yc[li] -= ki * gy[k]; zc[li] -= ki * gz[k];
-        xc[lj] += kj * gx[k]; yc[lj] += kj * gy[k]; zc[lj] += kj * gz[k];
-        lam[k] += lambda;
-      }
-    }
-    if (converged) break;
-  }
-  if (a.stats) shake_stats(a.stats, (unsigned)it, a.max_iter);
-  if (a.lam_out)
-    for (unsigned k = 0; k < nc; ++k) a.lam_out[c0 + k] = lam[k] * ip[k];
-  float cxg, cyg, czg;
-  shake_com(xc, yc, zc, m, na, cxg, cyg, czg);
-  float inv_dt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
-  float inv_dt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
-  for (unsigned k = 0; k < na; ++k) {
-    unsigned i = a.gatoms[a0 + k];
-    float dxg = xc[k] - xu[k], dyg = yc[k] - yu[k], dzg = zc[k] - zu[k];
-    a.vx[i] += dxg * inv_dt; a.vy[i] += dyg * inv_dt; a.vz[i] += dzg * inv_dt;
-    float4 p = a.posq[i];
-    if (k == 0) { p.x = xc[0]; p.y = yc[0]; p.z = zc[0]; }
-    else { p.x += dxg; p.y += dyg; p.z += dzg; }
-    a.posq[i] = p;
-    float rx = xc[k] - cxg, ry = yc[k] - cyg, rz = zc[k] - czg;
-    float scale = m[k] * inv_dt2;
-    a.w_cons[i] = scale * (dxg * rx + dyg * ry + dzg * rz);
-  }
-}
-
-template <bool ACC>
-__global__ void k_rattle_velocities(HdShakeArgs a) {
-  extern __shared__ float sh[];
-  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n_short) return;
-  unsigned g = a.slist[t];
-  const unsigned MA = a.max_atoms, MC = a.max_cons;
-  float* base = sh + (size_t)threadIdx.x * (11u * MA + 6u * MC);
-  float *px = base, *py = px + MA, *pz = py + MA;
-  float *v0 = pz + MA, *v1 = v0 + MA, *v2 = v1 + MA;
-  float *dv0 = v2 + MA, *dv1 = dv0 + MA, *dv2 = dv1 + MA;
-  float *m = dv2 + MA, *im = m + MA;
-  float *dx = im + MA, *dy = dx + MC, *dz = dy + MC, *d2 = dz + MC;
-  float* ipc = d2 + MC;  // 1/m_i + 1/m_j per constraint (5th constraint slot)
-  HdLattice L = hd_load_lattice(a.lattice);
-  unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
-  unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
-  for (unsigned k = 0; k < na; ++k) {
-    unsigned i = a.gatoms[a0 + k];
-    float4 p = a.posq[i];
-    px[k] = p.x; py[k] = p.y; pz[k] = p.z;
-    v0[k] = a.vx[i]; v1[k] = a.vy[i]; v2[k] = a.vz[i];
-    dv0[k] = dv1[k] = dv2[k] = 0.f;
-    float mk = a.mass[i];
-    m[k] = mk;
-    im[k] = mk > 0.f ? 1.0f / mk : 0.f;
-  }
-  for (unsigned k = 1; k < na; ++k) shake_min_image_to(px[0], py[0], pz[0], px[k], py[k], pz[k], L);
-  for (unsigned k = 0; k < nc; ++k) {
-    unsigned pk = a.cij[c0 + k];
-    unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
-    dx[k] = px[li] - px[lj];
-    dy[k] = py[li] - py[lj];
-    dz[k] = pz[li] - pz[lj];
-    d2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
-    ipc[k] = im[li] + im[lj];
-  }
-  int it = 0;
-  for (; it < a.max_iter; ++it) {
-    bool converged = true;
-    for (unsigned k = 0; k < nc; ++k) {
-      unsigned pk = a.cij[c0 + k];
-      unsigned li = pk & 0xFFu, lj = (pk >> 8) & 0xFFu;
-      float vxr = v0[li] - v0[lj], vyr = v1[li] - v1[lj], vzr = v2[li] - v2[lj];
-      float vrel = vxr * dx[k] + vyr * dy[k] + vzr * dz[k];
-      if (fabsf(vrel) > a.rattle_tol) {
-        converged = false;
-        float denom = d2[k] * ipc[k];
-        if (denom == 0.f) continue;
-        float mu = vrel / denom;
-        float ki = mu * im[li], kj = mu * im[lj];
-        v0[li] -= ki * dx[k]; v1[li] -= ki * dy[k]; v2[li] -= ki * dz[k];
-        v0[lj] += kj * dx[k]; v1[lj] += kj * dy[k]; v2[lj] += kj * dz[k];
-        dv0[li] -= ki * dx[k]; dv1[li] -= ki * dy[k]; dv2[li] -= ki * dz[k];
-        dv0[lj] += kj * dx[k]; dv1[lj] += kj * dy[k]; dv2[lj] += kj * dz[k];
-      }
-    }
-    if (converged) break;
-  }
-  if (a.stats) shake_stats(a.stats + 3, (unsigned)it, a.max_iter);
-  if (ACC && a.dt > 0.f) {
-    float inv_dt = 1.0f / a.dt;
-    float cx, cy, cz;
-    shake_com(px, py, pz, m, na, cx, cy, cz);
-    for (unsigned k = 0; k < na; ++k) {
-      float rx = px[k] - cx, ry = py[k] - cy, rz = pz[k] - cz;
-      float scale = m[k] * inv_dt;
-      unsigned i = a.gatoms[a0 + k];
-      a.w_cons[i] += scale * (dv0[k] * rx + dv1[k] * ry + dv2[k] * rz);
-    }
-  }
-  for (unsigned k = 0; k < na; ++k) {
-    unsigned i = a.gatoms[a0 + k];
-    a.vx[i] = v0[k]; a.vy[i] = v1[k]; a.vz[i] = v2[k];
-  }
-}
-
-// Groups beyond the reference's 8-atom cap (chains): one wave per group.
-// Lane k owns atom k and constraint k (a group holds at most 64 of each).
-//   * Frame: lane k takes the bond vector d_k = minimg(x_k - x_(k-1)), an
-//     inclusive wave scan gives S_k, and the group is solved in
-//     coordinates S_k - S_mid relative to the middle atom, so a chain longer
-//     than half the box does not fold back and |r|^2 - d^2 resolves to the
-//     f32 spacing of the group's extent rather than of its place in the box.
-//   * Sweeps: the host colours the constraints so that constraints of one
-//     colour share no atom (a chain alternates two colours); a sweep visits
-//     the colours in order and the lanes of one colour update their
-//     disjoint atoms at once, which is a sequential Gauss-Seidel sweep in
-//     (colour, index) order.
-//   * Every atom moves by its delta; centre-of-mass sums are xor butterflies
-//     read from lane 0.
-// The CPU oracle restates the scan and butterfly orders.
-
 // inclusive scan over the wave: lane k gets v_0 + ... + v_k, grouped by
 // the offsets 1, 2, 4, ..., 32
-__device__ __forceinline__ float chain_scan(float v, unsigned lane) {
+__device__ __forceinline__ float wave_scan(float v, unsigned lane) {
 #pragma unroll
   for (unsigned off = 1; off < 64; off <<= 1) {
     float u = __shfl_up(v, off);
@@ -1051,207 +898,221 @@ __device__ __forceinline__ float chain_scan(float v, unsigned lane) {
 }
 
 // sum over the wave as lane 0 of an xor butterfly, broadcast
-__device__ __forceinline__ float chain_sum(float v) {
+__device__ __forceinline__ float wave_total(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
   return __shfl(v, 0);
 }
 
 // the group frame of the wrapped coordinates (x, y, z) of lane k's atom
-__device__ __forceinline__ void chain_frame(float& x, float& y, float& z, unsigned lane,
+__device__ __forceinline__ void group_frame(float& x, float& y, float& z, unsigned lane,
                                             unsigned na, const HdLattice& L) {
-  float px = __shfl_up(x, 1u), py = __shfl_up(y, 1u), pz = __shfl_up(z, 1u);
-  float dx = 0.f, dy = 0.f, dz = 0.f;
+  float xl = __shfl_up(x, 1u), yl = __shfl_up(y, 1u), zl = __shfl_up(z, 1u);
+  float bx = 0.f, by = 0.f, bz = 0.f;
   if (lane >= 1 && lane < na) {
-    dx = x - px; dy = y - py; dz = z - pz;
-    hd_min_image(dx, dy, dz, L);
+    bx = x - xl; by = y - yl; bz = z - zl;
+    hd_min_image(bx, by, bz, L);
   }
-  dx = chain_scan(dx, lane);
-  dy = chain_scan(dy, lane);
-  dz = chain_scan(dz, lane);
+  bx = wave_scan(bx, lane);
+  by = wave_scan(by, lane);
+  bz = wave_scan(bz, lane);
   unsigned mid = na / 2;
-  x = dx - __shfl(dx, (int)mid);
-  y = dy - __shfl(dy, (int)mid);
-  z = dz - __shfl(dz, (int)mid);
+  x = bx - __shfl(bx, (int)mid);
+  y = by - __shfl(by, (int)mid);
+  z = bz - __shfl(bz, (int)mid);
 }
 
-__device__ __forceinline__ void chain_com(float x, float y, float z, float m, float& cx,
-                                          float& cy, float& cz) {
-  float total = chain_sum(m);
-  float inv = total > 0.f ? 1.0f / total : 0.f;
-  cx = chain_sum(m * x) * inv;
-  cy = chain_sum(m * y) * inv;
-  cz = chain_sum(m * z) * inv;
+__device__ __forceinline__ void wave_com(float x, float y, float z, float m, float& cx,
+                                         float& cy, float& cz) {
+  float mtot = wave_total(m);
+  float inv = mtot > 0.f ? 1.0f / mtot : 0.f;
+  cx = wave_total(m * x) * inv;
+  cy = wave_total(m * y) * inv;
+  cz = wave_total(m * z) * inv;
 }
 
-__global__ __launch_bounds__(64) void k_shake_positions_long(HdShakeArgs a) {
-  __shared__ float x0[64], y0[64], z0[64], xc[64], yc[64], zc[64], ims[64];
+// One constraint's end atoms and colour, from the packed (i | j << 8) form
+struct GroupBond {
+  unsigned ai, aj, colour;
+};
+
+__device__ __forceinline__ GroupBond group_bond(const HdShakeArgs& a, unsigned c0, unsigned lane,
+                                                unsigned nc) {
+  GroupBond b{0u, 0u, 0xFFFFFFFFu};
+  if (lane < nc) {
+    unsigned packed = a.cij[c0 + lane];
+    b.ai = packed & 0xFFu;
+    b.aj = (packed >> 8) & 0xFFu;
+    b.colour = a.color[c0 + lane];
+  }
+  return b;
+}
+
+// Positions: the drifted atoms (pos*) are moved along the snapshot bonds
+// (g*) until every |r|^2 - d^2 is within the tolerance.
+__global__ __launch_bounds__(64) void k_shake_group(HdShakeArgs a) {
+  __shared__ float sx[64], sy[64], sz[64], posx[64], posy[64], posz[64], winv[64];
   const unsigned lane = threadIdx.x;
-  const unsigned g = a.llist[blockIdx.x];
+  const unsigned g = blockIdx.x;
   HdLattice L = hd_load_lattice(a.lattice);
   unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
   unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
-  unsigned ia = 0;
-  float ux = 0.f, uy = 0.f, uz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, mk = 0.f, imk = 0.f;
+  unsigned atom = 0;
+  float ux = 0.f, uy = 0.f, uz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, mk = 0.f, imk = 0.f;
   if (lane < na) {
-    ia = a.gatoms[a0 + lane];
-    float4 p = a.posq[ia], q = a.snap[ia];
+    atom = a.gatoms[a0 + lane];
+    float4 p = a.posq[atom], q = a.snap[atom];
     ux = p.x; uy = p.y; uz = p.z;
-    sx = q.x; sy = q.y; sz = q.z;
-    mk = a.mass[ia];
+    ox = q.x; oy = q.y; oz = q.z;
+    mk = a.mass[atom];
     imk = mk > 0.f ? 1.0f / mk : 0.f;
   }
-  chain_frame(sx, sy, sz, lane, na, L);
-  chain_frame(ux, uy, uz, lane, na, L);
-  x0[lane] = sx; y0[lane] = sy; z0[lane] = sz;
-  xc[lane] = ux; yc[lane] = uy; zc[lane] = uz;
-  ims[lane] = imk;
+  group_frame(ox, oy, oz, lane, na, L);
+  group_frame(ux, uy, uz, lane, na, L);
+  sx[lane] = ox; sy[lane] = oy; sz[lane] = oz;
+  posx[lane] = ux; posy[lane] = uy; posz[lane] = uz;
+  winv[lane] = imk;
   __syncthreads();
-  unsigned li = 0, lj = 0, col = 0xFFFFFFFFu;
-  float gx = 0.f, gy = 0.f, gz = 0.f, ip = 0.f, r2 = 0.f, lam = 0.f;
+  const GroupBond b = group_bond(a, c0, lane, nc);
+  float gx = 0.f, gy = 0.f, gz = 0.f, wsum = 0.f, dd = 0.f, acc = 0.f;
   if (lane < nc) {
-    unsigned pk = a.cij[c0 + lane];
-    li = pk & 0xFFu;
-    lj = (pk >> 8) & 0xFFu;
-    col = a.color[c0 + lane];
-    gx = x0[li] - x0[lj];
-    gy = y0[li] - y0[lj];
-    gz = z0[li] - z0[lj];
-    ip = ims[li] + ims[lj];
-    r2 = a.cr2[c0 + lane];
+    gx = sx[b.ai] - sx[b.aj];
+    gy = sy[b.ai] - sy[b.aj];
+    gz = sz[b.ai] - sz[b.aj];
+    wsum = winv[b.ai] + winv[b.aj];
+    dd = a.cr2[c0 + lane];
   }
-  const float imi = ims[li], imj = ims[lj];
+  const float wi = winv[b.ai], wj = winv[b.aj];
   if (a.lam0) {
-    for (unsigned q = 0; q < a.n_colors; ++q) {
-      if (col == q) {
-        float lambda = ip > 0.f ? a.lam0[c0 + lane] / ip : 0.f;
-        float ki = lambda * imi, kj = lambda * imj;
-        xc[li] -= ki * gx; yc[li] -= ki * gy; zc[li] -= ki * gz;
-        xc[lj] += kj * gx; yc[lj] += kj * gy; zc[lj] += kj * gz;
-        lam = lambda;
+    // warm start: one move by the predicted multiplier, in sweep order
+    for (unsigned col = 0; col < a.n_colors; ++col) {
+      if (b.colour == col) {
+        float lam = wsum > 0.f ? a.lam0[c0 + lane] / wsum : 0.f;
+        float si = lam * wi, sj = lam * wj;
+        posx[b.ai] -= si * gx; posy[b.ai] -= si * gy; posz[b.ai] -= si * gz;
+        posx[b.aj] += sj * gx; posy[b.aj] += sj * gy; posz[b.aj] += sj * gz;
+        acc = lam;
       }
       __syncthreads();
     }
   }
-  int it = 0;
-  for (; it < a.max_iter; ++it) {
-    int bad = 0;
-    for (unsigned q = 0; q < a.n_colors; ++q) {
-      if (col == q) {
-        float dx = xc[li] - xc[lj], dy = yc[li] - yc[lj], dz = zc[li] - zc[lj];
-        float dist2 = dx * dx + dy * dy + dz * dz;
-        float sigma = dist2 - r2;
-        if (fabsf(sigma) > a.tol) {
-          bad = 1;
-          float ddot = dx * gx + dy * gy + dz * gz;
-          float denom = 2.0f * ddot * ip;
-          if (denom != 0.f) {
-            float lambda = sigma / denom;
-            float ki = lambda * imi, kj = lambda * imj;
-            xc[li] -= ki * gx; yc[li] -= ki * gy; zc[li] -= ki * gz;
-            xc[lj] += kj * gx; yc[lj] += kj * gy; zc[lj] += kj * gz;
-            lam += lambda;
+  int sweeps = 0;
+  for (; sweeps < a.max_iter; ++sweeps) {
+    int open = 0;
+    for (unsigned col = 0; col < a.n_colors; ++col) {
+      if (b.colour == col) {
+        float rx = posx[b.ai] - posx[b.aj], ry = posy[b.ai] - posy[b.aj],
+              rz = posz[b.ai] - posz[b.aj];
+        float rr = rx * rx + ry * ry + rz * rz;
+        float resid = rr - dd;
+        if (fabsf(resid) > a.tol) {
+          open = 1;
+          float proj = rx * gx + ry * gy + rz * gz;
+          float slope = 2.0f * proj * wsum;
+          if (slope != 0.f) {
+            float lam = resid / slope;
+            float si = lam * wi, sj = lam * wj;
+            posx[b.ai] -= si * gx; posy[b.ai] -= si * gy; posz[b.ai] -= si * gz;
+            posx[b.aj] += sj * gx; posy[b.aj] += sj * gy; posz[b.aj] += sj * gz;
+            acc += lam;
           }
         }
       }
       __syncthreads();
     }
-    if (!__syncthreads_or(bad)) break;
+    if (!__syncthreads_or(open)) break;
   }
-  if (lane == 0 && a.stats) shake_stats(a.stats, (unsigned)it, a.max_iter);
-  if (a.lam_out && lane < nc) a.lam_out[c0 + lane] = lam * ip;
-  const float cxl = xc[lane], cyl = yc[lane], czl = zc[lane];
+  if (lane == 0 && a.stats) shake_stats(a.stats, (unsigned)sweeps, a.max_iter);
+  if (a.lam_out && lane < nc) a.lam_out[c0 + lane] = acc * wsum;
+  const float fx = posx[lane], fy = posy[lane], fz = posz[lane];
   float cmx, cmy, cmz;
-  chain_com(cxl, cyl, czl, mk, cmx, cmy, cmz);
+  wave_com(fx, fy, fz, mk, cmx, cmy, cmz);
   if (lane < na) {
-    float inv_dt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
-    float inv_dt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
-    float dxg = cxl - ux, dyg = cyl - uy, dzg = czl - uz;
-    a.vx[ia] += dxg * inv_dt; a.vy[ia] += dyg * inv_dt; a.vz[ia] += dzg * inv_dt;
-    float4 p = a.posq[ia];
-    p.x += dxg; p.y += dyg; p.z += dzg;
-    a.posq[ia] = p;
-    float rx = cxl - cmx, ry = cyl - cmy, rz = czl - cmz;
-    float scale = mk * inv_dt2;
-    a.w_cons[ia] = scale * (dxg * rx + dyg * ry + dzg * rz);
+    float rdt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
+    float rdt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+    float mx = fx - ux, my = fy - uy, mz = fz - uz;
+    a.vx[atom] += mx * rdt; a.vy[atom] += my * rdt; a.vz[atom] += mz * rdt;
+    float4 p = a.posq[atom];
+    p.x += mx; p.y += my; p.z += mz;
+    a.posq[atom] = p;
+    float armx = fx - cmx, army = fy - cmy, armz = fz - cmz;
+    a.w_cons[atom] = mk * rdt2 * (mx * armx + my * army + mz * armz);
   }
 }
 
+// Velocities: relative velocities along the final bonds are removed until
+// every |v_rel . r| is within the tolerance; with ACC the velocity-level
+// virial is added to the position-level one.
 template <bool ACC>
-__global__ __launch_bounds__(64) void k_rattle_velocities_long(HdShakeArgs a) {
-  __shared__ float px[64], py[64], pz[64], v0[64], v1[64], v2[64];
-  __shared__ float dv0[64], dv1[64], dv2[64], ims[64];
+__global__ __launch_bounds__(64) void k_rattle_group(HdShakeArgs a) {
+  __shared__ float px[64], py[64], pz[64], vxs[64], vys[64], vzs[64];
+  __shared__ float kx[64], ky[64], kz[64], winv[64];
   const unsigned lane = threadIdx.x;
-  const unsigned g = a.llist[blockIdx.x];
+  const unsigned g = blockIdx.x;
   HdLattice L = hd_load_lattice(a.lattice);
   unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
   unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
-  unsigned ia = 0;
+  unsigned atom = 0;
   float x = 0.f, y = 0.f, z = 0.f, mk = 0.f, imk = 0.f;
-  v0[lane] = v1[lane] = v2[lane] = 0.f;
+  vxs[lane] = vys[lane] = vzs[lane] = 0.f;
   if (lane < na) {
-    ia = a.gatoms[a0 + lane];
-    float4 p = a.posq[ia];
+    atom = a.gatoms[a0 + lane];
+    float4 p = a.posq[atom];
     x = p.x; y = p.y; z = p.z;
-    v0[lane] = a.vx[ia]; v1[lane] = a.vy[ia]; v2[lane] = a.vz[ia];
-    mk = a.mass[ia];
+    vxs[lane] = a.vx[atom]; vys[lane] = a.vy[atom]; vzs[lane] = a.vz[atom];
+    mk = a.mass[atom];
     imk = mk > 0.f ? 1.0f / mk : 0.f;
   }
-  chain_frame(x, y, z, lane, na, L);
+  group_frame(x, y, z, lane, na, L);
   px[lane] = x; py[lane] = y; pz[lane] = z;
-  dv0[lane] = dv1[lane] = dv2[lane] = 0.f;
-  ims[lane] = imk;
+  kx[lane] = ky[lane] = kz[lane] = 0.f;
+  winv[lane] = imk;
   __syncthreads();
-  unsigned li = 0, lj = 0, col = 0xFFFFFFFFu;
-  float dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, ipc = 0.f;
+  const GroupBond b = group_bond(a, c0, lane, nc);
+  float rx = 0.f, ry = 0.f, rz = 0.f, rr = 0.f, wsum = 0.f;
   if (lane < nc) {
-    unsigned pk = a.cij[c0 + lane];
-    li = pk & 0xFFu;
-    lj = (pk >> 8) & 0xFFu;
-    col = a.color[c0 + lane];
-    dx = px[li] - px[lj];
-    dy = py[li] - py[lj];
-    dz = pz[li] - pz[lj];
-    d2 = dx * dx + dy * dy + dz * dz;
-    ipc = ims[li] + ims[lj];
+    rx = px[b.ai] - px[b.aj];
+    ry = py[b.ai] - py[b.aj];
+    rz = pz[b.ai] - pz[b.aj];
+    rr = rx * rx + ry * ry + rz * rz;
+    wsum = winv[b.ai] + winv[b.aj];
   }
-  const float imi = ims[li], imj = ims[lj];
-  int it = 0;
-  for (; it < a.max_iter; ++it) {
-    int bad = 0;
-    for (unsigned q = 0; q < a.n_colors; ++q) {
-      if (col == q) {
-        float vxr = v0[li] - v0[lj], vyr = v1[li] - v1[lj], vzr = v2[li] - v2[lj];
-        float vrel = vxr * dx + vyr * dy + vzr * dz;
-        if (fabsf(vrel) > a.rattle_tol) {
-          bad = 1;
-          float denom = d2 * ipc;
-          if (denom != 0.f) {
-            float mu = vrel / denom;
-            float ki = mu * imi, kj = mu * imj;
-            v0[li] -= ki * dx; v1[li] -= ki * dy; v2[li] -= ki * dz;
-            v0[lj] += kj * dx; v1[lj] += kj * dy; v2[lj] += kj * dz;
-            dv0[li] -= ki * dx; dv1[li] -= ki * dy; dv2[li] -= ki * dz;
-            dv0[lj] += kj * dx; dv1[lj] += kj * dy; dv2[lj] += kj * dz;
+  const float wi = winv[b.ai], wj = winv[b.aj];
+  int sweeps = 0;
+  for (; sweeps < a.max_iter; ++sweeps) {
+    int open = 0;
+    for (unsigned col = 0; col < a.n_colors; ++col) {
+      if (b.colour == col) {
+        float ux = vxs[b.ai] - vxs[b.aj], uy = vys[b.ai] - vys[b.aj], uz = vzs[b.ai] - vzs[b.aj];
+        float along = ux * rx + uy * ry + uz * rz;
+        if (fabsf(along) > a.rattle_tol) {
+          open = 1;
+          float slope = rr * wsum;
+          if (slope != 0.f) {
+            float mu = along / slope;
+            float si = mu * wi, sj = mu * wj;
+            vxs[b.ai] -= si * rx; vys[b.ai] -= si * ry; vzs[b.ai] -= si * rz;
+            vxs[b.aj] += sj * rx; vys[b.aj] += sj * ry; vzs[b.aj] += sj * rz;
+            kx[b.ai] -= si * rx; ky[b.ai] -= si * ry; kz[b.ai] -= si * rz;
+            kx[b.aj] += sj * rx; ky[b.aj] += sj * ry; kz[b.aj] += sj * rz;
           }
         }
       }
       __syncthreads();
     }
-    if (!__syncthreads_or(bad)) break;
+    if (!__syncthreads_or(open)) break;
   }
-  if (lane == 0 && a.stats) shake_stats(a.stats + 3, (unsigned)it, a.max_iter);
+  if (lane == 0 && a.stats) shake_stats(a.stats + 3, (unsigned)sweeps, a.max_iter);
   if (ACC && a.dt > 0.f) {
-    float cx, cy, cz;
-    chain_com(x, y, z, mk, cx, cy, cz);
+    float cmx, cmy, cmz;
+    wave_com(x, y, z, mk, cmx, cmy, cmz);
     if (lane < na) {
-      float rx = x - cx, ry = y - cy, rz = z - cz;
-      float scale = mk * (1.0f / a.dt);
-      a.w_cons[ia] += scale * (dv0[lane] * rx + dv1[lane] * ry + dv2[lane] * rz);
+      float armx = x - cmx, army = y - cmy, armz = z - cmz;
+      a.w_cons[atom] += mk * (1.0f / a.dt) * (kx[lane] * armx + ky[lane] * army + kz[lane] * armz);
     }
   }
   if (lane < na) {
-    a.vx[ia] = v0[lane]; a.vy[ia] = v1[lane]; a.vz[ia] = v2[lane];
+    a.vx[atom] = vxs[lane]; a.vy[atom] = vys[lane]; a.vz[atom] = vzs[lane];
   }
 }
 
@@ -1344,31 +1205,16 @@ void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a) {
 }
 
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a) {
-  if (a.n_short) {
-    unsigned T = a.threads;
-    size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
-    hipLaunchKernelGGL(k_shake_positions, dim3((a.n_short + T - 1) / T), dim3(T), lds, st, a);
-  }
-  if (a.n_long)
-    hipLaunchKernelGGL(k_shake_positions_long, dim3(a.n_long), dim3(64), 0, st, a);
+  if (a.n_groups)
+    hipLaunchKernelGGL(k_shake_group, dim3(a.n_groups), dim3(64), 0, st, a);
 }
 
 void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool acc) {
-  if (a.n_short) {
-    unsigned T = a.threads;
-    size_t lds = (size_t)T * hd_shake_lds_per_thread(a.max_atoms, a.max_cons);
-    dim3 grid((a.n_short + T - 1) / T);
-    if (acc)
-      hipLaunchKernelGGL(k_rattle_velocities<true>, grid, dim3(T), lds, st, a);
-    else
-      hipLaunchKernelGGL(k_rattle_velocities<false>, grid, dim3(T), lds, st, a);
-  }
-  if (a.n_long) {
-    if (acc)
-      hipLaunchKernelGGL(k_rattle_velocities_long<true>, dim3(a.n_long), dim3(64), 0, st, a);
-    else
-      hipLaunchKernelGGL(k_rattle_velocities_long<false>, dim3(a.n_long), dim3(64), 0, st, a);
-  }
+  if (!a.n_groups) return;
+  if (acc)
+    hipLaunchKernelGGL(k_rattle_group<true>, dim3(a.n_groups), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_rattle_group<false>, dim3(a.n_groups), dim3(64), 0, st, a);
 }
 
 // ---------------------------------------------------------------------

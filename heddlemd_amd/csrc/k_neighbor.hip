@@ -712,6 +712,12 @@ unsigned hd_voxel_scan_parts(unsigned nvox) { return cdiv(nvox, NB_SCAN_CHUNK); 
 // retire quickly, and on rebuild steps the loops cover the work.
 #define NB_GATED_WGS 512u
 
+void hd_launch_disp_check(hipStream_t st, const HdNeighborArgs& a) {
+  const unsigned T = 256;
+  hipLaunchKernelGGL(k_disp_check, dim3(cdiv(a.n, T) ? cdiv(a.n, T) : 1), dim3(T), 0, st,
+                     a.posq, a.ref_pos, a.lattice, a.ref_lattice, a.half_skin, a.status, a.n);
+}
+
 void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a, bool check) {
   const unsigned T = 256;
   unsigned gridN = min(cdiv(a.n, T), NB_GATED_WGS);

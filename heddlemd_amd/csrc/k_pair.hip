@@ -177,8 +177,10 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
     float qj = pq.w;
     float dx = pix - pq.x, dy = piy - pq.y, dz = piz - pq.z;
     float r2 = dx * dx + dy * dy + dz * dz;
-    float inv_r = __builtin_amdgcn_rsqf(r2);
-    float inv_r2 = inv_r * inv_r;
+    // LJ only: 1/r^2 from one reciprocal (half the rounding of squaring a
+    // reciprocal square root, and one instruction fewer)
+    float inv_r = COUL ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    float inv_r2 = COUL ? inv_r * inv_r : __builtin_amdgcn_rcpf(r2);
     float ir6 = inv_r2 * inv_r2 * inv_r2;
     float lj = ir6 * (ljc.x * ir6 - ljc.y);
     float qq = 0.f, gauss = 0.f, poly = 0.f, factor;

@@ -210,6 +210,18 @@ class Engine:
     def setup(self):
         self._chk(self._lib.hd_setup(self._ctx), "hd_setup")
 
+    def set_positions(self, posq):
+        self._chk(self._lib.hd_set_positions(self._ctx, _c(posq, np.float32).reshape(-1)),
+                  "hd_set_positions")
+
+    def neighbor_pre_step(self):
+        st = C.c_uint32(0)
+        self._chk(self._lib.hd_neighbor_pre_step(self._ctx, C.byref(st)), "hd_neighbor_pre_step")
+        return int(st.value)
+
+    def set_fused_settle(self, on):
+        self._chk(self._lib.hd_set_fused_settle(self._ctx, 1 if on else 0), "hd_set_fused_settle")
+
     def use_graph(self, on):
         self._chk(self._lib.hd_use_graph(self._ctx, int(bool(on))), "hd_use_graph")
 

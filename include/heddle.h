@@ -150,6 +150,19 @@ int hd_setup(hd_ctx *ctx);
  * src/forces/mod.rs:662-757); want_scalars also fills per-atom energy and
  * virial and the totals. */
 int hd_force_eval(hd_ctx *ctx, int want_scalars);
+/* Replace the positions (and charges) after hd_setup: 4n floats x, y, z, q
+ * (the position upload of src/gpu/buffers.rs ParticleBuffers).  The next
+ * evaluation recomputes the forces; the neighbour list is rebuilt when the
+ * displacement check asks for it. */
+int hd_set_positions(hd_ctx *ctx, const float *posq);
+/* Neighbour pre-step (NeighborListState::pre_step,
+ * src/forces/neighbor_list.rs:1265-1364, and neighbor_displacement_check_flag,
+ * kernels/neighbor.cu:50-75): runs the displacement check of the current
+ * positions against those of the last rebuild and returns the status word:
+ * bit 0 rebuild due (some |x - x_ref| plus sqrt(3) max|dL| exceeds
+ * r_skin / 2), bit 1 high water, bit 3 overflow.  The rebuild itself runs
+ * inside the next force evaluation, gated on bit 0 on the device. */
+int hd_neighbor_pre_step(hd_ctx *ctx, uint32_t *status_out);
 /* Project velocities onto the constraint manifold
  * (Constraint::apply_initial_velocity_projection, settle.rs:406-430). */
 int hd_constrain_velocities(hd_ctx *ctx);
@@ -161,6 +174,10 @@ int hd_step(hd_ctx *ctx, int n_steps, int scalars_last);
  * (the velocity flip of tests/integration_lossless.rs:302-395). */
 int hd_negate_velocities(hd_ctx *ctx);
 int hd_use_graph(hd_ctx *ctx, int enable);
+/* When every atom belongs to a SETTLE water, the drift and both SETTLE
+ * stages run as one kernel (default on); 0 runs the three separate
+ * kernels, with bit-identical results.  No reference counterpart. */
+int hd_set_fused_settle(hd_ctx *ctx, int enable);
 int hd_sync(hd_ctx *ctx);
 
 /* Batched replicas.  A batch advances many independent, set-up contexts

@@ -843,15 +843,25 @@ static void or_long_com(const float *x, const float *y, const float *z, const fl
     *cz = or_lane0_sum(pz, n) * inv;
 }
 
-/* Sweep order of a group's constraints.  Groups within the reference's
- * 8-atom cap: topology order.  Longer groups: greedy colouring in topology
+/* Form of the group solve.  0: the reference's for groups within its
+ * 8-atom cap (one image relative to atom 0, topology-order sweeps,
+ * sequential centre of mass) and the device's wave form beyond it.  1: the
+ * device's wave form for every group (the device runs all groups, whatever
+ * their size, one wave per group). */
+static int or_shake_wave_all = 0;
+
+void or_set_shake_form(int wave_all) { or_shake_wave_all = wave_all != 0; }
+
+static int or_wave_form(int na) { return or_shake_wave_all || na > OR_SHAKE_REF_ATOMS; }
+
+/* Sweep order of a group's constraints.  Reference form: topology order.  Longer groups: greedy colouring in topology
  * order (each constraint takes the lowest colour no earlier constraint on
  * either of its atoms holds), then colour by colour, topology order within
  * a colour; constraints of one colour share no atom, which lets the device
  * run each colour's updates at once. */
 static void or_sweep_order(int na, int nc, const uint8_t *li, const uint8_t *lj, int *order)
 {
-    if (na <= OR_SHAKE_REF_ATOMS) {
+    if (!or_wave_form(na)) {
         for (int k = 0; k < nc; ++k) order[k] = k;
         return;
     }
@@ -905,7 +915,7 @@ void or_shake_positions_f32(int n_groups, const uint32_t *goff, const uint32_t *
          * relative to the middle atom (a chain longer than half the box
          * would fold back onto atom 0's image, and |r|^2 - d^2 resolves to
          * the f32 spacing of the group's extent instead of the box's) */
-        const int lg = na > OR_SHAKE_REF_ATOMS;
+        const int lg = or_wave_form(na);
         if (lg) {
             or_chain_frame(na, x0, y0, z0, L);
             or_chain_frame(na, xu, yu, zu, L);
@@ -1103,7 +1113,7 @@ void or_rattle_velocities_f32(int n_groups, const uint32_t *goff, const uint32_t
             m[a] = mass[i];
             im[a] = m[a] > 0.0f ? 1.0f / m[a] : 0.0f;
         }
-        const int lg = na > OR_SHAKE_REF_ATOMS;
+        const int lg = or_wave_form(na);
         if (lg)
             or_chain_frame(na, px, py, pz, L);
         else

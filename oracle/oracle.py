@@ -99,6 +99,7 @@ def lib():
         L.or_kinetic_energy.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.or_num_threads.restype = C.c_int
         L.or_set_erfc_mode.argtypes = [C.c_int]
+        L.or_set_shake_form.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -275,12 +276,19 @@ class ShakeGroups:
         self.n = len(groups)
 
 
+def set_shake_form(wave_all: bool):
+    """Group solve form: False = the reference's sequential form for groups
+    of up to 8 atoms (the device's wave form beyond); True = the device's
+    wave form (group frame, colour-ordered sweeps) for every group."""
+    lib().or_set_shake_form(1 if wave_all else 0)
+
+
 def shake_positions(G, posq, v, snap, mass, lattice, dt, tol=SHAKE_TOL, max_iter=SHAKE_MAX_ITER,
                     nu0=None, nu_out=None):
     """In-place SHAKE (shake.cu shake_positions), f32; returns (w_cons
     per atom, sweeps per group).  nu0 (one per constraint) warm-starts the
     solve with multipliers nu = lambda (1/m_i + 1/m_j); nu_out receives
-    the total nu of each constraint."""
+    the total nu of each constraint.  The form follows set_shake_form."""
     w = np.zeros(posq.shape[0], np.float32)
     iters = np.zeros(max(G.n, 1), np.int32)
     p0 = None

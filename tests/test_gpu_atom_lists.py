@@ -12,6 +12,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from heddlemd_amd import systems, units as U  # noqa: E402
+from tests.forcecheck import check_forces, force_errors  # noqa: E402
 from heddlemd_amd.engine import engine_for  # noqa: E402
 
 
@@ -28,11 +29,6 @@ def _jittered_argon(seed=1):
     return s
 
 
-def _force_errors(f, ref, band=None):
-    d = np.linalg.norm(f.astype(np.float64) - ref, axis=1)
-    frms = np.sqrt(np.mean(np.sum(ref ** 2, axis=1)))
-    ok = np.ones(len(d), bool) if band is None else ~band
-    return np.sqrt(np.mean(d[ok] ** 2)) / frms, d[ok].max() / frms
 
 
 def test_atom_lists_pair_set_and_argon_forces(oracle):
@@ -57,9 +53,7 @@ def test_atom_lists_pair_set_and_argon_forces(oracle):
     within_rc, _ = oracle.pair_set(s.posq, s.lattice, float(s.lj.cutoff[0, 0]))
     assert abs(useful - len(within_rc)) <= 4
     F, E, W, fband = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice, band=1e-6)
-    rms, mx = _force_errors(f, F, fband)
-    say(f"  argon rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 1e-4
+    check_forces(f, F, fband, "  argon atom lists", say)
     assert sc["pe"] == pytest.approx(E.sum(), rel=1e-5)
     assert sc["virial"] == pytest.approx(W.sum(), rel=1e-5)
     assert pe.sum() == pytest.approx(E.sum(), rel=1e-5)
@@ -79,13 +73,11 @@ def test_atom_lists_water_forces_and_agreement_with_cluster_list(oracle):
                                           excl=s.exclusion_csr(), band=1e-6)
     Fr, Er, Wr, er, wr = oracle.spme_recip(s.posq, s.lattice, s.spme.grid, 4, s.spme.alpha)
     f1, sc1, (pe1, w1) = out[1]
-    rms, mx = _force_errors(f1, Fp + Fr, band)
-    say(f"  atom lists vs oracle: rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 5e-5
+    check_forces(f1, Fp + Fr, band, "  water atom lists vs oracle", say)
     assert sc1["pe"] == pytest.approx((Ep + Er).sum(), rel=2e-6)
     assert sc1["virial"] == pytest.approx((Wp + Wr).sum(), rel=2e-6)
     f0, sc0, (pe0, w0) = out[0]
-    rms01, mx01 = _force_errors(f1, f0.astype(np.float64))
+    rms01, mx01, _ = force_errors(f1, f0.astype(np.float64))
     say(f"  atom lists vs cluster list: rms {rms01:.3e} max {mx01:.3e}; "
         f"pe {sc1['pe']:.9e} vs {sc0['pe']:.9e}")
     assert rms01 <= 1e-6 and mx01 <= 2e-5

@@ -3,9 +3,10 @@ oracle on seeded inputs, plus determinism, time reversal and conservation
 properties at benchmark sizes.
 
 Tolerances (f32 device arithmetic against the FP64 oracle):
-  * forces: RMS |dF| / RMS |F| <= 5e-6 and max |dF| / RMS |F| <= 1e-4
-    (argon) / 5e-5 (water); atoms in a pair whose r^2 lies within 1e-6
-    of a cutoff^2 are excluded (their inclusion depends on rounding order);
+  * forces: max_i |dF_i| / max(|F_i|, F_rms) <= 1e-5 (the FP32-mode bar)
+    and RMS |dF| / F_rms <= 5e-6 (tests/forcecheck.py); atoms in a pair
+    whose r^2 lies within 1e-6 of a cutoff^2 are excluded (their inclusion
+    depends on rounding order);
   * energy and virial totals: relative 1e-5 (argon: the LJ sum cancels ~30x);
   * velocity Verlet f32 update: bit-exact against the scalar restatement;
   * repeated runs, graph vs eager replay: byte-identical state.
@@ -18,6 +19,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from heddlemd_amd import systems, units as U  # noqa: E402
+from tests.forcecheck import check_forces, force_errors  # noqa: E402
 from heddlemd_amd.engine import engine_for  # noqa: E402
 
 
@@ -34,11 +36,6 @@ def _jittered_argon(n_atoms=10000, seed=1):
     return s
 
 
-def _force_errors(f, ref, band=None):
-    d = np.linalg.norm(f.astype(np.float64) - ref, axis=1)
-    frms = np.sqrt(np.mean(np.sum(ref ** 2, axis=1)))
-    ok = np.ones(len(d), bool) if band is None else ~band
-    return np.sqrt(np.mean(d[ok] ** 2)) / frms, d[ok].max() / frms
 
 
 def test_dpp_rotation_lane_layout():
@@ -78,9 +75,7 @@ def test_argon_forces_energy_virial(oracle):
     sc = e.scalars()
     e.close()
     F, E, W, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice, band=1e-6)
-    rms, mx = _force_errors(f, F, band)
-    say(f"argon rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 1e-4
+    check_forces(f, F, band, "argon 10000", say)
     assert sc["pe"] == pytest.approx(E.sum(), rel=1e-5)
     assert sc["virial"] == pytest.approx(W.sum(), rel=1e-5)
     assert pe.sum() == pytest.approx(E.sum(), rel=1e-5)
@@ -93,9 +88,7 @@ def _argon_parity(s, tag, oracle):
     sc = e.scalars()
     e.close()
     F, E, W, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice, band=1e-6)
-    rms, mx = _force_errors(f, F, band)
-    say(f"{tag} rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 1e-4
+    check_forces(f, F, band, tag, say)
     assert sc["pe"] == pytest.approx(E.sum(), rel=1e-5)
     assert sc["virial"] == pytest.approx(W.sum(), rel=1e-5)
 
@@ -157,9 +150,7 @@ def test_water_forces_energy_virial(oracle):
                                           excl=s.exclusion_csr(), band=1e-6)
     Fr, Er, Wr, er, wr, V = oracle.spme_recip(s.posq, s.lattice, s.spme.grid, 4, s.spme.alpha,
                                               want_v=True)
-    rms, mx = _force_errors(f, Fp + Fr, band)
-    say(f"water rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 5e-5
+    check_forces(f, Fp + Fr, band, "water 8192", say)
     assert sc["pe"] == pytest.approx((Ep + Er).sum(), rel=2e-6)
     assert sc["virial"] == pytest.approx((Wp + Wr).sum(), rel=2e-6)
     assert sc["e_recip"] == pytest.approx(er, rel=2e-6)
@@ -476,8 +467,14 @@ def _vv_constrained_step_oracle(oracle, s, G, posq0, img0, v0, f0, f1, dt, w=Non
     L32 = s.lattice.astype(np.float32)
     snap = p.copy()
     oracle.vv_kick_drift_f32(p, img, v, f0, s.mass, L32, dt)
-    wc, it_s = oracle.shake_positions(G, p, v, snap, s.mass, L32, dt)
-    it_r = oracle.rattle_velocities(G, p, v, s.mass, L32, dt, w_cons=wc)
+    # the device solves every group in the wave form (group frame,
+    # colour-ordered sweeps)
+    oracle.set_shake_form(True)
+    try:
+        wc, it_s = oracle.shake_positions(G, p, v, snap, s.mass, L32, dt)
+        it_r = oracle.rattle_velocities(G, p, v, s.mass, L32, dt, w_cons=wc)
+    finally:
+        oracle.set_shake_form(False)
     oracle.vv_kick_f32(v, f1, s.mass, dt)
     return p, img, v, it_s, it_r
 
@@ -512,6 +509,13 @@ def test_shake_water_step_bit_exact(oracle):
         f"{st['rattle_max']} (oracle {it_r.max()})")
     assert st["shake_sweeps"] == int(it_s.sum())
     assert st["rattle_sweeps"] == int(it_r.sum())
+    # groups that stop at the sweep cap: the reference's 3.57e-6 Bohr^2 is
+    # at the f32 resolution of |r|^2 here, so some do, on both sides alike
+    max_iter = oracle.SHAKE_MAX_ITER
+    say(f"  capped: shake {st['shake_capped']} (oracle {int((it_s >= max_iter).sum())}), "
+        f"rattle {st['rattle_capped']} (oracle {int((it_r >= max_iter).sum())}) of {G.n}")
+    assert st["shake_capped"] == int((it_s >= max_iter).sum())
+    assert st["rattle_capped"] == int((it_r >= max_iter).sum())
     assert np.array_equal(img.T, img1)
     assert np.array_equal(p, posq1)
     assert np.array_equal(v, v1)
@@ -641,12 +645,10 @@ def test_split_precision_dft_matches_fp64(oracle):
                                               want_v=True)
     for mode, name in ((0, "fp32 fft"), (1, "fp16 split dft")):
         f, sc, v_g = out[mode]
-        rms, mx = _force_errors(f, Fp + Fr, band)
-        frr = np.abs(Fr).max()
         v_err = float(np.abs(v_g - V).max() / np.abs(V).max())
-        say(f"  {name}: force rms {rms:.3e} max {mx:.3e} (of F_rms), V err {v_err:.3e}, "
-            f"E_recip rel {abs(sc['e_recip'] / er - 1):.3e}, W_recip rel {abs(sc['w_recip'] / wr - 1):.3e}")
-        assert rms <= 5e-6 and mx <= 5e-5
+        say(f"  {name}: V err {v_err:.3e}, E_recip rel {abs(sc['e_recip'] / er - 1):.3e}, "
+            f"W_recip rel {abs(sc['w_recip'] / wr - 1):.3e}")
+        check_forces(f, Fp + Fr, band, f"  {name}", say)
         assert v_err <= 2e-5
         assert sc["e_recip"] == pytest.approx(er, rel=2e-6)
         assert sc["w_recip"] == pytest.approx(wr, rel=2e-6)
@@ -718,9 +720,7 @@ def test_charged_chains_real_space_key_map(oracle):
                                           excl=s.exclusion_csr(), band=1e-6)
     Fr, Er, Wr, er, wr = oracle.spme_recip(s.posq, s.lattice, spme.grid, 4, spme.alpha)
     Fa, Ea, Wa = oracle.angle_forces(s.posq, s.lattice, *s.angles)
-    rms, mx = _force_errors(f, Fp + Fr + Fa, band)
-    say(f"  forces rms {rms:.3e} max {mx:.3e}")
-    assert rms <= 5e-6 and mx <= 1e-4
+    check_forces(f, Fp + Fr + Fa, band, "  charged polymer", say)
     assert sc["pe"] == pytest.approx((Ep + Er + Ea).sum(), rel=1e-5)
     assert sc["virial"] == pytest.approx((Wp + Wr + Wa).sum(), rel=1e-5,
                                          abs=1e-6 * np.abs(Wp).sum())
@@ -746,9 +746,8 @@ def test_polymer_melt_forces_and_shake_dynamics(oracle):
     Fp, Ep, Wp, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice,
                                           excl=s.exclusion_csr(), band=1e-6)
     Fa, Ea, Wa = oracle.angle_forces(s.posq, s.lattice, *s.angles)
-    rms, mx = _force_errors(f, Fp + Fa, band)
-    say(f"  forces rms {rms:.3e} max {mx:.3e}; pe {sc['pe']:.6e} vs {(Ep + Ea).sum():.6e}")
-    assert rms <= 5e-6 and mx <= 1e-4
+    say(f"  pe {sc['pe']:.6e} vs {(Ep + Ea).sum():.6e}")
+    check_forces(f, Fp + Fa, band, "  polymer melt", say)
     assert sc["pe"] == pytest.approx((Ep + Ea).sum(), rel=1e-5)
     assert sc["virial"] == pytest.approx((Wp + Wa).sum(), rel=1e-5, abs=1e-6 * np.abs(Wp).sum())
     e.step(50, scalars_last=True)

@@ -241,9 +241,20 @@ def _perturbed_water(seed, n_mol=8, dt=82.7):
     return s, snap, posq, vel
 
 
-def test_shake_water_matches_settle_and_restores_distances(oracle):
+@pytest.fixture(params=[False, True], ids=["reference-form", "wave-form"])
+def shake_form(request, oracle):
+    """Both group-solve forms of the oracle: the reference's sequential
+    sweep in atom 0's image, and the device's wave form (group frame,
+    colour-ordered sweeps) that every device group runs."""
+    oracle.set_shake_form(request.param)
+    yield request.param
+    oracle.set_shake_form(False)
+
+
+def test_shake_water_matches_settle_and_restores_distances(oracle, shake_form):
     # SHAKE with snapshot-direction gradients converges to the constrained
-    # positions SETTLE computes analytically (the same projection)
+    # positions SETTLE computes analytically (the same projection), in
+    # either sweep form
     s, snap, posq, vel = _perturbed_water(2)
     dt = 82.7
     d_oh, d_hh = s.water_geom
@@ -274,7 +285,7 @@ def test_shake_water_matches_settle_and_restores_distances(oracle):
         assert np.allclose(c0, c1, atol=2e-5)
 
 
-def test_rattle_removes_bond_velocities_and_keeps_momentum(oracle):
+def test_rattle_removes_bond_velocities_and_keeps_momentum(oracle, shake_form):
     s, snap, posq, vel = _perturbed_water(3)
     d_oh, d_hh = s.water_geom
     G = oracle.ShakeGroups([(g, [(0, 1, d_oh), (0, 2, d_oh), (1, 2, d_hh)])
@@ -293,6 +304,32 @@ def test_rattle_removes_bond_velocities_and_keeps_momentum(oracle):
             dv = v[:, g[a]].astype(np.float64) - v[:, g[b]]
             assert abs(r @ dv) < 2e-8
     assert np.allclose((s.mass[None, :].astype(np.float64) * v).sum(1), p_before, atol=1e-7)
+
+
+def test_shake_forms_converge_to_the_same_positions(oracle):
+    # the reference's sweep order and the device's colour order are two
+    # Gauss-Seidel orders of one system of equations: converged to a tight
+    # tolerance they land on the same constrained positions
+    s, snap, posq, vel = _perturbed_water(5)
+    d_oh, d_hh = s.water_geom
+    G = oracle.ShakeGroups([(g, [(0, 1, d_oh), (0, 2, d_oh), (1, 2, d_hh)])
+                            for g in s.water_groups])
+    L32 = s.lattice.astype(np.float32)
+    out = []
+    for form in (False, True):
+        oracle.set_shake_form(form)
+        try:
+            p, v = posq.copy(), vel.copy()
+            w, iters = oracle.shake_positions(G, p, v, snap, s.mass, L32, 82.7, tol=1e-5,
+                                              max_iter=1000)
+            assert iters.max() < 1000
+            out.append((p, v, w))
+        finally:
+            oracle.set_shake_form(False)
+    (p0, v0, w0), (p1, v1, w1) = out
+    assert np.abs(p0[:, :3] - p1[:, :3]).max() < 1e-5
+    assert np.abs(v0 - v1).max() < 1e-5 / 82.7 * 2
+    assert np.allclose(w0.sum(), w1.sum(), rtol=1e-3)
 
 
 def test_shake_long_chain_converges(oracle):
