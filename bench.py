@@ -110,9 +110,10 @@ def cpu_baseline(system, args):
     from oracle import oracle as O
 
     t0 = time.perf_counter()
-    F, E, W, _ = O.pair_forces(system.posq, system.types, system.lj.as_dict(), system.lattice,
-                               coul=dict(alpha=system.spme.alpha, r_cut=system.spme.r_cut),
-                               excl=system.exclusion_csr())
+    F, E, W, band = O.pair_forces(system.posq, system.types, system.lj.as_dict(),
+                                  system.lattice,
+                                  coul=dict(alpha=system.spme.alpha, r_cut=system.spme.r_cut),
+                                  excl=system.exclusion_csr(), band=1e-6)
     Fr, Er, Wr, er, wr = O.spme_recip(system.posq, system.lattice, system.spme.grid,
                                       system.spme.order, system.spme.alpha)
     f = (F + Fr).T.astype(np.float32).copy()
@@ -124,7 +125,7 @@ def cpu_baseline(system, args):
     O.vv_kick_f32(v, f, system.mass, dt)
     t = time.perf_counter() - t0
     ns_day = args.dt_fs * 1e-6 * 86400.0 / t
-    ref = {"F": F + Fr, "Fr": Fr, "e_recip": er, "w_recip": wr}
+    ref = {"F": F + Fr, "Fr": Fr, "e_recip": er, "w_recip": wr, "band": band}
     return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
             "sample": f"1 MD step (pair forces + SPME 128^3 + VV update) of the same "
                       f"{system.n}-atom state on 1 host thread of {os.cpu_count()} "
@@ -148,9 +149,15 @@ def precision_check(system, ref):
         f = e.forces().astype(np.float64)
         sc = e.scalars()
         e.close()
-        d = np.linalg.norm(f - F, axis=1)
+        # atoms of a pair whose r^2 lies within 1e-6 of a cutoff^2 are left
+        # out: which side of the cutoff such a pair falls depends on rounding
+        ok = ~ref["band"]
+        d = np.linalg.norm(f - F, axis=1)[ok]
+        fi = np.linalg.norm(F, axis=1)[ok]
         out[name] = {"force_err_rms_rel": float(np.sqrt(np.mean(d ** 2)) / frms),
                      "force_err_max_rel": float(d.max() / frms),
+                     # per atom |dF_i| / max(|F_i|, F_rms): the FP32-mode bar (<= 1e-5)
+                     "force_err_max_rel_per_atom": float((d / np.maximum(fi, frms)).max()),
                      "e_recip_rel": float(abs(sc["e_recip"] / ref["e_recip"] - 1.0)),
                      "w_recip_rel": float(abs(sc["w_recip"] / ref["w_recip"] - 1.0))}
     return out
@@ -387,13 +394,11 @@ def main():
     e, kt = build_engine(system, U, args)
     log(f"setup {time.time() - t0:.1f} s; neighbour list {e.neighbor_stats()}")
 
-    # the equilibration phases re-parameterise the thermostat often; they run
-    # as plain stream launches and only production captures step graphs
+    # the equilibration re-parameterises the thermostat and redraws the
+    # velocities often; every change drops and re-captures the step graphs
     t0 = time.time()
-    e.use_graph(False)
     equilibrate(e, system, U, kt, args)
     log(f"equilibration {time.time() - t0:.1f} s")
-    e.use_graph(True)
     e.step(args.warmup, scalars_last=True)
     e.sync()
     reb0 = e.scalars()["rebuilds"]
@@ -446,7 +451,7 @@ def main():
         result["roofline"] = {
             "kernel": "k_pair<2, 2, false> (fused LJ + real-space Ewald forces and virial, "
                       "packed wave64 cluster list; FP32 vector ALU, no matrix-core form)",
-            "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "bound": "fp32-valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic_from_profile(PAIR_KERNEL),
             "traffic_source": "profiles/water196k_step_traffic.md (rocprofv3 --pmc, calibrated)",

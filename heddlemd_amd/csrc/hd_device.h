@@ -54,6 +54,20 @@ __device__ __forceinline__ void hd_min_image(float& dx, float& dy, float& dz, co
   dz -= kc * L.lz;
 }
 
+// The same minimum image in FP64 (the lattice's f32 values widened): exact
+// differences of f32 coordinates, for the constraint solvers.
+__device__ __forceinline__ void hd_min_image_f64(double& dx, double& dy, double& dz,
+                                                 const HdLattice& L) {
+  double lx = L.lx, ly = L.ly, lz = L.lz, xy = L.xy, xz = L.xz, yz = L.yz;
+  double sc = dz / lz;
+  double sb = (dy - sc * yz) / ly;
+  double sa = (dx - sb * xy - sc * xz) / lx;
+  double ka = floor(sa + 0.5), kb = floor(sb + 0.5), kc = floor(sc + 0.5);
+  dx -= ka * lx + kb * xy + kc * xz;
+  dy -= kb * ly + kc * yz;
+  dz -= kc * lz;
+}
+
 // Minimum image with the reciprocal box lengths precomputed (inv = 1/lx,
 // 1/ly, 1/lz).  It differs from hd_min_image only by the rounding of the
 // fractional coordinate, i.e. only for separations within an ulp of half a

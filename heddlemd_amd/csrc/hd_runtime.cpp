@@ -1983,8 +1983,15 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
   HIPCHK(c, hipSetDevice(c->device));
   bool timed = c->timing && !c->use_graph;
   if (!c->forces_valid) {
+    // first forces of new positions: their rebuild's status is read before
+    // any step runs on the list
     enqueue_forces(c, false, false, false);
     c->forces_valid = true;
+    int r = check_status(c);
+    if (r) {
+      c->forces_valid = false;
+      return r;
+    }
   }
   // constraint iteration counters cover this call's steps
   if (c->d_sh_stats)

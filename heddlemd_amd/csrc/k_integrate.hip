@@ -148,11 +148,12 @@ __device__ __forceinline__ double dot3(const double (&a)[3], const double (&b)[3
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// f32 minimum-image difference a - b, widened
+// minimum-image difference a - b of two f32 positions, exact in FP64
 __device__ __forceinline__ void rel_f64(float4 a, float4 b, const HdLattice& L, double (&d)[3]) {
-  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-  hd_min_image(dx, dy, dz, L);
-  d[0] = dx; d[1] = dy; d[2] = dz;
+  d[0] = (double)a.x - (double)b.x;
+  d[1] = (double)a.y - (double)b.y;
+  d[2] = (double)a.z - (double)b.z;
+  hd_min_image_f64(d[0], d[1], d[2], L);
 }
 
 struct SettleOut {
@@ -206,7 +207,7 @@ __device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& 
       res[c] = dot3(bond[c], bond[c]) - target[c];
       worst = fmax(worst, fabs(res[c]) / target[c]);
     }
-    if (worst <= 1e-15) break;
+    if (worst <= 1e-14) break;
     // Jacobian of the residuals in lambda: 2 K_cc' (r_c . g_c')
     double J[3][3], rhs[3], dl[3];
 #pragma unroll
@@ -305,7 +306,7 @@ __device__ __forceinline__ void settle_load_ids(const HdSettleArgs& a, unsigned 
   ids[2] = a.groups[3 * g + 2];
 }
 
-__global__ void k_settle_positions(HdSettleArgs a) {
+__global__ void __launch_bounds__(128) k_settle_positions(HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   HdLattice L = hd_load_lattice(a.lattice);
@@ -329,7 +330,7 @@ __global__ void k_settle_positions(HdSettleArgs a) {
 }
 
 template <bool ACC>
-__global__ void k_settle_velocities(HdSettleArgs a) {
+__global__ void __launch_bounds__(128) k_settle_velocities(HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   HdLattice L = hd_load_lattice(a.lattice);
@@ -358,7 +359,7 @@ __global__ void k_settle_velocities(HdSettleArgs a) {
 // SETTLE stages on the f32 values held in registers, the same values the
 // unfused kernels pass through memory, so the results are bit-identical
 // and the intermediate state and the snapshot skip the round trips.
-__global__ void k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
+__global__ void __launch_bounds__(128) k_kick_drift_settle(HdIntegrateArgs ia, HdSettleArgs a) {
   unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.n_groups) return;
   HdLattice L = hd_load_lattice(a.lattice);

@@ -144,6 +144,19 @@ __device__ __forceinline__ void pair_rel_to_center(float x, float y, float z, fl
   rz = (float)(((double)z - (double)cb.z) - (double)L.lz * nz);
 }
 
+// One axis of a minimum-image pair displacement in an orthorhombic box.
+// Within the box the f32 difference of the two coordinates is exact or
+// rounds at the pair's own scale; across a face it would round at the
+// box's scale (an error of up to half an ulp of L, ~1e-6 of a contact
+// distance, which the r^-13 of the LJ repulsion multiplies), so such pairs
+// take the difference in FP64 and round once.
+__device__ __forceinline__ float pair_axis_disp(float a, float b, float len, float ilen) {
+  float d = a - b;
+  float k = rintf(d * ilen);
+  if (k != 0.f) d = (float)(((double)a - (double)b) - (double)len * (double)k);
+  return d;
+}
+
 // Far-away coordinates for padding lanes on the shifted path: any pair
 // with one of them fails both cutoffs, and |d|^2 stays finite (< 1e37).
 #define PAIR_FAR_I 1.0e18f
@@ -308,7 +321,15 @@ __device__ __forceinline__ void pair_entries(
 #pragma unroll 4
     for (int t = 0; t < 64; ++t) {
       float dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
-      if (!SPC) hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
+      if (!SPC) {
+        if (P.ortho) {
+          dx = pair_axis_disp(pix, pjx, L.lx, ilx);
+          dy = pair_axis_disp(piy, pjy, L.ly, ily);
+          dz = pair_axis_disp(piz, pjz, L.lz, ilz);
+        } else {
+          hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
+        }
+      }
       float r2 = dx * dx + dy * dy + dz * dz;
       float inv_r = __builtin_amdgcn_rsqf(r2);
       float inv_r2 = inv_r * inv_r;
@@ -509,9 +530,9 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
   const float two_a_sqrtpi = 2.0f * P.alpha * 0.5641895835477563f;
   float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
   if (ORTHO) {
-    dx -= L.lx * rintf(dx * ilx);
-    dy -= L.ly * rintf(dy * ily);
-    dz -= L.lz * rintf(dz * ilz);
+    dx = pair_axis_disp(pi.x, pj.x, L.lx, ilx);
+    dy = pair_axis_disp(pi.y, pj.y, L.ly, ily);
+    dz = pair_axis_disp(pi.z, pj.z, L.lz, ilz);
   } else {
     hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
   }
@@ -662,36 +683,46 @@ __global__ void k_angles(const float4* __restrict__ posq, const float* __restric
   HdLattice L = hd_load_lattice(lattice);
   unsigned ai = ijk[3 * m], aj = ijk[3 * m + 1], ak = ijk[3 * m + 2];
   float4 pi = posq[ai], pj = posq[aj], pk = posq[ak];
-  float dxi = pi.x - pj.x, dyi = pi.y - pj.y, dzi = pi.z - pj.z;
-  float dxk = pk.x - pj.x, dyk = pk.y - pj.y, dzk = pk.z - pj.z;
-  hd_min_image(dxi, dyi, dzi, L);
-  hd_min_image(dxk, dyk, dzk, L);
-  float dij2 = dxi * dxi + dyi * dyi + dzi * dzi;
-  float dkj2 = dxk * dxk + dyk * dyk + dzk * dzk;
-  float fix = 0.f, fiy = 0.f, fiz = 0.f, fkx = 0.f, fky = 0.f, fkz = 0.f, u = 0.f, w = 0.f;
-  if (dij2 != 0.f && dkj2 != 0.f) {
-    float dij = sqrtf(dij2), dkj = sqrtf(dkj2);
-    float inv_dd = 1.0f / (dij * dkj);
-    float dot = dxi * dxk + dyi * dyk + dzi * dzk;
-    float c = fminf(fmaxf(dot * inv_dd, -1.0f), 1.0f);
-    float s2 = 1.0f - c * c;
-    float sn = sqrtf(s2 > 0.f ? s2 : 0.f);
-    if (sn >= 1.0e-7f) {
-      float theta = atan2f(dij * dkj * sn, dot);
-      float k = kth[m], dth = theta - th0[m];
-      float g = -k * dth / sn;
-      float ii = 1.0f / dij2, kk = 1.0f / dkj2;
-      fix = g * (c * ii * dxi - inv_dd * dxk);
-      fiy = g * (c * ii * dyi - inv_dd * dyk);
-      fiz = g * (c * ii * dzi - inv_dd * dzk);
-      fkx = g * (c * kk * dxk - inv_dd * dxi);
-      fky = g * (c * kk * dyk - inv_dd * dyi);
-      fkz = g * (c * kk * dzk - inv_dd * dzi);
-      u = 0.5f * k * dth * dth;
-      w = (dxi * fix + dyi * fiy + dzi * fiz) + (dxk * fkx + dyk * fky + dzk * fkz);
+  // FP64 throughout: the force is proportional to theta - theta0, a small
+  // difference of two angles near 2 rad that f32 would resolve only to
+  // ~1e-7 rad, i.e. to a relative 1e-4 of the force for a 1e-3 rad strain
+  double ui[3] = {(double)pi.x - (double)pj.x, (double)pi.y - (double)pj.y,
+                  (double)pi.z - (double)pj.z};
+  double uk[3] = {(double)pk.x - (double)pj.x, (double)pk.y - (double)pj.y,
+                  (double)pk.z - (double)pj.z};
+  hd_min_image_f64(ui[0], ui[1], ui[2], L);
+  hd_min_image_f64(uk[0], uk[1], uk[2], L);
+  double li2 = ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2];
+  double lk2 = uk[0] * uk[0] + uk[1] * uk[1] + uk[2] * uk[2];
+  double gi[3] = {0.0, 0.0, 0.0}, gk[3] = {0.0, 0.0, 0.0}, u = 0.0, w = 0.0;
+  if (li2 > 0.0 && lk2 > 0.0) {
+    double li = sqrt(li2), lk = sqrt(lk2);
+    double cr[3] = {ui[1] * uk[2] - ui[2] * uk[1], ui[2] * uk[0] - ui[0] * uk[2],
+                    ui[0] * uk[1] - ui[1] * uk[0]};
+    double sn_l = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);  // |ui||uk| sin
+    double dt_l = ui[0] * uk[0] + ui[1] * uk[1] + ui[2] * uk[2];        // |ui||uk| cos
+    if (sn_l > 1e-12 * li * lk) {
+      double theta = atan2(sn_l, dt_l);
+      double dth = theta - (double)th0[m];
+      double kf = (double)kth[m];
+      // dtheta/dx_i = (cos ui/|ui|^2 - uk/(|ui||uk|)) / sin, and for k
+      // alike; F = -k dth dtheta/dx
+      double cs = dt_l / (li * lk), sn = sn_l / (li * lk);
+      double g = -kf * dth / sn, inv_ik = 1.0 / (li * lk);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        gi[d] = g * (cs * ui[d] / li2 - uk[d] * inv_ik);
+        gk[d] = g * (cs * uk[d] / lk2 - ui[d] * inv_ik);
+      }
+      u = 0.5 * kf * dth * dth;
+      w = ui[0] * gi[0] + ui[1] * gi[1] + ui[2] * gi[2] + uk[0] * gk[0] + uk[1] * gk[1] +
+          uk[2] * gk[2];
     }
   }
-  float fjx = -(fix + fkx), fjy = -(fiy + fky), fjz = -(fiz + fkz);
+  float fix = (float)gi[0], fiy = (float)gi[1], fiz = (float)gi[2];
+  float fkx = (float)gk[0], fky = (float)gk[1], fkz = (float)gk[2];
+  float fjx = (float)-(gi[0] + gk[0]), fjy = (float)-(gi[1] + gk[1]),
+        fjz = (float)-(gi[2] + gk[2]);
   unsigned si = slot_of[ai], sj = slot_of[aj], sk = slot_of[ak];
   const float f3[3][3] = {{fix, fiy, fiz}, {fjx, fjy, fjz}, {fkx, fky, fkz}};
   const unsigned sl[3] = {si, sj, sk};
@@ -703,7 +734,7 @@ __global__ void k_angles(const float4* __restrict__ posq, const float* __restric
       if (v) atomicAdd(fpj + (size_t)sl[a] * 4 + d, (unsigned long long)v);
     }
   if (ev) {
-    long long e3 = hd_to_fixed48(u * (1.0f / 3.0f)), w3 = hd_to_fixed48(w * (1.0f / 3.0f));
+    long long e3 = hd_to_fixed48((float)(u / 3.0)), w3 = hd_to_fixed48((float)(w / 3.0));
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (e3) atomicAdd(fpj_ew + (size_t)sl[a] * 2, (unsigned long long)e3);

@@ -37,9 +37,11 @@ def _settle_reference_step(oracle, s, posq0, img0, v0, f0, f1, dt, groups):
     geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
     pos = p[:, :3].astype(np.float64)
     vel = v.T.astype(np.float64).copy()
+    # the device holds the box in f32: the oracle takes the same lengths
+    L32 = s.lattice.astype(np.float32).astype(np.float64)
     oracle.settle_positions(groups, pos, vel, posq0[:, :3].astype(np.float64), geom, m_o, m_h,
-                            s.lattice, float(dt))
-    oracle.settle_velocities(groups, pos, vel, m_o, m_h, s.lattice, float(dt))
+                            L32, float(dt))
+    oracle.settle_velocities(groups, pos, vel, m_o, m_h, L32, float(dt))
     vel += 0.5 * float(dt) * f1.T.astype(np.float64) / s.mass[:, None].astype(np.float64)
     return p, img, pos, vel
 
@@ -157,27 +159,59 @@ def test_displacement_check_matches_the_rule(oracle):
     assert 0 < trips < 12
 
 
-def test_status_polled_between_graph_launches():
-    # an inert gas squeezed into 0.8 of the box after set-up overflows its
-    # neighbour capacity at the first rebuild; a long graph-replayed call
-    # stops within a few launches instead of running every step on the
-    # truncated list (the Langevin draw counter counts the steps taken)
-    say("case: argon 4000 inert, squeezed: overflow seen between graph launches")
+def _inert_argon():
     s = systems.argon_lattice(n_atoms=4000, n_side=16)
-    s.lj.epsilon[:] = 0.0
-    kt = U.temperature(100.0)
-    e = engine_for(s, dt=U.time(1e-15), langevin=dict(friction=1e-3, kt=kt, seed=3))
+    s.lj.epsilon[:] = 0.0  # no forces: every step is a straight flight
+    return s
+
+
+def test_list_grows_between_graph_launches():
+    # an inert gas imploding at constant velocity towards the box centre
+    # outgrows its neighbour capacity part-way through one 400-step
+    # graph-replayed call; the status word read between graph launches
+    # sees the high-water mark and the list grows before it overflows, so
+    # every step runs on a complete list and the call succeeds
+    say("case: argon 4000 inert, imploding: high water seen between graph launches")
+    s = _inert_argon()
+    dt = U.time(1e-15)
+    x0 = s.posq[:, :3].astype(np.float64)
+    v = (-0.45 * x0 / (400 * dt)).T.astype(np.float32)  # the cloud is centred at 0
+    e = engine_for(s, dt=dt, velocities=v)
     e.force_eval(scalars=False)
-    c = s.lattice[:3] * 0.5
+    cap0 = e.neighbor_stats()["capacity"]
+    e.step(400)
+    st = e.neighbor_stats()
+    p = e.positions()[:, :3].astype(np.float64)
+    e.close()
+    say(f"  capacity {cap0} -> {st['capacity']}, rebuilds {st['rebuilds']}, status {st['status']}")
+    assert st["capacity"] > cap0
+    assert st["status"] & 8 == 0
+    # all 400 steps ran: each atom flew 400 v dt
+    assert np.abs(p - (x0 + 400 * dt * v.T.astype(np.float64))).max() < 1e-3
+
+
+def test_overflow_stops_a_long_call_within_a_few_graph_launches():
+    # the same gas squeezed into half the box after set-up overflows at the
+    # first rebuild of the next call; the call stops a few launches later
+    # instead of running its 400 steps on a truncated list (a uniform drift
+    # velocity measures the steps taken)
+    say("case: argon 4000 inert, squeezed: overflow seen between graph launches")
+    s = _inert_argon()
+    dt = U.time(1e-15)
+    vd = 1e-4
+    v = np.full((3, s.n), vd, np.float32)
+    e = engine_for(s, dt=dt, velocities=v)
+    e.force_eval(scalars=False)
     p = s.posq.copy()
-    p[:, :3] = (c + 0.8 * (p[:, :3].astype(np.float64) - c)).astype(np.float32)
+    p[:, :3] = (0.5 * p[:, :3].astype(np.float64)).astype(np.float32)
     e.set_positions(p)
     with pytest.raises(HeddleError, match="overflow"):
         e.step(400)
-    steps = e.langevin_counter()
+    q = e.positions()[:, :3].astype(np.float64)
     e.close()
-    say(f"  stopped after {steps} of 400 steps")
-    assert steps <= 3 * 8
+    taken = np.median((q - p[:, :3]).ravel()) / (vd * dt)
+    say(f"  stopped after {taken:.1f} of 400 steps")
+    assert round(taken) <= 3 * 8
 
 
 def test_graph_recapture_after_reparameterisation_matches_eager():

@@ -252,9 +252,10 @@ def test_settle_water_step_matches_oracle(oracle):
     geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
     pos = p[:, :3].astype(np.float64)
     vel = v.T.astype(np.float64).copy()
+    L64 = s.lattice.astype(np.float32).astype(np.float64)  # the device's f32 box
     oracle.settle_positions(s.water_groups, pos, vel, posq0[:, :3].astype(np.float64), geom,
-                            m_o, m_h, s.lattice, float(dt))
-    oracle.settle_velocities(s.water_groups, pos, vel, m_o, m_h, s.lattice, float(dt))
+                            m_o, m_h, L64, float(dt))
+    oracle.settle_velocities(s.water_groups, pos, vel, m_o, m_h, L64, float(dt))
     vel += 0.5 * float(dt) * f1.T.astype(np.float64) / s.mass[:, None].astype(np.float64)
     dp = np.abs(posq1[:, :3] - pos).max()
     dv = np.abs(v1.T - vel).max()
