@@ -102,34 +102,64 @@ def equilibrate(e, system, U, kt, args):
                        compressibility=U.inv_pressure(4.5e-10), kt=kt, seed=13)
 
 
-def cpu_baseline(system, args):
-    """Oracle restatement timed on the host: one full force evaluation
-    (pair + reciprocal) plus the velocity-Verlet/SETTLE update of the same
-    196,608-atom state, single thread, scaled to ns/day."""
-    import numpy as np
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(system, v, args):
+    """The reference algorithm on the host: the CPU restatement of the same
+    MD step (oracle/cpu_step.c: cell-list half list, LJ + Hastings-erfc pair
+    terms, SPME order 4 on the same grid, SETTLE, NHC, C-rescale; f32 with
+    FP64 reductions, OpenMP over all the threads this process may use),
+    started from the GPU run's final state and timed over --cpu-steps steps
+    after 2 warm-up steps."""
+    from heddlemd_amd import units as U
+    from oracle.cpu_step import CpuStep
+
+    kt = U.temperature(args.temperature)
+    c = CpuStep(system, dt=U.time(args.dt_fs * 1e-15), r_skin=U.length(args.skin_nm * 1e-9),
+                kt=kt, tau=U.time(args.tau_fs * 1e-15),
+                pressure=0.0 if args.no_barostat else U.pressure(1.0e5),
+                tau_p=0.0 if args.no_barostat else U.time(1.0e-12),
+                kappa=U.inv_pressure(4.5e-10))
+    c.set_velocities(v)
+    c.step(2)
+    t0 = time.perf_counter()
+    c.step(args.cpu_steps)
+    t = time.perf_counter() - t0
+    _, _, st = c.state()
+    c.close()
+    threads = int(st["threads"])
+    ns_day = args.dt_fs * 1e-6 * (args.cpu_steps / t) * 86400.0
+    return {"value": round(ns_day, 4), "unit": "ns/day", "cores": threads, "kind": "port",
+            "ms_per_step": round(t / args.cpu_steps * 1e3, 2),
+            "atom_steps_per_s": round(system.n * args.cpu_steps / t, 1),
+            "sample": f"{args.cpu_steps} MD steps (after 2 warm-up steps) of the same "
+                      f"{system.n}-atom system from the GPU run's final state, "
+                      f"oracle/cpu_step.c (f32, FP64 reductions, OpenMP) on {threads} threads "
+                      f"of {os.cpu_count()} logical CPUs ({_cpu_model()}): {t:.2f} s, "
+                      f"{int(st['rebuilds'])} list builds"}
+
+
+def fp64_reference(system):
+    """FP64 oracle forces, reciprocal energy and virial of the benchmark's
+    initial state (for the precision block; not timed)."""
     from oracle import oracle as O
 
-    t0 = time.perf_counter()
     F, E, W, band = O.pair_forces(system.posq, system.types, system.lj.as_dict(),
                                   system.lattice,
                                   coul=dict(alpha=system.spme.alpha, r_cut=system.spme.r_cut),
                                   excl=system.exclusion_csr(), band=1e-6)
     Fr, Er, Wr, er, wr = O.spme_recip(system.posq, system.lattice, system.spme.grid,
                                       system.spme.order, system.spme.alpha)
-    f = (F + Fr).T.astype(np.float32).copy()
-    posq = system.posq.copy()
-    img = np.zeros((system.n, 3), np.int32)
-    v = np.zeros((3, system.n), np.float32)
-    dt = np.float32(args.dt_fs * 1e-15 / 2.4188843265857195e-17)
-    O.vv_kick_drift_f32(posq, img, v, f, system.mass, system.lattice.astype(np.float32), dt)
-    O.vv_kick_f32(v, f, system.mass, dt)
-    t = time.perf_counter() - t0
-    ns_day = args.dt_fs * 1e-6 * 86400.0 / t
-    ref = {"F": F + Fr, "Fr": Fr, "e_recip": er, "w_recip": wr, "band": band}
-    return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
-            "sample": f"1 MD step (pair forces + SPME 128^3 + VV update) of the same "
-                      f"{system.n}-atom state on 1 host thread of {os.cpu_count()} "
-                      f"({platform.processor() or platform.machine()}): {t:.2f} s"}, ref
+    return {"F": F + Fr, "Fr": Fr, "e_recip": er, "w_recip": wr, "band": band}
 
 
 def precision_check(system, ref):
@@ -336,6 +366,8 @@ def main():
     ap.add_argument("--molecules", type=int, default=65536)
     ap.add_argument("--no-barostat", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="timed steps of the CPU baseline (after 2 warm-up steps)")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-replica pass")
     ap.add_argument("--replicas", type=int, default=256)
@@ -487,16 +519,21 @@ def main():
             "fp32_fft_spme_recip_ms": round(result.get("stages_ms", {}).get("spme_recip", float("nan")), 4),
         }
         log(f"split-precision DFT: {result['split_dft']}")
+    if not args.no_cpu_baseline:
+        import dataclasses
+        state = dataclasses.replace(system, posq=e.positions(), lattice=e.lattice())
+        v_final = e.velocities()
     e.close()
     if not args.no_batched:
         result["batched"] = batched_replicas(args)
     if not args.no_polymer:
         result["polymer_shake"] = polymer_shake(args)
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle restatement) ...")
-        result["cpu_baseline"], ref = cpu_baseline(system, args)
+        log("cpu baseline (the step restated for the host's cores) ...")
+        result["cpu_baseline"] = cpu_baseline(state, v_final, args)
+        log(f"cpu baseline: {result['cpu_baseline']}")
         log("precision check against the FP64 oracle ...")
-        result["precision"] = precision_check(system, ref)
+        result["precision"] = precision_check(system, fp64_reference(system))
     print(json.dumps(result), flush=True)
 
 
