@@ -79,6 +79,8 @@ SIGNATURES = {
     "hd_mlp_predict": (C.c_int, [C.c_void_p, _f32p, C.c_uint64, _f32p]),
     "hd_mlp_gradient": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_uint64, _f32p,
                                   C.POINTER(C.c_double)]),
+    "hd_mlp_gradient_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                          C.c_void_p, C.c_void_p]),
     "hd_mlp_train": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_uint64, C.c_int, C.c_int,
                                C.c_float, C.c_uint64, _vp]),
     "hd_mlp_train_log": (C.c_int, [C.c_void_p, _ctx, C.c_int, C.c_int, C.c_float, C.c_uint64,

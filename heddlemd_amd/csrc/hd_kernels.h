@@ -263,6 +263,7 @@ void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a);
 #define HD_MLP_NPARAM (HD_MLP_NTRAIN + 1)
 #define HD_MLP_PSTRIDE 4864  // floats per block partial: gradient, then the loss sum
 #define HD_MLP_SAMPLES 64    // samples per block of the gradient kernel
+#define HD_MLP_NOALLOC_ROWS 16384  // rows per hd_mlp_gradient_noalloc call
 // nu[c] = prediction for feature row c
 void hd_launch_mlp_predict(hipStream_t st, const float* params, const float* feat, unsigned n,
                            float* nu);

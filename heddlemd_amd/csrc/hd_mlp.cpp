@@ -16,6 +16,15 @@
 #include "../../include/heddle.h"
 #include "hd_kernels.h"
 
+// Scratch of hd_mlp_gradient_noalloc: the per-workgroup gradient partials
+// of at most HD_MLP_NOALLOC_ROWS rows, one copy per GPU, held from the
+// library's first use of that GPU until the process ends.  Host code never
+// passes this symbol itself to a kernel (on the host it names a stand-in,
+// not device memory): each device's address is resolved once with
+// hipGetSymbolAddress and kept in scratch_dev below.
+#define HD_MLP_NOALLOC_BLOCKS (HD_MLP_NOALLOC_ROWS / HD_MLP_SAMPLES)
+static __device__ __align__(256) float scratch[HD_MLP_NOALLOC_BLOCKS * HD_MLP_PSTRIDE];
+
 struct hd_mlp {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -28,6 +37,7 @@ struct hd_mlp {
   float *x = nullptr, *t = nullptr, *y = nullptr;  // staging for host arrays
   uint64_t stage_rows = 0;
   long adam_step = 0;
+  float* scratch_dev = nullptr;  // this device's copy of `scratch`
 };
 
 namespace {
@@ -55,6 +65,23 @@ uint64_t splitmix64(uint64_t& s) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
+}
+
+// Device address of `scratch` on the current device, resolved once per
+// device (the first predictor created there) and cached.
+#define HD_MLP_MAX_DEVICES 64
+static float* g_scratch_dev[HD_MLP_MAX_DEVICES] = {nullptr};
+
+hipError_t scratch_address(int device, float** out) {
+  if (device < 0 || device >= HD_MLP_MAX_DEVICES) return hipErrorInvalidDevice;
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = (float*)p;
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
 }
 
 uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -182,6 +209,7 @@ int hd_mlp_create(int device, uint64_t seed, hd_mlp** out) {
   if (bad(hipMalloc((void**)&m->v, HD_MLP_NTRAIN * sizeof(float)))) return HD_ERR_HIP;
   if (bad(hipMalloc((void**)&m->grad, HD_MLP_NTRAIN * sizeof(float)))) return HD_ERR_HIP;
   if (bad(hipMalloc((void**)&m->loss, sizeof(double)))) return HD_ERR_HIP;
+  if (bad(scratch_address(device, &m->scratch_dev))) return HD_ERR_HIP;
   // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) hidden weights, zero biases,
   // zero output layer (the untrained predictor returns the first-order
   // multiplier); scale 0 = derive it from the first training data
@@ -271,6 +299,25 @@ int hd_mlp_gradient(hd_mlp* m, const float* x, const float* t, uint64_t n, float
   MCHK(m, hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
   if (loss) *loss = l / (double)n;
+  return HD_OK;
+}
+
+int hd_mlp_gradient_noalloc(hd_mlp* m, const float* x, const float* t, uint64_t n, float* grad,
+                            double* loss) {
+  if (!m || !x || !t || !grad || !loss || n == 0) return HD_ERR_ARG;
+  if (n > HD_MLP_NOALLOC_ROWS)
+    return mfail(m, HD_ERR_ARG, "hd_mlp_gradient_noalloc takes at most %d rows (got %llu)",
+                 HD_MLP_NOALLOC_ROWS, (unsigned long long)n);
+  if (!m->scratch_dev) return mfail(m, HD_ERR_STATE, "scratch address not resolved");
+  MCHK(m, hipSetDevice(m->device));
+  // the reduction adds the loss sum into *loss: cleared by a kernel (no
+  // memset node, so the call stays capturable)
+  hd_launch_fill_f32(m->stream, (float*)loss, 0.f, 2);
+  unsigned nb = hd_launch_mlp_grad(m->stream, m->params, x, t, n, 0, (unsigned)n, 1, 0,
+                                   1.0f / (float)n, m->scratch_dev);
+  hd_launch_mlp_reduce(m->stream, m->scratch_dev, nb, m->params, grad, nullptr, loss);
+  MCHK(m, hipGetLastError());
+  MCHK(m, hipStreamSynchronize(m->stream));
   return HD_OK;
 }
 

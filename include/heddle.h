@@ -251,6 +251,19 @@ int hd_mlp_predict(hd_mlp *mlp, const float *x, uint64_t n, float *nu);
  * floats, the scale excluded) and the loss itself; no update. */
 int hd_mlp_gradient(hd_mlp *mlp, const float *x, const float *t, uint64_t n, float *grad,
                     double *loss);
+/* hd_mlp_gradient on device memory, allocating nothing.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  x (8 n
+ * floats), t (n), grad (param_count - 1 floats) and loss (one double) are
+ * device pointers on the predictor's device; loss receives the loss SUM
+ * over the rows (hd_mlp_gradient reports the mean: divide by n); the
+ * gradient is the same as hd_mlp_gradient's, bit for bit.  n <= 16384
+ * (HD_ERR_ARG beyond).  The per-workgroup partials live in one static
+ * device buffer per GPU (about 5 MB, held until the process ends): calls
+ * that overlap on one GPU race on it.  The call synchronises the
+ * predictor's stream before it returns. */
+int hd_mlp_gradient_noalloc(hd_mlp *mlp, const float *x, const float *t, uint64_t n,
+                            float *grad, double *loss);
 /* Adam over `epochs` passes of mini-batches of `batch` rows (a multiple of
  * 64); loss[e] = mean loss during epoch e (may be NULL). */
 int hd_mlp_train(hd_mlp *mlp, const float *x, const float *t, uint64_t n, int epochs, int batch,
