@@ -239,6 +239,13 @@ void dfree(hd_ctx* c, void* p) {
 }
 
 bool is_pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+// n = 2^a 3^b 5^c: the FFT's radices
+bool is_smooth235(int v) {
+  if (v < 1) return false;
+  for (int p : {2, 3, 5})
+    while (v % p == 0) v /= p;
+  return v == 1;
+}
 
 // 3D Hilbert index of cell (a, b, c) on a 2^bits cube, by Skilling's
 // transpose algorithm ("Programming the Hilbert curve", AIP Conf. Proc. 707,
@@ -1034,9 +1041,13 @@ int hd_set_spme(hd_ctx* c, double alpha, double r_cut, int na, int nb, int nc, i
   if (!c) return HD_ERR_ARG;
   if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_spme after hd_setup");
   if (order != 4) return fail(c, HD_ERR_UNSUPPORTED, "spline order %d: only 4 is built", order);
-  if (!is_pow2(na) || !is_pow2(nb) || !is_pow2(nc) || na < 8 || nb < 8 || nc < 8 || na > 256 ||
-      nb > 256 || nc > 512)
-    return fail(c, HD_ERR_UNSUPPORTED, "SPME grid %dx%dx%d: powers of two in [8,256] required",
+  // the LDS FFT takes lines of 2^a 3^b 5^c points: up to 256 along a and
+  // b, and an even count up to 512 along c (packed as nc/2 complex points)
+  if (!is_smooth235(na) || !is_smooth235(nb) || !is_smooth235(nc) || nc % 2 || na < 8 ||
+      nb < 8 || nc < 8 || na > 256 || nb > 256 || nc > 512)
+    return fail(c, HD_ERR_UNSUPPORTED,
+                "SPME grid %dx%dx%d: each side must be 2^a 3^b 5^c, at least 8, at most 256 "
+                "(a, b) or 512 and even (c)",
                 na, nb, nc);
   if (!(alpha > 0) || !(r_cut > 0)) return fail(c, HD_ERR_ARG, "alpha and r_cut must be > 0");
   c->have_spme = true;
@@ -1116,8 +1127,10 @@ static void dft_split_host(const std::vector<double>& w, _Float16* hi, _Float16*
 static int build_dft_mats(hd_ctx* c) {
   if (c->dft_mats) return HD_OK;
   unsigned na = c->grid[0], nb = c->grid[1], nc = c->grid[2], nch = nc / 2 + 1;
-  if (2 * na > 256 || 2 * nb > 256 || nc > 256 || na < 32 || nb < 32 || nc < 32)
-    return fail(c, HD_ERR_UNSUPPORTED, "the matrix-core DFT handles axes of 32 to 128 points");
+  if (2 * na > 256 || 2 * nb > 256 || nc > 256 || na < 32 || nb < 32 || nc < 32 ||
+      !is_pow2((int)na) || !is_pow2((int)nb) || !is_pow2((int)nc))
+    return fail(c, HD_ERR_UNSUPPORTED,
+                "the matrix-core DFT handles power-of-two axes of 32 to 128 points");
   unsigned cf_m = (2 * nch + 31) / 32 * 32, cf_k = nc;
   unsigned ci_m = nc, ci_k = (2 * nch + 7) / 8 * 8;
   size_t sz_cf = (size_t)cf_m * cf_k, sz_ci = (size_t)ci_m * ci_k;

@@ -704,11 +704,14 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     for (int d = 0; d < GATHER_P; ++d) {
       bspline_wd((float)d + rec.y, wb[d], dwb[d]);
       bspline_wd((float)d + rec.z, wc[d], dwc[d]);
-      // periodic wrap; the grid sides are powers of two
-      gib[d] = (gb0 - d) & ((int)a.nb - 1);
-      gic[d] = (gc0 - d) & ((int)a.nc - 1);
+      // periodic wrap (the origin cell lies in [0, n))
+      gib[d] = gb0 - d;
+      gib[d] += gib[d] < 0 ? (int)a.nb : 0;
+      gic[d] = gc0 - d;
+      gic[d] += gic[d] < 0 ? (int)a.nc : 0;
     }
-    int gia = (ga0 - (int)q) & ((int)a.na - 1);
+    int gia = ga0 - (int)q;
+    gia += gia < 0 ? (int)a.na : 0;
 #pragma unroll
     for (int db = 0; db < GATHER_P; ++db) {
       const float* row = a.vgrid + ((size_t)gia * a.nb + gib[db]) * a.nc;

@@ -117,11 +117,12 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   if (g == SPME_NO_CELL) return;
   int ga = (int)(g & 1023u), gb = (int)((g >> 10) & 1023u), gc = (int)(g >> 20);
   // brick-relative index of the stencil's first cell per axis; cell
-  // (g - d) mod n lies in the brick when 0 <= rel - d < extent (the grid
-  // sides are powers of two, so mod n is a mask)
-  int rla = (ga - c0a) & ((int)na - 1);
-  int rlb = (gb - c0b) & ((int)nb - 1);
-  int rlc = (gc - c0c) & ((int)nc - 1);
+  // (g - d) mod n lies in the brick when 0 <= rel - d < extent (g and the
+  // brick origin both lie in [0, n), so one conditional add is mod n)
+  int rla = ga - c0a, rlb = gb - c0b, rlc = gc - c0c;
+  rla += rla < 0 ? (int)na : 0;
+  rlb += rlb < 0 ? (int)nb : 0;
+  rlc += rlc < 0 ? (int)nc : 0;
   if (rla >= (int)ba + SPME_P - 1 || rlb >= (int)bb + SPME_P - 1 ||
       rlc >= (int)bc + SPME_P - 1)
     return;
@@ -252,12 +253,19 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
   }
 }
 
+// the largest divisor of n that is at most cap
+static unsigned divisor_at_most(unsigned n, unsigned cap) {
+  for (unsigned d = cap < n ? cap : n; d > 1; --d)
+    if (n % d == 0) return d;
+  return 1;
+}
+
 unsigned hd_spread_bricks(unsigned na, unsigned nb, unsigned nc, unsigned brick[3]) {
-  // up to 8 x 16 x 16 cells; smaller grids keep at least 8 per axis or the
-  // whole axis
-  brick[0] = na < 8 ? na : 8;
-  brick[1] = nb < 16 ? nb : 16;
-  brick[2] = nc < 16 ? nc : 16;
+  // up to 8 x 16 x 16 cells, each side a divisor of the grid's (8 and 16
+  // for the power-of-two grids; e.g. 5 x 10 x 16 for 20 x 20 x 80)
+  brick[0] = divisor_at_most(na, 8);
+  brick[1] = divisor_at_most(nb, 16);
+  brick[2] = divisor_at_most(nc, 16);
   return (na / brick[0]) * (nb / brick[1]) * (nc / brick[2]);
 }
 
@@ -386,6 +394,132 @@ __device__ __forceinline__ void lds_fft_dif(float2* d, unsigned n, unsigned log2
   }
 }
 
+// ---------------------------------------------------------------------
+// Mixed-radix line FFT for lengths n = 2^a 3^b 5^c that are not powers of
+// two (the reference sizes its grids for cuFFT's small primes, e.g.
+// 48 x 48 x 96 and 20 x 20 x 80).  Self-sorting (Stockham) stages of radix
+// 4, then 2, 3, 5: natural order in and out, so no bit reversal.  A stage
+// of radix R with span ns (the product of the earlier radices) takes, for
+// every j < n/R, the R values x[j + r n/R], multiplies value r by
+// w^(r k) with w = exp(sign 2 pi i / (ns R)) and k = j mod ns, applies the
+// R-point DFT and stores output r at (j div ns) ns R + k + r ns.  Each
+// thread holds its butterflies' outputs in registers across a barrier, so
+// the stage runs in place in LDS.
+// ---------------------------------------------------------------------
+#define FFT_MR_PER_THREAD 8  // butterflies per thread: nlines * n / R <= 2048 at 256 threads
+
+// exp(sign * 2 pi i t / twn) for 0 <= t < twn from the half-circle table
+// stw[t] = exp(-2 pi i t / twn), t < twn / 2
+__device__ __forceinline__ float2 tw_full(const float2* stw, unsigned t, unsigned twn, int sign) {
+  unsigned half = twn >> 1;
+  float2 w = (t < half) ? stw[t] : make_float2(-stw[t - half].x, -stw[t - half].y);
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float f) { return make_float2(a.x * f, a.y * f); }
+// a * (i f)
+__device__ __forceinline__ float2 cmul_if(float2 a, float f) { return make_float2(-a.y * f, a.x * f); }
+
+// R-point DFT with exp(sign 2 pi i jk / R), in place
+template <int R>
+__device__ __forceinline__ void small_dft(float2 (&a)[R], int sign) {
+  const float sg = (float)sign;
+  if constexpr (R == 2) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  } else if constexpr (R == 4) {
+    float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]), t3 = cmul_if(csub(a[1], a[3]), sg);
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  } else if constexpr (R == 3) {
+    const float h3 = 0.86602540378443865f;  // sin(2 pi / 3)
+    float2 t = cadd(a[1], a[2]);
+    float2 u = cmul_if(csub(a[1], a[2]), sg * h3);
+    float2 m = csub(a[0], cscale(t, 0.5f));
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, u);
+    a[2] = csub(m, u);
+  } else {  // R == 5
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;  // cos 2pi/5, cos 4pi/5
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;   // sin 2pi/5, sin 4pi/5
+    float2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+    float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+    float2 e1 = cadd(a[0], cadd(cscale(b1, c1), cscale(b2, c2)));
+    float2 e2 = cadd(a[0], cadd(cscale(b1, c2), cscale(b2, c1)));
+    float2 f1 = cmul_if(cadd(cscale(d1, s1), cscale(d2, s2)), sg);
+    float2 f2 = cmul_if(csub(cscale(d1, s2), cscale(d2, s1)), sg);
+    a[0] = cadd(a[0], cadd(b1, b2));
+    a[1] = cadd(e1, f1);
+    a[4] = csub(e1, f1);
+    a[2] = cadd(e2, f2);
+    a[3] = csub(e2, f2);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void mr_stage(float2* d, unsigned n, unsigned nlines, unsigned lstride,
+                                         unsigned ns, int sign, const float2* stw, unsigned twn) {
+  const unsigned m = n / R, total = nlines * m;
+  const unsigned tstep = twn / (ns * R);
+  float2 v[FFT_MR_PER_THREAD][R];
+  unsigned dst[FFT_MR_PER_THREAD];
+#pragma unroll
+  for (int b = 0; b < FFT_MR_PER_THREAD; ++b) {
+    unsigned idx = threadIdx.x + (unsigned)b * blockDim.x;
+    dst[b] = 0xFFFFFFFFu;
+    if (idx < total) {
+      unsigned line = idx / m, j = idx - line * m;
+      unsigned k = j % ns;
+      const float2* row = d + line * lstride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = row[j + (unsigned)r * m];
+        v[b][r] = r ? cmul(x, tw_full(stw, (unsigned)r * k * tstep, twn, sign)) : x;
+      }
+      small_dft<R>(v[b], sign);
+      dst[b] = line * lstride + (j / ns) * ns * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < FFT_MR_PER_THREAD; ++b)
+    if (dst[b] != 0xFFFFFFFFu) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[dst[b] + (unsigned)r * ns] = v[b][r];
+    }
+  __syncthreads();
+}
+
+// Complex FFT of `nlines` LDS lines of length n (2^a 3^b 5^c), natural
+// order in and out; stw: exp(-2 pi i t / twn), t < twn / 2, with n | twn.
+__device__ __forceinline__ void lds_fft_mixed(float2* d, unsigned n, unsigned nlines,
+                                              unsigned lstride, int sign, const float2* stw,
+                                              unsigned twn) {
+  unsigned ns = 1, rem = n;
+  while (rem > 1) {
+    if (rem % 4 == 0) {
+      mr_stage<4>(d, n, nlines, lstride, ns, sign, stw, twn);
+      ns *= 4; rem /= 4;
+    } else if (rem % 2 == 0) {
+      mr_stage<2>(d, n, nlines, lstride, ns, sign, stw, twn);
+      ns *= 2; rem /= 2;
+    } else if (rem % 3 == 0) {
+      mr_stage<3>(d, n, nlines, lstride, ns, sign, stw, twn);
+      ns *= 3; rem /= 3;
+    } else {
+      mr_stage<5>(d, n, nlines, lstride, ns, sign, stw, twn);
+      ns *= 5; rem /= 5;
+    }
+  }
+}
+
 __device__ __forceinline__ unsigned bitrev(unsigned k, unsigned log2n) {
   return log2n ? (__brev(k) >> (32 - log2n)) : 0u;
 }
@@ -394,6 +528,7 @@ __device__ __forceinline__ unsigned bitrev(unsigned k, unsigned log2n) {
 // (converted to f32 exactly as spme_recip.cu:237-253), packed as nc/2
 // complex values, FFT'd, then untangled into nc/2+1 spectrum values.
 // twc: twiddles for length nc (exp(-2 pi i k/nc), k < nc/2).
+template <bool MIXED>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
     const long long* __restrict__ rho, float2* __restrict__ cgrid, unsigned nrows, unsigned nc,
     unsigned log2h, const float2* __restrict__ twc) {
@@ -408,10 +543,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
   for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
     unsigned r = e / h, m = e - r * h;
     longlong2 v = reinterpret_cast<const longlong2*>(rho + (size_t)(r0 + r) * nc)[m];
-    d[r * h + bitrev(m, log2h)] = make_float2((float)v.x * inv, (float)v.y * inv);
+    d[r * h + (MIXED ? m : bitrev(m, log2h))] = make_float2((float)v.x * inv, (float)v.y * inv);
   }
   __syncthreads();
-  lds_fft(d, h, log2h, nr, h, -1, stw, nc);
+  if (MIXED)
+    lds_fft_mixed(d, h, nr, h, -1, stw, nc);
+  else
+    lds_fft(d, h, log2h, nr, h, -1, stw, nc);
   for (unsigned e = threadIdx.x; e < nr * nch; e += blockDim.x) {
     unsigned r = e / nch, k = e - r * nch;
     float2 zk = d[r * h + (k % h)];
@@ -428,6 +566,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
 
 // c-axis inverse C2R: nc/2+1 spectrum values per row -> nc reals,
 // unnormalised (round trip = nc * x).
+template <bool MIXED>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_inv(
     const float2* __restrict__ cgrid, float* __restrict__ vgrid, unsigned nrows, unsigned nc,
     unsigned log2h, const float2* __restrict__ twc) {
@@ -450,10 +589,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_inv(
     w.y = -w.y;  // exp(+2 pi i k / nc)
     float2 wB = make_float2(w.x * B.x - w.y * B.y, w.x * B.y + w.y * B.x);
     // Z' = A + i * wB
-    d[r * h + bitrev(k, log2h)] = make_float2(A.x - wB.y, A.y + wB.x);
+    d[r * h + (MIXED ? k : bitrev(k, log2h))] = make_float2(A.x - wB.y, A.y + wB.x);
   }
   __syncthreads();
-  lds_fft(d, h, log2h, nr, h, +1, stw, nc);
+  if (MIXED)
+    lds_fft_mixed(d, h, nr, h, +1, stw, nc);
+  else
+    lds_fft(d, h, log2h, nr, h, +1, stw, nc);
   for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
     unsigned r = e / h, m = e - r * h;
     reinterpret_cast<float2*>(vgrid + (size_t)(r0 + r) * nc)[m] = d[r * h + m];
@@ -487,6 +629,7 @@ __device__ __forceinline__ void strided_store(const float2* d, float2* __restric
 }
 
 // b-axis pass (forward or inverse), in place.  grid: (na * nchunks).
+template <bool MIXED>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgrid, unsigned na,
                                                        unsigned nb, unsigned nch,
                                                        unsigned log2b, int sign,
@@ -498,9 +641,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgri
   unsigned ncol = min(FFT_CW, nch - c0);
   size_t base = (size_t)a * nb * nch + c0;
   fft_stage_twiddles(stw, twb, nb);
-  strided_load(d, cgrid, base, nb, log2b, nch, ncol, true);
+  strided_load(d, cgrid, base, nb, log2b, nch, ncol, !MIXED);
   __syncthreads();
-  lds_fft(d, nb, log2b, FFT_CW, nb + 1, sign, stw, nb);
+  if (MIXED)
+    lds_fft_mixed(d, nb, FFT_CW, nb + 1, sign, stw, nb);
+  else
+    lds_fft(d, nb, log2b, FFT_CW, nb + 1, sign, stw, nb);
   strided_store(d, cgrid, base, nb, log2b, nch, ncol, false);
 }
 
@@ -510,6 +656,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgri
 //   virial partials hw*G*|rho^|^2 and hw*vf*|rho^|^2 (spme_recip.cu:
 //   296-342) summed in FP64 by a fixed tree -> V^ = G rho^ -> inverse FFT
 //   over a.  grid: (nb * nchunks); partials[blockIdx] = (E, W).
+template <bool MIXED>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
     float2* __restrict__ cgrid, unsigned na, unsigned nb, unsigned nc, unsigned nch,
     unsigned log2a, const float2* __restrict__ twa, const float* __restrict__ bfa,
@@ -525,9 +672,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
   size_t estride = (size_t)nb * nch;
   unsigned ls = na + 1;
   fft_stage_twiddles(stw, twa, na);
-  strided_load(d, cgrid, base, na, log2a, estride, ncol, true);
+  strided_load(d, cgrid, base, na, log2a, estride, ncol, !MIXED);
   __syncthreads();
-  lds_fft(d, na, log2a, FFT_CW, ls, -1, stw, na);
+  if (MIXED)
+    lds_fft_mixed(d, na, FFT_CW, ls, -1, stw, na);
+  else
+    lds_fft(d, na, log2a, FFT_CW, ls, -1, stw, na);
   HdLattice L = hd_load_lattice(lattice);
   float v_box = L.lx * L.ly * L.lz;
   float four_alpha2 = 4.0f * alpha * alpha;
@@ -572,8 +722,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_a_conv(
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = make_double2(red_e[0], red_w[0]);
-  lds_fft_dif(d, na, log2a, FFT_CW, ls, +1, stw, na);
-  strided_store(d, cgrid, base, na, log2a, estride, ncol, true);
+  if (MIXED) {
+    lds_fft_mixed(d, na, FFT_CW, ls, +1, stw, na);
+    strided_store(d, cgrid, base, na, log2a, estride, ncol, false);
+  } else {
+    lds_fft_dif(d, na, log2a, FFT_CW, ls, +1, stw, na);
+    strided_store(d, cgrid, base, na, log2a, estride, ncol, true);
+  }
 }
 
 // Fixed-order FP64 reduction of the (E, W) partials; writes
@@ -630,24 +785,48 @@ void hd_launch_recip_reduce(hipStream_t st, const HdSpmeArgs& a, unsigned n_part
                      a.w_share, a.n);
 }
 
+static bool pow2u(unsigned v) { return v && !(v & (v - 1)); }
+
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
   unsigned nch = a.nc / 2 + 1;
   hd_launch_spme_spread(st, a);
   unsigned nrows = a.na * a.nb;
   unsigned h = a.nc / 2;
   unsigned rows = FFT_TILE / h;
-  hipLaunchKernelGGL(k_fft_c_fwd, dim3((nrows + rows - 1) / rows), dim3(FFT_THREADS), 0, st,
-                     a.rho_fixed, a.cgrid, nrows, a.nc, ilog2u(h), a.tw_c);
+  dim3 gc((nrows + rows - 1) / rows), tb(FFT_THREADS);
+  bool mc = !pow2u(h), mb = !pow2u(a.nb), ma = !pow2u(a.na);
+  if (mc)
+    hipLaunchKernelGGL(k_fft_c_fwd<true>, gc, tb, 0, st, a.rho_fixed, a.cgrid, nrows, a.nc, 0u,
+                       a.tw_c);
+  else
+    hipLaunchKernelGGL(k_fft_c_fwd<false>, gc, tb, 0, st, a.rho_fixed, a.cgrid, nrows, a.nc,
+                       ilog2u(h), a.tw_c);
   unsigned nchunk = (nch + FFT_CW - 1) / FFT_CW;
-  hipLaunchKernelGGL(k_fft_b, dim3(a.na * nchunk), dim3(FFT_THREADS), 0, st, a.cgrid, a.na, a.nb,
-                     nch, ilog2u(a.nb), -1, a.tw_b);
-  hipLaunchKernelGGL(k_fft_a_conv, dim3(a.nb * nchunk), dim3(FFT_THREADS), 0, st, a.cgrid, a.na,
-                     a.nb, a.nc, nch, ilog2u(a.na), a.tw_a, a.bf_a, a.bf_b, a.bf_c, a.lattice,
-                     (float)a.kc, (float)a.alpha, a.partials);
-  hipLaunchKernelGGL(k_fft_b, dim3(a.na * nchunk), dim3(FFT_THREADS), 0, st, a.cgrid, a.na, a.nb,
-                     nch, ilog2u(a.nb), +1, a.tw_b);
-  hipLaunchKernelGGL(k_fft_c_inv, dim3((nrows + rows - 1) / rows), dim3(FFT_THREADS), 0, st,
-                     a.cgrid, a.vgrid, nrows, a.nc, ilog2u(h), a.tw_c);
+  dim3 gb(a.na * nchunk), ga(a.nb * nchunk);
+  for (int sign = -1; sign <= 1; sign += 2) {
+    if (sign > 0) {
+      if (ma)
+        hipLaunchKernelGGL(k_fft_a_conv<true>, ga, tb, 0, st, a.cgrid, a.na, a.nb, a.nc, nch, 0u,
+                           a.tw_a, a.bf_a, a.bf_b, a.bf_c, a.lattice, (float)a.kc,
+                           (float)a.alpha, a.partials);
+      else
+        hipLaunchKernelGGL(k_fft_a_conv<false>, ga, tb, 0, st, a.cgrid, a.na, a.nb, a.nc, nch,
+                           ilog2u(a.na), a.tw_a, a.bf_a, a.bf_b, a.bf_c, a.lattice, (float)a.kc,
+                           (float)a.alpha, a.partials);
+    }
+    if (mb)
+      hipLaunchKernelGGL(k_fft_b<true>, gb, tb, 0, st, a.cgrid, a.na, a.nb, nch, 0u, sign,
+                         a.tw_b);
+    else
+      hipLaunchKernelGGL(k_fft_b<false>, gb, tb, 0, st, a.cgrid, a.na, a.nb, nch, ilog2u(a.nb),
+                         sign, a.tw_b);
+  }
+  if (mc)
+    hipLaunchKernelGGL(k_fft_c_inv<true>, gc, tb, 0, st, a.cgrid, a.vgrid, nrows, a.nc, 0u,
+                       a.tw_c);
+  else
+    hipLaunchKernelGGL(k_fft_c_inv<false>, gc, tb, 0, st, a.cgrid, a.vgrid, nrows, a.nc,
+                       ilog2u(h), a.tw_c);
   hipLaunchKernelGGL(k_recip_reduce, dim3(1), dim3(256), 0, st, a.partials, a.nb * nchunk,
                      a.recip, a.w_share, a.n);
 }

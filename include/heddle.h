@@ -77,13 +77,16 @@ int hd_set_images(hd_ctx *ctx, const int32_t *ix, const int32_t *iy, const int32
 int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilon,
               const double *cutoff, const double *r_switch);
 /* SPME: real-space erfc pair term + reciprocal grid
- * (src/forces/spme.rs:36-52 SpmeParameters).  Grid dims must be powers of
- * two >= 8; order must be 4. */
+ * (src/forces/spme.rs:36-52 SpmeParameters).  Grid sides of 2^a 3^b 5^c
+ * points (the small primes the reference's cuFFT plans take), at least 8,
+ * at most 256 along a and b and at most 512 (even) along c; order must be
+ * 4. */
 int hd_set_spme(hd_ctx *ctx, double alpha, double r_cut, int na, int nb, int nc, int order,
                 double k_c);
 /* Reciprocal-space transform: mode 0 = FP32 radix FFT in LDS (default),
  * mode 1 = split-precision FP16 matrix-core DFT (each 1D stage a GEMM with
- * high/low FP16 operands accumulated in FP32; axes up to 128 points).  The
+ * high/low FP16 operands accumulated in FP32; power-of-two axes of 32 to
+ * 128 points).  The
  * reference uses cuFFT in FP32 (src/gpu/cufft.rs); mode 1 has no
  * counterpart there.  May be switched between phases. */
 int hd_set_spme_dft(hd_ctx *ctx, int mode);

@@ -49,7 +49,7 @@ def test_b_factors_closed_form(oracle):
     assert np.all(b > 0)
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 32), (6, 10, 4)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 32), (6, 10, 4), (12, 20, 30)])
 def test_fft3d_matches_numpy(oracle, shape):
     rng = np.random.default_rng(0)
     x = rng.normal(size=shape)
