@@ -50,6 +50,15 @@ SIGNATURES = {
     "hd_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "hd_set_exclusion_span": (C.c_int, [_ctx, C.c_uint32]),
     "hd_set_angles": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),
+    "hd_set_csvr": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64, C.c_uint64]),
+    "hd_set_berendsen": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64]),
+    "hd_get_thermostat": (C.c_int, [_ctx, _f64p]),
+    "hd_set_berendsen_barostat": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double]),
+    "hd_minimize_begin": (C.c_int, [_ctx, C.c_double, _f64p]),
+    "hd_minimize_step": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, _f64p]),
+    "hd_set_bonds": (C.c_int, [_ctx, C.c_uint64, _u32p, C.POINTER(C.c_int), _f64p]),
+    "hd_set_dihedrals": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p, _f64p]),
+    "hd_set_exclusions": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),
     "hd_set_neighbor": (C.c_int, [_ctx, C.c_double]),
     "hd_set_pair_list": (C.c_int, [_ctx, C.c_int]),
     "hd_set_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,

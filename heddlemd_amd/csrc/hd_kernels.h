@@ -69,9 +69,33 @@ struct HdPairArgs {
 // only (no pair energy is accumulated)
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev);
 void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,
-                      const unsigned* ijk, const float* kth, const float* th0,
+                      const unsigned* ijk, const double* kth, const double* th0,
                       const unsigned* slot_of, long long* fpj, long long* fpj_ew,
                       unsigned n_angles, bool ev);
+
+// Two-body bonded terms: kind 0 harmonic (k, r0), 1 Morse (De, a, re);
+// three parameters per bond.
+void hd_launch_bonds(hipStream_t st, const float4* posq, const float* lattice,
+                     const unsigned* ij, const int* kind, const double* prm,
+                     const unsigned* slot_of, long long* fpj, long long* fpj_ew,
+                     unsigned n_bonds, bool ev);
+// Periodic dihedrals: k, n, phi0 per dihedral.
+void hd_launch_dihedrals(hipStream_t st, const float4* posq, const float* lattice,
+                         const unsigned* ijkl, const double* prm, const unsigned* slot_of,
+                         long long* fpj, long long* fpj_ew, unsigned n_dihedrals, bool ev);
+
+// Scaled exclusions: for each listed pair the pair kernel's contribution
+// (weight 1, or 0 when the keys exclude the pair) is moved to the pair's
+// scale by adding w times the LJ and real-space Coulomb terms.
+struct HdExclPair {
+  unsigned i, j;
+  double sigma, eps, cut2, rsw2;  // LJ of the pair's types
+  double w_lj, w_coul;            // scale minus the pair kernel's weight
+};
+void hd_launch_excl_scaled(hipStream_t st, const float4* posq, const float* lattice,
+                           const HdExclPair* pairs, unsigned n_pairs, double kc, double alpha,
+                           double coul_rc2, const unsigned* slot_of, long long* fpj,
+                           long long* fpj_ew, bool ev);
 
 struct HdSpmeArgs {
   unsigned n, na, nb, nc;
@@ -298,6 +322,35 @@ struct hd_ctx;
 extern "C" int hd_ctx_shake_log_view(hd_ctx* c, int* device, const float** x,
                                      const float** t, uint64_t* rows);
 
+// Stochastic velocity rescaling (kind 2: CSVR, Bussi, Donadio and
+// Parrinello, J. Chem. Phys. 126, 014101 (2007); src/integrator/csvr.rs,
+// kernels/nose_hoover.cu:145-255) and the weak-coupling rescale (kind 3:
+// Berendsen, src/integrator/berendsen.rs, nose_hoover.cu:390-417), applied
+// as the post-force velocity factor.
+#define HD_CSVR_BLOCKS 64
+// Steepest descent: save posq, posq += F * scale (and copy the moved
+// positions to `frame` when non-null); restore; max |F|^2 in FP64.
+void hd_launch_sd_move(hipStream_t st, float4* posq, const float* fx, const float* fy,
+                       const float* fz, float4* save, float4* frame, float scale, unsigned n);
+void hd_launch_sd_restore(hipStream_t st, float4* posq, const float4* save, unsigned n);
+void hd_launch_fmax2(hipStream_t st, const float* fx, const float* fy, const float* fz,
+                     unsigned n, double* out);
+struct HdDispArgs;
+struct HdRescaleArgs {
+  int kind;                        // 2 CSVR, 3 Berendsen
+  unsigned seed_lo, seed_hi, g_dof;
+  double c, one_minus_c, kt_per_dof;  // CSVR: exp(-dt/tau), 1 - c, K_target / g
+  double k_target, dt_over_tau;       // Berendsen
+  unsigned long long* counter;  // CSVR Philox draw counter (one per step)
+  double* injection;            // cumulative K_new - K_old
+  double* partials;             // CSVR: HD_CSVR_BLOCKS sums of xi^2
+};
+// The kinetic energy (and the displacement check) as hd_launch_ke, then the
+// rescale factor of `t` into factor[0].
+void hd_launch_ke_rescale(hipStream_t st, unsigned n, const float* vx, const float* vy,
+                          const float* vz, const float* mass, double* partials, double* ke,
+                          const HdRescaleArgs& t, float* factor, const HdDispArgs* disp);
+
 struct HdNhcArgs {
   int chain, n_yoshida, n_resp;
   double w[7];
@@ -331,6 +384,7 @@ void hd_launch_ke(hipStream_t st, unsigned n, const float* vx, const float* vy, 
 struct HdBarostatArgs {
   unsigned seed_lo, seed_hi;
   double pressure, tau, compressibility, kt, dt, mu3_min;
+  int stochastic;  // 1 C-rescale (noise, draw counter, P0 dV), 0 Berendsen
 };
 // C-rescale: the scalar reduction of hd_launch_sum2_final whose last
 // thread also computes mu from K = k[0], W = the reduced virial (out[1])

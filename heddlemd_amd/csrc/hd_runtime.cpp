@@ -23,6 +23,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -78,9 +80,23 @@ struct hd_ctx {
   unsigned excl_span = 0;
   int atom_lists = 0;  // 1: per-atom full neighbour lists instead of the packed cluster list
   std::vector<uint32_t> h_angles;  // i, j, k per angle
-  std::vector<float> h_ang_k, h_ang_t0;
+  std::vector<double> h_ang_k, h_ang_t0;
   unsigned* d_angles = nullptr;
-  float *d_ang_k = nullptr, *d_ang_t0 = nullptr;
+  double *d_ang_k = nullptr, *d_ang_t0 = nullptr;
+  std::vector<uint32_t> h_bonds;  // i, j per bond
+  std::vector<int> h_bond_kind;
+  std::vector<double> h_bond_prm;  // 3 per bond
+  unsigned* d_bonds = nullptr;
+  int* d_bond_kind = nullptr;
+  double* d_bond_prm = nullptr;
+  std::vector<uint32_t> h_dih;  // i, j, k, l per dihedral
+  std::vector<double> h_dih_prm;  // k, n, phi0 per dihedral
+  unsigned* d_dih = nullptr;
+  double* d_dih_prm = nullptr;
+  // scaled exclusions: (i, j, w_lj, w_coul) with w = scale - the pair
+  // kernel's weight; the LJ columns are filled at hd_setup
+  std::vector<HdExclPair> h_excl;
+  HdExclPair* d_excl = nullptr;
   std::vector<uint32_t> h_groups;
   double d_oh = 0, d_hh = 0, m_o = 0, m_h = 0;
   double dt = 0;
@@ -93,6 +109,19 @@ struct hd_ctx {
   double baro_p = 0, baro_tau = 0, baro_kappa = 0, baro_kt = 0;
   uint64_t baro_seed = 0;
   bool have_lan = false;  // Langevin BAOAB instead of velocity Verlet
+  // velocity-rescale thermostats: 0 none, 2 CSVR, 3 Berendsen
+  int rescale = 0;
+  double rs_kt = 0, rs_tau = 0;
+  uint64_t rs_seed = 0, rs_nc = 0;
+  unsigned long long* rs_counter = nullptr;  // CSVR draw counter
+  double* rs_injection = nullptr;            // cumulative K_new - K_old
+  double* rs_partials = nullptr;             // CSVR partial sums
+  bool baro_stochastic = true;  // C-rescale; false: Berendsen barostat
+  // steepest descent
+  float4* sd_save = nullptr;
+  double* sd_fmax2 = nullptr;
+  double sd_step = 0, sd_energy = 0;
+  bool sd_begun = false;
   double lan_friction = 0, lan_kt = 0;
   uint64_t lan_seed = 0;
   bool use_graph = true;
@@ -550,6 +579,14 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   if (!c->h_ang_k.empty())
     hd_launch_angles(st, c->posq, c->d_lattice, c->d_angles, c->d_ang_k, c->d_ang_t0, c->slot_of,
                      c->fpj, c->fpj_ew, (unsigned)c->h_ang_k.size(), ev);
+  hd_launch_bonds(st, c->posq, c->d_lattice, c->d_bonds, c->d_bond_kind, c->d_bond_prm,
+                  c->slot_of, c->fpj, c->fpj_ew, (unsigned)c->h_bond_kind.size(), ev);
+  hd_launch_dihedrals(st, c->posq, c->d_lattice, c->d_dih, c->d_dih_prm, c->slot_of, c->fpj,
+                      c->fpj_ew, (unsigned)(c->h_dih_prm.size() / 3), ev);
+  hd_launch_excl_scaled(st, c->posq, c->d_lattice, c->d_excl, (unsigned)c->h_excl.size(), c->kc,
+                        c->have_spme ? c->alpha : 0.0,
+                        c->have_spme ? c->coul_rc * c->coul_rc : 0.0, c->slot_of, c->fpj,
+                        c->fpj_ew, ev);
   stage_end(c, timed);
   if (fork)
     (void)hipStreamWaitEvent(st, c->ev_join, 0);
@@ -590,7 +627,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   fa.vylo = c->lossless ? c->vylo : nullptr;
   fa.vzlo = c->lossless ? c->vzlo : nullptr;
   fa.mass = c->mass;
-  fa.factor = c->have_nhc ? c->factors + 1 : nullptr;
+  fa.factor = (c->have_nhc || c->rescale) ? c->factors + 1 : nullptr;
   fa.partials = ev ? c->fin_partials : nullptr;
   stage_begin(c, kick ? "finalize_kick" : "finalize", timed);
   hd_launch_finalize(st, fa, ev);
@@ -738,6 +775,7 @@ HdBarostatArgs baro_args(hd_ctx* c) {
   b.kt = c->baro_kt;
   b.dt = c->dt;
   b.mu3_min = 1e-18;
+  b.stochastic = c->baro_stochastic ? 1 : 0;
   return b;
 }
 
@@ -786,6 +824,29 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   if (c->have_baro) enqueue_crescale(c, timed);
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
+}
+
+// CSVR (src/integrator/csvr.rs:188-240) and Berendsen
+// (src/integrator/berendsen.rs:139-170) in thermal degrees of freedom
+// g = 3N - n_constraints - 3 (at least 1)
+HdRescaleArgs rescale_args(hd_ctx* c) {
+  HdRescaleArgs t;
+  memset(&t, 0, sizeof(t));
+  t.kind = c->rescale;
+  long long g = std::max(3LL * (long long)c->n - (long long)c->rs_nc - 3LL, 1LL);
+  t.g_dof = (unsigned)g;
+  t.seed_lo = (unsigned)(c->rs_seed & 0xFFFFFFFFull);
+  t.seed_hi = (unsigned)(c->rs_seed >> 32);
+  t.c = exp(-(double)(float)c->dt / c->rs_tau);
+  t.one_minus_c = 1.0 - t.c;
+  double k_target = 0.5 * (double)g * c->rs_kt;
+  t.kt_per_dof = k_target / (double)g;
+  t.k_target = k_target;
+  t.dt_over_tau = (double)(float)c->dt / c->rs_tau;
+  t.counter = c->rs_counter;
+  t.injection = c->rs_injection;
+  t.partials = c->rs_partials;
+  return t;
 }
 
 void enqueue_step(hd_ctx* c, bool ev, bool timed) {
@@ -846,8 +907,13 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   disp.ref_lattice = c->ref_lattice;
   disp.half_skin = (float)(0.5 * c->r_skin);
   disp.status = c->ctrl;
-  bool disp_done = c->have_nhc || c->have_baro;
-  if (c->have_nhc) {
+  bool disp_done = c->have_nhc || c->rescale || c->have_baro;
+  if (c->rescale) {
+    stage_begin(c, c->rescale == 2 ? "csvr" : "berendsen", timed);
+    hd_launch_ke_rescale(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3,
+                         rescale_args(c), c->factors + 1, &disp);
+    stage_end(c, timed);
+  } else if (c->have_nhc) {
     stage_begin(c, "nhc_post", timed);
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, &nhc,
                  (float)c->dt, c->factors + 1, &disp);
@@ -1094,12 +1160,163 @@ int hd_set_angles(hd_ctx* c, uint64_t n_angles, const uint32_t* ijk, const doubl
   for (uint64_t a = 0; a < 3 * n_angles; ++a)
     if (ijk[a] >= c->n) return fail(c, HD_ERR_ARG, "angle atom %u out of range", ijk[a]);
   c->h_angles.assign(ijk, ijk + 3 * n_angles);
-  c->h_ang_k.resize(n_angles);
-  c->h_ang_t0.resize(n_angles);
-  for (uint64_t a = 0; a < n_angles; ++a) {
-    c->h_ang_k[a] = (float)k_theta[a];
-    c->h_ang_t0[a] = (float)theta0[a];
+  c->h_ang_k.assign(k_theta, k_theta + n_angles);
+  c->h_ang_t0.assign(theta0, theta0 + n_angles);
+  return HD_OK;
+}
+
+int hd_set_bonds(hd_ctx* c, uint64_t n_bonds, const uint32_t* ij, const int* kind,
+                 const double* prm) {
+  if (!c || (n_bonds && (!ij || !kind || !prm))) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_bonds after hd_setup");
+  for (uint64_t b = 0; b < n_bonds; ++b) {
+    if (ij[2 * b] >= c->n || ij[2 * b + 1] >= c->n || ij[2 * b] == ij[2 * b + 1])
+      return fail(c, HD_ERR_ARG, "bond %llu: bad atoms %u %u", (unsigned long long)b, ij[2 * b],
+                  ij[2 * b + 1]);
+    if (kind[b] != 0 && kind[b] != 1)
+      return fail(c, HD_ERR_ARG, "bond %llu: kind %d (0 harmonic, 1 Morse)",
+                  (unsigned long long)b, kind[b]);
   }
+  c->h_bonds.assign(ij, ij + 2 * n_bonds);
+  c->h_bond_kind.assign(kind, kind + n_bonds);
+  c->h_bond_prm.assign(prm, prm + 3 * n_bonds);
+  return HD_OK;
+}
+
+int hd_set_dihedrals(hd_ctx* c, uint64_t n_dih, const uint32_t* ijkl, const double* k_phi,
+                     const double* n_mult, const double* phi0) {
+  if (!c || (n_dih && (!ijkl || !k_phi || !n_mult || !phi0))) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_dihedrals after hd_setup");
+  c->h_dih.assign(ijkl, ijkl + 4 * n_dih);
+  c->h_dih_prm.resize(3 * n_dih);
+  for (uint64_t d = 0; d < n_dih; ++d) {
+    for (int a = 0; a < 4; ++a)
+      if (ijkl[4 * d + a] >= c->n)
+        return fail(c, HD_ERR_ARG, "dihedral %llu: atom %u out of range", (unsigned long long)d,
+                    ijkl[4 * d + a]);
+    c->h_dih_prm[3 * d] = k_phi[d];
+    c->h_dih_prm[3 * d + 1] = n_mult[d];
+    c->h_dih_prm[3 * d + 2] = phi0[d];
+  }
+  return HD_OK;
+}
+
+// Full exclusions (both scales 0) become a key rule the pair kernel applies
+// per pair: either every component of the full-exclusion graph is a clique
+// (molecules; one key per component, span 0), or the components are runs
+// of consecutive atoms in which exactly the pairs at most s apart are
+// excluded (chains with 1-2 ... 1-(s+1) exclusions; key = index plus s per
+// earlier component, span s).  Scaled pairs keep their listed scales
+// through the correction pass.
+int hd_set_exclusions(hd_ctx* c, uint64_t n_pairs, const uint32_t* ij, const double* s_lj,
+                      const double* s_coul) {
+  if (!c || (n_pairs && (!ij || !s_lj || !s_coul))) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_exclusions after hd_setup");
+  if (!c->n) return fail(c, HD_ERR_STATE, "hd_set_exclusions before hd_set_system");
+  uint64_t n = c->n;
+  std::vector<std::pair<uint64_t, uint64_t>> full;
+  std::set<std::pair<uint32_t, uint32_t>> seen;
+  for (uint64_t p = 0; p < n_pairs; ++p) {
+    uint32_t a = ij[2 * p], b = ij[2 * p + 1];
+    if (a >= n || b >= n || a == b)
+      return fail(c, HD_ERR_ARG, "exclusion %llu: bad atoms %u %u", (unsigned long long)p, a, b);
+    if (!(s_lj[p] >= 0 && s_lj[p] <= 1 && s_coul[p] >= 0 && s_coul[p] <= 1))
+      return fail(c, HD_ERR_ARG, "exclusion %llu: scales must lie in [0, 1]",
+                  (unsigned long long)p);
+    auto key = std::make_pair(std::min(a, b), std::max(a, b));
+    if (!seen.insert(key).second)
+      return fail(c, HD_ERR_ARG, "exclusion (%u, %u) listed twice", key.first, key.second);
+    if (s_lj[p] == 0.0 && s_coul[p] == 0.0) full.emplace_back(key.first, key.second);
+  }
+  // components of the full-exclusion graph
+  std::vector<uint64_t> parent(n);
+  for (uint64_t i = 0; i < n; ++i) parent[i] = i;
+  std::function<uint64_t(uint64_t)> root = [&](uint64_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (auto& e : full) {
+    uint64_t ra = root(e.first), rb = root(e.second);
+    if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+  }
+  std::vector<uint64_t> size(n, 0), edges(n, 0), lo(n, UINT64_MAX), hi(n, 0);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t r = root(i);
+    size[r]++;
+    lo[r] = std::min(lo[r], i);
+    hi[r] = std::max(hi[r], i);
+  }
+  uint64_t span = 0;
+  for (auto& e : full) {
+    edges[root(e.first)]++;
+    span = std::max(span, e.second - e.first);
+  }
+  bool cliques = true;
+  for (uint64_t i = 0; i < n; ++i)
+    if (root(i) == i && edges[i] != size[i] * (size[i] - 1) / 2) cliques = false;
+  std::vector<uint32_t> key(n);
+  unsigned new_span = 0;
+  if (cliques) {
+    // one key per component, numbered in order of first atom
+    std::vector<uint32_t> id(n, 0);
+    uint32_t next = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t r = root(i);
+      if (r == i) id[r] = next++;
+      key[i] = id[r];
+    }
+  } else {
+    // runs of consecutive atoms excluded exactly up to index distance span
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t r = root(i);
+      if (hi[r] - lo[r] + 1 != size[r])
+        return fail(c, HD_ERR_UNSUPPORTED,
+                    "full exclusions around atom %llu: a molecule that is not one clique must "
+                    "be a run of consecutive atoms",
+                    (unsigned long long)i);
+    }
+    uint64_t want = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t r = root(i);
+      if (r != i) continue;
+      uint64_t m = size[r];
+      uint64_t k = std::min(span, m - 1);
+      want = m * k - k * (k + 1) / 2;  // pairs at most span apart inside the run
+      if (edges[r] != want)
+        return fail(c, HD_ERR_UNSUPPORTED,
+                    "full exclusions of atoms %llu..%llu are neither all pairs nor exactly the "
+                    "pairs at most %llu apart",
+                    (unsigned long long)lo[r], (unsigned long long)hi[r],
+                    (unsigned long long)span);
+    }
+    if (span > (1u << 20) || (uint64_t)n * (span + 1) >= HD_SENTINEL_KEY_LIMIT)
+      return fail(c, HD_ERR_UNSUPPORTED, "exclusion span %llu too large",
+                  (unsigned long long)span);
+    uint64_t comp = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (i > 0 && root(i) != root(i - 1)) ++comp;
+      key[i] = (uint32_t)(i + comp * span);
+    }
+    new_span = (unsigned)span;
+  }
+  // scaled pairs: weight relative to what the key rule leaves
+  std::vector<HdExclPair> sc;
+  for (uint64_t p = 0; p < n_pairs; ++p) {
+    if (s_lj[p] == 0.0 && s_coul[p] == 0.0) continue;
+    uint32_t a = ij[2 * p], b = ij[2 * p + 1];
+    uint32_t d = key[a] > key[b] ? key[a] - key[b] : key[b] - key[a];
+    double base = d <= new_span ? 0.0 : 1.0;
+    HdExclPair e;
+    memset(&e, 0, sizeof(e));
+    e.i = a;
+    e.j = b;
+    e.w_lj = s_lj[p] - base;
+    e.w_coul = s_coul[p] - base;
+    if (e.w_lj != 0.0 || e.w_coul != 0.0) sc.push_back(e);
+  }
+  c->h_key = key;
+  c->excl_span = new_span;
+  c->h_excl = sc;
   return HD_OK;
 }
 
@@ -1483,8 +1700,8 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
     c->have_nhc = false;
     return HD_OK;
   }
-  if (c->have_lan)
-    return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: no Nose-Hoover chain");
+  if (c->have_lan || c->rescale)
+    return fail(c, HD_ERR_STATE, "one thermostat per context: no Nose-Hoover chain beside it");
   if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
       !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
     return fail(c, HD_ERR_ARG, "invalid Nose-Hoover chain parameters");
@@ -1510,12 +1727,57 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
   return HD_OK;
 }
 
+static int set_rescale(hd_ctx* c, int kind, double kt, double tau, uint64_t seed,
+                       uint64_t n_constraints) {
+  if (!c) return HD_ERR_ARG;
+  if (!(kt > 0) || !(tau > 0) || !std::isfinite(kt) || !std::isfinite(tau))
+    return fail(c, HD_ERR_ARG, "thermostat temperature and tau must be finite and > 0");
+  if (c->have_nhc || c->have_lan)
+    return fail(c, HD_ERR_STATE, "one thermostat per context: a Nose-Hoover chain or Langevin "
+                                 "BAOAB is already set");
+  if (c->ready) {
+    // a new phase re-parameterises the same kind; the counter and the
+    // injection account carry on
+    if (c->rescale != kind)
+      return fail(c, HD_ERR_STATE, "the thermostat kind is fixed at hd_setup");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  c->rescale = kind;
+  c->rs_kt = kt;
+  c->rs_tau = tau;
+  c->rs_seed = seed;
+  c->rs_nc = n_constraints;
+  return HD_OK;
+}
+
+int hd_set_csvr(hd_ctx* c, double kt, double tau, uint64_t seed, uint64_t n_constraints) {
+  return set_rescale(c, 2, kt, tau, seed, n_constraints);
+}
+
+int hd_set_berendsen(hd_ctx* c, double kt, double tau, uint64_t n_constraints) {
+  return set_rescale(c, 3, kt, tau, 0, n_constraints);
+}
+
+int hd_get_thermostat(hd_ctx* c, double out[2]) {
+  if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
+  out[0] = out[1] = 0.0;
+  if (!c->rescale) return HD_OK;
+  unsigned long long ctr = 0;
+  HIPCHK(c, hipMemcpyAsync(out, c->rs_injection, sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(&ctr, c->rs_counter, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[1] = (double)ctr;
+  return HD_OK;
+}
+
 int hd_set_langevin(hd_ctx* c, double friction, double kt, uint64_t seed) {
   if (!c) return HD_ERR_ARG;
   if (!(friction > 0) || !(kt > 0) || !std::isfinite(friction) || !std::isfinite(kt))
     return fail(c, HD_ERR_ARG, "Langevin friction and temperature must be finite and > 0");
-  if (c->have_nhc)
-    return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: remove the Nose-Hoover chain");
+  if (c->have_nhc || c->rescale)
+    return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: remove the other thermostat");
   if (c->lossless) return fail(c, HD_ERR_UNSUPPORTED, "Langevin BAOAB has no lossless mode");
   if (c->ready) {
     if (!c->have_lan)
@@ -1555,11 +1817,36 @@ int hd_set_crescale(hd_ctx* c, double pressure, double tau, double compressibili
   }
   if (!(compressibility > 0) || !(kt >= 0)) return fail(c, HD_ERR_ARG, "invalid C-rescale parameters");
   c->have_baro = true;
+  c->baro_stochastic = true;
   c->baro_p = pressure;
   c->baro_tau = tau;
   c->baro_kappa = compressibility;
   c->baro_kt = kt;
   c->baro_seed = seed;
+  return HD_OK;
+}
+
+int hd_set_berendsen_barostat(hd_ctx* c, double pressure, double tau, double compressibility) {
+  if (!c) return HD_ERR_ARG;
+  if (c->lossless && tau > 0)
+    return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support a barostat");
+  if (c->ready) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  if (tau <= 0) {
+    c->have_baro = false;
+    return HD_OK;
+  }
+  if (!(compressibility > 0) || !std::isfinite(pressure))
+    return fail(c, HD_ERR_ARG, "invalid Berendsen barostat parameters");
+  c->have_baro = true;
+  c->baro_stochastic = false;
+  c->baro_p = pressure;
+  c->baro_tau = tau;
+  c->baro_kappa = compressibility;
+  c->baro_kt = 0.0;
+  c->baro_seed = 0;
   return HD_OK;
 }
 
@@ -1773,6 +2060,39 @@ int hd_setup(hd_ctx* c) {
     if ((r = upload(c, c->d_ang_k, c->h_ang_k.data(), c->h_ang_k.size()))) return r;
     if ((r = upload(c, c->d_ang_t0, c->h_ang_t0.data(), c->h_ang_t0.size()))) return r;
   }
+  if (!c->h_bond_kind.empty()) {
+    DALLOC(c, c->d_bonds, c->h_bonds.size());
+    DALLOC(c, c->d_bond_kind, c->h_bond_kind.size());
+    DALLOC(c, c->d_bond_prm, c->h_bond_prm.size());
+    if ((r = upload(c, c->d_bonds, c->h_bonds.data(), c->h_bonds.size()))) return r;
+    if ((r = upload(c, c->d_bond_kind, c->h_bond_kind.data(), c->h_bond_kind.size()))) return r;
+    if ((r = upload(c, c->d_bond_prm, c->h_bond_prm.data(), c->h_bond_prm.size()))) return r;
+  }
+  if (c->rescale) {
+    if (c->have_lan) return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat");
+    DALLOC(c, c->rs_counter, 1);
+    DALLOC(c, c->rs_injection, 1);
+    DALLOC(c, c->rs_partials, HD_CSVR_BLOCKS);
+  }
+  if (!c->h_dih_prm.empty()) {
+    DALLOC(c, c->d_dih, c->h_dih.size());
+    DALLOC(c, c->d_dih_prm, c->h_dih_prm.size());
+    if ((r = upload(c, c->d_dih, c->h_dih.data(), c->h_dih.size()))) return r;
+    if ((r = upload(c, c->d_dih_prm, c->h_dih_prm.data(), c->h_dih_prm.size()))) return r;
+  }
+  if (!c->h_excl.empty()) {
+    int nt = c->ntypes;
+    if (c->lj_sigma.empty()) return fail(c, HD_ERR_STATE, "exclusions need hd_set_lj");
+    for (auto& e : c->h_excl) {
+      size_t k = (size_t)c->h_type[e.i] * nt + c->h_type[e.j];
+      e.sigma = c->lj_sigma[k];
+      e.eps = c->lj_eps[k];
+      e.cut2 = c->lj_cut[k] * c->lj_cut[k];
+      e.rsw2 = c->lj_rsw[k] * c->lj_rsw[k];
+    }
+    DALLOC(c, c->d_excl, c->h_excl.size());
+    if ((r = upload(c, c->d_excl, c->h_excl.data(), c->h_excl.size()))) return r;
+  }
   if (has_shake(c)) {
     DALLOC(c, c->d_sh_goff, c->h_sh_goff.size());
     DALLOC(c, c->d_sh_atoms, c->h_sh_atoms.size());
@@ -1870,6 +2190,103 @@ int hd_force_eval(hd_ctx* c, int want_scalars) {
                  c->scal + 0, nullptr, 0.f, nullptr);
   HIPCHK(c, hipGetLastError());
   return check_status(c);
+}
+
+// Position-only constraint projection of the minimizer
+// (settle.cu:511-583, shake.cu:562-679): the solvers with the moved
+// positions as their direction frame and no time step, so velocities and
+// the constraint virial are left alone.
+static void enqueue_project_positions(hd_ctx* c) {
+  if (!c->h_groups.empty()) {
+    HdSettleArgs sa = settle_args(c);
+    sa.dt = 0.f;
+    hd_launch_settle_positions(c->stream, sa);
+  }
+  if (has_shake(c)) {
+    HdShakeArgs sa = shake_args(c, 0.f);
+    hd_launch_shake_positions(c->stream, sa);
+  }
+}
+
+static int sd_eval(hd_ctx* c, double* energy, double* fmax) {
+  enqueue_forces(c, 1, false, false);
+  c->forces_valid = true;
+  hd_launch_fmax2(c->stream, c->fx, c->fy, c->fz, (unsigned)c->n, c->sd_fmax2);
+  HIPCHK(c, hipGetLastError());
+  int r = check_status(c);
+  if (r) return r;
+  double pe = 0.0, f2 = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&pe, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&f2, c->sd_fmax2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *energy = pe;
+  *fmax = sqrt(f2);
+  return HD_OK;
+}
+
+int hd_minimize_begin(hd_ctx* c, double initial_step, double out[4]) {
+  if (!c || !out || !(initial_step > 0)) return HD_ERR_ARG;
+  if (!c->ready) return fail(c, HD_ERR_STATE, "hd_minimize_begin before hd_setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->sd_save) {
+    DALLOC(c, c->sd_save, c->n);
+    DALLOC(c, c->sd_fmax2, 1);
+  }
+  if (has_constraints(c) && !c->snap) return fail(c, HD_ERR_STATE, "no constraint frame");
+  double e, f;
+  int r = sd_eval(c, &e, &f);
+  if (r) return r;
+  c->sd_step = initial_step;
+  c->sd_energy = e;
+  c->sd_begun = true;
+  out[0] = e;
+  out[1] = f;
+  out[2] = initial_step;
+  out[3] = 1.0;
+  return HD_OK;
+}
+
+int hd_minimize_step(hd_ctx* c, double max_step, double step_increase, double step_decrease,
+                     double out[4]) {
+  if (!c || !out) return HD_ERR_ARG;
+  if (!c->ready || !c->sd_begun) return fail(c, HD_ERR_STATE, "hd_minimize_step before hd_minimize_begin");
+  HIPCHK(c, hipSetDevice(c->device));
+  double prev = c->sd_energy, used = c->sd_step;
+  double f2 = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&f2, c->sd_fmax2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double fmax = sqrt(f2);
+  out[2] = used;
+  if (!(fmax > 0.0)) {
+    out[0] = prev;
+    out[1] = 0.0;
+    out[3] = 0.0;
+    return HD_OK;
+  }
+  unsigned n = (unsigned)c->n;
+  float4* frame = has_constraints(c) ? c->snap : nullptr;
+  hd_launch_sd_move(c->stream, c->posq, c->fx, c->fy, c->fz, c->sd_save, frame,
+                    (float)(used / fmax), n);
+  if (frame) enqueue_project_positions(c);
+  double e, f;
+  int r = sd_eval(c, &e, &f);
+  if (r) return r;
+  if (e < prev) {
+    c->sd_energy = e;
+    c->sd_step = std::min(c->sd_step * step_increase, max_step);
+    out[0] = e;
+    out[1] = f;
+    out[3] = 1.0;
+  } else {
+    hd_launch_sd_restore(c->stream, c->posq, c->sd_save, n);
+    r = sd_eval(c, &e, &f);
+    if (r) return r;
+    c->sd_step = c->sd_step * step_decrease;
+    out[0] = prev;
+    out[1] = f;
+    out[3] = 0.0;
+  }
+  return HD_OK;
 }
 
 int hd_set_positions(hd_ctx* c, const float* posq) {
@@ -2202,6 +2619,13 @@ int hd_get_scalars(hd_ctx* c, double* out) {
     for (int j = 0; j < m; ++j) chain += st[m + j] * st[m + j] / (2.0 * st[2 * m + j]);
     chain += (double)dof * c->nhc_kt * st[0];
     for (int j = 1; j < m; ++j) chain += c->nhc_kt * st[j];
+  }
+  if (c->rescale) {
+    // CSVR / Berendsen: the conserved quantity is K + U minus the energy
+    // the rescales injected (csvr.rs:266-280, berendsen.rs:191-200)
+    double inj = 0.0;
+    HIPCHK(c, hipMemcpy(&inj, c->rs_injection, sizeof(inj), hipMemcpyDeviceToHost));
+    chain = -inj;
   }
   out[5] = chain;
   out[6] = ctrl[1];

@@ -109,6 +109,9 @@ void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a) {
 // constraint forces of a water sum to zero).
 // ---------------------------------------------------------------------
 #define HD_SETTLE_NEWTON_MAX 8
+// the minimizer's projection (dt = 0) starts from moves of up to max_step,
+// far from the manifold: Newton gets more iterations there
+#define HD_SETTLE_PROJECT_MAX 64
 
 // bond c joins water atoms tri_i(c) and tri_j(c): O-H1, O-H2, H1-H2
 __device__ __forceinline__ int tri_i(int c) { return c == 2 ? 1 : 0; }
@@ -198,7 +201,8 @@ __device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& 
   for (int k = 0; k < 3; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[k][d] = u[k][d];
-  for (int it = 0; it < HD_SETTLE_NEWTON_MAX; ++it) {
+  const int n_newton = a.dt != 0.f ? HD_SETTLE_NEWTON_MAX : HD_SETTLE_PROJECT_MAX;
+  for (int it = 0; it < n_newton; ++it) {
     double bond[3][3], res[3], worst = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -643,6 +647,65 @@ void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count) {
 }
 
 // ---------------------------------------------------------------------
+// Steepest descent (src/minimizer/steepest_descent.rs, kernels/minimize.cu):
+// the trial move x += F * (step / F_max) keeps the pre-move positions for a
+// restore; with a constraint frame it also writes the moved positions as
+// the frame the position-only projection takes its directions from.
+__global__ void k_sd_move(float4* __restrict__ posq, const float* __restrict__ fx,
+                          const float* __restrict__ fy, const float* __restrict__ fz,
+                          float4* __restrict__ save, float4* __restrict__ frame, float scale,
+                          unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = posq[i];
+  save[i] = p;
+  p.x = p.x + fx[i] * scale;
+  p.y = p.y + fy[i] * scale;
+  p.z = p.z + fz[i] * scale;
+  posq[i] = p;
+  if (frame) frame[i] = p;
+}
+
+__global__ void k_sd_restore(float4* __restrict__ posq, const float4* __restrict__ save,
+                             unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) posq[i] = save[i];
+}
+
+// max_i |F_i|^2 in FP64 by one workgroup (a max is exact in any order)
+__global__ void __launch_bounds__(1024) k_fmax2(const float* __restrict__ fx,
+                                                const float* __restrict__ fy,
+                                                const float* __restrict__ fz, unsigned n,
+                                                double* __restrict__ out) {
+  __shared__ double sh[1024];
+  double m = 0.0;
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    double x = fx[i], y = fy[i], z = fz[i];
+    m = fmax(m, x * x + y * y + z * z);
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (unsigned s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+void hd_launch_sd_move(hipStream_t st, float4* posq, const float* fx, const float* fy,
+                       const float* fz, float4* save, float4* frame, float scale, unsigned n) {
+  hipLaunchKernelGGL(k_sd_move, dim3((n + 255) / 256), dim3(256), 0, st, posq, fx, fy, fz, save,
+                     frame, scale, n);
+}
+void hd_launch_sd_restore(hipStream_t st, float4* posq, const float4* save, unsigned n) {
+  hipLaunchKernelGGL(k_sd_restore, dim3((n + 255) / 256), dim3(256), 0, st, posq, save, n);
+}
+void hd_launch_fmax2(hipStream_t st, const float* fx, const float* fy, const float* fz,
+                     unsigned n, double* out) {
+  hipLaunchKernelGGL(k_fmax2, dim3(1), dim3(1024), 0, st, fx, fy, fz, n, out);
+}
+
+// ---------------------------------------------------------------------
 // Force finalize: pair fixed point + SPME gather (+ constraint virial),
 // scattered back to atom order, optionally fused with the post-force kick.
 // ---------------------------------------------------------------------
@@ -858,6 +921,80 @@ __device__ __forceinline__ double philox_gaussian_f64(unsigned k0, unsigned k1, 
   const double sc = 1.0 / 4294967296.0;
   double u1 = ((double)o[0] + 0.5) * sc, u2 = ((double)o[1] + 0.5) * sc;
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// ---------------------------------------------------------------------
+// CSVR: K_new = c K + (K_target/g)(1 - c)(sum_{i=1}^{g-1} xi_i^2 + r^2)
+//   + 2 r sqrt(c (1 - c) K K_target / g)           (Bussi et al. 2007, A7)
+// with xi_i the Philox normal (seed, counter, i, 0) and r that of index 0
+// (kernels/nose_hoover.cu:145-255): the g - 1 squares are summed in
+// HD_CSVR_BLOCKS contiguous chunks (one block each, fixed tree), the chunks
+// by the final kernel, so the sum is the same on every run.
+__global__ void __launch_bounds__(256) k_csvr_partials(HdRescaleArgs t) {
+  __shared__ double sh[256];
+  unsigned long long ctr = *t.counter;
+  unsigned clo = (unsigned)(ctr & 0xFFFFFFFFull), chi = (unsigned)(ctr >> 32);
+  unsigned m = t.g_dof > 0u ? t.g_dof - 1u : 0u;
+  unsigned chunk = (m + gridDim.x - 1u) / gridDim.x;
+  unsigned start = blockIdx.x * chunk, end = min(start + chunk, m);
+  double s = 0.0;
+  for (unsigned j = start + threadIdx.x; j < end; j += blockDim.x) {
+    double xi = philox_gaussian_f64(t.seed_lo, t.seed_hi, clo, chi, j + 1u, 0u);
+    s += xi * xi;
+  }
+  double tot = block_tree_sum(s, sh);
+  if (threadIdx.x == 0) t.partials[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_ke_final_rescale(const double* __restrict__ partials,
+                                                          unsigned np, double* __restrict__ ke,
+                                                          HdRescaleArgs t,
+                                                          float* __restrict__ factor) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+  double k = block_tree_sum(s, sh);
+  double sx = 0.0;
+  if (t.kind == 2) {
+    __syncthreads();
+    double a = 0.0;
+    for (unsigned i = threadIdx.x; i < HD_CSVR_BLOCKS; i += blockDim.x) a += t.partials[i];
+    sx = block_tree_sum(a, sh);
+  }
+  if (threadIdx.x != 0) return;
+  ke[0] = k;
+  double f = 1.0;
+  if (t.kind == 2) {
+    unsigned long long ctr = *t.counter;
+    double r = philox_gaussian_f64(t.seed_lo, t.seed_hi, (unsigned)(ctr & 0xFFFFFFFFull),
+                                   (unsigned)(ctr >> 32), 0u, 0u);
+    double cross = k > 0.0 ? 2.0 * r * sqrt(t.c * t.one_minus_c * k * t.kt_per_dof) : 0.0;
+    double kn = t.c * k + t.kt_per_dof * t.one_minus_c * (sx + r * r) + cross;
+    if (!isfinite(kn) || kn <= 0.0) kn = k;
+    if (k > 0.0 && kn != k) f = sqrt(kn / k);
+    t.injection[0] += kn - k;
+    *t.counter = ctr + 1ull;
+  } else if (k > 0.0) {
+    double l2 = 1.0 + t.dt_over_tau * (t.k_target / k - 1.0);
+    if (!(l2 > 0.0)) l2 = 0.0;
+    f = sqrt(l2);
+    t.injection[0] += k * (l2 - 1.0);
+  }
+  factor[0] = (float)f;
+}
+
+void hd_launch_ke_rescale(hipStream_t st, unsigned n, const float* vx, const float* vy,
+                          const float* vz, const float* mass, double* partials, double* ke,
+                          const HdRescaleArgs& t, float* factor, const HdDispArgs* disp) {
+  HdDispArgs d;
+  memset(&d, 0, sizeof(d));
+  if (disp) d = *disp;
+  hipLaunchKernelGGL(k_ke_partials, dim3(HD_REDUCE_BLOCKS), dim3(256), 0, st, n, vx, vy, vz,
+                     mass, partials, d);
+  if (t.kind == 2)
+    hipLaunchKernelGGL(k_csvr_partials, dim3(HD_CSVR_BLOCKS), dim3(256), 0, st, t);
+  hipLaunchKernelGGL(k_ke_final_rescale, dim3(1), dim3(256), 0, st, partials,
+                     (unsigned)HD_REDUCE_BLOCKS, ke, t, factor);
 }
 
 // ---------------------------------------------------------------------
@@ -1303,10 +1440,14 @@ __device__ void crescale_mu_update(double k, double w, float* lattice, float* mu
   double lx = lattice[0], ly = lattice[1], lz = lattice[2];
   double v_pre = lx * ly * lz;
   double p = (2.0 * k + w) / (3.0 * v_pre);
-  double r = philox_gaussian_f64(b.seed_lo, b.seed_hi, (unsigned)(ctr & 0xFFFFFFFFull),
-                                 (unsigned)(ctr >> 32), 0u, 0u);
+  // Berendsen (barostat.cu:228-275): the same step without the noise term,
+  // the draw or the P0 dV account
+  double r = b.stochastic ? philox_gaussian_f64(b.seed_lo, b.seed_hi,
+                                                (unsigned)(ctr & 0xFFFFFFFFull),
+                                                (unsigned)(ctr >> 32), 0u, 0u)
+                          : 0.0;
   double det = -b.compressibility * (b.dt / b.tau) * (b.pressure - p);
-  double amp = sqrt(2.0 * b.compressibility * b.kt * b.dt / (b.tau * v_pre));
+  double amp = b.stochastic ? sqrt(2.0 * b.compressibility * b.kt * b.dt / (b.tau * v_pre)) : 0.0;
   double mu3 = 1.0 + det + amp * r;
   double mu3c = mu3 > b.mu3_min ? mu3 : b.mu3_min;
   double mu = cbrt(mu3c);
@@ -1316,8 +1457,10 @@ __device__ void crescale_mu_update(double k, double w, float* lattice, float* mu
   for (int i = 0; i < 6; ++i) lattice[i] *= muf;
   diag[0] = p;
   diag[1] = v_post;
-  diag[2] += b.pressure * (v_post - v_pre);
-  *counter = ctr + 1ull;
+  if (b.stochastic) {
+    diag[2] += b.pressure * (v_post - v_pre);
+    *counter = ctr + 1ull;
+  }
 }
 
 // k_sum2_final's reduction, then its thread 0 runs the C-rescale update

@@ -674,8 +674,8 @@ __global__ void __launch_bounds__(256) k_pair_alist(
 // j-side fixed-point accumulators at 2^48 (integer sums, so the total is
 // independent of the order in which angles land).
 __global__ void k_angles(const float4* __restrict__ posq, const float* __restrict__ lattice,
-                         const unsigned* __restrict__ ijk, const float* __restrict__ kth,
-                         const float* __restrict__ th0, const unsigned* __restrict__ slot_of,
+                         const unsigned* __restrict__ ijk, const double* __restrict__ kth,
+                         const double* __restrict__ th0, const unsigned* __restrict__ slot_of,
                          unsigned long long* __restrict__ fpj,
                          unsigned long long* __restrict__ fpj_ew, unsigned n_angles, int ev) {
   unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -703,8 +703,8 @@ __global__ void k_angles(const float4* __restrict__ posq, const float* __restric
     double dt_l = ui[0] * uk[0] + ui[1] * uk[1] + ui[2] * uk[2];        // |ui||uk| cos
     if (sn_l > 1e-12 * li * lk) {
       double theta = atan2(sn_l, dt_l);
-      double dth = theta - (double)th0[m];
-      double kf = (double)kth[m];
+      double dth = theta - th0[m];
+      double kf = kth[m];
       // dtheta/dx_i = (cos ui/|ui|^2 - uk/(|ui||uk|)) / sin, and for k
       // alike; F = -k dth dtheta/dx
       double cs = dt_l / (li * lk), sn = sn_l / (li * lk);
@@ -744,13 +744,238 @@ __global__ void k_angles(const float4* __restrict__ posq, const float* __restric
 }
 
 void hd_launch_angles(hipStream_t st, const float4* posq, const float* lattice,
-                      const unsigned* ijk, const float* kth, const float* th0,
+                      const unsigned* ijk, const double* kth, const double* th0,
                       const unsigned* slot_of, long long* fpj, long long* fpj_ew,
                       unsigned n_angles, bool ev) {
   if (n_angles == 0) return;
   hipLaunchKernelGGL(k_angles, dim3((n_angles + 255) / 256), dim3(256), 0, st, posq, lattice,
                      ijk, kth, th0, slot_of, (unsigned long long*)fpj,
                      (unsigned long long*)fpj_ew, n_angles, ev ? 1 : 0);
+}
+
+// Per-atom scatter of a bonded term's force, energy share and virial share
+// into the j-side fixed-point accumulators (slot order).
+__device__ __forceinline__ void bonded_scatter(unsigned long long* __restrict__ fpj,
+                                               unsigned long long* __restrict__ fpj_ew,
+                                               unsigned slot, double fx, double fy, double fz,
+                                               long long e_share, long long w_share, int ev) {
+  const float f3[3] = {(float)fx, (float)fy, (float)fz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    long long v = hd_to_fixed48(f3[d]);
+    if (v) atomicAdd(fpj + (size_t)slot * 4 + d, (unsigned long long)v);
+  }
+  if (ev) {
+    if (e_share) atomicAdd(fpj_ew + (size_t)slot * 2, (unsigned long long)e_share);
+    if (w_share) atomicAdd(fpj_ew + (size_t)slot * 2 + 1, (unsigned long long)w_share);
+  }
+}
+
+__device__ __forceinline__ void disp_f64(const float4& a, const float4& b, const HdLattice& L,
+                                         double d[3]) {
+  d[0] = (double)a.x - (double)b.x;
+  d[1] = (double)a.y - (double)b.y;
+  d[2] = (double)a.z - (double)b.z;
+  hd_min_image_f64(d[0], d[1], d[2], L);
+}
+
+// Two-body bonded terms in FP64 (src/forces/harmonic_bond.rs:288-330,
+// src/forces/morse.rs:290-320): U = k/2 (r - r0)^2, or
+// U = De (1 - exp(-a (r - re)))^2; F_i = -dU/dr (r_i - r_j)/r, F_j = -F_i;
+// W = F_i . (r_i - r_j); energy and virial shared half and half.
+__global__ void k_bonds(const float4* __restrict__ posq, const float* __restrict__ lattice,
+                        const unsigned* __restrict__ ij, const int* __restrict__ kind,
+                        const double* __restrict__ prm, const unsigned* __restrict__ slot_of,
+                        unsigned long long* __restrict__ fpj,
+                        unsigned long long* __restrict__ fpj_ew, unsigned n_bonds, int ev) {
+  unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_bonds) return;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned ai = ij[2 * m], aj = ij[2 * m + 1];
+  double d[3];
+  disp_f64(posq[ai], posq[aj], L, d);
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double r = sqrt(r2);
+  double dudr = 0.0, u = 0.0;
+  const double* p = prm + 3 * (size_t)m;
+  if (r > 1e-7) {
+    if (kind[m] == 0) {
+      double dr = r - p[1];
+      dudr = p[0] * dr;
+      u = 0.5 * p[0] * dr * dr;
+    } else {
+      double e = exp(-p[1] * (r - p[2]));
+      dudr = 2.0 * p[0] * p[1] * (1.0 - e) * e;
+      u = p[0] * (1.0 - e) * (1.0 - e);
+    }
+  }
+  double g = r > 1e-7 ? -dudr / r : 0.0;
+  long long e2 = 0, w2 = 0;
+  if (ev) {
+    e2 = hd_to_fixed48((float)(0.5 * u));
+    w2 = hd_to_fixed48((float)(0.5 * g * r2));
+  }
+  bonded_scatter(fpj, fpj_ew, slot_of[ai], g * d[0], g * d[1], g * d[2], e2, w2, ev);
+  bonded_scatter(fpj, fpj_ew, slot_of[aj], -g * d[0], -g * d[1], -g * d[2], e2, w2, ev);
+}
+
+void hd_launch_bonds(hipStream_t st, const float4* posq, const float* lattice, const unsigned* ij,
+                     const int* kind, const double* prm, const unsigned* slot_of, long long* fpj,
+                     long long* fpj_ew, unsigned n_bonds, bool ev) {
+  if (n_bonds == 0) return;
+  hipLaunchKernelGGL(k_bonds, dim3((n_bonds + 255) / 256), dim3(256), 0, st, posq, lattice, ij,
+                     kind, prm, slot_of, (unsigned long long*)fpj, (unsigned long long*)fpj_ew,
+                     n_bonds, ev ? 1 : 0);
+}
+
+// Periodic dihedral U = k (1 + cos(n phi - phi0)) (src/forces/dihedral.rs:23,
+// 360-460) in FP64.  With F = r_i - r_j, G = r_j - r_k, H = r_l - r_k,
+// A = F x G and B = H x G: cos phi = A.B / (|A||B|) and, since B.G = 0,
+// sin phi = (B x A).G / (|A||B||G|) = -|G| F.B / (|A||B|), the same angle
+// and sign as the reference's atan2(|b2| b1.n, m.n); the gradients are
+// those of Blondel and Karplus (J. Comput. Chem. 17, 1132 (1996)):
+//   dphi/dr_i = -|G|/|A|^2 A,  dphi/dr_l = |G|/|B|^2 B,
+//   dphi/dr_j = |G|/|A|^2 A + (F.G)/(|A|^2 |G|) A - (H.G)/(|B|^2 |G|) B,
+//   dphi/dr_k = -|G|/|B|^2 B - (F.G)/(|A|^2 |G|) A + (H.G)/(|B|^2 |G|) B.
+// W = sum_a (r_a - r_j) . F_a; energy and virial shared in quarters.
+__global__ void k_dihedrals(const float4* __restrict__ posq, const float* __restrict__ lattice,
+                            const unsigned* __restrict__ ijkl, const double* __restrict__ prm,
+                            const unsigned* __restrict__ slot_of,
+                            unsigned long long* __restrict__ fpj,
+                            unsigned long long* __restrict__ fpj_ew, unsigned n_dih, int ev) {
+  unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_dih) return;
+  HdLattice L = hd_load_lattice(lattice);
+  unsigned a0 = ijkl[4 * m], a1 = ijkl[4 * m + 1], a2 = ijkl[4 * m + 2], a3 = ijkl[4 * m + 3];
+  float4 p0 = posq[a0], p1 = posq[a1], p2 = posq[a2], p3 = posq[a3];
+  double F[3], G[3], H[3];
+  disp_f64(p0, p1, L, F);
+  disp_f64(p1, p2, L, G);
+  disp_f64(p3, p2, L, H);
+  double A[3] = {F[1] * G[2] - F[2] * G[1], F[2] * G[0] - F[0] * G[2], F[0] * G[1] - F[1] * G[0]};
+  double B[3] = {H[1] * G[2] - H[2] * G[1], H[2] * G[0] - H[0] * G[2], H[0] * G[1] - H[1] * G[0]};
+  double a2n = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  double b2n = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+  double g2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+  double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0}, fk[3] = {0, 0, 0}, fl[3] = {0, 0, 0};
+  double u = 0.0, w = 0.0;
+  if (a2n >= 1e-14 && b2n >= 1e-14 && g2 >= 1e-14) {
+    double gl = sqrt(g2);
+    double fb = F[0] * B[0] + F[1] * B[1] + F[2] * B[2];
+    double ab = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+    double phi = atan2(-gl * fb, ab);
+    const double* p = prm + 3 * (size_t)m;
+    double k = p[0], nm = p[1], phi0 = p[2];
+    double delta = nm * phi - phi0;
+    double sd, cd;
+    sincos(delta, &sd, &cd);
+    u = k * (1.0 + cd);
+    double mdu = k * nm * sd;  // -dU/dphi
+    double fg = F[0] * G[0] + F[1] * G[1] + F[2] * G[2];
+    double hg = H[0] * G[0] + H[1] * G[1] + H[2] * G[2];
+    double ca = gl / a2n, cb = gl / b2n;
+    double sa = fg / (a2n * gl), sb = hg / (b2n * gl);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double gi = -ca * A[d], gl_ = cb * B[d];
+      double gj = ca * A[d] + sa * A[d] - sb * B[d];
+      double gk = -cb * B[d] - sa * A[d] + sb * B[d];
+      fi[d] = mdu * gi;
+      fj[d] = mdu * gj;
+      fk[d] = mdu * gk;
+      fl[d] = mdu * gl_;
+    }
+    // positions relative to r_j: r_i - r_j = F, r_k - r_j = -G, r_l - r_j = H - G
+    for (int d = 0; d < 3; ++d) w += F[d] * fi[d] - G[d] * fk[d] + (H[d] - G[d]) * fl[d];
+  }
+  long long e4 = 0, w4 = 0;
+  if (ev) {
+    e4 = hd_to_fixed48((float)(0.25 * u));
+    w4 = hd_to_fixed48((float)(0.25 * w));
+  }
+  bonded_scatter(fpj, fpj_ew, slot_of[a0], fi[0], fi[1], fi[2], e4, w4, ev);
+  bonded_scatter(fpj, fpj_ew, slot_of[a1], fj[0], fj[1], fj[2], e4, w4, ev);
+  bonded_scatter(fpj, fpj_ew, slot_of[a2], fk[0], fk[1], fk[2], e4, w4, ev);
+  bonded_scatter(fpj, fpj_ew, slot_of[a3], fl[0], fl[1], fl[2], e4, w4, ev);
+}
+
+void hd_launch_dihedrals(hipStream_t st, const float4* posq, const float* lattice,
+                         const unsigned* ijkl, const double* prm, const unsigned* slot_of,
+                         long long* fpj, long long* fpj_ew, unsigned n_dih, bool ev) {
+  if (n_dih == 0) return;
+  hipLaunchKernelGGL(k_dihedrals, dim3((n_dih + 255) / 256), dim3(256), 0, st, posq, lattice,
+                     ijkl, prm, slot_of, (unsigned long long*)fpj, (unsigned long long*)fpj_ew,
+                     n_dih, ev ? 1 : 0);
+}
+
+// Scaled exclusions (src/forces/jit_composed.rs:1445-1458 with the LJ and
+// SPME-real fragments' own scale tables, lj.rs:295-321, spme.rs:723-762):
+// the pair kernel applies weight 1 to every pair within the cutoff whose
+// keys differ by more than the span and 0 to the rest; this pass adds
+// (scale - that weight) times each term, evaluated in FP64 with the same
+// Hastings erfc and cubic LJ switch, so the pair ends at its scale.
+__global__ void k_excl_scaled(const float4* __restrict__ posq, const float* __restrict__ lattice,
+                              const HdExclPair* __restrict__ pairs, unsigned n_pairs, double kc,
+                              double alpha, double coul_rc2, const unsigned* __restrict__ slot_of,
+                              unsigned long long* __restrict__ fpj,
+                              unsigned long long* __restrict__ fpj_ew, int ev) {
+  unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_pairs) return;
+  HdLattice L = hd_load_lattice(lattice);
+  HdExclPair P = pairs[m];
+  float4 pi = posq[P.i], pj = posq[P.j];
+  double d[3];
+  disp_f64(pi, pj, L, d);
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double factor = 0.0, energy = 0.0;
+  if (r2 > 0.0) {
+    double inv_r2 = 1.0 / r2, inv_r = sqrt(inv_r2), r = r2 * inv_r;
+    if (P.w_lj != 0.0 && r2 <= P.cut2 && P.eps != 0.0) {
+      double sr2 = P.sigma * P.sigma * inv_r2;
+      double sr6 = sr2 * sr2 * sr2;
+      double f = 24.0 * P.eps * inv_r2 * (2.0 * sr6 * sr6 - sr6);
+      double e = 4.0 * P.eps * (sr6 * sr6 - sr6);
+      double delta = P.cut2 - P.rsw2;
+      if (delta > 0.0 && r2 > P.rsw2) {
+        double tau = (r2 - P.rsw2) / delta, omt = 1.0 - tau;
+        double s = omt * omt * (1.0 + 2.0 * tau);
+        f = s * f + 12.0 * tau * omt / delta * e;
+        e = s * e;
+      }
+      factor += P.w_lj * f;
+      energy += P.w_lj * e;
+    }
+    if (P.w_coul != 0.0 && coul_rc2 > 0.0 && r2 <= coul_rc2) {
+      double qq = kc * (double)pi.w * (double)pj.w;
+      double ar = alpha * r;
+      double gauss = exp(-ar * ar);
+      double t = 1.0 / (1.0 + 0.3275911 * ar);
+      double ec = (0.254829592 +
+                   (-0.284496736 + (1.421413741 + (-1.453152027 + 1.061405429 * t) * t) * t) * t) *
+                  t * gauss;
+      factor += P.w_coul * qq * inv_r2 * (ec * inv_r + 2.0 * alpha * 0.5641895835477563 * gauss);
+      energy += P.w_coul * qq * ec * inv_r;
+    }
+  }
+  long long e2 = 0, w2 = 0;
+  if (ev) {
+    e2 = hd_to_fixed48((float)(0.5 * energy));
+    w2 = hd_to_fixed48((float)(0.5 * factor * r2));
+  }
+  bonded_scatter(fpj, fpj_ew, slot_of[P.i], factor * d[0], factor * d[1], factor * d[2], e2, w2,
+                 ev);
+  bonded_scatter(fpj, fpj_ew, slot_of[P.j], -factor * d[0], -factor * d[1], -factor * d[2], e2,
+                 w2, ev);
+}
+
+void hd_launch_excl_scaled(hipStream_t st, const float4* posq, const float* lattice,
+                           const HdExclPair* pairs, unsigned n_pairs, double kc, double alpha,
+                           double coul_rc2, const unsigned* slot_of, long long* fpj,
+                           long long* fpj_ew, bool ev) {
+  if (n_pairs == 0) return;
+  hipLaunchKernelGGL(k_excl_scaled, dim3((n_pairs + 255) / 256), dim3(256), 0, st, posq, lattice,
+                     pairs, n_pairs, kc, alpha, coul_rc2, slot_of, (unsigned long long*)fpj,
+                     (unsigned long long*)fpj_ew, ev ? 1 : 0);
 }
 
 // Diagnostic: number of distinct pairs of the current list with

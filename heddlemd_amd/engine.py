@@ -102,6 +102,38 @@ class Engine:
                                             float(compressibility), float(kt), int(seed)),
                   "hd_set_crescale")
 
+    def set_csvr(self, kt, tau, seed, n_constraints=0):
+        self._chk(self._lib.hd_set_csvr(self._ctx, float(kt), float(tau), int(seed),
+                                        int(n_constraints)), "hd_set_csvr")
+
+    def set_berendsen(self, kt, tau, n_constraints=0):
+        self._chk(self._lib.hd_set_berendsen(self._ctx, float(kt), float(tau),
+                                             int(n_constraints)), "hd_set_berendsen")
+
+    def set_berendsen_barostat(self, pressure, tau, compressibility):
+        self._chk(self._lib.hd_set_berendsen_barostat(self._ctx, float(pressure), float(tau),
+                                                      float(compressibility)),
+                  "hd_set_berendsen_barostat")
+
+    def thermostat(self):
+        """CSVR / Berendsen: injected energy and draw counter."""
+        out = np.zeros(2)
+        self._chk(self._lib.hd_get_thermostat(self._ctx, out), "hd_get_thermostat")
+        return dict(injection=float(out[0]), draws=int(out[1]))
+
+    def minimize_begin(self, initial_step):
+        """Steepest descent start: dict(energy, max_force, step, accepted)."""
+        out = np.zeros(4)
+        self._chk(self._lib.hd_minimize_begin(self._ctx, float(initial_step), out),
+                  "hd_minimize_begin")
+        return dict(energy=out[0], max_force=out[1], step=out[2], accepted=bool(out[3]))
+
+    def minimize_step(self, max_step, step_increase=1.2, step_decrease=0.2):
+        out = np.zeros(4)
+        self._chk(self._lib.hd_minimize_step(self._ctx, float(max_step), float(step_increase),
+                                             float(step_decrease), out), "hd_minimize_step")
+        return dict(energy=out[0], max_force=out[1], step=out[2], accepted=bool(out[3]))
+
     def set_spme_dft(self, mode):
         """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT."""
         self._chk(self._lib.hd_set_spme_dft(self._ctx, int(mode)), "hd_set_spme_dft")
@@ -113,6 +145,39 @@ class Engine:
                                           np.ascontiguousarray(k_theta, np.float64),
                                           np.ascontiguousarray(theta0, np.float64)),
                   "hd_set_angles")
+
+    def set_bonds(self, ij, kind, prm):
+        """kind 0 harmonic (k, r0, -), 1 Morse (De, a, re) per bond."""
+        ij = np.ascontiguousarray(ij, np.uint32).reshape(-1)
+        kind = np.ascontiguousarray(kind, np.int32).reshape(-1)
+        prm = np.ascontiguousarray(prm, np.float64).reshape(-1)
+        if ij.size != 2 * kind.size or prm.size != 3 * kind.size:
+            raise ValueError("bonds: ij needs 2 and prm 3 entries per bond")
+        self._chk(self._lib.hd_set_bonds(self._ctx, kind.size, ij,
+                                         kind.ctypes.data_as(C.POINTER(C.c_int)), prm),
+                  "hd_set_bonds")
+
+    def set_dihedrals(self, ijkl, prm):
+        """Periodic dihedrals; prm rows (k, n, phi0)."""
+        ijkl = np.ascontiguousarray(ijkl, np.uint32).reshape(-1)
+        prm = np.asarray(prm, np.float64).reshape(-1, 3)
+        if ijkl.size != 4 * len(prm):
+            raise ValueError("dihedrals: ijkl needs 4 entries per dihedral")
+        self._chk(self._lib.hd_set_dihedrals(self._ctx, len(prm), ijkl,
+                                             np.ascontiguousarray(prm[:, 0]),
+                                             np.ascontiguousarray(prm[:, 1]),
+                                             np.ascontiguousarray(prm[:, 2])),
+                  "hd_set_dihedrals")
+
+    def set_exclusions(self, ij, scale_lj, scale_coul):
+        """The effective exclusion list: pairs (n, 2) with per-term scales."""
+        ij = np.ascontiguousarray(ij, np.uint32).reshape(-1)
+        slj = np.ascontiguousarray(scale_lj, np.float64).reshape(-1)
+        sc = np.ascontiguousarray(scale_coul, np.float64).reshape(-1)
+        if ij.size != 2 * slj.size or sc.size != slj.size:
+            raise ValueError("exclusions: 2 atoms and 2 scales per pair")
+        self._chk(self._lib.hd_set_exclusions(self._ctx, slj.size, ij, slj, sc),
+                  "hd_set_exclusions")
 
     def set_exclusion_span(self, span):
         self._chk(self._lib.hd_set_exclusion_span(self._ctx, int(span)), "hd_set_exclusion_span")
@@ -338,7 +403,8 @@ class Engine:
 
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
-               velocities=None, crescale=None, langevin=None, device=0, pair_list=None):
+               velocities=None, crescale=None, langevin=None, device=0, pair_list=None,
+               csvr=None, berendsen=None, berendsen_barostat=None):
     """Configure an Engine for a ``systems.System``; ``langevin`` =
     dict(friction, kt, seed) selects the BAOAB integrator; ``pair_list``
     selects the neighbour-list form (hd_set_pair_list)."""
@@ -359,6 +425,12 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_angles(ijk, kth, th0)
     if getattr(system, "excl_span", 0):
         e.set_exclusion_span(system.excl_span)
+    if getattr(system, "bonds", None) is not None:
+        e.set_bonds(*system.bonds)
+    if getattr(system, "dihedrals", None) is not None:
+        e.set_dihedrals(*system.dihedrals)
+    if getattr(system, "exclusions", None) is not None:
+        e.set_exclusions(*system.exclusions)
     if system.water_groups is not None:
         d_oh, d_hh = system.water_geom
         m_o = float(system.mass[system.water_groups[0, 0]])
@@ -370,6 +442,12 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_langevin(**langevin)
     if nhc_kt is not None:
         e.set_nhc(nhc_kt, nhc_tau, n_constraints=system.n_constraints)
+    if csvr is not None:
+        e.set_csvr(n_constraints=system.n_constraints, **csvr)
+    if berendsen is not None:
+        e.set_berendsen(n_constraints=system.n_constraints, **berendsen)
+    if berendsen_barostat is not None:
+        e.set_berendsen_barostat(**berendsen_barostat)
     if crescale is not None:
         e.set_crescale(**crescale)
     if velocities is not None:

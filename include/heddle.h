@@ -67,6 +67,28 @@ int hd_set_angles(hd_ctx *ctx, uint64_t n_angles, const uint32_t *ijk, const dou
  * exclusions of linear chains (the CSR lists of src/forces/topology.rs
  * for these topologies).  Before hd_setup. */
 int hd_set_exclusion_span(hd_ctx *ctx, uint32_t span);
+/* Two-body bonded terms (src/forces/harmonic_bond.rs, src/forces/morse.rs;
+ * the [bonds] section of src/forces/topology.rs): atoms ij[2b], ij[2b+1];
+ * kind[b] 0 harmonic U = k/2 (r - r0)^2 with prm[3b..] = (k, r0, unused),
+ * 1 Morse U = De (1 - exp(-a (r - re)))^2 with prm[3b..] = (De, a, re).
+ * FP64 arithmetic.  Before hd_setup. */
+int hd_set_bonds(hd_ctx *ctx, uint64_t n_bonds, const uint32_t *ij, const int *kind,
+                 const double *prm);
+/* Periodic dihedrals U = k (1 + cos(n phi - phi0)) (src/forces/dihedral.rs:23,
+ * 360-460): atoms ijkl[4d..4d+3], IUPAC angle (trans = pi).  FP64.  Before
+ * hd_setup. */
+int hd_set_dihedrals(hd_ctx *ctx, uint64_t n_dihedrals, const uint32_t *ijkl,
+                     const double *k_phi, const double *n_mult, const double *phi0);
+/* The effective exclusion list of src/forces/topology.rs:1184-1290 (explicit
+ * rows, then bond, angle, constraint-group and scaled 1-4 pairs): pair p is
+ * (ij[2p], ij[2p+1]) with LJ scale s_lj[p] and real-space Coulomb scale
+ * s_coul[p] in [0, 1] (lj.rs:295-321, spme.rs:723-762; unlisted pairs have
+ * scale 1).  Replaces the keys of hd_set_system and the span: the fully
+ * excluded pairs must be whole molecules (cliques) or runs of consecutive
+ * atoms excluded up to a fixed index distance, else HD_ERR_UNSUPPORTED;
+ * scaled pairs are any.  After hd_set_system, before hd_setup. */
+int hd_set_exclusions(hd_ctx *ctx, uint64_t n_pairs, const uint32_t *ij, const double *s_lj,
+                      const double *s_coul);
 /* Box (src/pbc.rs SimulationBox lattice). */
 int hd_set_lattice(hd_ctx *ctx, const double lattice[6]);
 int hd_set_velocities(hd_ctx *ctx, const float *vx, const float *vy, const float *vz);
@@ -128,6 +150,21 @@ int hd_set_integrator(hd_ctx *ctx, double dt, int lossless);
  * kt <= 0 removes it.  n_constraints feeds the thermal DOF 3N - n_c - 3. */
 int hd_set_nhc(hd_ctx *ctx, double kt, double tau, int chain, int yoshida, int n_resp,
                uint64_t n_constraints);
+/* CSVR stochastic velocity rescaling (src/integrator/csvr.rs,
+ * kernels/nose_hoover.cu:145-255): after the drift, K_new from K, the
+ * relaxation c = exp(-dt/tau) and Philox normals (key seed, counter = the
+ * device draw counter, one per step) sums the g - 1 squares of draw
+ * indices 1..g-1 and the normal of index 0, g = 3N - n_constraints - 3;
+ * the post-force half kick is scaled by sqrt(K_new / K).  One thermostat
+ * per context; after hd_setup only re-parameterisable. */
+int hd_set_csvr(hd_ctx *ctx, double kt, double tau, uint64_t seed, uint64_t n_constraints);
+/* Berendsen weak-coupling thermostat (src/integrator/berendsen.rs,
+ * nose_hoover.cu:390-417): factor sqrt(max(0, 1 + dt/tau (K_target/K - 1)))
+ * on the post-force half kick, K_target = g kT / 2. */
+int hd_set_berendsen(hd_ctx *ctx, double kt, double tau, uint64_t n_constraints);
+/* CSVR / Berendsen state: out[0] cumulative energy the rescales injected
+ * (the conserved quantity is K + U - out[0]), out[1] CSVR draw counter. */
+int hd_get_thermostat(hd_ctx *ctx, double out[2]);
 
 /* Langevin BAOAB integrator (src/integrator/langevin_baoab.rs,
  * kernels/langevin.cu): friction gamma (1/time), k_B*T and the Philox seed
@@ -145,6 +182,11 @@ int hd_get_langevin_counter(hd_ctx *ctx, uint64_t *counter);
  * seed.  tau <= 0 removes it.  Forces scalars (virial) on every step. */
 int hd_set_crescale(hd_ctx *ctx, double pressure, double tau, double compressibility, double kt,
                     uint64_t seed);
+/* Berendsen barostat (src/integrator/berendsen_barostat.rs,
+ * kernels/barostat.cu:228-275): the C-rescale step without the noise term
+ * (mu^3 = 1 - kappa dt/tau (P0 - P)), no draw and no P0 dV account.
+ * Replaces a C-rescale barostat; tau <= 0 removes it. */
+int hd_set_berendsen_barostat(hd_ctx *ctx, double pressure, double tau, double compressibility);
 
 /* Allocate device state and build the first neighbour list; must follow
  * the hd_set_* calls and precede any evaluation. */
@@ -153,6 +195,20 @@ int hd_setup(hd_ctx *ctx);
  * src/forces/mod.rs:662-757); want_scalars also fills per-atom energy and
  * virial and the totals. */
 int hd_force_eval(hd_ctx *ctx, int want_scalars);
+/* Steepest-descent minimizer (src/minimizer/steepest_descent.rs,
+ * kernels/minimize.cu).  hd_minimize_begin evaluates the forces and the
+ * potential energy (SteepestDescentMinimizer::initial_state) and sets the
+ * step length; each hd_minimize_step (::step) moves x += F step / F_max,
+ * projects SETTLE / SHAKE groups back onto their constraints (positions
+ * only, directions at the moved positions), evaluates the trial energy and
+ * keeps the move if it is lower (step *= step_increase, at most max_step)
+ * or restores the positions (step *= step_decrease).  out[0] energy after
+ * the call, out[1] max |F_i| of the current forces, out[2] step length
+ * used, out[3] 1 accepted / 0 rejected.  Velocities are not touched;
+ * convergence tests are the caller's (check_convergence). */
+int hd_minimize_begin(hd_ctx *ctx, double initial_step, double out[4]);
+int hd_minimize_step(hd_ctx *ctx, double max_step, double step_increase, double step_decrease,
+                     double out[4]);
 /* Replace the positions (and charges) after hd_setup: 4n floats x, y, z, q
  * (the position upload of src/gpu/buffers.rs ParticleBuffers).  The next
  * evaluation recomputes the forces; the neighbour list is rebuilt when the
@@ -289,7 +345,8 @@ int hd_get_lattice(hd_ctx *ctx, double lattice[6]);
 /* Scalars of the last scalar evaluation, FP64:
  * out[0] kinetic energy, out[1] potential energy, out[2] virial,
  * out[3] E_recip (without self term), out[4] W_recip,
- * out[5] Nose-Hoover chain term of the conserved quantity,
+ * out[5] thermostat term of the conserved quantity (Nose-Hoover chain
+ * energy, or minus the energy CSVR / Berendsen rescales injected),
  * out[6] neighbour rebuild count, out[7] neighbour status word. */
 int hd_get_scalars(hd_ctx *ctx, double out[8]);
 /* Canonical pair set of the current packed list: unique (min id, max id)

@@ -247,17 +247,20 @@ typedef struct {
     double kc, alpha, rc_coul;             /* rc_coul <= 0 disables Coulomb */
     const double *q;
     const uint32_t *ex_off, *ex_partner;   /* CSR exclusions (may be NULL) */
-    const double *ex_scale;
+    const double *ex_scale;                /* LJ scale per CSR entry */
+    const double *ex_scale_coul;           /* real-space Coulomb scale per entry */
     double band;
     double *F, *E, *W;
     uint8_t *band_flag;
 } pairforce_ctx;
 
-static double excl_scale(const pairforce_ctx *c, int i, int j)
+/* heddle_jit_exclusion_scale (src/forces/jit_composed.rs:1445-1458): a
+ * linear scan of i's CSR row; unlisted pairs have scale 1. */
+static double excl_scale_of(const pairforce_ctx *c, const double *tab, int i, int j)
 {
     if (!c->ex_off) return 1.0;
     for (uint32_t m = c->ex_off[i]; m < c->ex_off[i + 1]; ++m)
-        if ((int)c->ex_partner[m] == j) return c->ex_scale[m];
+        if ((int)c->ex_partner[m] == j) return tab[m];
     return 1.0;
 }
 
@@ -268,7 +271,8 @@ static void pairforce_visit(int i, int j, double dx, double dy, double dz,
     double factor = 0.0, energy = 0.0;
     double inv_r = 1.0 / sqrt(r2);
     double r = r2 * inv_r;
-    double sc = excl_scale(c, i, j);
+    double sc = excl_scale_of(c, c->ex_scale, i, j);
+    double sc_coul = excl_scale_of(c, c->ex_scale_coul, i, j);
     /* LJ fragment, per-type-pair cutoff and optional cubic switch */
     {
         int p = (int)(c->type[i] * (uint32_t)c->ntypes + c->type[j]);
@@ -323,8 +327,8 @@ static void pairforce_visit(int i, int j, double dx, double dy, double dz,
             double e = c->kc * qq * ec * inv_r;
             double f = c->kc * qq * inv_r * inv_r *
                        (ec * inv_r + 2.0 * c->alpha * 0.5641895835477563 * gauss);
-            factor += f * sc;
-            energy += e * sc;
+            factor += f * sc_coul;
+            energy += e * sc_coul;
         }
     }
     double virial = factor * r2;
@@ -348,8 +352,8 @@ void or_pair_forces(int n, const float *posq, const uint32_t *type, int ntypes,
                     const double *sigma, const double *eps, const double *cut,
                     const double *rsw, double kc, double alpha, double rc_coul,
                     const uint32_t *ex_off, const uint32_t *ex_partner,
-                    const double *ex_scale, const double *L, double band,
-                    double *F, double *E, double *W, uint8_t *band_flag)
+                    const double *ex_scale, const double *ex_scale_coul, const double *L,
+                    double band, double *F, double *E, double *W, uint8_t *band_flag)
 {
     double *x = (double *)malloc(sizeof(double) * 4 * (n > 0 ? n : 1));
     double *y = x + n, *z = y + n, *q = z + n;
@@ -367,7 +371,7 @@ void or_pair_forces(int n, const float *posq, const uint32_t *type, int ntypes,
     memset(W, 0, sizeof(double) * n);
     if (band_flag) memset(band_flag, 0, n);
     pairforce_ctx c = {type, ntypes, sigma, eps, cut, rsw, kc, alpha, rc_coul,
-                       q, ex_off, ex_partner, ex_scale,
+                       q, ex_off, ex_partner, ex_scale, ex_scale_coul ? ex_scale_coul : ex_scale,
                        band_flag ? band : 0.0, F, E, W, band_flag};
     double rmax2 = rmax * rmax * (1.0 + 4.0 * (band > 0.0 ? band : 0.0));
     for_each_pair(n, x, y, z, L, rmax2, pairforce_visit, &c);
@@ -747,6 +751,101 @@ void or_angle_forces(int n_angles, const uint32_t *ijk, const double *kth, const
         }
         E[i] += u / 3.0; E[j] += u / 3.0; E[k] += u / 3.0;
         W[i] += w / 3.0; W[j] += w / 3.0; W[k] += w / 3.0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Two-body bonded terms: harmonic (src/forces/harmonic_bond.rs:288-330) */
+/* and Morse (src/forces/morse.rs:290-320).  fmag = -dU/dr / r applied   */
+/* along d = r_i - r_j; w = fmag r^2; energy and virial split in halves. */
+/* ------------------------------------------------------------------ */
+void or_bond_forces(int n_bonds, const uint32_t *ij, const int *kind, const double *prm,
+                    const float *posq, const double *L, double *F, double *E, double *W)
+{
+    for (int m = 0; m < n_bonds; ++m) {
+        uint32_t i = ij[2 * m], j = ij[2 * m + 1];
+        double d[3];
+        for (int a = 0; a < 3; ++a) d[a] = (double)posq[4 * i + a] - posq[4 * j + a];
+        or_min_image(&d[0], &d[1], &d[2], L);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double r = sqrt(r2);
+        if (r < 1.0e-7) continue;
+        const double *p = prm + 3 * m;
+        double fmag, u;
+        if (kind[m] == 0) {
+            double dr = r - p[1];
+            fmag = -p[0] * dr / r;
+            u = 0.5 * p[0] * dr * dr;
+        } else {
+            double e = exp(-p[1] * (r - p[2]));
+            fmag = -2.0 * p[0] * p[1] * (1.0 - e) * e / r;
+            u = p[0] * (1.0 - e) * (1.0 - e);
+        }
+        for (int a = 0; a < 3; ++a) {
+            F[3 * i + a] += fmag * d[a];
+            F[3 * j + a] -= fmag * d[a];
+        }
+        E[i] += 0.5 * u; E[j] += 0.5 * u;
+        W[i] += 0.5 * fmag * r2; W[j] += 0.5 * fmag * r2;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Periodic dihedral U = k (1 + cos(n phi - phi0))                       */
+/* (src/forces/dihedral.rs:360-460): b1 = r_i - r_j, b2 = r_k - r_j,      */
+/* b3 = r_k - r_l, m = b1 x b2, n = b2 x b3, phi = atan2(|b2| b1.n, m.n); */
+/* F_i = f |b2|/|m|^2 m, F_l = -f |b2|/|n|^2 n with f = k n sin(n phi -   */
+/* phi0); F_j = (s-1) F_i - t F_l, F_k = -s F_i + (t-1) F_l with          */
+/* s = b1.b2/|b2|^2, t = b3.b2/|b2|^2; W = b1.F_i + b2.F_k + (b2-b3).F_l. */
+/* ------------------------------------------------------------------ */
+void or_dihedral_forces(int n_dih, const uint32_t *ijkl, const double *prm, const float *posq,
+                        const double *L, double *F, double *E, double *W)
+{
+    for (int m = 0; m < n_dih; ++m) {
+        uint32_t a[4] = {ijkl[4 * m], ijkl[4 * m + 1], ijkl[4 * m + 2], ijkl[4 * m + 3]};
+        double b1[3], b2[3], b3[3];
+        for (int d = 0; d < 3; ++d) {
+            b1[d] = (double)posq[4 * a[0] + d] - posq[4 * a[1] + d];
+            b2[d] = (double)posq[4 * a[2] + d] - posq[4 * a[1] + d];
+            b3[d] = (double)posq[4 * a[2] + d] - posq[4 * a[3] + d];
+        }
+        or_min_image(&b1[0], &b1[1], &b1[2], L);
+        or_min_image(&b2[0], &b2[1], &b2[2], L);
+        or_min_image(&b3[0], &b3[1], &b3[2], L);
+        double mv[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2],
+                        b1[0] * b2[1] - b1[1] * b2[0]};
+        double nv[3] = {b2[1] * b3[2] - b2[2] * b3[1], b2[2] * b3[0] - b2[0] * b3[2],
+                        b2[0] * b3[1] - b2[1] * b3[0]};
+        double m2 = mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2];
+        double n2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
+        double bb = b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2];
+        if (m2 < 1e-14 || n2 < 1e-14 || bb < 1e-14) continue;
+        double bl = sqrt(bb);
+        double b1n = b1[0] * nv[0] + b1[1] * nv[1] + b1[2] * nv[2];
+        double mn = mv[0] * nv[0] + mv[1] * nv[1] + mv[2] * nv[2];
+        double phi = atan2(bl * b1n, mn);
+        const double *p = prm + 3 * m;
+        double delta = p[1] * phi - p[2];
+        double f = p[0] * p[1] * sin(delta);
+        double u = p[0] * (1.0 + cos(delta));
+        double fi[3], fl[3], fj[3], fk[3];
+        double s = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) / bb;
+        double t = (b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2]) / bb;
+        for (int d = 0; d < 3; ++d) {
+            fi[d] = f * bl / m2 * mv[d];
+            fl[d] = -f * bl / n2 * nv[d];
+            fj[d] = (s - 1.0) * fi[d] - t * fl[d];
+            fk[d] = -s * fi[d] + (t - 1.0) * fl[d];
+        }
+        double w = 0.0;
+        for (int d = 0; d < 3; ++d)
+            w += b1[d] * fi[d] + b2[d] * fk[d] + (b2[d] - b3[d]) * fl[d];
+        const double *fs[4] = {fi, fj, fk, fl};
+        for (int q = 0; q < 4; ++q) {
+            for (int d = 0; d < 3; ++d) F[3 * a[q] + d] += fs[q][d];
+            E[a[q]] += 0.25 * u;
+            W[a[q]] += 0.25 * w;
+        }
     }
 }
 
@@ -1210,6 +1309,43 @@ double or_philox_gaussian(uint32_t seed_lo, uint32_t seed_hi, uint32_t ctr_lo,
     double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
     double r = sqrt(-2.0 * log(u1));
     return r * cos(6.283185307179586 * u2);
+}
+
+/* CSVR velocity rescale (Bussi, Donadio, Parrinello 2007, eq. A7;        */
+/* src/integrator/csvr.rs:188-240, kernels/nose_hoover.cu:145-255):        */
+/* K_new = c K + (K_t/g)(1-c)(sum_{i=1}^{g-1} xi_i^2 + r^2)                 */
+/*       + 2 r sqrt(c (1-c) K K_t/g), xi_i = normal(seed, counter, i, 0),   */
+/* r = normal(seed, counter, 0, 0); a non-finite or non-positive K_new     */
+/* keeps K.  Returns sqrt(K_new/K) (1 when K = 0 or K_new = K).            */
+double or_csvr_factor(double k, uint32_t g_dof, uint64_t seed, uint64_t counter, double c,
+                      double kt_per_dof, double *k_new)
+{
+    uint32_t slo = (uint32_t)(seed & 0xFFFFFFFFu), shi = (uint32_t)(seed >> 32);
+    uint32_t clo = (uint32_t)(counter & 0xFFFFFFFFu), chi = (uint32_t)(counter >> 32);
+    double s = 0.0;
+    for (uint32_t i = 1; i < g_dof; ++i) {
+        double xi = or_philox_gaussian(slo, shi, clo, chi, i, 0u);
+        s += xi * xi;
+    }
+    double r = or_philox_gaussian(slo, shi, clo, chi, 0u, 0u);
+    double cross = k > 0.0 ? 2.0 * r * sqrt(c * (1.0 - c) * k * kt_per_dof) : 0.0;
+    double kn = c * k + kt_per_dof * (1.0 - c) * (s + r * r) + cross;
+    if (!isfinite(kn) || kn <= 0.0) kn = k;
+    *k_new = kn;
+    return (k > 0.0 && kn != k) ? sqrt(kn / k) : 1.0;
+}
+
+/* Berendsen thermostat factor (src/integrator/berendsen.rs:139-170,       */
+/* nose_hoover.cu:390-417): sqrt(max(0, 1 + dt/tau (K_t/K - 1))); the      */
+/* injected energy K (lambda^2 - 1) goes to *inj.                          */
+double or_berendsen_factor(double k, double k_target, double dt_over_tau, double *inj)
+{
+    *inj = 0.0;
+    if (!(k > 0.0)) return 1.0;
+    double l2 = 1.0 + dt_over_tau * (k_target / k - 1.0);
+    if (!(l2 > 0.0)) l2 = 0.0;
+    *inj = k * (l2 - 1.0);
+    return sqrt(l2);
 }
 
 float or_lan_alpha(float friction, float dt)

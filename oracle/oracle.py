@@ -45,8 +45,14 @@ def lib():
         L.or_pair_forces.restype = None
         L.or_pair_forces.argtypes = [C.c_int, _f32p, _u32p, C.c_int, _f64p, _f64p, _f64p,
                                      _f64p, C.c_double, C.c_double, C.c_double,
-                                     C.c_void_p, C.c_void_p, C.c_void_p, _f64p, C.c_double,
-                                     _f64p, _f64p, _f64p, _u8p]
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _f64p,
+                                     C.c_double, _f64p, _f64p, _f64p, _u8p]
+        L.or_bond_forces.restype = None
+        L.or_bond_forces.argtypes = [C.c_int, _u32p, C.POINTER(C.c_int), _f64p, _f32p, _f64p,
+                                     _f64p, _f64p, _f64p]
+        L.or_dihedral_forces.restype = None
+        L.or_dihedral_forces.argtypes = [C.c_int, _u32p, _f64p, _f32p, _f64p, _f64p, _f64p,
+                                         _f64p]
         L.or_bspline.restype = C.c_double
         L.or_bspline.argtypes = [C.c_int, C.c_double]
         L.or_b_factors.argtypes = [C.c_int, C.c_int, _f64p]
@@ -95,6 +101,12 @@ def lib():
         L.or_nhc_half_step.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, C.c_int,
                                        C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.POINTER(C.c_double)]
+        L.or_csvr_factor.restype = C.c_double
+        L.or_csvr_factor.argtypes = [C.c_double, C.c_uint32, C.c_uint64, C.c_uint64, C.c_double,
+                                     C.c_double, C.POINTER(C.c_double)]
+        L.or_berendsen_factor.restype = C.c_double
+        L.or_berendsen_factor.argtypes = [C.c_double, C.c_double, C.c_double,
+                                          C.POINTER(C.c_double)]
         L.or_kinetic_energy.restype = C.c_double
         L.or_kinetic_energy.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.or_num_threads.restype = C.c_int
@@ -133,7 +145,8 @@ def pair_forces(posq, types, lj, lattice, coul=None, excl=None, band=0.0):
     lj: dict with ntypes and (ntypes x ntypes) arrays sigma, epsilon, cutoff,
         r_switch (atomic units).
     coul: None or dict(alpha=, r_cut=, k_c=1.0).
-    excl: None or (offsets u32[n+1], partners u32[], scales f64[]).
+    excl: None or (offsets u32[n+1], partners u32[], scales f64[]) with one
+        scale for both terms, or (offsets, partners, scale_lj, scale_coul).
     Returns F (n,3), E (n,), W (n,), band_flag (n,) bool.
     """
     posq = np.ascontiguousarray(posq, np.float32)
@@ -153,13 +166,14 @@ def pair_forces(posq, types, lj, lattice, coul=None, excl=None, band=0.0):
         off = np.ascontiguousarray(excl[0], np.uint32)
         par = np.ascontiguousarray(excl[1], np.uint32)
         sca = np.ascontiguousarray(excl[2], np.float64)
-        keep = [off, par, sca]
+        scc = np.ascontiguousarray(excl[3] if len(excl) > 3 else excl[2], np.float64)
+        keep = [off, par, sca, scc]
         ptrs = [a.ctypes.data for a in keep]
     else:
-        ptrs = [None, None, None]
+        ptrs = [None, None, None, None]
     lib().or_pair_forces(n, posq.reshape(-1), np.ascontiguousarray(types, np.uint32), nt,
                          arr("sigma"), arr("epsilon"), arr("cutoff"), arr("r_switch"),
-                         kc, alpha, rc, ptrs[0], ptrs[1], ptrs[2], _lat(lattice), band,
+                         kc, alpha, rc, ptrs[0], ptrs[1], ptrs[2], ptrs[3], _lat(lattice), band,
                          F, E, W, bf)
     del keep
     return F.reshape(n, 3), E, W, bf[:n].astype(bool)
@@ -242,6 +256,38 @@ def angle_forces(posq, lattice, ijk, k_theta, theta0):
     lib().or_angle_forces(ijk.size // 3, ijk, np.ascontiguousarray(k_theta, np.float64),
                           np.ascontiguousarray(theta0, np.float64), np.ascontiguousarray(posq),
                           np.ascontiguousarray(lattice, np.float64), F, E, W)
+    return F.reshape(n, 3), E, W
+
+
+def bond_forces(posq, lattice, ij, kind, prm):
+    """Harmonic (kind 0: k, r0) and Morse (kind 1: De, a, re) bond forces
+    (n,3), per-atom energy and virial halves, FP64
+    (src/forces/harmonic_bond.rs:288-330, src/forces/morse.rs:290-320)."""
+    n = posq.shape[0]
+    ij = np.ascontiguousarray(ij, np.uint32).reshape(-1)
+    kind = np.ascontiguousarray(kind, np.int32).reshape(-1)
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(n)
+    lib().or_bond_forces(kind.size, ij, kind.ctypes.data_as(C.POINTER(C.c_int)),
+                         np.ascontiguousarray(prm, np.float64).reshape(-1),
+                         np.ascontiguousarray(posq, np.float32).reshape(-1),
+                         np.ascontiguousarray(lattice, np.float64), F, E, W)
+    return F.reshape(n, 3), E, W
+
+
+def dihedral_forces(posq, lattice, ijkl, prm):
+    """Periodic dihedral forces (n,3), per-atom energy and virial quarters,
+    FP64 (src/forces/dihedral.rs:360-460); prm rows (k, n, phi0)."""
+    n = posq.shape[0]
+    ijkl = np.ascontiguousarray(ijkl, np.uint32).reshape(-1)
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(n)
+    lib().or_dihedral_forces(ijkl.size // 4, ijkl,
+                             np.ascontiguousarray(prm, np.float64).reshape(-1),
+                             np.ascontiguousarray(posq, np.float32).reshape(-1),
+                             np.ascontiguousarray(lattice, np.float64), F, E, W)
     return F.reshape(n, 3), E, W
 
 
@@ -453,6 +499,22 @@ class NHCState:
         return c
 
 
+def csvr_factor(k, g_dof, seed, counter, c, kt_per_dof):
+    """CSVR rescale (src/integrator/csvr.rs:188-240): (factor, K_new)."""
+    kn = C.c_double(0.0)
+    f = lib().or_csvr_factor(float(k), int(g_dof), int(seed), int(counter), float(c),
+                             float(kt_per_dof), C.byref(kn))
+    return f, kn.value
+
+
+def berendsen_factor(k, k_target, dt_over_tau):
+    """Berendsen thermostat (src/integrator/berendsen.rs:139-170):
+    (factor, injected energy)."""
+    inj = C.c_double(0.0)
+    f = lib().or_berendsen_factor(float(k), float(k_target), float(dt_over_tau), C.byref(inj))
+    return f, inj.value
+
+
 def kinetic_energy(v, mass):
     return lib().or_kinetic_energy(v.shape[1], v[0], v[1], v[2], mass)
 
@@ -529,3 +591,55 @@ def mlp_loss_grad(p, x, t):
     gb1 = D1.sum(axis=0)
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
     return loss, g
+
+
+class SteepestDescent:
+    """Steepest descent (src/minimizer/steepest_descent.rs initial_state,
+    step, check_convergence): a trial move x += F step / F_max from the
+    current forces, kept when the energy drops (step *= step_increase, at
+    most max_step) and undone otherwise (step *= step_decrease).
+    ``energy_forces(posq) -> (E, F (n,3))``; positions are f32 like the
+    device state, the move is formed in f32."""
+
+    def __init__(self, energy_forces, posq, initial_step, max_step, step_increase=1.2,
+                 step_decrease=0.2):
+        self.ef = energy_forces
+        self.posq = np.array(posq, np.float32, copy=True)
+        self.step_len = float(initial_step)
+        self.max_step = float(max_step)
+        self.inc = float(step_increase)
+        self.dec = float(step_decrease)
+        self.energy, self.F = self.ef(self.posq)
+
+    def max_force(self):
+        return float(np.sqrt((self.F.astype(np.float64) ** 2).sum(1).max()))
+
+    def step(self):
+        prev, used = self.energy, self.step_len
+        fmax = self.max_force()
+        if not fmax > 0.0:
+            return dict(energy=prev, max_force=0.0, step=used, accepted=False)
+        scale = np.float32(used / fmax)
+        trial = self.posq.copy()
+        trial[:, :3] = trial[:, :3] + self.F.astype(np.float32) * scale
+        e, F = self.ef(trial)
+        if e < prev:
+            self.posq, self.energy, self.F = trial, e, F
+            self.step_len = min(self.step_len * self.inc, self.max_step)
+            return dict(energy=e, max_force=self.max_force(), step=used, accepted=True)
+        self.step_len *= self.dec
+        return dict(energy=prev, max_force=self.max_force(), step=used, accepted=False)
+
+
+def sd_converged(report, prev_energy, force_tolerance, energy_tolerance):
+    """check_convergence (src/minimizer/steepest_descent.rs): 'force_zero',
+    'force_tolerance', 'energy_tolerance' or None."""
+    if report["max_force"] == 0.0:
+        return "force_zero"
+    if force_tolerance > 0.0 and report["max_force"] <= force_tolerance:
+        return "force_tolerance"
+    e = report["energy"]
+    if report["accepted"] and energy_tolerance > 0.0 and \
+            abs(e - prev_energy) <= energy_tolerance * max(abs(e), abs(prev_energy), 1e-30):
+        return "energy_tolerance"
+    return None

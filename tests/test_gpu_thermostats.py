@@ -1,0 +1,125 @@
+"""CSVR and Berendsen thermostats and the Berendsen barostat on the GPU,
+through the C ABI, against the oracle's restatement of the reference's
+steps (src/integrator/csvr.rs, berendsen.rs, berendsen_barostat.rs)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from heddlemd_amd import systems, units as U  # noqa: E402
+from heddlemd_amd.engine import HeddleError, engine_for  # noqa: E402
+from tests.test_gpu_parity import _jittered_argon, _water_engine  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gas(seed):
+    s = _jittered_argon(n_atoms=4000, seed=seed)
+    s.lj.epsilon[:] = 0.0
+    return s
+
+
+@pytest.mark.parametrize("kind", ["csvr", "berendsen"])
+def test_rescale_thermostat_matches_oracle(oracle, kind):
+    # force-free gas started at 1.7x the bath temperature: each step is the
+    # drift and the post-force rescale (factor on v), so the velocities and
+    # the injected energy follow the oracle's factors alone.  The kinetic
+    # energies are FP64 sums in different orders on the two sides, so a
+    # factor may round to a neighbouring f32 value: velocities within 1e-6
+    # relative, KE and the injection within 1e-9 relative.
+    say(f"case: argon 4000 force-free {kind} 60 steps")
+    s = _gas(3)
+    kt = U.temperature(300.0)
+    tau = U.time(5e-14)
+    dt = np.float32(U.time(2e-15))
+    rng = np.random.default_rng(17)
+    v0 = (rng.normal(size=(3, s.n)) * np.sqrt(1.7 * kt / s.mass)).astype(np.float32)
+    steps = 60
+    kw = dict(csvr=dict(kt=kt, tau=tau, seed=11)) if kind == "csvr" else \
+        dict(berendsen=dict(kt=kt, tau=tau))
+    e = engine_for(s, dt=float(dt), velocities=v0, **kw)
+    e.force_eval(scalars=False)
+    e.step(steps, scalars_last=True)
+    v1 = e.velocities()
+    sc = e.scalars()
+    th = e.thermostat()
+    assert np.abs(e.forces()).max() == 0.0
+    e.close()
+    g = 3 * s.n - 3
+    c = np.exp(-float(dt) / tau)
+    v = v0.copy()
+    inj = 0.0
+    for step in range(steps):
+        k = oracle.kinetic_energy(v, s.mass)
+        if kind == "csvr":
+            f, kn = oracle.csvr_factor(k, g, 11, step, c, kt / 2)
+            inj += kn - k
+        else:
+            f, d = oracle.berendsen_factor(k, 0.5 * g * kt, float(dt) / tau)
+            inj += d
+        v = v * np.float32(f)
+    k_ref = oracle.kinetic_energy(v, s.mass)
+    k0 = oracle.kinetic_energy(v0, s.mass)
+    say(f"  KE {k0:.6e} -> {sc['ke']:.6e} (oracle {k_ref:.6e}); injected {th['injection']:.9e} "
+        f"(oracle {inj:.9e}); draws {th['draws']}; identical words {np.mean(v1 == v):.4f}")
+    assert abs(k_ref / k0 - 1.0) > 0.05
+    assert np.all(np.abs(v1 - v) <= 1e-6 * np.abs(v))
+    assert sc["ke"] == pytest.approx(k_ref, rel=1e-9)
+    assert th["injection"] == pytest.approx(inj, rel=1e-9)
+    assert sc["nhc_chain"] == pytest.approx(-inj, rel=1e-9)  # conserved-quantity term
+    assert th["draws"] == (steps if kind == "csvr" else 0)
+
+
+def test_csvr_relaxes_a_lj_liquid_to_the_bath():
+    # argon at 120 K started from 300 K velocities under a 50 fs CSVR: the
+    # temperature over the last 200 of 600 steps is the bath's within 5 %
+    say("case: argon 10000 CSVR 600 steps from 300 K to 120 K")
+    from heddlemd_amd.rng import maxwell_boltzmann
+    s = systems.config_argon()
+    kt = U.temperature(120.0)
+    v = maxwell_boltzmann(s.mass, U.temperature(300.0), seed=42, n_constraints=0)
+    e = engine_for(s, dt=U.time(5e-15), velocities=v,
+                   csvr=dict(kt=kt, tau=U.time(5e-14), seed=3))
+    e.force_eval(scalars=False)
+    e.step(400)
+    temps = []
+    for _ in range(20):
+        e.step(10, scalars_last=True)
+        temps.append(2 * e.scalars()["ke"] / (3 * s.n - 3))
+    e.close()
+    t = np.mean(temps) / kt
+    say(f"  <T>/T_bath over the last 200 steps {t:.4f}")
+    assert abs(t - 1.0) < 0.05
+
+
+def test_berendsen_barostat_move_matches_formula():
+    say("case: water 8192 Berendsen barostat one step")
+    s = systems.config_water_8192()
+    kappa, p0, tau = U.inv_pressure(4.5e-10), U.pressure(1e5), U.time(1e-12)
+    e = _water_engine(s)
+    e.set_berendsen_barostat(p0, tau, kappa)
+    L0 = e.lattice()
+    e.force_eval(scalars=False)
+    e.step(1, scalars_last=True)
+    b = e.barostat()
+    L1 = e.lattice()
+    e.close()
+    dt = U.time(2e-15)
+    mu3 = 1 - kappa * (dt / tau) * (p0 - b["pressure"])
+    say(f"  P {b['pressure']:.6e}, mu {b['mu']:.9f}")
+    assert b["mu"] == pytest.approx(float(np.float32(mu3 ** (1 / 3))), rel=1e-6)
+    assert L1[0] == pytest.approx(float(np.float32(L0[0]) * np.float32(b["mu"])), rel=1e-7)
+    assert b["draws"] == 0 and b["injection"] == 0.0
+
+
+def test_one_thermostat_per_context():
+    s = _gas(5)
+    kt = U.temperature(300.0)
+    with pytest.raises(HeddleError):
+        engine_for(s, dt=U.time(2e-15), nhc_kt=kt, nhc_tau=U.time(1e-13),
+                   csvr=dict(kt=kt, tau=U.time(1e-13), seed=1))

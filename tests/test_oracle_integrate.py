@@ -357,3 +357,77 @@ def test_shake_long_chain_converges(oracle):
     assert 1 < iters[0] < 5000
     r = np.linalg.norm(np.diff(posq[:, :3].astype(np.float64), axis=0), axis=1)
     assert np.abs(r * r - np.float32(d) ** 2).max() <= 2 * tol
+
+
+def test_csvr_draws_the_canonical_kinetic_energy(oracle):
+    # tau -> 0 (c = 0): K_new = (kT/2) (sum of g squared normals), a
+    # chi-squared of g degrees of freedom: mean g kT/2, variance g (kT)^2/2;
+    # independent of K and fresh for every counter value
+    g, kt = 30, 0.7
+    ks = np.array([oracle.csvr_factor(5.0, g, 99, ctr, 0.0, kt / 2)[1] for ctr in range(4000)])
+    mean, var = ks.mean(), ks.var()
+    assert abs(mean - g * kt / 2) < 4 * np.sqrt(g * kt * kt / 2 / len(ks))
+    assert var == pytest.approx(g * kt * kt / 2, rel=0.1)
+    # the factor rescales K to K_new exactly
+    f, kn = oracle.csvr_factor(5.0, g, 99, 7, 0.3, kt / 2)
+    assert f * f * 5.0 == pytest.approx(kn, rel=1e-14)
+
+
+def test_csvr_limits(oracle):
+    # c = 1 (tau -> inf) keeps K; K = 0 keeps factor 1
+    f, kn = oracle.csvr_factor(3.0, 40, 5, 11, 1.0, 0.2)
+    assert kn == pytest.approx(3.0, rel=1e-15) and f == pytest.approx(1.0, rel=1e-15)
+    f0, _ = oracle.csvr_factor(0.0, 40, 5, 11, 0.5, 0.2)
+    assert f0 == 1.0
+    # the closed form with the draws taken from the Philox normals
+    g, seed, ctr, c, kpd, k = 6, 123, 4, 0.8, 0.05, 0.9
+    xi = [oracle.philox_gaussian(seed, ctr, i, 0) for i in range(g)]
+    want = c * k + kpd * (1 - c) * (sum(x * x for x in xi[1:]) + xi[0] ** 2) \
+        + 2 * xi[0] * np.sqrt(c * (1 - c) * k * kpd)
+    assert oracle.csvr_factor(k, g, seed, ctr, c, kpd)[1] == pytest.approx(want, rel=1e-14)
+
+
+def test_berendsen_factor_relaxes_to_the_target(oracle):
+    kt_target = 2.0
+    k = 1.0
+    for _ in range(200):
+        f, inj = oracle.berendsen_factor(k, kt_target, 0.05)
+        assert inj == pytest.approx(k * (f * f - 1))
+        k *= f * f
+    assert k == pytest.approx(kt_target, rel=1e-4)
+    assert oracle.berendsen_factor(0.0, 1.0, 0.1) == (1.0, 0.0)
+    # a step longer than tau past a hot system clamps lambda^2 at 0
+    assert oracle.berendsen_factor(10.0, 0.0, 2.0)[0] == 0.0
+
+
+def test_steepest_descent_on_a_quadratic_well(oracle):
+    # U = k/2 |x - x0|^2 per particle: F = -k (x - x0).  From a start 1.0
+    # off, each accepted move goes step along -grad/|grad|_max; the step
+    # grows by 1.2 up to max_step while moves are downhill, is cut by 0.2
+    # after an overshoot, and the iterate ends at x0
+    x0 = np.array([[0.5, -0.25, 0.125, 0.0], [1.0, 2.0, -1.0, 0.0]], np.float32)
+
+    def ef(p):
+        d = p[:, :3].astype(np.float64) - x0[:, :3]
+        return 0.5 * 2.0 * float((d ** 2).sum()), -2.0 * d
+
+    start = x0.copy()
+    start[:, 0] += 1.0
+    sd = oracle.SteepestDescent(ef, start, initial_step=0.1, max_step=0.3)
+    r1 = sd.step()
+    assert r1["accepted"] and r1["step"] == 0.1
+    assert sd.step_len == pytest.approx(0.12)
+    steps = [sd.step() for _ in range(60)]
+    assert any(not s["accepted"] for s in steps)  # an overshoot was undone
+    assert max(s["step"] for s in steps) <= 0.3 + 1e-12
+    assert sd.energy < 1e-6
+    energies = [r1["energy"]] + [s["energy"] for s in steps]
+    assert all(b <= a for a, b in zip(energies, energies[1:]))
+    # convergence rules
+    assert oracle.sd_converged(dict(max_force=0.0, energy=1, accepted=True), 1, 0, 0) == "force_zero"
+    assert oracle.sd_converged(dict(max_force=1e-11, energy=1, accepted=True), 2, 1e-10, 0) \
+        == "force_tolerance"
+    assert oracle.sd_converged(dict(max_force=1.0, energy=1.0, accepted=True), 1.0 + 1e-8, 0,
+                               1e-7) == "energy_tolerance"
+    assert oracle.sd_converged(dict(max_force=1.0, energy=1.0, accepted=False), 1.0, 0,
+                               1e-7) is None
