@@ -116,15 +116,18 @@ def maxwell_boltzmann(masses, kt, seed, n_constraints=0):
     z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
     sigma = np.sqrt(kt / m)
     v = (z * sigma[:, None]).astype(np.float32).T.copy()
-    total = float(np.sum(m))
+    # sums in index order, as the reference's iterator sums (a cumulative
+    # sum adds left to right; np.sum would add pairwise)
+    total = float(np.cumsum(m)[-1])
     if total > 0:
         for ax in range(3):
-            p = float(np.sum(m * v[ax].astype(np.float64)))
+            p = float(np.cumsum(m * v[ax].astype(np.float64))[-1])
             v[ax] = (v[ax].astype(np.float64) - p / total).astype(np.float32)
     ndof = 3 * n - n_constraints - 3
     if ndof <= 0:
         return np.zeros((3, n), np.float32)
-    ke = 0.5 * float(np.sum(m * np.sum(v.astype(np.float64) ** 2, axis=0)))
+    w = v.astype(np.float64)
+    ke = float(np.cumsum(0.5 * m * ((w[0] ** 2 + w[1] ** 2) + w[2] ** 2))[-1])
     if ke > 0:
         scale = np.sqrt(0.5 * ndof * kt / ke)
         v = (v.astype(np.float64) * scale).astype(np.float32)

@@ -1,0 +1,132 @@
+"""`heddlemd run` end to end on the GPU: the argon bundle twice (the .xyz
+and .log outputs byte-identical across runs), its Maxwell-Boltzmann start
+and formats against the host restatement, its dynamics against the Python
+engine; and the water bundle through minimization, CSVR NVT and CSVR +
+C-rescale NPT with SETTLE or SHAKE."""
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from heddlemd_amd import units as U  # noqa: E402
+from tests.cli_inputs import BIN, run_cli, rust_e, write_argon, write_water  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _read_xyz_frames(path):
+    frames = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while k < len(lines):
+        n = int(lines[k])
+        frames.append((lines[k + 1], lines[k + 2:k + 2 + n]))
+        k += 2 + n
+    return frames
+
+
+def test_argon_run_is_deterministic_and_matches_the_engine(tmp_path):
+    say("case: heddlemd run argon 10000, 20 steps, twice")
+    outs = []
+    for rep in range(2):
+        d = tmp_path / f"r{rep}"
+        d.mkdir()
+        cfg, s = write_argon(str(d))
+        r = run_cli("run", cfg, timeout=300)
+        say("  " + r.stdout.strip().replace("\n", "\n  "))
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.splitlines()
+        assert re.fullmatch(r"\[heddlemd\] phase `nve`: 20 steps in \d+ (ms|µs) "
+                            r"\(frames: 3, log rows: 5\)", lines[0])
+        assert re.fullmatch(r"\[heddlemd\] complete: 1 phases, 20 steps in \d+ (ms|µs)", lines[1])
+        outs.append({ext: open(os.path.join(str(d), f"argon.out.nve.{ext}"), "rb").read()
+                     for ext in ("xyz", "log", "timings")})
+    assert outs[0]["xyz"] == outs[1]["xyz"]
+    assert outs[0]["log"] == outs[1]["log"]
+    log = outs[0]["log"].decode().splitlines()
+    assert log[0] == "step,time,kinetic_energy,temperature"
+    assert [row.split(",")[0] for row in log[1:]] == ["0", "5", "10", "15", "20"]
+    timings = outs[0]["timings"].decode().splitlines()
+    assert timings[0].split() == ["stage", "count", "total_ms", "mean_us", "min_us", "max_us"]
+    frames = _read_xyz_frames(str(tmp_path / "r0" / "argon.out.nve.xyz"))
+    assert len(frames) == 3
+    head = frames[0][0]
+    assert "Properties=species:S:1:pos:R:3:velo:R:3:image:I:3 Step=0 Time=0.000000000e0" in head
+    # the step-0 velocities: the Maxwell-Boltzmann start (ChaCha8 seed 42,
+    # 120 K, f64 type masses), scaled to m/s in f32 and printed as {:.9e}
+    from heddlemd_amd.rng import maxwell_boltzmann
+    m_au = 6.6335e-26 / U.MASS_ME_TO_KG
+    v = maxwell_boltzmann(np.full(s.n, m_au), U.temperature(120.0), seed=42, n_constraints=0)
+    vf = np.float32(U.VELOCITY_AU_TO_M_PER_S)
+    rows = frames[0][1]
+    bad = 0
+    for i in (0, 1, 2, 4999, 9999):
+        tok = rows[i].split()
+        want = [rust_e(float(np.float32(v[a, i]) * vf)) for a in range(3)]
+        bad += tok[4:7] != want
+    assert bad == 0
+    # dynamics: the logged kinetic energy at step 20 against the Python
+    # engine on the same start (positions read back from the text at 9+
+    # digits agree to an f32 ulp)
+    from heddlemd_amd.engine import engine_for
+    e = engine_for(s, dt=U.time(5e-15), velocities=v)
+    e.force_eval(scalars=False)
+    e.step(20, scalars_last=True)
+    ke = e.scalars()["ke"]
+    e.close()
+    ke_log = float(log[-1].split(",")[2]) / U.ENERGY_HARTREE_TO_J
+    say(f"  KE(step 20) log {ke_log:.9e} Eh, engine {ke:.9e} Eh")
+    assert ke_log == pytest.approx(ke, rel=1e-5)
+
+
+@pytest.mark.parametrize("solver", ["settle", "shake"])
+def test_water_bundle_minimize_nvt_npt(tmp_path, solver):
+    say(f"case: heddlemd run water 8192 ({solver}): minimization, CSVR NVT, CSVR + C-rescale NPT")
+    cfg, s = write_water(str(tmp_path), solver=solver, min_iter=500)
+    r = run_cli("run", cfg, timeout=600)
+    say("  " + (r.stdout + r.stderr).strip().replace("\n", "\n  "))
+    assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    m = re.fullmatch(r"\[heddlemd\] phase `min`: (\d+) iters in \d+ (ms|µs) \(converged: "
+                     r"(force_tolerance|energy_tolerance|force_zero), frames: 0, log rows: (\d+)\)",
+                     lines[0])
+    assert m
+    assert lines[1].startswith("[heddlemd] phase `equil`: 20 steps")
+    assert lines[2].startswith("[heddlemd] phase `prod`: 20 steps")
+    minlog = open(os.path.join(str(tmp_path), "water.out.min.minlog")).read().splitlines()
+    assert minlog[0] == "iter,energy,max_force,step,accepted"
+    e_min = [float(x.split(",")[1]) for x in minlog[1:]]
+    assert e_min[-1] < e_min[0]
+    eq = open(os.path.join(str(tmp_path), "water.out.equil.log")).read().splitlines()
+    assert eq[0] == "step,time,kinetic_energy,temperature,csvr_conserved"
+    pr = open(os.path.join(str(tmp_path), "water.out.prod.log")).read().splitlines()
+    assert pr[0] == ("step,time,kinetic_energy,temperature,csvr_conserved,pressure,"
+                     "box_volume,crescale_conserved")
+    temps = [float(x.split(",")[3]) for x in eq[1:] + pr[1:]]
+    assert all(50.0 < t < 600.0 for t in temps), temps
+    frames = _read_xyz_frames(os.path.join(str(tmp_path), "water.out.prod.xyz"))
+    assert len(frames) == 2
+    assert "Properties=species:S:1:pos:R:3:image:I:3 Step=20 Time=4.000000000e-14" in frames[1][0]
+    # the final frame keeps every water rigid up to the common factor of
+    # the step's barostat move (positions are scaled after the constraint
+    # solve, which the next step's SETTLE / SHAKE undoes)
+    pos = np.array([[float(t) for t in row.split()[1:4]] for row in frames[1][1]])
+    img = np.array([[int(t) for t in row.split()[4:7]] for row in frames[1][1]])
+    assert img.shape == (s.n, 3)
+    g = s.water_groups
+    lat = frames[1][0].split('"')[1].split()
+    L = np.array([float(lat[0]), float(lat[4]), float(lat[8])])
+    d = pos[g[:, 0]] - pos[g[:, 1]]
+    d -= L * np.round(d / L)
+    r_oh = np.linalg.norm(d, axis=1)
+    say(f"  final O-H {r_oh.min():.9e} .. {r_oh.max():.9e} m")
+    assert (r_oh.max() - r_oh.min()) / r_oh.mean() < 2e-5
+    assert np.abs(r_oh / 1.0e-10 - 1).max() < 1e-3
