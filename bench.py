@@ -66,10 +66,36 @@ def build_engine(system, U, args):
     return e, kt
 
 
+def minimize(e, U, args):
+    """Steepest descent from the lattice start, with the reference examples'
+    settings (examples/spc-water-8192/water.in.toml [[minimization]]:
+    force tolerance 1e-9 N, energy tolerance 1e-5, default steps)."""
+    f_tol = 1.0e-9 / U.FORCE_AU_TO_N
+    r = e.minimize_begin(U.length(1.0e-12))
+    e0 = prev = r["energy"]
+    why, it = "max_iterations", 0
+    for it in range(1, args.min_iters + 1):
+        r = e.minimize_step(U.length(1.0e-10))
+        if r["max_force"] == 0.0:
+            why = "force_zero"
+            break
+        if r["max_force"] <= f_tol:
+            why = "force_tolerance"
+            break
+        if r["accepted"] and abs(r["energy"] - prev) <= 1e-5 * max(abs(r["energy"]), abs(prev),
+                                                                    1e-30):
+            why = "energy_tolerance"
+            break
+        if r["accepted"]:
+            prev = r["energy"]
+    log(f"minimization: {it} iterations ({why}), PE {e0:.2f} -> {r['energy']:.2f} Eh")
+
+
 def equilibrate(e, system, U, kt, args):
-    """The lattice start releases a large amount of potential energy; drain
-    it with Maxwell-Boltzmann redraws at 1 fs, then settle at the production
-    time step with a tight thermostat."""
+    """The lattice start holds a large amount of potential energy: relax it
+    with steepest descent, drain what remains with Maxwell-Boltzmann redraws
+    at 1 fs, then settle at the production time step with a tight
+    thermostat."""
     from heddlemd_amd.rng import maxwell_boltzmann
 
     ndof = 3 * system.n - system.n_constraints - 3
@@ -78,6 +104,9 @@ def equilibrate(e, system, U, kt, args):
         sc = e.scalars()
         return 2 * sc["ke"] / ndof * U.TEMPERATURE_AU_TO_K, sc
 
+    if args.min_iters > 0:
+        minimize(e, U, args)
+        e.force_eval(scalars=False)
     tau = U.time(args.tau_fs * 1e-15)
     for k in range(args.equil_redraws):
         e.step(50, scalars_last=True)
@@ -244,7 +273,7 @@ def batched_replicas(args):
            "graph_lanes": lanes,
            "workload": "half LJ argon (2.1e28 m^-3) / half SPC water (1 g/cm^3), atom counts "
                        "log-uniform in [256, 8192] (seed 11), 280-380 K Langevin (1/ps), "
-                       "SPME grids 16^3-64^3"}
+                       "SPME grid sides from {32, 40, 48, 64} at ~1 Angstrom"}
     log(f"batched: {out}")
     return out
 
@@ -357,7 +386,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--equil", type=int, default=1000,
                     help="untimed equilibration steps at the production dt, tight thermostat")
-    ap.add_argument("--equil-redraws", type=int, default=10,
+    ap.add_argument("--min-iters", type=int, default=500,
+                    help="steepest-descent iterations at most before the MD equilibration")
+    ap.add_argument("--equil-redraws", type=int, default=4,
                     help="50-step blocks at 1 fs, each ended by a velocity redraw")
     ap.add_argument("--equil-tau-fs", type=float, default=100.0)
     ap.add_argument("--tau-fs", type=float, default=100.0)
