@@ -56,6 +56,9 @@ SIGNATURES = {
     "hd_set_berendsen_barostat": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double]),
     "hd_minimize_begin": (C.c_int, [_ctx, C.c_double, _f64p]),
     "hd_minimize_step": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, _f64p]),
+    "hd_set_molecules": (C.c_int, [_ctx, C.c_uint64, _u32p, _u32p]),
+    "hd_mc_barostat_move": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                      C.c_uint64, _f64p]),
     "hd_set_bonds": (C.c_int, [_ctx, C.c_uint64, _u32p, C.POINTER(C.c_int), _f64p]),
     "hd_set_dihedrals": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p, _f64p]),
     "hd_set_exclusions": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),

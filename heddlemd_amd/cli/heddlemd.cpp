@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <set>
 #include <sstream>
@@ -496,7 +497,7 @@ Config load_config(const std::string& path) {
     };
     check(ph.integrator, "integrator", {"velocity-verlet", "langevin-baoab"}, {"mtk-npt"});
     check(ph.thermostat, "thermostat", {"nose-hoover-chain", "csvr", "berendsen"}, {"andersen"});
-    check(ph.barostat, "barostat", {"c-rescale", "berendsen"}, {"monte-carlo"});
+    check(ph.barostat, "barostat", {"c-rescale", "berendsen", "monte-carlo"}, {});
   }
   std::sort(c.phases.begin(), c.phases.end(),
             [](const PhaseCfg& a, const PhaseCfg& b) { return a.order < b.order; });
@@ -612,6 +613,10 @@ struct Topology {
   std::vector<std::vector<uint32_t>> groups;
   std::vector<int> group_type;
   uint64_t n_constraints = 0;
+  // molecules: connected components of bonds and constraint groups
+  // (MoleculeList::from_topology, topology.rs:1496-1560), ordered by their
+  // smallest atom, atoms ascending; empty when every atom stands alone
+  std::vector<uint32_t> mol_off, mol_atoms;
 };
 
 Topology load_topology(const Config& c, State& st) {
@@ -712,6 +717,34 @@ Topology load_topology(const Config& c, State& st) {
   for (auto& dd : dihs) {
     auto& ty = c.dihedral_types.at(dd.type);
     add(dd.a[0], dd.a[3], ty[3], ty[4]);
+  }
+  if (!t.bond_ij.empty() || !t.groups.empty()) {
+    std::vector<uint32_t> parent(st.n);
+    for (uint64_t i = 0; i < st.n; ++i) parent[i] = (uint32_t)i;
+    std::function<uint32_t(uint32_t)> root = [&](uint32_t x) {
+      while (parent[x] != x) x = parent[x] = parent[parent[x]];
+      return x;
+    };
+    auto join = [&](uint32_t a, uint32_t b) {
+      uint32_t ra = root(a), rb = root(b);
+      if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+    };
+    for (size_t k = 0; k < t.bond_ij.size(); k += 2) join(t.bond_ij[k], t.bond_ij[k + 1]);
+    for (auto& g : t.groups)
+      for (size_t k = 1; k < g.size(); ++k) join(g[0], g[k]);
+    // roots are each component's smallest atom: bucket atoms by root
+    std::vector<uint32_t> count(st.n, 0), start(st.n, 0);
+    for (uint64_t i = 0; i < st.n; ++i) count[root((uint32_t)i)]++;
+    t.mol_off.push_back(0);
+    uint32_t acc = 0;
+    for (uint64_t i = 0; i < st.n; ++i)
+      if (count[i]) {
+        start[i] = acc;
+        acc += count[i];
+        t.mol_off.push_back(acc);
+      }
+    t.mol_atoms.resize(st.n);
+    for (uint64_t i = 0; i < st.n; ++i) t.mol_atoms[start[root((uint32_t)i)]++] = (uint32_t)i;
   }
   return t;
 }
@@ -1053,8 +1086,16 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
         e.check(hd_set_berendsen_barostat(e.ctx, param(ba, "pressure", PRESSURE, U), param(ba, "tau", TIME, U),
                                           param(ba, "compressibility", INV_PRESSURE, U)),
                 "barostat", 1);
+      } else if (ba.kind == "monte-carlo") {
+        // host-orchestrated moves between steps (run_md); the device needs
+        // only the molecule partition
+        check_keys(ba.params, {"kind", "pressure", "temperature", "frequency", "volume_step", "seed"},
+                   "phase.barostat");
+        if (!t.mol_off.empty())
+          e.check(hd_set_molecules(e.ctx, t.mol_off.size() - 1, t.mol_off.data(), t.mol_atoms.data()),
+                  "molecules", 1);
       } else {
-        fail1("phase.barostat: kind `" + ba.kind + "` is not built (c-rescale, berendsen)");
+        fail1("phase.barostat: kind `" + ba.kind + "` is not built (c-rescale, berendsen, monte-carlo)");
       }
     }
   }
@@ -1080,16 +1121,30 @@ std::vector<std::pair<std::string, Dim>> extra_columns(const PhaseCfg& ph) {
   if (ph.thermostat.kind == "nose-hoover-chain") cols.push_back({"nhc_conserved", ENERGY});
   if (ph.thermostat.kind == "csvr") cols.push_back({"csvr_conserved", ENERGY});
   if (ph.thermostat.kind == "berendsen") cols.push_back({"berendsen_conserved", ENERGY});
-  if (!ph.barostat.kind.empty()) {
+  if (ph.barostat.kind == "c-rescale" || ph.barostat.kind == "berendsen") {
     cols.push_back({"pressure", PRESSURE});
     cols.push_back({"box_volume", NONE});
-    if (ph.barostat.kind == "c-rescale") cols.push_back({"crescale_conserved", ENERGY});
+    if (ph.barostat.kind == "c-rescale") cols.push_back({"c_rescale_conserved", ENERGY});
+  } else if (ph.barostat.kind == "monte-carlo") {
+    cols.push_back({"box_volume", NONE});
+    cols.push_back({"mc_acceptance", NONE});
+    cols.push_back({"mc_volume_step", NONE});
+    cols.push_back({"mc_conserved", ENERGY});
   }
   return cols;
 }
 
+// Monte-Carlo barostat bookkeeping on the host (mc_barostat.rs:122-150,
+// 353-385): draw counter, acceptance, the adaptive step every 10 attempts
+struct McState {
+  bool on = false;
+  double pressure = 0, kt = 0, max_dv = 0, injection = 0, volume = 0;
+  uint64_t freq = 25, seed = 0, counter = 0, attempted = 0, accepted = 0, n_att = 0, n_acc = 0;
+  double acceptance() const { return attempted ? (double)accepted / (double)attempted : 0.0; }
+};
+
 void log_row(OutFile& o, const Config& c, const PhaseCfg& ph, Engine& e, uint64_t step, uint64_t ndof,
-             const std::vector<std::pair<std::string, Dim>>& cols) {
+             const std::vector<std::pair<std::string, Dim>>& cols, const McState& mc) {
   double sc[8];
   e.check(hd_get_scalars(e.ctx, sc), "scalars", 2);
   double ke = sc[0], pe = sc[1];
@@ -1098,12 +1153,17 @@ void log_row(OutFile& o, const Config& c, const PhaseCfg& ph, Engine& e, uint64_
   std::string row = std::to_string(step) + "," + fmt_e(U.to_user(TIME, (double)step * ph.dt)) + "," +
                     fmt_e(U.to_user(ENERGY, ke)) + "," + fmt_e(U.to_user(TEMPERATURE, T));
   double baro[5] = {0, 0, 0, 0, 0};
-  if (!ph.barostat.kind.empty()) e.check(hd_get_barostat(e.ctx, baro), "barostat", 2);
+  if (!ph.barostat.kind.empty() && !mc.on) e.check(hd_get_barostat(e.ctx, baro), "barostat", 2);
+  double p0 = 0.0;
+  if (ph.barostat.kind == "c-rescale") p0 = param(ph.barostat, "pressure", PRESSURE, U);
   for (auto& col : cols) {
     double v = 0;
     if (col.first == "pressure") v = baro[1];
-    else if (col.first == "box_volume") v = baro[2];
-    else if (col.first == "crescale_conserved") v = ke + pe + baro[3];
+    else if (col.first == "box_volume") v = mc.on ? mc.volume : baro[2];
+    else if (col.first == "c_rescale_conserved") v = ke + pe + p0 * baro[2] - baro[3];
+    else if (col.first == "mc_acceptance") v = mc.acceptance();
+    else if (col.first == "mc_volume_step") v = mc.max_dv;
+    else if (col.first == "mc_conserved") v = ke + pe + mc.pressure * mc.volume - mc.injection;
     else v = ke + pe + sc[5];  // thermostat conserved quantity
     row += "," + fmt_e(U.to_user(col.second, v));
   }
@@ -1128,6 +1188,20 @@ void run_md(const Config& c, State& s, const Topology& t, const PhaseCfg& ph, ui
   if (ph.log_every) log.open(ph.log_path);
   uint64_t ndof = thermal_dof(s, t);
   auto cols = extra_columns(ph);
+  McState mc;
+  if (ph.barostat.kind == "monte-carlo") {
+    const Units& U = c.units;
+    uint64_t d25 = 25;
+    mc.on = true;
+    mc.pressure = param(ph.barostat, "pressure", PRESSURE, U);
+    mc.kt = param(ph.barostat, "temperature", TEMPERATURE, U);
+    mc.freq = get_u64(ph.barostat.params, "frequency", "phase.barostat", &d25);
+    if (mc.freq == 0) fail1("phase.barostat.frequency must be >= 1");
+    mc.seed = get_u64(ph.barostat.params, "seed", "phase.barostat");
+    mc.volume = s.lattice[0] * s.lattice[1] * s.lattice[2];
+    double dflt = 0.01 * mc.volume;
+    mc.max_dv = get_num(ph.barostat.params, "volume_step", "phase.barostat", &dflt);
+  }
   if (ph.log_every) {
     std::string head = "step,time,kinetic_energy,temperature";
     for (auto& col : cols) head += "," + col.first;
@@ -1136,7 +1210,7 @@ void run_md(const Config& c, State& s, const Topology& t, const PhaseCfg& ph, ui
   // forces, energies and the kinetic energy of the start
   e.check(hd_force_eval(e.ctx, 1), "force evaluation", 2);
   if (ph.log_every) {
-    log_row(log, c, ph, e, 0, ndof, cols);
+    log_row(log, c, ph, e, 0, ndof, cols, mc);
     ++rows;
   }
   if (ph.traj_every) {
@@ -1149,6 +1223,7 @@ void run_md(const Config& c, State& s, const Topology& t, const PhaseCfg& ph, ui
     uint64_t next = ph.n_steps;
     if (ph.log_every) next = std::min(next, (step / ph.log_every + 1) * ph.log_every);
     if (ph.traj_every) next = std::min(next, (step / ph.traj_every + 1) * ph.traj_every);
+    if (mc.on) next = std::min(next, (step / mc.freq + 1) * mc.freq);
     bool want_log = ph.log_every && next % ph.log_every == 0;
     bool want_frame = ph.traj_every && next % ph.traj_every == 0;
     double ts = now_ms();
@@ -1156,9 +1231,34 @@ void run_md(const Config& c, State& s, const Topology& t, const PhaseCfg& ph, ui
     e.check(hd_sync(e.ctx), "step", 2);
     clock.add("md_steps", now_ms() - ts);
     step = next;
+    if (mc.on && step % mc.freq == 0) {
+      // the move follows the dynamics step and precedes its output
+      // (runner.rs, rq-03a5a290)
+      ts = now_ms();
+      double out[4];
+      mc.counter += 1;
+      e.check(hd_mc_barostat_move(e.ctx, mc.pressure, mc.kt, mc.max_dv, mc.seed, mc.counter, out),
+              "monte-carlo barostat", 2);
+      bool acc = out[0] != 0.0;
+      mc.attempted++;
+      mc.n_att++;
+      if (acc) {
+        mc.accepted++;
+        mc.n_acc++;
+        mc.injection += mc.pressure * (out[2] - out[1]);
+      }
+      mc.volume = out[2];
+      if (mc.n_att >= 10) {
+        if ((double)mc.n_acc < 0.25 * (double)mc.n_att) mc.max_dv /= 1.1;
+        else if ((double)mc.n_acc > 0.75 * (double)mc.n_att)
+          mc.max_dv = std::min(mc.max_dv * 1.1, 0.3 * mc.volume);
+        mc.n_att = mc.n_acc = 0;
+      }
+      clock.add("mc_barostat", now_ms() - ts);
+    }
     if (want_log) {
       ts = now_ms();
-      log_row(log, c, ph, e, step, ndof, cols);
+      log_row(log, c, ph, e, step, ndof, cols, mc);
       ++rows;
       clock.add("log_write", now_ms() - ts);
     }

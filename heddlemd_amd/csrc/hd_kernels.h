@@ -333,6 +333,10 @@ extern "C" int hd_ctx_shake_log_view(hd_ctx* c, int* device, const float** x,
 void hd_launch_sd_move(hipStream_t st, float4* posq, const float* fx, const float* fy,
                        const float* fz, float4* save, float4* frame, float scale, unsigned n);
 void hd_launch_sd_restore(hipStream_t st, float4* posq, const float4* save, unsigned n);
+// MC barostat: molecule centres scaled by `scale` (rigid shift per molecule)
+void hd_launch_mc_scale_com(hipStream_t st, float4* posq, const unsigned* mol_off,
+                            const unsigned* mol_atoms, const float* mass, const float* lattice,
+                            float scale, unsigned n_mol);
 void hd_launch_fmax2(hipStream_t st, const float* fx, const float* fy, const float* fz,
                      unsigned n, double* out);
 struct HdDispArgs;

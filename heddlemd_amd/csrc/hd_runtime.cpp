@@ -117,6 +117,12 @@ struct hd_ctx {
   double* rs_injection = nullptr;            // cumulative K_new - K_old
   double* rs_partials = nullptr;             // CSVR partial sums
   bool baro_stochastic = true;  // C-rescale; false: Berendsen barostat
+  // Monte-Carlo barostat: molecule partition and the move's snapshots
+  std::vector<uint32_t> h_mol_off, h_mol_atoms;
+  unsigned *d_mol_off = nullptr, *d_mol_atoms = nullptr;
+  float4* mc_pos = nullptr;
+  float* mc_f = nullptr;     // 3n saved forces
+  double* mc_scal = nullptr;  // saved scalars
   // steepest descent
   float4* sd_save = nullptr;
   double* sd_fmax2 = nullptr;
@@ -2206,6 +2212,138 @@ static void enqueue_project_positions(hd_ctx* c) {
     HdShakeArgs sa = shake_args(c, 0.f);
     hd_launch_shake_positions(c->stream, sa);
   }
+}
+
+// Philox4x32-10 on the host (Salmon et al., SC11): the MC barostat's two
+// uniforms come from one block, as the reference's host mirror draws them
+// (src/integrator/philox.rs:21-73)
+static void philox_host(uint32_t k0, uint32_t k1, uint32_t c[4]) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)c[0] * 0xD2511F53u, p2 = (uint64_t)c[2] * 0xCD9E8D57u;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi2 = (uint32_t)(p2 >> 32), lo2 = (uint32_t)p2;
+    uint32_t n0 = hi2 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo2;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+int hd_set_molecules(hd_ctx* c, uint64_t n_mol, const uint32_t* offsets, const uint32_t* atoms) {
+  if (!c || !n_mol || !offsets || !atoms) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_molecules after hd_setup");
+  if (offsets[0] != 0 || offsets[n_mol] != c->n)
+    return fail(c, HD_ERR_ARG, "molecule offsets must run from 0 to n");
+  std::vector<char> seen(c->n, 0);
+  for (uint64_t m = 0; m < n_mol; ++m) {
+    if (offsets[m + 1] <= offsets[m]) return fail(c, HD_ERR_ARG, "molecule %llu is empty",
+                                                  (unsigned long long)m);
+    for (uint32_t k = offsets[m]; k < offsets[m + 1]; ++k) {
+      if (atoms[k] >= c->n || seen[atoms[k]])
+        return fail(c, HD_ERR_ARG, "molecule atoms must list every atom once");
+      seen[atoms[k]] = 1;
+    }
+  }
+  c->h_mol_off.assign(offsets, offsets + n_mol + 1);
+  c->h_mol_atoms.assign(atoms, atoms + c->n);
+  return HD_OK;
+}
+
+// One Monte-Carlo volume move (src/integrator/mc_barostat.rs:218-330):
+// U_old from a force evaluation at the current state; dV = max_dv (2 u1 - 1)
+// and u2 from Philox (seed, counter, 0, 0); early rejection when V_new <= 0
+// or the scaled box would be narrower than twice the search radius;
+// otherwise the box and the molecule centres scale by (V_new/V_old)^(1/3),
+// U_new is evaluated and the move kept when w = dU + P dV
+// - N_mol kT ln(V_new/V_old) <= 0 or u2 < exp(-w/kT); a rejected move
+// restores positions, forces, scalars and the box.  out[0] accepted,
+// out[1] V_old, out[2] V after the move, out[3] U_new - U_old (0 when
+// rejected early).
+int hd_mc_barostat_move(hd_ctx* c, double pressure, double kt, double max_dv, uint64_t seed,
+                        uint64_t counter, double out[4]) {
+  if (!c || !out || !(kt > 0) || !(max_dv > 0)) return HD_ERR_ARG;
+  if (!c->ready) return fail(c, HD_ERR_STATE, "hd_mc_barostat_move before hd_setup");
+  if (c->have_baro) return fail(c, HD_ERR_STATE, "a C-rescale or Berendsen barostat is set");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned n = (unsigned)c->n;
+  bool mol = !c->h_mol_off.empty();
+  unsigned n_mol = mol ? (unsigned)(c->h_mol_off.size() - 1) : n;
+  if (!c->mc_pos) {
+    DALLOC(c, c->mc_pos, c->n);
+    DALLOC(c, c->mc_f, 3 * c->n);
+    DALLOC(c, c->mc_scal, 8);
+    if (mol) {
+      DALLOC(c, c->d_mol_off, c->h_mol_off.size());
+      DALLOC(c, c->d_mol_atoms, c->h_mol_atoms.size());
+      int r;
+      if ((r = upload(c, c->d_mol_off, c->h_mol_off.data(), c->h_mol_off.size()))) return r;
+      if ((r = upload(c, c->d_mol_atoms, c->h_mol_atoms.data(), c->h_mol_atoms.size()))) return r;
+    }
+  }
+  // U_old at the current positions
+  enqueue_forces(c, 1, false, false);
+  c->forces_valid = true;
+  int r = check_status(c);
+  if (r) return r;
+  float lat[6];
+  double u_old = 0.0;
+  HIPCHK(c, hipMemcpyAsync(lat, c->d_lattice, sizeof(lat), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&u_old, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double v_old = (double)lat[0] * (double)lat[1] * (double)lat[2];
+  double min_w = std::min(std::min((double)lat[0], (double)lat[1]), (double)lat[2]);
+  uint32_t ctr[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u};
+  philox_host((uint32_t)seed, (uint32_t)(seed >> 32), ctr);
+  double u1 = ((double)ctr[0] + 0.5) / 4294967296.0, u2 = ((double)ctr[1] + 0.5) / 4294967296.0;
+  double dv = max_dv * (2.0 * u1 - 1.0);
+  double v_new = v_old + dv;
+  out[0] = 0.0;
+  out[1] = v_old;
+  out[2] = v_old;
+  out[3] = 0.0;
+  double r_search = c->rmax + c->r_skin;
+  if (v_new <= 0.0 || cbrt(v_new / v_old) * min_w < 2.0 * r_search) return HD_OK;
+  double scale = cbrt(v_new / v_old);
+  // snapshot, then the trial box and centres
+  HIPCHK(c, hipMemcpyAsync(c->mc_pos, c->posq, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mc_f, c->fx, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mc_f + n, c->fy, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mc_f + 2 * n, c->fz, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mc_scal, c->scal, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hd_launch_mc_scale_com(c->stream, c->posq, c->d_mol_off, mol ? c->d_mol_atoms : nullptr,
+                         c->mass, c->d_lattice, (float)scale, n_mol);
+  float lat_new[6];
+  for (int k = 0; k < 6; ++k) lat_new[k] = lat[k] * (float)scale;
+  unsigned force = NB_STATUS_FORCE;
+  HIPCHK(c, hipMemcpyAsync(c->d_lattice, lat_new, sizeof(lat_new), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ctrl, &force, sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+  enqueue_forces(c, 1, false, false);
+  if ((r = check_status(c))) return r;
+  double u_new = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&u_new, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double v_trial = (double)lat_new[0] * (double)lat_new[1] * (double)lat_new[2];
+  double w = (u_new - u_old) + pressure * dv - (double)n_mol * kt * log(v_new / v_old);
+  bool accept = w <= 0.0 || u2 < exp(-w / kt);
+  out[3] = u_new - u_old;
+  if (accept) {
+    for (int k = 0; k < 6; ++k) c->lattice[k] = (double)lat_new[k];
+    out[0] = 1.0;
+    out[2] = v_trial;
+    return HD_OK;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->posq, c->mc_pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->fx, c->mc_f, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->fy, c->mc_f + n, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->fz, c->mc_f + 2 * n, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->scal, c->mc_scal, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_lattice, lat, sizeof(lat), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ctrl, &force, sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
 }
 
 static int sd_eval(hd_ctx* c, double* energy, double* fmax) {

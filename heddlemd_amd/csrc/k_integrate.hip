@@ -692,6 +692,56 @@ __global__ void __launch_bounds__(1024) k_fmax2(const float* __restrict__ fx,
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// Monte-Carlo barostat trial move (src/integrator/mc_barostat.rs:218-330,
+// kernels/mc_barostat.cu): every molecule is shifted by (scale - 1) times
+// its centre of mass (taken relative to its first atom through minimum
+// images, in FP64), so molecules keep their shapes while their centres
+// scale with the box.  mol_atoms == null: every atom is its own molecule.
+__global__ void k_mc_scale_com(float4* __restrict__ posq, const unsigned* __restrict__ mol_off,
+                               const unsigned* __restrict__ mol_atoms,
+                               const float* __restrict__ mass, const float* __restrict__ lattice,
+                               float scale, unsigned n_mol) {
+  unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol) return;
+  unsigned lo = mol_atoms ? mol_off[m] : m, hi = mol_atoms ? mol_off[m + 1] : m + 1;
+  unsigned a0 = mol_atoms ? mol_atoms[lo] : m;
+  float4 ref = posq[a0];
+  HdLattice L = hd_load_lattice(lattice);
+  double cx = 0.0, cy = 0.0, cz = 0.0, mt = 0.0;
+  for (unsigned k = lo; k < hi; ++k) {
+    unsigned a = mol_atoms ? mol_atoms[k] : k;
+    float4 p = posq[a];
+    double dx = (double)p.x - (double)ref.x, dy = (double)p.y - (double)ref.y,
+           dz = (double)p.z - (double)ref.z;
+    hd_min_image_f64(dx, dy, dz, L);
+    double w = mass[a];
+    cx += w * dx;
+    cy += w * dy;
+    cz += w * dz;
+    mt += w;
+  }
+  double f = (double)scale - 1.0;
+  float sx = (float)(f * ((double)ref.x + cx / mt));
+  float sy = (float)(f * ((double)ref.y + cy / mt));
+  float sz = (float)(f * ((double)ref.z + cz / mt));
+  for (unsigned k = lo; k < hi; ++k) {
+    unsigned a = mol_atoms ? mol_atoms[k] : k;
+    float4 p = posq[a];
+    p.x += sx;
+    p.y += sy;
+    p.z += sz;
+    posq[a] = p;
+  }
+}
+
+void hd_launch_mc_scale_com(hipStream_t st, float4* posq, const unsigned* mol_off,
+                            const unsigned* mol_atoms, const float* mass, const float* lattice,
+                            float scale, unsigned n_mol) {
+  if (n_mol == 0) return;
+  hipLaunchKernelGGL(k_mc_scale_com, dim3((n_mol + 255) / 256), dim3(256), 0, st, posq, mol_off,
+                     mol_atoms, mass, lattice, scale, n_mol);
+}
+
 void hd_launch_sd_move(hipStream_t st, float4* posq, const float* fx, const float* fy,
                        const float* fz, float4* save, float4* frame, float scale, unsigned n) {
   hipLaunchKernelGGL(k_sd_move, dim3((n + 255) / 256), dim3(256), 0, st, posq, fx, fy, fz, save,

@@ -134,6 +134,20 @@ class Engine:
                                              float(step_decrease), out), "hd_minimize_step")
         return dict(energy=out[0], max_force=out[1], step=out[2], accepted=bool(out[3]))
 
+    def set_molecules(self, offsets, atoms):
+        off = np.ascontiguousarray(offsets, np.uint32)
+        at = np.ascontiguousarray(atoms, np.uint32)
+        self._chk(self._lib.hd_set_molecules(self._ctx, len(off) - 1, off, at),
+                  "hd_set_molecules")
+
+    def mc_barostat_move(self, pressure, kt, max_dv, seed, counter):
+        """One Monte-Carlo volume move: dict(accepted, v_old, v_new, du)."""
+        out = np.zeros(4)
+        self._chk(self._lib.hd_mc_barostat_move(self._ctx, float(pressure), float(kt),
+                                                float(max_dv), int(seed), int(counter), out),
+                  "hd_mc_barostat_move")
+        return dict(accepted=bool(out[0]), v_old=out[1], v_new=out[2], du=out[3])
+
     def set_spme_dft(self, mode):
         """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT."""
         self._chk(self._lib.hd_set_spme_dft(self._ctx, int(mode)), "hd_set_spme_dft")

@@ -182,6 +182,26 @@ int hd_get_langevin_counter(hd_ctx *ctx, uint64_t *counter);
  * seed.  tau <= 0 removes it.  Forces scalars (virial) on every step. */
 int hd_set_crescale(hd_ctx *ctx, double pressure, double tau, double compressibility, double kt,
                     uint64_t seed);
+/* Molecule partition for the Monte-Carlo barostat (MoleculeList,
+ * src/forces/topology.rs:1466-1560: connected components of bonds and
+ * constraint groups): molecule m holds atoms[offsets[m] .. offsets[m+1]).
+ * Without it every atom is its own molecule.  Before hd_setup. */
+int hd_set_molecules(hd_ctx *ctx, uint64_t n_mol, const uint32_t *offsets,
+                     const uint32_t *atoms);
+/* One Monte-Carlo volume move (src/integrator/mc_barostat.rs:218-330,
+ * kernels/mc_barostat.cu): U_old from a force evaluation at the current
+ * state; u1, u2 from Philox (key seed, counter (lo, hi, 0, 0)), the caller
+ * advancing the counter by one per move as the reference does; dV =
+ * max_dv (2 u1 - 1); rejected at once when V + dV <= 0 or the scaled box
+ * would be narrower than twice the search radius; otherwise the box and
+ * every molecule's centre of mass scale by ((V + dV)/V)^(1/3), U_new is
+ * evaluated and the move is kept when w = dU + P dV - N_mol kT ln(V'/V) <= 0
+ * or u2 < exp(-w/kT); a rejected move restores positions, forces, scalars
+ * and box.  out[0] 1 accepted / 0 not, out[1] V before, out[2] V after,
+ * out[3] U_new - U_old (0 when rejected early).  The adaptive step size,
+ * the move cadence and the acceptance statistics stay with the caller. */
+int hd_mc_barostat_move(hd_ctx *ctx, double pressure, double kt, double max_dv, uint64_t seed,
+                        uint64_t counter, double out[4]);
 /* Berendsen barostat (src/integrator/berendsen_barostat.rs,
  * kernels/barostat.cu:228-275): the C-rescale step without the noise term
  * (mu^3 = 1 - kappa dt/tau (P0 - P)), no draw and no P0 dV account.

@@ -643,3 +643,34 @@ def sd_converged(report, prev_energy, force_tolerance, energy_tolerance):
             abs(e - prev_energy) <= energy_tolerance * max(abs(e), abs(prev_energy), 1e-30):
         return "energy_tolerance"
     return None
+
+
+def mc_volume_draw(seed, counter, max_dv):
+    """The Monte-Carlo barostat's draw (src/integrator/mc_barostat.rs:263-275):
+    u1, u2 from one Philox4x32-10 block with counter (lo, hi, 0, 0); returns
+    (dV = max_dv (2 u1 - 1), u2)."""
+    o = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(seed & 0xFFFFFFFF, seed >> 32,
+                           np.array([counter & 0xFFFFFFFF, counter >> 32, 0, 0], np.uint32), o)
+    u1 = (float(o[0]) + 0.5) / 4294967296.0
+    u2 = (float(o[1]) + 0.5) / 4294967296.0
+    return max_dv * (2.0 * u1 - 1.0), u2
+
+
+def mc_scale_com(posq, mass, molecules, lattice, scale):
+    """The Monte-Carlo barostat's trial positions (kernels/mc_barostat.cu:
+    19-74): every molecule (a list of atom indices) shifted by (scale - 1)
+    times its centre of mass, the centre taken relative to its first atom
+    through minimum images, in double; returns new f32 positions."""
+    L = np.asarray(lattice[:3], np.float64)
+    out = np.array(posq, np.float32, copy=True)
+    f = float(scale) - 1.0
+    for atoms in molecules:
+        atoms = np.asarray(atoms)
+        ref = posq[atoms[0], :3].astype(np.float64)
+        d = posq[atoms, :3].astype(np.float64) - ref
+        d -= L * np.floor(d / L + 0.5)
+        m = np.asarray(mass, np.float64)[atoms]
+        com = ref + (m[:, None] * d).sum(0) / m.sum()
+        out[atoms, :3] = out[atoms, :3] + (f * com).astype(np.float32)
+    return out

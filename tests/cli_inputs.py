@@ -200,3 +200,104 @@ def rust_e(x, digits=9):
     leading zeros."""
     m, e = f"{x:.{digits}e}".split("e")
     return f"{m}e{int(e)}"
+
+
+FLEX_TOML = """schema_version = 1
+init = "flex.in.xyz"
+topology = "flex.in.topology"
+
+[simulation]
+seed = 3
+temperature = 300.0
+
+[[phase]]
+name = "npt"
+n_steps = {n_steps}
+dt = 5.0e-16
+[phase.integrator]
+kind = "velocity-verlet"
+[phase.thermostat]
+kind = "berendsen"
+temperature = 300.0
+tau = 1.0e-13
+[phase.barostat]
+kind = "monte-carlo"
+pressure = 1.0e5
+temperature = 300.0
+frequency = 5
+seed = 17
+[phase.output]
+trajectory_every = {n_steps}
+log_every = 5
+
+[[particle_types]]
+name = "O"
+mass = 2.6566e-26
+charge = -1.3137e-19
+
+[[particle_types]]
+name = "H"
+mass = 1.6735e-27
+charge = 6.5686e-20
+
+[[pair_interactions]]
+between = ["O", "O"]
+potential = "lennard-jones"
+sigma = 3.166e-10
+epsilon = 1.0794e-21
+cutoff = 9.0e-10
+r_switch = 9.0e-10
+
+[[pair_interactions]]
+between = ["H", "H"]
+potential = "lennard-jones"
+sigma = 1.0e-10
+epsilon = 0.0
+cutoff = 9.0e-10
+r_switch = 9.0e-10
+
+[[pair_interactions]]
+between = ["H", "O"]
+potential = "lennard-jones"
+sigma = 1.0e-10
+epsilon = 0.0
+cutoff = 9.0e-10
+r_switch = 9.0e-10
+
+[[bond_types]]
+name = "OH"
+potential = "harmonic"
+k = 502.0
+r0 = 1.0e-10
+
+[[angle_types]]
+name = "HOH"
+potential = "harmonic"
+k_theta = 5.2e-19
+theta_0 = 1.9106
+
+[spme]
+alpha = 3.4705e9
+r_cut_real = 9.0e-10
+grid = [{grid}, {grid}, {grid}]
+"""
+
+
+def write_flexible_water(d, n_steps=20):
+    """SPC water with harmonic O-H bonds and H-O-H angles instead of
+    constraints (the spc-water-256 example's form), Berendsen thermostat
+    and Monte-Carlo barostat."""
+    s = systems.config_water_8192()
+    names = ["O", "H", "H"] * (s.n // 3)
+    write_xyz(os.path.join(d, "flex.in.xyz"), names, s.posq[:, :3], s.lattice)
+    with open(os.path.join(d, "flex.in.topology"), "w") as f:
+        f.write("[bonds]\n")
+        for g in s.water_groups:
+            f.write(f"{g[0]} {g[1]} OH\n{g[0]} {g[2]} OH\n")
+        f.write("[angles]\n")
+        for g in s.water_groups:
+            f.write(f"{g[1]} {g[0]} {g[2]} HOH\n")
+    cfg = os.path.join(d, "flex.in.toml")
+    with open(cfg, "w") as f:
+        f.write(FLEX_TOML.format(n_steps=n_steps, grid=s.spme.grid[0]))
+    return cfg, s

@@ -13,7 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from heddlemd_amd import units as U  # noqa: E402
-from tests.cli_inputs import BIN, run_cli, rust_e, write_argon, write_water  # noqa: E402
+from tests.cli_inputs import (BIN, run_cli, rust_e, write_argon, write_flexible_water,  # noqa: E402
+                              write_water)
 
 
 def say(*a):
@@ -109,7 +110,7 @@ def test_water_bundle_minimize_nvt_npt(tmp_path, solver):
     assert eq[0] == "step,time,kinetic_energy,temperature,csvr_conserved"
     pr = open(os.path.join(str(tmp_path), "water.out.prod.log")).read().splitlines()
     assert pr[0] == ("step,time,kinetic_energy,temperature,csvr_conserved,pressure,"
-                     "box_volume,crescale_conserved")
+                     "box_volume,c_rescale_conserved")
     temps = [float(x.split(",")[3]) for x in eq[1:] + pr[1:]]
     assert all(50.0 < t < 600.0 for t in temps), temps
     frames = _read_xyz_frames(os.path.join(str(tmp_path), "water.out.prod.xyz"))
@@ -130,3 +131,29 @@ def test_water_bundle_minimize_nvt_npt(tmp_path, solver):
     say(f"  final O-H {r_oh.min():.9e} .. {r_oh.max():.9e} m")
     assert (r_oh.max() - r_oh.min()) / r_oh.mean() < 2e-5
     assert np.abs(r_oh / 1.0e-10 - 1).max() < 1e-3
+
+
+def test_flexible_water_bonds_angles_berendsen_monte_carlo(tmp_path):
+    # the spc-water-256 form: bonds and angles from the topology (their 1-2
+    # and 1-3 pairs excluded), molecules from the bonds for the Monte-Carlo
+    # barostat, a Berendsen thermostat
+    say("case: heddlemd run flexible water 8192: bonds + angles, Berendsen, MC barostat")
+    cfg, s = write_flexible_water(str(tmp_path), n_steps=20)
+    r = run_cli("lint", cfg, "--with-gpu")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "8192 angles" in r.stdout and "16384 bonds" in r.stdout
+    r = run_cli("run", cfg, timeout=600)
+    say("  " + (r.stdout + r.stderr).strip().replace("\n", "\n  "))
+    assert r.returncode == 0
+    log = open(os.path.join(str(tmp_path), "flex.out.npt.log")).read().splitlines()
+    assert log[0] == ("step,time,kinetic_energy,temperature,berendsen_conserved,box_volume,"
+                      "mc_acceptance,mc_volume_step,mc_conserved")
+    rows = [x.split(",") for x in log[1:]]
+    assert [x[0] for x in rows] == ["0", "5", "10", "15", "20"]
+    acc = [float(x[6]) for x in rows[1:]]
+    assert all(0.0 <= a <= 1.0 for a in acc)
+    vols = [float(x[5]) for x in rows]
+    L = s.lattice[:3]
+    assert vols[0] == pytest.approx(float(np.prod(np.float64(L))), rel=1e-6)
+    temps = [float(x[3]) for x in rows]
+    assert all(np.isfinite(temps)) and max(temps) < 2000.0

@@ -1,6 +1,7 @@
-"""CSVR and Berendsen thermostats and the Berendsen barostat on the GPU,
-through the C ABI, against the oracle's restatement of the reference's
-steps (src/integrator/csvr.rs, berendsen.rs, berendsen_barostat.rs)."""
+"""CSVR and Berendsen thermostats, the Berendsen barostat and the
+Monte-Carlo barostat on the GPU, through the C ABI, against the oracle's
+restatement of the reference's steps (src/integrator/csvr.rs,
+berendsen.rs, berendsen_barostat.rs, mc_barostat.rs)."""
 import sys
 
 import numpy as np
@@ -123,3 +124,87 @@ def test_one_thermostat_per_context():
     with pytest.raises(HeddleError):
         engine_for(s, dt=U.time(2e-15), nhc_kt=kt, nhc_tau=U.time(1e-13),
                    csvr=dict(kt=kt, tau=U.time(1e-13), seed=1))
+
+
+def test_mc_barostat_moves_match_the_oracle(oracle):
+    # 12 moves on jittered argon (every atom a molecule) with counters 1..12:
+    # the volume proposal from the oracle's Philox draw, the trial positions
+    # from the oracle's centre scaling, U_new - U_old against the oracle's
+    # energies, and the Metropolis decision from those numbers
+    say("case: argon 4000, 12 Monte-Carlo volume moves")
+    s = _jittered_argon(n_atoms=4000, seed=21)
+    kt = U.temperature(120.0)
+    p0 = U.pressure(1.0e8)
+    e = engine_for(s, dt=U.time(5e-15))
+    lj = s.lj.as_dict()
+
+    def energy(p, L):
+        return float(oracle.pair_forces(p, s.types, lj, L)[1].sum())
+
+    n_acc = 0
+    for ctr in range(1, 13):
+        L = e.lattice()
+        pos = e.positions()
+        v_old = float(np.float32(L[0])) * float(np.float32(L[1])) * float(np.float32(L[2]))
+        max_dv = 0.004 * v_old
+        r = e.mc_barostat_move(p0, kt, max_dv, 77, ctr)
+        dv, u2 = oracle.mc_volume_draw(77, ctr, max_dv)
+        assert r["v_old"] == pytest.approx(v_old, rel=1e-12)
+        scale = np.cbrt((v_old + dv) / v_old)
+        L_new = np.float32(L[:3]) * np.float32(scale)
+        trial = oracle.mc_scale_com(pos, s.mass, [[i] for i in range(s.n)], L, scale)
+        Lt = np.array([*L_new.astype(np.float64), 0, 0, 0])
+        du = energy(trial, Lt) - energy(pos, L)
+        # (a pair within rounding of the cutoff may count on one side only)
+        assert r["du"] == pytest.approx(du, rel=1e-4, abs=5e-5)
+        w = r["du"] + p0 * dv - s.n * kt * np.log((v_old + dv) / v_old)
+        assert r["accepted"] == (w <= 0 or u2 < np.exp(-w / kt)), ctr
+        after = e.positions()
+        if r["accepted"]:
+            n_acc += 1
+            assert np.abs(after[:, :3].astype(np.float64) - trial[:, :3]).max() < 1e-5
+            assert np.allclose(e.lattice()[:3], L_new, rtol=1e-7)
+        else:
+            assert after.tobytes() == pos.tobytes()
+            assert np.array_equal(e.lattice(), L)
+    e.step(5)  # the dynamics carry on from the last state
+    e.close()
+    say(f"  accepted {n_acc} of 12")
+    assert n_acc > 0
+
+
+def test_mc_barostat_keeps_molecules_rigid():
+    say("case: water 8192 (SETTLE), 6 Monte-Carlo volume moves on molecule centres")
+    s = systems.config_water_8192()
+    g = s.water_groups
+    off = np.arange(0, s.n + 1, 3, dtype=np.uint32)
+    from heddlemd_amd.engine import Engine
+    e = Engine(0)
+    e.set_system(s.posq, s.mass, s.types, s.excl_key)
+    e.set_lattice(s.lattice)
+    e.set_lj(s.lj)
+    e.set_spme(s.spme)
+    d_oh, d_hh = s.water_geom
+    e.set_settle(g, d_oh, d_hh, float(s.mass[g[0, 0]]), float(s.mass[g[0, 1]]))
+    e.set_molecules(off, g.reshape(-1))
+    e.set_integrator(U.time(2e-15))
+    e.setup()
+    kt = U.temperature(300.0)
+    pos0 = e.positions()
+    moved = 0
+    for ctr in range(1, 7):
+        L = e.lattice()
+        r = e.mc_barostat_move(U.pressure(1e5), kt, 0.02 * L[0] * L[1] * L[2], 5, ctr)
+        moved += r["accepted"]
+    pos = e.positions()
+    L = e.lattice()
+    e.close()
+
+    def oh(p):
+        d = p[g[:, 0], :3].astype(np.float64) - p[g[:, 1], :3]
+        d -= L[:3] * np.round(d / L[:3])
+        return np.linalg.norm(d, axis=1)
+
+    say(f"  accepted {moved} of 6; O-H drift {np.abs(oh(pos) - oh(pos0)).max():.2e} Bohr")
+    assert moved > 0
+    assert np.abs(oh(pos) - oh(pos0)).max() < 2e-5

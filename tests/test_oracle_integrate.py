@@ -431,3 +431,26 @@ def test_steepest_descent_on_a_quadratic_well(oracle):
                                1e-7) == "energy_tolerance"
     assert oracle.sd_converged(dict(max_force=1.0, energy=1.0, accepted=False), 1.0, 0,
                                1e-7) is None
+
+
+def test_mc_volume_move_pieces(oracle):
+    # the draw: dV uniform in (-max, max) from Philox lane 0, u2 from lane 1
+    dvs = np.array([oracle.mc_volume_draw(5, c, 2.0)[0] for c in range(1, 2001)])
+    assert np.abs(dvs).max() < 2.0 and abs(dvs.mean()) < 0.1 and dvs.std() == pytest.approx(
+        2.0 / np.sqrt(3.0), rel=0.05)
+    o = oracle.philox4x32_10((5, 0), (7, 0, 0, 0))
+    dv, u2 = oracle.mc_volume_draw(5, 7, 2.0)
+    assert dv == pytest.approx(2.0 * (2 * (o[0] + 0.5) / 2 ** 32 - 1), rel=1e-15)
+    assert u2 == pytest.approx((o[1] + 0.5) / 2 ** 32, rel=1e-15)
+    # centre scaling: a molecule split across a face keeps its shape, its
+    # centre of mass moves to scale x centre
+    L = np.array([20.0, 20.0, 20.0, 0, 0, 0])
+    p = np.array([[9.5, 1.0, 2.0, 0], [-9.7, 1.0, 2.0, 0], [3.0, 3.0, 3.0, 0]], np.float32)
+    m = np.array([2.0, 1.0, 1.0])
+    q = oracle.mc_scale_com(p, m, [[0, 1], [2]], L, 1.1)
+    d0 = p[1, :3] - p[0, :3]
+    d1 = q[1, :3] - q[0, :3]
+    assert np.allclose(d0, d1, atol=1e-5)
+    com = (2 * 9.5 + 1 * (20 - 9.7)) / 3.0
+    assert q[0, 0] - p[0, 0] == pytest.approx(0.1 * com, rel=1e-6)
+    assert np.allclose(q[2, :3], 1.1 * p[2, :3], rtol=1e-6)
