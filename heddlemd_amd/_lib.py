@@ -50,6 +50,7 @@ SIGNATURES = {
     "hd_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "hd_set_exclusion_span": (C.c_int, [_ctx, C.c_uint32]),
     "hd_set_angles": (C.c_int, [_ctx, C.c_uint64, _u32p, _f64p, _f64p]),
+    "hd_set_andersen": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64]),
     "hd_set_csvr": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64, C.c_uint64]),
     "hd_set_berendsen": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_uint64]),
     "hd_get_thermostat": (C.c_int, [_ctx, _f64p]),

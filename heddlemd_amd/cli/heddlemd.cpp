@@ -496,7 +496,7 @@ Config load_config(const std::string& path) {
       fail1("phase `" + ph.name + "`: unknown " + slot + " kind `" + sl.kind + "`");
     };
     check(ph.integrator, "integrator", {"velocity-verlet", "langevin-baoab"}, {"mtk-npt"});
-    check(ph.thermostat, "thermostat", {"nose-hoover-chain", "csvr", "berendsen"}, {"andersen"});
+    check(ph.thermostat, "thermostat", {"nose-hoover-chain", "csvr", "berendsen", "andersen"}, {});
     check(ph.barostat, "barostat", {"c-rescale", "berendsen", "monte-carlo"}, {});
   }
   std::sort(c.phases.begin(), c.phases.end(),
@@ -1067,8 +1067,15 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
         e.check(hd_set_berendsen(e.ctx, param(th, "temperature", TEMPERATURE, U), param(th, "tau", TIME, U),
                                  t.n_constraints),
                 "thermostat", 1);
+      } else if (th.kind == "andersen") {
+        check_keys(th.params, {"kind", "temperature", "collision_rate", "seed"}, "phase.thermostat");
+        e.check(hd_set_andersen(e.ctx, param(th, "temperature", TEMPERATURE, U),
+                                param(th, "collision_rate", INV_TIME, U),
+                                get_u64(th.params, "seed", "phase.thermostat")),
+                "thermostat", 1);
       } else {
-        fail1("phase.thermostat: kind `" + th.kind + "` is not built (nose-hoover-chain, csvr, berendsen)");
+        fail1("phase.thermostat: kind `" + th.kind +
+              "` is not built (nose-hoover-chain, csvr, berendsen, andersen)");
       }
     }
     const Slot& ba = ph.barostat;
@@ -1121,6 +1128,7 @@ std::vector<std::pair<std::string, Dim>> extra_columns(const PhaseCfg& ph) {
   if (ph.thermostat.kind == "nose-hoover-chain") cols.push_back({"nhc_conserved", ENERGY});
   if (ph.thermostat.kind == "csvr") cols.push_back({"csvr_conserved", ENERGY});
   if (ph.thermostat.kind == "berendsen") cols.push_back({"berendsen_conserved", ENERGY});
+  if (ph.thermostat.kind == "andersen") cols.push_back({"andersen_conserved", ENERGY});
   if (ph.barostat.kind == "c-rescale" || ph.barostat.kind == "berendsen") {
     cols.push_back({"pressure", PRESSURE});
     cols.push_back({"box_volume", NONE});

@@ -339,6 +339,10 @@ void hd_launch_mc_scale_com(hipStream_t st, float4* posq, const unsigned* mol_of
                             float scale, unsigned n_mol);
 void hd_launch_fmax2(hipStream_t st, const float* fx, const float* fy, const float* fz,
                      unsigned n, double* out);
+// Andersen collisions after the post-force kick, then the draw counter + 1
+void hd_launch_andersen(hipStream_t st, float* vx, float* vy, float* vz, const float* mass,
+                        unsigned long long* counter, unsigned long long seed, float p, float kt,
+                        unsigned n);
 struct HdDispArgs;
 struct HdRescaleArgs {
   int kind;                        // 2 CSVR, 3 Berendsen

@@ -111,6 +111,11 @@ struct hd_ctx {
   bool have_lan = false;  // Langevin BAOAB instead of velocity Verlet
   // velocity-rescale thermostats: 0 none, 2 CSVR, 3 Berendsen
   int rescale = 0;
+  // Andersen collisions (a thermostat of its own kind)
+  bool have_andersen = false;
+  double and_kt = 0, and_rate = 0;
+  uint64_t and_seed = 0;
+  unsigned long long* and_counter = nullptr;
   double rs_kt = 0, rs_tau = 0;
   uint64_t rs_seed = 0, rs_nc = 0;
   unsigned long long* rs_counter = nullptr;  // CSVR draw counter
@@ -932,6 +937,14 @@ void enqueue_step(hd_ctx* c, bool ev, bool timed) {
   int mode = ev ? 1 : (c->have_baro ? 2 : 0);
   enqueue_forces(c, mode, true, timed, disp_done, c->have_baro);
   if (c->have_baro) enqueue_crescale(c, timed);
+  if (c->have_andersen) {
+    // the collision probability is formed in f32 (andersen.rs:160)
+    float p = (float)std::min(std::max(c->and_rate * (double)(float)c->dt, 0.0), 1.0);
+    stage_begin(c, "andersen", timed);
+    hd_launch_andersen(st, c->vx, c->vy, c->vz, c->mass, c->and_counter, c->and_seed, p,
+                       (float)c->and_kt, n);
+    stage_end(c, timed);
+  }
   if (ev) hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 0,
                        nullptr, 0.f, nullptr);
 }
@@ -1706,7 +1719,7 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
     c->have_nhc = false;
     return HD_OK;
   }
-  if (c->have_lan || c->rescale)
+  if (c->have_lan || c->rescale || c->have_andersen)
     return fail(c, HD_ERR_STATE, "one thermostat per context: no Nose-Hoover chain beside it");
   if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
       !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
@@ -1738,9 +1751,9 @@ static int set_rescale(hd_ctx* c, int kind, double kt, double tau, uint64_t seed
   if (!c) return HD_ERR_ARG;
   if (!(kt > 0) || !(tau > 0) || !std::isfinite(kt) || !std::isfinite(tau))
     return fail(c, HD_ERR_ARG, "thermostat temperature and tau must be finite and > 0");
-  if (c->have_nhc || c->have_lan)
-    return fail(c, HD_ERR_STATE, "one thermostat per context: a Nose-Hoover chain or Langevin "
-                                 "BAOAB is already set");
+  if (c->have_nhc || c->have_lan || c->have_andersen)
+    return fail(c, HD_ERR_STATE, "one thermostat per context: a Nose-Hoover chain, Andersen or "
+                                 "Langevin BAOAB is already set");
   if (c->ready) {
     // a new phase re-parameterises the same kind; the counter and the
     // injection account carry on
@@ -1757,6 +1770,24 @@ static int set_rescale(hd_ctx* c, int kind, double kt, double tau, uint64_t seed
   return HD_OK;
 }
 
+int hd_set_andersen(hd_ctx* c, double kt, double collision_rate, uint64_t seed) {
+  if (!c) return HD_ERR_ARG;
+  if (!(kt > 0) || !(collision_rate >= 0) || !std::isfinite(kt) || !std::isfinite(collision_rate))
+    return fail(c, HD_ERR_ARG, "Andersen temperature must be > 0 and the collision rate >= 0");
+  if (c->have_nhc || c->have_lan || c->rescale)
+    return fail(c, HD_ERR_STATE, "one thermostat per context");
+  if (c->ready) {
+    if (!c->have_andersen) return fail(c, HD_ERR_STATE, "the thermostat kind is fixed at hd_setup");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  c->have_andersen = true;
+  c->and_kt = kt;
+  c->and_rate = collision_rate;
+  c->and_seed = seed;
+  return HD_OK;
+}
+
 int hd_set_csvr(hd_ctx* c, double kt, double tau, uint64_t seed, uint64_t n_constraints) {
   return set_rescale(c, 2, kt, tau, seed, n_constraints);
 }
@@ -1768,6 +1799,13 @@ int hd_set_berendsen(hd_ctx* c, double kt, double tau, uint64_t n_constraints) {
 int hd_get_thermostat(hd_ctx* c, double out[2]) {
   if (!c || !c->ready || !out) return fail(c, HD_ERR_STATE, "not set up");
   out[0] = out[1] = 0.0;
+  if (c->have_andersen) {
+    unsigned long long ctr = 0;
+    HIPCHK(c, hipMemcpyAsync(&ctr, c->and_counter, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[1] = (double)ctr;
+    return HD_OK;
+  }
   if (!c->rescale) return HD_OK;
   unsigned long long ctr = 0;
   HIPCHK(c, hipMemcpyAsync(out, c->rs_injection, sizeof(double), hipMemcpyDeviceToHost,
@@ -1782,7 +1820,7 @@ int hd_set_langevin(hd_ctx* c, double friction, double kt, uint64_t seed) {
   if (!c) return HD_ERR_ARG;
   if (!(friction > 0) || !(kt > 0) || !std::isfinite(friction) || !std::isfinite(kt))
     return fail(c, HD_ERR_ARG, "Langevin friction and temperature must be finite and > 0");
-  if (c->have_nhc || c->rescale)
+  if (c->have_nhc || c->rescale || c->have_andersen)
     return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: remove the other thermostat");
   if (c->lossless) return fail(c, HD_ERR_UNSUPPORTED, "Langevin BAOAB has no lossless mode");
   if (c->ready) {
@@ -2073,6 +2111,10 @@ int hd_setup(hd_ctx* c) {
     if ((r = upload(c, c->d_bonds, c->h_bonds.data(), c->h_bonds.size()))) return r;
     if ((r = upload(c, c->d_bond_kind, c->h_bond_kind.data(), c->h_bond_kind.size()))) return r;
     if ((r = upload(c, c->d_bond_prm, c->h_bond_prm.data(), c->h_bond_prm.size()))) return r;
+  }
+  if (c->have_andersen) {
+    if (c->have_lan) return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat");
+    DALLOC(c, c->and_counter, 1);
   }
   if (c->rescale) {
     if (c->have_lan) return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat");

@@ -1048,6 +1048,42 @@ void hd_launch_ke_rescale(hipStream_t st, unsigned n, const float* vx, const flo
 }
 
 // ---------------------------------------------------------------------
+// Andersen thermostat (src/integrator/andersen.rs, kernels/andersen.cu):
+// after the post-force kick, atom i collides with probability p: u from
+// Philox (seed, counter, i, 3) lane 0 mapped to (0, 1); on a collision its
+// velocity is redrawn as sqrt(kT/m) (f32) times the Philox normals of
+// (seed, counter, i, 0..2).  The counter advances once per step, after.
+__global__ void k_andersen(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+                           const float* __restrict__ mass, const unsigned long long* counter,
+                           unsigned seed_lo, unsigned seed_hi, float p, float kt, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned long long ctr = *counter;
+  unsigned clo = (unsigned)(ctr & 0xFFFFFFFFull), chi = (unsigned)(ctr >> 32);
+  unsigned o[4];
+  philox4x32_10(seed_lo, seed_hi, clo, chi, i, 3u, o);
+  double u = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+  if (u >= (double)p) return;
+  float sigma = sqrtf(kt / mass[i]);
+  vx[i] = sigma * (float)philox_gaussian_f64(seed_lo, seed_hi, clo, chi, i, 0u);
+  vy[i] = sigma * (float)philox_gaussian_f64(seed_lo, seed_hi, clo, chi, i, 1u);
+  vz[i] = sigma * (float)philox_gaussian_f64(seed_lo, seed_hi, clo, chi, i, 2u);
+}
+
+__global__ void k_counter_advance(unsigned long long* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter += 1ull;
+}
+
+void hd_launch_andersen(hipStream_t st, float* vx, float* vy, float* vz, const float* mass,
+                        unsigned long long* counter, unsigned long long seed, float p, float kt,
+                        unsigned n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_andersen, dim3((n + 255) / 256), dim3(256), 0, st, vx, vy, vz, mass,
+                     counter, (unsigned)(seed & 0xFFFFFFFFull), (unsigned)(seed >> 32), p, kt, n);
+  hipLaunchKernelGGL(k_counter_advance, dim3(1), dim3(64), 0, st, counter);
+}
+
+// ---------------------------------------------------------------------
 // SHAKE / RATTLE for general constraint groups (the constraint contract of
 // kernels/shake.cu:96-490 and src/integrator/shake.rs: Gauss-Seidel
 // sweeps, gradients from the pre-drift snapshot, the same tolerances and

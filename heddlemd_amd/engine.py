@@ -106,6 +106,10 @@ class Engine:
         self._chk(self._lib.hd_set_csvr(self._ctx, float(kt), float(tau), int(seed),
                                         int(n_constraints)), "hd_set_csvr")
 
+    def set_andersen(self, kt, collision_rate, seed):
+        self._chk(self._lib.hd_set_andersen(self._ctx, float(kt), float(collision_rate),
+                                            int(seed)), "hd_set_andersen")
+
     def set_berendsen(self, kt, tau, n_constraints=0):
         self._chk(self._lib.hd_set_berendsen(self._ctx, float(kt), float(tau),
                                              int(n_constraints)), "hd_set_berendsen")
@@ -418,7 +422,7 @@ class Engine:
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
                velocities=None, crescale=None, langevin=None, device=0, pair_list=None,
-               csvr=None, berendsen=None, berendsen_barostat=None):
+               csvr=None, berendsen=None, berendsen_barostat=None, andersen=None):
     """Configure an Engine for a ``systems.System``; ``langevin`` =
     dict(friction, kt, seed) selects the BAOAB integrator; ``pair_list``
     selects the neighbour-list form (hd_set_pair_list)."""
@@ -460,6 +464,8 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_csvr(n_constraints=system.n_constraints, **csvr)
     if berendsen is not None:
         e.set_berendsen(n_constraints=system.n_constraints, **berendsen)
+    if andersen is not None:
+        e.set_andersen(**andersen)
     if berendsen_barostat is not None:
         e.set_berendsen_barostat(**berendsen_barostat)
     if crescale is not None:

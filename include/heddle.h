@@ -162,6 +162,13 @@ int hd_set_csvr(hd_ctx *ctx, double kt, double tau, uint64_t seed, uint64_t n_co
  * nose_hoover.cu:390-417): factor sqrt(max(0, 1 + dt/tau (K_target/K - 1)))
  * on the post-force half kick, K_target = g kT / 2. */
 int hd_set_berendsen(hd_ctx *ctx, double kt, double tau, uint64_t n_constraints);
+/* Andersen thermostat (src/integrator/andersen.rs, kernels/andersen.cu):
+ * after the post-force kick every atom collides with probability
+ * p = clamp(rate dt, 0, 1) (f32) and then takes a fresh Maxwell-Boltzmann
+ * velocity; Philox key seed, counter = the device draw counter (one per
+ * step, read by hd_get_thermostat out[1]).  The conserved-quantity column
+ * is K + U.  One thermostat per context. */
+int hd_set_andersen(hd_ctx *ctx, double kt, double collision_rate, uint64_t seed);
 /* CSVR / Berendsen state: out[0] cumulative energy the rescales injected
  * (the conserved quantity is K + U - out[0]), out[1] CSVR draw counter. */
 int hd_get_thermostat(hd_ctx *ctx, double out[2]);

@@ -1348,6 +1348,27 @@ double or_berendsen_factor(double k, double k_target, double dt_over_tau, double
     return sqrt(l2);
 }
 
+/* Andersen collisions (kernels/andersen.cu:13-60,                        */
+/* src/integrator/andersen.rs:276-300): u = ((x0 + 0.5) 2^-32) of the      */
+/* Philox block (seed, counter, i, 3); when u < p the velocity becomes      */
+/* sqrt(kT/m) (f32) times the normals of (seed, counter, i, 0..2).          */
+void or_andersen(int n, float *vx, float *vy, float *vz, const float *mass, uint64_t seed,
+                 uint64_t counter, float p, float kt)
+{
+    uint32_t slo = (uint32_t)(seed & 0xFFFFFFFFu), shi = (uint32_t)(seed >> 32);
+    uint32_t clo = (uint32_t)(counter & 0xFFFFFFFFu), chi = (uint32_t)(counter >> 32);
+    for (int i = 0; i < n; ++i) {
+        uint32_t ctr[4] = {clo, chi, (uint32_t)i, 3u}, o[4];
+        or_philox4x32_10(slo, shi, ctr, o);
+        double u = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+        if (u >= (double)p) continue;
+        float sigma = sqrtf(kt / mass[i]);
+        vx[i] = sigma * (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 0u);
+        vy[i] = sigma * (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 1u);
+        vz[i] = sigma * (float)or_philox_gaussian(slo, shi, clo, chi, (uint32_t)i, 2u);
+    }
+}
+
 float or_lan_alpha(float friction, float dt)
 {
     /* alpha = exp(-gamma dt), f32 on the host (langevin_baoab.rs, "O") */

@@ -107,6 +107,9 @@ def lib():
         L.or_berendsen_factor.restype = C.c_double
         L.or_berendsen_factor.argtypes = [C.c_double, C.c_double, C.c_double,
                                           C.POINTER(C.c_double)]
+        L.or_andersen.restype = None
+        L.or_andersen.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, C.c_uint64, C.c_uint64,
+                                  C.c_float, C.c_float]
         L.or_kinetic_energy.restype = C.c_double
         L.or_kinetic_energy.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.or_num_threads.restype = C.c_int
@@ -513,6 +516,12 @@ def berendsen_factor(k, k_target, dt_over_tau):
     inj = C.c_double(0.0)
     f = lib().or_berendsen_factor(float(k), float(k_target), float(dt_over_tau), C.byref(inj))
     return f, inj.value
+
+
+def andersen(v, mass, seed, counter, p, kt):
+    """Andersen collisions in place on v (3, n) f32 (kernels/andersen.cu)."""
+    lib().or_andersen(v.shape[1], v[0], v[1], v[2], np.ascontiguousarray(mass, np.float32),
+                      int(seed), int(counter), float(p), float(kt))
 
 
 def kinetic_energy(v, mass):

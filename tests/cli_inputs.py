@@ -43,7 +43,7 @@ n_steps = {n_steps}
 dt = 5.0e-15
 [phase.integrator]
 kind = "velocity-verlet"
-[phase.output]
+{thermo}[phase.output]
 trajectory_every = {traj_every}
 log_every = {log_every}
 
@@ -61,12 +61,21 @@ r_switch = 8.5e-10
 """
 
 
-def write_argon(d, n_steps=20, traj_every=10, log_every=5):
+ANDERSEN = """[phase.thermostat]
+kind = "andersen"
+temperature = 120.0
+collision_rate = 5.0e12
+seed = 4
+"""
+
+
+def write_argon(d, n_steps=20, traj_every=10, log_every=5, thermo=""):
     s = systems.config_argon()
     write_xyz(os.path.join(d, "argon.in.xyz"), ["Ar"] * s.n, s.posq[:, :3], s.lattice)
     cfg = os.path.join(d, "argon.in.toml")
     with open(cfg, "w") as f:
-        f.write(ARGON_TOML.format(n_steps=n_steps, traj_every=traj_every, log_every=log_every))
+        f.write(ARGON_TOML.format(n_steps=n_steps, traj_every=traj_every, log_every=log_every,
+                                  thermo=thermo))
     return cfg, s
 
 

@@ -13,7 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from heddlemd_amd import units as U  # noqa: E402
-from tests.cli_inputs import (BIN, run_cli, rust_e, write_argon, write_flexible_water,  # noqa: E402
+from tests.cli_inputs import (ANDERSEN, BIN, run_cli, rust_e, write_argon,  # noqa: E402
+                              write_flexible_water,
                               write_water)
 
 
@@ -157,3 +158,15 @@ def test_flexible_water_bonds_angles_berendsen_monte_carlo(tmp_path):
     assert vols[0] == pytest.approx(float(np.prod(np.float64(L))), rel=1e-6)
     temps = [float(x[3]) for x in rows]
     assert all(np.isfinite(temps)) and max(temps) < 2000.0
+
+
+def test_argon_andersen_run(tmp_path):
+    say("case: heddlemd run argon 10000 with an Andersen thermostat, 40 steps")
+    cfg, s = write_argon(str(tmp_path), n_steps=40, traj_every=40, log_every=10, thermo=ANDERSEN)
+    r = run_cli("run", cfg, timeout=300)
+    say("  " + (r.stdout + r.stderr).strip().replace("\n", "\n  "))
+    assert r.returncode == 0
+    log = open(os.path.join(str(tmp_path), "argon.out.nve.log")).read().splitlines()
+    assert log[0] == "step,time,kinetic_energy,temperature,andersen_conserved"
+    temps = [float(x.split(",")[3]) for x in log[1:]]
+    assert len(temps) == 5 and all(50.0 < t < 400.0 for t in temps)

@@ -208,3 +208,33 @@ def test_mc_barostat_keeps_molecules_rigid():
     say(f"  accepted {moved} of 6; O-H drift {np.abs(oh(pos) - oh(pos0)).max():.2e} Bohr")
     assert moved > 0
     assert np.abs(oh(pos) - oh(pos0)).max() < 2e-5
+
+
+def test_andersen_collisions_bit_exact_with_the_oracle(oracle):
+    # force-free gas: each step the kick and drift leave v alone and the
+    # collisions redraw about rate x dt of the atoms; the device's draws
+    # equal the oracle's word for word over 30 steps
+    say("case: argon 4000 force-free Andersen 30 steps")
+    s = _gas(7)
+    kt = U.temperature(150.0)
+    dt = np.float32(U.time(2e-15))
+    rate = 1.0 / U.time(2e-13)
+    rng = np.random.default_rng(3)
+    v0 = (rng.normal(size=(3, s.n)) * np.sqrt(2.0 * kt / s.mass)).astype(np.float32)
+    e = engine_for(s, dt=float(dt), velocities=v0, andersen=dict(kt=kt, collision_rate=rate,
+                                                                 seed=99))
+    e.force_eval(scalars=False)
+    e.step(30, scalars_last=True)
+    v1 = e.velocities()
+    th = e.thermostat()
+    e.close()
+    v = v0.copy()
+    p = np.float32(min(max(rate * float(dt), 0.0), 1.0))
+    for step in range(30):
+        oracle.andersen(v, s.mass, 99, step, p, np.float32(kt))
+    changed = np.mean(np.any(v != v0, axis=0))
+    say(f"  p = {p:.4f}, atoms redrawn {changed:.3f} (1 - (1-p)^30 = {1 - (1 - p) ** 30:.3f}), "
+        f"draws {th['draws']}")
+    assert th["draws"] == 30
+    assert v1.tobytes() == v.tobytes()
+    assert abs(changed - (1 - (1 - p) ** 30)) < 0.05

@@ -454,3 +454,23 @@ def test_mc_volume_move_pieces(oracle):
     com = (2 * 9.5 + 1 * (20 - 9.7)) / 3.0
     assert q[0, 0] - p[0, 0] == pytest.approx(0.1 * com, rel=1e-6)
     assert np.allclose(q[2, :3], 1.1 * p[2, :3], rtol=1e-6)
+
+
+def test_andersen_collision_statistics(oracle):
+    # a collision redraws an atom with probability p, from N(0, kT/m) per
+    # axis; the others keep their velocities bit for bit
+    n, kt, p = 20000, 0.5, np.float32(0.3)
+    m = np.full(n, 4.0, np.float32)
+    v = np.full((3, n), 7.0, np.float32)
+    oracle.andersen(v, m, 11, 5, p, np.float32(kt))
+    hit = v[0] != 7.0
+    assert abs(hit.mean() - 0.3) < 0.02
+    assert np.all(v[:, ~hit] == 7.0)
+    assert v[:, hit].var() == pytest.approx(kt / 4.0, rel=0.05)
+    # the decision and the draws come from the documented Philox blocks
+    i = int(np.argmax(hit))
+    o = oracle.philox4x32_10((11, 0), (5, 0, i, 3))
+    assert (o[0] + 0.5) / 2 ** 32 < 0.3
+    want = np.float32(np.sqrt(np.float32(kt) / np.float32(4.0))) * np.float32(
+        oracle.philox_gaussian(11, 5, i, 0))
+    assert v[0, i] == want
