@@ -310,3 +310,30 @@ def write_flexible_water(d, n_steps=20):
     with open(cfg, "w") as f:
         f.write(FLEX_TOML.format(n_steps=n_steps, grid=s.spme.grid[0]))
     return cfg, s
+
+
+EXAMPLES = os.path.join(ROOT, "tests", "golden", "examples")
+
+
+def example_bundle(name, d):
+    """Copy one of the reference's example bundles (tests/golden/examples)
+    into d; the lj-10000-argon init is written here: 20 x 20 x 25 atoms on a
+    simple-cubic lattice of spacing 4 Angstrom centred on the origin, in a
+    box of 8 x 8 x 10 nm, printed as %.9e."""
+    import shutil
+    src = os.path.join(EXAMPLES, name)
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), os.path.join(d, f))
+    if name == "lj-10000-argon":
+        nx, ny, nz, a = 20, 20, 25, 4.0e-10
+        with open(os.path.join(d, "argon.in.xyz"), "w") as f:
+            f.write(f"{nx * ny * nz}\n")
+            f.write(f'Lattice="{nx * a:.9e} 0 0 0 {ny * a:.9e} 0 0 0 {nz * a:.9e}" '
+                    "Properties=species:S:1:pos:R:3\n")
+            for i in range(nx):
+                for j in range(ny):
+                    for k in range(nz):
+                        x, y, z = ((i - 9.5) * a, (j - 9.5) * a, (k - 12.0) * a)
+                        f.write(f"Ar {x:.9e} {y:.9e} {z:.9e}\n")
+        return os.path.join(d, "argon.in.toml")
+    return os.path.join(d, [f for f in os.listdir(src) if f.endswith(".in.toml")][0])

@@ -85,3 +85,13 @@ def test_config_name_must_end_in_in_toml(tmp_path):
     os.rename(cfg, other)
     r = run_cli("lint", other)
     assert r.returncode == 1 and ".in.toml" in r.stderr
+
+
+@pytest.mark.parametrize("name", ["lj-10000-argon", "spc-water-256"])
+def test_lint_accepts_the_reference_example_bundles(tmp_path, name):
+    from tests.cli_inputs import example_bundle
+    cfg = example_bundle(name, str(tmp_path))
+    r = run_cli("lint", cfg)
+    assert r.returncode == 0, r.stdout + r.stderr
+    if name == "spc-water-256":
+        assert "512 bonds, 256 angles" in r.stdout

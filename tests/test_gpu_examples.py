@@ -1,0 +1,48 @@
+"""The reference's own example bundles through `heddlemd run` on the GPU
+(inputs under tests/golden/examples): lj-10000-argon (100 NVE steps, frames
+with velocities every 10, log every 5) and spc-water-256 (flexible SPC
+water: Morse O-H bonds, harmonic angles, switched LJ, SPME on a 20 x 20 x 80
+grid, 100 steps of 0.5 fs).  Each runs twice; the .xyz and .log files must
+be byte-identical across the two runs."""
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from tests.cli_inputs import example_bundle, run_cli  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name,root,frames,rows", [
+    ("lj-10000-argon", "argon", 11, 21),
+    ("spc-water-256", "spc", 11, 21),
+])
+def test_example_bundle_runs_and_reruns_identically(tmp_path, name, root, frames, rows):
+    say(f"case: heddlemd run examples/{name}, twice")
+    outs = []
+    for rep in range(2):
+        d = tmp_path / f"r{rep}"
+        d.mkdir()
+        cfg = example_bundle(name, str(d))
+        r = run_cli("run", cfg, timeout=300)
+        say("  " + (r.stdout + r.stderr).strip().replace("\n", "\n  "))
+        assert r.returncode == 0
+        assert re.fullmatch(rf"\[heddlemd\] phase `run`: 100 steps in \d+ (ms|µs) "
+                            rf"\(frames: {frames}, log rows: {rows}\)", r.stdout.splitlines()[0])
+        outs.append({ext: open(os.path.join(str(d), f"{root}.out.run.{ext}"), "rb").read()
+                     for ext in ("xyz", "log")})
+    assert outs[0]["xyz"] == outs[1]["xyz"]
+    assert outs[0]["log"] == outs[1]["log"]
+    log = outs[0]["log"].decode().splitlines()
+    assert log[0] == "step,time,kinetic_energy,temperature"
+    temps = np.array([float(x.split(",")[3]) for x in log[1:]])
+    say(f"  T from {temps[0]:.1f} to {temps[-1]:.1f} K (max {temps.max():.1f})")
+    assert np.all(np.isfinite(temps)) and temps.max() < 1500.0
