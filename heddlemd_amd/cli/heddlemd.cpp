@@ -1370,7 +1370,12 @@ std::vector<std::string> output_paths(const Config& c) {
 void check_box(const Config& c, const State& s, const Topology& t) {
   (void)t;
   double need = 3.0 * (c.max_cutoff() + (c.r_skin >= 0 ? c.r_skin : 0.3 * c.max_cutoff()));
-  double w[3] = {s.lattice[0], s.lattice[1], s.lattice[2]};
+  // widths perpendicular to the other two box vectors (min_perpendicular_width)
+  const double* L = s.lattice;
+  double ilx = 1.0 / L[0], ily = 1.0 / L[1], ilz = 1.0 / L[2];
+  double a1 = -L[3] * ilx * ily, a2 = (L[3] * L[5] - L[4] * L[1]) * ilx * ily * ilz;
+  double b2 = -L[5] * ily * ilz;
+  double w[3] = {1.0 / sqrt(ilx * ilx + a1 * a1 + a2 * a2), 1.0 / sqrt(ily * ily + b2 * b2), L[2]};
   for (int d = 0; d < 3; ++d)
     if (w[d] < need) {
       char b[200];

@@ -411,6 +411,23 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   return a;
 }
 
+// Norm of row d of the inverse box matrix (a = (lx,0,0), b = (xy,ly,0),
+// c = (xz,yz,lz)): a Cartesian move of length r changes fractional
+// coordinate d by at most r times it, and its inverse is the box's width
+// perpendicular to the other two box vectors.
+double inv_row_norm(const double* L, int d) {
+  double ilx = 1.0 / L[0], ily = 1.0 / L[1], ilz = 1.0 / L[2];
+  if (d == 0) {
+    double a1 = -L[3] * ilx * ily, a2 = (L[3] * L[5] - L[4] * L[1]) * ilx * ily * ilz;
+    return sqrt(ilx * ilx + a1 * a1 + a2 * a2);
+  }
+  if (d == 1) {
+    double b2 = -L[5] * ily * ilz;
+    return sqrt(ily * ily + b2 * b2);
+  }
+  return ilz;
+}
+
 bool box_ortho(const hd_ctx* c) {
   return c->lattice[3] == 0.0 && c->lattice[4] == 0.0 && c->lattice[5] == 0.0;
 }
@@ -486,7 +503,7 @@ static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed) {
     sa.nv[d] = c->nv[d];
     // atoms move at most r_skin/2 between rebuilds; 20 % slack for box
     // changes since set-up, plus one voxel for the binning's rounding
-    sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] / c->lattice[d]) + 1;
+    sa.mv[d] = (int)floor(0.6 * c->r_skin * (double)c->nv[d] * inv_row_norm(c->lattice, d)) + 1;
   }
   stage_begin(c, "spme_recip", timed);
   if (c->dft_mode == 1) {
@@ -1981,8 +1998,13 @@ int hd_setup(hd_ctx* c) {
   for (uint64_t i = 0; i < c->n; ++i)
     if (c->h_type[i] >= (uint32_t)c->ntypes) return fail(c, HD_ERR_ARG, "type[%llu] out of range",
                                                          (unsigned long long)i);
-  if (c->lattice[3] != 0 || c->lattice[4] != 0 || c->lattice[5] != 0)
-    return fail(c, HD_ERR_UNSUPPORTED, "triclinic boxes are not built (orthorhombic only)");
+  // tilted cells as the reference's lower-triangular lattice (pbc.cuh):
+  // each tilt at most half the box length it tilts along, so fractional
+  // rounding finds the nearest image of any separation below half a width
+  if (fabs(c->lattice[3]) > 0.5 * c->lattice[0] || fabs(c->lattice[4]) > 0.5 * c->lattice[0] ||
+      fabs(c->lattice[5]) > 0.5 * c->lattice[1])
+    return fail(c, HD_ERR_UNSUPPORTED, "box tilts must be at most half the box lengths "
+                                       "(|xy|, |xz| <= lx/2, |yz| <= ly/2)");
   // the compensated mode carries FP64 residuals of x and v that constraint
   // projections and barostat scaling would leave stale; the reference
   // refuses the combination (src/integrator/velocity_verlet.rs:222-229)
@@ -1996,10 +2018,10 @@ int hd_setup(hd_ctx* c) {
   if (c->r_skin < 0) c->r_skin = 0.3 * c->rmax;
   double rs = c->rmax + c->r_skin;
   for (int d = 0; d < 3; ++d)
-    if (rs > 0.5 * c->lattice[d])
+    if (rs > 0.5 / inv_row_norm(c->lattice, d))
       return fail(c, HD_ERR_UNSUPPORTED,
-                  "r_cut + r_skin = %g exceeds half the box (%g): minimum image breaks", rs,
-                  0.5 * c->lattice[d]);
+                  "r_cut + r_skin = %g exceeds half the box width (%g): minimum image breaks", rs,
+                  0.5 / inv_row_norm(c->lattice, d));
   unsigned n = (unsigned)c->n;
   c->nblocks = (n + 63) / 64;
   c->npad = c->nblocks * 64;
@@ -2336,7 +2358,10 @@ int hd_mc_barostat_move(hd_ctx* c, double pressure, double kt, double max_dv, ui
   HIPCHK(c, hipMemcpyAsync(&u_old, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double v_old = (double)lat[0] * (double)lat[1] * (double)lat[2];
-  double min_w = std::min(std::min((double)lat[0], (double)lat[1]), (double)lat[2]);
+  double latd[6];
+  for (int k = 0; k < 6; ++k) latd[k] = lat[k];
+  double min_w = 1.0 / std::max(std::max(inv_row_norm(latd, 0), inv_row_norm(latd, 1)),
+                                inv_row_norm(latd, 2));
   uint32_t ctr[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u};
   philox_host((uint32_t)seed, (uint32_t)(seed >> 32), ctr);
   double u1 = ((double)ctr[0] + 0.5) / 4294967296.0, u2 = ((double)ctr[1] + 0.5) / 4294967296.0;

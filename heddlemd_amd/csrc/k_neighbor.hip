@@ -292,15 +292,31 @@ __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* _
 // per block).  Padding lanes are ignored.
 
 // Closest approach of two boxes under the minimum image (per axis; exact
-// for orthorhombic cells), as kernels/neighbor.cu:364-388.
-__device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, float4 hb,
-                                              const HdLattice& L) {
-  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
-  hd_min_image(dx, dy, dz, L);
+// for orthorhombic cells), as kernels/neighbor.cu:364-388.  In a tilted
+// cell a pair's own minimum image (fractional rounding of its separation)
+// can differ by one lattice vector per axis from the one the box centres
+// round to, so the 27 images around the centres' are all tried there.
+__device__ __forceinline__ float nb_box_gap2(float dx, float dy, float dz, float4 ha, float4 hb) {
   float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.f);
   float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.f);
   float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.f);
   return ex * ex + ey * ey + ez * ez;
+}
+__device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, float4 hb,
+                                              const HdLattice& L) {
+  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
+  hd_min_image(dx, dy, dz, L);
+  if (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) return nb_box_gap2(dx, dy, dz, ha, hb);
+  float best = 3.0e38f;
+  for (int kc = -1; kc <= 1; ++kc)
+    for (int kb = -1; kb <= 1; ++kb)
+      for (int ka = -1; ka <= 1; ++ka) {
+        float sx = dx + (float)ka * L.lx + (float)kb * L.xy + (float)kc * L.xz;
+        float sy = dy + (float)kb * L.ly + (float)kc * L.yz;
+        float sz = dz + (float)kc * L.lz;
+        best = fminf(best, nb_box_gap2(sx, sy, sz, ha, hb));
+      }
+  return best;
 }
 
 // Block-pair candidates: bit (jb % 64) of mask[b * nchunks + jb / 64] is

@@ -157,6 +157,27 @@ __device__ __forceinline__ float pair_axis_disp(float a, float b, float len, flo
   return d;
 }
 
+// The same for a tilted cell: the image triple from the f32 fractional
+// rounding of hd_min_image_rcp, and, when it is not (0, 0, 0), the
+// difference and the lattice shift formed in FP64 and rounded once.
+__device__ __forceinline__ void pair_tri_disp(float pix, float piy, float piz, float pjx,
+                                              float pjy, float pjz, const HdLattice& L,
+                                              float ilx, float ily, float ilz, float& dx,
+                                              float& dy, float& dz) {
+  dx = pix - pjx;
+  dy = piy - pjy;
+  dz = piz - pjz;
+  float sc = dz * ilz;
+  float sb = (dy - sc * L.yz) * ily;
+  float sa = (dx - sb * L.xy - sc * L.xz) * ilx;
+  float ka = floorf(sa + 0.5f), kb = floorf(sb + 0.5f), kc = floorf(sc + 0.5f);
+  if (ka == 0.f && kb == 0.f && kc == 0.f) return;
+  double a = ka, b = kb, c = kc;
+  dx = (float)(((double)pix - (double)pjx) - (a * L.lx + b * L.xy + c * L.xz));
+  dy = (float)(((double)piy - (double)pjy) - (b * L.ly + c * L.yz));
+  dz = (float)(((double)piz - (double)pjz) - c * L.lz);
+}
+
 // Far-away coordinates for padding lanes on the shifted path: any pair
 // with one of them fails both cutoffs, and |d|^2 stays finite (< 1e37).
 #define PAIR_FAR_I 1.0e18f
@@ -327,7 +348,7 @@ __device__ __forceinline__ void pair_entries(
           dy = pair_axis_disp(piy, pjy, L.ly, ily);
           dz = pair_axis_disp(piz, pjz, L.lz, ilz);
         } else {
-          hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
+          pair_tri_disp(pix, piy, piz, pjx, pjy, pjz, L, ilx, ily, ilz, dx, dy, dz);
         }
       }
       float r2 = dx * dx + dy * dy + dz * dz;
@@ -534,7 +555,7 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
     dy = pair_axis_disp(pi.y, pj.y, L.ly, ily);
     dz = pair_axis_disp(pi.z, pj.z, L.lz, ilz);
   } else {
-    hd_min_image_rcp(dx, dy, dz, L, ilx, ily, ilz);
+    pair_tri_disp(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, L, ilx, ily, ilz, dx, dy, dz);
   }
   float r2 = dx * dx + dy * dy + dz * dz;
   float inv_r = __builtin_amdgcn_rsqf(r2);
