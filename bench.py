@@ -66,6 +66,45 @@ def build_engine(system, U, args):
     return e, kt
 
 
+def autotune_skin(e, system, U, kt, args):
+    """Time the production step from the equilibrated state at several
+    neighbour skins (heddlemd_amd.autotune; the forces do not depend on the
+    skin) and continue with the fastest.  Returns the engine to measure and
+    the tuning record."""
+    import dataclasses
+    from heddlemd_amd.autotune import tune_step
+    from heddlemd_amd.engine import engine_for
+
+    state = dataclasses.replace(system, posq=e.positions(), lattice=e.lattice())
+    v = e.velocities()
+    baro = None if args.no_barostat else dict(
+        pressure=U.pressure(1.0e5), tau=U.time(1.0e-12), compressibility=U.inv_pressure(4.5e-10),
+        kt=kt, seed=13)
+
+    def make(skin):
+        return engine_for(state, dt=U.time(args.dt_fs * 1e-15), nhc_kt=kt,
+                          nhc_tau=U.time(args.tau_fs * 1e-15), velocities=v, r_skin=skin,
+                          crescale=baro, pair_list=args.pair_list)
+
+    skins_nm = sorted(set([args.skin_nm, 0.08, 0.1, 0.14, 0.2]))
+    best, table = tune_step(make, [U.length(x * 1e-9) for x in skins_nm],
+                            pair_lists=(args.pair_list,), dft_modes=(0,), steps=100, warmup=20)
+    nm = lambda b: round(b / U.length(1e-9), 3)
+    rec = {"candidates": [{"skin_nm": nm(r["r_skin"]),
+                           "ms_per_step": round(r.get("ms_per_step", float("nan")), 4)}
+                          for r in table],
+           "chosen_skin_nm": nm(best["r_skin"]), "steps_per_candidate": 100,
+           "method": "heddlemd_amd.autotune.tune_step from the equilibrated state"}
+    log(f"autotune: {rec}")
+    if abs(nm(best["r_skin"]) - args.skin_nm) > 1e-9:
+        e.close()
+        args.skin_nm = nm(best["r_skin"])
+        e = make(best["r_skin"])
+        e.constrain_velocities()
+        e.force_eval(scalars=True)
+    return e, rec
+
+
 def minimize(e, U, args):
     """Steepest descent from the lattice start, with the reference examples'
     settings (examples/spc-water-8192/water.in.toml [[minimization]]:
@@ -401,6 +440,8 @@ def main():
                     help="timed steps of the CPU baseline (after 2 warm-up steps)")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-replica pass")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="measure at --skin-nm instead of the fastest tuned skin")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-equil", type=int, default=100)
@@ -462,6 +503,9 @@ def main():
     t0 = time.time()
     equilibrate(e, system, U, kt, args)
     log(f"equilibration {time.time() - t0:.1f} s")
+    tuned = None
+    if not args.no_autotune:
+        e, tuned = autotune_skin(e, system, U, kt, args)
     e.step(args.warmup, scalars_last=True)
     e.sync()
     reb0 = e.scalars()["rebuilds"]
@@ -489,13 +533,16 @@ def main():
             "workload": (f"SPC water {system.n // 3} molecules ({system.n} atoms), SPME "
                          f"{'x'.join(map(str, system.spme.grid))} order 4, r_c 0.9 nm, SETTLE, "
                          f"NHC 300 K" + ("" if args.no_barostat else " + C-rescale 1 bar")
-                         + f", dt {args.dt_fs} fs, neighbour skin {args.skin_nm} nm"),
+                         + f", dt {args.dt_fs} fs, neighbour skin {args.skin_nm} nm"
+                         + (" (autotuned)" if tuned else "")),
             "atoms": system.n, "parallelism": "single GPU",
         },
         "atom_steps_per_s": round(atom_steps, 1),
         "neighbour_rebuilds_in_timed_region": rebuilds,
         "temperature_K": round(T, 2),
     }
+    if tuned:
+        result["autotune"] = tuned
 
     # ---- roofline pass: same steps, no graph, hipEvents per stage ------------
     if not args.no_roofline_pass:
