@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <set>
 #include <string>
 #include <vector>
@@ -1290,10 +1291,31 @@ int hd_set_exclusions(hd_ctx* c, uint64_t n_pairs, const uint32_t* ij, const dou
   bool cliques = true;
   for (uint64_t i = 0; i < n; ++i)
     if (root(i) == i && edges[i] != size[i] * (size[i] - 1) / 2) cliques = false;
+  // runs of consecutive atoms excluded exactly up to index distance span
+  bool chains = !cliques && span <= (1u << 20) &&
+                (uint64_t)n * (span + 1) < HD_SENTINEL_KEY_LIMIT;
+  for (uint64_t i = 0; chains && i < n; ++i) {
+    uint64_t r = root(i);
+    if (hi[r] - lo[r] + 1 != size[r]) chains = false;
+    if (chains && r == i) {
+      uint64_t m = size[r], k = std::min(span, m - 1);
+      if (edges[r] != m * k - k * (k + 1) / 2) chains = false;
+    }
+  }
   std::vector<uint32_t> key(n);
   unsigned new_span = 0;
-  if (cliques) {
-    // one key per component, numbered in order of first atom
+  if (chains) {
+    uint64_t comp = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (i > 0 && root(i) != root(i - 1)) ++comp;
+      key[i] = (uint32_t)(i + comp * span);
+    }
+    new_span = (unsigned)span;
+  } else {
+    // one key per component (numbered in order of first atom): the pair
+    // kernel leaves out every pair inside a component; in a component that
+    // is not a clique the pairs that are not fully excluded are added back
+    // below with their weights (1, or the listed scales)
     std::vector<uint32_t> id(n, 0);
     uint32_t next = 0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -1301,54 +1323,58 @@ int hd_set_exclusions(hd_ctx* c, uint64_t n_pairs, const uint32_t* ij, const dou
       if (r == i) id[r] = next++;
       key[i] = id[r];
     }
-  } else {
-    // runs of consecutive atoms excluded exactly up to index distance span
-    for (uint64_t i = 0; i < n; ++i) {
-      uint64_t r = root(i);
-      if (hi[r] - lo[r] + 1 != size[r])
-        return fail(c, HD_ERR_UNSUPPORTED,
-                    "full exclusions around atom %llu: a molecule that is not one clique must "
-                    "be a run of consecutive atoms",
-                    (unsigned long long)i);
-    }
-    uint64_t want = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-      uint64_t r = root(i);
-      if (r != i) continue;
-      uint64_t m = size[r];
-      uint64_t k = std::min(span, m - 1);
-      want = m * k - k * (k + 1) / 2;  // pairs at most span apart inside the run
-      if (edges[r] != want)
-        return fail(c, HD_ERR_UNSUPPORTED,
-                    "full exclusions of atoms %llu..%llu are neither all pairs nor exactly the "
-                    "pairs at most %llu apart",
-                    (unsigned long long)lo[r], (unsigned long long)hi[r],
-                    (unsigned long long)span);
-    }
-    if (span > (1u << 20) || (uint64_t)n * (span + 1) >= HD_SENTINEL_KEY_LIMIT)
-      return fail(c, HD_ERR_UNSUPPORTED, "exclusion span %llu too large",
-                  (unsigned long long)span);
-    uint64_t comp = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-      if (i > 0 && root(i) != root(i - 1)) ++comp;
-      key[i] = (uint32_t)(i + comp * span);
-    }
-    new_span = (unsigned)span;
   }
   // scaled pairs: weight relative to what the key rule leaves
   std::vector<HdExclPair> sc;
+  std::map<std::pair<uint32_t, uint32_t>, std::pair<double, double>> scaled;
   for (uint64_t p = 0; p < n_pairs; ++p) {
     if (s_lj[p] == 0.0 && s_coul[p] == 0.0) continue;
     uint32_t a = ij[2 * p], b = ij[2 * p + 1];
+    scaled[std::make_pair(std::min(a, b), std::max(a, b))] = {s_lj[p], s_coul[p]};
+  }
+  for (auto& kv : scaled) {
+    uint32_t a = kv.first.first, b = kv.first.second;
     uint32_t d = key[a] > key[b] ? key[a] - key[b] : key[b] - key[a];
     double base = d <= new_span ? 0.0 : 1.0;
     HdExclPair e;
     memset(&e, 0, sizeof(e));
     e.i = a;
     e.j = b;
-    e.w_lj = s_lj[p] - base;
-    e.w_coul = s_coul[p] - base;
+    e.w_lj = kv.second.first - base;
+    e.w_coul = kv.second.second - base;
     if (e.w_lj != 0.0 || e.w_coul != 0.0) sc.push_back(e);
+  }
+  if (!chains && !cliques) {
+    // add-back pairs of the components that are not cliques
+    std::vector<std::vector<uint32_t>> members(n);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t r = root(i);
+      if (edges[r] != size[r] * (size[r] - 1) / 2) members[r].push_back((uint32_t)i);
+    }
+    for (uint64_t r = 0; r < n; ++r) total += members[r].size() * (members[r].size() - (members[r].empty() ? 0 : 1)) / 2;
+    if (total > (8u << 20))
+      return fail(c, HD_ERR_UNSUPPORTED,
+                  "exclusions: %llu intramolecular pairs to add back (limit 8 Mi); the molecules "
+                  "are neither cliques nor chains",
+                  (unsigned long long)total);
+    std::set<std::pair<uint32_t, uint32_t>> full_set;
+    for (auto& e : full) full_set.insert(std::make_pair((uint32_t)e.first, (uint32_t)e.second));
+    for (uint64_t r = 0; r < n; ++r) {
+      const auto& m = members[r];
+      for (size_t x = 0; x < m.size(); ++x)
+        for (size_t y = x + 1; y < m.size(); ++y) {
+          auto kp = std::make_pair(m[x], m[y]);
+          if (full_set.count(kp) || scaled.count(kp)) continue;
+          HdExclPair e;
+          memset(&e, 0, sizeof(e));
+          e.i = m[x];
+          e.j = m[y];
+          e.w_lj = 1.0;
+          e.w_coul = 1.0;
+          sc.push_back(e);
+        }
+    }
   }
   c->h_key = key;
   c->excl_span = new_span;

@@ -83,10 +83,14 @@ int hd_set_dihedrals(hd_ctx *ctx, uint64_t n_dihedrals, const uint32_t *ijkl,
  * rows, then bond, angle, constraint-group and scaled 1-4 pairs): pair p is
  * (ij[2p], ij[2p+1]) with LJ scale s_lj[p] and real-space Coulomb scale
  * s_coul[p] in [0, 1] (lj.rs:295-321, spme.rs:723-762; unlisted pairs have
- * scale 1).  Replaces the keys of hd_set_system and the span: the fully
- * excluded pairs must be whole molecules (cliques) or runs of consecutive
- * atoms excluded up to a fixed index distance, else HD_ERR_UNSUPPORTED;
- * scaled pairs are any.  After hd_set_system, before hd_setup. */
+ * scale 1).  Replaces the keys of hd_set_system and the span.  The pair
+ * kernel leaves out pairs by key: when the fully excluded pairs form whole
+ * molecules (cliques) or runs of consecutive atoms excluded up to a fixed
+ * index distance, exactly those; otherwise every pair inside each connected
+ * molecule, and the molecule's pairs that are not fully excluded are added
+ * back in FP64 by a correction pass (at most 8 Mi such pairs, else
+ * HD_ERR_UNSUPPORTED).  Scaled pairs are any.  After hd_set_system, before
+ * hd_setup. */
 int hd_set_exclusions(hd_ctx *ctx, uint64_t n_pairs, const uint32_t *ij, const double *s_lj,
                       const double *s_coul);
 /* Box (src/pbc.rs SimulationBox lattice). */

@@ -99,10 +99,10 @@ def test_exclusion_rules_accepted_and_refused():
           if b + d < 6]
     e.set_exclusions(np.array(ch), np.zeros(len(ch)), np.zeros(len(ch)))
     e.close()
-    # a chain missing one 1-3 pair is neither form
+    # a chain missing one 1-2 pair is neither form: its molecule is left out
+    # by key and its other pairs added back
     e = _tiny_engine(12)
-    with pytest.raises(HeddleError):
-        e.set_exclusions(np.array(ch[1:]), np.zeros(len(ch) - 1), np.zeros(len(ch) - 1))
+    e.set_exclusions(np.array(ch[1:]), np.zeros(len(ch) - 1), np.zeros(len(ch) - 1))
     e.close()
     # scales outside [0, 1], a repeated pair, a self pair
     for ij, sl in (([(0, 1)], [1.5]), ([(0, 1), (1, 0)], [0.5, 0.5]), ([(2, 2)], [0.5])):
@@ -110,3 +110,26 @@ def test_exclusion_rules_accepted_and_refused():
         with pytest.raises(HeddleError):
             e.set_exclusions(np.array(ij), np.array(sl), np.array(sl))
         e.close()
+
+
+def test_irregular_exclusions_are_added_back(oracle):
+    # chains whose full exclusions are 1-2, 1-3 and one extra pair (bead 0 -
+    # bead 5): neither cliques nor fixed-span runs, so every intramolecular
+    # pair is left out by key and the non-excluded ones (1-4 scaled, the
+    # rest full) come back through the correction pass
+    say("case: bonded melt 60 x 20 with an irregular exclusion per chain")
+    s = systems.bonded_melt(n_chains=60)
+    ij, slj, sco = s.exclusions
+    nb = 20
+    extra = np.array([(c * nb, c * nb + 5) for c in range(60)], np.uint32)
+    s.exclusions = (np.concatenate([ij, extra]), np.concatenate([slj, np.zeros(60)]),
+                    np.concatenate([sco, np.zeros(60)]))
+    e = engine_for(s, dt=U.time(1e-15))
+    e.force_eval(scalars=True)
+    f, sc = e.forces(), e.scalars()
+    e.close()
+    F, E, W, band, parts = _reference(oracle, s)
+    say(f"  pe {sc['pe']:.9e} (oracle {E:.9e})")
+    check_forces(f, F, band, "  irregular exclusions", say)
+    assert sc["pe"] == pytest.approx(E, rel=2e-6)
+    assert sc["virial"] == pytest.approx(W, rel=2e-5, abs=1e-5 * abs(parts["pair"]))
