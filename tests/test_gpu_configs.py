@@ -178,3 +178,74 @@ def test_polymer_melt_200x50_forces_and_shake_step(oracle):
     assert np.array_equal(img.T, img1)
     assert np.array_equal(p, posq1)
     assert np.array_equal(v, v1)
+
+
+def test_water_8192_compensated_mode():
+    # the 8,192-molecule SPC box in the compensated (f32 value + f32 residual
+    # per coordinate) mode.  The reference refuses constraints in that mode
+    # (src/integrator/velocity_verlet.rs:222-231), so the waters are held by
+    # harmonic O-H bonds and an H-O-H angle, as in its spc-water-256
+    # example.  (1) NVE with SPME is time-reversible: 20 steps, negate the
+    # velocities, 20 steps, and every velocity word and every position word
+    # not within a rounding of zero is back.  (2) Under the Nose-Hoover chain at 300 K the compensated and
+    # plain f32 runs agree after 10 steps, and the chain's conserved
+    # quantity moves by under 1 % of the kinetic energy over 200 steps in
+    # both.
+    s = systems.flexible_water(systems.config_water_8192())
+    from heddlemd_amd.rng import maxwell_boltzmann
+    kt = U.temperature(300.0)
+    v0 = maxwell_boltzmann(s.mass, kt, seed=4, n_constraints=0).astype(np.float32)
+    dt = U.time(5e-16)
+    say("case: flexible water 8192 lossless reversal, 20 + 20 steps")
+    e = engine_for(s, dt=dt, lossless=True, velocities=v0, r_skin=U.length(0.3e-9))
+    e.force_eval(scalars=True)
+    x0 = e.positions()
+    reb0 = e.scalars()["rebuilds"]
+    e.step(20)
+    e.negate_velocities()
+    e.step(20)
+    x1 = e.positions()
+    v1 = e.velocities()
+    reb = e.scalars()["rebuilds"] - reb0
+    e.close()
+    # the (f32 + FP64 residual) sums are exact, and so reversible, except
+    # for a coordinate that starts at or passes within a double rounding of
+    # zero: there the FP64 residual of x + dx loses bits below 2^-53 |dx|
+    # (the lattice puts 1,230 oxygen coordinates at exactly 0), and the
+    # coordinate comes back as a few 1e-18 Bohr
+    far = np.abs(x0[:, :3]) > 1e-9
+    near = ~far
+    say(f"  rebuilds between the passes {reb}; coordinates equal "
+        f"{np.mean(x0[:, :3] == x1[:, :3]):.6f} ({near.sum()} within 1e-9 Bohr of zero, max "
+        f"|x1 - x0| there {np.abs(x1[:, :3][near]).max():.1e}); velocities equal "
+        f"{np.mean(-v1 == v0):.6f}")
+    assert reb == 0, "a rebuild between the passes would change the f32 sum order"
+    assert np.array_equal(x0[:, :3][far], x1[:, :3][far])
+    assert np.abs(x1[:, :3][near].astype(np.float64) - x0[:, :3][near]).max() < 1e-15
+    assert np.array_equal(x0[:, 3], x1[:, 3])
+    assert np.array_equal(-v1, v0)
+
+    say("case: flexible water 8192 NHC 300 K, compensated vs f32, 200 steps")
+    runs = {}
+    for lossless in (True, False):
+        e = engine_for(s, dt=dt, lossless=lossless, velocities=v0, nhc_kt=kt,
+                       nhc_tau=U.time(1e-13))
+        e.force_eval(scalars=True)
+        e.step(1, scalars_last=True)
+        a = e.scalars()
+        e.step(9)
+        p10 = e.positions()
+        e.step(190, scalars_last=True)
+        b = e.scalars()
+        e.close()
+        h0 = a["ke"] + a["pe"] + a["nhc_chain"]
+        h1 = b["ke"] + b["pe"] + b["nhc_chain"]
+        runs[lossless] = (p10, abs(h1 - h0) / a["ke"], 2 * b["ke"] / (3 * s.n - 3) / kt)
+    dp = np.abs(runs[True][0][:, :3].astype(np.float64) - runs[False][0][:, :3]).max()
+    say(f"  after 10 steps max |dx| {dp:.2e} Bohr; conserved drift / KE: compensated "
+        f"{runs[True][1]:.2e}, f32 {runs[False][1]:.2e}; T/T0 {runs[True][2]:.3f}")
+    assert dp < 1e-3
+    assert runs[True][1] < 1e-2 and runs[False][1] < 1e-2
+    # (the unrelaxed lattice start releases its strain into heat faster
+    # than the 100 fs chain removes it: T/T0 is about 2.6 at 100 fs)
+    assert np.isfinite(runs[True][2]) and runs[True][2] < 5.0
