@@ -6,7 +6,8 @@ ABI, against the CPU oracle:
     DFT;
   * the pair sets of the 8,192- and 65,536-molecule water boxes against the
     oracle's brute-force enumeration (every pair within r_search present);
-  * the 256 batched replicas, each byte-identical to its own solo run;
+  * the 256 batched replicas, each byte-identical to its own solo run, and
+    eight of them, stepped as a batch, against the oracle's forces;
   * the 200 x 50 polymer melt: forces, energy, virial, and one SHAKE step
     bit-exact against the oracle's restatement of the device's sweeps.
 """
@@ -125,6 +126,44 @@ def test_all_256_replicas_byte_identical_to_solo_runs():
         e.close()
     say(f"  {len(specs)} replicas, {sum(sp['n_atoms'] for sp in specs)} atoms, mismatches {bad}")
     assert bad == []
+
+
+def test_replica_forces_in_a_batch_match_the_oracle(oracle):
+    # eight of the 256 replicas (the smallest, the largest and two between of
+    # each kind; the water grids include a mixed-radix side) stepped 200
+    # times (0.2 ps) as one batch; every replica's forces at its own final
+    # positions against the FP64 oracle, to the force bar of the single
+    # systems.  (The argon replicas start on a perfect lattice, where every
+    # atom's force is a near-cancelling sum and F_rms is tiny; 0.2 ps of
+    # Langevin dynamics leaves a disordered state.)
+    specs = list(systems.replica_specs(count=256))
+    pick = []
+    for kind in ("argon", "water"):
+        idx = sorted((r for r, sp in enumerate(specs) if sp["kind"] == kind),
+                     key=lambda r: specs[r]["n_atoms"])
+        pick += [idx[0], idx[len(idx) // 3], idx[2 * len(idx) // 3], idx[-1]]
+    say("case: replicas " + ", ".join(f"{specs[r]['kind']} {specs[r]['n_atoms']}" for r in pick)
+        + " batched 200 steps, forces vs oracle")
+    from heddlemd_amd.engine import Batch
+    engines = [_replica_engine(specs[r], r) for r in pick]
+    b = Batch(engines)
+    b.step(200)
+    b.close()
+    for r, e in zip(pick, engines):
+        s = systems.replica_system(specs[r])
+        pos = e.positions()
+        f = e.forces()
+        e.close()
+        if specs[r]["kind"] == "argon":
+            F, _, _, band = oracle.pair_forces(pos, s.types, s.lj.as_dict(), s.lattice, band=1e-6)
+        else:
+            F, _, _, band = oracle.pair_forces(pos, s.types, s.lj.as_dict(), s.lattice,
+                                               coul=dict(alpha=s.spme.alpha, r_cut=s.spme.r_cut),
+                                               excl=s.exclusion_csr(), band=1e-6)
+            Fr = oracle.spme_recip(pos, s.lattice, s.spme.grid, 4, s.spme.alpha)[0]
+            F = F + Fr
+        check_forces(f, F, band, f"replica {r} ({specs[r]['kind']} {specs[r]['n_atoms']}, "
+                     f"grid {s.spme.grid[0] if s.spme else '-'})", say)
 
 
 def test_polymer_melt_200x50_forces_and_shake_step(oracle):
