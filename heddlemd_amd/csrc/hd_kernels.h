@@ -409,6 +409,7 @@ void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count);
 void hd_launch_negate_velocities(hipStream_t st, float* vx, float* vy, float* vz, double* vxlo,
                                  double* vylo, double* vzlo, unsigned n);
 void hd_launch_selftest_mfma(hipStream_t st, float* out);
+void hd_launch_selftest_mfma_k16(hipStream_t st, float* out, bool bf16);
 void hd_launch_selftest_dpp(hipStream_t st, int* out);
 
 #define HD_REDUCE_BLOCKS 256

@@ -1392,14 +1392,24 @@ int hd_set_exclusion_span(hd_ctx* c, uint32_t span) {
 
 static void drop_graphs(hd_ctx* c);
 
-// Split one DFT matrix into FP16 high and 2^12-scaled low parts.
-static void dft_split_host(const std::vector<double>& w, _Float16* hi, _Float16* lo) {
-  for (size_t k = 0; k < w.size(); ++k) {
-    float x = (float)w[k];
-    _Float16 h = (_Float16)x;
-    hi[k] = h;
-    lo[k] = (_Float16)((x - (float)h) * 4096.0f);
-  }
+// Split one DFT matrix (M x K row-major, M a multiple of 32, K of 16) into
+// FP16 high and 2^12-scaled low parts, stored in the fragment order of the
+// 32x32x16 matrix-core step: element e of lane l of (row tile mt, K step
+// ks) at ((mt (K/16) + ks) 64 + l) 8 + e holds W[32 mt + l % 32][16 ks +
+// 8 (l / 32) + e].
+static void dft_split_host(const std::vector<double>& w, unsigned M, unsigned K, _Float16* hi,
+                           _Float16* lo) {
+  unsigned nks = K / 16;
+  for (unsigned mt = 0; mt < M / 32; ++mt)
+    for (unsigned ks = 0; ks < nks; ++ks)
+      for (unsigned l = 0; l < 64; ++l)
+        for (unsigned e = 0; e < 8; ++e) {
+          size_t dst = (((size_t)mt * nks + ks) * 64 + l) * 8 + e;
+          float x = (float)w[(size_t)(32 * mt + l % 32) * K + 16 * ks + 8 * (l / 32) + e];
+          _Float16 h = (_Float16)x;
+          hi[dst] = h;
+          lo[dst] = (_Float16)((x - (float)h) * 4096.0f);
+        }
 }
 
 // DFT matrices of the split-precision path (layouts in hd_kernels.h)
@@ -1411,7 +1421,7 @@ static int build_dft_mats(hd_ctx* c) {
     return fail(c, HD_ERR_UNSUPPORTED,
                 "the matrix-core DFT handles power-of-two axes of 32 to 128 points");
   unsigned cf_m = (2 * nch + 31) / 32 * 32, cf_k = nc;
-  unsigned ci_m = nc, ci_k = (2 * nch + 7) / 8 * 8;
+  unsigned ci_m = nc, ci_k = (2 * nch + 15) / 16 * 16;
   size_t sz_cf = (size_t)cf_m * cf_k, sz_ci = (size_t)ci_m * ci_k;
   size_t sz_b = (size_t)(2 * nb) * (2 * nb), sz_a = (size_t)(2 * na) * (2 * na);
   size_t total = 2 * (sz_cf + sz_ci + 2 * sz_b + 2 * sz_a);
@@ -1419,8 +1429,9 @@ static int build_dft_mats(hd_ctx* c) {
   const double two_pi = 2.0 * M_PI;
   std::vector<double> w;
   size_t off = 0;
-  auto put = [&](const std::vector<double>& m, const _Float16** hi, const _Float16** lo) {
-    dft_split_host(m, h.data() + off, h.data() + off + m.size());
+  auto put = [&](const std::vector<double>& m, unsigned M, unsigned K, const _Float16** hi,
+                 const _Float16** lo) {
+    dft_split_host(m, M, K, h.data() + off, h.data() + off + m.size());
     *hi = (const _Float16*)(uintptr_t)off;  // offsets for now, pointers after upload
     *lo = (const _Float16*)(uintptr_t)(off + m.size());
     off += 2 * m.size();
@@ -1433,7 +1444,7 @@ static int build_dft_mats(hd_ctx* c) {
       double th = two_pi * (double)((k * x) % nc) / nc;
       w[(size_t)m * cf_k + x] = m < nch ? cos(th) : -sin(th);
     }
-  put(w, &c->dft.cf_hi, &c->dft.cf_lo);
+  put(w, cf_m, cf_k, &c->dft.cf_hi, &c->dft.cf_lo);
   // c-axis inverse over the Hermitian half spectrum
   w.assign(sz_ci, 0.0);
   for (unsigned x = 0; x < nc; ++x)
@@ -1443,7 +1454,7 @@ static int build_dft_mats(hd_ctx* c) {
       double th = two_pi * (double)((kk * x) % nc) / nc;
       w[(size_t)x * ci_k + k] = k < nch ? wt * cos(th) : -wt * sin(th);
     }
-  put(w, &c->dft.ci_hi, &c->dft.ci_lo);
+  put(w, ci_m, ci_k, &c->dft.ci_hi, &c->dft.ci_lo);
   // complex axes, [[Cr, -Ci], [Ci, Cr]] with C = exp(sign 2 pi i m k / n)
   auto cplx = [&](unsigned n, double sign) {
     std::vector<double> m2((size_t)(2 * n) * (2 * n), 0.0);
@@ -1458,10 +1469,10 @@ static int build_dft_mats(hd_ctx* c) {
       }
     return m2;
   };
-  put(cplx(nb, -1.0), &c->dft.bf_hi, &c->dft.bf_lo);
-  put(cplx(nb, +1.0), &c->dft.bi_hi, &c->dft.bi_lo);
-  put(cplx(na, -1.0), &c->dft.af_hi, &c->dft.af_lo);
-  put(cplx(na, +1.0), &c->dft.ai_hi, &c->dft.ai_lo);
+  put(cplx(nb, -1.0), 2 * nb, 2 * nb, &c->dft.bf_hi, &c->dft.bf_lo);
+  put(cplx(nb, +1.0), 2 * nb, 2 * nb, &c->dft.bi_hi, &c->dft.bi_lo);
+  put(cplx(na, -1.0), 2 * na, 2 * na, &c->dft.af_hi, &c->dft.af_lo);
+  put(cplx(na, +1.0), 2 * na, 2 * na, &c->dft.ai_hi, &c->dft.ai_lo);
   c->dft.cf_m = cf_m; c->dft.cf_k = cf_k; c->dft.ci_m = ci_m; c->dft.ci_k = ci_k;
   DALLOC(c, c->dft_mats, total);
   int r = upload(c, c->dft_mats, h.data(), total);
@@ -3187,6 +3198,20 @@ int hd_batch_step(hd_batch* b, int n_steps, int scalars_last) {
     int r = check_status(b->ctx[i]);
     if (r) return bfail(b, r, "replica %zu: %s", i, b->ctx[i]->err.c_str());
   }
+  return HD_OK;
+}
+
+int hd_selftest_mfma_k16(hd_ctx* c, int bf16, float* out) {
+  if (!c || !out) return HD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  float* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, 1024 * sizeof(float)));
+  hd_launch_selftest_mfma_k16(c->stream, d, bf16 != 0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, 1024 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, HD_ERR_HIP, "selftest: %s", hipGetErrorString(e));
   return HD_OK;
 }
 

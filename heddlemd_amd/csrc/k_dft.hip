@@ -5,13 +5,15 @@
 // n x n DFT matrix W.  Each operand is split into a high and a low FP16
 // part (x = hi + lo, lo = fp16(x - hi)), and W X is accumulated in FP32
 // as W_hi X_hi + W_hi X_lo + W_lo X_hi on the matrix cores
-// (v_mfma_f32_32x32x8_f16), which keeps ~22 significant bits.  Operands
-// are scaled by powers of two so that both parts stay in FP16's normal
-// range.
+// (v_mfma_f32_32x32x16_f16, gfx950), which keeps ~22 significant bits.
+// Operands are scaled by powers of two so that both parts stay in FP16's
+// normal range.
 #include "hd_device.h"
 #include "hd_kernels.h"
 
 typedef _Float16 hd_half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hd_half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 hd_bf8 __attribute__((ext_vector_type(8)));
 typedef float hd_float16v __attribute__((ext_vector_type(16)));
 
 // Lane layout of v_mfma_f32_32x32x8_f16 (one block):
@@ -43,6 +45,36 @@ void hd_launch_selftest_mfma(hipStream_t st, float* out) {
   hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, st, out);
 }
 
+// The doubled-K gfx950 forms v_mfma_f32_32x32x16_{f16,bf16}:
+//   A (32 x 16): lane l holds A[l % 32][8 (l / 32) + e], e = 0..7
+//   B (16 x 32): lane l holds B[8 (l / 32) + e][l % 32]
+//   D: the 32x32x8 layout above.
+// Self test with A[i][k] = (16 i + k) mod 61 - 30, B[k][j] = (32 k + j)
+// mod 23 - 11 (exact in both types).
+template <bool BF16>
+__global__ void k_selftest_mfma_k16(float* out) {
+  unsigned l = threadIdx.x;
+  hd_half8 a, b;
+  hd_bf8 ab, bb;
+  for (int e = 0; e < 8; ++e) {
+    unsigned i = l % 32u, k = 8u * (l / 32u) + (unsigned)e, j = l % 32u;
+    float av = (float)((int)((16u * i + k) % 61u) - 30);
+    float bv = (float)((int)((32u * k + j) % 23u) - 11);
+    a[e] = (_Float16)av; b[e] = (_Float16)bv;
+    ab[e] = (__bf16)av; bb[e] = (__bf16)bv;
+  }
+  hd_float16v c;
+  for (int v = 0; v < 16; ++v) c[v] = 0.f;
+  if (BF16) c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ab, bb, c, 0, 0, 0);
+  else c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  for (int v = 0; v < 16; ++v) out[l * 16u + (unsigned)v] = c[v];
+}
+
+void hd_launch_selftest_mfma_k16(hipStream_t st, float* out, bool bf16) {
+  if (bf16) hipLaunchKernelGGL(k_selftest_mfma_k16<true>, dim3(1), dim3(64), 0, st, out);
+  else hipLaunchKernelGGL(k_selftest_mfma_k16<false>, dim3(1), dim3(64), 0, st, out);
+}
+
 // ---------------------------------------------------------------------
 // DFT passes as GEMMs.  A workgroup takes DFT_NL lines (the GEMM's N),
 // stages them in LDS as FP16 high/low parts of the line scaled by 2^-s
@@ -66,15 +98,20 @@ void hd_launch_selftest_mfma(hipStream_t st, float* out) {
 #define DFT_NL 32
 #define DFT_THREADS 256
 #define DFT_KMAX 256
-#define DFT_KP (DFT_KMAX + 4)  // FP16 line pitch (halves): 130 dwords, 2-way at most
+#define DFT_KP (DFT_KMAX + 8)  // FP16 line pitch (halves): 16-byte rows, 132 dwords
 #define DFT_XP (DFT_KMAX + 1)  // f32 line pitch: odd, so column walks hit distinct banks
 #define DFT_LO_SCALE 4096.0f
 #define DFT_LO_UNSCALE (1.0f / 4096.0f)
 
 enum { DFT_C_FWD = 0, DFT_CPLX_B = 1, DFT_CPLX_A = 2, DFT_A_CONV = 3, DFT_C_INV = 4 };
 
+// A DFT matrix W (M x K, M a multiple of 32, K of 16) in fragment order:
+// the A operand of one 32x32x16 step (row tile mt, K step ks) is 64 lanes x
+// 8 halves stored contiguously, element e of lane l at
+// ((mt (K/16) + ks) 64 + l) 8 + e = W[32 mt + l % 32][16 ks + 8 (l / 32) + e],
+// so a wave's fragment load is one contiguous 1 KB read.
 struct DftMat {
-  const _Float16 *hi, *lo;  // [M][K] row-major
+  const _Float16 *hi, *lo;
   unsigned M, K;
 };
 
@@ -163,49 +200,46 @@ __device__ __forceinline__ int dft_split(const float* s_x, _Float16* xh, _Float1
 }
 
 // one 32-row tile of Y = W X for the staged lines; acc in D layout.  The
-// operand fragments of DFT_KB consecutive K-steps are loaded before their
-// MFMAs are issued, so the W reads (L2) overlap instead of serialising.
-#define DFT_KB 8
+// operand fragments of DFT_KB consecutive K steps are loaded before their
+// MFMAs are issued, so the W reads (L2, coalesced) overlap.
+#define DFT_KB 4
 __device__ __forceinline__ void dft_tile(const DftMat& w, unsigned mt, const _Float16* xh,
                                          const _Float16* xl, hd_float16v& main,
                                          hd_float16v& corr) {
   unsigned lane = threadIdx.x & 63u;
-  unsigned row = mt * 32u + (lane & 31u);
-  unsigned kq = 4u * (lane >> 5);
-  unsigned line = lane & 31u;
+  unsigned nks = w.K / 16u;
   for (int v = 0; v < 16; ++v) { main[v] = 0.f; corr[v] = 0.f; }
-  const _Float16* wh = w.hi + (size_t)row * w.K + kq;
-  const _Float16* wl = w.lo + (size_t)row * w.K + kq;
-  const _Float16* bxh = xh + line * DFT_KP + kq;
-  const _Float16* bxl = xl + line * DFT_KP + kq;
-  unsigned k0 = 0;
-  for (; k0 + 8u * DFT_KB <= w.K; k0 += 8u * DFT_KB) {
-    hd_half4 ah[DFT_KB], al[DFT_KB], bh[DFT_KB], bl[DFT_KB];
+  const hd_half8* wh = reinterpret_cast<const hd_half8*>(w.hi) + (size_t)mt * nks * 64u + lane;
+  const hd_half8* wl = reinterpret_cast<const hd_half8*>(w.lo) + (size_t)mt * nks * 64u + lane;
+  unsigned xo = (lane & 31u) * DFT_KP + 8u * (lane >> 5);
+  const hd_half8* bxh = reinterpret_cast<const hd_half8*>(xh + xo);
+  const hd_half8* bxl = reinterpret_cast<const hd_half8*>(xl + xo);
+  unsigned ks = 0;
+  for (; ks + DFT_KB <= nks; ks += DFT_KB) {
+    hd_half8 ah[DFT_KB], al[DFT_KB], bh[DFT_KB], bl[DFT_KB];
 #pragma unroll
     for (int u = 0; u < DFT_KB; ++u) {
-      ah[u] = *reinterpret_cast<const hd_half4*>(wh + k0 + 8u * u);
-      al[u] = *reinterpret_cast<const hd_half4*>(wl + k0 + 8u * u);
+      ah[u] = wh[(size_t)(ks + u) * 64u];
+      al[u] = wl[(size_t)(ks + u) * 64u];
     }
 #pragma unroll
     for (int u = 0; u < DFT_KB; ++u) {
-      bh[u] = *reinterpret_cast<const hd_half4*>(bxh + k0 + 8u * u);
-      bl[u] = *reinterpret_cast<const hd_half4*>(bxl + k0 + 8u * u);
+      bh[u] = bxh[2u * (ks + u)];
+      bl[u] = bxl[2u * (ks + u)];
     }
 #pragma unroll
     for (int u = 0; u < DFT_KB; ++u) {
-      main = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[u], bh[u], main, 0, 0, 0);
-      corr = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[u], bl[u], corr, 0, 0, 0);
-      corr = __builtin_amdgcn_mfma_f32_32x32x8f16(al[u], bh[u], corr, 0, 0, 0);
+      main = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bh[u], main, 0, 0, 0);
+      corr = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bl[u], corr, 0, 0, 0);
+      corr = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[u], bh[u], corr, 0, 0, 0);
     }
   }
-  for (; k0 < w.K; k0 += 8u) {
-    hd_half4 ah = *reinterpret_cast<const hd_half4*>(wh + k0);
-    hd_half4 al = *reinterpret_cast<const hd_half4*>(wl + k0);
-    hd_half4 bh = *reinterpret_cast<const hd_half4*>(bxh + k0);
-    hd_half4 bl = *reinterpret_cast<const hd_half4*>(bxl + k0);
-    main = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh, main, 0, 0, 0);
-    corr = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl, corr, 0, 0, 0);
-    corr = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh, corr, 0, 0, 0);
+  for (; ks < nks; ++ks) {
+    hd_half8 ah = wh[(size_t)ks * 64u], al = wl[(size_t)ks * 64u];
+    hd_half8 bh = bxh[2u * ks], bl = bxl[2u * ks];
+    main = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, main, 0, 0, 0);
+    corr = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, corr, 0, 0, 0);
+    corr = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, corr, 0, 0, 0);
   }
 }
 
@@ -241,7 +275,8 @@ __device__ __forceinline__ void dft_cplx_gemm(const HdDftArgs& a, const DftMat& 
 template <int MODE>
 __global__ void __launch_bounds__(DFT_THREADS) k_dft(HdDftArgs a, DftMat w, DftMat w2) {
   __shared__ float s_x[DFT_NL * DFT_XP];
-  __shared__ _Float16 xh[DFT_NL * DFT_KP], xl[DFT_NL * DFT_KP];
+  __shared__ __attribute__((aligned(16))) _Float16 xh[DFT_NL * DFT_KP];
+  __shared__ __attribute__((aligned(16))) _Float16 xl[DFT_NL * DFT_KP];
   __shared__ unsigned s_max;
   __shared__ double red_e[DFT_THREADS], red_w[DFT_THREADS];
   unsigned t = threadIdx.x, wave = t >> 6, lane = t & 63u;

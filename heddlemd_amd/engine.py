@@ -404,6 +404,12 @@ class Engine:
         self._chk(self._lib.hd_selftest_mfma(self._ctx, out), "hd_selftest_mfma")
         return out.reshape(64, 16)
 
+    def selftest_mfma_k16(self, bf16=False):
+        out = np.zeros(1024, np.float32)
+        self._chk(self._lib.hd_selftest_mfma_k16(self._ctx, int(bool(bf16)), out),
+                  "hd_selftest_mfma_k16")
+        return out.reshape(64, 16)
+
     def selftest_dpp(self):
         out = np.zeros(64, np.int32)
         self._chk(self._lib.hd_selftest_dpp(self._ctx, out), "hd_selftest_dpp")

@@ -396,10 +396,15 @@ int hd_get_neighbor_stats(hd_ctx *ctx, uint64_t out[6]);
 /* SPME grids of the last evaluation: rho (charge grid, na*nb*nc, from the
  * fixed-point spread) and V (potential grid after the convolution). */
 int hd_get_spme_grids(hd_ctx *ctx, float *rho, float *v);
-/* Device self test of the matrix-core lane layout used by the split
- * precision DFT (v_mfma_f32_32x32x8_f16): out[l*16 + v] = element v of
- * lane l of D = A B for the integer test matrices documented in k_dft.hip. */
+/* Device self test of a matrix-core lane layout (v_mfma_f32_32x32x8_f16):
+ * out[l*16 + v] = element v of lane l of D = A B for the integer test
+ * matrices documented in k_dft.hip.  No counterpart in the reference. */
 int hd_selftest_mfma(hd_ctx *ctx, float out[1024]);
+/* The same for the gfx950 doubled-K forms the split-precision DFT runs on,
+ * v_mfma_f32_32x32x16_f16 (bf16 = 0) or _bf16 (bf16 = 1), with the test
+ * matrices A[i][k] = (16 i + k) mod 61 - 30, B[k][j] = (32 k + j) mod 23 - 11.
+ * No counterpart in the reference. */
+int hd_selftest_mfma_k16(hd_ctx *ctx, int bf16, float out[1024]);
 /* Device self test of the wave64 rotation: out[l] = 7*((l+1)%64)+3. */
 int hd_selftest_dpp(hd_ctx *ctx, int32_t out[64]);
 /* Mean device time (ms) per occurrence of each stage over the steps of the

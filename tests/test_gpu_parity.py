@@ -47,6 +47,25 @@ def test_dpp_rotation_lane_layout():
     assert out.tolist() == [7 * ((l + 1) % 64) + 3 for l in range(64)]
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+def test_mfma_k16_lane_layout(bf16):
+    # the split-precision DFT relies on this A/B/D lane layout of the gfx950
+    # v_mfma_f32_32x32x16_{f16,bf16} (exact small-integer products)
+    say(f"case: mfma 32x32x16 selftest ({'bf16' if bf16 else 'f16'})")
+    from heddlemd_amd.engine import Engine
+    e = Engine(0)
+    out = e.selftest_mfma_k16(bf16)
+    e.close()
+    A = np.array([[(16 * i + k) % 61 - 30 for k in range(16)] for i in range(32)], np.float64)
+    B = np.array([[(32 * k + j) % 23 - 11 for j in range(32)] for k in range(16)], np.float64)
+    D = A @ B
+    want = np.zeros((64, 16))
+    for l in range(64):
+        for v in range(16):
+            want[l, v] = D[8 * (v // 4) + 4 * (l // 32) + v % 4, l % 32]
+    assert np.array_equal(out.astype(np.float64), want)
+
+
 def test_mfma_f16_lane_layout():
     # the split-precision DFT relies on this A/B/D lane layout of
     # v_mfma_f32_32x32x8_f16 (exact small-integer products)
