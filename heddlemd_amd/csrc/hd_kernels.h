@@ -64,7 +64,15 @@ struct HdPairArgs {
   int atom_lists;  // per-atom full lists (no j side) instead of the packed cluster list
   int ortho;       // no box tilt
   const unsigned* block_order;  // workgroup -> block for the packed-list kernel
+  const float4* lj_tab;  // ntypes > 4: the type-pair table (hd_pair_lj_table), device
 };
+// Largest type count of the LDS table form of the pair kernels.
+int hd_pair_table_max_types();
+// The type-pair table read by the pair kernels beyond four types: 8 floats
+// per pair k = i nt + j: 12 c12, 6 c6, c12, c6, cut^2 (-1 when eps = 0),
+// r_switch^2, 1/(cut^2 - r_switch^2) (0 without a switch), 0.
+void hd_pair_lj_table(int nt, const double* sigma, const double* eps, const double* cut,
+                      const double* rsw, float* out);
 // ev: 0 forces only, 1 + per-atom energy and virial, 2 + per-atom virial
 // only (no pair energy is accumulated)
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev);

@@ -158,6 +158,7 @@ struct hd_ctx {
          *vzlo = nullptr;
   float4 *snap = nullptr, *ref_pos = nullptr;
   float* d_lattice = nullptr;
+  float4* d_lj_tab = nullptr;  // more than four LJ types: the pair kernels' table
   float* ref_lattice = nullptr;
   float* baro_mu = nullptr;
   double* baro_diag = nullptr;
@@ -603,6 +604,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   pa.fpj_ew = c->fpj_ew;
   pa.atom_lists = c->atom_lists;
   pa.ortho = box_ortho(c) ? 1 : 0;
+  pa.lj_tab = c->d_lj_tab;
   stage_begin(c, "pair_force", timed);
   hd_launch_pair(st, pa, c->atom_lists && ev == 2 ? 1 : ev);
   if (!c->h_ang_k.empty())
@@ -1119,8 +1121,9 @@ int hd_set_images(hd_ctx* c, const int32_t* ix, const int32_t* iy, const int32_t
 int hd_set_lj(hd_ctx* c, int ntypes, const double* sigma, const double* eps, const double* cut,
               const double* rsw) {
   if (!c || !sigma || !eps || !cut || !rsw) return HD_ERR_ARG;
-  if (ntypes < 1 || ntypes > 4)
-    return fail(c, HD_ERR_UNSUPPORTED, "ntypes=%d: the pair kernel holds at most 4 types", ntypes);
+  if (ntypes < 1 || ntypes > hd_pair_table_max_types())
+    return fail(c, HD_ERR_UNSUPPORTED, "ntypes=%d: the pair kernels take at most %d types",
+                ntypes, hd_pair_table_max_types());
   if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_lj after hd_setup");
   int m = ntypes * ntypes;
   for (int k = 0; k < m; ++k) {
@@ -2099,6 +2102,14 @@ int hd_setup(hd_ctx* c) {
     DALLOC(c, c->vxlo, n); DALLOC(c, c->vylo, n); DALLOC(c, c->vzlo, n);
   }
   DALLOC(c, c->d_lattice, 6);
+  if (c->ntypes > 4) {
+    std::vector<float> tab(8 * (size_t)c->ntypes * c->ntypes);
+    hd_pair_lj_table(c->ntypes, c->lj_sigma.data(), c->lj_eps.data(), c->lj_cut.data(),
+                     c->lj_rsw.data(), tab.data());
+    DALLOC(c, c->d_lj_tab, tab.size() / 4);
+    int r = upload(c, c->d_lj_tab, reinterpret_cast<const float4*>(tab.data()), tab.size() / 4);
+    if (r) return r;
+  }
   DALLOC(c, c->ref_lattice, 6);
   DALLOC(c, c->baro_mu, 1);
   DALLOC(c, c->baro_diag, 4);

@@ -72,7 +72,7 @@ struct PairParams {
 template <int NT>
 struct PairJStage {
   float4 pq[128];
-  float2 lj[NT][128];
+  float2 lj[NT > 0 ? NT : 1][128];
   unsigned type[128];
   unsigned key[128];
 };
@@ -94,6 +94,28 @@ struct LjRow {
   float c6[NT], c12[NT], f6[NT], f12[NT], cut2[NT], rsw2[NT], idl[NT];
 };
 
+// NT = 0: more than four types.  The type-pair table sits in LDS (copied
+// in by every workgroup) and a lane keeps pointers to its i-type's row:
+// a[t] = (12 c12, 6 c6, c12, c6), b[t] = (cut^2, r_switch^2, 1/(cut^2 -
+// r_switch^2), 0) for j-type t.  One LDS read per pair replaces the
+// register select chain.
+#define PAIR_TAB_TYPES 32
+template <>
+struct LjRow<0> {
+  const float4* a;
+  const float4* b;
+  bool valid;
+};
+
+template <int NT>
+struct PairTab {
+  float4 a[1], b[1];
+};
+template <>
+struct PairTab<0> {
+  float4 a[PAIR_TAB_TYPES * PAIR_TAB_TYPES], b[PAIR_TAB_TYPES * PAIR_TAB_TYPES];
+};
+
 template <int NT>
 __device__ __forceinline__ float pick(const float (&v)[NT], unsigned t) {
   if (NT == 1) return v[0];
@@ -105,11 +127,58 @@ __device__ __forceinline__ float pick(const float (&v)[NT], unsigned t) {
   return r;
 }
 
+// Per-pair LJ coefficients of the i-row against j-type t.
+template <int NT>
+__device__ __forceinline__ float lj_f12(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.a[t].x; else return pick<NT>(R.f12, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_f6(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.a[t].y; else return pick<NT>(R.f6, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_c12(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.a[t].z; else return pick<NT>(R.c12, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_c6(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.a[t].w; else return pick<NT>(R.c6, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_cut2(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.valid ? R.b[t].x : -1.0f; else return pick<NT>(R.cut2, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_rsw2(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.b[t].y; else return pick<NT>(R.rsw2, t);
+}
+template <int NT>
+__device__ __forceinline__ float lj_idl(const LjRow<NT>& R, unsigned t) {
+  if constexpr (NT == 0) return R.b[t].z; else return pick<NT>(R.idl, t);
+}
+
 __device__ __forceinline__ void pin(float& x) { __asm__("" : "+v"(x)); }
 
+// Copy the type-pair table (2 float4 per pair, nt <= PAIR_TAB_TYPES) into
+// LDS; the caller synchronises before the first read.
+__device__ __forceinline__ void load_tab(PairTab<0>& T, const float4* __restrict__ lj_tab,
+                                         int nt) {
+  for (unsigned k = threadIdx.x; k < (unsigned)(nt * nt); k += blockDim.x) {
+    T.a[k] = lj_tab[2 * k];
+    T.b[k] = lj_tab[2 * k + 1];
+  }
+}
+
 template <int NT>
-__device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned itype, bool ivalid) {
+__device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned itype, bool ivalid,
+                                              const PairTab<NT>& T) {
   LjRow<NT> r;
+  if constexpr (NT == 0) {
+    r.a = &T.a[itype * (unsigned)P.ntypes];
+    r.b = &T.b[itype * (unsigned)P.ntypes];
+    r.valid = ivalid;
+    return r;
+  } else {
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     int k = (t < P.ntypes) ? (int)itype * P.ntypes + t : 0;
@@ -124,6 +193,7 @@ __device__ __forceinline__ LjRow<NT> load_row(const PairParams& P, unsigned ityp
     pin(r.cut2[t]); pin(r.rsw2[t]); pin(r.idl[t]);
   }
   return r;
+  }
 }
 
 // The j-atom's offset from the block centre in the image nearest it, as
@@ -207,7 +277,13 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
 #pragma unroll 8
   for (int t = 0; t < 64; ++t) {
     float4 pq = jb[t];
-    float2 ljc = (NT > 1) ? jl[t] : make_float2(R.f12[0], R.f6[0]);
+    float2 ljc;
+    if constexpr (NT == 0) {
+      float4 q = R.a[jt[t]];
+      ljc = make_float2(q.x, q.y);
+    } else {
+      ljc = (NT > 1) ? jl[t] : make_float2(R.f12[0], R.f6[0]);
+    }
     float qj = pq.w;
     float dx = pix - pq.x, dy = piy - pq.y, dz = piz - pq.z;
     float r2 = dx * dx + dy * dy + dz * dz;
@@ -240,8 +316,8 @@ __device__ __forceinline__ void uc_steps(const LjRow<NT>& R, float pix, float pi
     jfx = fmaf(nfactor, dx, jfx); jfy = fmaf(nfactor, dy, jfy); jfz = fmaf(nfactor, dz, jfz);
     if (EV == 1) {
       // LJ energy (lj.rs:237-247) and qq erfc(alpha r) / r (spme.rs:744-770)
-      unsigned jtype = (NT > 1) ? jt[t] : 0u;
-      float elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+      unsigned jtype = (NT != 1) ? jt[t] : 0u;
+      float elj = ir6 * (lj_c12(R, jtype) * ir6 - lj_c6(R, jtype));
       float energy = on ? (COUL ? fmaf(qq * gauss, poly * inv_r, elj) : elj) : 0.f;
       ee = fmaf(0.5f, energy, ee);
       je = fmaf(0.5f, energy, je);
@@ -274,7 +350,7 @@ __device__ __forceinline__ void pair_entries(
     bool jvalid = jslot < n;
     unsigned jsafe = jvalid ? jslot : 0u;
     float4 pj = tile_posq[jsafe];
-    unsigned jtype = (NT > 1) ? tile_type[jsafe] : 0u;
+    unsigned jtype = (NT != 1) ? tile_type[jsafe] : 0u;
     unsigned jkey = tile_key[jsafe];
     bool jin = jvalid && (jslot >> 6) == b;
     float pjx = pj.x, pjy = pj.y, pjz = pj.z, qj = jvalid ? pj.w : 0.f;
@@ -295,7 +371,7 @@ __device__ __forceinline__ void pair_entries(
       PairJStage<NT>& S = stage[w];
       float4 rec = make_float4(pjx, pjy, pjz, qj);
       S.pq[lane] = rec; S.pq[lane + 64] = rec;
-      if (NT > 1) {
+      if constexpr (NT > 1) {
         // the same products load_row forms for an i-atom of type ti
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti) {
@@ -310,7 +386,7 @@ __device__ __forceinline__ void pair_entries(
           S.lj[ti][lane] = c; S.lj[ti][lane + 64] = c;
         }
       }
-      if (EV == 1) { S.type[lane] = jtype; S.type[lane + 64] = jtype; }
+      if (EV == 1 || NT == 0) { S.type[lane] = jtype; S.type[lane + 64] = jtype; }
       S.key[lane] = jkey; S.key[lane + 64] = jkey;
       wave_lds_sync();
       const float2* jl = &S.lj[(NT > 1) ? itype : 0u][lane];
@@ -356,11 +432,11 @@ __device__ __forceinline__ void pair_entries(
       float inv_r2 = inv_r * inv_r;
       // LJ (lj.rs:237-247, 250-287) from the i-row, column = j type
       float ir6 = inv_r2 * inv_r2 * inv_r2;
-      float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+      float flj = inv_r2 * ir6 * (lj_f12(R, jtype) * ir6 - lj_f6(R, jtype));
       float elj = 0.f;
-      if (EV == 1 || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+      if (EV == 1 || SW) elj = ir6 * (lj_c12(R, jtype) * ir6 - lj_c6(R, jtype));
       if (SW) {
-        float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
+        float rs2 = lj_rsw2(R, jtype), idl = lj_idl(R, jtype);
         float tau = fmaxf((r2 - rs2) * idl, 0.0f);
         float omt = 1.0f - tau;
         float s = omt * omt * (1.0f + 2.0f * tau);
@@ -368,7 +444,7 @@ __device__ __forceinline__ void pair_entries(
         flj = s * flj + chain * elj;
         elj = s * elj;
       }
-      bool lj_on = r2 <= pick<NT>(R.cut2, jtype);
+      bool lj_on = r2 <= lj_cut2(R, jtype);
       // real-space Coulomb, f32 Hastings erfc (spme.rs:744-770)
       float r = r2 * inv_r;
       float qq = kqi * qj;
@@ -404,7 +480,7 @@ __device__ __forceinline__ void pair_entries(
       }
       pjx = hd_rot1(pjx); pjy = hd_rot1(pjy); pjz = hd_rot1(pjz); qj = hd_rot1(qj);
       jkey = hd_rot1_u(jkey);
-      if (NT > 1) jtype = hd_rot1_u(jtype);
+      if (NT != 1) jtype = hd_rot1_u(jtype);
       jfx = hd_rot1(jfx); jfy = hd_rot1(jfy); jfz = hd_rot1(jfz);
       if (EV == 1) je = hd_rot1(je);
       if (EV && !SPC) jw = hd_rot1(jw);
@@ -452,10 +528,16 @@ __global__ void __launch_bounds__(256) k_pair(
     const unsigned* __restrict__ entry_atoms, const unsigned* __restrict__ entry_count,
     const unsigned* __restrict__ block_order, unsigned cap, unsigned n,
     long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
-    unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew) {
+    unsigned long long* __restrict__ fpj, unsigned long long* __restrict__ fpj_ew,
+    const float4* __restrict__ lj_tab) {
   __shared__ long long red[PAIR_WAVES][5][64];
   __shared__ unsigned keymap[PAIR_KEYMAP_WORDS];
   __shared__ PairJStage<NT> stage[PAIR_WAVES];
+  __shared__ PairTab<NT> tab;
+  if constexpr (NT == 0) {
+    load_tab(tab, lj_tab, P.ntypes);
+    __syncthreads();
+  }
   unsigned b = block_order[blockIdx.x];  // heaviest blocks first
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HdLattice L = hd_load_lattice(lattice);
@@ -474,7 +556,7 @@ __global__ void __launch_bounds__(256) k_pair(
     hd_rel_to_center(pix, piy, piz, cb.x, cb.y, cb.z, L, pix, piy, piz);
     if (!ivalid) pix = piy = piz = PAIR_FAR_I;
   }
-  LjRow<NT> R = load_row<NT>(P, itype, ivalid);
+  LjRow<NT> R = load_row<NT>(P, itype, ivalid, tab);
   unsigned nent = entry_count[b];
   const unsigned* ents = entry_atoms + (size_t)b * cap * 64;
   long long ax = 0, ay = 0, az = 0, ae = 0, aw = 0;
@@ -561,11 +643,11 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
   float inv_r = __builtin_amdgcn_rsqf(r2);
   float inv_r2 = inv_r * inv_r;
   float ir6 = inv_r2 * inv_r2 * inv_r2;
-  float flj = inv_r2 * ir6 * (pick<NT>(R.f12, jtype) * ir6 - pick<NT>(R.f6, jtype));
+  float flj = inv_r2 * ir6 * (lj_f12(R, jtype) * ir6 - lj_f6(R, jtype));
   float elj = 0.f;
-  if (EV == 1 || SW) elj = ir6 * (pick<NT>(R.c12, jtype) * ir6 - pick<NT>(R.c6, jtype));
+  if (EV == 1 || SW) elj = ir6 * (lj_c12(R, jtype) * ir6 - lj_c6(R, jtype));
   if (SW) {
-    float rs2 = pick<NT>(R.rsw2, jtype), idl = pick<NT>(R.idl, jtype);
+    float rs2 = lj_rsw2(R, jtype), idl = lj_idl(R, jtype);
     float tau = fmaxf((r2 - rs2) * idl, 0.0f);
     float omt = 1.0f - tau;
     float sw = omt * omt * (1.0f + 2.0f * tau);
@@ -573,7 +655,7 @@ __device__ __forceinline__ void alist_pair(const PairParams& P, const HdLattice&
     flj = sw * flj + chain * elj;
     elj = sw * elj;
   }
-  bool lj_on = r2 <= pick<NT>(R.cut2, jtype);
+  bool lj_on = r2 <= lj_cut2(R, jtype);
   float r = r2 * inv_r;
   float qq = kqi * pj.w;
   float ar = P.alpha * r;
@@ -604,8 +686,14 @@ __global__ void __launch_bounds__(256) k_pair_alist(
     PairParams P, const float* __restrict__ lattice, const float4* __restrict__ tile_posq,
     const unsigned* __restrict__ tile_type, const unsigned* __restrict__ rows_in,
     const unsigned* __restrict__ row_count, unsigned cap, unsigned n, unsigned nblocks,
-    long long* __restrict__ fpi, long long* __restrict__ fpi_ew) {
+    long long* __restrict__ fpi, long long* __restrict__ fpi_ew,
+    const float4* __restrict__ lj_tab) {
   __shared__ long long red[PAIR_WAVES][5][64];
+  __shared__ PairTab<NT> tab;
+  if constexpr (NT == 0) {
+    load_tab(tab, lj_tab, P.ntypes);
+    __syncthreads();
+  }
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned b = blockIdx.x;
   HdLattice L = hd_load_lattice(lattice);
@@ -614,7 +702,7 @@ __global__ void __launch_bounds__(256) k_pair_alist(
   bool ivalid = islot < n;
   float4 pi = tile_posq[ivalid ? islot : 0u];
   unsigned itype = ivalid ? tile_type[islot] : 0u;
-  LjRow<NT> R = load_row<NT>(P, itype, ivalid);
+  LjRow<NT> R = load_row<NT>(P, itype, ivalid, tab);
   const float kqi = P.kc * (ivalid ? pi.w : 0.f);
   const unsigned rows = row_count[b];
   const unsigned* list = rows_in + (size_t)b * cap * 64 + lane;
@@ -634,7 +722,7 @@ __global__ void __launch_bounds__(256) k_pair_alist(
     for (int u = 0; u < ALIST_U; ++u) {
       unsigned jsafe = js[u] < n ? js[u] : 0u;
       pj[u] = tile_posq[jsafe];
-      tj[u] = (NT > 1) ? tile_type[jsafe] : 0u;
+      tj[u] = (NT != 1) ? tile_type[jsafe] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < ALIST_U; ++u)
@@ -1074,17 +1162,17 @@ static void launch_pair_t(hipStream_t st, const HdPairArgs& a, const PairParams&
     if (P.ortho)
       hipLaunchKernelGGL((k_pair_alist<EV, NT, SW, true>), grid, dim3(64 * PAIR_WAVES), 0, st, P,
                          a.lattice, a.tile_posq, a.tile_type, a.entry_atoms, a.entry_count,
-                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew);
+                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew, a.lj_tab);
     else
       hipLaunchKernelGGL((k_pair_alist<EV, NT, SW, false>), grid, dim3(64 * PAIR_WAVES), 0, st,
                          P, a.lattice, a.tile_posq, a.tile_type, a.entry_atoms, a.entry_count,
-                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew);
+                         a.cap, a.n, nblocks, a.fpi, a.fpi_ew, a.lj_tab);
     return;
   }
   hipLaunchKernelGGL((k_pair<EV, NT, SW>), dim3(nblocks), dim3(64 * PAIR_WAVES), 0, st, P,
                      a.lattice, a.tile_posq, a.tile_type, a.tile_key, a.center, a.half,
                      a.entry_atoms, a.entry_count, a.block_order, a.cap, a.n, a.fpi, a.fpi_ew,
-                     (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew);
+                     (unsigned long long*)a.fpj, (unsigned long long*)a.fpj_ew, a.lj_tab);
 }
 
 template <int EV, int NT>
@@ -1097,7 +1185,38 @@ template <int EV>
 static void launch_pair_ev(hipStream_t st, const HdPairArgs& a, const PairParams& P, bool sw) {
   if (P.ntypes <= 1) launch_pair_nt<EV, 1>(st, a, P, sw);
   else if (P.ntypes == 2) launch_pair_nt<EV, 2>(st, a, P, sw);
-  else launch_pair_nt<EV, 4>(st, a, P, sw);
+  else if (P.ntypes <= PAIR_MAXT) launch_pair_nt<EV, 4>(st, a, P, sw);
+  else launch_pair_nt<EV, 0>(st, a, P, sw);
+}
+
+// f32 coefficients of one type pair, as every pair kernel form uses them
+struct PairCoef {
+  float c6, c12, cut2, rsw2, idl;
+  bool sw;  // a switch region on an interacting pair
+};
+static PairCoef pair_coef(double s, double e, double rc, double rs) {
+  PairCoef q;
+  double s6 = s * s * s * s * s * s;
+  q.c6 = (float)(4.0 * e * s6);
+  q.c12 = (float)(4.0 * e * s6 * s6);
+  q.cut2 = (e != 0.0) ? (float)(rc * rc) : -1.0f;
+  q.rsw2 = (float)(rs * rs);
+  double d = rc * rc - rs * rs;
+  q.idl = d > 0.0 ? (float)(1.0 / d) : 0.0f;
+  q.sw = d > 0.0 && e != 0.0;
+  return q;
+}
+
+int hd_pair_table_max_types() { return PAIR_TAB_TYPES; }
+
+void hd_pair_lj_table(int nt, const double* sigma, const double* eps, const double* cut,
+                      const double* rsw, float* out) {
+  for (int k = 0; k < nt * nt; ++k) {
+    PairCoef q = pair_coef(sigma[k], eps[k], cut[k], rsw[k]);
+    float* o = out + 8 * (size_t)k;
+    o[0] = 12.0f * q.c12; o[1] = 6.0f * q.c6; o[2] = q.c12; o[3] = q.c6;
+    o[4] = q.cut2; o[5] = q.rsw2; o[6] = q.idl; o[7] = 0.0f;
+  }
 }
 
 void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
@@ -1105,19 +1224,21 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
   memset(&P, 0, sizeof(P));
   P.ntypes = a.ntypes;
   bool any_switch = false;
-  for (int i = 0; i < a.ntypes; ++i)
-    for (int j = 0; j < a.ntypes; ++j) {
-      int k = i * a.ntypes + j;
-      double s = a.sigma[k], e = a.eps[k];
-      double s6 = s * s * s * s * s * s;
-      P.c6[k] = (float)(4.0 * e * s6);
-      P.c12[k] = (float)(4.0 * e * s6 * s6);
-      double rc = a.cut[k], rs = a.rsw[k];
-      P.ljcut2[k] = (e != 0.0) ? (float)(rc * rc) : -1.0f;
-      P.rsw2[k] = (float)(rs * rs);
-      double d = rc * rc - rs * rs;
-      P.inv_delta[k] = d > 0.0 ? (float)(1.0 / d) : 0.0f;
-      if (d > 0.0 && e != 0.0) any_switch = true;
+  const int nt2 = a.ntypes * a.ntypes;
+  PairCoef q[PAIR_TAB_TYPES * PAIR_TAB_TYPES];
+  for (int k = 0; k < nt2; ++k) {
+    q[k] = pair_coef(a.sigma[k], a.eps[k], a.cut[k], a.rsw[k]);
+    if (q[k].sw) any_switch = true;
+  }
+  // up to four types the rows travel in the kernel arguments; beyond that
+  // the kernels read a.lj_tab (hd_pair_lj_table)
+  if (a.ntypes <= PAIR_MAXT)
+    for (int k = 0; k < nt2; ++k) {
+      P.c6[k] = q[k].c6;
+      P.c12[k] = q[k].c12;
+      P.ljcut2[k] = q[k].cut2;
+      P.rsw2[k] = q[k].rsw2;
+      P.inv_delta[k] = q[k].idl;
     }
   P.kc = (float)a.kc;
   P.alpha = (float)a.alpha;
@@ -1127,15 +1248,15 @@ void hd_launch_pair(hipStream_t st, const HdPairArgs& a, int ev) {
   P.excl_span = a.excl_span;
   P.ortho = a.ortho;
   bool uniform = P.coul_rc2 > 0.f && !any_switch;
-  for (int k = 0; k < a.ntypes * a.ntypes; ++k)
-    if (!(P.c6[k] == 0.f && P.c12[k] == 0.f) && P.ljcut2[k] != P.coul_rc2) uniform = false;
+  for (int k = 0; k < nt2; ++k)
+    if (!(q[k].c6 == 0.f && q[k].c12 == 0.f) && q[k].cut2 != P.coul_rc2) uniform = false;
   // LJ only: the interacting type pairs share one cutoff
   float ljc = -1.f;
   bool lj_uniform = !(P.coul_rc2 > 0.f) && !any_switch;
-  for (int k = 0; k < a.ntypes * a.ntypes; ++k) {
-    if (P.c6[k] == 0.f && P.c12[k] == 0.f) continue;
-    if (ljc < 0.f) ljc = P.ljcut2[k];
-    else if (P.ljcut2[k] != ljc) lj_uniform = false;
+  for (int k = 0; k < nt2; ++k) {
+    if (q[k].c6 == 0.f && q[k].c12 == 0.f) continue;
+    if (ljc < 0.f) ljc = q[k].cut2;
+    else if (q[k].cut2 != ljc) lj_uniform = false;
   }
   if (!(ljc > 0.f)) lj_uniform = false;
   P.uniform = uniform ? 1 : (lj_uniform ? 2 : 0);

@@ -99,7 +99,9 @@ int hd_set_velocities(hd_ctx *ctx, const float *vx, const float *vy, const float
 /* Image counters (io/init_state.rs InitImages); NULL resets to zero. */
 int hd_set_images(hd_ctx *ctx, const int32_t *ix, const int32_t *iy, const int32_t *iz);
 
-/* Lennard-Jones type-pair tables, ntypes <= 4 (src/forces/lj.rs:22-335). */
+/* Lennard-Jones type-pair tables, ntypes <= 32: up to four types the pair
+ * kernels hold the i-row in registers, beyond that they read the table from
+ * LDS (src/forces/lj.rs:22-335). */
 int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilon,
               const double *cutoff, const double *r_switch);
 /* SPME: real-space erfc pair term + reciprocal grid
