@@ -127,27 +127,52 @@ __device__ __forceinline__ size_t dft_cidx(const HdDftArgs& a, unsigned line, un
 // valid range), with coalesced global reads: rows for the c-axis passes,
 // consecutive c' for the b / a passes (real and imaginary parts of one
 // complex element land in rows k and n + k).
+// The global loads of DFT_SB elements per thread are issued before any of
+// them is stored, so their latencies overlap.
+#define DFT_SB 8
 template <int MODE>
 __device__ __forceinline__ void dft_stage(const HdDftArgs& a, unsigned line0, unsigned K,
                                           float* s_x) {
   unsigned t = threadIdx.x;
   if (MODE == DFT_C_FWD) {
-    unsigned nrow = a.na * a.nb;
-    for (unsigned e = t; e < DFT_NL * K; e += DFT_THREADS) {
-      unsigned ln = e / K, k = e - ln * K;
-      float v = 0.f;
-      if (line0 + ln < nrow && k < a.nc)
-        v = (float)a.rho[(size_t)(line0 + ln) * a.nc + k] * (1.0f / 4294967296.0f);
-      s_x[ln * DFT_XP + k] = v;
+    unsigned nrow = a.na * a.nb, total = DFT_NL * K;
+    for (unsigned e0 = t; e0 < total; e0 += DFT_SB * DFT_THREADS) {
+      float v[DFT_SB];
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e / K, k = e - ln * K;
+        v[u] = 0.f;
+        if (e < total && line0 + ln < nrow && k < a.nc)
+          v[u] = (float)a.rho[(size_t)(line0 + ln) * a.nc + k] * (1.0f / 4294967296.0f);
+      }
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e / K, k = e - ln * K;
+        if (e < total) s_x[ln * DFT_XP + k] = v[u];
+      }
     }
   } else if (MODE == DFT_C_INV) {
-    unsigned nrow = a.na * a.nb;
-    for (unsigned e = t; e < DFT_NL * a.nch; e += DFT_THREADS) {
-      unsigned ln = e / a.nch, k = e - ln * a.nch;
-      float2 v = make_float2(0.f, 0.f);
-      if (line0 + ln < nrow) v = a.cgrid[(size_t)(line0 + ln) * a.nch + k];
-      s_x[ln * DFT_XP + k] = v.x;
-      s_x[ln * DFT_XP + a.nch + k] = v.y;
+    unsigned nrow = a.na * a.nb, total = DFT_NL * a.nch;
+    for (unsigned e0 = t; e0 < total; e0 += DFT_SB * DFT_THREADS) {
+      float2 v[DFT_SB];
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e / a.nch, k = e - ln * a.nch;
+        v[u] = make_float2(0.f, 0.f);
+        if (e < total && line0 + ln < nrow) v[u] = a.cgrid[(size_t)(line0 + ln) * a.nch + k];
+      }
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e / a.nch, k = e - ln * a.nch;
+        if (e < total) {
+          s_x[ln * DFT_XP + k] = v[u].x;
+          s_x[ln * DFT_XP + a.nch + k] = v[u].y;
+        }
+      }
     }
     unsigned pad = K - 2 * a.nch;
     for (unsigned e = t; e < DFT_NL * pad; e += DFT_THREADS) {
@@ -157,13 +182,25 @@ __device__ __forceinline__ void dft_stage(const HdDftArgs& a, unsigned line0, un
   } else {
     constexpr bool along_b = (MODE == DFT_CPLX_B);
     unsigned n = along_b ? a.nb : a.na;
-    unsigned nl = (along_b ? a.na : a.nb) * a.nch;
-    for (unsigned e = t; e < DFT_NL * n; e += DFT_THREADS) {
-      unsigned ln = e % DFT_NL, kk = e / DFT_NL;
-      float2 v = make_float2(0.f, 0.f);
-      if (line0 + ln < nl) v = a.cgrid[dft_cidx<along_b>(a, line0 + ln, kk)];
-      s_x[ln * DFT_XP + kk] = v.x;
-      s_x[ln * DFT_XP + n + kk] = v.y;
+    unsigned nl = (along_b ? a.na : a.nb) * a.nch, total = DFT_NL * n;
+    for (unsigned e0 = t; e0 < total; e0 += DFT_SB * DFT_THREADS) {
+      float2 v[DFT_SB];
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e % DFT_NL, kk = e / DFT_NL;
+        v[u] = make_float2(0.f, 0.f);
+        if (e < total && line0 + ln < nl) v[u] = a.cgrid[dft_cidx<along_b>(a, line0 + ln, kk)];
+      }
+#pragma unroll
+      for (int u = 0; u < DFT_SB; ++u) {
+        unsigned e = e0 + u * DFT_THREADS;
+        unsigned ln = e % DFT_NL, kk = e / DFT_NL;
+        if (e < total) {
+          s_x[ln * DFT_XP + kk] = v[u].x;
+          s_x[ln * DFT_XP + n + kk] = v[u].y;
+        }
+      }
     }
   }
 }
