@@ -967,8 +967,10 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
   e.check(hd_set_lj(e.ctx, nt, c.sigma.data(), c.eps.data(), c.cutoff.data(), c.rswitch.data()),
           "pair interactions", 1);
   if (c.have_spme) {
-    if (c.spline_order != 4) fail1("spme.spline_order: only 4 is built");
-    e.check(hd_set_spme(e.ctx, c.alpha, c.r_cut_real, c.grid[0], c.grid[1], c.grid[2], 4, 1.0),
+    if (c.spline_order != 4 && c.spline_order != 5)
+      fail1("spme.spline_order: orders 4 and 5 are built");
+    e.check(hd_set_spme(e.ctx, c.alpha, c.r_cut_real, c.grid[0], c.grid[1], c.grid[2],
+                        c.spline_order, 1.0),
             "spme", 1);
   }
   if (c.all_pairs) fail1("neighbor_list.mode = \"all-pairs\" is not built (use cell-list)");

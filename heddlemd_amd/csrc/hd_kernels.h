@@ -124,6 +124,7 @@ struct HdSpmeArgs {
   const unsigned *voxel_rank, *voxel_off;
   unsigned nv[3];
   int mv[3];  // voxel margin per axis: >= half the skin in voxel units, plus one
+  int order;  // B-spline order, 4 or 5
 };
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
 // the spread half of hd_launch_spme (spline records + charge grid) alone
@@ -166,6 +167,7 @@ struct HdFinalizeArgs {
   const float4* spl;
   const unsigned* sg;
   unsigned na, nb, nc;
+  int order;  // B-spline order of the gather, 4 or 5
   double kc, alpha;
   const float* w_share;
   const float* lattice;

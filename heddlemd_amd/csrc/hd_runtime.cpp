@@ -77,6 +77,7 @@ struct hd_ctx {
   bool have_spme = false;
   double alpha = 0, coul_rc = 0, kc = 1.0;
   int grid[3] = {0, 0, 0};
+  int spline_order = 4;
   double r_skin = -1.0;
   unsigned excl_span = 0;
   int atom_lists = 0;  // 1: per-atom full neighbour lists instead of the packed cluster list
@@ -478,6 +479,7 @@ static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed) {
   HdSpmeArgs sa;
   memset(&sa, 0, sizeof(sa));
   sa.n = (unsigned)c->n;
+  sa.order = c->spline_order;
   sa.na = c->grid[0];
   sa.nb = c->grid[1];
   sa.nc = c->grid[2];
@@ -636,6 +638,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   fa.vgrid = c->have_spme ? c->vgrid : nullptr;
   fa.spl = c->spl;
   fa.sg = c->sg;
+  fa.order = c->spline_order;
   fa.na = c->grid[0];
   fa.nb = c->grid[1];
   fa.nc = c->grid[2];
@@ -1146,7 +1149,8 @@ int hd_set_spme(hd_ctx* c, double alpha, double r_cut, int na, int nb, int nc, i
                 double k_c) {
   if (!c) return HD_ERR_ARG;
   if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_spme after hd_setup");
-  if (order != 4) return fail(c, HD_ERR_UNSUPPORTED, "spline order %d: only 4 is built", order);
+  if (order != 4 && order != 5)
+    return fail(c, HD_ERR_UNSUPPORTED, "spline order %d: orders 4 and 5 are built", order);
   // the LDS FFT takes lines of 2^a 3^b 5^c points: up to 256 along a and
   // b, and an even count up to 512 along c (packed as nc/2 complex points)
   if (!is_smooth235(na) || !is_smooth235(nb) || !is_smooth235(nc) || nc % 2 || na < 8 ||
@@ -1162,6 +1166,7 @@ int hd_set_spme(hd_ctx* c, double alpha, double r_cut, int na, int nb, int nc, i
   c->grid[0] = na;
   c->grid[1] = nb;
   c->grid[2] = nc;
+  c->spline_order = order;
   c->kc = k_c;
   return HD_OK;
 }
@@ -2251,7 +2256,8 @@ int hd_setup(hd_ctx* c) {
     if ((r = upload(c, c->tw_a, ta.data(), ta.size()))) return r;
     if ((r = upload(c, c->tw_b, tb.data(), tb.size()))) return r;
     if ((r = upload(c, c->tw_c, tc.data(), tc.size()))) return r;
-    auto ba = b_factors(c->grid[0], 4), bb = b_factors(c->grid[1], 4), bc = b_factors(c->grid[2], 4);
+    int p = c->spline_order;
+    auto ba = b_factors(c->grid[0], p), bb = b_factors(c->grid[1], p), bc = b_factors(c->grid[2], p);
     DALLOC(c, c->bf_a, ba.size()); DALLOC(c, c->bf_b, bb.size()); DALLOC(c, c->bf_c, bc.size());
     if ((r = upload(c, c->bf_a, ba.data(), ba.size()))) return r;
     if ((r = upload(c, c->bf_b, bb.data(), bb.size()))) return r;

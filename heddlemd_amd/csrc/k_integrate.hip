@@ -759,7 +759,7 @@ void hd_launch_fmax2(hipStream_t st, const float* fx, const float* fy, const flo
 // Force finalize: pair fixed point + SPME gather (+ constraint virial),
 // scattered back to atom order, optionally fused with the post-force kick.
 // ---------------------------------------------------------------------
-#define GATHER_P 4
+template <int GATHER_P>
 __device__ __forceinline__ void bspline_wd(float x, float& w, float& dw) {
   // M_{p-1}(x) and M_{p-1}(x-1) by Cox-de Boor, then M_p (spme_spread_gather.cuh:56-86)
   float v[GATHER_P + 1];
@@ -782,12 +782,12 @@ __device__ __forceinline__ void bspline_wd(float x, float& w, float& dw) {
   w = x * inv * v[0] + ((float)GATHER_P - x) * inv * v[1];
 }
 
-template <bool EV, bool GATHER, bool KICK, bool LOSSLESS>
+template <bool EV, bool GATHER, bool KICK, bool LOSSLESS, int GATHER_P = 4>
 __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
   // With the SPME gather four lanes share a slot: lane q of the quad reads
-  // the a-plane da = q of the 4x4x4 stencil (16 grid values), so four
-  // times as many reads are in flight; the quad's partial sums are combined
-  // and its first lane finishes the slot.
+  // the a-planes da = q and q + 4 (< GATHER_P) of the P x P x P stencil
+  // (P^2 grid values each), so four times as many reads are in flight; the
+  // quad's partial sums are combined and its first lane finishes the slot.
   //
   // With a.partials set (energy or virial launches), the workgroup also sums
   // its slots' final per-atom energy and virial in FP64, in slot order, and
@@ -810,31 +810,37 @@ __global__ void __launch_bounds__(256) k_finalize(HdFinalizeArgs a) {
     float4 rec = a.spl[s];
     qi = rec.w;
     int ga0 = (int)(gcell & 1023u), gb0 = (int)((gcell >> 10) & 1023u), gc0 = (int)(gcell >> 20);
-    float wa, dwa, wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
+    float wb[GATHER_P], dwb[GATHER_P], wc[GATHER_P], dwc[GATHER_P];
     int gib[GATHER_P], gic[GATHER_P];
-    bspline_wd((float)q + rec.x, wa, dwa);
 #pragma unroll
     for (int d = 0; d < GATHER_P; ++d) {
-      bspline_wd((float)d + rec.y, wb[d], dwb[d]);
-      bspline_wd((float)d + rec.z, wc[d], dwc[d]);
+      bspline_wd<GATHER_P>((float)d + rec.y, wb[d], dwb[d]);
+      bspline_wd<GATHER_P>((float)d + rec.z, wc[d], dwc[d]);
       // periodic wrap (the origin cell lies in [0, n))
       gib[d] = gb0 - d;
       gib[d] += gib[d] < 0 ? (int)a.nb : 0;
       gic[d] = gc0 - d;
       gic[d] += gic[d] < 0 ? (int)a.nc : 0;
     }
-    int gia = ga0 - (int)q;
-    gia += gia < 0 ? (int)a.na : 0;
 #pragma unroll
-    for (int db = 0; db < GATHER_P; ++db) {
-      const float* row = a.vgrid + ((size_t)gia * a.nb + gib[db]) * a.nc;
+    for (int h = 0; h < (GATHER_P + 3) / 4; ++h) {
+      int da = (int)q + 4 * h;
+      if (da >= GATHER_P) break;
+      float wa, dwa;
+      bspline_wd<GATHER_P>((float)da + rec.x, wa, dwa);
+      int gia = ga0 - da;
+      gia += gia < 0 ? (int)a.na : 0;
 #pragma unroll
-      for (int dc = 0; dc < GATHER_P; ++dc) {
-        float v = row[gic[dc]];
-        phi += v * wa * wb[db] * wc[dc];
-        gra += v * dwa * wb[db] * wc[dc];
-        grb += v * wa * dwb[db] * wc[dc];
-        grc += v * wa * wb[db] * dwc[dc];
+      for (int db = 0; db < GATHER_P; ++db) {
+        const float* row = a.vgrid + ((size_t)gia * a.nb + gib[db]) * a.nc;
+#pragma unroll
+        for (int dc = 0; dc < GATHER_P; ++dc) {
+          float v = row[gic[dc]];
+          phi += v * wa * wb[db] * wc[dc];
+          gra += v * dwa * wb[db] * wc[dc];
+          grb += v * wa * dwb[db] * wc[dc];
+          grc += v * wa * wb[db] * dwc[dc];
+        }
       }
     }
   }
@@ -931,6 +937,17 @@ void hd_launch_finalize(hipStream_t st, const HdFinalizeArgs& a, bool ev) {
   if (!a.n) return;
   bool ga = a.vgrid != nullptr, k = a.kick, ll = a.vxlo != nullptr;
   dim3 g(hd_finalize_blocks(a.n, ga)), b(256);
+  if (ga && a.order == 5) {
+#define HD_FIN5(E, K, LL) \
+  if (ev == E && k == K && ll == LL) { \
+    hipLaunchKernelGGL((k_finalize<E, true, K, LL, 5>), g, b, 0, st, a); \
+    return; \
+  }
+    HD_FIN5(false, false, false) HD_FIN5(false, true, false) HD_FIN5(true, false, false)
+    HD_FIN5(true, true, false) HD_FIN5(false, false, true) HD_FIN5(false, true, true)
+    HD_FIN5(true, false, true) HD_FIN5(true, true, true)
+#undef HD_FIN5
+  }
 #define HD_FIN(E, G, K, LL) \
   if (ev == E && ga == G && k == K && ll == LL) { \
     hipLaunchKernelGGL((k_finalize<E, G, K, LL>), g, b, 0, st, a); \

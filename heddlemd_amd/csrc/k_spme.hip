@@ -24,23 +24,24 @@
 #include "hd_device.h"
 #include "hd_kernels.h"
 
-#define SPME_P 4
 #define FFT_THREADS 256
 #define FFT_TILE 2048  // complex elements staged per workgroup
 
-// Cardinal B-spline by Cox-de Boor at order SPME_P (spme_spread_gather.cuh:36-54).
+// Cardinal B-spline by Cox-de Boor at order P (spme_spread_gather.cuh:36-54);
+// the spread and the gather are built for P = 4 and 5.
+template <int P>
 __device__ __forceinline__ float bspline_w(float x) {
-  float v[SPME_P];
+  float v[P];
 #pragma unroll
-  for (int i = 0; i < SPME_P; ++i) {
+  for (int i = 0; i < P; ++i) {
     float xi = x - (float)i;
     v[i] = (xi >= 0.f && xi < 1.f) ? 1.f : 0.f;
   }
 #pragma unroll
-  for (int order = 2; order <= SPME_P; ++order) {
+  for (int order = 2; order <= P; ++order) {
     float inv = 1.0f / (float)(order - 1);
 #pragma unroll
-    for (int i = 0; i < SPME_P - order + 1; ++i) {
+    for (int i = 0; i < P - order + 1; ++i) {
       float xi = x - (float)i;
       v[i] = xi * inv * v[i] + ((float)order - xi) * inv * v[i + 1];
     }
@@ -107,6 +108,7 @@ __device__ __forceinline__ long long spread_fixed(float x) {
   return __float2ll_rn(r);
 }
 
+template <int SPME_P>
 __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
                                             const unsigned* __restrict__ sg, unsigned s,
                                             unsigned na, unsigned nb, unsigned nc, int c0a,
@@ -129,9 +131,9 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   float wa[SPME_P], wb[SPME_P], wc[SPME_P];
 #pragma unroll
   for (int d = 0; d < SPME_P; ++d) {
-    wa[d] = bspline_w((float)d + r.x);
-    wb[d] = bspline_w((float)d + r.y);
-    wc[d] = bspline_w((float)d + r.z);
+    wa[d] = bspline_w<SPME_P>((float)d + r.x);
+    wb[d] = bspline_w<SPME_P>((float)d + r.y);
+    wc[d] = bspline_w<SPME_P>((float)d + r.z);
   }
 #pragma unroll
   for (int iz = 0; iz < SPME_P; ++iz) {
@@ -158,6 +160,7 @@ __device__ __forceinline__ void spread_atom(const float4* __restrict__ spl,
   }
 }
 
+template <int SPME_P>
 __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
     const float4* __restrict__ spl, const unsigned* __restrict__ sg, unsigned na, unsigned nb,
     unsigned nc, unsigned ba, unsigned bb, unsigned bc, const unsigned* __restrict__ voxel_rank,
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
       for (unsigned i = 0; i < vcnt[j]; ++i) wslot[run++] = vs0[j] + i;
     __syncthreads();
     for (unsigned f = t; f < total; f += SPREAD_THREADS)
-      spread_atom(spl, sg, wslot[f], na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
+      spread_atom<SPME_P>(spl, sg, wslot[f], na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
   } else {
     // a window too wide to list: every thread walks its own voxels
     for (unsigned j = 0; j < per; ++j) {
@@ -242,7 +245,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) k_spread(
       unsigned r = voxel_rank[(va * nvb + vb) * nvc + vc];
       unsigned s0 = voxel_off[r], s1 = voxel_off[r + 1];
       for (unsigned sl = s0; sl < s1; ++sl)
-        spread_atom(spl, sg, sl, na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
+        spread_atom<SPME_P>(spl, sg, sl, na, nb, nc, c0a, c0b, c0c, ba, bb, bc, acc);
     }
   }
   __syncthreads();
@@ -774,10 +777,16 @@ void hd_launch_spme_spread(hipStream_t st, const HdSpmeArgs& a) {
   if (a.n > 0)
     hipLaunchKernelGGL(k_spline_setup, dim3((a.n + 255) / 256), dim3(256), 0, st, a.tile_posq,
                        a.lattice, a.na, a.nb, a.nc, a.spl, a.sg, a.n);
-  hipLaunchKernelGGL(k_spread, dim3(nbricks), dim3(SPREAD_THREADS), 0, st, a.spl, a.sg, a.na,
-                     a.nb, a.nc, brick[0], brick[1], brick[2], a.voxel_rank, a.voxel_off,
-                     a.nv[0], a.nv[1], a.nv[2], a.mv[0], a.mv[1], a.mv[2],
-                     (unsigned long long*)a.rho_fixed);
+  if (a.order == 5)
+    hipLaunchKernelGGL(k_spread<5>, dim3(nbricks), dim3(SPREAD_THREADS), 0, st, a.spl, a.sg,
+                       a.na, a.nb, a.nc, brick[0], brick[1], brick[2], a.voxel_rank, a.voxel_off,
+                       a.nv[0], a.nv[1], a.nv[2], a.mv[0], a.mv[1], a.mv[2],
+                       (unsigned long long*)a.rho_fixed);
+  else
+    hipLaunchKernelGGL(k_spread<4>, dim3(nbricks), dim3(SPREAD_THREADS), 0, st, a.spl, a.sg,
+                       a.na, a.nb, a.nc, brick[0], brick[1], brick[2], a.voxel_rank, a.voxel_off,
+                       a.nv[0], a.nv[1], a.nv[2], a.mv[0], a.mv[1], a.mv[2],
+                       (unsigned long long*)a.rho_fixed);
 }
 
 void hd_launch_recip_reduce(hipStream_t st, const HdSpmeArgs& a, unsigned n_partials) {

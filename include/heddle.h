@@ -107,8 +107,8 @@ int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilo
 /* SPME: real-space erfc pair term + reciprocal grid
  * (src/forces/spme.rs:36-52 SpmeParameters).  Grid sides of 2^a 3^b 5^c
  * points (the small primes the reference's cuFFT plans take), at least 8,
- * at most 256 along a and b and at most 512 (even) along c; order must be
- * 4. */
+ * at most 256 along a and b and at most 512 (even) along c; B-spline order
+ * 4 or 5. */
 int hd_set_spme(hd_ctx *ctx, double alpha, double r_cut, int na, int nb, int nc, int order,
                 double k_c);
 /* Reciprocal-space transform: mode 0 = FP32 radix FFT in LDS (default),

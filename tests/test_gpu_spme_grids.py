@@ -3,7 +3,9 @@ FFT: radix 4, 2, 3 and 5 stages), against the FP64 oracle on the same grid:
 the 8,192-molecule water box on 48 x 40 x 96 (radix 3 along a, 5 along b,
 the c axis packed to 48 complex points) and on 20 x 80 x 30 (radix 5 along
 a, an odd 15-point packed c axis).  The reference sizes its example grids
-this way for cuFFT (48 x 48 x 96, 20 x 20 x 80)."""
+this way for cuFFT (48 x 48 x 96, 20 x 20 x 80).  Also the order-5 B-spline
+interpolation (the reference's chignolin example) on 64^3 and 40^3, with
+both reciprocal transforms where they apply."""
 import dataclasses
 import sys
 
@@ -67,3 +69,33 @@ def test_mixed_radix_grid_runs_steps_deterministically():
     assert out[0][0].tobytes() == out[1][0].tobytes()
     assert out[0][1].tobytes() == out[1][1].tobytes()
     assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("grid,dft", [((64, 64, 64), 0), ((64, 64, 64), 1), ((40, 40, 40), 0)])
+def test_water_forces_spline_order_5(oracle, grid, dft):
+    say(f"case: water 8192, SPME order 5, grid {grid}, {'split DFT' if dft else 'FFT'}")
+    s = systems.config_water_8192()
+    s = dataclasses.replace(s, spme=dataclasses.replace(s.spme, grid=grid, order=5))
+    e = engine_for(s, dt=U.time(2e-15))
+    e.set_spme_dft(dft)
+    e.force_eval(scalars=True)
+    f = e.forces()
+    sc = e.scalars()
+    rho_g, v_g = e.spme_grids(grid)
+    e.close()
+    Fp, Ep, Wp, band = oracle.pair_forces(s.posq, s.types, s.lj.as_dict(), s.lattice,
+                                          coul=dict(alpha=s.spme.alpha, r_cut=s.spme.r_cut),
+                                          excl=s.exclusion_csr(), band=1e-6)
+    Fr, Er, Wr, er, wr, V = oracle.spme_recip(s.posq, s.lattice, grid, 5, s.spme.alpha,
+                                              want_v=True)
+    rho_o = oracle.spme_spread(s.posq, s.lattice, grid, 5)
+    rho_err = float(np.abs(rho_g - rho_o).max() / np.abs(rho_o).max())
+    v_err = float(np.abs(v_g - V).max() / np.abs(V).max())
+    say(f"  rho err {rho_err:.2e}, V err {v_err:.2e}, E_recip rel {abs(sc['e_recip'] / er - 1):.2e}, "
+        f"W_recip rel {abs(sc['w_recip'] / wr - 1):.2e}")
+    assert rho_err <= 4e-6
+    assert v_err <= 2e-5
+    assert sc["e_recip"] == pytest.approx(er, rel=2e-6)
+    assert sc["w_recip"] == pytest.approx(wr, rel=2e-6)
+    check_forces(f, Fp + Fr, band, f"  order 5 grid {grid}", say)
+    assert sc["pe"] == pytest.approx((Ep + Er).sum(), rel=2e-6)

@@ -75,7 +75,7 @@ def _golden(name):
         return json.load(f)
 
 
-def run_ewald_case(oracle, g):
+def run_ewald_case(oracle, g, order=4):
     pos = np.array(g["pos_bohr"])
     q = np.array(g["q"])
     Lb = g["box_bohr"]
@@ -83,7 +83,7 @@ def run_ewald_case(oracle, g):
     posq = np.zeros((len(q), 4), np.float32)
     posq[:, :3] = pos
     posq[:, 3] = q
-    F, E, W, er, wr = oracle.spme_recip(posq, L, tuple(g["grid"]), 4, g["alpha"])
+    F, E, W, er, wr = oracle.spme_recip(posq, L, tuple(g["grid"]), order, g["alpha"])
     ref = explicit_ewald(posq[:, :3].astype(np.float64), posq[:, 3].astype(np.float64),
                          L[:3], g["alpha"], g["kmax"])
     assert ref == pytest.approx(g["e_recip_explicit"], rel=1e-9)
@@ -98,6 +98,35 @@ def test_ewald_fixtures(oracle, case):
     run_ewald_case(oracle, _golden("spme_ewald_fixtures.json")[case])
 
 
+@pytest.mark.parametrize("case", ["two_charges", "four_charges", "four_charges_fine"])
+def test_ewald_fixtures_order5(oracle, case):
+    # the order-5 interpolation (the reference's chignolin example) meets
+    # the same explicit-Ewald bar, and is at least as close as order 4
+    g = _golden("spme_ewald_fixtures.json")[case]
+    run_ewald_case(oracle, g, order=5)
+    posq = np.zeros((len(g["q"]), 4), np.float32)
+    posq[:, :3] = np.array(g["pos_bohr"])
+    posq[:, 3] = np.array(g["q"])
+    L = np.array([g["box_bohr"]] * 3 + [0, 0, 0])
+    ref = g["e_recip_explicit"]
+    e4 = oracle.spme_recip(posq, L, tuple(g["grid"]), 4, g["alpha"])[3]
+    e5 = oracle.spme_recip(posq, L, tuple(g["grid"]), 5, g["alpha"])[3]
+    assert abs(e5 - ref) <= abs(e4 - ref) * 1.05 + 1e-15
+
+
+def test_b_factors_order5_finite(oracle):
+    # for odd p the modulus vanishes at k = n/2 (n even): M_5(1..4) = 1/24,
+    # 11/24, 11/24, 1/24 alternate to 0 there; the modulus is then the mean
+    # of its neighbours' (Essmann et al. 1995), equal by symmetry, so every
+    # factor is finite
+    for n in (20, 32, 48):
+        b = oracle.b_factors(n, 5)
+        assert b[0] == pytest.approx(1.0)
+        assert np.all(np.isfinite(b)) and np.all(b > 0)
+        if n % 2 == 0:
+            assert b[n // 2] == pytest.approx(0.5 * (b[n // 2 - 1] + b[n // 2 + 1]), rel=1e-12)
+
+
 def _water_like(n=60, L=18.0, seed=4):
     rng = np.random.default_rng(seed)
     posq = np.zeros((n, 4), np.float32)
@@ -106,17 +135,18 @@ def _water_like(n=60, L=18.0, seed=4):
     return posq, np.array([L, L, L, 0, 0, 0])
 
 
-def test_recip_force_is_energy_gradient(oracle):
+@pytest.mark.parametrize("order", [4, 5])
+def test_recip_force_is_energy_gradient(oracle, order):
     posq, L = _water_like()
     grid, alpha = (32, 32, 32), 0.35
-    F, E, W, er, wr = oracle.spme_recip(posq, L, grid, 4, alpha)
+    F, E, W, er, wr = oracle.spme_recip(posq, L, grid, order, alpha)
     h = 1.0 / 1024.0
     for atom in (0, 7, 31):
         for ax in range(3):
             p1 = posq.copy(); p1[atom, ax] += h
             p0 = posq.copy(); p0[atom, ax] -= h
-            e1 = oracle.spme_recip(p1, L, grid, 4, alpha)[3]
-            e0 = oracle.spme_recip(p0, L, grid, 4, alpha)[3]
+            e1 = oracle.spme_recip(p1, L, grid, order, alpha)[3]
+            e0 = oracle.spme_recip(p0, L, grid, order, alpha)[3]
             fd = -(e1 - e0) / (float(p1[atom, ax]) - float(p0[atom, ax]))
             assert F[atom, ax] == pytest.approx(fd, rel=2e-4, abs=2e-7)
 
