@@ -315,15 +315,69 @@ def write_flexible_water(d, n_steps=20):
 EXAMPLES = os.path.join(ROOT, "tests", "golden", "examples")
 
 
+def _water_topology(path, n_mol, label):
+    """The SPC/E water topology of the spc-water examples: one SETTLE group
+    (O, H1, H2 = 3m, 3m + 1, 3m + 2) of constraint type SPCE per molecule."""
+    with open(path, "w") as f:
+        f.write(f"# Topology for {label} SPC/E water molecules\n")
+        f.write("# O at 3m, H1 at 3m+1, H2 at 3m+2; each triple rigidly\n")
+        f.write("# constrained via SETTLE under the `SPCE` constraint type.\n\n")
+        f.write("[constraints]\n")
+        for m in range(n_mol):
+            f.write(f"{3 * m} {3 * m + 1} {3 * m + 2} SPCE\n")
+
+
+def _water_lattice_init(path, nx, ny, nz, a=4.0e-10, r_oh=1.0e-10, theta=1.910611931, seed=42):
+    """nx x ny x nz waters on a simple-cubic lattice of spacing a centred on
+    the origin, each oxygen on a site and the molecule turned by a rotation
+    uniform on SO(3) (Shoemake's quaternion construction) drawn from
+    Python's Mersenne Twister seeded with `seed`, three draws per molecule in
+    lattice order; H1 at r_oh along the local x axis, H2 at the H-O-H angle
+    in the local xy plane; SI units printed as %.9e."""
+    import math
+    import random
+    rng = random.Random(seed)
+    h1l = (r_oh, 0.0, 0.0)
+    h2l = (r_oh * math.cos(theta), r_oh * math.sin(theta), 0.0)
+    c = ((nx - 1) / 2.0, (ny - 1) / 2.0, (nz - 1) / 2.0)
+    out = [f"{3 * nx * ny * nz}\n",
+           f'Lattice="{nx * a:.9e} 0 0 0 {ny * a:.9e} 0 0 0 {nz * a:.9e}" '
+           "Properties=species:S:1:pos:R:3\n"]
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                o = ((i - c[0]) * a, (j - c[1]) * a, (k - c[2]) * a)
+                u1, u2, u3 = rng.random(), rng.random(), rng.random()
+                s1, s2 = math.sqrt(1.0 - u1), math.sqrt(u1)
+                w, x = s1 * math.sin(2 * math.pi * u2), s1 * math.cos(2 * math.pi * u2)
+                y, z = s2 * math.sin(2 * math.pi * u3), s2 * math.cos(2 * math.pi * u3)
+                R = ((1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)),
+                     (2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)),
+                     (2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)))
+                out.append(f"O  {o[0]:.9e} {o[1]:.9e} {o[2]:.9e}\n")
+                for hl in (h1l, h2l):
+                    h = [R[r][0] * hl[0] + R[r][1] * hl[1] + R[r][2] * hl[2] for r in range(3)]
+                    out.append(f"H  {o[0] + h[0]:.9e} {o[1] + h[1]:.9e} {o[2] + h[2]:.9e}\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
 def example_bundle(name, d):
     """Copy one of the reference's example bundles (tests/golden/examples)
-    into d; the lj-10000-argon init is written here: 20 x 20 x 25 atoms on a
-    simple-cubic lattice of spacing 4 Angstrom centred on the origin, in a
-    box of 8 x 8 x 10 nm, printed as %.9e."""
+    into d.  Written here instead of stored: the lj-10000-argon init (20 x
+    20 x 25 atoms on a simple-cubic lattice of spacing 4 Angstrom centred on
+    the origin, in a box of 8 x 8 x 10 nm, printed as %.9e), the one-line-
+    per-molecule SETTLE topologies of the spc-water-8192 and -65536 bundles,
+    and the -65536 init (32 x 32 x 64 waters, _water_lattice_init)."""
     import shutil
     src = os.path.join(EXAMPLES, name)
     for f in os.listdir(src):
         shutil.copy(os.path.join(src, f), os.path.join(d, f))
+    if name == "spc-water-8192":
+        _water_topology(os.path.join(d, "water.in.topology"), 8192, "8192")
+    if name == "spc-water-65536":
+        _water_topology(os.path.join(d, "water.in.topology"), 65536, "65,536")
+        _water_lattice_init(os.path.join(d, "water.in.xyz"), 32, 32, 64)
     if name == "lj-10000-argon":
         nx, ny, nz, a = 20, 20, 25, 4.0e-10
         with open(os.path.join(d, "argon.in.xyz"), "w") as f:

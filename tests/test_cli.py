@@ -87,7 +87,8 @@ def test_config_name_must_end_in_in_toml(tmp_path):
     assert r.returncode == 1 and ".in.toml" in r.stderr
 
 
-@pytest.mark.parametrize("name", ["lj-10000-argon", "spc-water-256", "chignolin"])
+@pytest.mark.parametrize("name", ["lj-10000-argon", "spc-water-256", "chignolin",
+                                  "spc-water-8192", "spc-water-65536"])
 def test_lint_accepts_the_reference_example_bundles(tmp_path, name):
     from tests.cli_inputs import example_bundle
     cfg = example_bundle(name, str(tmp_path))
@@ -95,6 +96,9 @@ def test_lint_accepts_the_reference_example_bundles(tmp_path, name):
     assert r.returncode == 0, r.stdout + r.stderr
     if name == "spc-water-256":
         assert "512 bonds, 256 angles" in r.stdout
+    if name.startswith("spc-water-") and name != "spc-water-256":
+        n_mol = int(name.split("-")[-1])
+        assert f"{3 * n_mol} particles" in r.stdout and f"{n_mol} constraint groups" in r.stdout
     if name == "chignolin":
         assert "2446 particles" in r.stdout
         assert "171 bonds, 299 angles, 567 dihedrals, 758 constraint groups" in r.stdout
