@@ -118,6 +118,9 @@ SIGNATURES = {
     "hd_get_spme_grids": (C.c_int, [_ctx, _vp, _vp]),
     "hd_selftest_mfma": (C.c_int, [_ctx, _f32p]),
     "hd_selftest_mfma_k16": (C.c_int, [_ctx, C.c_int, _f32p]),
+    "hd_set_mtk": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                             C.c_int, C.c_int, C.c_uint64]),
+    "hd_get_mtk": (C.c_int, [_ctx, _f64p]),
     "hd_selftest_dpp": (C.c_int, [_ctx, _i32p]),
     "hd_stage_times": (C.c_int, [_ctx, C.c_char_p, C.c_int, _f64p, C.c_int]),
     "hd_set_timing": (C.c_int, [_ctx, C.c_int]),

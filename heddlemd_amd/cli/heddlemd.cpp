@@ -495,7 +495,14 @@ Config load_config(const std::string& path) {
         fail1("phase `" + ph.name + "`: " + slot + " kind `" + sl.kind + "` is not built here");
       fail1("phase `" + ph.name + "`: unknown " + slot + " kind `" + sl.kind + "`");
     };
-    check(ph.integrator, "integrator", {"velocity-verlet", "langevin-baoab"}, {"mtk-npt"});
+    check(ph.integrator, "integrator", {"velocity-verlet", "langevin-baoab", "mtk-npt"}, {});
+    // mtk-npt owns the thermostat and the barostat (mtk_npt.rs:572-573)
+    if (ph.integrator.kind == "mtk-npt" && !ph.thermostat.kind.empty())
+      fail1("phase `" + ph.name + "`: integrator `mtk-npt` owns the thermostat; remove "
+            "[phase.thermostat]");
+    if (ph.integrator.kind == "mtk-npt" && !ph.barostat.kind.empty())
+      fail1("phase `" + ph.name + "`: integrator `mtk-npt` owns the barostat; remove "
+            "[phase.barostat]");
     check(ph.thermostat, "thermostat", {"nose-hoover-chain", "csvr", "berendsen", "andersen"}, {});
     check(ph.barostat, "barostat", {"c-rescale", "berendsen", "monte-carlo"}, {});
   }
@@ -1045,8 +1052,22 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
       e.check(hd_set_langevin(e.ctx, param(in, "friction", INV_TIME, U), param(in, "temperature", TEMPERATURE, U),
                               get_u64(in.params, "seed", "phase.integrator")),
               "integrator", 1);
+    } else if (in.kind == "mtk-npt") {
+      check_keys(in.params, {"kind", "temperature", "pressure", "tau_t", "tau_p", "chain_length",
+                             "yoshida_order", "n_resp"},
+                 "phase.integrator");
+      uint64_t d3 = 3, d1 = 1;
+      e.check(hd_set_integrator(e.ctx, ph.dt, 0), "integrator", 1);
+      e.check(hd_set_mtk(e.ctx, param(in, "temperature", TEMPERATURE, U),
+                         param(in, "pressure", PRESSURE, U), param(in, "tau_t", TIME, U),
+                         param(in, "tau_p", TIME, U),
+                         (int)get_u64(in.params, "chain_length", "phase.integrator", &d3),
+                         (int)get_u64(in.params, "yoshida_order", "phase.integrator", &d3),
+                         (int)get_u64(in.params, "n_resp", "phase.integrator", &d1), t.n_constraints),
+              "integrator", 1);
     } else {
-      fail1("phase.integrator: kind `" + in.kind + "` is not built (velocity-verlet, langevin-baoab)");
+      fail1("phase.integrator: kind `" + in.kind +
+            "` is not built (velocity-verlet, langevin-baoab, mtk-npt)");
     }
     const Slot& th = ph.thermostat;
     if (!th.kind.empty()) {
@@ -1131,6 +1152,11 @@ std::vector<std::pair<std::string, Dim>> extra_columns(const PhaseCfg& ph) {
   if (ph.thermostat.kind == "csvr") cols.push_back({"csvr_conserved", ENERGY});
   if (ph.thermostat.kind == "berendsen") cols.push_back({"berendsen_conserved", ENERGY});
   if (ph.thermostat.kind == "andersen") cols.push_back({"andersen_conserved", ENERGY});
+  if (ph.integrator.kind == "mtk-npt") {
+    cols.push_back({"pressure", PRESSURE});
+    cols.push_back({"box_volume", NONE});
+    cols.push_back({"mtk_npt_conserved", ENERGY});
+  }
   if (ph.barostat.kind == "c-rescale" || ph.barostat.kind == "berendsen") {
     cols.push_back({"pressure", PRESSURE});
     cols.push_back({"box_volume", NONE});
@@ -1164,6 +1190,12 @@ void log_row(OutFile& o, const Config& c, const PhaseCfg& ph, Engine& e, uint64_
                     fmt_e(U.to_user(ENERGY, ke)) + "," + fmt_e(U.to_user(TEMPERATURE, T));
   double baro[5] = {0, 0, 0, 0, 0};
   if (!ph.barostat.kind.empty() && !mc.on) e.check(hd_get_barostat(e.ctx, baro), "barostat", 2);
+  double mtk[5] = {0, 0, 0, 0, 0};
+  if (ph.integrator.kind == "mtk-npt") {
+    e.check(hd_get_mtk(e.ctx, mtk), "integrator", 2);
+    baro[1] = mtk[0];
+    baro[2] = mtk[1];
+  }
   double p0 = 0.0;
   if (ph.barostat.kind == "c-rescale") p0 = param(ph.barostat, "pressure", PRESSURE, U);
   for (auto& col : cols) {
@@ -1174,6 +1206,7 @@ void log_row(OutFile& o, const Config& c, const PhaseCfg& ph, Engine& e, uint64_
     else if (col.first == "mc_acceptance") v = mc.acceptance();
     else if (col.first == "mc_volume_step") v = mc.max_dv;
     else if (col.first == "mc_conserved") v = ke + pe + mc.pressure * mc.volume - mc.injection;
+    else if (col.first == "mtk_npt_conserved") v = ke + pe + mtk[2];
     else v = ke + pe + sc[5];  // thermostat conserved quantity
     row += "," + fmt_e(U.to_user(col.second, v));
   }

@@ -416,6 +416,16 @@ void hd_launch_scale_positions(hipStream_t st, float4* posq, const float* mu, un
 // Pairs within r_c (both atoms valid, not excluded) in the current list.
 void hd_launch_count_pairs(hipStream_t st, const HdPairArgs& a, unsigned long long* out);
 void hd_launch_fill_f32(hipStream_t st, float* p, float v, size_t count);
+// MTK-NPT cell-coupled half kick and drift (the drift wraps into the
+// lattice it is given, the already scaled one), and a plain velocity scale
+void hd_launch_mtk_kick(hipStream_t st, float* vx, float* vy, float* vz, const float* fx,
+                        const float* fy, const float* fz, const float* mass, float e, float phi,
+                        unsigned n);
+void hd_launch_mtk_drift(hipStream_t st, float4* posq, int* ix, int* iy, int* iz, const float* vx,
+                         const float* vy, const float* vz, const float* lattice, float e,
+                         float phi, unsigned n);
+void hd_launch_scale_velocities(hipStream_t st, float* vx, float* vy, float* vz, float f,
+                                unsigned n);
 void hd_launch_negate_velocities(hipStream_t st, float* vx, float* vy, float* vz, double* vxlo,
                                  double* vylo, double* vzlo, unsigned n);
 void hd_launch_selftest_mfma(hipStream_t st, float* out);

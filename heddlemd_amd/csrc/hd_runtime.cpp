@@ -160,6 +160,14 @@ struct hd_ctx {
   float4 *snap = nullptr, *ref_pos = nullptr;
   float* d_lattice = nullptr;
   float4* d_lj_tab = nullptr;  // more than four LJ types: the pair kernels' table
+  // MTK-NPT: fused thermostat + isotropic barostat, driven from the host
+  // (the reference's mtk_npt.rs is not graph-compatible either)
+  bool have_mtk = false;
+  double mtk_kt = 0, mtk_p = 0, mtk_tau_t = 0, mtk_tau_p = 0, mtk_g = 1, mtk_w = 0;
+  double mtk_peps = 0, mtk_eps = 0, mtk_last_p = 0, mtk_last_v = 0;
+  int mtk_chain = 3, mtk_yoshida = 3, mtk_nresp = 1;
+  uint64_t mtk_nc = 0;
+  std::vector<double> mtk_qp, mtk_xp, mtk_pp, mtk_qc, mtk_xc, mtk_pc;
   float* ref_lattice = nullptr;
   float* baro_mu = nullptr;
   double* baro_diag = nullptr;
@@ -1771,6 +1779,93 @@ static int upload_nhc_state(hd_ctx* c) {
   return HD_OK;
 }
 
+// Yoshida-Suzuki weights of the chain sub-steps (nose_hoover_chain.rs:60-94)
+static const double* yoshida_w(int order) {
+  static const double w1[1] = {1.0};
+  static const double w3[3] = {1.3512071919596577, -1.7024143839193155, 1.3512071919596577};
+  static const double w5[5] = {0.41449077179437574, 0.41449077179437574, -0.6579630871775030,
+                               0.41449077179437574, 0.41449077179437574};
+  static const double w7[7] = {0.7845136104775573, 0.2355732133593582, -1.1776799841788710,
+                               1.3151863206839023, -1.1776799841788710, 0.2355732133593582,
+                               0.7845136104775573};
+  return order == 1 ? w1 : order == 3 ? w3 : order == 5 ? w5 : w7;
+}
+
+// One Nose-Hoover chain sub-step of length dt on the host, the
+// Martyna-Tuckerman-Klein chain (Mol. Phys. 87, 1117 (1996)) in the
+// reference's form (nose_hoover_chain.rs:109-172): quarter-step kicks of
+// the chain momenta from the top of the chain down, each scaled by the
+// next link's exp(-dt/8 p/Q) before and after; the thermalised degrees of
+// freedom scaled by exp(-dt/2 p_1/Q_1); the chain positions moved by
+// dt/2 p/Q; the kicks again from the bottom up.  k2 = the thermalised
+// kinetic energy doubled (2K for the particles, p_eps^2/W for the cell);
+// returns the scale factor.
+static double chain_sub_step(std::vector<double>& xi, std::vector<double>& p,
+                             const std::vector<double>& q, double dt, double k2, double g,
+                             double kt) {
+  int m = (int)xi.size();
+  if (m == 0) return 1.0;
+  double k = k2;
+  for (int j = m - 1; j >= 0; --j) {
+    double sc = j == m - 1 ? 1.0 : exp(-dt / 8.0 * p[j + 1] / q[j + 1]);
+    p[j] *= sc;
+    double gj = j == 0 ? k - g * kt : p[j - 1] * p[j - 1] / q[j - 1] - kt;
+    p[j] += dt / 4.0 * gj;
+    p[j] *= sc;
+  }
+  double factor = exp(-dt / 2.0 * p[0] / q[0]);
+  k *= factor * factor;
+  for (int j = 0; j < m; ++j) xi[j] += dt / 2.0 * p[j] / q[j];
+  for (int j = 0; j < m; ++j) {
+    double sc = j == m - 1 ? 1.0 : exp(-dt / 8.0 * p[j + 1] / q[j + 1]);
+    p[j] *= sc;
+    double gj = j == 0 ? k - g * kt : p[j - 1] * p[j - 1] / q[j - 1] - kt;
+    p[j] += dt / 4.0 * gj;
+    p[j] *= sc;
+  }
+  return factor;
+}
+
+int hd_set_mtk(hd_ctx* c, double kt, double pressure, double tau_t, double tau_p, int chain,
+               int yoshida, int n_resp, uint64_t n_constraints) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_set_mtk after hd_setup");
+  if (!(kt > 0) || !std::isfinite(kt) || !std::isfinite(pressure) || !(tau_t > 0) ||
+      !std::isfinite(tau_t) || !(tau_p > 0) || !std::isfinite(tau_p) || chain < 1 ||
+      chain > 16 || n_resp < 1 || !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
+    return fail(c, HD_ERR_ARG, "invalid MTK-NPT parameters");
+  if (c->have_nhc || c->rescale || c->have_andersen || c->have_lan || c->have_baro)
+    return fail(c, HD_ERR_STATE,
+                "mtk-npt owns the thermostat and the barostat: no other one beside it");
+  c->have_mtk = true;
+  c->mtk_kt = kt;
+  c->mtk_p = pressure;
+  c->mtk_tau_t = tau_t;
+  c->mtk_tau_p = tau_p;
+  c->mtk_chain = chain;
+  c->mtk_yoshida = yoshida;
+  c->mtk_nresp = n_resp;
+  c->mtk_nc = n_constraints;
+  return HD_OK;
+}
+
+// masses and a fresh state (mtk_npt.rs:124-188); at hd_setup, once n is known
+static void mtk_init(hd_ctx* c) {
+  int m = c->mtk_chain;
+  long long g = (long long)(3 * c->n) - (long long)c->mtk_nc - 3;
+  c->mtk_g = (double)std::max(g, 1LL);
+  double kt = c->mtk_kt, t2 = c->mtk_tau_t * c->mtk_tau_t, p2 = c->mtk_tau_p * c->mtk_tau_p;
+  c->mtk_qp.assign(m, kt * t2);
+  c->mtk_qp[0] = c->mtk_g * kt * t2;
+  c->mtk_qc.assign(m, kt * t2);
+  c->mtk_w = (c->mtk_g + 3.0) * kt * p2;
+  c->mtk_xp.assign(m, 0.0);
+  c->mtk_pp.assign(m, 0.0);
+  c->mtk_xc.assign(m, 0.0);
+  c->mtk_pc.assign(m, 0.0);
+  c->mtk_peps = c->mtk_eps = c->mtk_last_p = c->mtk_last_v = 0.0;
+}
+
 int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_resp,
                uint64_t n_constraints) {
   if (!c) return HD_ERR_ARG;
@@ -1781,7 +1876,7 @@ int hd_set_nhc(hd_ctx* c, double kt, double tau, int chain, int yoshida, int n_r
     c->have_nhc = false;
     return HD_OK;
   }
-  if (c->have_lan || c->rescale || c->have_andersen)
+  if (c->have_lan || c->rescale || c->have_andersen || c->have_mtk)
     return fail(c, HD_ERR_STATE, "one thermostat per context: no Nose-Hoover chain beside it");
   if (!(tau > 0) || chain < 1 || chain > 16 || n_resp < 1 ||
       !(yoshida == 1 || yoshida == 3 || yoshida == 5 || yoshida == 7))
@@ -1813,9 +1908,9 @@ static int set_rescale(hd_ctx* c, int kind, double kt, double tau, uint64_t seed
   if (!c) return HD_ERR_ARG;
   if (!(kt > 0) || !(tau > 0) || !std::isfinite(kt) || !std::isfinite(tau))
     return fail(c, HD_ERR_ARG, "thermostat temperature and tau must be finite and > 0");
-  if (c->have_nhc || c->have_lan || c->have_andersen)
-    return fail(c, HD_ERR_STATE, "one thermostat per context: a Nose-Hoover chain, Andersen or "
-                                 "Langevin BAOAB is already set");
+  if (c->have_nhc || c->have_lan || c->have_andersen || c->have_mtk)
+    return fail(c, HD_ERR_STATE, "one thermostat per context: a Nose-Hoover chain, Andersen, "
+                                 "Langevin BAOAB or MTK-NPT is already set");
   if (c->ready) {
     // a new phase re-parameterises the same kind; the counter and the
     // injection account carry on
@@ -1836,7 +1931,7 @@ int hd_set_andersen(hd_ctx* c, double kt, double collision_rate, uint64_t seed) 
   if (!c) return HD_ERR_ARG;
   if (!(kt > 0) || !(collision_rate >= 0) || !std::isfinite(kt) || !std::isfinite(collision_rate))
     return fail(c, HD_ERR_ARG, "Andersen temperature must be > 0 and the collision rate >= 0");
-  if (c->have_nhc || c->have_lan || c->rescale)
+  if (c->have_nhc || c->have_lan || c->rescale || c->have_mtk)
     return fail(c, HD_ERR_STATE, "one thermostat per context");
   if (c->ready) {
     if (!c->have_andersen) return fail(c, HD_ERR_STATE, "the thermostat kind is fixed at hd_setup");
@@ -1882,7 +1977,7 @@ int hd_set_langevin(hd_ctx* c, double friction, double kt, uint64_t seed) {
   if (!c) return HD_ERR_ARG;
   if (!(friction > 0) || !(kt > 0) || !std::isfinite(friction) || !std::isfinite(kt))
     return fail(c, HD_ERR_ARG, "Langevin friction and temperature must be finite and > 0");
-  if (c->have_nhc || c->rescale || c->have_andersen)
+  if (c->have_nhc || c->rescale || c->have_andersen || c->have_mtk)
     return fail(c, HD_ERR_STATE, "Langevin BAOAB owns the thermostat: remove the other thermostat");
   if (c->lossless) return fail(c, HD_ERR_UNSUPPORTED, "Langevin BAOAB has no lossless mode");
   if (c->ready) {
@@ -1922,6 +2017,7 @@ int hd_set_crescale(hd_ctx* c, double pressure, double tau, double compressibili
     return HD_OK;
   }
   if (!(compressibility > 0) || !(kt >= 0)) return fail(c, HD_ERR_ARG, "invalid C-rescale parameters");
+  if (c->have_mtk) return fail(c, HD_ERR_STATE, "mtk-npt owns the barostat");
   c->have_baro = true;
   c->baro_stochastic = true;
   c->baro_p = pressure;
@@ -1946,6 +2042,7 @@ int hd_set_berendsen_barostat(hd_ctx* c, double pressure, double tau, double com
   }
   if (!(compressibility > 0) || !std::isfinite(pressure))
     return fail(c, HD_ERR_ARG, "invalid Berendsen barostat parameters");
+  if (c->have_mtk) return fail(c, HD_ERR_STATE, "mtk-npt owns the barostat");
   c->have_baro = true;
   c->baro_stochastic = false;
   c->baro_p = pressure;
@@ -2055,6 +2152,10 @@ int hd_setup(hd_ctx* c) {
   // refuses the combination (src/integrator/velocity_verlet.rs:222-229)
   if (c->lossless && has_constraints(c))
     return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support constraints (SETTLE/SHAKE)");
+  if (c->have_mtk && (has_constraints(c) || c->lossless))
+    return fail(c, HD_ERR_UNSUPPORTED,
+                "mtk-npt does not support constraints (SETTLE/SHAKE) or lossless mode");
+  if (c->have_mtk) mtk_init(c);
   if (c->lossless && c->have_baro)
     return fail(c, HD_ERR_UNSUPPORTED, "lossless mode does not support a barostat");
   HIPCHK(c, hipSetDevice(c->device));
@@ -2665,6 +2766,126 @@ static int poll_status(hd_ctx* c, unsigned k, int* flagged) {
   return HD_OK;
 }
 
+// MTK-NPT steps, the symmetric Trotter splitting of mtk_npt.rs:311-470 with
+// the host holding the chains and the barostat momentum p_eps: K and W are
+// read back after their device reductions (FP64 trees), so each step
+// synchronises a few times and runs eagerly, as the reference's does.
+static double sinh_over_x(double a) { return fabs(a) < 1e-6 ? 1.0 + a * a / 6.0 : sinh(a) / a; }
+
+static int mtk_ke(hd_ctx* c, double* ke) {
+  hd_launch_ke(c->stream, (unsigned)c->n, c->vx, c->vy, c->vz, c->mass, c->red_partials,
+               c->scal + 0, nullptr, 0.f, nullptr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(ke, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+static int mtk_virial(hd_ctx* c, double* w) {
+  HIPCHK(c, hipMemcpyAsync(w, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HD_OK;
+}
+
+static void mtk_cell_chain(hd_ctx* c, double dt) {
+  const double* yw = yoshida_w(c->mtk_yoshida);
+  for (int a = 0; a < c->mtk_yoshida; ++a)
+    for (int r = 0; r < c->mtk_nresp; ++r) {
+      double h = yw[a] * dt / (2.0 * c->mtk_nresp);
+      double f = chain_sub_step(c->mtk_xc, c->mtk_pc, c->mtk_qc, h,
+                                c->mtk_peps * c->mtk_peps / c->mtk_w, 1.0, c->mtk_kt);
+      c->mtk_peps *= f;
+    }
+}
+
+// particle chain: each sub-step's factor, rounded to f32, scales the
+// velocities; k follows in FP64 with the rounded factor
+static void mtk_particle_chain(hd_ctx* c, double dt, double* k) {
+  const double* yw = yoshida_w(c->mtk_yoshida);
+  for (int a = 0; a < c->mtk_yoshida; ++a)
+    for (int r = 0; r < c->mtk_nresp; ++r) {
+      double h = yw[a] * dt / (2.0 * c->mtk_nresp);
+      float f = (float)chain_sub_step(c->mtk_xp, c->mtk_pp, c->mtk_qp, h, 2.0 * *k, c->mtk_g,
+                                      c->mtk_kt);
+      hd_launch_scale_velocities(c->stream, c->vx, c->vy, c->vz, f, (unsigned)c->n);
+      *k *= (double)f * (double)f;
+    }
+}
+
+static void mtk_kick(hd_ctx* c, double dt) {
+  double alpha = (1.0 + 3.0 / c->mtk_g) * (c->mtk_peps / c->mtk_w);
+  double e = exp(-alpha * dt / 2.0);
+  double phi = 0.5 * dt * sinh_over_x(alpha * dt / 4.0) * exp(-alpha * dt / 4.0);
+  hd_launch_mtk_kick(c->stream, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->mass, (float)e,
+                     (float)phi, (unsigned)c->n);
+}
+
+static int mtk_steps(hd_ctx* c, int n_steps) {
+  const double dt = (double)(float)c->dt;  // the f32 time step, widened
+  float lat[6];
+  HIPCHK(c, hipMemcpyAsync(lat, c->d_lattice, sizeof(lat), hipMemcpyDeviceToHost, c->stream));
+  // W of the current positions (the forces are recomputed identically)
+  enqueue_forces(c, 1, false, false);
+  c->forces_valid = true;
+  int r = check_status(c);
+  if (r) return r;
+  for (int s = 0; s < n_steps; ++s) {
+    double k, w;
+    if ((r = mtk_ke(c, &k)) || (r = mtk_virial(c, &w))) return r;
+    double vol = (double)(lat[0] * lat[1] * lat[2]);
+    double pres = (2.0 * k + w) / (3.0 * vol);
+    mtk_cell_chain(c, dt);
+    mtk_particle_chain(c, dt, &k);
+    c->mtk_peps += 0.5 * dt * (3.0 * vol * (pres - c->mtk_p) + 6.0 / c->mtk_g * k);
+    mtk_kick(c, dt);
+    double beta = c->mtk_peps / c->mtk_w;
+    double eb = exp(beta * dt);
+    double phx = dt * sinh_over_x(beta * dt / 2.0) * exp(beta * dt / 2.0);
+    float mu = (float)eb;
+    for (int q = 0; q < 6; ++q) lat[q] = lat[q] * mu;
+    HIPCHK(c, hipMemcpyAsync(c->d_lattice, lat, sizeof(lat), hipMemcpyHostToDevice, c->stream));
+    hd_launch_mtk_drift(c->stream, c->posq, c->ix, c->iy, c->iz, c->vx, c->vy, c->vz,
+                        c->d_lattice, (float)eb, (float)phx, (unsigned)c->n);
+    c->mtk_eps += beta * dt;
+    enqueue_forces(c, 1, false, false);
+    if ((r = check_status(c))) return r;
+    if ((r = mtk_ke(c, &k)) || (r = mtk_virial(c, &w))) return r;
+    vol = (double)(lat[0] * lat[1] * lat[2]);
+    pres = (2.0 * k + w) / (3.0 * vol);
+    mtk_kick(c, dt);
+    double kk;
+    if ((r = mtk_ke(c, &kk))) return r;
+    c->mtk_peps += 0.5 * dt * (3.0 * vol * (pres - c->mtk_p) + 6.0 / c->mtk_g * kk);
+    mtk_particle_chain(c, dt, &kk);
+    c->mtk_last_p = pres;
+    c->mtk_last_v = vol;
+    mtk_cell_chain(c, dt);
+  }
+  for (int q = 0; q < 6; ++q) c->lattice[q] = (double)lat[q];
+  // the kinetic energy of the final velocities for the caller's scalars
+  double k;
+  return mtk_ke(c, &k);
+}
+
+int hd_get_mtk(hd_ctx* c, double* out) {
+  if (!c || !out) return HD_ERR_ARG;
+  if (!c->have_mtk || !c->ready) return fail(c, HD_ERR_STATE, "no MTK-NPT integrator set up");
+  // the conserved quantity less K + U (mtk_npt.rs:246-268)
+  double h = c->mtk_p * c->mtk_last_v + 0.5 * c->mtk_peps * c->mtk_peps / c->mtk_w;
+  for (int j = 0; j < c->mtk_chain; ++j) {
+    h += c->mtk_pp[j] * c->mtk_pp[j] / (2.0 * c->mtk_qp[j]);
+    h += c->mtk_pc[j] * c->mtk_pc[j] / (2.0 * c->mtk_qc[j]);
+    h += (j == 0 ? c->mtk_g : 1.0) * c->mtk_kt * c->mtk_xp[j];
+    h += c->mtk_kt * c->mtk_xc[j];
+  }
+  out[0] = c->mtk_last_p;
+  out[1] = c->mtk_last_v;
+  out[2] = h;
+  out[3] = c->mtk_peps;
+  out[4] = c->mtk_eps;
+  return HD_OK;
+}
+
 int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
   if (!c || !c->ready) return fail(c, HD_ERR_STATE, "hd_step before hd_setup");
   if (n_steps < 0) return fail(c, HD_ERR_ARG, "n_steps < 0");
@@ -2682,6 +2903,7 @@ int hd_step(hd_ctx* c, int n_steps, int scalars_last) {
       return r;
     }
   }
+  if (c->have_mtk) return mtk_steps(c, n_steps);
   // constraint iteration counters cover this call's steps
   if (c->d_sh_stats)
     HIPCHK(c, hipMemsetAsync(c->d_sh_stats, 0, 6 * sizeof(unsigned), c->stream));

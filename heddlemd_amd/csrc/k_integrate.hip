@@ -636,6 +636,81 @@ void hd_launch_negate_velocities(hipStream_t st, float* vx, float* vy, float* vz
                      vxlo, vylo, vzlo, n);
 }
 
+// MTK-NPT cell-coupled updates, isotropic (kernels/mtk.cu:1-70,
+// src/integrator/mtk_npt.rs:392-420).  The host forms the scalars in FP64
+// and rounds them once:
+//   kick:  v <- e v + phi (F / m),  e = exp(-alpha dt/2),
+//          phi = (dt/2) sinh(alpha dt/4)/(alpha dt/4) exp(-alpha dt/4),
+//          the exact solution of dv/dt = F/m - alpha v over dt/2;
+//   drift: x <- e x + phi v,  e = exp(beta dt),
+//          phi = dt sinh(beta dt/2)/(beta dt/2) exp(beta dt/2),
+//          the exact solution of dx/dt = v + beta x over dt, after which
+//          the box is scaled by e.  Positions are then wrapped into the
+//          scaled cell with their image counts (scaling keeps fractional
+//          coordinates, so only atoms that crossed a face move).
+__global__ void k_mtk_kick(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+                           const float* __restrict__ fx, const float* __restrict__ fy,
+                           const float* __restrict__ fz, const float* __restrict__ mass, float e,
+                           float phi, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float inv_m = 1.0f / mass[i];
+  vx[i] = e * vx[i] + phi * (fx[i] * inv_m);
+  vy[i] = e * vy[i] + phi * (fy[i] * inv_m);
+  vz[i] = e * vz[i] + phi * (fz[i] * inv_m);
+}
+
+__global__ void k_mtk_drift(float4* __restrict__ posq, int* __restrict__ ix, int* __restrict__ iy,
+                            int* __restrict__ iz, const float* __restrict__ vx,
+                            const float* __restrict__ vy, const float* __restrict__ vz,
+                            const float* __restrict__ lattice, float e, float phi, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  HdLattice L = hd_load_lattice(lattice);
+  float4 pq = posq[i];
+  float px = e * pq.x + phi * vx[i];
+  float py = e * pq.y + phi * vy[i];
+  float pz = e * pq.z + phi * vz[i];
+  int ka, kb, kc;
+  hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+  posq[i] = make_float4(px, py, pz, pq.w);
+  ix[i] += ka;
+  iy[i] += kb;
+  iz[i] += kc;
+}
+
+__global__ void k_scale_velocities(float* __restrict__ vx, float* __restrict__ vy,
+                                   float* __restrict__ vz, float f, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] *= f;
+  vy[i] *= f;
+  vz[i] *= f;
+}
+
+void hd_launch_mtk_kick(hipStream_t st, float* vx, float* vy, float* vz, const float* fx,
+                        const float* fy, const float* fz, const float* mass, float e, float phi,
+                        unsigned n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_mtk_kick, dim3((n + 255) / 256), dim3(256), 0, st, vx, vy, vz, fx, fy, fz,
+                     mass, e, phi, n);
+}
+
+void hd_launch_mtk_drift(hipStream_t st, float4* posq, int* ix, int* iy, int* iz, const float* vx,
+                         const float* vy, const float* vz, const float* lattice, float e,
+                         float phi, unsigned n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_mtk_drift, dim3((n + 255) / 256), dim3(256), 0, st, posq, ix, iy, iz, vx,
+                     vy, vz, lattice, e, phi, n);
+}
+
+void hd_launch_scale_velocities(hipStream_t st, float* vx, float* vy, float* vz, float f,
+                                unsigned n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_scale_velocities, dim3((n + 255) / 256), dim3(256), 0, st, vx, vy, vz, f,
+                     n);
+}
+
 __global__ void k_fill_f32(float* p, float v, size_t count) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;

@@ -110,6 +110,20 @@ class Engine:
         self._chk(self._lib.hd_set_andersen(self._ctx, float(kt), float(collision_rate),
                                             int(seed)), "hd_set_andersen")
 
+    def set_mtk(self, kt, pressure, tau_t, tau_p, chain_length=3, yoshida_order=3, n_resp=1,
+                n_constraints=0):
+        """MTK-NPT integrator (owns the thermostat and the barostat)."""
+        self._chk(self._lib.hd_set_mtk(self._ctx, float(kt), float(pressure), float(tau_t),
+                                       float(tau_p), int(chain_length), int(yoshida_order),
+                                       int(n_resp), int(n_constraints)), "hd_set_mtk")
+
+    def mtk(self):
+        """dict(pressure, volume, extra, p_eps, eps) of the last MTK step;
+        extra = the conserved quantity less K + U."""
+        out = np.zeros(5, np.float64)
+        self._chk(self._lib.hd_get_mtk(self._ctx, out), "hd_get_mtk")
+        return dict(zip(["pressure", "volume", "extra", "p_eps", "eps"], (float(x) for x in out)))
+
     def set_berendsen(self, kt, tau, n_constraints=0):
         self._chk(self._lib.hd_set_berendsen(self._ctx, float(kt), float(tau),
                                              int(n_constraints)), "hd_set_berendsen")
@@ -428,7 +442,7 @@ class Engine:
 
 def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_skin=None,
                velocities=None, crescale=None, langevin=None, device=0, pair_list=None,
-               csvr=None, berendsen=None, berendsen_barostat=None, andersen=None):
+               csvr=None, berendsen=None, berendsen_barostat=None, andersen=None, mtk=None):
     """Configure an Engine for a ``systems.System``; ``langevin`` =
     dict(friction, kt, seed) selects the BAOAB integrator; ``pair_list``
     selects the neighbour-list form (hd_set_pair_list)."""
@@ -472,6 +486,8 @@ def engine_for(system, dt=None, lossless=False, nhc_kt=None, nhc_tau=None, r_ski
         e.set_berendsen(n_constraints=system.n_constraints, **berendsen)
     if andersen is not None:
         e.set_andersen(**andersen)
+    if mtk is not None:
+        e.set_mtk(**mtk)
     if berendsen_barostat is not None:
         e.set_berendsen_barostat(**berendsen_barostat)
     if crescale is not None:

@@ -179,6 +179,21 @@ int hd_set_andersen(hd_ctx *ctx, double kt, double collision_rate, uint64_t seed
  * (the conserved quantity is K + U - out[0]), out[1] CSVR draw counter. */
 int hd_get_thermostat(hd_ctx *ctx, double out[2]);
 
+/* MTK-NPT integrator (src/integrator/mtk_npt.rs:88-470, kernels/mtk.cu):
+ * isotropic Martyna-Tobias-Klein NPT with a Nose-Hoover chain on the
+ * particles and one on the cell, in the reference's symmetric Trotter
+ * splitting; kt = k_B T, pressure the target, tau_t / tau_p the chain and
+ * cell periods, chain_length >= 1, yoshida_order 1, 3, 5 or 7, n_resp >= 1,
+ * n_constraints for the degrees of freedom.  It owns the thermostat and the
+ * barostat (no other one may be set) and, like the reference's, takes no
+ * constraints and no lossless mode; steps run eagerly (the chains and the
+ * cell momentum live on the host).  Set before hd_setup. */
+int hd_set_mtk(hd_ctx *ctx, double kt, double pressure, double tau_t, double tau_p,
+               int chain_length, int yoshida_order, int n_resp, uint64_t n_constraints);
+/* MTK-NPT state of the last step: out[0] pressure (2K + W)/(3V), out[1]
+ * volume, out[2] the conserved quantity less K + U (P V, the cell and chain
+ * energies; mtk_npt.rs:246-268), out[3] p_eps, out[4] eps = ln(V/V0)/3. */
+int hd_get_mtk(hd_ctx *ctx, double out[5]);
 /* Langevin BAOAB integrator (src/integrator/langevin_baoab.rs,
  * kernels/langevin.cu): friction gamma (1/time), k_B*T and the Philox seed
  * of the Ornstein-Uhlenbeck draws.  Replaces velocity Verlet for the whole

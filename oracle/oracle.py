@@ -502,6 +502,83 @@ class NHCState:
         return c
 
 
+class MtkNpt:
+    """MTK-NPT integrator in FP64 (src/integrator/mtk_npt.rs:124-470): the
+    symmetric Trotter splitting cell chain / particle chain / p_eps kick /
+    cell-coupled velocity half kick / cell-coupled drift with the box scaled
+    by exp(p_eps dt / W), the force evaluation, then the mirror image.  The
+    particle chain is the oracle's Nose-Hoover chain (NHCState, pinned
+    separately) applied with its cumulative factor; the cell chain is the
+    same sub-step on the one cell degree of freedom (K = p_eps^2 / W)."""
+
+    def __init__(self, n_atoms, n_constraints, kt, pressure, tau_t, tau_p, chain=3, yoshida=3,
+                 n_resp=1):
+        self.part = NHCState(n_atoms, n_constraints, kt, tau_t, chain, yoshida, n_resp)
+        self.g = max(float(3 * n_atoms - n_constraints - 3), 1.0)
+        self.part.g_dof = self.g
+        t2 = tau_t * tau_t
+        self.part.q = np.array([self.g * kt * t2] + [kt * t2] * (chain - 1), np.float64)
+        self.cell = NHCState(1, 0, kt, tau_t, chain, yoshida, n_resp)
+        self.cell.g_dof = 1.0
+        self.cell.q = np.full(chain, kt * t2)
+        self.kt = float(kt)
+        self.p0 = float(pressure)
+        self.w = (self.g + 3.0) * kt * tau_p * tau_p
+        self.p_eps = 0.0
+        self.last_p = 0.0
+        self.last_v = 0.0
+
+    @staticmethod
+    def _shx(a):
+        return 1.0 + a * a / 6.0 if abs(a) < 1e-6 else np.sinh(a) / a
+
+    def _cell_chain(self, dt):
+        k2 = self.p_eps * self.p_eps / self.w
+        f, _ = self.cell.half_step(dt, 0.5 * k2)
+        self.p_eps *= f
+
+    def _kick(self, v, f, m, dt):
+        a = (1.0 + 3.0 / self.g) * (self.p_eps / self.w)
+        e = np.exp(-a * dt / 2.0)
+        phi = 0.5 * dt * self._shx(a * dt / 4.0) * np.exp(-a * dt / 4.0)
+        return e * v + phi * (f / m)
+
+    def step(self, x, v, m, L, forces, dt, f, w):
+        """One step from positions x (n, 3), velocities v (n, 3), masses m
+        (n, 1), box lengths L (3,), the forces f and virial w at x;
+        forces(x, L) -> (F, W).  Returns x, v, L, f, w after the step."""
+        ke = 0.5 * float(np.sum(m * v * v))
+        vol = float(np.prod(L))
+        pres = (2.0 * ke + w) / (3.0 * vol)
+        self._cell_chain(dt)
+        fac, ke = self.part.half_step(dt, ke)
+        v = v * fac
+        self.p_eps += 0.5 * dt * (3.0 * vol * (pres - self.p0) + 6.0 / self.g * ke)
+        v = self._kick(v, f, m, dt)
+        b = self.p_eps / self.w
+        eb = np.exp(b * dt)
+        x = eb * x + dt * self._shx(b * dt / 2.0) * np.exp(b * dt / 2.0) * v
+        L = L * eb
+        f, w = forces(x, L)
+        ke = 0.5 * float(np.sum(m * v * v))
+        vol = float(np.prod(L))
+        pres = (2.0 * ke + w) / (3.0 * vol)
+        v = self._kick(v, f, m, dt)
+        ke = 0.5 * float(np.sum(m * v * v))
+        self.p_eps += 0.5 * dt * (3.0 * vol * (pres - self.p0) + 6.0 / self.g * ke)
+        fac, ke = self.part.half_step(dt, ke)
+        v = v * fac
+        self.last_p, self.last_v = pres, vol
+        self._cell_chain(dt)
+        return x, v, L, f, w
+
+    def conserved_extra(self):
+        """H - K - U (mtk_npt.rs:246-268)."""
+        return (self.p0 * self.last_v + 0.5 * self.p_eps ** 2 / self.w
+                + self.part.conserved_extra() + float(np.sum(self.cell.pxi ** 2 / (2 * self.cell.q)))
+                + self.kt * float(np.sum(self.cell.xi)))
+
+
 def csvr_factor(k, g_dof, seed, counter, c, kt_per_dof):
     """CSVR rescale (src/integrator/csvr.rs:188-240): (factor, K_new)."""
     kn = C.c_double(0.0)

@@ -238,3 +238,89 @@ def test_andersen_collisions_bit_exact_with_the_oracle(oracle):
     assert th["draws"] == 30
     assert v1.tobytes() == v.tobytes()
     assert abs(changed - (1 - (1 - p) ** 30)) < 0.05
+
+
+def _mtk_setup(seed=31):
+    s = _jittered_argon(n_atoms=4000, seed=seed)
+    s.lj.r_switch[:] = 0.9 * s.lj.cutoff
+    from heddlemd_amd.rng import maxwell_boltzmann
+    kt = U.temperature(150.0)
+    v0 = maxwell_boltzmann(s.mass, kt, seed=seed, n_constraints=0).astype(np.float32)
+    prm = dict(kt=kt, pressure=U.pressure(5e7), tau_t=U.time(1e-13), tau_p=U.time(5e-13))
+    return s, v0, prm
+
+
+def test_mtk_npt_steps_match_the_oracle(oracle):
+    # 10 MTK-NPT steps of 2 fs on 4,000 LJ atoms (switched LJ, 150 K, 500
+    # bar): positions (unwrapped), velocities, box and the cell momentum
+    # against the oracle's FP64 restatement of the same splitting
+    say("case: argon 4000 MTK-NPT, 10 steps vs the oracle")
+    s, v0, prm = _mtk_setup()
+    dt = float(np.float32(U.time(2e-15)))
+    e = engine_for(s, dt=dt, velocities=v0, mtk=prm)
+    e.force_eval(scalars=True)
+    e.step(10, scalars_last=True)
+    p, im = e.positions(images=True)
+    v1 = e.velocities()
+    L1 = e.lattice()
+    st = e.mtk()
+    sc = e.scalars()
+    e.close()
+    lj = s.lj.as_dict()
+
+    def forces(x, L):
+        posq = np.zeros((s.n, 4), np.float32)
+        posq[:, :3] = x
+        F, E, W, _ = oracle.pair_forces(posq, s.types, lj, np.array([*L, 0, 0, 0]))
+        return F, float(W.sum())
+
+    m = s.mass.astype(np.float64)[:, None]
+    x = s.posq[:, :3].astype(np.float64)
+    v = v0.T.astype(np.float64)
+    L = np.array(s.lattice[:3], np.float64)
+    mtk = oracle.MtkNpt(s.n, 0, prm["kt"], prm["pressure"], prm["tau_t"], prm["tau_p"])
+    f, w = forces(x, L)
+    for _ in range(10):
+        x, v, L, f, w = mtk.step(x, v, m, L, forces, dt, f, w)
+    xd = p[:, :3].astype(np.float64) + im.T * np.asarray(L1[:3])
+    dx = np.abs(xd - x).max()
+    dv = np.abs(v1.T - v).max() / np.abs(v).max()
+    say(f"  box {L1[0]:.6f} (oracle {L[0]:.6f}), p_eps {st['p_eps']:.6e} (oracle "
+        f"{mtk.p_eps:.6e}), max |dx| {dx:.2e} Bohr, max |dv|/|v|max {dv:.2e}, "
+        f"P {st['pressure']:.4e} (oracle {mtk.last_p:.4e})")
+    assert abs(L1[0] / L[0] - 1) < 2e-7
+    assert st["p_eps"] == pytest.approx(mtk.p_eps, rel=1e-3, abs=1e-3 * abs(mtk.p_eps) + 1e-9)
+    assert dx < 2e-4
+    assert dv < 1e-4
+    assert st["pressure"] == pytest.approx(mtk.last_p, rel=1e-4)
+    assert st["volume"] == pytest.approx(mtk.last_v, rel=1e-6)
+    assert sc["ke"] > 0
+
+
+def test_mtk_npt_conserves_its_extended_energy():
+    say("case: argon 4000 MTK-NPT, 400 steps, conserved quantity")
+    s, v0, prm = _mtk_setup(seed=33)
+    e = engine_for(s, dt=U.time(2e-15), velocities=v0, mtk=prm)
+    e.force_eval(scalars=True)
+    hs, vols = [], []
+    for _ in range(40):
+        e.step(10, scalars_last=True)
+        sc = e.scalars()
+        st = e.mtk()
+        hs.append(sc["ke"] + sc["pe"] + st["extra"])
+        vols.append(st["volume"])
+    e.close()
+    k_ref = 1.5 * s.n * prm["kt"]
+    drift = (max(hs) - min(hs)) / k_ref
+    say(f"  H spread / K {drift:.2e}; volume {vols[0]:.1f} -> {vols[-1]:.1f} Bohr^3")
+    assert drift < 5e-3
+    assert abs(vols[-1] / vols[0] - 1) > 1e-3
+
+
+def test_mtk_npt_refuses_other_thermostats_and_constraints():
+    s, v0, prm = _mtk_setup()
+    with pytest.raises(HeddleError, match="one thermostat|owns"):
+        engine_for(s, dt=U.time(2e-15), nhc_kt=prm["kt"], nhc_tau=U.time(1e-13), mtk=prm)
+    w = systems.spc_water(n_mol=512, n_side=8, grid=(32, 32, 32), r_cut_m=0.7e-9)
+    with pytest.raises(HeddleError, match="mtk-npt does not support constraints"):
+        engine_for(w, dt=U.time(2e-15), mtk=prm)

@@ -474,3 +474,96 @@ def test_andersen_collision_statistics(oracle):
     want = np.float32(np.sqrt(np.float32(kt) / np.float32(4.0))) * np.float32(
         oracle.philox_gaussian(11, 5, i, 0))
     assert v[0, i] == want
+
+
+def _lj_box(oracle, n_side=6, seed=2):
+    """n_side^3 argon atoms, jittered simple-cubic at liquid density, LJ
+    with the reference's cubic switch from 0.9 r_c (smooth forces)."""
+    from heddlemd_amd import systems, units as U
+    n = n_side ** 3
+    s = systems.argon_lattice(n_atoms=n, n_side=n_side, spacing_m=3.6e-10,
+                              box_m=3.6e-10 * n_side, cutoff_m=8.5e-10)
+    s.lj.r_switch[:] = 0.9 * s.lj.cutoff
+    rng = np.random.default_rng(seed)
+    x = s.posq[:, :3].astype(np.float64) + rng.normal(0, 0.15, size=(n, 3))
+    m = s.mass.astype(np.float64)[:, None]
+    lj = s.lj.as_dict()
+
+    def forces(xx, L):
+        posq = np.zeros((n, 4), np.float32)
+        posq[:, :3] = xx
+        F, E, W, _ = oracle.pair_forces(posq, s.types, lj, np.array([*L, 0, 0, 0]))
+        return F, float(W.sum()), float(E.sum())
+
+    return s, x, m, forces
+
+
+def test_mtk_reduces_to_nose_hoover_verlet_without_a_barostat(oracle):
+    # with an infinitely heavy cell (tau_p -> inf) the cell velocity
+    # p_eps / W stays 0: the MTK splitting is then the velocity Verlet step
+    # inside the particle chain's half steps, and the box does not move
+    from heddlemd_amd import units as U
+    s, x, m, forces = _lj_box(oracle)
+    kt = U.temperature(150.0)
+    rng = np.random.default_rng(4)
+    v = rng.normal(size=x.shape) * np.sqrt(kt / m)
+    L = np.array(s.lattice[:3], np.float64)
+    dt = float(np.float32(U.time(2e-15)))
+    mtk = oracle.MtkNpt(s.n, 0, kt, U.pressure(1e5), U.time(2e-13), 1e30)
+    nhc = oracle.NHCState(s.n, 0, kt, U.time(2e-13))
+    f, w, _ = forces(x, L)
+    xa, va, La, fa, wa = x.copy(), v.copy(), L.copy(), f.copy(), w
+    xb, vb, fb = x.copy(), v.copy(), f.copy()
+    for _ in range(30):
+        xa, va, La, fa, wa = mtk.step(xa, va, m, La, lambda q, l: forces(q, l)[:2], dt, fa, wa)
+        fac, _ = nhc.half_step(dt, 0.5 * float(np.sum(m * vb * vb)))
+        vb = vb * fac
+        vb = vb + 0.5 * dt * fb / m
+        xb = xb + dt * vb
+        fb = forces(xb, L)[0]
+        vb = vb + 0.5 * dt * fb / m
+        fac, _ = nhc.half_step(dt, 0.5 * float(np.sum(m * vb * vb)))
+        vb = vb * fac
+    assert abs(mtk.p_eps / mtk.w) < 1e-20
+    assert np.abs(La / L - 1).max() < 1e-15
+    assert np.abs(xa - xb).max() < 1e-9
+    assert np.abs(va - vb).max() < 1e-9 * np.abs(vb).max()
+
+
+def test_mtk_conserves_its_extended_energy_and_moves_the_box(oracle):
+    # 216 LJ atoms at 150 K under 500 bar for 400 steps of 2 fs: the MTK
+    # conserved quantity H = K + U + P V + cell and chain terms holds to
+    # 2e-3 of K while the volume changes by more than 0.5 %, and a wrong sign
+    # in the p_eps kick (planted here) breaks the conservation
+    from heddlemd_amd import units as U
+    s, x, m, forces = _lj_box(oracle)
+    kt = U.temperature(150.0)
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(size=x.shape) * np.sqrt(kt / m)
+    dt = float(np.float32(U.time(2e-15)))
+
+    def run(sign):
+        mtk = oracle.MtkNpt(s.n, 0, kt, U.pressure(5e7), U.time(1e-13), U.time(5e-13))
+        if sign < 0:
+            mtk.p0 = -mtk.p0  # planted mistake: the pressure term with the wrong sign
+        xx, vv, L = x.copy(), v0.copy(), np.array(s.lattice[:3], np.float64)
+        f, w, _ = forces(xx, L)
+        hs, vols = [], []
+        for i in range(400):
+            xx, vv, L, f, w = mtk.step(xx, vv, m, L, lambda q, l: forces(q, l)[:2], dt, f, w)
+            if sign < 0:
+                mtk.p0 = -mtk.p0
+                h = 0.5 * float(np.sum(m * vv * vv)) + forces(xx, L)[2] + mtk.conserved_extra()
+                mtk.p0 = -mtk.p0
+            else:
+                h = 0.5 * float(np.sum(m * vv * vv)) + forces(xx, L)[2] + mtk.conserved_extra()
+            hs.append(h)
+            vols.append(float(np.prod(L)))
+        k_mean = 0.5 * s.n * 3 * kt
+        return (max(hs) - min(hs)) / k_mean, abs(vols[-1] / vols[0] - 1)
+
+    drift, dv = run(+1)
+    assert drift < 2e-3, drift
+    assert dv > 5e-3, dv
+    drift_bad, _ = run(-1)
+    assert drift_bad > 10 * drift
