@@ -121,6 +121,8 @@ SIGNATURES = {
     "hd_set_mtk": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                              C.c_int, C.c_int, C.c_uint64]),
     "hd_get_mtk": (C.c_int, [_ctx, _f64p]),
+    "hd_add_settle": (C.c_int, [_ctx, C.c_uint64, _u32p, C.c_double, C.c_double, C.c_double,
+                                C.c_double]),
     "hd_selftest_dpp": (C.c_int, [_ctx, _i32p]),
     "hd_stage_times": (C.c_int, [_ctx, C.c_char_p, C.c_int, _f64p, C.c_int]),
     "hd_set_timing": (C.c_int, [_ctx, C.c_int]),

@@ -16,6 +16,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cinttypes>
 #include <cmath>
@@ -982,19 +983,28 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
   }
   if (c.all_pairs) fail1("neighbor_list.mode = \"all-pairs\" is not built (use cell-list)");
   if (c.r_skin >= 0) e.check(hd_set_neighbor(e.ctx, c.r_skin), "neighbor_list", 1);
-  // constraints: SETTLE groups (one geometry) and SHAKE groups
-  std::vector<uint32_t> settle;
-  int settle_type = -1;
+  // constraints: SETTLE groups by geometry (the type's d_OH, d_HH and the
+  // group's O and H masses, settle.rs:196-240) and SHAKE groups
+  std::vector<std::array<double, 4>> skeys;
+  std::vector<std::vector<uint32_t>> settle;
   std::vector<uint32_t> goff{0}, gatoms, coff{0};
   std::vector<uint8_t> ci, cj;
   std::vector<double> cd;
   for (size_t g = 0; g < t.groups.size(); ++g) {
     const ConstraintType& ct = c.ctypes[t.group_type[g]];
     if (ct.kind == "settle") {
-      if (settle_type >= 0 && (c.ctypes[settle_type].d_oh != ct.d_oh || c.ctypes[settle_type].d_hh != ct.d_hh))
-        fail1("settle constraint types with different geometries are not built");
-      settle_type = t.group_type[g];
-      for (uint32_t a : t.groups[g]) settle.push_back(a);
+      const auto& at = t.groups[g];
+      double m_o = s.mass[at[0]], m_h1 = s.mass[at[1]], m_h2 = s.mass[at[2]];
+      if (m_h1 != m_h2)
+        fail1("settle group " + std::to_string(g) + ": the two hydrogens must have equal masses");
+      std::array<double, 4> key = {ct.d_oh, ct.d_hh, m_o, m_h1};
+      size_t k = 0;
+      while (k < skeys.size() && skeys[k] != key) ++k;
+      if (k == skeys.size()) {
+        skeys.push_back(key);
+        settle.emplace_back();
+      }
+      for (uint32_t a : at) settle[k].push_back(a);
     } else {
       for (uint32_t a : t.groups[g]) gatoms.push_back(a);
       goff.push_back((uint32_t)gatoms.size());
@@ -1006,11 +1016,11 @@ void build_engine(Engine& e, const Config& c, const State& s, const Topology& t,
       coff.push_back((uint32_t)ci.size());
     }
   }
-  if (!settle.empty()) {
-    const ConstraintType& ct = c.ctypes[settle_type];
-    double m_o = s.mass[settle[0]], m_h = s.mass[settle[1]];
-    e.check(hd_set_settle(e.ctx, settle.size() / 3, settle.data(), ct.d_oh, ct.d_hh, m_o, m_h),
-            "settle", 1);
+  for (size_t k = 0; k < skeys.size(); ++k) {
+    const auto& q = skeys[k];
+    auto setter = k == 0 ? hd_set_settle : hd_add_settle;
+    e.check(setter(e.ctx, settle[k].size() / 3, settle[k].data(), q[0], q[1], q[2], q[3]), "settle",
+            1);
   }
   if (goff.size() > 1)
     e.check(hd_set_shake(e.ctx, goff.size() - 1, goff.data(), gatoms.data(), coff.data(), ci.data(),

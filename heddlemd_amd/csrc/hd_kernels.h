@@ -216,11 +216,17 @@ void hd_launch_lan_kick_drift(hipStream_t st, const HdIntegrateArgs& a, const Hd
 // O + A: Ornstein-Uhlenbeck velocity update, then x += v dt/2
 void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
 
+#define HD_SETTLE_KINDS 8
 struct HdSettleArgs {
   unsigned n_groups;
   const unsigned* groups;  // 3 per group: O, H1, H2
-  double d_oh, d_hh;  // constrained O-H and H-H distances
-  float m_o, m_h, dt;
+  double d_oh, d_hh;  // constrained O-H and H-H distances (kind 0)
+  float m_o, m_h, dt;  // masses of kind 0
+  // further geometries: kind k >= 1 of group g = gkind[g] (nullptr: every
+  // group is kind 0)
+  const unsigned char* gkind;
+  double kd_oh[HD_SETTLE_KINDS], kd_hh[HD_SETTLE_KINDS];
+  float km_o[HD_SETTLE_KINDS], km_h[HD_SETTLE_KINDS];
   float4* posq;
   const float4* snap;
   float *vx, *vy, *vz;

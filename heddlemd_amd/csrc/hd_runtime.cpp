@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -101,6 +102,11 @@ struct hd_ctx {
   HdExclPair* d_excl = nullptr;
   std::vector<uint32_t> h_groups;
   double d_oh = 0, d_hh = 0, m_o = 0, m_h = 0;
+  // SETTLE geometries beyond the first (hd_add_settle): kind per group and
+  // the kinds' (d_oh, d_hh, m_o, m_h); kind 0 is the scalar fields above
+  std::vector<uint8_t> h_gkind;
+  std::vector<std::array<double, 4>> settle_kinds;
+  unsigned char* d_gkind = nullptr;
   double dt = 0;
   bool lossless = false;
   bool have_nhc = false;
@@ -786,6 +792,16 @@ HdIntegrateArgs integrate_args(hd_ctx* c) {
   return ia;
 }
 
+static void settle_kinds_args(const hd_ctx* c, HdSettleArgs& sa) {
+  sa.gkind = c->d_gkind;
+  for (size_t k = 0; k < c->settle_kinds.size(); ++k) {
+    sa.kd_oh[k] = c->settle_kinds[k][0];
+    sa.kd_hh[k] = c->settle_kinds[k][1];
+    sa.km_o[k] = (float)c->settle_kinds[k][2];
+    sa.km_h[k] = (float)c->settle_kinds[k][3];
+  }
+}
+
 HdSettleArgs settle_args(hd_ctx* c) {
   HdSettleArgs sa;
   memset(&sa, 0, sizeof(sa));
@@ -795,6 +811,7 @@ HdSettleArgs settle_args(hd_ctx* c) {
   sa.d_hh = c->d_hh;
   sa.m_o = (float)c->m_o;
   sa.m_h = (float)c->m_h;
+  settle_kinds_args(c, sa);
   sa.dt = (float)c->dt;
   sa.posq = c->posq;
   sa.snap = c->snap;
@@ -1203,6 +1220,35 @@ int hd_set_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, d
   c->d_hh = d_hh;
   c->m_o = m_o;
   c->m_h = m_h;
+  c->h_gkind.assign(ng, 0);
+  c->settle_kinds.assign(1, {d_oh, d_hh, m_o, m_h});
+  return HD_OK;
+}
+
+int hd_add_settle(hd_ctx* c, uint64_t ng, const uint32_t* groups, double d_oh, double d_hh,
+                  double m_o, double m_h) {
+  if (!c || (ng && !groups)) return HD_ERR_ARG;
+  if (c->ready) return fail(c, HD_ERR_STATE, "hd_add_settle after hd_setup");
+  if (c->h_groups.empty()) return hd_set_settle(c, ng, groups, d_oh, d_hh, m_o, m_h);
+  if (!(d_oh > 0 && d_hh > 0 && d_hh < 2 * d_oh && m_o > 0 && m_h > 0))
+    return fail(c, HD_ERR_ARG, "invalid SETTLE geometry");
+  std::vector<char> seen(c->n, 0);
+  for (uint32_t a : c->h_groups) seen[a] = 1;
+  for (uint64_t k = 0; k < 3 * ng; ++k) {
+    if (groups[k] >= c->n) return fail(c, HD_ERR_ARG, "SETTLE atom %u out of range", groups[k]);
+    if (seen[groups[k]]) return fail(c, HD_ERR_ARG, "atom %u in two SETTLE groups", groups[k]);
+    seen[groups[k]] = 1;
+  }
+  std::array<double, 4> g = {d_oh, d_hh, m_o, m_h};
+  size_t kind = 0;
+  while (kind < c->settle_kinds.size() && c->settle_kinds[kind] != g) ++kind;
+  if (kind == c->settle_kinds.size()) {
+    if (kind >= HD_SETTLE_KINDS)
+      return fail(c, HD_ERR_UNSUPPORTED, "more than %d SETTLE geometries", HD_SETTLE_KINDS);
+    c->settle_kinds.push_back(g);
+  }
+  c->h_groups.insert(c->h_groups.end(), groups, groups + 3 * ng);
+  c->h_gkind.insert(c->h_gkind.end(), ng, (uint8_t)kind);
   return HD_OK;
 }
 
@@ -2271,6 +2317,10 @@ int hd_setup(hd_ctx* c) {
   if (!c->h_groups.empty()) {
     DALLOC(c, c->d_groups, c->h_groups.size());
     if ((r = upload(c, c->d_groups, c->h_groups.data(), c->h_groups.size()))) return r;
+    if (c->settle_kinds.size() > 1) {
+      DALLOC(c, c->d_gkind, c->h_gkind.size());
+      if ((r = upload(c, c->d_gkind, c->h_gkind.data(), c->h_gkind.size()))) return r;
+    }
   }
   if (!c->h_ang_k.empty()) {
     DALLOC(c, c->d_angles, c->h_angles.size());
@@ -2692,8 +2742,11 @@ int hd_constrain_velocities(hd_ctx* c) {
   memset(&sa, 0, sizeof(sa));
   sa.n_groups = (unsigned)(c->h_groups.size() / 3);
   sa.groups = c->d_groups;
+  sa.d_oh = c->d_oh;
+  sa.d_hh = c->d_hh;
   sa.m_o = (float)c->m_o;
   sa.m_h = (float)c->m_h;
+  settle_kinds_args(c, sa);
   sa.dt = 0.f;
   sa.posq = c->posq;
   sa.vx = c->vx;

@@ -169,11 +169,24 @@ struct SettleOut {
 // vin the velocities after the kick (f32).  Returns the constrained
 // positions, the velocities with the correction / dt, and the position
 // virials.
-__device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& a,
+// the geometry of group g: kind 0 from the scalar fields, others from the
+// kind table
+struct SettleGeom {
+  double d_oh, d_hh;
+  float m_o, m_h;
+};
+__device__ __forceinline__ SettleGeom settle_geom(const HdSettleArgs& a, unsigned g) {
+  unsigned k = a.gkind ? (unsigned)a.gkind[g] : 0u;
+  if (k == 0) return SettleGeom{a.d_oh, a.d_hh, a.m_o, a.m_h};
+  return SettleGeom{a.kd_oh[k], a.kd_hh[k], a.km_o[k], a.km_h[k]};
+}
+
+__device__ __forceinline__ SettleOut settle_water_positions(const SettleGeom& a,
                                                             const HdLattice& L,
                                                             const float4 (&p)[3],
                                                             const float4 (&q)[3],
-                                                            const float (&vin)[3][3]) {
+                                                            const float (&vin)[3][3],
+                                                            float dt) {
   const double mass[3] = {(double)a.m_o, (double)a.m_h, (double)a.m_h};
   const double im[3] = {1.0 / mass[0], 1.0 / mass[1], 1.0 / mass[2]};
   const double target[3] = {a.d_oh * a.d_oh, a.d_oh * a.d_oh, a.d_hh * a.d_hh};
@@ -201,7 +214,7 @@ __device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& 
   for (int k = 0; k < 3; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[k][d] = u[k][d];
-  const int n_newton = a.dt != 0.f ? HD_SETTLE_NEWTON_MAX : HD_SETTLE_PROJECT_MAX;
+  const int n_newton = dt != 0.f ? HD_SETTLE_NEWTON_MAX : HD_SETTLE_PROJECT_MAX;
   for (int it = 0; it < n_newton; ++it) {
     double bond[3][3], res[3], worst = 0.0;
 #pragma unroll
@@ -236,7 +249,7 @@ __device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& 
   double msum = mass[0] + mass[1] + mass[2], com[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) com[d] = (mass[0] * x[0][d] + mass[1] * x[1][d] + mass[2] * x[2][d]) / msum;
-  double idt = a.dt != 0.f ? 1.0 / (double)a.dt : 0.0;
+  double idt = dt != 0.f ? 1.0 / (double)dt : 0.0;
   SettleOut o;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -255,9 +268,9 @@ __device__ __forceinline__ SettleOut settle_water_positions(const HdSettleArgs& 
 // the velocities v are projected; with ACC (dt > 0) the velocity virial is
 // added to w.
 template <bool ACC>
-__device__ __forceinline__ void settle_water_velocities(const HdSettleArgs& a, const HdLattice& L,
+__device__ __forceinline__ void settle_water_velocities(const SettleGeom& a, const HdLattice& L,
                                                         const float4 (&p)[3], float (&v)[3][3],
-                                                        float (&w)[3]) {
+                                                        float (&w)[3], float dt) {
   const double mass[3] = {(double)a.m_o, (double)a.m_h, (double)a.m_h};
   const double im[3] = {1.0 / mass[0], 1.0 / mass[1], 1.0 / mass[2]};
   double y[3][3] = {{0.0, 0.0, 0.0}};
@@ -284,7 +297,7 @@ __device__ __forceinline__ void settle_water_velocities(const HdSettleArgs& a, c
   double msum = mass[0] + mass[1] + mass[2], com[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) com[d] = (mass[0] * y[0][d] + mass[1] * y[1][d] + mass[2] * y[2][d]) / msum;
-  double idt = (ACC && a.dt > 0.f) ? 1.0 / (double)a.dt : 0.0;
+  double idt = (ACC && dt > 0.f) ? 1.0 / (double)dt : 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     double kick[3];
@@ -296,7 +309,7 @@ __device__ __forceinline__ void settle_water_velocities(const HdSettleArgs& a, c
       kick[d] = im[k] * sh;
       v[k][d] = (float)((double)v[k][d] + kick[d]);
     }
-    if (ACC && a.dt > 0.f) {
+    if (ACC && dt > 0.f) {
       double arm[3] = {y[k][0] - com[0], y[k][1] - com[1], y[k][2] - com[2]};
       w[k] = (float)((double)w[k] + mass[k] * idt * dot3(kick, arm));
     }
@@ -324,7 +337,7 @@ __global__ void __launch_bounds__(128) k_settle_positions(HdSettleArgs a) {
     q[k] = a.snap[ids[k]];
     v[k][0] = a.vx[ids[k]]; v[k][1] = a.vy[ids[k]]; v[k][2] = a.vz[ids[k]];
   }
-  SettleOut o = settle_water_positions(a, L, p, q, v);
+  SettleOut o = settle_water_positions(settle_geom(a, g), L, p, q, v, a.dt);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     a.posq[ids[k]] = o.p[k];
@@ -348,7 +361,7 @@ __global__ void __launch_bounds__(128) k_settle_velocities(HdSettleArgs a) {
     v[k][0] = a.vx[ids[k]]; v[k][1] = a.vy[ids[k]]; v[k][2] = a.vz[ids[k]];
     w[k] = ACC ? a.w_cons[ids[k]] : 0.f;
   }
-  settle_water_velocities<ACC>(a, L, p, v, w);
+  settle_water_velocities<ACC>(settle_geom(a, g), L, p, v, w, a.dt);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     a.vx[ids[k]] = v[k][0]; a.vy[ids[k]] = v[k][1]; a.vz[ids[k]] = v[k][2];
@@ -394,8 +407,9 @@ __global__ void __launch_bounds__(128) k_kick_drift_settle(HdIntegrateArgs ia, H
     p[k] = make_float4(px, py, pz, pq.w);
     v[k][0] = vx; v[k][1] = vy; v[k][2] = vz;
   }
-  SettleOut o = settle_water_positions(a, L, p, q, v);
-  settle_water_velocities<true>(a, L, o.p, o.v, o.w);
+  SettleGeom G = settle_geom(a, g);
+  SettleOut o = settle_water_positions(G, L, p, q, v, a.dt);
+  settle_water_velocities<true>(G, L, o.p, o.v, o.w, a.dt);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     unsigned i = ids[k];

@@ -236,6 +236,12 @@ class Engine:
                                          np.asarray(d, np.float64)), "hd_set_shake")
         self._n_constraints_shake = len(ci)
 
+    def add_settle(self, groups, d_oh, d_hh, m_o, m_h):
+        """Further SETTLE groups with their own geometry (hd_add_settle)."""
+        g = _c(groups, np.uint32).reshape(-1)
+        self._chk(self._lib.hd_add_settle(self._ctx, g.shape[0] // 3, g, d_oh, d_hh, m_o, m_h),
+                  "hd_add_settle")
+
     def set_shake_tolerance(self, tol, rattle_tol, max_iter):
         self._chk(self._lib.hd_set_shake_tolerance(self._ctx, float(tol), float(rattle_tol),
                                                    int(max_iter)), "hd_set_shake_tolerance")

@@ -124,6 +124,13 @@ int hd_set_neighbor(hd_ctx *ctx, double r_skin);
  * groups = 3 * n_groups atom ids (O, H1, H2). */
 int hd_set_settle(hd_ctx *ctx, uint64_t n_groups, const uint32_t *groups, double d_oh,
                   double d_hh, double m_o, double m_h);
+/* A further set of SETTLE groups with their own geometry (another water
+ * model, or another constraint type in the reference's topology: settle.rs
+ * keeps d_OH, d_HH and the O/H masses per group, settle.rs:196-240); groups
+ * with an already known geometry join its kind.  At most 8 geometries per
+ * context; the first call behaves as hd_set_settle. */
+int hd_add_settle(hd_ctx *ctx, uint64_t n_groups, const uint32_t *groups, double d_oh,
+                  double d_hh, double m_o, double m_h);
 /* General SHAKE / RATTLE constraint groups (src/integrator/shake.rs,
  * kernels/shake.cu): group g holds atoms group_atoms[group_offsets[g] ..
  * group_offsets[g+1]) and constraints con_offsets[g] .. con_offsets[g+1]
