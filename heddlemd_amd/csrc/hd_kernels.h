@@ -271,6 +271,11 @@ struct HdShakeArgs {
   // fraction of the snapshot bond vector that the constraint moves
   const float* lam0;  // per-constraint nu applied before the sweeps (or null)
   float* lam_out;     // per-constraint total nu of this call (or null)
+  // with `big` set, the wave kernels take the groups big[0 .. n_big) and
+  // the lane-per-group kernels the groups small[0 .. n_small) (at most 4
+  // atoms and 6 constraints); without it every group runs on a wave
+  const unsigned *big, *small;
+  unsigned n_big, n_small;
   const unsigned* cgrp;  // constraint -> group (feature kernel)
   float* feat;           // n_cons x HD_SHAKE_NFEAT predictor inputs (feature kernel)
   unsigned n_cons;

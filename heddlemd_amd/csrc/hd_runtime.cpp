@@ -221,6 +221,10 @@ struct hd_ctx {
   unsigned *d_sh_goff = nullptr, *d_sh_atoms = nullptr, *d_sh_coff = nullptr,
            *d_sh_cij = nullptr, *d_sh_stats = nullptr;
   float* d_sh_r2 = nullptr;
+  // groups of at most 4 atoms and 6 constraints run one per lane
+  // (d_sh_small), the others one per wave (d_sh_big); both null when no
+  // group is small
+  unsigned *d_sh_small = nullptr, *d_sh_big = nullptr, sh_n_small = 0, sh_n_big = 0;
   // SHAKE multiplier predictor and the step log of plain-SHAKE solutions
   unsigned* d_sh_cgrp = nullptr;  // constraint -> group
   float *d_sh_feat = nullptr, *d_sh_nu0 = nullptr, *d_sh_nu = nullptr;
@@ -727,6 +731,10 @@ HdShakeArgs shake_args(hd_ctx* c, float dt) {
   a.feat = c->d_sh_feat;
   a.lam0 = c->sh_predict ? c->d_sh_nu0 : nullptr;
   a.lam_out = c->d_sh_nu;
+  a.big = c->d_sh_big;
+  a.small = c->d_sh_small;
+  a.n_big = c->sh_n_big;
+  a.n_small = c->sh_n_small;
   return a;
 }
 
@@ -2386,6 +2394,22 @@ int hd_setup(hd_ctx* c) {
     if (!c->h_sh_color.empty() &&
         (r = upload(c, c->d_sh_color, c->h_sh_color.data(), c->h_sh_color.size())))
       return r;
+    {
+      std::vector<uint32_t> small, big;
+      for (size_t g = 0; g + 1 < c->h_sh_goff.size(); ++g) {
+        uint32_t na = c->h_sh_goff[g + 1] - c->h_sh_goff[g];
+        uint32_t nc = c->h_sh_coff[g + 1] - c->h_sh_coff[g];
+        (na <= 4 && nc <= 6 ? small : big).push_back((uint32_t)g);
+      }
+      if (!small.empty()) {
+        DALLOC(c, c->d_sh_small, small.size());
+        DALLOC(c, c->d_sh_big, big.size());
+        if ((r = upload(c, c->d_sh_small, small.data(), small.size()))) return r;
+        if (!big.empty() && (r = upload(c, c->d_sh_big, big.data(), big.size()))) return r;
+        c->sh_n_small = (unsigned)small.size();
+        c->sh_n_big = (unsigned)big.size();
+      }
+    }
     if ((r = upload(c, c->d_sh_goff, c->h_sh_goff.data(), c->h_sh_goff.size()))) return r;
     if ((r = upload(c, c->d_sh_atoms, c->h_sh_atoms.data(), c->h_sh_atoms.size()))) return r;
     if ((r = upload(c, c->d_sh_coff, c->h_sh_coff.data(), c->h_sh_coff.size()))) return r;

@@ -1286,7 +1286,7 @@ __device__ __forceinline__ GroupBond group_bond(const HdShakeArgs& a, unsigned c
 __global__ __launch_bounds__(64) void k_shake_group(HdShakeArgs a) {
   __shared__ float sx[64], sy[64], sz[64], posx[64], posy[64], posz[64], winv[64];
   const unsigned lane = threadIdx.x;
-  const unsigned g = blockIdx.x;
+  const unsigned g = a.big ? a.big[blockIdx.x] : blockIdx.x;
   HdLattice L = hd_load_lattice(a.lattice);
   unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
   unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
@@ -1381,7 +1381,7 @@ __global__ __launch_bounds__(64) void k_rattle_group(HdShakeArgs a) {
   __shared__ float px[64], py[64], pz[64], vxs[64], vys[64], vzs[64];
   __shared__ float kx[64], ky[64], kz[64], winv[64];
   const unsigned lane = threadIdx.x;
-  const unsigned g = blockIdx.x;
+  const unsigned g = a.big ? a.big[blockIdx.x] : blockIdx.x;
   HdLattice L = hd_load_lattice(a.lattice);
   unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
   unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
@@ -1447,6 +1447,321 @@ __global__ __launch_bounds__(64) void k_rattle_group(HdShakeArgs a) {
   if (lane < na) {
     a.vx[atom] = vxs[lane]; a.vy[atom] = vys[lane]; a.vz[atom] = vzs[lane];
   }
+}
+
+// Small groups (at most SHAKE_SMALL_ATOMS atoms and SHAKE_SMALL_CONS
+// constraints: waters as SHAKE groups, short fragments) run one group per
+// lane instead of one per wave.  Each lane performs the wave kernels'
+// f32 operations in the same order: the frame scan is the wave scan
+// restricted to its first four lanes (offsets 1 and 2; the lanes beyond
+// the group hold zero bonds), a colour's constraints share no atom so they
+// are applied one after another in index order, and the centre-of-mass
+// sums are the xor butterfly's (v0 + v2) + (v1 + v3) (the other lanes add
+// exact zeros).  The sweep statistics are reduced over the wave before
+// the atomics.
+#define SHAKE_SMALL_ATOMS 4
+#define SHAKE_SMALL_CONS 6
+
+__device__ __forceinline__ void small_frame(float (&x)[SHAKE_SMALL_ATOMS],
+                                            float (&y)[SHAKE_SMALL_ATOMS],
+                                            float (&z)[SHAKE_SMALL_ATOMS], unsigned na,
+                                            const HdLattice& L) {
+  float b[3][SHAKE_SMALL_ATOMS];
+#pragma unroll
+  for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+    float bx = 0.f, by = 0.f, bz = 0.f;
+    if (k >= 1 && (unsigned)k < na) {
+      bx = x[k] - x[k - 1]; by = y[k] - y[k - 1]; bz = z[k] - z[k - 1];
+      hd_min_image(bx, by, bz, L);
+    }
+    b[0][k] = bx; b[1][k] = by; b[2][k] = bz;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float s1[SHAKE_SMALL_ATOMS], s2[SHAKE_SMALL_ATOMS];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) s1[k] = k >= 1 ? b[d][k - 1] + b[d][k] : b[d][k];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) s2[k] = k >= 2 ? s1[k - 2] + s1[k] : s1[k];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) b[d][k] = s2[k];
+  }
+  unsigned mid = na / 2;
+  float px = b[0][0], py = b[1][0], pz = b[2][0];
+#pragma unroll
+  for (int k = 1; k < SHAKE_SMALL_ATOMS; ++k)
+    if ((unsigned)k == mid) { px = b[0][k]; py = b[1][k]; pz = b[2][k]; }
+#pragma unroll
+  for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+    x[k] = b[0][k] - px; y[k] = b[1][k] - py; z[k] = b[2][k] - pz;
+  }
+}
+
+// the wave butterfly's total of four lane values with zeros elsewhere
+__device__ __forceinline__ float small_total(const float (&v)[SHAKE_SMALL_ATOMS]) {
+  return (v[0] + v[2]) + (v[1] + v[3]);
+}
+
+__device__ __forceinline__ void small_stats(unsigned* st, bool live, unsigned it, int max_iter) {
+  unsigned sum = live ? it : 0u, mx = sum, capped = (live && (int)it >= max_iter) ? 1u : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += (unsigned)__shfl_xor((int)sum, off);
+    mx = max(mx, (unsigned)__shfl_xor((int)mx, off));
+    capped += (unsigned)__shfl_xor((int)capped, off);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    if (sum) atomicAdd(&st[0], sum);
+    atomicMax(&st[1], mx);
+    if (capped) atomicAdd(&st[2], capped);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_shake_small(HdShakeArgs a) {
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = t < a.n_small;
+  unsigned g = live ? a.small[t] : 0u;
+  HdLattice L = hd_load_lattice(a.lattice);
+  int sweeps = 0;
+  if (live) {
+    unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+    unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+    unsigned atom[SHAKE_SMALL_ATOMS];
+    float ux[SHAKE_SMALL_ATOMS], uy[SHAKE_SMALL_ATOMS], uz[SHAKE_SMALL_ATOMS];
+    float sx[SHAKE_SMALL_ATOMS], sy[SHAKE_SMALL_ATOMS], sz[SHAKE_SMALL_ATOMS];
+    float mk[SHAKE_SMALL_ATOMS], wv[SHAKE_SMALL_ATOMS];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+      atom[k] = 0u; ux[k] = uy[k] = uz[k] = sx[k] = sy[k] = sz[k] = mk[k] = wv[k] = 0.f;
+      if ((unsigned)k < na) {
+        atom[k] = a.gatoms[a0 + k];
+        float4 p = a.posq[atom[k]], q = a.snap[atom[k]];
+        ux[k] = p.x; uy[k] = p.y; uz[k] = p.z;
+        sx[k] = q.x; sy[k] = q.y; sz[k] = q.z;
+        mk[k] = a.mass[atom[k]];
+        wv[k] = mk[k] > 0.f ? 1.0f / mk[k] : 0.f;
+      }
+    }
+    small_frame(sx, sy, sz, na, L);
+    small_frame(ux, uy, uz, na, L);
+    float px[SHAKE_SMALL_ATOMS], py[SHAKE_SMALL_ATOMS], pz[SHAKE_SMALL_ATOMS];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) { px[k] = ux[k]; py[k] = uy[k]; pz[k] = uz[k]; }
+    unsigned ci[SHAKE_SMALL_CONS], cj[SHAKE_SMALL_CONS], col[SHAKE_SMALL_CONS];
+    float gx[SHAKE_SMALL_CONS], gy[SHAKE_SMALL_CONS], gz[SHAKE_SMALL_CONS];
+    float ws[SHAKE_SMALL_CONS], dd[SHAKE_SMALL_CONS], acc[SHAKE_SMALL_CONS];
+#pragma unroll
+    for (int c = 0; c < SHAKE_SMALL_CONS; ++c) {
+      ci[c] = cj[c] = 0u; col[c] = 0xFFFFFFFFu;
+      gx[c] = gy[c] = gz[c] = ws[c] = dd[c] = acc[c] = 0.f;
+      if ((unsigned)c < nc) {
+        unsigned pk = a.cij[c0 + c];
+        ci[c] = pk & 0xFFu; cj[c] = (pk >> 8) & 0xFFu;
+        col[c] = a.color[c0 + c];
+        float sxi = 0.f, syi = 0.f, szi = 0.f, sxj = 0.f, syj = 0.f, szj = 0.f, wi = 0.f, wj = 0.f;
+#pragma unroll
+        for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+          if ((unsigned)k == ci[c]) { sxi = sx[k]; syi = sy[k]; szi = sz[k]; wi = wv[k]; }
+          if ((unsigned)k == cj[c]) { sxj = sx[k]; syj = sy[k]; szj = sz[k]; wj = wv[k]; }
+        }
+        gx[c] = sxi - sxj; gy[c] = syi - syj; gz[c] = szi - szj;
+        ws[c] = wi + wj;
+        dd[c] = a.cr2[c0 + c];
+      }
+    }
+    // one constraint's move of its two atoms
+    auto move = [&](int c, float lam) {
+      float wi = 0.f, wj = 0.f;
+#pragma unroll
+      for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+        if ((unsigned)k == ci[c]) wi = wv[k];
+        if ((unsigned)k == cj[c]) wj = wv[k];
+      }
+      float si = lam * wi, sj = lam * wj;
+#pragma unroll
+      for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+        if ((unsigned)k == ci[c]) { px[k] -= si * gx[c]; py[k] -= si * gy[c]; pz[k] -= si * gz[c]; }
+        if ((unsigned)k == cj[c]) { px[k] += sj * gx[c]; py[k] += sj * gy[c]; pz[k] += sj * gz[c]; }
+      }
+    };
+    if (a.lam0) {
+      for (unsigned q = 0; q < a.n_colors; ++q)
+#pragma unroll
+        for (int c = 0; c < SHAKE_SMALL_CONS; ++c)
+          if (col[c] == q) {
+            float lam = ws[c] > 0.f ? a.lam0[c0 + c] / ws[c] : 0.f;
+            move(c, lam);
+            acc[c] = lam;
+          }
+    }
+    for (; sweeps < a.max_iter; ++sweeps) {
+      int open = 0;
+      for (unsigned q = 0; q < a.n_colors; ++q)
+#pragma unroll
+        for (int c = 0; c < SHAKE_SMALL_CONS; ++c)
+          if (col[c] == q) {
+            float xi = 0.f, yi = 0.f, zi = 0.f, xj = 0.f, yj = 0.f, zj = 0.f;
+#pragma unroll
+            for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+              if ((unsigned)k == ci[c]) { xi = px[k]; yi = py[k]; zi = pz[k]; }
+              if ((unsigned)k == cj[c]) { xj = px[k]; yj = py[k]; zj = pz[k]; }
+            }
+            float rx = xi - xj, ry = yi - yj, rz = zi - zj;
+            float rr = rx * rx + ry * ry + rz * rz;
+            float resid = rr - dd[c];
+            if (fabsf(resid) > a.tol) {
+              open = 1;
+              float proj = rx * gx[c] + ry * gy[c] + rz * gz[c];
+              float slope = 2.0f * proj * ws[c];
+              if (slope != 0.f) {
+                float lam = resid / slope;
+                move(c, lam);
+                acc[c] += lam;
+              }
+            }
+          }
+      if (!open) break;
+    }
+    if (a.lam_out)
+#pragma unroll
+      for (int c = 0; c < SHAKE_SMALL_CONS; ++c)
+        if ((unsigned)c < nc) a.lam_out[c0 + c] = acc[c] * ws[c];
+    float mx[SHAKE_SMALL_ATOMS], my[SHAKE_SMALL_ATOMS], mz[SHAKE_SMALL_ATOMS];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+      mx[k] = mk[k] * px[k]; my[k] = mk[k] * py[k]; mz[k] = mk[k] * pz[k];
+    }
+    float mtot = small_total(mk);
+    float inv = mtot > 0.f ? 1.0f / mtot : 0.f;
+    float cmx = small_total(mx) * inv, cmy = small_total(my) * inv, cmz = small_total(mz) * inv;
+    float rdt = a.dt != 0.f ? 1.0f / a.dt : 0.f;
+    float rdt2 = a.dt != 0.f ? 1.0f / (a.dt * a.dt) : 0.f;
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+      if ((unsigned)k >= na) continue;
+      float dx = px[k] - ux[k], dy = py[k] - uy[k], dz = pz[k] - uz[k];
+      a.vx[atom[k]] += dx * rdt; a.vy[atom[k]] += dy * rdt; a.vz[atom[k]] += dz * rdt;
+      float4 p = a.posq[atom[k]];
+      p.x += dx; p.y += dy; p.z += dz;
+      a.posq[atom[k]] = p;
+      float armx = px[k] - cmx, army = py[k] - cmy, armz = pz[k] - cmz;
+      a.w_cons[atom[k]] = mk[k] * rdt2 * (dx * armx + dy * army + dz * armz);
+    }
+  }
+  if (a.stats) small_stats(a.stats, live, (unsigned)sweeps, a.max_iter);
+}
+
+template <bool ACC>
+__global__ __launch_bounds__(256) void k_rattle_small(HdShakeArgs a) {
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = t < a.n_small;
+  unsigned g = live ? a.small[t] : 0u;
+  HdLattice L = hd_load_lattice(a.lattice);
+  int sweeps = 0;
+  if (live) {
+    unsigned a0 = a.goff[g], na = a.goff[g + 1] - a0;
+    unsigned c0 = a.coff[g], nc = a.coff[g + 1] - c0;
+    unsigned atom[SHAKE_SMALL_ATOMS];
+    float x[SHAKE_SMALL_ATOMS], y[SHAKE_SMALL_ATOMS], z[SHAKE_SMALL_ATOMS];
+    float vx[SHAKE_SMALL_ATOMS], vy[SHAKE_SMALL_ATOMS], vz[SHAKE_SMALL_ATOMS];
+    float kx[SHAKE_SMALL_ATOMS], ky[SHAKE_SMALL_ATOMS], kz[SHAKE_SMALL_ATOMS];
+    float mk[SHAKE_SMALL_ATOMS], wv[SHAKE_SMALL_ATOMS];
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+      atom[k] = 0u;
+      x[k] = y[k] = z[k] = vx[k] = vy[k] = vz[k] = kx[k] = ky[k] = kz[k] = mk[k] = wv[k] = 0.f;
+      if ((unsigned)k < na) {
+        atom[k] = a.gatoms[a0 + k];
+        float4 p = a.posq[atom[k]];
+        x[k] = p.x; y[k] = p.y; z[k] = p.z;
+        vx[k] = a.vx[atom[k]]; vy[k] = a.vy[atom[k]]; vz[k] = a.vz[atom[k]];
+        mk[k] = a.mass[atom[k]];
+        wv[k] = mk[k] > 0.f ? 1.0f / mk[k] : 0.f;
+      }
+    }
+    small_frame(x, y, z, na, L);
+    unsigned ci[SHAKE_SMALL_CONS], cj[SHAKE_SMALL_CONS], col[SHAKE_SMALL_CONS];
+    float rx[SHAKE_SMALL_CONS], ry[SHAKE_SMALL_CONS], rz[SHAKE_SMALL_CONS];
+    float rr[SHAKE_SMALL_CONS], ws[SHAKE_SMALL_CONS];
+#pragma unroll
+    for (int c = 0; c < SHAKE_SMALL_CONS; ++c) {
+      ci[c] = cj[c] = 0u; col[c] = 0xFFFFFFFFu;
+      rx[c] = ry[c] = rz[c] = rr[c] = ws[c] = 0.f;
+      if ((unsigned)c < nc) {
+        unsigned pk = a.cij[c0 + c];
+        ci[c] = pk & 0xFFu; cj[c] = (pk >> 8) & 0xFFu;
+        col[c] = a.color[c0 + c];
+        float xi = 0.f, yi = 0.f, zi = 0.f, xj = 0.f, yj = 0.f, zj = 0.f, wi = 0.f, wj = 0.f;
+#pragma unroll
+        for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+          if ((unsigned)k == ci[c]) { xi = x[k]; yi = y[k]; zi = z[k]; wi = wv[k]; }
+          if ((unsigned)k == cj[c]) { xj = x[k]; yj = y[k]; zj = z[k]; wj = wv[k]; }
+        }
+        rx[c] = xi - xj; ry[c] = yi - yj; rz[c] = zi - zj;
+        rr[c] = rx[c] * rx[c] + ry[c] * ry[c] + rz[c] * rz[c];
+        ws[c] = wi + wj;
+      }
+    }
+    for (; sweeps < a.max_iter; ++sweeps) {
+      int open = 0;
+      for (unsigned q = 0; q < a.n_colors; ++q)
+#pragma unroll
+        for (int c = 0; c < SHAKE_SMALL_CONS; ++c)
+          if (col[c] == q) {
+            float vxi = 0.f, vyi = 0.f, vzi = 0.f, vxj = 0.f, vyj = 0.f, vzj = 0.f, wi = 0.f,
+                  wj = 0.f;
+#pragma unroll
+            for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+              if ((unsigned)k == ci[c]) { vxi = vx[k]; vyi = vy[k]; vzi = vz[k]; wi = wv[k]; }
+              if ((unsigned)k == cj[c]) { vxj = vx[k]; vyj = vy[k]; vzj = vz[k]; wj = wv[k]; }
+            }
+            float ux = vxi - vxj, uy = vyi - vyj, uz = vzi - vzj;
+            float along = ux * rx[c] + uy * ry[c] + uz * rz[c];
+            if (fabsf(along) > a.rattle_tol) {
+              open = 1;
+              float slope = rr[c] * ws[c];
+              if (slope != 0.f) {
+                float mu = along / slope;
+                float si = mu * wi, sj = mu * wj;
+#pragma unroll
+                for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+                  if ((unsigned)k == ci[c]) {
+                    vx[k] -= si * rx[c]; vy[k] -= si * ry[c]; vz[k] -= si * rz[c];
+                    kx[k] -= si * rx[c]; ky[k] -= si * ry[c]; kz[k] -= si * rz[c];
+                  }
+                  if ((unsigned)k == cj[c]) {
+                    vx[k] += sj * rx[c]; vy[k] += sj * ry[c]; vz[k] += sj * rz[c];
+                    kx[k] += sj * rx[c]; ky[k] += sj * ry[c]; kz[k] += sj * rz[c];
+                  }
+                }
+              }
+            }
+          }
+      if (!open) break;
+    }
+    if (ACC && a.dt > 0.f) {
+      float mx[SHAKE_SMALL_ATOMS], my[SHAKE_SMALL_ATOMS], mz[SHAKE_SMALL_ATOMS];
+#pragma unroll
+      for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+        mx[k] = mk[k] * x[k]; my[k] = mk[k] * y[k]; mz[k] = mk[k] * z[k];
+      }
+      float mtot = small_total(mk);
+      float inv = mtot > 0.f ? 1.0f / mtot : 0.f;
+      float cmx = small_total(mx) * inv, cmy = small_total(my) * inv, cmz = small_total(mz) * inv;
+#pragma unroll
+      for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+        if ((unsigned)k >= na) continue;
+        float armx = x[k] - cmx, army = y[k] - cmy, armz = z[k] - cmz;
+        a.w_cons[atom[k]] += mk[k] * (1.0f / a.dt) * (kx[k] * armx + ky[k] * army + kz[k] * armz);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SHAKE_SMALL_ATOMS; ++k) {
+      if ((unsigned)k >= na) continue;
+      a.vx[atom[k]] = vx[k]; a.vy[atom[k]] = vy[k]; a.vz[atom[k]] = vz[k];
+    }
+  }
+  if (a.stats) small_stats(a.stats + 3, live, (unsigned)sweeps, a.max_iter);
 }
 
 // Inputs of the SHAKE multiplier predictor, one thread per constraint,
@@ -1538,16 +1853,25 @@ void hd_launch_shake_features(hipStream_t st, const HdShakeArgs& a) {
 }
 
 void hd_launch_shake_positions(hipStream_t st, const HdShakeArgs& a) {
-  if (a.n_groups)
-    hipLaunchKernelGGL(k_shake_group, dim3(a.n_groups), dim3(64), 0, st, a);
+  unsigned nb = a.big ? a.n_big : a.n_groups;
+  if (nb) hipLaunchKernelGGL(k_shake_group, dim3(nb), dim3(64), 0, st, a);
+  if (a.big && a.n_small)
+    hipLaunchKernelGGL(k_shake_small, dim3((a.n_small + 255) / 256), dim3(256), 0, st, a);
 }
 
 void hd_launch_rattle_velocities(hipStream_t st, const HdShakeArgs& a, bool acc) {
-  if (!a.n_groups) return;
-  if (acc)
-    hipLaunchKernelGGL(k_rattle_group<true>, dim3(a.n_groups), dim3(64), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_rattle_group<false>, dim3(a.n_groups), dim3(64), 0, st, a);
+  unsigned nb = a.big ? a.n_big : a.n_groups;
+  if (nb) {
+    if (acc)
+      hipLaunchKernelGGL(k_rattle_group<true>, dim3(nb), dim3(64), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_rattle_group<false>, dim3(nb), dim3(64), 0, st, a);
+  }
+  if (a.big && a.n_small) {
+    dim3 g((a.n_small + 255) / 256);
+    if (acc) hipLaunchKernelGGL(k_rattle_small<true>, g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_rattle_small<false>, g, dim3(256), 0, st, a);
+  }
 }
 
 // ---------------------------------------------------------------------
