@@ -443,7 +443,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="measure at --skin-nm instead of the fastest tuned skin")
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-equil", type=int, default=100)
     ap.add_argument("--skin-nm", type=float, default=0.1,
                     help="neighbour-list skin (the reference's default is 0.3 r_c = 0.27 nm; "
@@ -453,7 +453,7 @@ def main():
     ap.add_argument("--no-polymer", action="store_true",
                     help="skip the polymer-melt SHAKE / learned warm-start pass")
     ap.add_argument("--polymer-equil", type=int, default=1000)
-    ap.add_argument("--polymer-steps", type=int, default=1000)
+    ap.add_argument("--polymer-steps", type=int, default=5000)
     ap.add_argument("--polymer-log", type=int, default=50000,
                     help="plain-SHAKE steps logged for training the predictor")
     ap.add_argument("--polymer-epochs", type=int, default=1)
