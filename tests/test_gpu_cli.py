@@ -202,3 +202,42 @@ def test_argon_mtk_npt_run(tmp_path):
         f.write(bad)
     r = run_cli("lint", cfg)
     assert r.returncode == 1 and "owns the thermostat" in r.stderr
+
+
+def test_water_two_settle_types(tmp_path):
+    # two SETTLE constraint types with different geometries (SPC and a
+    # 0.9572 A / 104.52 deg model) in one topology: each water is held to
+    # its own type's geometry through minimization and both MD phases
+    say("case: heddlemd run water 8192 with two SETTLE types")
+    cfg, s = write_water(str(tmp_path), solver="settle", min_iter=500)
+    text = open(cfg).read()
+    text += ('\n[[constraint_types]]\nname = "TIP"\nkind = "settle"\nd_OH = 0.9572e-10\n'
+             'd_HH = 1.5139e-10\n')
+    with open(cfg, "w") as f:
+        f.write(text)
+    g = s.water_groups
+    half = len(g) // 2
+    with open(os.path.join(str(tmp_path), "water.in.topology"), "w") as f:
+        f.write("[constraints]\n")
+        for k, grp in enumerate(g):
+            f.write(f"{grp[0]} {grp[1]} {grp[2]} {'SPC' if k < half else 'TIP'}\n")
+    r = run_cli("run", cfg, timeout=600)
+    say("  " + (r.stdout + r.stderr).strip().replace("\n", "\n  "))
+    assert r.returncode == 0
+    frames = _read_xyz_frames(os.path.join(str(tmp_path), "water.out.prod.xyz"))
+    pos = np.array([[float(t) for t in row.split()[1:4]] for row in frames[-1][1]])
+    lat = frames[-1][0].split('"')[1].split()
+    L = np.array([float(lat[0]), float(lat[4]), float(lat[8])])
+
+    def dist(i, j):
+        d = pos[i] - pos[j]
+        d -= L * np.round(d / L)
+        return np.linalg.norm(d, axis=1)
+
+    for sel, oh, tag in ((slice(0, half), 1.0e-10, "SPC"), (slice(half, None), 0.9572e-10, "TIP")):
+        gg = g[sel]
+        r_oh = np.concatenate([dist(gg[:, 0], gg[:, 1]), dist(gg[:, 0], gg[:, 2])])
+        say(f"  {tag}: O-H {r_oh.min():.6e} .. {r_oh.max():.6e} m")
+        # (the final frame carries the step's common barostat factor)
+        assert np.abs(r_oh / oh - 1).max() < 1e-3
+        assert (r_oh.max() - r_oh.min()) / r_oh.mean() < 2e-5
