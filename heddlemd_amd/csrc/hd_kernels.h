@@ -208,7 +208,9 @@ void hd_launch_kick_drift(hipStream_t st, const HdIntegrateArgs& a);
 struct HdLangevinArgs {
   float alpha, kt;                    // exp(-gamma dt) and k_B T, f32
   unsigned seed_lo, seed_hi;          // Philox key
-  unsigned long long* counter;        // device draw counter
+  unsigned long long* counter;        // device draw counter; counter[1] is
+                                      // hd_launch_lan_settle's block ticket,
+                                      // 0 between launches
 };
 // B + A: kick with the current forces, then x += v dt/2 (advances the
 // draw counter by one for the O step that follows)
@@ -240,6 +242,11 @@ void hd_launch_settle_positions(hipStream_t st, const HdSettleArgs& a);
 void hd_launch_kick_drift_settle(hipStream_t st, const HdIntegrateArgs& ia,
                                  const HdSettleArgs& a);
 void hd_launch_settle_velocities(hipStream_t st, const HdSettleArgs& a, bool accumulate_virial);
+// B + A, SETTLE (positions), O + A, SETTLE (positions, then velocities with
+// the virial) in one kernel, bit-identical to the five separate kernels;
+// valid when the SETTLE groups hold every atom and no SHAKE group exists
+void hd_launch_lan_settle(hipStream_t st, const HdIntegrateArgs& ia, const HdSettleArgs& a,
+                          const HdLangevinArgs& l);
 
 // General SHAKE / RATTLE groups (kernels/shake.cu): atoms goff[g] ..
 // goff[g+1] of gatoms, constraints coff[g] .. coff[g+1] with group-local

@@ -873,15 +873,23 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
   la.seed_lo = (unsigned)(c->lan_seed & 0xFFFFFFFFull);
   la.seed_hi = (unsigned)(c->lan_seed >> 32);
   la.counter = c->lan_counter;
-  stage_begin(c, "langevin_kick_drift", timed);
-  hd_launch_lan_kick_drift(st, ia, la);
-  stage_end(c, timed);
-  enqueue_constrain_positions(c, timed);
-  stage_begin(c, "langevin_ou_drift", timed);
-  hd_launch_lan_ou_drift(st, ia, la);
-  stage_end(c, timed);
-  enqueue_constrain_positions(c, timed);
-  enqueue_constrain_velocities(c, timed);
+  if (c->fused_settle && !has_shake(c) && c->h_groups.size() == (size_t)c->n) {
+    // every atom in a SETTLE water: both half steps and the three SETTLE
+    // stages in one kernel
+    stage_begin(c, "langevin_settle", timed);
+    hd_launch_lan_settle(st, ia, settle_args(c), la);
+    stage_end(c, timed);
+  } else {
+    stage_begin(c, "langevin_kick_drift", timed);
+    hd_launch_lan_kick_drift(st, ia, la);
+    stage_end(c, timed);
+    enqueue_constrain_positions(c, timed);
+    stage_begin(c, "langevin_ou_drift", timed);
+    hd_launch_lan_ou_drift(st, ia, la);
+    stage_end(c, timed);
+    enqueue_constrain_positions(c, timed);
+    enqueue_constrain_velocities(c, timed);
+  }
   if (c->have_baro)
     hd_launch_ke(st, n, c->vx, c->vy, c->vz, c->mass, c->red_partials, c->scal + 3, nullptr,
                  0.f, nullptr);
@@ -2274,7 +2282,7 @@ int hd_setup(hd_ctx* c) {
   DALLOC(c, c->baro_mu, 1);
   DALLOC(c, c->baro_diag, 4);
   DALLOC(c, c->baro_counter, 1);
-  DALLOC(c, c->lan_counter, 1);
+  DALLOC(c, c->lan_counter, 2);  // draw counter, fused-kernel block ticket
   DALLOC(c, c->voxel_rank, c->nvox); DALLOC(c, c->atom_key, n);
   DALLOC(c, c->voxel_count, c->nvox); DALLOC(c, c->voxel_off, c->nvox + 1);
   DALLOC(c, c->voxel_cursor, c->nvox);

@@ -1944,6 +1944,97 @@ void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLa
   else hipLaunchKernelGGL((k_lan_ou_drift<false>), g, b, 0, st, a, l);
 }
 
+// B + A, SETTLE on positions, O + A, SETTLE on positions and SETTLE on
+// velocities (with the constraint virial) in one kernel, for a system
+// whose atoms all belong to SETTLE groups: one thread per water runs
+// k_lan_kick_drift's and k_lan_ou_drift's arithmetic on its three atoms
+// and the SETTLE stages on the f32 values held in registers, the values
+// the five separate kernels pass through memory, so the results are
+// bit-identical.  Every block reads the draw counter before it takes a
+// ticket (counter[1]); the block that takes the last ticket advances the
+// counter and returns the ticket to 0, so all blocks draw with the value
+// the O + A kernel would see and a captured step replays unchanged.
+__global__ void __launch_bounds__(128) k_lan_settle(HdIntegrateArgs ia, HdSettleArgs a,
+                                                    HdLangevinArgs l) {
+  __shared__ unsigned long long s_ctr;
+  if (threadIdx.x == 0) s_ctr = l.counter[0];
+  __syncthreads();
+  unsigned long long ctr = s_ctr;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned long long t = atomicAdd(l.counter + 1, 1ull);
+    if (t == (unsigned long long)gridDim.x - 1ull) {
+      atomicExch(l.counter, ctr + 1ull);
+      atomicExch(l.counter + 1, 0ull);
+    }
+  }
+  unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_groups) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  float half_dt = ia.dt * 0.5f;
+  unsigned ids[3];
+  settle_load_ids(a, g, ids);
+  int img[3][3];
+  float4 p[3], q[3];
+  float v[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned i = ids[k];
+    float m = ia.mass[i];
+    float ax = ia.fx[i] / m, ay = ia.fy[i] / m, az = ia.fz[i] / m;
+    float vx = ia.vx[i] + ax * half_dt;
+    float vy = ia.vy[i] + ay * half_dt;
+    float vz = ia.vz[i] + az * half_dt;
+    float4 pq = ia.posq[i];
+    q[k] = pq;
+    float px = pq.x + vx * half_dt, py = pq.y + vy * half_dt, pz = pq.z + vz * half_dt;
+    int ka, kb, kc;
+    hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+    img[k][0] = ia.ix[i] + ka; img[k][1] = ia.iy[i] + kb; img[k][2] = ia.iz[i] + kc;
+    p[k] = make_float4(px, py, pz, pq.w);
+    v[k][0] = vx; v[k][1] = vy; v[k][2] = vz;
+  }
+  SettleGeom G = settle_geom(a, g);
+  SettleOut o = settle_water_positions(G, L, p, q, v, a.dt);
+  unsigned clo = (unsigned)(ctr & 0xFFFFFFFFull), chi = (unsigned)(ctr >> 32);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned i = ids[k];
+    float m = ia.mass[i];
+    float sigma = sqrtf((1.0f - l.alpha * l.alpha) * l.kt / m);
+    float xi0 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 0u);
+    float xi1 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 1u);
+    float xi2 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 2u);
+    float vx = l.alpha * o.v[k][0] + sigma * xi0;
+    float vy = l.alpha * o.v[k][1] + sigma * xi1;
+    float vz = l.alpha * o.v[k][2] + sigma * xi2;
+    float4 pq = o.p[k];
+    q[k] = pq;
+    float px = pq.x + vx * half_dt, py = pq.y + vy * half_dt, pz = pq.z + vz * half_dt;
+    int ka, kb, kc;
+    hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+    img[k][0] += ka; img[k][1] += kb; img[k][2] += kc;
+    p[k] = make_float4(px, py, pz, pq.w);
+    v[k][0] = vx; v[k][1] = vy; v[k][2] = vz;
+  }
+  o = settle_water_positions(G, L, p, q, v, a.dt);
+  settle_water_velocities<true>(G, L, o.p, o.v, o.w, a.dt);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned i = ids[k];
+    a.posq[i] = o.p[k];
+    a.vx[i] = o.v[k][0]; a.vy[i] = o.v[k][1]; a.vz[i] = o.v[k][2];
+    a.w_cons[i] = o.w[k];
+    ia.ix[i] = img[k][0]; ia.iy[i] = img[k][1]; ia.iz[i] = img[k][2];
+  }
+}
+
+void hd_launch_lan_settle(hipStream_t st, const HdIntegrateArgs& ia, const HdSettleArgs& a,
+                          const HdLangevinArgs& l) {
+  if (!a.n_groups) return;
+  hipLaunchKernelGGL(k_lan_settle, dim3((a.n_groups + 127) / 128), dim3(128), 0, st, ia, a, l);
+}
+
 // One thread: P = (2K + W)/(3V), mu^3 = 1 - kappa (dt/tau)(P0 - P)
 // + sqrt(2 kappa kT dt/(tau V)) r, clamped, lattice *= (f32) mu; the
 // draw counter advances by one.  K and W are read from FP64 scalars.

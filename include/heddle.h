@@ -290,7 +290,9 @@ int hd_negate_velocities(hd_ctx *ctx);
 int hd_use_graph(hd_ctx *ctx, int enable);
 /* When every atom belongs to a SETTLE water, the drift and both SETTLE
  * stages run as one kernel (default on); 0 runs the three separate
- * kernels, with bit-identical results.  No reference counterpart. */
+ * kernels, with bit-identical results.  Under Langevin BAOAB the fused
+ * kernel covers both half steps and the three SETTLE stages (five
+ * kernels).  No reference counterpart. */
 int hd_set_fused_settle(hd_ctx *ctx, int enable);
 int hd_sync(hd_ctx *ctx);
 

@@ -87,6 +87,41 @@ def test_settle_fused_and_unfused_paths_bit_identical_and_match_oracle(oracle):
     assert dv <= 1e-5 * np.abs(vel).max()
 
 
+def test_langevin_settle_fused_and_unfused_paths_bit_identical():
+    # BAOAB on 8192 SETTLE waters: the fused kernel (both half steps and the
+    # three SETTLE stages) against the five separate kernels, eager and
+    # through captured step graphs: the same bits after 12 steps, and the
+    # draw counter advanced once per step (the fused kernel's last block
+    # advances it)
+    say("case: water 8192 Langevin + SETTLE, fused vs unfused, eager and graph, 12 steps")
+    s = systems.config_water_8192()
+    rng = np.random.default_rng(31)
+    v0 = rng.normal(0, 2e-4, size=(3, s.n)).astype(np.float32)
+    kt = U.temperature(300.0)
+    out = {}
+    for fused in (True, False):
+        for graph in (False, True):
+            e = engine_for(s, dt=U.time(1e-15), velocities=v0,
+                           langevin=dict(friction=1.0 / U.time(1e-13), kt=kt, seed=5))
+            e.use_graph(graph)
+            e.set_fused_settle(fused)
+            e.force_eval(scalars=False)
+            e.step(6)
+            e.step(6, scalars_last=True)
+            p, img = e.positions(images=True)
+            out[fused, graph] = (p, img, e.velocities(), e.forces(), e.langevin_counter(),
+                                 e.scalars())
+            e.close()
+    ref = out[False, False]
+    assert ref[4] == 12
+    for key, o in out.items():
+        for k in range(4):
+            assert o[k].tobytes() == ref[k].tobytes(), f"{key} output {k}"
+        assert o[4] == ref[4], key
+        assert o[5] == ref[5], key
+    say(f"  4 paths byte-identical, counter {ref[4]}, ke {ref[5]['ke']:.6e}")
+
+
 def test_settle_waters_beside_free_atoms(oracle):
     # the last two molecules are left out of the SETTLE groups (six free
     # atoms, their intramolecular pairs still excluded), so the step runs
