@@ -37,6 +37,7 @@ SIGNATURES = {
     "hd_set_positions": (C.c_int, [_ctx, _f32p]),
     "hd_neighbor_pre_step": (C.c_int, [_ctx, C.POINTER(C.c_uint32)]),
     "hd_set_fused_settle": (C.c_int, [_ctx, C.c_int]),
+    "hd_set_small_neighbor": (C.c_int, [_ctx, C.c_int]),
     "hd_set_velocities": (C.c_int, [_ctx, _f32p, _f32p, _f32p]),
     "hd_set_images": (C.c_int, [_ctx, _vp, _vp, _vp]),
     "hd_set_lj": (C.c_int, [_ctx, C.c_int, _f64p, _f64p, _f64p, _f64p]),

@@ -48,6 +48,13 @@ void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* s
 // bounding boxes (which the neighbour build then uses)
 void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long long* fpj,
                                    long long* fpj_ew);
+// Systems of at most 8192 atoms and 8192 voxels: the displacement check
+// (when check), the gated voxel binning and the tile gather with its
+// rebuild-step extras in one one-workgroup launch, with the results of
+// hd_launch_neighbor_update + hd_launch_tile_gather_rebuild
+bool hd_neighbor_small(const HdNeighborArgs& a);
+void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool check, long long* fpj,
+                              long long* fpj_ew);
 
 struct HdPairArgs {
   unsigned n, npad, cap;

@@ -145,6 +145,7 @@ struct hd_ctx {
   uint64_t lan_seed = 0;
   bool use_graph = true;
   bool fused_settle = true;  // all-water systems: drift and SETTLE in one kernel
+  bool small_neighbor = true;  // small systems: check, binning and gather in one workgroup
   bool timing = false;
   bool ready = false;
   bool forces_valid = false;
@@ -572,12 +573,18 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
                     bool baro_mu = false) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
-  stage_begin(c, "neighbor_update", timed);
-  hd_launch_neighbor_update(st, na, !disp_done);
-  stage_end(c, timed);
-  stage_begin(c, "tile_gather", timed);
-  hd_launch_tile_gather_rebuild(st, na, c->fpj, ev ? c->fpj_ew : nullptr);
-  stage_end(c, timed);
+  if (c->small_neighbor && hd_neighbor_small(na)) {
+    stage_begin(c, "neighbor_update", timed);
+    hd_launch_neighbor_small(st, na, !disp_done, c->fpj, ev ? c->fpj_ew : nullptr);
+    stage_end(c, timed);
+  } else {
+    stage_begin(c, "neighbor_update", timed);
+    hd_launch_neighbor_update(st, na, !disp_done);
+    stage_end(c, timed);
+    stage_begin(c, "tile_gather", timed);
+    hd_launch_tile_gather_rebuild(st, na, c->fpj, ev ? c->fpj_ew : nullptr);
+    stage_end(c, timed);
+  }
   // The SPME pipeline reads the tile positions, the lattice and the voxel
   // tables, all final here, and nothing on the build and pair path writes
   // what it reads: it runs on the second stream, concurrent with them, and
@@ -3073,6 +3080,16 @@ int hd_set_fused_settle(hd_ctx* c, int enable) {
     drop_graphs(c);
   }
   c->fused_settle = enable != 0;
+  return HD_OK;
+}
+
+int hd_set_small_neighbor(hd_ctx* c, int enable) {
+  if (!c) return HD_ERR_ARG;
+  if (c->ready) {
+    HIPCHK(c, hipSetDevice(c->device));
+    drop_graphs(c);
+  }
+  c->small_neighbor = enable != 0;
   return HD_OK;
 }
 

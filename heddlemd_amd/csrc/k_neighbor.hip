@@ -50,6 +50,25 @@ __global__ void k_disp_check(const float4* __restrict__ posq, const float4* __re
   if (__any(over) && (threadIdx.x & 63) == 0) atomicOr(status, NB_STATUS_REBUILD);
 }
 
+// Curve rank of the fractional voxel that holds the atom (wrapped into the
+// primary cell).
+__device__ __forceinline__ unsigned nb_voxel_key(float4 p, const HdLattice& L,
+                                                 const unsigned* __restrict__ voxel_rank,
+                                                 unsigned nva, unsigned nvb, unsigned nvc) {
+  float x = p.x, y = p.y, z = p.z;
+  int ia, ib, ic;
+  hd_wrap_with_image(x, y, z, ia, ib, ic, L);
+  float sa, sb, sc;
+  hd_cart_to_frac(x, y, z, L, sa, sb, sc);
+  int va = (int)floorf((sa + 0.5f) * (float)nva);
+  int vb = (int)floorf((sb + 0.5f) * (float)nvb);
+  int vc = (int)floorf((sc + 0.5f) * (float)nvc);
+  va = min(max(va, 0), (int)nva - 1);
+  vb = min(max(vb, 0), (int)nvb - 1);
+  vc = min(max(vc, 0), (int)nvc - 1);
+  return voxel_rank[((unsigned)va * nvb + (unsigned)vb) * nvc + (unsigned)vc];
+}
+
 // Voxel key = curve rank of the atom's fractional voxel; histogram.
 // Also records the positions and lattice of this rebuild, against which the
 // displacement check measures (positions do not change within a force
@@ -66,18 +85,7 @@ __global__ void k_voxel_keys(const float4* __restrict__ posq, const float* __res
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 p = posq[i];
     ref[i] = p;
-    float x = p.x, y = p.y, z = p.z;
-    int ia, ib, ic;
-    hd_wrap_with_image(x, y, z, ia, ib, ic, L);
-    float sa, sb, sc;
-    hd_cart_to_frac(x, y, z, L, sa, sb, sc);
-    int va = (int)floorf((sa + 0.5f) * (float)nva);
-    int vb = (int)floorf((sb + 0.5f) * (float)nvb);
-    int vc = (int)floorf((sc + 0.5f) * (float)nvc);
-    va = min(max(va, 0), (int)nva - 1);
-    vb = min(max(vb, 0), (int)nvb - 1);
-    vc = min(max(vc, 0), (int)nvc - 1);
-    unsigned key = voxel_rank[((unsigned)va * nvb + (unsigned)vb) * nvc + (unsigned)vc];
+    unsigned key = nb_voxel_key(p, L, voxel_rank, nva, nvb, nvc);
     atom_key[i] = key;
     atomicAdd(&voxel_count[key], 1u);
   }
@@ -207,6 +215,22 @@ __global__ void k_voxel_scatter(const unsigned* __restrict__ atom_key,
   }
 }
 
+__device__ __forceinline__ void nb_sort_voxel(unsigned v, const unsigned* __restrict__ voxel_off,
+                                              unsigned* __restrict__ sorted_id,
+                                              unsigned* __restrict__ slot_of) {
+  unsigned b = voxel_off[v], e = voxel_off[v + 1];
+  for (unsigned k = b + 1; k < e; ++k) {
+    unsigned key = sorted_id[k];
+    int pos = (int)k - 1;
+    while (pos >= (int)b && sorted_id[pos] > key) {
+      sorted_id[pos + 1] = sorted_id[pos];
+      --pos;
+    }
+    sorted_id[pos + 1] = key;
+  }
+  for (unsigned k = b; k < e; ++k) slot_of[sorted_id[k]] = k;
+}
+
 // Per voxel: sort its atom ids ascending (a voxel holds a handful of
 // atoms) so the slot order is independent of atomic arrival order; then
 // clear the voxel histogram for the next rebuild and record slot_of[id].
@@ -217,17 +241,7 @@ __global__ void k_voxel_sort(const unsigned* __restrict__ voxel_off,
   if (!nb_rebuild_due(status)) return;
   for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
        v += gridDim.x * blockDim.x) {
-    unsigned b = voxel_off[v], e = voxel_off[v + 1];
-    for (unsigned k = b + 1; k < e; ++k) {
-      unsigned key = sorted_id[k];
-      int pos = (int)k - 1;
-      while (pos >= (int)b && sorted_id[pos] > key) {
-        sorted_id[pos + 1] = sorted_id[pos];
-        --pos;
-      }
-      sorted_id[pos + 1] = key;
-    }
-    for (unsigned k = b; k < e; ++k) slot_of[sorted_id[k]] = k;
+    nb_sort_voxel(v, voxel_off, sorted_id, slot_of);
     voxel_count[v] = 0u;
   }
 }
@@ -241,15 +255,21 @@ __global__ void k_voxel_sort(const unsigned* __restrict__ voxel_off,
 // the j-side accumulators, every step.  On rebuild steps (status given and
 // the request set) it also fills the slots' types and keys and each 64-slot
 // block's bounding box (one wave per block), from the same positions.
-__global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* __restrict__ sorted_id,
-                              const float* __restrict__ lattice, float4* __restrict__ tile_posq,
-                              long long* __restrict__ fpj, long long* __restrict__ fpj_ew,
-                              unsigned n, unsigned npad, const unsigned* status,
-                              const unsigned* __restrict__ type, const unsigned* __restrict__ key,
-                              unsigned* __restrict__ tile_type, unsigned* __restrict__ tile_key,
-                              float4* __restrict__ center, float4* __restrict__ half) {
-  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= npad) return;  // whole waves: npad is a multiple of 64
+// One slot s < npad; the wave's 64 lanes hold the 64 slots of one block
+// (the bounding-box reduction is a wave reduction).
+__device__ __forceinline__ void nb_gather_slot(unsigned s, bool rebuild,
+                                               const float4* __restrict__ posq,
+                                               const unsigned* __restrict__ sorted_id,
+                                               const float* __restrict__ lattice,
+                                               float4* __restrict__ tile_posq,
+                                               long long* __restrict__ fpj,
+                                               long long* __restrict__ fpj_ew, unsigned n,
+                                               const unsigned* __restrict__ type,
+                                               const unsigned* __restrict__ key,
+                                               unsigned* __restrict__ tile_type,
+                                               unsigned* __restrict__ tile_key,
+                                               float4* __restrict__ center,
+                                               float4* __restrict__ half) {
   float4 out;
   unsigned id = 0u;
   if (s < n) {
@@ -267,7 +287,7 @@ __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* _
   reinterpret_cast<longlong2*>(fpj)[2 * s] = z;
   reinterpret_cast<longlong2*>(fpj)[2 * s + 1] = z;
   if (fpj_ew) reinterpret_cast<longlong2*>(fpj_ew)[s] = z;
-  if (!status || !nb_rebuild_due(status)) return;
+  if (!rebuild) return;
   bool v = s < n;
   tile_type[s] = v ? type[id] : 0u;
   tile_key[s] = v ? key[id] : HD_SENTINEL - 1u;
@@ -286,6 +306,127 @@ __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* _
     center[blk] = make_float4(0.5f * (lox + hix), 0.5f * (loy + hiy), 0.5f * (loz + hiz), 0.f);
     half[blk] = make_float4(0.5f * (hix - lox), 0.5f * (hiy - loy), 0.5f * (hiz - loz), 0.f);
   }
+}
+
+__global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* __restrict__ sorted_id,
+                              const float* __restrict__ lattice, float4* __restrict__ tile_posq,
+                              long long* __restrict__ fpj, long long* __restrict__ fpj_ew,
+                              unsigned n, unsigned npad, const unsigned* status,
+                              const unsigned* __restrict__ type, const unsigned* __restrict__ key,
+                              unsigned* __restrict__ tile_type, unsigned* __restrict__ tile_key,
+                              float4* __restrict__ center, float4* __restrict__ half) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;  // whole waves: npad is a multiple of 64
+  nb_gather_slot(s, status && nb_rebuild_due(status), posq, sorted_id, lattice, tile_posq, fpj,
+                 fpj_ew, n, type, key, tile_type, tile_key, center, half);
+}
+
+// Systems of at most NB_SMALL_ATOMS atoms and NB_SMALL_VOX voxels: the
+// displacement check, the rebuild's voxel binning (keys, histogram, scan,
+// scatter, per-voxel sort) and the tile gather run in one workgroup, with
+// barriers between the phases in place of the seven launches of the
+// general path.  The histogram and the scatter cursors live in LDS (the
+// global voxel_count stays zero, as the general path leaves it).  Results
+// are those of the general path: the per-atom arithmetic is shared with its
+// kernels, the scan is exact and the per-voxel sort makes the slot order
+// independent of the order of the atomics.
+#define NB_SMALL_THREADS 1024u
+#define NB_SMALL_ATOMS 8192u
+#define NB_SMALL_VOX 8192u
+
+__global__ void __launch_bounds__(NB_SMALL_THREADS) k_nb_small(HdNeighborArgs a, int check,
+                                                               long long* __restrict__ fpj,
+                                                               long long* __restrict__ fpj_ew) {
+  __shared__ unsigned cnt[NB_SMALL_VOX];
+  __shared__ unsigned scan[NB_SMALL_THREADS];
+  __shared__ int s_over;
+  const unsigned t = threadIdx.x;
+  HdLattice L = hd_load_lattice(a.lattice);
+  if (t == 0) s_over = 0;
+  __syncthreads();
+  if (check) {
+    // k_disp_check's test per atom
+    float dl = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dl = fmaxf(dl, fabsf(a.lattice[k] - a.ref_lattice[k]));
+    bool over = false;
+    for (unsigned i = t; i < a.n; i += NB_SMALL_THREADS) {
+      float4 p = a.posq[i], r = a.ref_pos[i];
+      float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+      hd_min_image(dx, dy, dz, L);
+      over = over || sqrtf(dx * dx + dy * dy + dz * dz) + 1.7320508f * dl > a.half_skin;
+    }
+    if (__any(over) && (t & 63u) == 0) {
+      atomicOr(a.status, NB_STATUS_REBUILD);
+      s_over = 1;
+    }
+  }
+  __syncthreads();
+  // one thread decides and every thread reads its decision, so the
+  // barriers of the rebuild phases are reached by the whole workgroup
+  if (t == 0) s_over = (s_over != 0 || nb_rebuild_due(a.status)) ? 1 : 0;
+  __syncthreads();
+  const bool rebuild = s_over != 0;
+  if (rebuild) {
+    if (t < 6) a.ref_lattice[t] = a.lattice[t];
+    for (unsigned v = t; v < a.nvox; v += NB_SMALL_THREADS) cnt[v] = 0u;
+    __syncthreads();
+    for (unsigned i = t; i < a.n; i += NB_SMALL_THREADS) {
+      float4 p = a.posq[i];
+      a.ref_pos[i] = p;
+      unsigned key = nb_voxel_key(p, L, a.voxel_rank, a.nv[0], a.nv[1], a.nv[2]);
+      a.atom_key[i] = key;
+      atomicAdd(&cnt[key], 1u);
+    }
+    __syncthreads();
+    // exclusive scan: thread t owns voxels [t per, (t + 1) per)
+    const unsigned per = (a.nvox + NB_SMALL_THREADS - 1u) / NB_SMALL_THREADS;
+    const unsigned b0 = t * per;
+    unsigned sum = 0u;
+    for (unsigned k = 0; k < per; ++k)
+      if (b0 + k < a.nvox) sum += cnt[b0 + k];
+    scan[t] = sum;
+    __syncthreads();
+    for (unsigned off = 1; off < NB_SMALL_THREADS; off <<= 1) {
+      unsigned u = t >= off ? scan[t - off] : 0u;
+      __syncthreads();
+      scan[t] += u;
+      __syncthreads();
+    }
+    unsigned run = scan[t] - sum;
+    for (unsigned k = 0; k < per; ++k) {
+      unsigned v = b0 + k;
+      if (v < a.nvox) {
+        unsigned c = cnt[v];
+        a.voxel_off[v] = run;
+        cnt[v] = run;  // the scatter cursor
+        run += c;
+      }
+    }
+    if (t == NB_SMALL_THREADS - 1u) a.voxel_off[a.nvox] = scan[t];
+    __syncthreads();
+    for (unsigned i = t; i < a.n; i += NB_SMALL_THREADS)
+      a.sorted_id[atomicAdd(&cnt[a.atom_key[i]], 1u)] = i;
+    __syncthreads();
+    for (unsigned v = t; v < a.nvox; v += NB_SMALL_THREADS)
+      nb_sort_voxel(v, a.voxel_off, a.sorted_id, a.slot_of);
+    __syncthreads();
+  }
+  // npad is a multiple of 64 and so is the stride: each wave covers whole
+  // blocks in every pass
+  for (unsigned s = t; s < a.npad; s += NB_SMALL_THREADS)
+    nb_gather_slot(s, rebuild, a.posq, a.sorted_id, a.lattice, a.tile_posq, fpj, fpj_ew, a.n,
+                   a.type, a.key, a.tile_type, a.tile_key, a.center, a.half);
+}
+
+bool hd_neighbor_small(const HdNeighborArgs& a) {
+  return a.n <= NB_SMALL_ATOMS && a.nvox <= NB_SMALL_VOX;
+}
+
+void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool check, long long* fpj,
+                              long long* fpj_ew) {
+  hipLaunchKernelGGL(k_nb_small, dim3(1), dim3(NB_SMALL_THREADS), 0, st, a, check ? 1 : 0, fpj,
+                     fpj_ew);
 }
 
 // Per-block axis-aligned bounding box: centre and half extents (one wave

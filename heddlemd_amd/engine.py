@@ -325,6 +325,13 @@ class Engine:
     def set_fused_settle(self, on):
         self._chk(self._lib.hd_set_fused_settle(self._ctx, 1 if on else 0), "hd_set_fused_settle")
 
+    def set_small_neighbor(self, on):
+        """Small systems: neighbour check, binning and gather in one
+        workgroup (default) or the general multi-kernel path (identical
+        results)."""
+        self._chk(self._lib.hd_set_small_neighbor(self._ctx, 1 if on else 0),
+                  "hd_set_small_neighbor")
+
     def use_graph(self, on):
         self._chk(self._lib.hd_use_graph(self._ctx, int(bool(on))), "hd_use_graph")
 

@@ -294,6 +294,11 @@ int hd_use_graph(hd_ctx *ctx, int enable);
  * kernel covers both half steps and the three SETTLE stages (five
  * kernels).  No reference counterpart. */
 int hd_set_fused_settle(hd_ctx *ctx, int enable);
+/* Systems of at most 8192 atoms run the neighbour displacement check, the
+ * rebuild's voxel binning and the slot-order gather in one one-workgroup
+ * kernel (default on); 0 takes the general multi-kernel path, with
+ * identical results.  No reference counterpart. */
+int hd_set_small_neighbor(hd_ctx *ctx, int enable);
 int hd_sync(hd_ctx *ctx);
 
 /* Batched replicas.  A batch advances many independent, set-up contexts

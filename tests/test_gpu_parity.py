@@ -715,6 +715,44 @@ def test_batched_replicas_byte_identical_to_solo_runs():
         e.close()
 
 
+def test_small_neighbor_kernel_matches_the_general_path():
+    # systems of at most 8192 atoms run the displacement check, the voxel
+    # binning and the slot gather in one workgroup: argon and water replicas
+    # (ragged atom counts, SPME, SETTLE) under Langevin, and one argon
+    # replica under NHC (the check then runs in the KE pass), end in the
+    # same bits, pair list and rebuild count as on the general path
+    say("case: one-workgroup neighbour kernel vs general path, 200 steps")
+    specs = [s for s in systems.replica_specs(count=256)][::43][:6]
+    say("  " + ", ".join(f"{s['kind']}:{s['n_atoms']}" for s in specs))
+    runs = [(sp, r, False) for r, sp in enumerate(specs)] + [(specs[0], 0, True)]
+    for sp, r, nhc in runs:
+        out = []
+        for small in (True, False):
+            if nhc:
+                s = systems.replica_system(sp)
+                kt = U.temperature(sp["temperature_K"])
+                from heddlemd_amd.rng import maxwell_boltzmann
+                v = maxwell_boltzmann(s.mass, kt, seed=7)
+                e = engine_for(s, dt=U.time(2e-15), velocities=v, nhc_kt=kt,
+                               nhc_tau=U.time(1e-13))
+            else:
+                e = _replica_engine(sp, r)
+            e.set_small_neighbor(small)
+            e.step(150)
+            e.step(50, scalars_last=True)
+            (p, i), v, f, sc = e.positions(images=True), e.velocities(), e.forces(), e.scalars()
+            out.append((p, i, v, f, sc, e.pairs()))
+            e.close()
+        (a, b) = out
+        say(f"  {sp['kind']}:{sp['n_atoms']}{' nhc' if nhc else ''}: rebuilds "
+            f"{a[4]['rebuilds']:.0f}, pairs {len(a[5])}")
+        assert a[4]["rebuilds"] >= 2
+        for k in range(4):
+            assert a[k].tobytes() == b[k].tobytes(), f"{sp} output {k}"
+        assert a[4] == b[4]
+        assert np.array_equal(a[5], b[5])
+
+
 def test_charged_chains_real_space_key_map(oracle):
     # chains with alternating charges under SPME: one cutoff for LJ and
     # Coulomb sends every block through the regrouped pair loop, and the
