@@ -40,7 +40,9 @@ void hd_launch_neighbor_update(hipStream_t st, const HdNeighborArgs& a, bool che
 void hd_launch_disp_check(hipStream_t st, const HdNeighborArgs& a);
 // number of chunk sums of the voxel-count scan (the buffer needs one more)
 unsigned hd_voxel_scan_parts(unsigned nvox);
-void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a);
+// block_pairs = false when hd_launch_neighbor_small already ran the
+// block-pair box tests of this step
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs = true);
 void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
                            const float* lattice, float4* tile_posq, long long* fpj,
                            long long* fpj_ew, unsigned n, unsigned npad);
@@ -49,9 +51,10 @@ void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* s
 void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long long* fpj,
                                    long long* fpj_ew);
 // Systems of at most 8192 atoms and 8192 voxels: the displacement check
-// (when check), the gated voxel binning and the tile gather with its
-// rebuild-step extras in one one-workgroup launch, with the results of
-// hd_launch_neighbor_update + hd_launch_tile_gather_rebuild
+// (when check), the gated voxel binning, the tile gather with its
+// rebuild-step extras and the block-pair box tests in one one-workgroup
+// launch, with the results of hd_launch_neighbor_update +
+// hd_launch_tile_gather_rebuild + the box tests of hd_launch_neighbor_build
 bool hd_neighbor_small(const HdNeighborArgs& a);
 void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool check, long long* fpj,
                               long long* fpj_ew);
@@ -224,6 +227,9 @@ struct HdLangevinArgs {
 void hd_launch_lan_kick_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
 // O + A: Ornstein-Uhlenbeck velocity update, then x += v dt/2
 void hd_launch_lan_ou_drift(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
+// B + A and O + A in one kernel for a system without constraints,
+// bit-identical to the two separate kernels
+void hd_launch_lan_free(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l);
 
 #define HD_SETTLE_KINDS 8
 struct HdSettleArgs {

@@ -573,7 +573,8 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
                     bool baro_mu = false) {
   hipStream_t st = c->stream;
   HdNeighborArgs na = nb_args(c);
-  if (c->small_neighbor && hd_neighbor_small(na)) {
+  const bool small_nb = c->small_neighbor && hd_neighbor_small(na);
+  if (small_nb) {
     stage_begin(c, "neighbor_update", timed);
     hd_launch_neighbor_small(st, na, !disp_done, c->fpj, ev ? c->fpj_ew : nullptr);
     stage_end(c, timed);
@@ -598,7 +599,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
     (void)hipEventRecord(c->ev_join, c->side);
   }
   stage_begin(c, "neighbor_build", timed);
-  hd_launch_neighbor_build(st, na);
+  hd_launch_neighbor_build(st, na, !small_nb);
   stage_end(c, timed);
   HdPairArgs pa;
   memset(&pa, 0, sizeof(pa));
@@ -885,6 +886,10 @@ void enqueue_langevin_step(hd_ctx* c, bool ev, bool timed) {
     // stages in one kernel
     stage_begin(c, "langevin_settle", timed);
     hd_launch_lan_settle(st, ia, settle_args(c), la);
+    stage_end(c, timed);
+  } else if (c->fused_settle && !has_constraints(c)) {
+    stage_begin(c, "langevin_kick_drift_ou_drift", timed);
+    hd_launch_lan_free(st, ia, la);
     stage_end(c, timed);
   } else {
     stage_begin(c, "langevin_kick_drift", timed);

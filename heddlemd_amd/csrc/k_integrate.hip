@@ -2029,6 +2029,57 @@ __global__ void __launch_bounds__(128) k_lan_settle(HdIntegrateArgs ia, HdSettle
   }
 }
 
+// B + A and O + A in one kernel for a system without constraints: one
+// thread per atom runs k_lan_kick_drift's and k_lan_ou_drift's arithmetic
+// in registers (bit-identical to the two kernels); the draw counter is
+// advanced by the block that takes the last ticket, as in k_lan_settle.
+__global__ void __launch_bounds__(256) k_lan_free(HdIntegrateArgs a, HdLangevinArgs l) {
+  __shared__ unsigned long long s_ctr;
+  if (threadIdx.x == 0) s_ctr = l.counter[0];
+  __syncthreads();
+  unsigned long long ctr = s_ctr;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned long long t = atomicAdd(l.counter + 1, 1ull);
+    if (t == (unsigned long long)gridDim.x - 1ull) {
+      atomicExch(l.counter, ctr + 1ull);
+      atomicExch(l.counter + 1, 0ull);
+    }
+  }
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  HdLattice L = hd_load_lattice(a.lattice);
+  float m = a.mass[i];
+  float ax = a.fx[i] / m, ay = a.fy[i] / m, az = a.fz[i] / m;
+  float half_dt = a.dt * 0.5f;
+  float vx = a.vx[i] + ax * half_dt;
+  float vy = a.vy[i] + ay * half_dt;
+  float vz = a.vz[i] + az * half_dt;
+  float4 pq = a.posq[i];
+  float px = pq.x + vx * half_dt, py = pq.y + vy * half_dt, pz = pq.z + vz * half_dt;
+  int ka, kb, kc;
+  hd_wrap_with_image(px, py, pz, ka, kb, kc, L);
+  int nx = a.ix[i] + ka, ny = a.iy[i] + kb, nz = a.iz[i] + kc;
+  unsigned clo = (unsigned)(ctr & 0xFFFFFFFFull), chi = (unsigned)(ctr >> 32);
+  float sigma = sqrtf((1.0f - l.alpha * l.alpha) * l.kt / m);
+  float xi0 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 0u);
+  float xi1 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 1u);
+  float xi2 = (float)philox_gaussian_f64(l.seed_lo, l.seed_hi, clo, chi, i, 2u);
+  vx = l.alpha * vx + sigma * xi0;
+  vy = l.alpha * vy + sigma * xi1;
+  vz = l.alpha * vz + sigma * xi2;
+  float qx = px + vx * half_dt, qy = py + vy * half_dt, qz = pz + vz * half_dt;
+  hd_wrap_with_image(qx, qy, qz, ka, kb, kc, L);
+  a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+  a.posq[i] = make_float4(qx, qy, qz, pq.w);
+  a.ix[i] = nx + ka; a.iy[i] = ny + kb; a.iz[i] = nz + kc;
+}
+
+void hd_launch_lan_free(hipStream_t st, const HdIntegrateArgs& a, const HdLangevinArgs& l) {
+  // one block at least: the counter advances as with the separate kernels
+  hipLaunchKernelGGL(k_lan_free, dim3(a.n ? (a.n + 255) / 256 : 1), dim3(256), 0, st, a, l);
+}
+
 void hd_launch_lan_settle(hipStream_t st, const HdIntegrateArgs& ia, const HdSettleArgs& a,
                           const HdLangevinArgs& l) {
   if (!a.n_groups) return;

@@ -321,11 +321,79 @@ __global__ void k_tile_gather(const float4* __restrict__ posq, const unsigned* _
                  fpj_ew, n, type, key, tile_type, tile_key, center, half);
 }
 
+
+// Per-block axis-aligned bounding box: centre and half extents (one wave
+// per block).  Padding lanes are ignored.
+
+// Closest approach of two boxes under the minimum image (per axis; exact
+// for orthorhombic cells), as kernels/neighbor.cu:364-388.  In a tilted
+// cell a pair's own minimum image (fractional rounding of its separation)
+// can differ by one lattice vector per axis from the one the box centres
+// round to, so the 27 images around the centres' are all tried there.
+__device__ __forceinline__ float nb_box_gap2(float dx, float dy, float dz, float4 ha, float4 hb) {
+  float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.f);
+  float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.f);
+  float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.f);
+  return ex * ex + ey * ey + ez * ez;
+}
+__device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, float4 hb,
+                                              const HdLattice& L) {
+  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
+  hd_min_image(dx, dy, dz, L);
+  if (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) return nb_box_gap2(dx, dy, dz, ha, hb);
+  float best = 3.0e38f;
+  for (int kc = -1; kc <= 1; ++kc)
+    for (int kb = -1; kb <= 1; ++kb)
+      for (int ka = -1; ka <= 1; ++ka) {
+        float sx = dx + (float)ka * L.lx + (float)kb * L.xy + (float)kc * L.xz;
+        float sy = dy + (float)kb * L.ly + (float)kc * L.yz;
+        float sz = dz + (float)kc * L.lz;
+        best = fminf(best, nb_box_gap2(sx, sy, sz, ha, hb));
+      }
+  return best;
+}
+
+// Block-pair candidates: bit (jb % 64) of mask[b * nchunks + jb / 64] is
+// set when j-block jb's box lies within r_search of block b's (jb >= b only
+// for the half list).  Each wave handles (b, chunk of 64 j-blocks) items in
+// a grid-stride loop, so the box tests of a rebuild run at full width
+// instead of as a dependent loop inside each block's wave, while the launch
+// stays small on the steps where the rebuild gate closes it at once.
+#define NB_PAIR_WAVES 4096
+// One (block, chunk of 64 j-blocks) item, one wave.
+__device__ __forceinline__ void nb_block_pair_item(unsigned wv, const float4* __restrict__ center,
+                                                   const float4* __restrict__ half,
+                                                   const HdLattice& L, float rs2,
+                                                   unsigned long long* __restrict__ mask,
+                                                   unsigned nblocks, unsigned nchunks, int full) {
+  unsigned lane = threadIdx.x & 63;
+  unsigned b = wv / nchunks, c = wv % nchunks;
+  unsigned jb = c * 64 + lane;
+  bool cand = false;
+  if (jb < nblocks && (full || jb >= b))
+    cand = nb_box_dist2(center[b], half[b], center[jb], half[jb], L) <= rs2;
+  unsigned long long m = __ballot(cand);
+  if (lane == 0) mask[wv] = m;
+}
+__global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ center,
+                                                     const float4* __restrict__ half,
+                                                     const float* __restrict__ lattice, float rs2,
+                                                     unsigned long long* __restrict__ mask,
+                                                     unsigned nblocks, unsigned nchunks, int full,
+                                                     const unsigned* status) {
+  if (!nb_rebuild_due(status)) return;
+  unsigned nwaves = gridDim.x * (blockDim.x >> 6);
+  HdLattice L = hd_load_lattice(lattice);
+  for (unsigned wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; wv < nblocks * nchunks;
+       wv += nwaves)
+    nb_block_pair_item(wv, center, half, L, rs2, mask, nblocks, nchunks, full);
+}
+
 // Systems of at most NB_SMALL_ATOMS atoms and NB_SMALL_VOX voxels: the
 // displacement check, the rebuild's voxel binning (keys, histogram, scan,
-// scatter, per-voxel sort) and the tile gather run in one workgroup, with
-// barriers between the phases in place of the seven launches of the
-// general path.  The histogram and the scatter cursors live in LDS (the
+// scatter, per-voxel sort), the tile gather and the block-pair box tests
+// run in one workgroup, with barriers between the phases in place of the
+// eight launches of the general path.  The histogram and the scatter cursors live in LDS (the
 // global voxel_count stays zero, as the general path leaves it).  Results
 // are those of the general path: the per-atom arithmetic is shared with its
 // kernels, the scan is exact and the per-voxel sort makes the slot order
@@ -417,6 +485,13 @@ __global__ void __launch_bounds__(NB_SMALL_THREADS) k_nb_small(HdNeighborArgs a,
   for (unsigned s = t; s < a.npad; s += NB_SMALL_THREADS)
     nb_gather_slot(s, rebuild, a.posq, a.sorted_id, a.lattice, a.tile_posq, fpj, fpj_ew, a.n,
                    a.type, a.key, a.tile_type, a.tile_key, a.center, a.half);
+  if (!rebuild) return;
+  __syncthreads();
+  // the box tests read every block's bounding box from the gather
+  const unsigned nblocks = a.npad / 64u, nchunks = (nblocks + 63u) / 64u;
+  for (unsigned wv = t >> 6; wv < nblocks * nchunks; wv += NB_SMALL_THREADS / 64u)
+    nb_block_pair_item(wv, a.center, a.half, L, a.rs2, a.block_mask, nblocks, nchunks,
+                       a.atom_lists);
 }
 
 bool hd_neighbor_small(const HdNeighborArgs& a) {
@@ -427,66 +502,6 @@ void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool chec
                               long long* fpj_ew) {
   hipLaunchKernelGGL(k_nb_small, dim3(1), dim3(NB_SMALL_THREADS), 0, st, a, check ? 1 : 0, fpj,
                      fpj_ew);
-}
-
-// Per-block axis-aligned bounding box: centre and half extents (one wave
-// per block).  Padding lanes are ignored.
-
-// Closest approach of two boxes under the minimum image (per axis; exact
-// for orthorhombic cells), as kernels/neighbor.cu:364-388.  In a tilted
-// cell a pair's own minimum image (fractional rounding of its separation)
-// can differ by one lattice vector per axis from the one the box centres
-// round to, so the 27 images around the centres' are all tried there.
-__device__ __forceinline__ float nb_box_gap2(float dx, float dy, float dz, float4 ha, float4 hb) {
-  float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.f);
-  float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.f);
-  float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.f);
-  return ex * ex + ey * ey + ez * ez;
-}
-__device__ __forceinline__ float nb_box_dist2(float4 ca, float4 ha, float4 cb, float4 hb,
-                                              const HdLattice& L) {
-  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
-  hd_min_image(dx, dy, dz, L);
-  if (L.xy == 0.f && L.xz == 0.f && L.yz == 0.f) return nb_box_gap2(dx, dy, dz, ha, hb);
-  float best = 3.0e38f;
-  for (int kc = -1; kc <= 1; ++kc)
-    for (int kb = -1; kb <= 1; ++kb)
-      for (int ka = -1; ka <= 1; ++ka) {
-        float sx = dx + (float)ka * L.lx + (float)kb * L.xy + (float)kc * L.xz;
-        float sy = dy + (float)kb * L.ly + (float)kc * L.yz;
-        float sz = dz + (float)kc * L.lz;
-        best = fminf(best, nb_box_gap2(sx, sy, sz, ha, hb));
-      }
-  return best;
-}
-
-// Block-pair candidates: bit (jb % 64) of mask[b * nchunks + jb / 64] is
-// set when j-block jb's box lies within r_search of block b's (jb >= b only
-// for the half list).  Each wave handles (b, chunk of 64 j-blocks) items in
-// a grid-stride loop, so the box tests of a rebuild run at full width
-// instead of as a dependent loop inside each block's wave, while the launch
-// stays small on the steps where the rebuild gate closes it at once.
-#define NB_PAIR_WAVES 4096
-__global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ center,
-                                                     const float4* __restrict__ half,
-                                                     const float* __restrict__ lattice, float rs2,
-                                                     unsigned long long* __restrict__ mask,
-                                                     unsigned nblocks, unsigned nchunks, int full,
-                                                     const unsigned* status) {
-  if (!nb_rebuild_due(status)) return;
-  unsigned lane = threadIdx.x & 63;
-  unsigned nwaves = gridDim.x * (blockDim.x >> 6);
-  HdLattice L = hd_load_lattice(lattice);
-  for (unsigned wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; wv < nblocks * nchunks;
-       wv += nwaves) {
-    unsigned b = wv / nchunks, c = wv % nchunks;
-    unsigned jb = c * 64 + lane;
-    bool cand = false;
-    if (jb < nblocks && (full || jb >= b))
-      cand = nb_box_dist2(center[b], half[b], center[jb], half[jb], L) <= rs2;
-    unsigned long long m = __ballot(cand);
-    if (lane == 0) mask[wv] = m;
-  }
 }
 
 #define NB_CAND_MAX 1024
@@ -917,13 +932,14 @@ void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long
 
 unsigned hd_block_pair_chunks(unsigned nblocks) { return cdiv(nblocks, 64); }
 
-void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a) {
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs) {
   unsigned nblocks = a.npad / 64;
   unsigned nchunks = hd_block_pair_chunks(nblocks);
   unsigned pair_waves = min(nblocks * nchunks, (unsigned)NB_PAIR_WAVES);
-  hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(pair_waves * 64, 256)), dim3(256), 0, st,
-                     a.center, a.half, a.lattice, a.rs2, a.block_mask, nblocks, nchunks,
-                     a.atom_lists, a.status);
+  if (block_pairs)
+    hipLaunchKernelGGL(k_block_pairs, dim3(cdiv(pair_waves * 64, 256)), dim3(256), 0, st,
+                       a.center, a.half, a.lattice, a.rs2, a.block_mask, nblocks, nchunks,
+                       a.atom_lists, a.status);
   if (a.atom_lists)
     hipLaunchKernelGGL(k_build_alist, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0, st,
                        a.tile_posq, a.tile_key, a.excl_span, a.center, a.half, a.lattice, a.rs2,

@@ -288,11 +288,12 @@ int hd_step(hd_ctx *ctx, int n_steps, int scalars_last);
  * (the velocity flip of tests/integration_lossless.rs:302-395). */
 int hd_negate_velocities(hd_ctx *ctx);
 int hd_use_graph(hd_ctx *ctx, int enable);
-/* When every atom belongs to a SETTLE water, the drift and both SETTLE
- * stages run as one kernel (default on); 0 runs the three separate
- * kernels, with bit-identical results.  Under Langevin BAOAB the fused
- * kernel covers both half steps and the three SETTLE stages (five
- * kernels).  No reference counterpart. */
+/* Fused integrator kernels (default on): when every atom belongs to a
+ * SETTLE water, the drift and both SETTLE stages run as one kernel (under
+ * Langevin BAOAB: both half steps and the three SETTLE stages, five
+ * kernels); under Langevin BAOAB without constraints the two half-step
+ * kernels run as one.  0 runs the separate kernels, with bit-identical
+ * results.  No reference counterpart. */
 int hd_set_fused_settle(hd_ctx *ctx, int enable);
 /* Systems of at most 8192 atoms run the neighbour displacement check, the
  * rebuild's voxel binning and the slot-order gather in one one-workgroup

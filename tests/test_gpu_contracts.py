@@ -122,6 +122,38 @@ def test_langevin_settle_fused_and_unfused_paths_bit_identical():
     say(f"  4 paths byte-identical, counter {ref[4]}, ke {ref[5]['ke']:.6e}")
 
 
+def test_langevin_free_atoms_fused_and_unfused_paths_bit_identical():
+    # BAOAB on argon without constraints: the fused half-step kernel against
+    # the separate B + A and O + A kernels, eager and through captured step
+    # graphs (a ragged atom count: the last block is partial)
+    say("case: argon 10000 Langevin, fused vs unfused, eager and graph, 12 steps")
+    s = systems.config_argon()
+    kt = U.temperature(120.0)
+    v0 = maxwell_boltzmann(s.mass, kt, seed=8)
+    out = {}
+    for fused in (True, False):
+        for graph in (False, True):
+            e = engine_for(s, dt=U.time(2e-15), velocities=v0,
+                           langevin=dict(friction=1.0 / U.time(1e-13), kt=kt, seed=9))
+            e.use_graph(graph)
+            e.set_fused_settle(fused)
+            e.force_eval(scalars=False)
+            e.step(5)
+            e.step(7, scalars_last=True)
+            p, img = e.positions(images=True)
+            out[fused, graph] = (p, img, e.velocities(), e.forces(), e.langevin_counter(),
+                                 e.scalars())
+            e.close()
+    ref = out[False, False]
+    assert ref[4] == 12
+    for key, o in out.items():
+        for k in range(4):
+            assert o[k].tobytes() == ref[k].tobytes(), f"{key} output {k}"
+        assert o[4] == ref[4], key
+        assert o[5] == ref[5], key
+    say(f"  4 paths byte-identical, counter {ref[4]}, ke {ref[5]['ke']:.6e}")
+
+
 def test_settle_waters_beside_free_atoms(oracle):
     # the last two molecules are left out of the SETTLE groups (six free
     # atoms, their intramolecular pairs still excluded), so the step runs
