@@ -174,6 +174,24 @@ __device__ __forceinline__ void hd_spline_origin(float x, float y, float z, cons
   gc = ((lc % (int)nc) + (int)nc) % (int)nc;
 }
 
+// Per-slot SPME spline record of a gathered position: (t_a, t_b, t_c, q)
+// and the packed origin cell g_a | g_b << 10 | g_c << 20, or HD_SPME_NO_CELL
+// for an uncharged atom (nothing written to spl then).
+#define HD_SPME_NO_CELL 0xFFFFFFFFu
+__device__ __forceinline__ void hd_spline_record(float4 pq, const HdLattice& L, unsigned na,
+                                                 unsigned nb, unsigned nc, float4* spl,
+                                                 unsigned* sg, unsigned s) {
+  if (pq.w == 0.f) {
+    sg[s] = HD_SPME_NO_CELL;
+    return;
+  }
+  float ta, tb, tc;
+  int ga, gb, gc;
+  hd_spline_origin(pq.x, pq.y, pq.z, L, na, nb, nc, ta, tb, tc, ga, gb, gc);
+  spl[s] = make_float4(ta, tb, tc, pq.w);
+  sg[s] = (unsigned)ga | ((unsigned)gb << 10) | ((unsigned)gc << 20);
+}
+
 // Wave64 rotation through DPP: every lane receives the value held by lane
 // (lane + 1) mod 64 ("wave_rol:1"; a full-rate VALU move, no LDS traffic).
 __device__ __forceinline__ int hd_rot1_i(int v) {

@@ -31,6 +31,7 @@ struct HdNeighborArgs {
   unsigned excl_span;
   unsigned long long* block_mask;  // nblocks x hd_block_pair_chunks(nblocks) candidate masks
   unsigned* block_order;           // blocks by entry count, heaviest first (rebuild steps)
+  unsigned* build_ticket;          // zeroed word for the folded rebuild-clear
 };
 unsigned hd_block_pair_chunks(unsigned nblocks);
 
@@ -41,8 +42,11 @@ void hd_launch_disp_check(hipStream_t st, const HdNeighborArgs& a);
 // number of chunk sums of the voxel-count scan (the buffer needs one more)
 unsigned hd_voxel_scan_parts(unsigned nvox);
 // block_pairs = false when hd_launch_neighbor_small already ran the
-// block-pair box tests of this step
-void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs = true);
+// block-pair box tests of this step; fold_clear: the packed-list build's
+// last workgroup orders the blocks and clears the rebuild request (no
+// separate launch; needs build_ticket)
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs = true,
+                              bool fold_clear = false);
 void hd_launch_tile_gather(hipStream_t st, const float4* posq, const unsigned* sorted_id,
                            const float* lattice, float4* tile_posq, long long* fpj,
                            long long* fpj_ew, unsigned n, unsigned npad);
@@ -56,8 +60,11 @@ void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long
 // launch, with the results of hd_launch_neighbor_update +
 // hd_launch_tile_gather_rebuild + the box tests of hd_launch_neighbor_build
 bool hd_neighbor_small(const HdNeighborArgs& a);
+// spl != nullptr: the gather also writes the SPME spline records of a grid
+// of na x nb x nc cells (k_spline_setup's output)
 void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool check, long long* fpj,
-                              long long* fpj_ew);
+                              long long* fpj_ew, float4* spl = nullptr, unsigned* sg = nullptr,
+                              unsigned na = 0, unsigned nb = 0, unsigned nc = 0);
 
 struct HdPairArgs {
   unsigned n, npad, cap;
@@ -135,6 +142,7 @@ struct HdSpmeArgs {
   unsigned nv[3];
   int mv[3];  // voxel margin per axis: >= half the skin in voxel units, plus one
   int order;  // B-spline order, 4 or 5
+  int splines_done;  // spl / sg already hold this evaluation's records
 };
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
 // the spread half of hd_launch_spme (spline records + charge grid) alone

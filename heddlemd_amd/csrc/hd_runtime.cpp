@@ -188,7 +188,7 @@ struct hd_ctx {
   unsigned *entry_atoms = nullptr, *entry_count = nullptr;
   unsigned long long* block_mask = nullptr;  // block-pair candidate masks
   unsigned* block_order = nullptr;           // pair workgroup -> block
-  unsigned* ctrl = nullptr;  // [0] status, [1] rebuild count, [2] max entries
+  unsigned* ctrl = nullptr;  // [0] status, [1] rebuild count, [2] max entries, [3] build ticket
   long long *fpi = nullptr, *fpi_ew = nullptr, *fpj = nullptr, *fpj_ew = nullptr;
   // device: SPME
   long long* rho_fixed = nullptr;
@@ -426,6 +426,7 @@ HdNeighborArgs nb_args(hd_ctx* c) {
   a.max_count = c->ctrl + 2;
   a.status = c->ctrl;
   a.rebuild_count = c->ctrl + 1;
+  a.build_ticket = c->ctrl + 3;
   a.atom_lists = c->atom_lists;
   a.excl_span = c->excl_span;
   a.block_mask = c->block_mask;
@@ -494,9 +495,10 @@ void stage_end(hd_ctx* c, bool on) {
 }
 
 // The SPME reciprocal pipeline of one force evaluation, on stream sst.
-static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed) {
+static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed, bool splines_done) {
   HdSpmeArgs sa;
   memset(&sa, 0, sizeof(sa));
+  sa.splines_done = splines_done ? 1 : 0;
   sa.n = (unsigned)c->n;
   sa.order = c->spline_order;
   sa.na = c->grid[0];
@@ -576,7 +578,11 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   const bool small_nb = c->small_neighbor && hd_neighbor_small(na);
   if (small_nb) {
     stage_begin(c, "neighbor_update", timed);
-    hd_launch_neighbor_small(st, na, !disp_done, c->fpj, ev ? c->fpj_ew : nullptr);
+    if (c->have_spme)
+      hd_launch_neighbor_small(st, na, !disp_done, c->fpj, ev ? c->fpj_ew : nullptr, c->spl,
+                               c->sg, c->grid[0], c->grid[1], c->grid[2]);
+    else
+      hd_launch_neighbor_small(st, na, !disp_done, c->fpj, ev ? c->fpj_ew : nullptr);
     stage_end(c, timed);
   } else {
     stage_begin(c, "neighbor_update", timed);
@@ -595,11 +601,11 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   if (fork) {
     (void)hipEventRecord(c->ev_fork, st);
     (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-    enqueue_spme(c, c->side, false);
+    enqueue_spme(c, c->side, false, small_nb);
     (void)hipEventRecord(c->ev_join, c->side);
   }
   stage_begin(c, "neighbor_build", timed);
-  hd_launch_neighbor_build(st, na, !small_nb);
+  hd_launch_neighbor_build(st, na, !small_nb, small_nb);
   stage_end(c, timed);
   HdPairArgs pa;
   memset(&pa, 0, sizeof(pa));
@@ -650,7 +656,7 @@ void enqueue_forces(hd_ctx* c, int ev, bool kick, bool timed, bool disp_done = f
   if (fork)
     (void)hipStreamWaitEvent(st, c->ev_join, 0);
   else if (c->have_spme)
-    enqueue_spme(c, st, timed);
+    enqueue_spme(c, st, timed, small_nb);
   HdFinalizeArgs fa;
   memset(&fa, 0, sizeof(fa));
   fa.n = (unsigned)c->n;

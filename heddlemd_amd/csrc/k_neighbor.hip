@@ -391,9 +391,10 @@ __global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ 
 
 // Systems of at most NB_SMALL_ATOMS atoms and NB_SMALL_VOX voxels: the
 // displacement check, the rebuild's voxel binning (keys, histogram, scan,
-// scatter, per-voxel sort), the tile gather and the block-pair box tests
-// run in one workgroup, with barriers between the phases in place of the
-// eight launches of the general path.  The histogram and the scatter cursors live in LDS (the
+// scatter, per-voxel sort), the tile gather (with the SPME spline records
+// when asked) and the block-pair box tests run in one workgroup, with
+// barriers between the phases in place of the nine launches of the general
+// path.  The histogram and the scatter cursors live in LDS (the
 // global voxel_count stays zero, as the general path leaves it).  Results
 // are those of the general path: the per-atom arithmetic is shared with its
 // kernels, the scan is exact and the per-voxel sort makes the slot order
@@ -404,7 +405,11 @@ __global__ void __launch_bounds__(256) k_block_pairs(const float4* __restrict__ 
 
 __global__ void __launch_bounds__(NB_SMALL_THREADS) k_nb_small(HdNeighborArgs a, int check,
                                                                long long* __restrict__ fpj,
-                                                               long long* __restrict__ fpj_ew) {
+                                                               long long* __restrict__ fpj_ew,
+                                                               float4* __restrict__ spl,
+                                                               unsigned* __restrict__ sg,
+                                                               unsigned gna, unsigned gnb,
+                                                               unsigned gnc) {
   __shared__ unsigned cnt[NB_SMALL_VOX];
   __shared__ unsigned scan[NB_SMALL_THREADS];
   __shared__ int s_over;
@@ -485,6 +490,12 @@ __global__ void __launch_bounds__(NB_SMALL_THREADS) k_nb_small(HdNeighborArgs a,
   for (unsigned s = t; s < a.npad; s += NB_SMALL_THREADS)
     nb_gather_slot(s, rebuild, a.posq, a.sorted_id, a.lattice, a.tile_posq, fpj, fpj_ew, a.n,
                    a.type, a.key, a.tile_type, a.tile_key, a.center, a.half);
+  if (spl) {
+    // SPME spline records from the gathered positions (each thread reads
+    // back the slots it wrote)
+    for (unsigned s = t; s < a.n; s += NB_SMALL_THREADS)
+      hd_spline_record(a.tile_posq[s], L, gna, gnb, gnc, spl, sg, s);
+  }
   if (!rebuild) return;
   __syncthreads();
   // the box tests read every block's bounding box from the gather
@@ -499,9 +510,10 @@ bool hd_neighbor_small(const HdNeighborArgs& a) {
 }
 
 void hd_launch_neighbor_small(hipStream_t st, const HdNeighborArgs& a, bool check, long long* fpj,
-                              long long* fpj_ew) {
+                              long long* fpj_ew, float4* spl, unsigned* sg, unsigned na,
+                              unsigned nb, unsigned nc) {
   hipLaunchKernelGGL(k_nb_small, dim3(1), dim3(NB_SMALL_THREADS), 0, st, a, check ? 1 : 0, fpj,
-                     fpj_ew);
+                     fpj_ew, spl, sg, na, nb, nc);
 }
 
 #define NB_CAND_MAX 1024
@@ -597,12 +609,22 @@ __device__ __forceinline__ bool nb_test_queued(const float4* ipos, float4 pj, un
   return hit;
 }
 
+#define NB_FOLD_MAX_BLOCKS 128u  // block counts the folded ordering handles
+
+// FOLD (at most NB_FOLD_MAX_BLOCKS blocks): the workgroup that takes the
+// last ticket (*ticket, 0 between launches) also does k_rebuild_clear's
+// work once every block's entry count is written: blocks ordered by entry
+// count, heaviest first (ties by block index), the rebuild request cleared
+// and the rebuild counted.  The order only maps workgroups to blocks in
+// the pair kernel, so any permutation gives the same forces.
+template <bool FOLD>
 __global__ void __launch_bounds__(256) k_build_entries(
     const float4* __restrict__ tile_posq, const float4* __restrict__ center,
     const float4* __restrict__ half, const float* __restrict__ lattice, float rs2,
     const unsigned long long* __restrict__ mask, unsigned nchunks,
     unsigned* __restrict__ entry_atoms, unsigned* __restrict__ entry_count, unsigned cap,
-    unsigned* __restrict__ max_count, unsigned* status, unsigned n, unsigned nblocks) {
+    unsigned* __restrict__ max_count, unsigned* status, unsigned n, unsigned nblocks,
+    unsigned* ticket, unsigned* rebuild_count, unsigned* __restrict__ order) {
   if (!nb_rebuild_due(status)) return;
   __shared__ float4 ipos[NB_WAVES][64];
   __shared__ unsigned buf[NB_WAVES][128];
@@ -611,7 +633,7 @@ __global__ void __launch_bounds__(256) k_build_entries(
   __shared__ unsigned cand[NB_WAVES][NB_CAND_MAX];
   unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned b = blockIdx.x * NB_WAVES + w;
-  if (b >= nblocks) return;
+  if (b < nblocks) {
   HdLattice L = hd_load_lattice(lattice);
   unsigned islot = b * 64 + lane;
   float4 pi = tile_posq[islot];
@@ -728,6 +750,33 @@ __global__ void __launch_bounds__(256) k_build_entries(
     atomicMax(max_count, nent);
     if (nent > cap || too_many) atomicOr(status, NB_STATUS_OVERFLOW);
     else if (4 * nent > 3 * cap) atomicOr(status, NB_STATUS_HIGHWATER);
+  }
+  }  // b < nblocks
+  if (!FOLD) return;
+  __shared__ unsigned s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();  // every block's counts and status bits are visible
+  __shared__ unsigned s_cnt[NB_FOLD_MAX_BLOCKS];
+  for (unsigned i = threadIdx.x; i < nblocks; i += blockDim.x) s_cnt[i] = entry_count[i];
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    unsigned ci = s_cnt[i], pos = 0;
+    for (unsigned j = 0; j < nblocks; ++j) {
+      unsigned cj = s_cnt[j];
+      pos += (cj > ci || (cj == ci && j < i)) ? 1u : 0u;
+    }
+    order[pos] = i;
+  }
+  if (threadIdx.x == 0) {
+    atomicAnd(status, ~(NB_STATUS_REBUILD | NB_STATUS_FORCE));
+    atomicAdd(rebuild_count, 1u);
+    atomicExch(ticket, 0u);
   }
 }
 
@@ -932,8 +981,10 @@ void hd_launch_tile_gather_rebuild(hipStream_t st, const HdNeighborArgs& a, long
 
 unsigned hd_block_pair_chunks(unsigned nblocks) { return cdiv(nblocks, 64); }
 
-void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs) {
+void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool block_pairs,
+                              bool fold_clear) {
   unsigned nblocks = a.npad / 64;
+  const bool fold = fold_clear && !a.atom_lists && nblocks <= NB_FOLD_MAX_BLOCKS;
   unsigned nchunks = hd_block_pair_chunks(nblocks);
   unsigned pair_waves = min(nblocks * nchunks, (unsigned)NB_PAIR_WAVES);
   if (block_pairs)
@@ -945,10 +996,17 @@ void hd_launch_neighbor_build(hipStream_t st, const HdNeighborArgs& a, bool bloc
                        a.tile_posq, a.tile_key, a.excl_span, a.center, a.half, a.lattice, a.rs2,
                        a.block_mask, nchunks, a.entry_atoms, a.entry_count, a.cap, a.max_count,
                        a.status, a.n, nblocks);
+  else if (fold)
+    hipLaunchKernelGGL(k_build_entries<true>, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES),
+                       0, st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask,
+                       nchunks, a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n,
+                       nblocks, a.build_ticket, a.rebuild_count, a.block_order);
   else
-    hipLaunchKernelGGL(k_build_entries, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES), 0,
-                       st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask, nchunks,
-                       a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n, nblocks);
-  hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(NB_ORDER_BINS), 0, st, a.status,
-                     a.rebuild_count, a.entry_count, nblocks, a.block_order);
+    hipLaunchKernelGGL(k_build_entries<false>, dim3(cdiv(nblocks, NB_WAVES)), dim3(64 * NB_WAVES),
+                       0, st, a.tile_posq, a.center, a.half, a.lattice, a.rs2, a.block_mask,
+                       nchunks, a.entry_atoms, a.entry_count, a.cap, a.max_count, a.status, a.n,
+                       nblocks, nullptr, nullptr, nullptr);
+  if (!fold)
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(1), dim3(NB_ORDER_BINS), 0, st, a.status,
+                       a.rebuild_count, a.entry_count, nblocks, a.block_order);
 }

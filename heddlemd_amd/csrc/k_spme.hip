@@ -53,7 +53,7 @@ __device__ __forceinline__ float bspline_w(float x) {
 // (hd_spline_origin) of every charged atom, as (t_a, t_b, t_c, q) plus
 // the packed origin cell g_a | g_b << 10 | g_c << 20 (SPME_NO_CELL for
 // uncharged atoms).  Read by the spread and by the force gather.
-#define SPME_NO_CELL 0xFFFFFFFFu
+#define SPME_NO_CELL HD_SPME_NO_CELL
 
 __global__ void k_spline_setup(const float4* __restrict__ tile_posq,
                                const float* __restrict__ lattice, unsigned na, unsigned nb,
@@ -61,17 +61,7 @@ __global__ void k_spline_setup(const float4* __restrict__ tile_posq,
                                unsigned n) {
   unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  float4 pq = tile_posq[s];
-  if (pq.w == 0.f) {
-    sg[s] = SPME_NO_CELL;
-    return;
-  }
-  HdLattice L = hd_load_lattice(lattice);
-  float ta, tb, tc;
-  int ga, gb, gc;
-  hd_spline_origin(pq.x, pq.y, pq.z, L, na, nb, nc, ta, tb, tc, ga, gb, gc);
-  spl[s] = make_float4(ta, tb, tc, pq.w);
-  sg[s] = (unsigned)ga | ((unsigned)gb << 10) | ((unsigned)gc << 20);
+  hd_spline_record(tile_posq[s], hd_load_lattice(lattice), na, nb, nc, spl, sg, s);
 }
 
 // Spread: the reference accumulates each atom's p^3 contributions with
@@ -774,7 +764,7 @@ static unsigned ilog2u(unsigned v) {
 void hd_launch_spme_spread(hipStream_t st, const HdSpmeArgs& a) {
   unsigned brick[3];
   unsigned nbricks = hd_spread_bricks(a.na, a.nb, a.nc, brick);
-  if (a.n > 0)
+  if (a.n > 0 && !a.splines_done)
     hipLaunchKernelGGL(k_spline_setup, dim3((a.n + 255) / 256), dim3(256), 0, st, a.tile_posq,
                        a.lattice, a.na, a.nb, a.nc, a.spl, a.sg, a.n);
   if (a.order == 5)

@@ -718,17 +718,21 @@ def test_batched_replicas_byte_identical_to_solo_runs():
 def test_small_neighbor_kernel_matches_the_general_path():
     # systems of at most 8192 atoms run the displacement check, the voxel
     # binning and the slot gather in one workgroup: argon and water replicas
-    # (ragged atom counts, SPME, SETTLE) under Langevin, and one argon
-    # replica under NHC (the check then runs in the KE pass), end in the
-    # same bits, pair list and rebuild count as on the general path
+    # (ragged atom counts, SPME with the spline records written by the
+    # neighbour kernel, SETTLE) under Langevin, one water replica with the
+    # matrix-core DFT, and one argon replica under NHC (the check then runs
+    # in the KE pass), end in the same bits, pair list and rebuild count as
+    # on the general path
     say("case: one-workgroup neighbour kernel vs general path, 200 steps")
     specs = [s for s in systems.replica_specs(count=256)][::43][:6]
     say("  " + ", ".join(f"{s['kind']}:{s['n_atoms']}" for s in specs))
-    runs = [(sp, r, False) for r, sp in enumerate(specs)] + [(specs[0], 0, True)]
-    for sp, r, nhc in runs:
+    runs = [(sp, r, "") for r, sp in enumerate(specs)] + [(specs[0], 0, "nhc")]
+    runs += [(sp, r, "dft") for r, sp in enumerate(specs)
+             if sp["kind"] == "water" and systems.replica_system(sp).spme.grid[0] == 32][:1]
+    for sp, r, mode in runs:
         out = []
         for small in (True, False):
-            if nhc:
+            if mode == "nhc":
                 s = systems.replica_system(sp)
                 kt = U.temperature(sp["temperature_K"])
                 from heddlemd_amd.rng import maxwell_boltzmann
@@ -737,6 +741,8 @@ def test_small_neighbor_kernel_matches_the_general_path():
                                nhc_tau=U.time(1e-13))
             else:
                 e = _replica_engine(sp, r)
+            if mode == "dft":
+                e.set_spme_dft(1)
             e.set_small_neighbor(small)
             e.step(150)
             e.step(50, scalars_last=True)
@@ -744,7 +750,7 @@ def test_small_neighbor_kernel_matches_the_general_path():
             out.append((p, i, v, f, sc, e.pairs()))
             e.close()
         (a, b) = out
-        say(f"  {sp['kind']}:{sp['n_atoms']}{' nhc' if nhc else ''}: rebuilds "
+        say(f"  {sp['kind']}:{sp['n_atoms']} {mode}: rebuilds "
             f"{a[4]['rebuilds']:.0f}, pairs {len(a[5])}")
         assert a[4]["rebuilds"] >= 2
         for k in range(4):
