@@ -559,12 +559,50 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_fwd(
 
 // c-axis inverse C2R: nc/2+1 spectrum values per row -> nc reals,
 // unnormalised (round trip = nc * x).
+// 256 threads; se, sw: 256 doubles of LDS each
+static_assert(FFT_THREADS == 256, "the inverse c pass runs the 256-thread reduction");
+__device__ __forceinline__ void recip_reduce_block(const double2* __restrict__ partials,
+                                                   unsigned np, double* __restrict__ recip,
+                                                   float* __restrict__ w_share, unsigned n,
+                                                   double* se, double* sw) {
+  double e = 0.0, w = 0.0;
+  for (unsigned i = threadIdx.x; i < np; i += 256) {
+    double2 p = partials[i];
+    e += p.x;
+    w += p.y;
+  }
+  se[threadIdx.x] = e;
+  sw[threadIdx.x] = w;
+  __syncthreads();
+  for (unsigned s = 1; s < 256; s <<= 1) {
+    if ((threadIdx.x % (2 * s)) == 0) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      sw[threadIdx.x] += sw[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    recip[0] = 0.5 * se[0];
+    recip[1] = 0.5 * sw[0];
+    w_share[0] = n ? (float)(0.5 * sw[0] / (double)n) : 0.f;
+  }
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_c_inv(
     const float2* __restrict__ cgrid, float* __restrict__ vgrid, unsigned nrows, unsigned nc,
-    unsigned log2h, const float2* __restrict__ twc) {
+    unsigned log2h, const float2* __restrict__ twc, const double2* __restrict__ partials,
+    unsigned np, double* __restrict__ recip, float* __restrict__ w_share, unsigned n_atoms) {
   __shared__ float2 d[FFT_TILE];
   __shared__ float2 stw[FFT_TW_MAX];
+  if (partials && blockIdx.x == 0) {
+    // the convolution's energy and virial partials are complete (written
+    // by the a-axis pass before this launch): block 0 reduces them first,
+    // in k_recip_reduce's order, with the tile's LDS as its scratch
+    double* se = reinterpret_cast<double*>(d);
+    recip_reduce_block(partials, np, recip, w_share, n_atoms, se, se + 256);
+    __syncthreads();
+  }
   unsigned h = nc >> 1, nch = h + 1;
   unsigned rows = FFT_TILE / h;
   unsigned r0 = blockIdx.x * rows;
@@ -732,27 +770,7 @@ __global__ void __launch_bounds__(256) k_recip_reduce(const double2* __restrict_
                                                       unsigned np, double* __restrict__ recip,
                                                       float* __restrict__ w_share, unsigned n) {
   __shared__ double se[256], sw[256];
-  double e = 0.0, w = 0.0;
-  for (unsigned i = threadIdx.x; i < np; i += 256) {
-    double2 p = partials[i];
-    e += p.x;
-    w += p.y;
-  }
-  se[threadIdx.x] = e;
-  sw[threadIdx.x] = w;
-  __syncthreads();
-  for (unsigned s = 1; s < 256; s <<= 1) {
-    if ((threadIdx.x % (2 * s)) == 0) {
-      se[threadIdx.x] += se[threadIdx.x + s];
-      sw[threadIdx.x] += sw[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    recip[0] = 0.5 * se[0];
-    recip[1] = 0.5 * sw[0];
-    w_share[0] = n ? (float)(0.5 * sw[0] / (double)n) : 0.f;
-  }
+  recip_reduce_block(partials, np, recip, w_share, n, se, sw);
 }
 
 static unsigned ilog2u(unsigned v) {
@@ -820,14 +838,13 @@ void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
       hipLaunchKernelGGL(k_fft_b<false>, gb, tb, 0, st, a.cgrid, a.na, a.nb, nch, ilog2u(a.nb),
                          sign, a.tw_b);
   }
+  // the inverse c pass also reduces the energy and virial partials
   if (mc)
     hipLaunchKernelGGL(k_fft_c_inv<true>, gc, tb, 0, st, a.cgrid, a.vgrid, nrows, a.nc, 0u,
-                       a.tw_c);
+                       a.tw_c, a.partials, a.nb * nchunk, a.recip, a.w_share, a.n);
   else
     hipLaunchKernelGGL(k_fft_c_inv<false>, gc, tb, 0, st, a.cgrid, a.vgrid, nrows, a.nc,
-                       ilog2u(h), a.tw_c);
-  hipLaunchKernelGGL(k_recip_reduce, dim3(1), dim3(256), 0, st, a.partials, a.nb * nchunk,
-                     a.recip, a.w_share, a.n);
+                       ilog2u(h), a.tw_c, a.partials, a.nb * nchunk, a.recip, a.w_share, a.n);
 }
 
 unsigned hd_spme_partials_count(unsigned nb, unsigned nc) {
