@@ -3475,6 +3475,16 @@ int hd_batch_set_lanes(hd_batch* b, int lanes) {
   return HD_OK;
 }
 
+// Relative cost of one replica step in a batch graph: launches dominate
+// small replicas (about 4 us each), atoms and SPME grid points the large
+// ones (per-atom and per-point rates of the large single-system step).
+static double batch_step_cost(const hd_ctx* c) {
+  double launches = c->have_spme ? 11.0 : 5.0;
+  double us = 4.0 * launches + 0.002 * (double)c->n;
+  if (c->have_spme) us += 0.0005 * (double)c->grid[0] * c->grid[1] * c->grid[2];
+  return us;
+}
+
 static int batch_capture(hd_batch* b, bool ev, hipGraphExec_t* out) {
   *out = nullptr;
   for (auto c : b->ctx) {
@@ -3486,19 +3496,36 @@ static int batch_capture(hd_batch* b, bool ev, hipGraphExec_t* out) {
   // errors inside the capture are collected, and the capture is always
   // ended, so no stream is left capturing after a failure
   hipError_t e = hipEventRecord(b->fork, b->stream);
-  // replicas form `lanes` chains: replica i follows replica i - lanes
+  // replicas form `lanes` chains, packed by estimated step cost: longest
+  // first, each onto the chain with the least cost so far, so the chains
+  // end together (the graph takes as long as its longest chain)
   size_t n = b->ctx.size();
   size_t lanes = (b->lanes <= 0 || (size_t)b->lanes >= n) ? n : (size_t)b->lanes;
-  for (size_t i = 0; i < n && e == hipSuccess; ++i) {
+  std::vector<size_t> by_cost(n);
+  std::vector<double> cost(n);
+  for (size_t i = 0; i < n; ++i) {
+    by_cost[i] = i;
+    cost[i] = batch_step_cost(b->ctx[i]);
+  }
+  std::stable_sort(by_cost.begin(), by_cost.end(),
+                   [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::vector<double> load(lanes, 0.0);
+  std::vector<long> tail(lanes, -1);  // last replica of each chain
+  for (size_t k = 0; k < n && e == hipSuccess; ++k) {
+    size_t i = by_cost[k];
+    size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[l] += cost[i];
     hd_ctx* c = b->ctx[i];
-    e = hipStreamWaitEvent(c->stream, i < lanes ? b->fork : b->join[i - lanes], 0);
+    e = hipStreamWaitEvent(c->stream, tail[l] < 0 ? b->fork : b->join[(size_t)tail[l]], 0);
     if (e != hipSuccess) break;
+    tail[l] = (long)i;
     c->spme_fork = false;
     enqueue_step(c, ev, false);
     c->spme_fork = true;
     e = hipEventRecord(b->join[i], c->stream);
-    if (e == hipSuccess && i + lanes >= n) e = hipStreamWaitEvent(b->stream, b->join[i], 0);
   }
+  for (size_t l = 0; l < lanes && e == hipSuccess; ++l)
+    if (tail[l] >= 0) e = hipStreamWaitEvent(b->stream, b->join[(size_t)tail[l]], 0);
   if (e == hipSuccess) e = hipGetLastError();
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(b->stream, &g);
