@@ -681,6 +681,117 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgri
   strided_store(d, cgrid, base, nb, log2b, nch, ncol, false);
 }
 
+// Grids whose a-plane fits in LDS (nb * nc/2 <= FFT_TILE complex values
+// and the plane's nc/2 + 1 columns of nb, padded, within FFT_PLANE_MAX):
+// one workgroup per a-plane runs the c-axis and b-axis passes of the
+// forward transform back to back (and of the inverse, b then c), with the
+// plane staged in LDS between them instead of a round trip through the
+// complex grid and a second launch.  Each line is transformed by the same
+// LDS routines, on the same values, as in k_fft_c_fwd / k_fft_b /
+// k_fft_c_inv, so the results are the same.
+#define FFT_PLANE_MAX 2176  // 33 columns x 65 (c <= 64, b <= 64)
+
+__host__ __device__ inline bool fft_plane_fits(unsigned nb, unsigned nc) {
+  return nb * (nc / 2) <= FFT_TILE && (nc / 2 + 1) * (nb + 1) <= FFT_PLANE_MAX;
+}
+
+template <bool MC, bool MB>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_cb_fwd(
+    const long long* __restrict__ rho, float2* __restrict__ cgrid, unsigned nb, unsigned nc,
+    unsigned log2h, unsigned log2b, const float2* __restrict__ twc,
+    const float2* __restrict__ twb) {
+  __shared__ float2 d[FFT_TILE];
+  __shared__ float2 col[FFT_PLANE_MAX];
+  __shared__ float2 stw[FFT_TW_MAX], stwb[FFT_TW_MAX];
+  const unsigned h = nc >> 1, nch = h + 1, nr = nb, ls = nb + 1;
+  const unsigned r0 = blockIdx.x * nb;
+  const float inv = 1.0f / 4294967296.0f;
+  fft_stage_twiddles(stw, twc, nc);
+  fft_stage_twiddles(stwb, twb, nb);
+  for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
+    unsigned r = e / h, m = e - r * h;
+    longlong2 v = reinterpret_cast<const longlong2*>(rho + (size_t)(r0 + r) * nc)[m];
+    d[r * h + (MC ? m : bitrev(m, log2h))] = make_float2((float)v.x * inv, (float)v.y * inv);
+  }
+  __syncthreads();
+  if (MC)
+    lds_fft_mixed(d, h, nr, h, -1, stw, nc);
+  else
+    lds_fft(d, h, log2h, nr, h, -1, stw, nc);
+  // k_fft_c_fwd's untangling, into the b-axis columns
+  for (unsigned e = threadIdx.x; e < nr * nch; e += blockDim.x) {
+    unsigned r = e / nch, k = e - r * nch;
+    float2 zk = d[r * h + (k % h)];
+    float2 zc = d[r * h + ((h - k) % h)];
+    zc.y = -zc.y;
+    float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    float2 O = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+    float2 w = (k < h) ? stw[k] : make_float2(-1.f, 0.f);
+    float2 wO = make_float2(w.x * O.x - w.y * O.y, w.x * O.y + w.y * O.x);
+    col[k * ls + (MB ? r : bitrev(r, log2b))] = make_float2(E.x + wO.x, E.y + wO.y);
+  }
+  __syncthreads();
+  if (MB)
+    lds_fft_mixed(col, nb, nch, ls, -1, stwb, nb);
+  else
+    lds_fft(col, nb, log2b, nch, ls, -1, stwb, nb);
+  for (unsigned e = threadIdx.x; e < nr * nch; e += blockDim.x) {
+    unsigned r = e / nch, k = e - r * nch;
+    cgrid[(size_t)(r0 + r) * nch + k] = col[k * ls + r];
+  }
+}
+
+template <bool MB, bool MC>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_bc_inv(
+    const float2* __restrict__ cgrid, float* __restrict__ vgrid, unsigned nb, unsigned nc,
+    unsigned log2h, unsigned log2b, const float2* __restrict__ twc,
+    const float2* __restrict__ twb, const double2* __restrict__ partials, unsigned np,
+    double* __restrict__ recip, float* __restrict__ w_share, unsigned n_atoms) {
+  __shared__ float2 d[FFT_TILE];
+  __shared__ float2 col[FFT_PLANE_MAX];
+  __shared__ float2 stw[FFT_TW_MAX], stwb[FFT_TW_MAX];
+  const unsigned h = nc >> 1, nch = h + 1, nr = nb, ls = nb + 1;
+  const unsigned r0 = blockIdx.x * nb;
+  if (partials && blockIdx.x == 0) {
+    // as in k_fft_c_inv: block 0 reduces the energy and virial partials
+    double* se = reinterpret_cast<double*>(d);
+    recip_reduce_block(partials, np, recip, w_share, n_atoms, se, se + 256);
+    __syncthreads();
+  }
+  fft_stage_twiddles(stw, twc, nc);
+  fft_stage_twiddles(stwb, twb, nb);
+  for (unsigned e = threadIdx.x; e < nr * nch; e += blockDim.x) {
+    unsigned r = e / nch, k = e - r * nch;
+    col[k * ls + (MB ? r : bitrev(r, log2b))] = cgrid[(size_t)(r0 + r) * nch + k];
+  }
+  __syncthreads();
+  if (MB)
+    lds_fft_mixed(col, nb, nch, ls, +1, stwb, nb);
+  else
+    lds_fft(col, nb, log2b, nch, ls, +1, stwb, nb);
+  // k_fft_c_inv's packing of the row spectrum, from the b-axis columns
+  for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
+    unsigned r = e / h, k = e - r * h;
+    float2 xk = col[k * ls + r], xc = col[(h - k) * ls + r];
+    xc.y = -xc.y;
+    float2 A = make_float2(xk.x + xc.x, xk.y + xc.y);
+    float2 B = make_float2(xk.x - xc.x, xk.y - xc.y);
+    float2 w = stw[k];
+    w.y = -w.y;
+    float2 wB = make_float2(w.x * B.x - w.y * B.y, w.x * B.y + w.y * B.x);
+    d[r * h + (MC ? k : bitrev(k, log2h))] = make_float2(A.x - wB.y, A.y + wB.x);
+  }
+  __syncthreads();
+  if (MC)
+    lds_fft_mixed(d, h, nr, h, +1, stw, nc);
+  else
+    lds_fft(d, h, log2h, nr, h, +1, stw, nc);
+  for (unsigned e = threadIdx.x; e < nr * h; e += blockDim.x) {
+    unsigned r = e / h, m = e - r * h;
+    reinterpret_cast<float2*>(vgrid + (size_t)(r0 + r) * nc)[m] = d[r * h + m];
+  }
+}
+
 // a-axis pass fused with the influence multiply:
 //   forward FFT over a -> for every k: G(k), virial factor vf(k)
 //   (spme_recip.cu:34-115 in f32 from the live lattice), energy and
@@ -804,6 +915,26 @@ void hd_launch_recip_reduce(hipStream_t st, const HdSpmeArgs& a, unsigned n_part
 
 static bool pow2u(unsigned v) { return v && !(v & (v - 1)); }
 
+template <bool MC, bool MB>
+static void launch_plane_passes(hipStream_t st, const HdSpmeArgs& a, bool fwd, unsigned log2h,
+                                unsigned log2b, unsigned np) {
+  if (fwd)
+    hipLaunchKernelGGL((k_fft_cb_fwd<MC, MB>), dim3(a.na), dim3(FFT_THREADS), 0, st,
+                       a.rho_fixed, a.cgrid, a.nb, a.nc, log2h, log2b, a.tw_c, a.tw_b);
+  else
+    hipLaunchKernelGGL((k_fft_bc_inv<MB, MC>), dim3(a.na), dim3(FFT_THREADS), 0, st, a.cgrid,
+                       a.vgrid, a.nb, a.nc, log2h, log2b, a.tw_c, a.tw_b, a.partials, np,
+                       a.recip, a.w_share, a.n);
+}
+
+static void plane_passes(hipStream_t st, const HdSpmeArgs& a, bool fwd, bool mc, bool mb,
+                         unsigned log2h, unsigned log2b, unsigned np) {
+  if (mc && mb) launch_plane_passes<true, true>(st, a, fwd, log2h, log2b, np);
+  else if (mc) launch_plane_passes<true, false>(st, a, fwd, log2h, log2b, np);
+  else if (mb) launch_plane_passes<false, true>(st, a, fwd, log2h, log2b, np);
+  else launch_plane_passes<false, false>(st, a, fwd, log2h, log2b, np);
+}
+
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
   unsigned nch = a.nc / 2 + 1;
   hd_launch_spme_spread(st, a);
@@ -812,6 +943,23 @@ void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
   unsigned rows = FFT_TILE / h;
   dim3 gc((nrows + rows - 1) / rows), tb(FFT_THREADS);
   bool mc = !pow2u(h), mb = !pow2u(a.nb), ma = !pow2u(a.na);
+  if (fft_plane_fits(a.nb, a.nc)) {
+    // c and b passes per a-plane in one launch each way
+    unsigned nchunk = (nch + FFT_CW - 1) / FFT_CW;
+    unsigned lh = mc ? 0u : ilog2u(h), lb = mb ? 0u : ilog2u(a.nb);
+    plane_passes(st, a, true, mc, mb, lh, lb, 0);
+    dim3 ga(a.nb * nchunk);
+    if (ma)
+      hipLaunchKernelGGL(k_fft_a_conv<true>, ga, tb, 0, st, a.cgrid, a.na, a.nb, a.nc, nch, 0u,
+                         a.tw_a, a.bf_a, a.bf_b, a.bf_c, a.lattice, (float)a.kc, (float)a.alpha,
+                         a.partials);
+    else
+      hipLaunchKernelGGL(k_fft_a_conv<false>, ga, tb, 0, st, a.cgrid, a.na, a.nb, a.nc, nch,
+                         ilog2u(a.na), a.tw_a, a.bf_a, a.bf_b, a.bf_c, a.lattice, (float)a.kc,
+                         (float)a.alpha, a.partials);
+    plane_passes(st, a, false, mc, mb, lh, lb, a.nb * nchunk);
+    return;
+  }
   if (mc)
     hipLaunchKernelGGL(k_fft_c_fwd<true>, gc, tb, 0, st, a.rho_fixed, a.cgrid, nrows, a.nc, 0u,
                        a.tw_c);
