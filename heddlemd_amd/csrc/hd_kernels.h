@@ -143,6 +143,7 @@ struct HdSpmeArgs {
   int mv[3];  // voxel margin per axis: >= half the skin in voxel units, plus one
   int order;  // B-spline order, 4 or 5
   int splines_done;  // spl / sg already hold this evaluation's records
+  int no_plane;      // FFT: separate c and b passes even where a plane fits in LDS
 };
 void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a);
 // the spread half of hd_launch_spme (spline records + charge grid) alone

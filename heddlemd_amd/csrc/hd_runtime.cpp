@@ -195,7 +195,7 @@ struct hd_ctx {
   float2* cgrid = nullptr;
   float* vgrid = nullptr;
   float4* spl = nullptr;  // SPME spline records per slot
-  int dft_mode = 0;        // 0 FP32 radix FFT, 1 split-precision FP16 MFMA DFT
+  int dft_mode = 0;        // 0 FP32 radix FFT, 1 split-precision FP16 MFMA DFT, 2 FFT, no plane fusion
   _Float16* dft_mats = nullptr;
   HdDftMats dft;
   unsigned* sg = nullptr;
@@ -499,6 +499,7 @@ static void enqueue_spme(hd_ctx* c, hipStream_t sst, bool timed, bool splines_do
   HdSpmeArgs sa;
   memset(&sa, 0, sizeof(sa));
   sa.splines_done = splines_done ? 1 : 0;
+  sa.no_plane = c->dft_mode == 2 ? 1 : 0;
   sa.n = (unsigned)c->n;
   sa.order = c->spline_order;
   sa.na = c->grid[0];
@@ -1581,7 +1582,7 @@ static int build_dft_mats(hd_ctx* c) {
 }
 
 int hd_set_spme_dft(hd_ctx* c, int mode) {
-  if (!c || mode < 0 || mode > 1) return HD_ERR_ARG;
+  if (!c || mode < 0 || mode > 2) return HD_ERR_ARG;
   if (!c->have_spme) return fail(c, HD_ERR_STATE, "no SPME configured");
   if (c->ready) {
     HIPCHK(c, hipStreamSynchronize(c->stream));

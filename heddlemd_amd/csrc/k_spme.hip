@@ -688,7 +688,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_b(float2* __restrict__ cgri
 // plane staged in LDS between them instead of a round trip through the
 // complex grid and a second launch.  Each line is transformed by the same
 // LDS routines, on the same values, as in k_fft_c_fwd / k_fft_b /
-// k_fft_c_inv, so the results are the same.
+// k_fft_c_inv; the compiler may round a few multiply-adds differently in
+// the two kernels, so the results agree to f32 rounding.
 #define FFT_PLANE_MAX 2176  // 33 columns x 65 (c <= 64, b <= 64)
 
 __host__ __device__ inline bool fft_plane_fits(unsigned nb, unsigned nc) {
@@ -943,7 +944,7 @@ void hd_launch_spme(hipStream_t st, const HdSpmeArgs& a) {
   unsigned rows = FFT_TILE / h;
   dim3 gc((nrows + rows - 1) / rows), tb(FFT_THREADS);
   bool mc = !pow2u(h), mb = !pow2u(a.nb), ma = !pow2u(a.na);
-  if (fft_plane_fits(a.nb, a.nc)) {
+  if (!a.no_plane && fft_plane_fits(a.nb, a.nc)) {
     // c and b passes per a-plane in one launch each way
     unsigned nchunk = (nch + FFT_CW - 1) / FFT_CW;
     unsigned lh = mc ? 0u : ilog2u(h), lb = mb ? 0u : ilog2u(a.nb);

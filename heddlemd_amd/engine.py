@@ -167,7 +167,8 @@ class Engine:
         return dict(accepted=bool(out[0]), v_old=out[1], v_new=out[2], du=out[3])
 
     def set_spme_dft(self, mode):
-        """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT."""
+        """0: FP32 radix FFT; 1: split-precision FP16 matrix-core DFT; 2: the
+        FFT with every pass a launch of its own (equal to 0 to f32 rounding)."""
         self._chk(self._lib.hd_set_spme_dft(self._ctx, int(mode)), "hd_set_spme_dft")
 
     def set_angles(self, ijk, k_theta, theta0):

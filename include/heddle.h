@@ -111,10 +111,13 @@ int hd_set_lj(hd_ctx *ctx, int ntypes, const double *sigma, const double *epsilo
  * 4 or 5. */
 int hd_set_spme(hd_ctx *ctx, double alpha, double r_cut, int na, int nb, int nc, int order,
                 double k_c);
-/* Reciprocal-space transform: mode 0 = FP32 radix FFT in LDS (default),
- * mode 1 = split-precision FP16 matrix-core DFT (each 1D stage a GEMM with
- * high/low FP16 operands accumulated in FP32; power-of-two axes of 32 to
- * 128 points).  The
+/* Reciprocal-space transform: mode 0 = FP32 radix FFT in LDS (default;
+ * where an a-plane fits in LDS the c and b passes run per plane in one
+ * kernel each way), mode 1 = split-precision FP16 matrix-core DFT (each 1D
+ * stage a GEMM with high/low FP16 operands accumulated in FP32;
+ * power-of-two axes of 32 to 128 points), mode 2 = the FFT of mode 0 with
+ * every pass a launch of its own (the same transform, equal to f32
+ * rounding; for comparison).  The
  * reference uses cuFFT in FP32 (src/gpu/cufft.rs); mode 1 has no
  * counterpart there.  May be switched between phases. */
 int hd_set_spme_dft(hd_ctx *ctx, int mode);

@@ -99,3 +99,35 @@ def test_water_forces_spline_order_5(oracle, grid, dft):
     assert sc["w_recip"] == pytest.approx(wr, rel=2e-6)
     check_forces(f, Fp + Fr, band, f"  order 5 grid {grid}", say)
     assert sc["pe"] == pytest.approx((Ep + Er).sum(), rel=2e-6)
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 32), (40, 40, 40), (48, 48, 48), (64, 64, 64),
+                                  (40, 48, 64)])
+def test_plane_fused_fft_matches_separate_passes(grid):
+    # where an a-plane fits in LDS the c and b passes run per plane in one
+    # kernel each way; mode 2 runs every pass as its own launch.  Each line
+    # gets the same transform of the same values, but the two kernels may
+    # round a few multiply-adds differently, so the bar is f32 rounding:
+    # forces within 1e-6 F_rms per atom, E_recip and W_recip within 1e-7
+    # relative, the potential grid within 1e-6 of its largest value
+    say(f"case: water 8192, SPME grid {grid}, plane-fused FFT vs separate passes")
+    s = systems.config_water_8192()
+    s = dataclasses.replace(s, spme=dataclasses.replace(s.spme, grid=grid))
+    out = []
+    for mode in (0, 2):
+        e = engine_for(s, dt=U.time(2e-15))
+        e.set_spme_dft(mode)
+        e.force_eval(scalars=True)
+        out.append((e.forces().astype(np.float64), e.scalars(),
+                    e.spme_grids(grid)[1].astype(np.float64)))
+        e.close()
+    (fa, sa, va), (fb, sb, vb) = out
+    f_rms = np.sqrt((fb ** 2).sum(axis=0).mean())
+    df = np.sqrt(((fa - fb) ** 2).sum(axis=0)).max() / f_rms
+    dv = np.abs(va - vb).max() / np.abs(vb).max()
+    de = abs(sa["e_recip"] / sb["e_recip"] - 1)
+    dw = abs(sa["w_recip"] / sb["w_recip"] - 1)
+    say(f"  max |dF|/F_rms {df:.2e}, grid {dv:.2e}, E_recip {de:.2e}, W_recip {dw:.2e}")
+    assert df <= 1e-6
+    assert dv <= 1e-6
+    assert de <= 1e-7 and dw <= 1e-7
